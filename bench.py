@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Element-assembly throughput benchmark (one MI355X).
+
+Workload: 40x40x80 NURBS3_Cube2 hex block (128,000 elements, 723,240 dofs), u(3)+phi+T,
+Glen viscous return map + AT2 phase field + thermal (the hex3d_glen_at2 workload).  A
+step is one complete K+f assembly of all three staggered steps from the committed
+history (ResetStep, then assemble u, phase and T in one fused pass), inputs resident in
+HBM.  value = elements assembled per second.
+
+Also reported:
+  roofline     dominant kernel (the element kernel), algorithmic FLOP per element from
+               the survey's count (1.83 MFLOP + 65 kFLOP x mean return-map iterations),
+               divided by the mean kernel-launch duration measured with HIP events on
+               the engine's stream over the timed region;
+  cpu_baseline the FP64 oracle (a restatement of the reference element loop) timed on
+               this host's cores over a bounded element sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def flops_per_element(kbar):
+    # structural count per 3D quadratic hex: shape functions, K_UU, D_eff, f_U/strains/
+    # energy split, phase step, thermal = 1.83 MFLOP; return map 27 x (2.4k*kbar + 1.2k)
+    return 1.83e6 + 65.0e3 * kbar
+
+
+def cpu_baseline(workload, sample_nel, threads):
+    from ice_fem_2025a_amd.config import make_problem
+    from oracle.oracle import Oracle
+    prob = make_problem(workload, nel=sample_nel)
+    o = Oracle(prob)
+    E = prob.mesh.n_elems
+    t = o.time_assembly(list(range(o.nsteps)), 0, E, threads)
+    return E / t, E, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--workload", default="hex3d_glen_at2")
+    ap.add_argument("--nel", type=int, nargs="*", default=None, help="override element counts")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from ice_fem_2025a_amd import hip
+    from ice_fem_2025a_amd.config import make_problem
+    from ice_fem_2025a_amd.engine import Physics
+
+    t0 = time.time()
+    prob = make_problem(args.workload, nel=args.nel)
+    ph = Physics(prob, precision=args.precision)
+    print(f"# setup {time.time() - t0:.1f}s: {prob.mesh.n_elems} elements, "
+          f"{sum(ph.n_rows(s) for s in range(ph.nsteps))} dofs, "
+          f"{sum(ph.nnz(s) for s in range(ph.nsteps))} nnz", file=sys.stderr, flush=True)
+    stream = hip.Stream()
+    E = prob.mesh.n_elems
+
+    def step():
+        ph.reset_step(0, stream)
+        ph.assemble_all(stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    st = ph.status(stream)
+    assert st["nonfinite"] == 0, "non-finite values in the assembly"
+    kbar = st["return_map_iterations"] / max(st["return_map_calls"], 1)
+
+    ph.set_profiling(True)
+    ph.timings()  # clear
+    hip.synchronize()
+    ev0, ev1 = hip.Event(), hip.Event()
+    t_wall = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    stream.synchronize()
+    hip.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms_total = ev0.elapsed_ms(ev1)
+    tim = ph.timings()
+    ph.set_profiling(False)
+    ms_per_step = ms_total / args.steps
+    value = E / (ms_per_step / 1e3)
+
+    avg_launch_ms = tim["kernel_ms_total"] / max(tim["launches"], 1)
+    elems_per_launch = tim["elements"] / max(tim["launches"], 1)
+    fpe = flops_per_element(kbar)
+    achieved_tflops = fpe * elems_per_launch / (avg_launch_ms / 1e3) / 1e12
+    peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
+
+    out = {
+        "metric": "elements assembled/sec (K+f), 3D Q2 Bezier hex, FP64 vs FP32-MFMA @<=1e-6 err",
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-mfma+f64",
+        "data": "synthetic (seeded Philox states on a generated B-spline patch)",
+        "config": {"workload": args.workload, "elements": E, "mesh": list(prob.inputs["mesh"]["nel"]),
+                   "dofs_per_node": "u(3)+phi+T", "precision": args.precision,
+                   "step": "ResetStep + fused K+f assembly of the u, phase and T steps"},
+        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / peak, "traffic": None,
+                     "kernel": "assemble_kernel<3>", "avg_launch_ms": avg_launch_ms,
+                     "launches_per_step": tim["launches"] / args.steps,
+                     "flop_per_element": fpe, "mean_return_map_iterations": kbar},
+        "wall_s": wall,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        rate, ne, secs = cpu_baseline(args.workload, (20, 20, 20) if args.workload.startswith("hex3d") else None,
+                                      threads)
+        out["cpu_baseline"] = {"value": rate, "unit": "elements/s", "cores": threads, "kind": "port",
+                               "sample": f"{ne} elements (20x20x20 block of the same workload), all three steps, "
+                                         f"{secs:.1f}s wall on {threads} threads"}
+        out["gpu_over_cpu"] = value / rate
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,905 @@
+// ife_host.cpp -- host side of the engine: the C-ABI of include/ife.h, DOF numbering,
+// CSR pattern, element->CSR scatter map, element colouring and device buffers.
+//
+// Reference correspondence (paths relative to the IceFEM source tree):
+//   numbering   Physics/DofSpace.cpp:452-510 (per step, per field in DofNames order,
+//               nodes ascending, dof = counter++) and getDofForNodes :974-1012
+//   pattern     Physics/physics.cpp:228-251 + 70-78: the AIJ pattern is the union of the
+//               element blocks the first assembly inserts (explicit zeros kept)
+//   assembly    Physics/physics.cpp:255-303 (zero K/f, models in order, finalise)
+//   history     Models/Fracture/PhaseField/PhaseField.cpp:123-163 (Commit / ResetStep)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ife_internal.h"
+
+using namespace ife;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct ife_exception : std::runtime_error {
+    int code;
+    ife_exception(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define THROW(code, msg) throw ife_exception(code, msg)
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) THROW(IFE_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return IFE_OK;
+    } catch (ife_exception& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return IFE_ERR_NOMEM;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return IFE_ERR_INVALID;
+    }
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t n_) {
+        free();
+        n = n_;
+        if (n_ == 0) return;
+        hipError_t e = hipMalloc((void**)&p, n_ * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            THROW(IFE_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(n_ * sizeof(T)) + " bytes: " + hipGetErrorString(e));
+        }
+    }
+    void upload(const T* h, size_t n_) {
+        alloc(n_);
+        if (n_) HIPCHK(hipMemcpy(p, h, n_ * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free(); }
+};
+
+static void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& fn) {
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096) nt = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++) {
+        int64_t a = n * t / nt, b = n * (t + 1) / nt;
+        th.emplace_back(fn, a, b);
+    }
+    for (auto& t : th) t.join();
+}
+
+enum { MT_PHASEFIELD = 0, MT_THERMAL = 1 };
+
+struct Model {
+    int type;
+    std::string name;
+    ife_material mat;
+    ife_phasefield_opts pf;
+    ife_model_opts opt;
+    int Step_u = -1, Step_phase = -1, Step_T = 99;
+    int field_u[3] = {-1, -1, -1}, field_phase = -1, field_T = -1;
+    bool HasT = false;
+    int HistSize = 0;
+    // device history [comp][E*nip]
+    DevBuf<double> E, EOld, PS, PSOld, VD, VDOld, DVD, DVDOld, MH, MHOld;
+};
+
+struct Step {
+    int64_t nrows = 0;
+    int nf = 0;
+    std::vector<int> fields;      // used fields of the step, DofNames order
+    bool has_matrix = false;
+    int64_t M = 0;                // nnz of one field's row block
+    std::vector<int64_t> row_ptr;
+    std::vector<int32_t> col;
+    DevBuf<int64_t> d_row_ptr;
+    DevBuf<int32_t> d_col;
+    DevBuf<double> K, f, x[4];
+};
+
+}  // namespace
+
+struct ife_physics {
+    // mesh
+    int dim = 0, ip1d = 3, n = 0, nip = 0;
+    int64_t NN = 0, NE = 0;
+    std::vector<double> coords, W;
+    std::vector<int64_t> elems;
+    std::vector<double> ext1d;  // [e][dim][3][3]
+    // dofs
+    std::vector<std::string> fname;
+    std::vector<int> fstep;
+    int nsteps = 0;
+    std::vector<char> fused;     // field registered by a model
+    std::vector<Model*> models;
+    // plan
+    bool planned = false, ready = false;
+    std::vector<int32_t> rank;      // node -> group rank (-1 if unused)
+    std::vector<int64_t> unique;    // group rank -> node
+    int64_t Ng = 0;
+    std::vector<int64_t> nbr_ptr;
+    std::vector<int32_t> nbr;
+    std::vector<int32_t> erank;     // [e][n]
+    std::vector<uint8_t> loc;       // [e][n][n]
+    std::vector<int64_t> color_ptr;
+    std::vector<int32_t> color_elems;
+    std::vector<Step> steps;
+    std::vector<int> fpos;          // field -> position within its step (-1 unused)
+    // device
+    DevBuf<double> d_nodes, d_ext, d_lcrack;
+    DevBuf<int32_t> d_enode, d_erank, d_colors;
+    DevBuf<uint8_t> d_loc;
+    DevBuf<int64_t> d_nbr_ptr;
+    DevBuf<Status> d_status;
+    // time / precision
+    double t = 0, dt = 1, du_dt = 1, ddu_dt = 1;
+    int precision = IFE_PREC_FP64;
+    bool last_had_phase = false;
+    // launch timing
+    bool profiling = false;
+    struct Rec { hipEvent_t a, b; int64_t elems; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> event_pool;
+
+    ~ife_physics() {
+        for (auto* m : models) delete m;
+        for (auto& r : recs) { event_pool.push_back(r.a); event_pool.push_back(r.b); }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t get_event() {
+        if (!event_pool.empty()) { hipEvent_t e = event_pool.back(); event_pool.pop_back(); return e; }
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) THROW(IFE_ERR_HIP, "hipEventCreate");
+        return e;
+    }
+    int field_index(const char* nm) const {
+        for (size_t i = 0; i < fname.size(); i++)
+            if (fname[i] == nm) return (int)i;
+        return -1;
+    }
+    Model* pf_model() const {
+        for (auto* m : models)
+            if (m->type == MT_PHASEFIELD) return m;
+        return nullptr;
+    }
+    Model* th_model() const {
+        for (auto* m : models)
+            if (m->type == MT_THERMAL) return m;
+        return nullptr;
+    }
+};
+
+namespace {
+
+// Factor a dense extraction operator into kron(Cz, kron(Cy, Cx)) (bezierExtraction3D.m:23)
+// normalised with Cy[0][0] = Cz[0][0] = 1; false when C is not such a product.
+bool factor_extraction(int dim, const double* C, double* out /* dim*9 */) {
+    const int n = dim == 2 ? 9 : 27;
+    double c00 = C[0];
+    if (c00 == 0.0) return false;
+    auto idx = [&](int x, int y, int z) { return x + 3 * y + 9 * z; };
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            out[0 * 9 + a * 3 + b] = C[idx(a, 0, 0) * n + idx(b, 0, 0)];
+            out[1 * 9 + a * 3 + b] = C[idx(0, a, 0) * n + idx(0, b, 0)] / c00;
+            if (dim == 3) out[2 * 9 + a * 3 + b] = C[idx(0, 0, a) * n + idx(0, 0, b)] / c00;
+        }
+    double mx = 0;
+    for (int i = 0; i < n * n; i++) mx = std::max(mx, std::fabs(C[i]));
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++) {
+            int ax = a % 3, ay = (a / 3) % 3, az = a / 9, bx = b % 3, by = (b / 3) % 3, bz = b / 9;
+            double v = out[ax * 3 + bx] * out[9 + ay * 3 + by];
+            if (dim == 3) v *= out[18 + az * 3 + bz];
+            if (std::fabs(v - C[a * n + b]) > 1e-14 * mx) return false;
+        }
+    return true;
+}
+
+MatConst make_matconst(const ife_physics* ph, const Model* pfm, const Model* thm) {
+    MatConst mc{};
+    const Model* base = pfm ? pfm : thm;
+    const ife_material& m = base->mat;
+    mc.Young0 = m.Young; mc.Poisson = m.Poisson; mc.Density = m.Density; mc.Damping = m.Damping;
+    mc.Rheology = m.Rheology;
+    mc.HistSize = pfm ? pfm->HistSize : 0;
+    mc.A0 = m.A; mc.nCreep = m.n; mc.PhiViscLim = m.PhiViscLim;
+    mc.l = m.l; mc.Gc_in = m.Gc; mc.pf_visc = m.Viscosity;
+    mc.Split = m.Split;
+    mc.TRef = m.TRef; mc.TOffset = m.TOffset; mc.QCreep = m.QCreep; mc.ft0 = m.ft0; mc.ftdeg = m.ftdeg;
+    // the thermal model reads its own material block (ThermalModel.cpp:61-62)
+    if (thm) { mc.k = thm->mat.ThermalConductivity; mc.cp = thm->mat.HeatCapacity; mc.Density_th = thm->mat.Density; }
+    if (pfm) {
+        const ife_phasefield_opts& pf = pfm->pf;
+        mc.ConsistentEnergy = pf.ConsistentEnergy;
+        mc.DamFunction = pf.DamFunction; mc.DensityDamage = pf.DensityDamage; mc.GravityDamage = pf.GravityDamage;
+        mc.ViscoDriving = pf.ViscoDriving; mc.ViscoDrivingFactor = pf.ViscoDrivingFactor; mc.k0 = pf.k0; mc.p = pf.p;
+        mc.Distribution = pf.DistributionFunction; mc.ConsistentDegradation = pf.ConsistentDegradation;
+        const ife_model_opts& o = pfm->opt;
+        mc.ViscousStab = o.ViscousStab; mc.Irreversible = o.Irreversible; mc.Inertia = o.Inertia;
+        mc.LumpedCap = o.LumpedPFCapacity; mc.LumpedDrive = o.LumpedDriving; mc.ViscousHeating = o.ViscousHeating;
+        mc.porosity = o.porosity; mc.phase_T_from_dof0 = o.phase_T_from_dof0; mc.T_default = o.T_default;
+        if (o.Gravity) {
+            if (ph->dim == 3) { mc.g[0] = 0; mc.g[1] = 0; mc.g[2] = -9.81; }
+            else { mc.g[0] = 0; mc.g[1] = -9.81; mc.g[2] = 0; }
+        }
+    }
+    return mc;
+}
+
+void plan(ife_physics* ph) {
+    if (ph->planned) return;
+    if (ph->models.empty()) THROW(IFE_ERR_STATE, "no models added");
+    const int n = ph->n;
+    // ---- element group: unique nodes (elementgroup::GetUniqueNodes) -----------------
+    ph->rank.assign(ph->NN, -1);
+    for (int64_t i = 0; i < ph->NE * n; i++) {
+        int64_t v = ph->elems[i];
+        if (v < 0 || v >= ph->NN) THROW(IFE_ERR_INVALID, "element node id out of range");
+        ph->rank[v] = 0;
+    }
+    ph->unique.clear();
+    for (int64_t v = 0; v < ph->NN; v++)
+        if (ph->rank[v] == 0) { ph->rank[v] = (int32_t)ph->unique.size(); ph->unique.push_back(v); }
+    ph->Ng = (int64_t)ph->unique.size();
+    const int64_t Ng = ph->Ng;
+    ph->erank.resize(ph->NE * n);
+    for (int64_t i = 0; i < ph->NE * n; i++) ph->erank[i] = ph->rank[ph->elems[i]];
+    // ---- numbering per step (DofSpace.cpp:486-510) --------------------------------
+    const int nfl = (int)ph->fname.size();
+    ph->fpos.assign(nfl, -1);
+    ph->steps.clear();
+    ph->steps.resize(ph->nsteps);
+    for (int s = 0; s < ph->nsteps; s++) {
+        Step& st = ph->steps[s];
+        for (int f = 0; f < nfl; f++)
+            if (ph->fstep[f] == s && ph->fused[f]) { ph->fpos[f] = st.nf++; st.fields.push_back(f); }
+        st.nrows = (int64_t)st.nf * Ng;
+    }
+    // ---- which steps receive element blocks, and the block must cover the step ----
+    for (auto* m : ph->models) {
+        auto need_exact = [&](int step, std::vector<int> flds) {
+            Step& st = ph->steps[step];
+            std::vector<int> a = st.fields, b = flds;
+            std::sort(a.begin(), a.end());
+            std::sort(b.begin(), b.end());
+            if (a != b)
+                THROW(IFE_ERR_UNSUPPORTED, "step " + std::to_string(step) +
+                                               " mixes fields of different element blocks; the engine requires each "
+                                               "matrix-assembling step to hold exactly one model block's fields");
+            st.has_matrix = true;
+        };
+        if (m->type == MT_PHASEFIELD) {
+            std::vector<int> fu;
+            for (int d = 0; d < ph->dim; d++) fu.push_back(m->field_u[d]);
+            need_exact(m->Step_u, fu);
+            need_exact(m->Step_phase, {m->field_phase});
+        } else {
+            need_exact(m->Step_T, {m->field_T});
+        }
+    }
+    // ---- node graph over group ranks: union of element node sets ------------------
+    std::vector<int64_t> n2e_ptr(Ng + 1, 0);
+    for (int64_t i = 0; i < ph->NE * n; i++) n2e_ptr[ph->erank[i] + 1]++;
+    for (int64_t v = 0; v < Ng; v++) n2e_ptr[v + 1] += n2e_ptr[v];
+    std::vector<int64_t> n2e(n2e_ptr[Ng]);
+    {
+        std::vector<int64_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
+        for (int64_t e = 0; e < ph->NE; e++)
+            for (int a = 0; a < n; a++) n2e[fill[ph->erank[e * n + a]]++] = e;
+    }
+    std::vector<std::vector<int32_t>> lists(Ng);
+    parallel_for(Ng, [&](int64_t v0, int64_t v1) {
+        std::vector<int64_t> mark(Ng, -1);
+        for (int64_t v = v0; v < v1; v++) {
+            auto& L = lists[v];
+            for (int64_t k = n2e_ptr[v]; k < n2e_ptr[v + 1]; k++) {
+                int64_t e = n2e[k];
+                for (int a = 0; a < n; a++) {
+                    int32_t w = ph->erank[e * n + a];
+                    if (mark[w] != v) { mark[w] = v; L.push_back(w); }
+                }
+            }
+            std::sort(L.begin(), L.end());
+        }
+    });
+    ph->nbr_ptr.assign(Ng + 1, 0);
+    for (int64_t v = 0; v < Ng; v++) ph->nbr_ptr[v + 1] = ph->nbr_ptr[v] + (int64_t)lists[v].size();
+    ph->nbr.resize(ph->nbr_ptr[Ng]);
+    for (int64_t v = 0; v < Ng; v++) std::copy(lists[v].begin(), lists[v].end(), ph->nbr.begin() + ph->nbr_ptr[v]);
+    lists.clear();
+    lists.shrink_to_fit();
+    int64_t maxdeg = 0;
+    for (int64_t v = 0; v < Ng; v++) maxdeg = std::max(maxdeg, ph->nbr_ptr[v + 1] - ph->nbr_ptr[v]);
+    if (maxdeg > 255) THROW(IFE_ERR_UNSUPPORTED, "node degree above 255 (8-bit scatter slots)");
+    // ---- per-step CSR: row (fp, v) = [g*Ng + nbr(v) for g in fields] --------------
+    const int64_t nnz_node = ph->nbr_ptr[Ng];
+    for (int s = 0; s < ph->nsteps; s++) {
+        Step& st = ph->steps[s];
+        st.row_ptr.assign(st.nrows + 1, 0);
+        if (!st.has_matrix) { st.M = 0; continue; }
+        const int nf = st.nf;
+        st.M = (int64_t)nf * nnz_node;
+        for (int fp = 0; fp < nf; fp++)
+            for (int64_t v = 0; v <= Ng; v++) st.row_ptr[(int64_t)fp * Ng + v] = fp * st.M + (int64_t)nf * ph->nbr_ptr[v];
+        st.col.resize((size_t)nf * st.M);
+        parallel_for(Ng, [&](int64_t v0, int64_t v1) {
+            for (int fp = 0; fp < nf; fp++)
+                for (int64_t v = v0; v < v1; v++) {
+                    int64_t deg = ph->nbr_ptr[v + 1] - ph->nbr_ptr[v];
+                    int32_t* out = st.col.data() + st.row_ptr[(int64_t)fp * Ng + v];
+                    for (int g = 0; g < nf; g++)
+                        for (int64_t k = 0; k < deg; k++) out[g * deg + k] = (int32_t)(g * Ng + ph->nbr[ph->nbr_ptr[v] + k]);
+                }
+        });
+    }
+    // ---- scatter slots: loc[e][a][b] = position of node b in node a's neighbour list
+    ph->loc.resize((size_t)ph->NE * n * n);
+    parallel_for(ph->NE, [&](int64_t e0, int64_t e1) {
+        for (int64_t e = e0; e < e1; e++)
+            for (int a = 0; a < n; a++) {
+                int32_t ra = ph->erank[e * n + a];
+                const int32_t* b0 = ph->nbr.data() + ph->nbr_ptr[ra];
+                const int32_t* b1 = ph->nbr.data() + ph->nbr_ptr[ra + 1];
+                for (int b = 0; b < n; b++) {
+                    const int32_t* it = std::lower_bound(b0, b1, ph->erank[e * n + b]);
+                    ph->loc[((size_t)e * n + a) * n + b] = (uint8_t)(it - b0);
+                }
+            }
+    });
+    // ---- greedy first-fit colouring: no two elements of a colour share a node -----
+    std::vector<uint64_t> used(Ng, 0);
+    std::vector<int> color(ph->NE);
+    int ncol = 0;
+    for (int64_t e = 0; e < ph->NE; e++) {
+        uint64_t m = 0;
+        for (int a = 0; a < n; a++) m |= used[ph->erank[e * n + a]];
+        if (m == ~0ull) THROW(IFE_ERR_UNSUPPORTED, "element colouring needs more than 64 colours");
+        int c = __builtin_ctzll(~m);
+        color[e] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < n; a++) used[ph->erank[e * n + a]] |= (1ull << c);
+    }
+    ph->color_ptr.assign(ncol + 1, 0);
+    for (int64_t e = 0; e < ph->NE; e++) ph->color_ptr[color[e] + 1]++;
+    for (int c = 0; c < ncol; c++) ph->color_ptr[c + 1] += ph->color_ptr[c];
+    ph->color_elems.resize(ph->NE);
+    {
+        std::vector<int64_t> fill(ph->color_ptr.begin(), ph->color_ptr.end() - 1);
+        for (int64_t e = 0; e < ph->NE; e++) ph->color_elems[fill[color[e]]++] = (int32_t)e;
+    }
+    ph->planned = true;
+}
+
+void setup_device(ife_physics* ph) {
+    plan(ph);
+    if (ph->ready) return;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) THROW(IFE_ERR_HIP, "no HIP device");
+    const int n = ph->n;
+    std::vector<double> nodes4((size_t)ph->NN * 4, 0.0);
+    for (int64_t v = 0; v < ph->NN; v++) {
+        for (int d = 0; d < ph->dim; d++) nodes4[v * 4 + d] = ph->coords[v * ph->dim + d];
+        nodes4[v * 4 + 3] = ph->W[v];
+    }
+    ph->d_nodes.upload(nodes4.data(), nodes4.size());
+    std::vector<int32_t> en(ph->NE * n);
+    for (int64_t i = 0; i < ph->NE * n; i++) en[i] = (int32_t)ph->elems[i];
+    ph->d_enode.upload(en.data(), en.size());
+    ph->d_erank.upload(ph->erank.data(), ph->erank.size());
+    ph->d_loc.upload(ph->loc.data(), ph->loc.size());
+    ph->d_ext.upload(ph->ext1d.data(), ph->ext1d.size());
+    ph->d_nbr_ptr.upload(ph->nbr_ptr.data(), ph->nbr_ptr.size());
+    ph->d_colors.upload(ph->color_elems.data(), ph->color_elems.size());
+    ph->d_status.alloc(1);
+    HIPCHK(hipMemset(ph->d_status.p, 0, sizeof(Status)));
+    ph->d_lcrack.alloc(ph->NE);
+    HIPCHK(hipMemset(ph->d_lcrack.p, 0, ph->NE * sizeof(double)));
+    for (auto& st : ph->steps) {
+        st.d_row_ptr.upload(st.row_ptr.data(), st.row_ptr.size());
+        st.d_col.upload(st.col.data(), st.col.size());
+        st.K.alloc(std::max<size_t>(st.col.size(), 1));
+        st.f.alloc(std::max<int64_t>(st.nrows, 1));
+        HIPCHK(hipMemset(st.K.p, 0, st.K.n * sizeof(double)));
+        HIPCHK(hipMemset(st.f.p, 0, st.f.n * sizeof(double)));
+        for (int k = 0; k < 4; k++) {
+            st.x[k].alloc(std::max<int64_t>(st.nrows, 1));
+            HIPCHK(hipMemset(st.x[k].p, 0, st.x[k].n * sizeof(double)));
+        }
+    }
+    const size_t NIPT = (size_t)ph->NE * ph->nip;
+    for (auto* m : ph->models) {
+        if (m->type != MT_PHASEFIELD) continue;
+        size_t hs = (size_t)std::max(m->HistSize, 1);
+        DevBuf<double>* one[] = {&m->E, &m->EOld, &m->VD, &m->VDOld, &m->DVD, &m->DVDOld};
+        for (auto* b : one) b->alloc(NIPT);
+        m->PS.alloc(6 * NIPT);
+        m->PSOld.alloc(6 * NIPT);
+        m->MH.alloc(hs * NIPT);
+        m->MHOld.alloc(hs * NIPT);
+        DevBuf<double>* all[] = {&m->E, &m->EOld, &m->VD, &m->VDOld, &m->DVD, &m->DVDOld, &m->PS, &m->PSOld, &m->MH, &m->MHOld};
+        for (auto* b : all) HIPCHK(hipMemset(b->p, 0, b->n * sizeof(double)));
+    }
+    HIPCHK(hipDeviceSynchronize());
+    ph->ready = true;
+}
+
+void require_ready(const ife_physics* ph) {
+    if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+    if (!ph->ready) THROW(IFE_ERR_STATE, "ife_physics_setup has not been called");
+}
+
+// Bernstein basis of degree 2 at x (BaseElemType.cpp:197-216) and the 3-point Gauss
+// rule on [0,1] (IPScheme.cpp:62-109)
+void bernstein_tables(KParams& p) {
+    const double xs[3] = {(-std::sqrt(3.0 / 5.0) + 1) / 2, (0.0 + 1) / 2, (std::sqrt(3.0 / 5.0) + 1) / 2};
+    const double ws[3] = {5.0 / 9.0 / 2, 8.0 / 9.0 / 2, 5.0 / 9.0 / 2};
+    const int pdeg = 2;
+    const int fact[3] = {1, 1, 2};
+    for (int q = 0; q < 3; q++) {
+        double x = xs[q];
+        for (int i = 0; i < 3; i++) {
+            double pre = (double)(fact[pdeg] / fact[pdeg - i] / fact[i]);
+            p.B1[q][i] = pre * std::pow(1.0 - x, std::max(0, pdeg - i)) * std::pow(x, std::max(0, i));
+            p.dB1[q][i] = pre * ((i - pdeg) * std::pow(1.0 - x, std::max(0, pdeg - i - 1)) * std::pow(x, std::max(0, i)) +
+                                 i * std::pow(1.0 - x, pdeg - i) * std::pow(x, std::max(0, i - 1)));
+        }
+        p.wq[q] = ws[q];
+    }
+}
+
+KParams base_params(ife_physics* ph) {
+    KParams p{};
+    p.nodes = ph->d_nodes.p;
+    p.enode = ph->d_enode.p;
+    p.erank = ph->d_erank.p;
+    p.loc = ph->d_loc.p;
+    p.ext = ph->d_ext.p;
+    p.nbr_ptr = ph->d_nbr_ptr.p;
+    p.Ng = ph->Ng;
+    p.NE = ph->NE;
+    bernstein_tables(p);
+    Model* pfm = ph->pf_model();
+    Model* thm = ph->th_model();
+    p.mc = make_matconst(ph, pfm, thm);
+    p.t = ph->t; p.dt = ph->dt; p.du_dt = ph->du_dt; p.ddu_dt = ph->ddu_dt;
+    p.has_pf = pfm != nullptr;
+    p.has_th = thm != nullptr;
+    p.has_T = (pfm && pfm->HasT) || thm != nullptr;
+    auto stepio = [&](int s) {
+        StepIO io{};
+        if (s < 0 || s >= ph->nsteps) return io;
+        Step& st = ph->steps[s];
+        io.x = st.x[IFE_STATE].p; io.xold = st.x[IFE_STATE_OLD].p; io.dx = st.x[IFE_DSTATE].p; io.ddx = st.x[IFE_DDSTATE].p;
+        io.K = st.K.p; io.f = st.f.p; io.M = st.M; io.nf = st.nf;
+        return io;
+    };
+    if (pfm) {
+        p.su = stepio(pfm->Step_u);
+        p.sph = stepio(pfm->Step_phase);
+        for (int d = 0; d < ph->dim; d++) p.fpos_u[d] = ph->fpos[pfm->field_u[d]];
+        p.fpos_ph = ph->fpos[pfm->field_phase];
+        p.E = pfm->E.p; p.EOld = pfm->EOld.p; p.PS = pfm->PS.p; p.PSOld = pfm->PSOld.p;
+        p.VD = pfm->VD.p; p.VDOld = pfm->VDOld.p; p.DVD = pfm->DVD.p; p.DVDOld = pfm->DVDOld.p;
+        p.MH = pfm->MH.p; p.MHOld = pfm->MHOld.p;
+    }
+    int fT = -1;
+    if (thm) fT = thm->field_T;
+    else if (pfm && pfm->HasT) fT = pfm->field_T;
+    if (fT >= 0) {
+        p.sT = stepio(ph->fstep[fT]);
+        p.fpos_T = ph->fpos[fT];
+        p.fpos_Tth = ph->fpos[fT];
+    }
+    p.NIPT = ph->NE * ph->nip;
+    p.st = ph->d_status.p;
+    p.lcrack_elem = ph->d_lcrack.p;
+    p.mixed = ph->precision == IFE_PREC_MIXED;
+    return p;
+}
+
+void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* fv, hipStream_t s) {
+    require_ready(ph);
+    KParams p = base_params(ph);
+    Model* pfm = ph->pf_model();
+    Model* thm = ph->th_model();
+    std::vector<int> which;
+    if (all) for (int i = 0; i < ph->nsteps; i++) which.push_back(i);
+    else {
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        which.push_back(step);
+    }
+    auto in = [&](int st) { return std::find(which.begin(), which.end(), st) != which.end(); };
+    // caller buffers override engine-owned ones
+    for (size_t i = 0; i < which.size(); i++) {
+        int st = which[i];
+        double* K = Kv ? Kv[i] : nullptr;
+        double* f = fv ? fv[i] : nullptr;
+        Step& S = ph->steps[st];
+        if (!K) K = S.K.p;
+        if (!f) f = S.f.p;
+        if (pfm && pfm->Step_u == st) { p.su.K = K; p.su.f = f; }
+        if (pfm && pfm->Step_phase == st) { p.sph.K = K; p.sph.f = f; }
+        if (p.has_T && ph->fstep[thm ? thm->field_T : pfm->field_T] == st) { p.sT.K = K; p.sT.f = f; }
+        // Physics::Assemble: VecZeroEntries(f), MatZeroEntries(K) (physics.cpp:260-262)
+        HIPCHK(launch_fill(K, (int64_t)S.col.size(), 0.0, s));
+        HIPCHK(launch_fill(f, S.nrows, 0.0, s));
+    }
+    p.do_u = pfm && in(pfm->Step_u);
+    p.do_ph = pfm && in(pfm->Step_phase);
+    p.do_Tpf = pfm && pfm->HasT && pfm->opt.ViscousHeating && in(pfm->Step_T);
+    p.do_th = thm && in(thm->Step_T);
+    ph->last_had_phase = p.do_ph;
+    HIPCHK(launch_zero_status(ph->d_status.p, s));
+    if (!(p.do_u || p.do_ph || p.do_Tpf || p.do_th)) return;
+    const int ncol = (int)ph->color_ptr.size() - 1;
+    for (int c = 0; c < ncol; c++) {
+        p.elist = ph->d_colors.p + ph->color_ptr[c];
+        p.nlist = ph->color_ptr[c + 1] - ph->color_ptr[c];
+        if (ph->profiling && p.nlist > 0) {
+            ife_physics::Rec r{ph->get_event(), ph->get_event(), p.nlist};
+            HIPCHK(hipEventRecord(r.a, s));
+            HIPCHK(launch_assemble(p, ph->dim, s));
+            HIPCHK(hipEventRecord(r.b, s));
+            ph->recs.push_back(r);
+        } else {
+            HIPCHK(launch_assemble(p, ph->dim, s));
+        }
+    }
+}
+
+DevBuf<double>* history_buf(Model* m, const std::string& nm, int64_t& ncomp) {
+    ncomp = 1;
+    if (nm == "Energy") return &m->E;
+    if (nm == "EnergyOld") return &m->EOld;
+    if (nm == "ViscDiss") return &m->VD;
+    if (nm == "ViscDissOld") return &m->VDOld;
+    if (nm == "DamViscDiss") return &m->DVD;
+    if (nm == "DamViscDissOld") return &m->DVDOld;
+    if (nm == "PlasticStrain") { ncomp = 6; return &m->PS; }
+    if (nm == "PlasticStrainOld") { ncomp = 6; return &m->PSOld; }
+    if (nm == "MatHist") { ncomp = std::max(m->HistSize, 1); return &m->MH; }
+    if (nm == "MatHistOld") { ncomp = std::max(m->HistSize, 1); return &m->MHOld; }
+    return nullptr;
+}
+
+}  // namespace
+
+// ==================================================================================
+// C-ABI
+// ==================================================================================
+extern "C" {
+
+const char* ife_last_error(void) { return g_err.c_str(); }
+int ife_api_version(void) { return IFE_API_VERSION; }
+
+int ife_physics_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, ife_physics** out) {
+    return guard([&] {
+        if (!mesh || !dofs || !out) THROW(IFE_ERR_INVALID, "null argument");
+        *out = nullptr;
+        if (!((mesh->dim == 2 && mesh->nodes_per_elem == 9) || (mesh->dim == 3 && mesh->nodes_per_elem == 27)))
+            THROW(IFE_ERR_UNSUPPORTED, "element type: dim 2 needs 9 nodes (B2_2), dim 3 needs 27 (NURBS3_Cube2)");
+        if (mesh->ipcount1D != 3) THROW(IFE_ERR_UNSUPPORTED, "ipcount1D must be 3");
+        if (mesh->n_nodes <= 0 || mesh->n_elems <= 0 || !mesh->coords || !mesh->elems)
+            THROW(IFE_ERR_INVALID, "empty mesh");
+        if (mesh->n_elems > INT32_MAX || mesh->n_nodes > INT32_MAX) THROW(IFE_ERR_UNSUPPORTED, "mesh too large for 32-bit ids");
+        if (!mesh->extraction && !mesh->extraction_1d) THROW(IFE_ERR_INVALID, "no extraction operators");
+        std::unique_ptr<ife_physics> ph(new ife_physics());
+        ph->dim = mesh->dim;
+        ph->n = mesh->nodes_per_elem;
+        ph->nip = ph->n;
+        ph->NN = mesh->n_nodes;
+        ph->NE = mesh->n_elems;
+        ph->coords.assign(mesh->coords, mesh->coords + ph->NN * ph->dim);
+        ph->W.assign(ph->NN, 1.0);
+        if (mesh->weights && ph->dim == 3) ph->W.assign(mesh->weights, mesh->weights + ph->NN);
+        ph->elems.assign(mesh->elems, mesh->elems + ph->NE * ph->n);
+        ph->ext1d.resize((size_t)ph->NE * ph->dim * 9);
+        if (mesh->extraction) {
+            const size_t nn = (size_t)ph->n * ph->n;
+            for (int64_t e = 0; e < ph->NE; e++)
+                if (!factor_extraction(ph->dim, mesh->extraction + e * nn, ph->ext1d.data() + e * ph->dim * 9))
+                    THROW(IFE_ERR_UNSUPPORTED, "extraction operator of element " + std::to_string(e) +
+                                                   " is not a Kronecker product of 1D operators");
+        } else {
+            std::copy(mesh->extraction_1d, mesh->extraction_1d + ph->ext1d.size(), ph->ext1d.begin());
+        }
+        for (int i = 0; i < dofs->n_fields; i++) {
+            if (!dofs->names[i]) THROW(IFE_ERR_INVALID, "null dof name");
+            ph->fname.push_back(dofs->names[i]);
+            if (dofs->steps[i] < 0) THROW(IFE_ERR_INVALID, "negative dof step");
+            ph->fstep.push_back(dofs->steps[i]);
+            ph->nsteps = std::max(ph->nsteps, dofs->steps[i] + 1);
+        }
+        ph->fused.assign(ph->fname.size(), 0);
+        *out = ph.release();
+    });
+}
+
+void ife_physics_destroy(ife_physics* ph) { delete ph; }
+
+int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* name, int32_t elem_group,
+                          const ife_material* mat, const ife_phasefield_opts* pf, const ife_model_opts* opts,
+                          int32_t* model_index) {
+    return guard([&] {
+        if (!ph || !type_name || !mat) THROW(IFE_ERR_INVALID, "null argument");
+        if (ph->planned) THROW(IFE_ERR_STATE, "models must be added before setup");
+        if (elem_group != 0) THROW(IFE_ERR_UNSUPPORTED, "only element group 0 (the whole mesh) is supported");
+        std::string tn(type_name);
+        auto m = std::make_unique<Model>();
+        m->name = name ? name : tn;
+        m->mat = *mat;
+        auto need = [&](const char* f) {
+            int i = ph->field_index(f);
+            if (i < 0) THROW(IFE_ERR_INVALID, std::string("dof type ") + f + " not in DofNames");
+            ph->fused[i] = 1;  // AddDofs on the group's unique nodes
+            return i;
+        };
+        if (tn == "Fracture/PhaseField") {
+            if (ph->pf_model()) THROW(IFE_ERR_UNSUPPORTED, "one Fracture/PhaseField model per physics");
+            if (!pf || !opts) THROW(IFE_ERR_INVALID, "phase-field options required");
+            m->type = MT_PHASEFIELD;
+            m->pf = *pf;
+            m->opt = *opts;
+            const char* un[3] = {"ux", "uy", "uz"};
+            for (int d = 0; d < ph->dim; d++) m->field_u[d] = need(un[d]);
+            for (int d = 1; d < ph->dim; d++)
+                if (ph->fstep[m->field_u[d]] != ph->fstep[m->field_u[0]])
+                    THROW(IFE_ERR_INVALID, "Fracture/PhaseField requires ux and uy to be in the same solver step");
+            m->Step_u = ph->fstep[m->field_u[0]];
+            m->field_phase = need("phase");
+            m->Step_phase = ph->fstep[m->field_phase];
+            m->HasT = ph->field_index("T") >= 0;  // physics->dofspace->hasDofType("T")
+            if (m->HasT) { m->field_T = need("T"); m->Step_T = ph->fstep[m->field_T]; }
+            const int r = mat->Rheology;
+            if (!(r == IFE_RHEO_LE || r == IFE_RHEO_VE || r == IFE_RHEO_DVE || r == IFE_RHEO_VE_EXPL || r == IFE_RHEO_DVE_EXPL))
+                THROW(IFE_ERR_UNSUPPORTED, "rheology not implemented by the engine (LE, VE, dVE, VE_expl, dVE_expl are)");
+            m->HistSize = (r == IFE_RHEO_LE) ? 0 : 1;
+            if (!(mat->Split == IFE_SPLIT_NONE || mat->Split == IFE_SPLIT_VOLSTRAINS || mat->Split == IFE_SPLIT_VOLSTRESS))
+                THROW(IFE_ERR_UNSUPPORTED, "energy split not implemented by the engine (None, VolStrains, VolStress are)");
+            int dfn = pf->DistributionFunction;
+            if (!(dfn == IFE_DIST_LINEAR || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR2 || dfn == IFE_DIST_QUADRATIC ||
+                  dfn == IFE_DIST_LOCAL))
+                THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
+            if (opts->LumpedPFCapacity && !(dfn == IFE_DIST_QUADRATIC || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR))
+                THROW(IFE_ERR_UNSUPPORTED, "lumped capacity form not implemented for this distribution");
+            if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE))
+                THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist and None are)");
+            int dams[] = {pf->DamFunction, pf->DensityDamage, pf->GravityDamage, pf->ViscoDriving};
+            for (int d : dams)
+                if (d < IFE_DAM_NONE || d > IFE_DAM_PORDER) THROW(IFE_ERR_INVALID, "unknown damage function");
+        } else if (tn == "Thermal/ThermalModel") {
+            if (ph->th_model()) THROW(IFE_ERR_UNSUPPORTED, "one Thermal/ThermalModel per physics");
+            m->type = MT_THERMAL;
+            m->field_T = need("T");
+            m->Step_T = ph->fstep[m->field_T];
+        } else {
+            THROW(IFE_ERR_NOT_FOUND, "Model " + tn + " has not been defined");  // ModelRegister.cpp:72-75
+        }
+        ph->models.push_back(m.release());
+        if (model_index) *model_index = (int32_t)ph->models.size() - 1;
+    });
+}
+
+int ife_physics_plan(ife_physics* ph) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        plan(ph);
+    });
+}
+
+int ife_physics_setup(ife_physics* ph) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        setup_device(ph);
+    });
+}
+
+int ife_physics_num_steps(const ife_physics* ph, int32_t* n_steps) {
+    return guard([&] {
+        if (!ph || !n_steps) THROW(IFE_ERR_INVALID, "null argument");
+        *n_steps = ph->nsteps;
+    });
+}
+
+int ife_physics_dof_numbers(const ife_physics* ph, int32_t field, int64_t* out) {
+    return guard([&] {
+        if (!ph || !out) THROW(IFE_ERR_INVALID, "null argument");
+        if (!ph->planned) THROW(IFE_ERR_STATE, "plan/setup first");
+        if (field < 0 || field >= (int)ph->fname.size()) THROW(IFE_ERR_INVALID, "field out of range");
+        int fp = ph->fpos[field];
+        for (int64_t v = 0; v < ph->NN; v++) {
+            int32_t r = ph->rank[v];
+            out[v] = (fp < 0 || r < 0) ? -1 : (int64_t)fp * ph->Ng + r;
+        }
+    });
+}
+
+int ife_physics_pattern(const ife_physics* ph, int32_t step, int64_t* n_rows, int64_t* nnz, const int64_t** row_ptr_host,
+                        const int32_t** col_host, const int64_t** row_ptr_dev, const int32_t** col_dev) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        if (!ph->planned) THROW(IFE_ERR_STATE, "plan/setup first");
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        const Step& st = ph->steps[step];
+        if (n_rows) *n_rows = st.nrows;
+        if (nnz) *nnz = (int64_t)st.col.size();
+        if (row_ptr_host) *row_ptr_host = st.row_ptr.data();
+        if (col_host) *col_host = st.col.data();
+        if (row_ptr_dev) *row_ptr_dev = st.d_row_ptr.p;
+        if (col_dev) *col_dev = st.d_col.p;
+    });
+}
+
+int ife_physics_coloring(const ife_physics* ph, int32_t* n_colors, const int64_t** color_ptr, const int32_t** color_elems) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        if (!ph->planned) THROW(IFE_ERR_STATE, "plan/setup first");
+        if (n_colors) *n_colors = (int32_t)ph->color_ptr.size() - 1;
+        if (color_ptr) *color_ptr = ph->color_ptr.data();
+        if (color_elems) *color_elems = ph->color_elems.data();
+    });
+}
+
+int ife_physics_state(ife_physics* ph, int32_t step, int32_t kind, double** dev_ptr) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps || kind < 0 || kind > 3 || !dev_ptr) THROW(IFE_ERR_INVALID, "bad argument");
+        *dev_ptr = ph->steps[step].x[kind].p;
+    });
+}
+
+int ife_physics_set_time(ife_physics* ph, double t, double dt, double du_dt, double ddu_dt) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        if (!(dt > 0)) THROW(IFE_ERR_INVALID, "dt must be positive");
+        ph->t = t; ph->dt = dt; ph->du_dt = du_dt; ph->ddu_dt = ddu_dt;
+    });
+}
+
+int ife_physics_set_precision(ife_physics* ph, int32_t precision) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        if (precision != IFE_PREC_FP64 && precision != IFE_PREC_MIXED) THROW(IFE_ERR_INVALID, "unknown precision");
+        ph->precision = precision;
+    });
+}
+
+int ife_physics_assemble(ife_physics* ph, int32_t step, double* K_vals, double* f, void* stream) {
+    return guard([&] {
+        double* K[1] = {K_vals};
+        double* F[1] = {f};
+        run(ph, false, step, K, F, (hipStream_t)stream);
+    });
+}
+
+int ife_physics_assemble_all(ife_physics* ph, double* const* K_vals, double* const* f, void* stream) {
+    return guard([&] { run(ph, true, 0, K_vals, f, (hipStream_t)stream); });
+}
+
+int ife_physics_Kf(ife_physics* ph, int32_t step, double** K_vals, double** f) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (K_vals) *K_vals = ph->steps[step].K.p;
+        if (f) *f = ph->steps[step].f.p;
+    });
+}
+
+int ife_physics_status(ife_physics* ph, void* stream, ife_assembly_status* out) {
+    return guard([&] {
+        require_ready(ph);
+        if (!out) THROW(IFE_ERR_INVALID, "null argument");
+        hipStream_t s = (hipStream_t)stream;
+        Status st;
+        HIPCHK(hipMemcpyAsync(&st, ph->d_status.p, sizeof(Status), hipMemcpyDeviceToHost, s));
+        std::vector<double> lc(ph->NE);
+        HIPCHK(hipMemcpyAsync(lc.data(), ph->d_lcrack.p, ph->NE * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        out->elements_assembled = (int64_t)st.elements;
+        out->return_map_calls = (int64_t)st.rm_calls;
+        out->return_map_iterations = (int64_t)st.rm_iters;
+        out->return_map_maxit_hits = (int64_t)st.rm_maxit;
+        out->nonfinite_seen = (int32_t)st.nonfinite;
+        double s_ = 0;
+        if (ph->last_had_phase)
+            for (int64_t e = 0; e < ph->NE; e++) s_ += lc[e];
+        out->LCrack = s_;
+    });
+}
+
+int ife_physics_set_profiling(ife_physics* ph, int32_t enable) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        ph->profiling = enable != 0;
+    });
+}
+
+int ife_physics_timings(ife_physics* ph, ife_timings* out) {
+    return guard([&] {
+        if (!ph || !out) THROW(IFE_ERR_INVALID, "null argument");
+        out->launches = 0; out->kernel_ms_total = 0; out->kernel_ms_max = 0; out->elements = 0;
+        for (auto& r : ph->recs) {
+            HIPCHK(hipEventSynchronize(r.b));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            out->launches++;
+            out->kernel_ms_total += ms;
+            out->kernel_ms_max = std::max(out->kernel_ms_max, (double)ms);
+            out->elements += r.elems;
+            ph->event_pool.push_back(r.a);
+            ph->event_pool.push_back(r.b);
+        }
+        ph->recs.clear();
+    });
+}
+
+int ife_model_history(ife_physics* ph, int32_t model, const char* name, double** dev_ptr, int64_t* n_components,
+                      int64_t* n_per_component) {
+    return guard([&] {
+        require_ready(ph);
+        if (model < 0 || model >= (int)ph->models.size() || !name) THROW(IFE_ERR_INVALID, "bad model index");
+        Model* m = ph->models[model];
+        if (m->type != MT_PHASEFIELD) THROW(IFE_ERR_NOT_FOUND, "model has no history");
+        int64_t nc = 0;
+        DevBuf<double>* b = history_buf(m, name, nc);
+        if (!b) THROW(IFE_ERR_NOT_FOUND, std::string("unknown history ") + name);
+        if (dev_ptr) *dev_ptr = b->p;
+        if (n_components) *n_components = nc;
+        if (n_per_component) *n_per_component = ph->NE * ph->nip;
+    });
+}
+
+static int copy_history(ife_physics* ph, int32_t model, bool commit, void* stream) {
+    return guard([&] {
+        require_ready(ph);
+        if (model < 0 || model >= (int)ph->models.size()) THROW(IFE_ERR_INVALID, "bad model index");
+        Model* m = ph->models[model];
+        if (m->type != MT_PHASEFIELD) return;  // ThermalModel keeps no history
+        hipStream_t s = (hipStream_t)stream;
+        std::pair<DevBuf<double>*, DevBuf<double>*> pr[] = {
+            {&m->E, &m->EOld}, {&m->PS, &m->PSOld}, {&m->VD, &m->VDOld}, {&m->DVD, &m->DVDOld}, {&m->MH, &m->MHOld}};
+        for (auto& q : pr) {
+            DevBuf<double>* cur = q.first;
+            DevBuf<double>* old = q.second;
+            if (commit) HIPCHK(launch_copy(old->p, cur->p, (int64_t)cur->n, s));
+            else HIPCHK(launch_copy(cur->p, old->p, (int64_t)cur->n, s));
+        }
+    });
+}
+
+int ife_model_commit(ife_physics* ph, int32_t model, int32_t commit_type, void* stream) {
+    if (commit_type != 1) return IFE_OK;  // only TIMEDEP_COMMIT_TYPE copies history (PhaseField.cpp:124)
+    return copy_history(ph, model, true, stream);
+}
+
+int ife_model_reset_step(ife_physics* ph, int32_t model, void* stream) { return copy_history(ph, model, false, stream); }
+
+}  // extern "C"

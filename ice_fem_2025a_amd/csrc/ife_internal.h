@@ -1,0 +1,96 @@
+// ife_internal.h -- structures shared by the engine's host set-up (ife_host.cpp) and its
+// HIP kernels (ife_kernels.hip).  Not part of the public C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ife.h"
+
+namespace ife {
+
+// Material constants after parsing (SolidMaterial.cpp:5-107, PhaseFieldMaterial.cpp:3-31,
+// IceMaterial.cpp:5-45, ThermalMaterial.cpp:5-12).  Temperature-dependent values are
+// recomputed per IP on the device from these (IceMaterial::SetTemperature).
+struct MatConst {
+    double Young0, Poisson, Density, Damping;
+    int Rheology, HistSize;
+    double A0, nCreep, PhiViscLim;
+    double l, Gc_in, pf_visc;
+    int Split, ConsistentEnergy;
+    double TRef, TOffset, QCreep, ft0, ftdeg;
+    double k, cp, Density_th;
+    // PF utility options
+    int DamFunction, DensityDamage, GravityDamage, ViscoDriving;
+    double ViscoDrivingFactor, k0, p;
+    int Distribution;
+    int ConsistentDegradation;
+    // model options
+    double ViscousStab;
+    int Irreversible, Inertia, LumpedCap, LumpedDrive, ViscousHeating;
+    double porosity;
+    int phase_T_from_dof0;
+    double T_default;
+    double g[3];
+};
+
+// One staggered step's buffers as seen by a kernel launch.
+struct StepIO {
+    const double* x;     // StateVectors[step]
+    const double* xold;  // StateVectorsOld[step]
+    const double* dx;    // dStateVectors[step]
+    const double* ddx;   // ddStateVectors[step]
+    double* K;           // CSR values (nnz)
+    double* f;           // residual (n_rows)
+    int64_t M;           // nnz of one field's rows: nf * nnz(node graph)
+    int nf;              // fields in the step (row blocks)
+};
+
+struct Status {
+    unsigned long long elements;
+    unsigned long long rm_calls;
+    unsigned long long rm_iters;
+    unsigned long long rm_maxit;
+    unsigned int nonfinite;
+    unsigned int pad;
+};
+
+struct KParams {
+    // mesh / maps
+    const double* nodes;     // [node][4] = x, y, z, W
+    const int32_t* enode;    // [e][n] global node ids
+    const int32_t* erank;    // [e][n] rank in the element group's sorted unique nodes
+    const uint8_t* loc;      // [e][n][n] slot of node b in node a's sorted neighbour list
+    const double* ext;       // [e][dim][3][3] 1D extraction operators
+    const int32_t* elist;    // elements of this launch (one colour)
+    int64_t nlist;
+    const int64_t* nbr_ptr;  // node-graph CSR offsets over group ranks
+    int64_t Ng;              // nodes in the group
+    int64_t NE;
+    // Bernstein values/derivatives at the 1D Gauss points (BezierBasis, ipcount1D = 3)
+    double B1[3][3], dB1[3][3], wq[3];
+    // physics
+    MatConst mc;
+    double t, dt, du_dt, ddu_dt;
+    // which parts run: u-step (Assemble_U), phase step (Assemble_Phase), T step of the
+    // phase-field model (Assemble_T), thermal model (ThermalModel::Assemble)
+    int do_u, do_ph, do_Tpf, do_th;
+    int has_pf, has_th, has_T;
+    StepIO su, sph, sT;
+    int fpos_u[3];           // position of ux,uy[,uz] among the u step's fields
+    int fpos_ph, fpos_T;
+    int fpos_Tth;            // T position in its step (thermal model)
+    // history [comp][e*nip + ip]
+    double *E, *EOld, *PS, *PSOld, *VD, *VDOld, *DVD, *DVDOld, *MH, *MHOld;
+    int64_t NIPT;
+    Status* st;
+    double* lcrack_elem;     // [e]
+    int mixed;               // tangent contractions in FP32
+};
+
+// Launchers (ife_kernels.hip)
+hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s);
+hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
+hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+hipError_t launch_zero_status(Status* st, hipStream_t s);
+
+}  // namespace ife

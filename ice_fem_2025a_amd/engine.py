@@ -1,0 +1,200 @@
+"""Python host mirror of the reference's Physics/BaseModel life cycle over the C-ABI.
+
+    ph = Physics(problem)        # Physics::Physics: models created, dofs numbered,
+                                 # pattern fixed (physics.cpp:15-88)
+    ph.assemble(step)            # Physics::Assemble(step) (physics.cpp:255-303)
+    ph.assemble_all()            # every staggered step in one fused pass
+    ph.commit(model) / ph.reset_step(model)   # BaseModel::Commit / ResetStep
+
+Every compute call goes to libife.so (HIP); there is no CPU path."""
+import ctypes as C
+
+import numpy as np
+
+from . import abi, hip
+
+_dp = C.c_void_p
+
+
+def _mesh_desc(mesh, keep):
+    coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
+    weights = np.ascontiguousarray(mesh.weights, dtype=np.float64)
+    elems = np.ascontiguousarray(mesh.elems, dtype=np.int64)
+    e1 = np.ascontiguousarray(mesh.extraction_1d, dtype=np.float64)
+    keep += [coords, weights, elems, e1]
+    d = abi.MeshDesc()
+    d.dim, d.ipcount1D = mesh.dim, mesh.ipcount1D
+    d.n_nodes, d.n_elems, d.nodes_per_elem = mesh.n_nodes, mesh.n_elems, mesh.nodes_per_elem
+    d.coords = coords.ctypes.data_as(C.POINTER(C.c_double))
+    d.weights = weights.ctypes.data_as(C.POINTER(C.c_double))
+    d.elems = elems.ctypes.data_as(C.POINTER(C.c_int64))
+    d.extraction_1d = e1.ctypes.data_as(C.POINTER(C.c_double))
+    return d
+
+
+def _dofs_desc(names, steps, keep):
+    arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    st = np.ascontiguousarray(steps, dtype=np.int32)
+    keep += [arr, st]
+    d = abi.DofsDesc()
+    d.n_fields = len(names)
+    d.names = C.cast(arr, C.POINTER(C.c_char_p))
+    d.steps = st.ctypes.data_as(C.POINTER(C.c_int32))
+    return d
+
+
+class Physics:
+    def __init__(self, prob, precision="fp64", device=True):
+        self.L = abi.load_library()
+        self.prob = prob
+        keep = []
+        md = _mesh_desc(prob.mesh, keep)
+        dd = _dofs_desc(prob.dof_names, prob.dof_steps, keep)
+        h = C.c_void_p()
+        abi.check(self.L.ife_physics_create(C.byref(md), C.byref(dd), C.byref(h)))
+        self.h = h
+        self.model_index = []
+        for m in prob.models:
+            idx = C.c_int32()
+            abi.check(self.L.ife_physics_add_model(h, m.type_name.encode(), m.name.encode(), 0, C.byref(m.material),
+                                                   C.byref(m.pf) if m.pf is not None else None,
+                                                   C.byref(m.opts) if m.opts is not None else None, C.byref(idx)))
+            self.model_index.append(idx.value)
+        ns = C.c_int32()
+        abi.check(self.L.ife_physics_num_steps(h, C.byref(ns)))
+        self.nsteps = ns.value
+        self.device = device
+        if device:
+            abi.check(self.L.ife_physics_setup(h))
+            abi.check(self.L.ife_physics_set_time(h, prob.t, prob.dt, prob.du_dt, prob.ddu_dt))
+            self.set_precision(precision)
+            self.load_state(prob)
+        else:
+            abi.check(self.L.ife_physics_plan(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ife_physics_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- numbering / pattern ---------------------------------------------------------
+    def dof_numbers(self, field):
+        out = np.empty(self.prob.mesh.n_nodes, dtype=np.int64)
+        abi.check(self.L.ife_physics_dof_numbers(self.h, field, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+    def pattern(self, step):
+        nr, nnz = C.c_int64(), C.c_int64()
+        rp, col = C.POINTER(C.c_int64)(), C.POINTER(C.c_int32)()
+        abi.check(self.L.ife_physics_pattern(self.h, step, C.byref(nr), C.byref(nnz), C.byref(rp), C.byref(col),
+                                             None, None))
+        r = np.ctypeslib.as_array(rp, shape=(nr.value + 1,)).copy()
+        c = np.ctypeslib.as_array(col, shape=(nnz.value,)).copy() if nnz.value else np.zeros(0, np.int32)
+        return r, c
+
+    def n_rows(self, step):
+        nr = C.c_int64()
+        abi.check(self.L.ife_physics_pattern(self.h, step, C.byref(nr), None, None, None, None, None))
+        return nr.value
+
+    def nnz(self, step):
+        nz = C.c_int64()
+        abi.check(self.L.ife_physics_pattern(self.h, step, None, C.byref(nz), None, None, None, None))
+        return nz.value
+
+    def coloring(self):
+        nc = C.c_int32()
+        cp, ce = C.POINTER(C.c_int64)(), C.POINTER(C.c_int32)()
+        abi.check(self.L.ife_physics_coloring(self.h, C.byref(nc), C.byref(cp), C.byref(ce)))
+        ptr = np.ctypeslib.as_array(cp, shape=(nc.value + 1,)).copy()
+        el = np.ctypeslib.as_array(ce, shape=(int(ptr[-1]),)).copy()
+        return ptr, el
+
+    # ---- device state ----------------------------------------------------------------
+    def set_precision(self, precision):
+        abi.check(self.L.ife_physics_set_precision(
+            self.h, abi.IFE_PREC_MIXED if precision == "mixed" else abi.IFE_PREC_FP64))
+
+    def set_time(self, t, dt, du_dt, ddu_dt):
+        abi.check(self.L.ife_physics_set_time(self.h, t, dt, du_dt, ddu_dt))
+
+    def state_ptr(self, step, kind):
+        p = C.c_void_p()
+        abi.check(self.L.ife_physics_state(self.h, step, kind, C.byref(p)))
+        return p.value
+
+    def history_ptr(self, model, name):
+        p, nc, npc = C.c_void_p(), C.c_int64(), C.c_int64()
+        abi.check(self.L.ife_model_history(self.h, model, name.encode(), C.byref(p), C.byref(nc), C.byref(npc)))
+        return p.value, nc.value, npc.value
+
+    def load_state(self, prob):
+        """Scatter per-node state into the engine's dof-indexed vectors; per-IP history."""
+        vecs = {}
+        for (kind, fname), vals in prob.node_state.items():
+            f = prob.dof_names.index(fname)
+            step = prob.dof_steps[f]
+            if (step, kind) not in vecs:
+                vecs[(step, kind)] = np.zeros(self.n_rows(step))
+            dofs = self.dof_numbers(f)
+            m = dofs >= 0
+            vecs[(step, kind)][dofs[m]] = vals[m]
+        for (step, kind), v in vecs.items():
+            hip.to_device(self.state_ptr(step, kind), v)
+        for (model, name), vals in prob.history.items():
+            p, nc, npc = self.history_ptr(model, name)
+            full = np.zeros((nc, npc))
+            full[:vals.shape[0]] = vals
+            hip.to_device(p, np.ascontiguousarray(full))
+
+    def history(self, model, name):
+        p, nc, npc = self.history_ptr(model, name)
+        out = np.empty((nc, npc))
+        hip.synchronize()
+        hip.from_device(out, p)
+        return out
+
+    # ---- the hot path ----------------------------------------------------------------
+    def assemble(self, step, stream=None):
+        abi.check(self.L.ife_physics_assemble(self.h, step, None, None, stream.handle if stream else None))
+
+    def assemble_all(self, stream=None):
+        abi.check(self.L.ife_physics_assemble_all(self.h, None, None, stream.handle if stream else None))
+
+    def Kf(self, step):
+        """Engine-owned K values and f of a step, copied to the host."""
+        kp, fp = C.c_void_p(), C.c_void_p()
+        abi.check(self.L.ife_physics_Kf(self.h, step, C.byref(kp), C.byref(fp)))
+        hip.synchronize()
+        K = np.empty(self.nnz(step))
+        f = np.empty(self.n_rows(step))
+        if K.size:
+            hip.from_device(K, kp.value)
+        if f.size:
+            hip.from_device(f, fp.value)
+        return K, f
+
+    def status(self, stream=None):
+        st = abi.AssemblyStatus()
+        abi.check(self.L.ife_physics_status(self.h, stream.handle if stream else None, C.byref(st)))
+        return {"elements": st.elements_assembled, "return_map_calls": st.return_map_calls,
+                "return_map_iterations": st.return_map_iterations, "return_map_maxit_hits": st.return_map_maxit_hits,
+                "nonfinite": st.nonfinite_seen, "LCrack": st.LCrack}
+
+    def set_profiling(self, on=True):
+        abi.check(self.L.ife_physics_set_profiling(self.h, 1 if on else 0))
+
+    def timings(self):
+        t = abi.Timings()
+        abi.check(self.L.ife_physics_timings(self.h, C.byref(t)))
+        return {"launches": t.launches, "kernel_ms_total": t.kernel_ms_total, "kernel_ms_max": t.kernel_ms_max,
+                "elements": t.elements}
+
+    def commit(self, model, commit_type=1, stream=None):
+        abi.check(self.L.ife_model_commit(self.h, model, commit_type, stream.handle if stream else None))
+
+    def reset_step(self, model, stream=None):
+        abi.check(self.L.ife_model_reset_step(self.h, model, stream.handle if stream else None))

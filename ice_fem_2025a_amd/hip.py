@@ -1,0 +1,96 @@
+"""Minimal ctypes access to the HIP runtime: copies, synchronisation, streams, events.
+
+Used by the Python host mirror to move state in and out of the engine's device
+buffers and to time kernels on the engine's own stream."""
+import ctypes as C
+import os
+
+_hip = None
+
+
+def lib():
+    global _hip
+    if _hip is None:
+        for p in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = C.CDLL(p)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise RuntimeError("libamdhip64.so not found")
+        h = _hip
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemcpy.restype = C.c_int
+        h.hipDeviceSynchronize.restype = C.c_int
+        h.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        h.hipGetDeviceCount.restype = C.c_int
+        h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        h.hipStreamCreate.restype = C.c_int
+        h.hipStreamDestroy.argtypes = [C.c_void_p]
+        h.hipStreamSynchronize.argtypes = [C.c_void_p]
+        h.hipStreamSynchronize.restype = C.c_int
+        h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        h.hipEventCreate.restype = C.c_int
+        h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        h.hipEventRecord.restype = C.c_int
+        h.hipEventSynchronize.argtypes = [C.c_void_p]
+        h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        h.hipEventElapsedTime.restype = C.c_int
+        h.hipEventDestroy.argtypes = [C.c_void_p]
+        h.hipGetErrorString.argtypes = [C.c_int]
+        h.hipGetErrorString.restype = C.c_char_p
+    return _hip
+
+
+H2D, D2H, D2D = 1, 2, 3
+
+
+def _chk(code, what):
+    if code != 0:
+        raise RuntimeError(f"{what}: {lib().hipGetErrorString(code).decode()}")
+
+
+def device_count():
+    n = C.c_int(0)
+    if lib().hipGetDeviceCount(C.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+def to_device(dev_ptr, arr):
+    _chk(lib().hipMemcpy(dev_ptr, arr.ctypes.data, arr.nbytes, H2D), "hipMemcpy H2D")
+
+
+def from_device(arr, dev_ptr):
+    _chk(lib().hipMemcpy(arr.ctypes.data, dev_ptr, arr.nbytes, D2H), "hipMemcpy D2H")
+
+
+def synchronize():
+    _chk(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        _chk(lib().hipStreamCreate(C.byref(s)), "hipStreamCreate")
+        self.handle = s
+
+    def synchronize(self):
+        _chk(lib().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        _chk(lib().hipEventCreate(C.byref(e)), "hipEventCreate")
+        self.handle = e
+
+    def record(self, stream=None):
+        _chk(lib().hipEventRecord(self.handle, stream.handle if stream else None), "hipEventRecord")
+
+    def elapsed_ms(self, end):
+        lib().hipEventSynchronize(end.handle)
+        ms = C.c_float()
+        _chk(lib().hipEventElapsedTime(C.byref(ms), self.handle, end.handle), "hipEventElapsedTime")
+        return ms.value

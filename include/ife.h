@@ -1,0 +1,294 @@
+/*
+ * ife.h -- C-ABI of the MI355X element-assembly engine for IceFEM Bezier elements.
+ *
+ * The engine replaces the element loop of IceFEM's models (residual f and tangent K,
+ * assembled per staggered step into the matrix/vector pair that DofSpace numbers and
+ * NonLinSolver consumes).  Every entry point is a plain C function over plain pointers
+ * and sizes; device pointers are HIP device addresses, streams are hipStream_t passed
+ * as void*.  Each entry point names the reference interface it replaces (paths relative
+ * to the IceFEM source tree).
+ *
+ * Object model (mirrors the reference):
+ *   ife_physics  <-> class Physics      (Physics/physics.h:21-77) -- owns numbering,
+ *                    per-step K/f, state vectors, the models.
+ *   model index  <-> BaseModel*         (Models/BaseModel/BaseModel.h:32-118), created
+ *                    by registered name like CreateModel (Models/ModelRegister.cpp:57-76).
+ *
+ * Life cycle (Physics::Physics, Physics/physics.cpp:15-88):
+ *   ife_physics_create -> ife_physics_add_model (xN, = CreateModel + init/AddDofs)
+ *   -> ife_physics_setup (= DofSpace::SyncDofs + InitKf + the pattern-fixing first assembly)
+ *   -> [ife_physics_assemble(step)]* -> ife_model_commit / ife_model_reset_step.
+ *
+ * Errors: every int-returning function returns IFE_OK (0) or an ife_status_code and
+ * stores a message retrievable with ife_last_error() (thread-local).  Kernel-side
+ * conditions (NaN/Inf seen, return map hitting its iteration cap) are reported through
+ * ife_physics_status(), so a host wrapper can throw like the reference does
+ * (NonLinSolver.cpp:250-269, TimeSolver.cpp:133-171).
+ */
+#ifndef IFE_H
+#define IFE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IFE_API_VERSION 1
+
+typedef enum {
+    IFE_OK = 0,
+    IFE_ERR_INVALID = 1,      /* bad argument / inconsistent description */
+    IFE_ERR_UNSUPPORTED = 2,  /* option the engine does not implement */
+    IFE_ERR_HIP = 3,          /* HIP runtime error */
+    IFE_ERR_NOMEM = 4,        /* allocation failed */
+    IFE_ERR_STATE = 5,        /* call out of life-cycle order */
+    IFE_ERR_NOT_FOUND = 6     /* unknown model name / history name */
+} ife_status_code;
+
+/* SolidMaterial::SolidMaterialModel, same order (Models/Materials/SolidMaterial.h:12-21) */
+typedef enum { IFE_RHEO_VE = 0, IFE_RHEO_VE_EXPL, IFE_RHEO_DVE, IFE_RHEO_DVE_EXPL,
+               IFE_RHEO_VEP, IFE_RHEO_DVEP, IFE_RHEO_LE } ife_rheology;
+/* PhaseFieldMaterial::SplittingMethods, same order (PhaseFieldMaterial.h:42-53) */
+typedef enum { IFE_SPLIT_NONE = 0, IFE_SPLIT_VOLSTRAINS, IFE_SPLIT_SPECSTRAINS, IFE_SPLIT_VOLSTRESS,
+               IFE_SPLIT_SPECSTRESS, IFE_SPLIT_DRUCKERPRAGER, IFE_SPLIT_ICE, IFE_SPLIT_STARCONVEX } ife_split;
+/* DamageMethods, same order (Models/Fracture/PhaseField/PhaseFieldUtil.h:11-25) */
+typedef enum { IFE_DAM_NONE = 0, IFE_DAM_STEP2, IFE_DAM_STEP5, IFE_DAM_STEP8, IFE_DAM_LINEAR,
+               IFE_DAM_QUADRATIC, IFE_DAM_CUBIC, IFE_DAM_LINEAR_NR, IFE_DAM_QUADRATIC_NR,
+               IFE_DAM_CUBIC_NR, IFE_DAM_CZM, IFE_DAM_PORDER } ife_damage;
+/* DistributionMethods, same order (PhaseFieldUtil.h:28-40) */
+typedef enum { IFE_DIST_LINEAR = 0, IFE_DIST_CZM, IFE_DIST_LINEAR2, IFE_DIST_QUADRATIC,
+               IFE_DIST_HO_LINEAR, IFE_DIST_HO_LINEAR2, IFE_DIST_HO_CZM, IFE_DIST_HO_QUADRATIC,
+               IFE_DIST_LOCAL, IFE_DIST_CUSTOM } ife_distribution;
+/* PhaseFieldUtility::IrreversibilityMethods, same order (PhaseFieldUtil.h:49-58) */
+typedef enum { IFE_IRR_NONE = 0, IFE_IRR_HIST, IFE_IRR_PENALTY, IFE_IRR_LAGRANGE,
+               IFE_IRR_AUGLAGR, IFE_IRR_AUGLAGR2, IFE_IRR_AUGLAGR3 } ife_irreversibility;
+
+/* Which state vector of a step (Physics::StateVectors/StateVectorsOld/dStateVectors/
+ * ddStateVectors, physics.h:34). */
+typedef enum { IFE_STATE = 0, IFE_STATE_OLD = 1, IFE_DSTATE = 2, IFE_DDSTATE = 3 } ife_state_kind;
+
+/* Arithmetic of the element contractions. */
+typedef enum {
+    IFE_PREC_FP64 = 0,   /* everything FP64 (parity mode) */
+    IFE_PREC_MIXED = 1   /* tangent contractions on FP32 MFMA; return map, history max,
+                            residual and the global CSR accumulation stay FP64 */
+} ife_precision;
+
+/*
+ * Mesh: one patch of Bezier-extracted elements as IceFEM's HDF5 mesh holds it
+ * (Mesh::LoadFromFile mesh/mesh.cpp:53-89, format TestCases/.../Meshgen/SaveToHDF.m:1-83).
+ *   coords     n_nodes x dim, row-major ("/nodes")
+ *   weights    n_nodes NURBS weights ("/node_Data" W); NULL means all ones (required for
+ *              NURBS3_Cube2, ignored by B2_2 which has no node data)
+ *   elems      n_elems x nodes_per_elem global node ids, local order x-fastest
+ *              ("/elementgroups/<name>")
+ *   extraction n_elems x n x n, row-major C[e][j][k] = ElemData(j,k) of
+ *              elementgroup.cpp:62 ("/elementgroups/<name>_Data")
+ *   extraction_1d  alternative to `extraction` (used when `extraction` is NULL):
+ *              n_elems x dim x 3 x 3 row-major 1D operators (Cx, Cy[, Cz]) with
+ *              C = kron(Cz, kron(Cy, Cx)) as bezierExtraction3D.m:17-27 builds it.
+ * Element type by dim: 2 -> "B2_2" (mesh/ElementTypes/Bezier/B2_2.cpp), 3 ->
+ * "NURBS3_Cube2" (mesh/ElementTypes/NURBS/NURBS3_Cube2.cpp).  ipcount1D = mesh.ipcount1D
+ * (3 is supported).  A dense `extraction` must be such a Kronecker product of 1D
+ * operators (what bezierExtraction2D/3D.m produce); other operators are rejected.
+ */
+typedef struct {
+    int32_t dim;
+    int32_t ipcount1D;
+    int64_t n_nodes;
+    const double* coords;
+    const double* weights;
+    int64_t n_elems;
+    int32_t nodes_per_elem;
+    const int64_t* elems;
+    const double* extraction;
+    const double* extraction_1d;
+} ife_mesh_desc;
+
+/* "Dofs": {"DofNames": [...], "DofStep": [...]} (Physics/DofSpace.cpp:14-15). */
+typedef struct {
+    int32_t n_fields;
+    const char* const* names;  /* e.g. "ux","uy","uz","phase","T" */
+    const int32_t* steps;
+} ife_dofs_desc;
+
+/* properties.<Mat>.* (SolidMaterial.cpp:5-107, PhaseFieldMaterial.cpp:3-31,
+ * IceMaterial.cpp:5-45, ThermalMaterial.cpp:5-12, CosseratMaterial.cpp:3-9). */
+typedef struct {
+    /* SolidProperties */
+    double Young, Poisson, Density, Damping;
+    int32_t Rheology;               /* ife_rheology */
+    double A, n, PhiViscLim;
+    double sy, F_deg, G_deg, hardening, pl_visc;  /* visco-plastic variants */
+    double G_coss, l_coss, hard_eRef;             /* Cosserat */
+    /* FractureProperties */
+    double l, Gc, Viscosity;
+    int32_t Split;                  /* ife_split */
+    /* TemperatureDependent */
+    double TRef, TOffset, QCreep, ft0, ftdeg, sydeg;
+    /* ThermalProperties */
+    double ThermalConductivity, HeatCapacity;
+} ife_material;
+
+/* properties.PhaseField.* (PhaseFieldUtil.cpp:4-39, PhaseField.cpp:196-267,
+ * PhaseFieldMaterial.cpp:19-20). */
+typedef struct {
+    int32_t DamFunction, DensityDamage, GravityDamage, ViscoDriving;  /* ife_damage */
+    double ViscoDrivingFactor, k0, p;
+    int32_t DistributionFunction;   /* ife_distribution */
+    int32_t ConsistentEnergy, ConsistentDegradation;
+} ife_phasefield_opts;
+
+/* Models.<name>.* (PhaseField.cpp:167-283). */
+typedef struct {
+    double ViscousStab;
+    int32_t Irreversible;           /* ife_irreversibility */
+    int32_t Inertia, Gravity, LumpedPFCapacity, LumpedDriving, ViscousHeating;
+    /* Porosity handed to IceMaterial::SetTemperature(T, poros).  The reference's
+     * Fracture/PhaseField passes 1.0 (PhaseField.cpp:383,598), which zeroes Young and
+     * makes Gc = 0/0; Fracture/CossPhaseField passes 0 (CossPhaseField.cpp:421-425). */
+    double porosity;
+    /* Assemble_Phase never fills its T dof list (PhaseField.cpp:517,562-565), so every
+     * node reads global T dof 0.  1 = reproduce that, 0 = gather T per node. */
+    int32_t phase_T_from_dof0;
+    /* Temperature used when the model has no T dof (the reference reads uninitialised
+     * memory there, PhaseField.cpp:383). */
+    double T_default;
+} ife_model_opts;
+
+/* Per-launch report (kernel status word + cheap reductions). */
+typedef struct {
+    int64_t elements_assembled;
+    int64_t return_map_calls;
+    int64_t return_map_iterations;  /* sum over IPs: mean = iterations / calls */
+    int64_t return_map_maxit_hits;  /* reference logs these (SolidMaterial.cpp:246-250) */
+    int32_t nonfinite_seen;         /* NaN/Inf in any element residual/tangent */
+    double  LCrack;                 /* sum gamma*w of the last phase assembly (PhaseField.cpp:650) */
+} ife_assembly_status;
+
+/* Kernel timing of assemble calls (HIP events around every element-kernel launch),
+ * the engine's counterpart of the per-model "Runtime: min .. max" report of
+ * Physics::Assemble (physics.cpp:266-284). */
+typedef struct {
+    int64_t launches;          /* element-kernel launches since the last query */
+    double kernel_ms_total;    /* sum of their durations */
+    double kernel_ms_max;
+    int64_t elements;          /* elements those launches processed */
+} ife_timings;
+
+typedef struct ife_physics ife_physics;
+
+/* ---- life cycle ---------------------------------------------------------------- */
+
+/* Physics::Physics (physics.cpp:15-46) with the mesh already loaded and DofSpace
+ * created from "Dofs".  Copies the descriptors; device buffers are allocated in setup. */
+int ife_physics_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs,
+                       ife_physics** out);
+
+/* CreateModel(name) + BaseModel::init (ModelRegister.cpp:57-76, BaseModel.h:62,
+ * PhaseField.cpp:33-101, ThermalModel.cpp:31-36).  type_name is the registered name:
+ * "Fracture/PhaseField" or "Thermal/ThermalModel".  u/phase/T fields are looked up by
+ * the reference's DOF names (ux,uy[,uz], phase, T).  Models run in the order added,
+ * as Physics::Assemble loops Models[i] (physics.cpp:269-284).  mat/pf/opts may be NULL
+ * for the thermal model except mat (ThermalProperties, Density). */
+int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* name,
+                          int32_t elem_group, const ife_material* mat,
+                          const ife_phasefield_opts* pf, const ife_model_opts* opts,
+                          int32_t* model_index);
+
+/* DofSpace::SyncDofs numbering (DofSpace.cpp:452-510) + Physics::InitKf
+ * (physics.cpp:228-251) + the sparsity the first Assemble fixes (physics.cpp:70-78):
+ * builds the per-step CSR pattern, the element->CSR scatter map and colouring, and
+ * allocates state vectors, history and K/f on the current HIP device. */
+int ife_physics_setup(ife_physics* ph);
+
+void ife_physics_destroy(ife_physics* ph);
+
+/* The host half of ife_physics_setup (numbering, pattern, scatter map, colouring)
+ * without touching a GPU; ife_physics_setup runs it when it has not run yet.  After
+ * it, numbering/pattern/colouring queries work and device pointers are NULL. */
+int ife_physics_plan(ife_physics* ph);
+
+/* ---- numbering and pattern ------------------------------------------------------- */
+
+int ife_physics_num_steps(const ife_physics* ph, int32_t* n_steps);
+
+/* Global dof of (field, node), -1 if the node carries no such dof
+ * (DofSpace::getDofForNodes, DofSpace.cpp:974-1012). dof_of_node has n_nodes entries. */
+int ife_physics_dof_numbers(const ife_physics* ph, int32_t field, int64_t* dof_of_node);
+
+/* CSR pattern of K[step]: rows sorted, columns sorted, explicit zeros of every
+ * element block kept, 64-bit row offsets.  Host pointers stay valid until destroy;
+ * device pointers (may be NULL args) too. */
+int ife_physics_pattern(const ife_physics* ph, int32_t step, int64_t* n_rows, int64_t* nnz,
+                        const int64_t** row_ptr_host, const int32_t** col_host,
+                        const int64_t** row_ptr_dev, const int32_t** col_dev);
+
+/* Element colouring used by the scatter (no two elements of one colour share a node):
+ * elements of colour c are color_elems[color_ptr[c] .. color_ptr[c+1]).  Host pointers.
+ * Engine-specific (the reference scatters through PETSc's MatSetValues instead). */
+int ife_physics_coloring(const ife_physics* ph, int32_t* n_colors, const int64_t** color_ptr,
+                         const int32_t** color_elems);
+
+/* ---- state, time, precision ------------------------------------------------------ */
+
+/* Device pointer of a state vector of a step (length = n_rows of that step), indexed
+ * by global dof like the reference's ghosted vectors. Callers write it in place. */
+int ife_physics_state(ife_physics* ph, int32_t step, int32_t kind, double** dev_ptr);
+
+/* physics->time and TimeScheme::{dt,du_dt,ddu_dt} (TimeSchemes.cpp:65-76). */
+int ife_physics_set_time(ife_physics* ph, double t, double dt, double du_dt, double ddu_dt);
+
+int ife_physics_set_precision(ife_physics* ph, int32_t precision);
+
+/* ---- the hot path -------------------------------------------------------------- */
+
+/* Physics::Assemble(step) (physics.cpp:255-303): zero f and K (pattern kept), run every
+ * model's Assemble(K, f, cons, step) in order, finalise.  K_vals (nnz doubles) and f
+ * (n_rows doubles) are device buffers; NULL selects the engine-owned buffers.  Runs
+ * asynchronously on `stream` (hipStream_t, NULL = default stream). */
+int ife_physics_assemble(ife_physics* ph, int32_t step, double* K_vals, double* f,
+                         void* stream);
+
+/* All staggered steps in one fused pass (shape functions computed once per element);
+ * the same values as calling ife_physics_assemble for step 0,1,... in order with no
+ * state change in between.  K_vals[s]/f[s] may be NULL (engine-owned). */
+int ife_physics_assemble_all(ife_physics* ph, double* const* K_vals, double* const* f,
+                             void* stream);
+
+/* Engine-owned K/f device buffers of a step. */
+int ife_physics_Kf(ife_physics* ph, int32_t step, double** K_vals, double** f);
+
+/* Synchronises `stream` and returns the status of the last assemble call. */
+int ife_physics_status(ife_physics* ph, void* stream, ife_assembly_status* out);
+
+/* Enable/disable per-launch event timing (off by default). */
+int ife_physics_set_profiling(ife_physics* ph, int32_t enable);
+
+/* Synchronises the recorded launches, returns their timings and clears the record. */
+int ife_physics_timings(ife_physics* ph, ife_timings* out);
+
+/* ---- per-IP history (model-owned, PhaseField.h:117-120) ---------------------------- */
+
+/* Device pointer and length of a history array, layout [component][element][ip].
+ * Names: "Energy", "EnergyOld", "PlasticStrain", "PlasticStrainOld" (6 comps),
+ * "ViscDiss", "ViscDissOld", "DamViscDiss", "DamViscDissOld", "MatHist",
+ * "MatHistOld" (HistSize comps). */
+int ife_model_history(ife_physics* ph, int32_t model, const char* name, double** dev_ptr,
+                      int64_t* n_components, int64_t* n_per_component);
+
+/* BaseModel::Commit (PhaseField.cpp:123-145): current -> old (commit_type 1 = TIMEDEP). */
+int ife_model_commit(ife_physics* ph, int32_t model, int32_t commit_type, void* stream);
+
+/* BaseModel::ResetStep (PhaseField.cpp:147-163): old -> current. */
+int ife_model_reset_step(ife_physics* ph, int32_t model, void* stream);
+
+const char* ife_last_error(void);
+int ife_api_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IFE_H */

@@ -1,0 +1,1668 @@
+// ife_oracle.cpp -- FP64 CPU oracle for the IceFEM element-assembly path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, function by function, the reference
+// algorithm the engine replaces, so that tests can check the engine against it.  Only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+// engine never links, calls or falls back to it.
+//
+// Every function cites the reference file:line it follows (paths relative to the IceFEM
+// source tree).  Eigen/PETSc are absent here, so dense algebra is restated by hand:
+// fixed-size inverses via partial-pivot LU (Eigen's inverse() for sizes > 4), 3x3 via
+// cofactors (Eigen's size-3 helper), and PETSc's MatSetValues(ADD_VALUES) via sorted
+// per-row insertion in call order.  Rounding-level agreement with Eigen/PETSc is
+// therefore "parity unpinned"; the oracle is pinned by closed forms and invariants in
+// tests/test_oracle_*.py instead.
+//
+// Cost structure kept on purpose (it doubles as the CPU baseline): dense per-IP
+// products in Eigen's evaluation order, shape functions recomputed on every call,
+// full 7x7 inverses in every return-map iteration.  DOF lookup uses arrays instead of
+// std::map (stated deviation).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/ife.h"
+
+namespace ifo {
+
+// --------------------------------------------------------------------------------
+// small dense helpers (column-major like Eigen's default storage)
+// --------------------------------------------------------------------------------
+struct Mat {
+    int r = 0, c = 0;
+    std::vector<double> a;  // column-major: a[i + r*j]
+    Mat() {}
+    Mat(int r_, int c_) : r(r_), c(c_), a((size_t)r_ * c_, 0.0) {}
+    double& operator()(int i, int j) { return a[i + (size_t)r * j]; }
+    double operator()(int i, int j) const { return a[i + (size_t)r * j]; }
+    void zero() { std::fill(a.begin(), a.end(), 0.0); }
+};
+using Vec = std::vector<double>;
+
+typedef double M6[6][6];   // row-major [row][col] small fixed matrices
+typedef double V6[6];
+
+static double sgn(double x) {  // utility/math.cpp:101-105
+    if (x > 0) return 1;
+    if (x < 0) return -1;
+    return 0;
+}
+
+// Eigen::PartialPivLU::inverse() restated for an n x n row-major matrix.
+static bool lu_inverse(int n, const double* A, double* Ainv) {
+    std::vector<double> LU(A, A + n * n);
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; i++) perm[i] = i;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = std::fabs(LU[k * n + k]);
+        for (int i = k + 1; i < n; i++) {
+            double v = std::fabs(LU[i * n + k]);
+            if (v > best) { best = v; p = i; }
+        }
+        if (p != k) {
+            for (int j = 0; j < n; j++) std::swap(LU[k * n + j], LU[p * n + j]);
+            std::swap(perm[k], perm[p]);
+        }
+        double piv = LU[k * n + k];
+        if (piv != 0.0) {
+            for (int i = k + 1; i < n; i++) {
+                LU[i * n + k] /= piv;
+                double lik = LU[i * n + k];
+                for (int j = k + 1; j < n; j++) LU[i * n + j] -= lik * LU[k * n + j];
+            }
+        }
+    }
+    // solve LU X = P I, column by column
+    for (int col = 0; col < n; col++) {
+        std::vector<double> y(n);
+        for (int i = 0; i < n; i++) y[i] = (perm[i] == col) ? 1.0 : 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < i; j++) y[i] -= LU[i * n + j] * y[j];
+        for (int i = n - 1; i >= 0; i--) {
+            for (int j = i + 1; j < n; j++) y[i] -= LU[i * n + j] * y[j];
+            y[i] /= LU[i * n + i];
+        }
+        for (int i = 0; i < n; i++) Ainv[i * n + col] = y[i];
+    }
+    return true;
+}
+
+// Eigen's cofactor-based 3x3 inverse and determinant (row-major in/out).
+static double det3(const double J[3][3]) {
+    return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+           J[1][0] * (J[0][1] * J[2][2] - J[0][2] * J[2][1]) +
+           J[2][0] * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+}
+static void inv3(const double J[3][3], double Ji[3][3]) {
+    double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    double c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    double c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    double det = J[0][0] * c00 + J[0][1] * c10 + J[0][2] * c20;
+    double id = 1.0 / det;
+    Ji[0][0] = c00 * id;
+    Ji[1][0] = c10 * id;
+    Ji[2][0] = c20 * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+}
+static double det2(const double J[2][2]) { return J[0][0] * J[1][1] - J[0][1] * J[1][0]; }
+static void inv2(const double J[2][2], double Ji[2][2]) {
+    double id = 1.0 / det2(J);
+    Ji[0][0] = J[1][1] * id;
+    Ji[1][0] = -J[1][0] * id;
+    Ji[0][1] = -J[0][1] * id;
+    Ji[1][1] = J[0][0] * id;
+}
+
+// --------------------------------------------------------------------------------
+// Bernstein basis and Gauss rules
+// --------------------------------------------------------------------------------
+static int fact(int n) { return (n == 0) || (n == 1) ? 1 : n * fact(n - 1); }  // BaseElemType.cpp:218-221
+
+// BaseElemType.cpp:197-216
+static void BezierBasis(int p, double x_ip, double* N, double* G, double* G2) {
+    for (int i = 0; i < p + 1; i++) {
+        double preFactor = (double)(fact(p) / fact(p - i) / fact(i));
+        N[i] = preFactor * std::pow(1.0 - x_ip, std::max(0, p - i)) * std::pow(x_ip, std::max(0, i));
+        G[i] = preFactor * ((i - p) * std::pow(1.0 - x_ip, std::max(0, p - i - 1)) * std::pow(x_ip, std::max(0, i)) +
+                            i * std::pow(1.0 - x_ip, p - i) * std::pow(x_ip, std::max(0, i - 1)));
+        G2[i] = preFactor * (2 * i * std::pow(x_ip, std::max(0, i - 1)) * (i - p) * std::pow(1.0 - x_ip, std::max(0, p - i - 1)) +
+                             i * std::pow(x_ip, std::max(0, i - 2)) * (i - 1) * std::pow(1.0 - x_ip, std::max(0, p - i)) +
+                             (i - p + 1) * std::pow(x_ip, std::max(0, i)) * (i - p) * std::pow(1.0 - x_ip, std::max(0, p - i - 2)));
+    }
+}
+
+// IPScheme.cpp:62-109 (Gauss-Legendre on [-1,1] mapped to [0,1], weights halved)
+static void getIPScheme1D(std::vector<double>& w, std::vector<double>& x, int n) {
+    switch (n) {
+        case 1: x = {0}; w = {2.0}; break;
+        case 2: x = {-1 / std::sqrt(3.0), 1 / std::sqrt(3.0)}; w = {1.0, 1.0}; break;
+        case 3: x = {-std::sqrt(3.0 / 5.0), 0, std::sqrt(3.0 / 5.0)}; w = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0}; break;
+        case 4:
+            x = {-std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * std::sqrt(6.0 / 5.0)), -std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * std::sqrt(6.0 / 5.0)),
+                 std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * std::sqrt(6.0 / 5.0)), std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * std::sqrt(6.0 / 5.0))};
+            w = {(18.0 - std::sqrt(30.0)) / 36.0, (18.0 + std::sqrt(30.0)) / 36.0, (18.0 + std::sqrt(30.0)) / 36.0,
+                 (18.0 - std::sqrt(30.0)) / 36.0};
+            break;
+        case 5:
+            x = {-1.0 / 3.0 * std::sqrt(5.0 + 2.0 * std::sqrt(10.0 / 7.0)), -1.0 / 3.0 * std::sqrt(5.0 - 2.0 * std::sqrt(10.0 / 7.0)), 0.0,
+                 1.0 / 3.0 * std::sqrt(5.0 - 2.0 * std::sqrt(10.0 / 7.0)), 1.0 / 3.0 * std::sqrt(5.0 + 2.0 * std::sqrt(10.0 / 7.0))};
+            w = {(322.0 - 13.0 * std::sqrt(70.0)) / 900.0, (322.0 + 13.0 * std::sqrt(70.0)) / 900.0, 128.0 / 225.0,
+                 (322.0 + 13.0 * std::sqrt(70.0)) / 900.0, (322.0 - 13.0 * std::sqrt(70.0)) / 900.0};
+            break;
+        default: throw std::invalid_argument("ip scheme not implemented");
+    }
+    for (size_t i = 0; i < w.size(); i++) {
+        x[i] = (x[i] + 1) / 2;
+        w[i] = w[i] / 2;
+    }
+}
+
+// --------------------------------------------------------------------------------
+// Element types: base shapes (x-fastest Kronecker, math.cpp:33-61) and getShapeGrads
+// --------------------------------------------------------------------------------
+struct ElemType {
+    int dim = 0, n = 0, nip = 0;
+    std::vector<double> w;      // nip  (IPScheme.cpp:9-56)
+    std::vector<double> Nbase;  // nip x n
+    std::vector<double> Gbase;  // nip x dim x n
+
+    void init(int dim_, int ip1d) {  // B2_2.cpp:16-70, NURBS3_Cube2.cpp:17-87
+        dim = dim_;
+        n = (dim == 2) ? 9 : 27;
+        std::vector<double> w1, x1;
+        getIPScheme1D(w1, x1, ip1d);
+        nip = (dim == 2) ? ip1d * ip1d : ip1d * ip1d * ip1d;
+        w.assign(nip, 0);
+        Nbase.assign((size_t)nip * n, 0);
+        Gbase.assign((size_t)nip * dim * n, 0);
+        int cnt = 0;
+        int nk = (dim == 2) ? 1 : ip1d;
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < ip1d; j++)
+                for (int i = 0; i < ip1d; i++) {
+                    double coords[3] = {x1[i], x1[j], dim == 3 ? x1[k] : 0.0};
+                    w[cnt] = (dim == 2) ? w1[i] * w1[j] : w1[i] * w1[j] * w1[k];
+                    double Nx[3], Gx[3], G2x[3], Ny[3], Gy[3], G2y[3], Nz[3] = {1, 0, 0}, Gz[3] = {0, 0, 0}, G2z[3];
+                    BezierBasis(2, coords[0], Nx, Gx, G2x);
+                    BezierBasis(2, coords[1], Ny, Gy, G2y);
+                    if (dim == 3) BezierBasis(2, coords[2], Nz, Gz, G2z);
+                    int nz = (dim == 2) ? 1 : 3;
+                    int c = 0;
+                    for (int kk = 0; kk < nz; kk++)
+                        for (int jj = 0; jj < 3; jj++)
+                            for (int ii = 0; ii < 3; ii++) {
+                                double zN = (dim == 2) ? 1.0 : Nz[kk];
+                                Nbase[(size_t)cnt * n + c] = (dim == 2) ? Nx[ii] * Ny[jj] : Nx[ii] * Ny[jj] * zN;
+                                Gbase[((size_t)cnt * dim + 0) * n + c] = (dim == 2) ? Gx[ii] * Ny[jj] : Gx[ii] * Ny[jj] * Nz[kk];
+                                Gbase[((size_t)cnt * dim + 1) * n + c] = (dim == 2) ? Nx[ii] * Gy[jj] : Nx[ii] * Gy[jj] * Nz[kk];
+                                if (dim == 3) Gbase[((size_t)cnt * dim + 2) * n + c] = Nx[ii] * Ny[jj] * Gz[kk];
+                                c++;
+                            }
+                    cnt++;
+                }
+    }
+};
+
+// Output of one getShapeGrads call: N[ip][a], G[ip][c][a], w[ip]
+struct Shapes {
+    int nip = 0, n = 0, dim = 0;
+    std::vector<double> N, G, w;
+    void resize(int nip_, int n_, int dim_) {
+        nip = nip_; n = n_; dim = dim_;
+        N.assign((size_t)nip * n, 0);
+        G.assign((size_t)nip * dim * n, 0);
+        w.assign(nip, 0);
+    }
+    double Nv(int ip, int a) const { return N[(size_t)ip * n + a]; }
+    double Gv(int ip, int c, int a) const { return G[((size_t)ip * dim + c) * n + a]; }
+};
+
+// B2_2::getShapeGrads (B2_2.cpp:72-88): N = Nbase*C^T, GB = Gbase*C^T, J = (GB*X)^T,
+// G = inv(J)*GB  (the transposed-Jacobian form is reproduced as written).
+static void shapeGrads_B2_2(const ElemType& et, const double* X, const double* C, Shapes& S) {
+    const int n = et.n;
+    for (int ip = 0; ip < et.nip; ip++) {
+        for (int j = 0; j < n; j++) {
+            double s = 0;
+            for (int k = 0; k < n; k++) s += et.Nbase[(size_t)ip * n + k] * C[j * n + k];
+            S.N[(size_t)ip * n + j] = s;
+        }
+        double GB[2][9];
+        for (int c = 0; c < 2; c++)
+            for (int j = 0; j < n; j++) {
+                double s = 0;
+                for (int k = 0; k < n; k++) s += et.Gbase[((size_t)ip * 2 + c) * n + k] * C[j * n + k];
+                GB[c][j] = s;
+            }
+        double J0[2][2];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                double s = 0;
+                for (int j = 0; j < n; j++) s += GB[a][j] * X[j * 2 + b];
+                J0[a][b] = s;
+            }
+        double J[2][2] = {{J0[0][0], J0[1][0]}, {J0[0][1], J0[1][1]}};  // transposeInPlace
+        double Ji[2][2];
+        inv2(J, Ji);
+        S.w[ip] = et.w[ip] * std::fabs(det2(J));
+        for (int j = 0; j < n; j++)
+            for (int a = 0; a < 2; a++)
+                S.G[((size_t)ip * 2 + a) * n + j] = Ji[a][0] * GB[0][j] + Ji[a][1] * GB[1][j];
+    }
+}
+
+// NURBS3_Cube2::getShapeGrads (NURBS3_Cube2.cpp:89-115)
+static void shapeGrads_NURBS3_Cube2(const ElemType& et, const double* X, const double* C, const double* W, Shapes& S) {
+    const int n = et.n;
+    for (int ip = 0; ip < et.nip; ip++) {
+        double CB[27], GB[27][3];
+        for (int j = 0; j < n; j++) {
+            double s = 0;
+            for (int k = 0; k < n; k++) s += C[j * n + k] * et.Nbase[(size_t)ip * n + k];
+            CB[j] = s;
+        }
+        double wFun = 0;
+        for (int j = 0; j < n; j++) wFun += CB[j] * W[j];
+        for (int j = 0; j < n; j++) S.N[(size_t)ip * n + j] = (W[j] * CB[j] / wFun);
+        for (int j = 0; j < n; j++)
+            for (int c = 0; c < 3; c++) {
+                double s = 0;
+                for (int k = 0; k < n; k++) s += C[j * n + k] * et.Gbase[((size_t)ip * 3 + c) * n + k];
+                GB[j][c] = s;
+            }
+        double dW[3];
+        for (int c = 0; c < 3; c++) {
+            double s = 0;
+            for (int j = 0; j < n; j++) s += GB[j][c] * W[j];
+            dW[c] = s;
+        }
+        double GXi[27][3];
+        for (int j = 0; j < n; j++)
+            for (int c = 0; c < 3; c++) GXi[j][c] = W[j] * (GB[j][c] / wFun - CB[j] * dW[c] / wFun / wFun);
+        double J[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double s = 0;
+                for (int j = 0; j < n; j++) s += GXi[j][a] * X[j * 3 + b];
+                J[a][b] = s;
+            }
+        double Ji[3][3];
+        inv3(J, Ji);
+        S.w[ip] = et.w[ip] * std::fabs(det3(J));
+        for (int j = 0; j < n; j++)
+            for (int a = 0; a < 3; a++)
+                S.G[((size_t)ip * 3 + a) * n + j] = Ji[a][0] * GXi[j][0] + Ji[a][1] * GXi[j][1] + Ji[a][2] * GXi[j][2];
+    }
+}
+
+// ConstructBMat (utility/math.cpp:3-26): Voigt rows xx,yy,zz,xy,yz,xz
+static void ConstructBMat(Mat& B, const Shapes& S, int ip, int dim) {
+    const int n = S.n;
+    B.zero();
+    for (int a = 0; a < n; a++) {
+        double g0 = S.Gv(ip, 0, a), g1 = S.Gv(ip, 1, a);
+        if (dim == 2) {
+            B(0, a) = g0;
+            B(3, a) = g1;
+            B(1, n + a) = g1;
+            B(3, n + a) = g0;
+        } else {
+            double g2 = S.Gv(ip, 2, a);
+            B(0, a) = g0; B(3, a) = g1; B(5, a) = g2;
+            B(1, n + a) = g1; B(3, n + a) = g0; B(4, n + a) = g2;
+            B(2, 2 * n + a) = g2; B(4, 2 * n + a) = g1; B(5, 2 * n + a) = g0;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------
+// Materials (SolidMaterial / PhaseFieldMaterial / IceMaterial / ThermalMaterial)
+// --------------------------------------------------------------------------------
+struct Material {
+    ife_material in;
+    // SolidMaterial.cpp:5-107
+    double Young, Poisson, Shear, Lame, Bulk, Damping, ACreep, nCreep, PhiViscLim, Density;
+    int Rheology;
+    int HistSize;
+    double D[6][6], D0[6][6], J2Mat_stress[6][6];
+    double VoightCorrect[6], VoightDoubleDot[6];
+    // PhaseFieldMaterial.cpp:3-31
+    double pf_l, Gc, pf_visc;
+    int StressSplit;
+    bool ConsistentEnergy;
+    // IceMaterial.cpp:5-45
+    double TRef, TOffset, A0, QCreep, ft0, ftdeg, sy, sy0, sydeg, Young0;
+    const double R = 8.31446261815324;  // BaseMaterial.h:26
+    // ThermalMaterial.cpp:5-12
+    double k, cp;
+    // PF utility used by dVE (SolidMaterial.cpp:35-46)
+    int DegFunction;
+    double k0, p;
+
+    void init(const ife_material& m, const ife_phasefield_opts* pf) {
+        in = m;
+        Young = m.Young; Poisson = m.Poisson; Density = m.Density;
+        Shear = Young / (2.0 * (1 + Poisson));
+        Lame = Young * Poisson / ((1.0 + Poisson) * (1.0 - 2.0 * Poisson));
+        Bulk = Lame + 2.0 / 3.0 * Shear;
+        HistSize = 0;
+        Damping = m.Damping;
+        Rheology = m.Rheology;
+        ACreep = m.A; nCreep = m.n; PhiViscLim = m.PhiViscLim;
+        if (Rheology == IFE_RHEO_VE || Rheology == IFE_RHEO_DVE || Rheology == IFE_RHEO_VE_EXPL ||
+            Rheology == IFE_RHEO_DVE_EXPL || Rheology == IFE_RHEO_VEP || Rheology == IFE_RHEO_DVEP)
+            HistSize = 1;
+        if (Rheology == IFE_RHEO_VEP || Rheology == IFE_RHEO_DVEP) HistSize = 4;
+        DegFunction = pf ? pf->DamFunction : IFE_DAM_NONE;
+        k0 = pf ? pf->k0 : 0.0;
+        p = pf ? pf->p : 2.0;
+        for (int i = 0; i < 6; i++) { VoightCorrect[i] = i < 3 ? 1.0 : 0.5; VoightDoubleDot[i] = i < 3 ? 1.0 : 2.0; }
+        // D = Lame*i*i^T + 2*Shear*I; D(3..5,3..5) = Shear
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) {
+                double ii = (i < 3 && j < 3) ? 1.0 : 0.0;
+                D[i][j] = Lame * ii + 2 * Shear * (i == j ? 1.0 : 0.0);
+            }
+        D[3][3] = Shear; D[4][4] = Shear; D[5][5] = Shear;
+        // J2Mat_stress = I - 1/3 i i^T, (3..5,3..5) diag = 2
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) {
+                double ii = (i < 3 && j < 3) ? 1.0 : 0.0;
+                J2Mat_stress[i][j] = (i == j ? 1.0 : 0.0) - 1.0 / 3.0 * ii;
+            }
+        J2Mat_stress[3][3] = 2.0; J2Mat_stress[4][4] = 2.0; J2Mat_stress[5][5] = 2.0;
+        pf_l = m.l; Gc = m.Gc; pf_visc = m.Viscosity;
+        StressSplit = m.Split;
+        ConsistentEnergy = pf ? (pf->ConsistentEnergy != 0) : true;
+        TOffset = m.TOffset; TRef = m.TRef;
+        A0 = ACreep; QCreep = m.QCreep; ft0 = m.ft0; ftdeg = m.ftdeg;
+        sy = m.sy; sy0 = sy; sydeg = m.sydeg;
+        std::memcpy(D0, D, sizeof(D));
+        Young0 = Young;
+        k = m.ThermalConductivity; cp = m.HeatCapacity;
+        SetTemperature(TRef, 0.0);  // IceMaterial.cpp:43
+    }
+
+    // IceMaterial::SetTemperature (IceMaterial.cpp:49-84)
+    void SetTemperature(double T, double poros) {
+        double dam = 1 - poros;
+        double TAbs = T + TOffset;
+        if (TAbs > 273.15) TAbs = 273.15;
+        Young = Young0 * dam;
+        Shear = Young / (2.0 * (1 + Poisson));
+        Lame = Young * Poisson / ((1.0 + Poisson) * (1.0 - 2.0 * Poisson));
+        Bulk = Lame + 2.0 / 3.0 * Shear;
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) D[i][j] = dam * D0[i][j];
+        double ExpTerm = -QCreep / R * (1 / TAbs - 1 / TRef);
+        ACreep = A0 * std::exp(ExpTerm);
+        double ft = dam * (ft0 - ftdeg * (TAbs - TRef));
+        Gc = 2.0 * ft * ft / Young * pf_l;
+        sy = dam * (sy0 - sydeg * (TAbs - TRef));
+    }
+
+    // PhaseFieldUtility::GenericDamageFunction (PhaseFieldUtil.cpp:77-183)
+    static void Damage(int damfunc, double phi, double k0, double p, double& D, double& dD, double& ddD) {
+        double phiLim = phi;
+        if (phiLim < 0.0) phiLim = 0.0;
+        if (phiLim > 1.0) phiLim = 1.0;
+        switch (damfunc) {
+            case IFE_DAM_NONE: D = 1.0; dD = 0.0; ddD = 0.0; break;
+            case IFE_DAM_LINEAR: D = (1.0 - k0) * (1.0 - phiLim) + k0; dD = -1.0 * (1.0 - k0); ddD = 0.0; break;
+            case IFE_DAM_LINEAR_NR: D = (1.0 - phiLim); dD = -1.0; ddD = 0.0; break;
+            case IFE_DAM_QUADRATIC:
+                D = (1.0 - k0) * std::pow((1.0 - phiLim), 2) + k0; dD = -2.0 * (1.0 - k0) * (1.0 - phi); ddD = 2.0 * (1.0 - k0); break;
+            case IFE_DAM_QUADRATIC_NR: D = std::pow((1.0 - phiLim), 2); dD = -2.0 * (1.0 - phi); ddD = 2.0; break;
+            case IFE_DAM_CUBIC:
+                D = (1.0 - k0) * std::pow((1.0 - phiLim), 3) + k0;
+                dD = -3.0 * (1.0 - k0) * (1.0 - phi) * (1.0 - phi);
+                ddD = 6.0 * (1.0 - k0) * (1.0 - phi);
+                break;
+            case IFE_DAM_CUBIC_NR:
+                D = std::pow((1.0 - phiLim), 3); dD = -3.0 * (1.0 - phi) * (1.0 - phi); ddD = 6.0 * (1.0 - phi); break;
+            case IFE_DAM_PORDER:
+                D = (1.0 - k0) * std::pow((1.0 - phiLim), p) + k0;
+                dD = -p * (1.0 - k0) * std::pow((1.0 - phi), p - 1);
+                ddD = (p > 1) ? p * (p - 1) * (1.0 - k0) * std::pow((1.0 - phi), p - 2) : 0.0;
+                break;
+            case IFE_DAM_CZM:
+                D = (1.0 - k0) * (2.0 * std::pow(1 - phi, 2) - 1.0 * std::pow(1 - phi, 3)) + k0;
+                dD = (1.0 - k0) * (3.0 * (1 - phi) * (1 - phi) - 4.0 * (1 - phi));
+                ddD = (1.0 - k0) * (-6.0 * (1 - phi) + 4.0);
+                break;
+            case IFE_DAM_STEP2: D = (phi < 0.2) ? 1.0 : 0.0; dD = 0.0; ddD = 0.0; break;
+            case IFE_DAM_STEP5: D = (phi < 0.5) ? 1.0 : 0.0; dD = 0.0; ddD = 0.0; break;
+            case IFE_DAM_STEP8: D = (phi < 0.8) ? 1.0 : 0.0; dD = 0.0; ddD = 0.0; break;
+            default: throw std::invalid_argument("damage function not defined");
+        }
+    }
+
+    // SolidMaterial::YieldFunctionVE (SolidMaterial.cpp:477-492); ddF is not returned
+    // because return_mapping_getKF zeroes it (:289).
+    double YieldFunctionVE(const V6 Stress, V6 dF) const {
+        const double sc = 1.0e6, eps = 1.0e-12;
+        V6 s;
+        for (int i = 0; i < 6; i++) s[i] = Stress[i] / sc;
+        V6 dJ2;
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += J2Mat_stress[i][j] * s[j];
+            dJ2[i] = t;
+        }
+        double J2 = 0;
+        for (int i = 0; i < 6; i++) J2 += s[i] * dJ2[i];
+        J2 *= 0.5;
+        double F = sc * std::sqrt(2.0 * J2);
+        for (int i = 0; i < 6; i++) dF[i] = 0.5 * std::sqrt(2.0) / std::sqrt(J2 + eps) * dJ2[i];
+        return F;
+    }
+
+    // SolidMaterial::return_mapping_getKF (SolidMaterial.cpp:278-301), row-major 7x7 K
+    void getKF(double K[49], double f[7], const double sol[7], const V6 strain_total, const V6 strain_ve_old, double Damage,
+               double dt) const {
+        V6 Stress;
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += D[i][j] * sol[j];
+            Stress[i] = t;
+        }
+        V6 dF;
+        double F_VE = YieldFunctionVE(Stress, dF);
+        for (int i = 0; i < 6; i++) f[i] = sol[i] - strain_total[i] + strain_ve_old[i] + sol[6] * dF[i];
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) K[i * 7 + j] = (i == j) ? 1.0 : 0.0;  // I + Damage*sol6*0*D
+        for (int i = 0; i < 6; i++) K[i * 7 + 6] = dF[i];
+        const double sc = 1.0e6;
+        double PreFac = ACreep * dt * std::pow(sc, nCreep);
+        f[6] = PreFac * std::pow(F_VE / sc, nCreep) - sol[6];
+        K[6 * 7 + 6] = -1.0;
+        double fac = nCreep * PreFac / sc * std::pow(F_VE / sc, nCreep - 1.0);
+        for (int j = 0; j < 6; j++) {
+            double t = 0;
+            for (int i = 0; i < 6; i++) t += Damage * dF[i] * D[i][j];
+            K[6 * 7 + j] = fac * t;
+        }
+    }
+
+    // SolidMaterial::return_mapping_Viscous (SolidMaterial.cpp:200-269)
+    double return_mapping_Viscous(M6 DMat, V6 Strain_Pl, const V6 Strain, const V6 Strain_Pl_Old, double Damage, double dt,
+                                  double* hist, int& iters, bool& maxit) const {
+        double K[49], Kinv[49], f[7], fOld[7], sol[7], dsol[7];
+        for (int i = 0; i < 6; i++) sol[i] = Strain[i] - Strain_Pl[i];
+        sol[6] = hist[0];
+        getKF(K, f, sol, Strain, Strain_Pl_Old, Damage, dt);
+        double e0 = 0;
+        for (int i = 0; i < 7; i++) e0 += f[i] * f[i];
+        e0 = std::max(e0, 1e-12);
+        bool conv = false;
+        unsigned it = 0;
+        maxit = false;
+        while (conv == false) {
+            it += 1;
+            lu_inverse(7, K, Kinv);
+            for (int i = 0; i < 7; i++) {
+                double t = 0;
+                for (int j = 0; j < 7; j++) t += Kinv[i * 7 + j] * f[j];
+                dsol[i] = -t;
+            }
+            for (int i = 0; i < 7; i++) sol[i] += dsol[i];
+            for (int i = 0; i < 7; i++) fOld[i] = f[i];
+            getKF(K, f, sol, Strain, Strain_Pl_Old, Damage, dt);
+            double ls1 = 0, ls2 = 0;
+            for (int i = 0; i < 7; i++) { ls1 += f[i] * dsol[i]; ls2 += fOld[i] * dsol[i]; }
+            double lineSearch;
+            if ((ls1 - ls2) == 0.0) {
+                lineSearch = 1.0;
+            } else {
+                lineSearch = -(ls2) / (std::fabs(ls1 - ls2) + 1e-12) * sgn(ls1 - ls2);
+                lineSearch = std::min(std::max(lineSearch, 0.2), 1.0);
+            }
+            for (int i = 0; i < 7; i++) sol[i] = sol[i] - (1.0 - lineSearch) * dsol[i];
+            getKF(K, f, sol, Strain, Strain_Pl_Old, Damage, dt);
+            double e = 0;
+            for (int i = 0; i < 7; i++) e += f[i] * f[i];
+            if (e < 1.0e-16 || e / e0 < 1.0e-9) conv = true;
+            if (it > 250) { conv = true; maxit = true; }
+        }
+        iters = (int)it;
+        V6 Stress;
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += D[i][j] * sol[j];
+            Stress[i] = t;
+        }
+        V6 dF;
+        YieldFunctionVE(Stress, dF);
+        for (int i = 0; i < 6; i++) Strain_Pl[i] = Strain_Pl_Old[i] + sol[6] * dF[i];
+        lu_inverse(7, K, Kinv);
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) DMat[i][j] = Kinv[i * 7 + j];
+        double Diss = 0;
+        for (int i = 0; i < 6; i++) Diss += Stress[i] * (Strain_Pl[i] - Strain_Pl_Old[i]);
+        Diss = Damage * Diss;
+        Diss = std::fabs(Diss) / dt;
+        hist[0] = sol[6];
+        return Diss;
+    }
+
+    // SolidMaterial::UpdatePlasticStrains (SolidMaterial.cpp:123-192)
+    void UpdatePlasticStrains(M6 DMat, V6 Strain_Pl, const V6 Strain, const V6 Strain_Pl_Old, const V6 StrainOld, double phi,
+                              double phiOld, double dt, double* hist, double& DissRate, double& DamDissRate, int& iters,
+                              bool& maxit, bool& called) const {
+        DissRate = 0.0; DamDissRate = 0.0;
+        double damage = 1.0, u1, u2;
+        iters = 0; maxit = false; called = false;
+        auto identity = [&]() {
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) DMat[i][j] = (i == j) ? 1.0 : 0.0;
+            for (int i = 0; i < 6; i++) Strain_Pl[i] = 1.0 * Strain_Pl_Old[i];
+        };
+        switch (Rheology) {
+            case IFE_RHEO_LE: identity(); DissRate = 0.0; break;
+            case IFE_RHEO_VE:
+                if (phi > PhiViscLim) identity();
+                else { called = true; DissRate = return_mapping_Viscous(DMat, Strain_Pl, Strain, Strain_Pl_Old, 1.0, dt, hist, iters, maxit); }
+                break;
+            case IFE_RHEO_DVE:
+                if (phi > PhiViscLim) identity();
+                else {
+                    Damage(DegFunction, phi, k0, p, damage, u1, u2);
+                    called = true;
+                    DissRate = return_mapping_Viscous(DMat, Strain_Pl, Strain, Strain_Pl_Old, damage, dt, hist, iters, maxit);
+                }
+                break;
+            case IFE_RHEO_VE_EXPL:
+                if (phiOld > PhiViscLim) identity();
+                else {
+                    called = true;
+                    DissRate = return_mapping_Viscous(DMat, Strain_Pl, StrainOld, Strain_Pl_Old, 1.0, dt, hist, iters, maxit);
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++) DMat[i][j] = (i == j) ? 1.0 : 0.0;
+                }
+                break;
+            case IFE_RHEO_DVE_EXPL:
+                if (phiOld > PhiViscLim) identity();
+                else {
+                    Damage(DegFunction, phiOld, k0, p, damage, u1, u2);
+                    called = true;
+                    DissRate = return_mapping_Viscous(DMat, Strain_Pl, StrainOld, Strain_Pl_Old, damage, dt, hist, iters, maxit);
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++) DMat[i][j] = (i == j) ? 1.0 : 0.0;
+                }
+                break;
+            default: throw std::invalid_argument("rheology not supported by the oracle");
+        }
+    }
+
+    // PhaseFieldMaterial::EnergySplit (PhaseFieldMaterial.cpp:47-94) with VolStress
+    // (:97-117) and VolStrains (:194-215).
+    double EnergySplit(M6 D_Dam, M6 D_UnDam, V6 S_Dam, V6 S_Undam, const V6 Strain) const {
+        double Energy = 0;
+        for (int i = 0; i < 6; i++) {
+            S_Dam[i] = 0; S_Undam[i] = 0;
+            for (int j = 0; j < 6; j++) { D_Dam[i][j] = 0; D_UnDam[i][j] = 0; }
+        }
+        auto Dstrain = [&](V6 out) {
+            for (int i = 0; i < 6; i++) {
+                double t = 0;
+                for (int j = 0; j < 6; j++) t += D[i][j] * Strain[j];
+                out[i] = t;
+            }
+        };
+        switch (StressSplit) {
+            case IFE_SPLIT_NONE: {
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 6; j++) D_Dam[i][j] = D[i][j];
+                Dstrain(S_Dam);
+                double t = 0;
+                for (int i = 0; i < 6; i++) t += S_Dam[i] * Strain[i];
+                Energy = 0.5 * t;
+            } break;
+            case IFE_SPLIT_VOLSTRAINS: {
+                double p = Strain[0] + Strain[1] + Strain[2];
+                if (p >= 0) {
+                    Energy += 0.5 * Bulk * p * p;
+                    for (int i = 0; i < 3; i++) S_Dam[i] += Bulk * p;
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) D_Dam[i][j] += Bulk;
+                } else {
+                    for (int i = 0; i < 3; i++) S_Undam[i] += Bulk * p;
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) D_UnDam[i][j] += Bulk;
+                }
+                V6 Dev;
+                for (int i = 0; i < 6; i++) Dev[i] = VoightCorrect[i] * Strain[i] - (i < 3 ? 1.0 / 3.0 * p : 0.0);
+                double t = 0;
+                for (int i = 0; i < 6; i++) t += Dev[i] * VoightDoubleDot[i] * Dev[i];
+                Energy += Shear * t;
+                for (int i = 0; i < 6; i++) S_Dam[i] += 2 * Shear * VoightCorrect[i] * VoightDoubleDot[i] * Dev[i];
+                // 2*Shear*VC*VDD*VC*(I - 1/3 i i^T)
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 6; j++) {
+                        double P = (i == j ? 1.0 : 0.0) - ((i < 3 && j < 3) ? 1.0 / 3.0 : 0.0);
+                        D_Dam[i][j] += 2 * Shear * VoightCorrect[i] * VoightDoubleDot[i] * VoightCorrect[i] * P;
+                    }
+            } break;
+            case IFE_SPLIT_VOLSTRESS: {
+                V6 Stress;
+                Dstrain(Stress);
+                double p = 1.0 / 3.0 * (Stress[0] + Stress[1] + Stress[2]);
+                // (1/3) i i^T D  and  (I - 1/3 i i^T) D
+                M6 iiD;
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 6; j++) {
+                        double t = 0;
+                        if (i < 3) for (int k = 0; k < 3; k++) t += D[k][j];
+                        iiD[i][j] = t;
+                    }
+                if (p >= 0) {
+                    for (int i = 0; i < 3; i++) S_Dam[i] += p;
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++) D_Dam[i][j] += 1.0 / 3.0 * iiD[i][j];
+                } else {
+                    for (int i = 0; i < 3; i++) S_Undam[i] += p;
+                    for (int i = 0; i < 6; i++)
+                        for (int j = 0; j < 6; j++) D_UnDam[i][j] += 1.0 / 3.0 * iiD[i][j];
+                }
+                for (int i = 0; i < 6; i++) S_Dam[i] += Stress[i] - (i < 3 ? p : 0.0);
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 6; j++) D_Dam[i][j] += D[i][j] - 1.0 / 3.0 * iiD[i][j];
+                double t = 0;
+                for (int i = 0; i < 6; i++) t += S_Dam[i] * Strain[i];
+                Energy = 0.5 * t;
+            } break;
+            default: throw std::invalid_argument("energy split not supported by the oracle");
+        }
+        if (ConsistentEnergy == false) {
+            Dstrain(S_Dam);
+            for (int i = 0; i < 6; i++) {
+                S_Undam[i] = 0;
+                for (int j = 0; j < 6; j++) { D_Dam[i][j] = D[i][j]; D_UnDam[i][j] = 0; }
+            }
+        }
+        return Energy;
+    }
+};
+
+// PhaseFieldUtility::GetDistributionFunction (PhaseFieldUtil.cpp:185-221), non-HO forms.
+// Dist (n), dDist (n x n column-major)
+static double GetDistributionFunction(int method, double l, int dim, Vec& Dist, Mat& dDist, const Shapes& S, int ip,
+                                      const double* Phase) {
+    const int n = S.n;
+    double phi = 0;
+    for (int a = 0; a < n; a++) phi += S.Nv(ip, a) * Phase[a];
+    double dG[3] = {0, 0, 0};
+    for (int c = 0; c < dim; c++) {
+        double t = 0;
+        for (int a = 0; a < n; a++) t += S.Gv(ip, c, a) * Phase[a];
+        dG[c] = t;
+    }
+    double dGdG = 0;
+    for (int c = 0; c < dim; c++) dGdG += dG[c] * dG[c];
+    auto GtdG = [&](int a) { double t = 0; for (int c = 0; c < dim; c++) t += S.Gv(ip, c, a) * dG[c]; return t; };
+    auto GtG = [&](int a, int b) { double t = 0; for (int c = 0; c < dim; c++) t += S.Gv(ip, c, a) * S.Gv(ip, c, b); return t; };
+    double gamma = 0;
+    switch (method) {
+        case IFE_DIST_QUADRATIC:
+            gamma = 1.0 / (2.0 * l) * (phi * phi + l * l * dGdG);
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (2.0 * l) * (2.0 * S.Nv(ip, a) * phi + 2.0 * l * l * GtdG(a));
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++)
+                    dDist(a, b) = 1.0 / (2.0 * l) * (2.0 * S.Nv(ip, a) * S.Nv(ip, b) + 2.0 * l * l * GtG(a, b));
+            break;
+        case IFE_DIST_CZM:
+            gamma = 1.0 / (3.141592 * l) * (2.0 * phi - phi * phi + l * l * dGdG);
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (3.141592 * l) * (2.0 * S.Nv(ip, a) * (1.0 - phi) + 2.0 * l * l * GtdG(a));
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++)
+                    dDist(a, b) = 1.0 / (3.141592 * l) * (-2.0 * S.Nv(ip, a) * S.Nv(ip, b) + 2.0 * l * l * GtG(a, b));
+            break;
+        case IFE_DIST_LINEAR:
+            gamma = 3.0 / (8.0 * l) * (phi + l * l * dGdG);
+            for (int a = 0; a < n; a++) Dist[a] = 3.0 / (8.0 * l) * (S.Nv(ip, a) + 2.0 * l * l * GtdG(a));
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = 3.0 / (8.0 * l) * (2.0 * l * l * GtG(a, b));
+            break;
+        case IFE_DIST_LINEAR2:
+            gamma = 1.0 / (4.0 * l) * (2.0 * phi + l * l * dGdG);
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (4.0 * l) * (2.0 * S.Nv(ip, a) + 2.0 * l * l * GtdG(a));
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / (4.0 * l) * (2.0 * l * l * GtG(a, b));
+            break;
+        case IFE_DIST_LOCAL:
+            gamma = 1.0 / (l) * (phi * phi);
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (l) * (2 * S.Nv(ip, a) * phi);
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / (l) * (2 * S.Nv(ip, a) * S.Nv(ip, b));
+            break;
+        default: throw std::invalid_argument("distribution function not supported by the oracle");
+    }
+    return gamma;
+}
+
+// PhaseFieldUtility::GetCapacityOnly (PhaseFieldUtil.cpp:282-348), non-HO forms
+static void GetCapacityOnly(int method, double l, Vec& Dist, Mat& dDist, double& DL, double& dDL, const Shapes& S, int ip,
+                            double phi) {
+    const int n = S.n;
+    switch (method) {
+        case IFE_DIST_QUADRATIC:
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (2.0 * l) * (2.0 * S.Nv(ip, a) * phi);
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / (2.0 * l) * (2.0 * S.Nv(ip, a) * S.Nv(ip, b));
+            DL = 1.0 / l * phi; dDL = 1.0 / l;
+            break;
+        case IFE_DIST_CZM:
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (3.141592 * l) * (2.0 * S.Nv(ip, a) * (1.0 - phi));
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / (3.141592 * l) * (-2.0 * S.Nv(ip, a) * S.Nv(ip, b));
+            DL = 2.0 / (3.141592 * l) * (1.0 - phi); dDL = -2.0 / (3.141592 * l);
+            break;
+        case IFE_DIST_LINEAR:
+            for (int a = 0; a < n; a++) Dist[a] = 3.0 / (8.0 * l) * (S.Nv(ip, a));
+            dDist.zero();
+            DL = 3.0 / (8.0 * l); dDL = 0.0;
+            break;
+        default: throw std::invalid_argument("capacity form not supported by the oracle");
+    }
+}
+
+// --------------------------------------------------------------------------------
+// Problem: mesh, numbering, pattern, state, history, models
+// --------------------------------------------------------------------------------
+enum { MODEL_PHASEFIELD = 0, MODEL_THERMAL = 1 };
+
+struct Model {
+    int type = 0;
+    Material M;
+    ife_phasefield_opts pf{};
+    ife_model_opts opt{};
+    // PhaseFieldDamage fields (PhaseField.cpp:167-283)
+    int Step_u = -1, Step_phase = -1, Step_T = 99;
+    std::vector<int> fields_u;
+    int field_phase = -1, field_T = -1;
+    bool HasT = false;
+    double g[3] = {0, 0, 0};
+    // history [comp][elem][ip]
+    std::vector<double> Energy, EnergyOld, PS, PSOld, VD, VDOld, DVD, DVDOld, MH, MHOld;
+    double LCrack = 0;
+};
+
+struct Ctx {
+    int dim, ip1d, n, nip;
+    int64_t NN, NE;
+    std::vector<double> X, W, C;
+    std::vector<int64_t> elems;
+    ElemType et;
+    int nfields;
+    std::vector<std::string> fname;
+    std::vector<int> fstep;
+    int nsteps;
+    std::vector<std::vector<int64_t>> dofOf;  // [field][node]
+    std::vector<int64_t> nrows;                // per step
+    // pattern per step (built by insertion), then frozen into CSR
+    std::vector<std::vector<int64_t>> row_ptr;
+    std::vector<std::vector<int32_t>> col;
+    std::vector<std::vector<double>> state[4];  // [kind][step][dof]
+    std::vector<Model> models;
+    double t = 0, dt = 1, du_dt = 1, ddu_dt = 1;
+    // stats
+    int64_t rm_calls = 0, rm_iters = 0, rm_maxit = 0;
+
+    int field_index(const char* nm) const {
+        for (int i = 0; i < nfields; i++)
+            if (fname[i] == nm) return i;
+        return -1;
+    }
+};
+
+static void elem_nodes(const Ctx& c, int64_t e, std::vector<int64_t>& nodes) {
+    for (int a = 0; a < c.n; a++) nodes[a] = c.elems[(size_t)e * c.n + a];
+}
+
+// Mesh::getShapeGrads (mesh/mesh.cpp:348-364): gather coords (+W) and dispatch
+static void getShapeGrads(const Ctx& c, int64_t e, Shapes& S) {
+    const int n = c.n;
+    std::vector<double> Xe((size_t)n * c.dim), We(n);
+    for (int a = 0; a < n; a++) {
+        int64_t v = c.elems[(size_t)e * n + a];
+        for (int d = 0; d < c.dim; d++) Xe[(size_t)a * c.dim + d] = c.X[(size_t)v * c.dim + d];
+        We[a] = c.W[(size_t)v];
+    }
+    const double* Ce = &c.C[(size_t)e * n * n];
+    if (c.dim == 2) shapeGrads_B2_2(c.et, Xe.data(), Ce, S);
+    else shapeGrads_NURBS3_Cube2(c.et, Xe.data(), Ce, We.data(), S);
+}
+
+// DofSpace::getDofForNodes (DofSpace.cpp:987-1008): out[i + j*n] = dof(node_i, field_j)
+static void getDofForNodes(const Ctx& c, const std::vector<int64_t>& nodes, const std::vector<int>& fields,
+                           std::vector<int64_t>& out) {
+    const size_t nn = nodes.size();
+    for (size_t j = 0; j < fields.size(); j++)
+        for (size_t i = 0; i < nn; i++) {
+            int64_t d = c.dofOf[fields[j]][nodes[i]];
+            if (d < 0) throw std::runtime_error("node has no dof of this type");
+            out[i + j * nn] = d;
+        }
+}
+
+// Element block sink: either records the pattern or adds values (MatSetValues ADD_VALUES,
+// column-oriented input, physics.cpp:243 + utility/math.cpp:301-331).
+struct Sink {
+    Ctx* c;
+    int step;
+    bool pattern_mode;
+    std::vector<std::vector<int32_t>>* rows_building;  // pattern mode
+    double* K;                                          // value mode
+    double* f;
+    bool atomic_add = false;                            // threaded CPU baseline
+    void mat(const std::vector<int64_t>& rows, const std::vector<int64_t>& cols, const Mat& Ke) {
+        if (pattern_mode) {
+            for (size_t i = 0; i < rows.size(); i++) {
+                auto& r = (*rows_building)[rows[i]];
+                for (size_t j = 0; j < cols.size(); j++) {
+                    int32_t cc = (int32_t)cols[j];
+                    auto it = std::lower_bound(r.begin(), r.end(), cc);
+                    if (it == r.end() || *it != cc) r.insert(it, cc);
+                }
+            }
+            return;
+        }
+        const auto& rp = c->row_ptr[step];
+        const auto& cl = c->col[step];
+        for (size_t j = 0; j < cols.size(); j++)
+            for (size_t i = 0; i < rows.size(); i++) {
+                int64_t r = rows[i];
+                auto b = cl.begin() + rp[r], e = cl.begin() + rp[r + 1];
+                auto it = std::lower_bound(b, e, (int32_t)cols[j]);
+                if (it == e || *it != (int32_t)cols[j]) throw std::runtime_error("entry outside pattern");
+                if (atomic_add) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    K[it - cl.begin()] += Ke(i, j);
+                } else {
+                    K[it - cl.begin()] += Ke(i, j);
+                }
+            }
+    }
+    void vec(const std::vector<int64_t>& rows, const Vec& Fe) {
+        if (pattern_mode) return;
+        for (size_t i = 0; i < rows.size(); i++) {
+            if (atomic_add) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                f[rows[i]] += Fe[i];
+            } else {
+                f[rows[i]] += Fe[i];
+            }
+        }
+    }
+};
+
+struct Stats {
+    int64_t calls = 0, iters = 0, maxit = 0;
+    double lcrack = 0;
+};
+
+// ---- PhaseFieldDamage::Assemble_U (PhaseField.cpp:311-483) ------------------------
+static void Assemble_U(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats& st) {
+    const int n = c.n, nip = c.nip, dim = c.dim, nu = dim * n;
+    Material M = md.M;  // SetTemperature mutates the material: per-call copy
+    Shapes Su, Sph, ST;
+    Su.resize(nip, n, dim); Sph.resize(nip, n, dim); ST.resize(nip, n, dim);
+    Mat B(6, nu);
+    std::vector<int64_t> El(n), dofsU(nu), dofsPhase(n), dofsT(n);
+    std::vector<int> fph = {md.field_phase}, fT = {md.field_T};
+    Vec U(nu), dU(nu), ddU(nu), UOld(nu), Phase(n), PhaseOld(n), T(n, 0.0);
+    Vec F_U(nu), WLumped(n);
+    Mat K_UU(nu, nu);
+    const int su = md.Step_u, sph = md.Step_phase;
+    double ddXdt2 = c.ddu_dt, dXdt = c.du_dt, dt = c.dt;
+    const size_t NIPT = (size_t)c.NE * nip;
+    for (int64_t el = e0; el < e1; el++) {
+        getShapeGrads(c, el, Su);
+        getShapeGrads(c, el, Sph);
+        elem_nodes(c, el, El);
+        getDofForNodes(c, El, md.fields_u, dofsU);
+        getDofForNodes(c, El, fph, dofsPhase);
+        for (int i = 0; i < nu; i++) {
+            U[i] = c.state[IFE_STATE][su][dofsU[i]];
+            UOld[i] = c.state[IFE_STATE_OLD][su][dofsU[i]];
+            dU[i] = c.state[IFE_DSTATE][su][dofsU[i]];
+            ddU[i] = c.state[IFE_DDSTATE][su][dofsU[i]];
+        }
+        for (int a = 0; a < n; a++) {
+            Phase[a] = c.state[IFE_STATE][sph][dofsPhase[a]];
+            PhaseOld[a] = c.state[IFE_STATE_OLD][sph][dofsPhase[a]];
+        }
+        if (md.HasT) {
+            getShapeGrads(c, el, ST);
+            getDofForNodes(c, El, fT, dofsT);
+            for (int a = 0; a < n; a++) T[a] = c.state[IFE_STATE][md.Step_T][dofsT[a]];
+        }
+        F_U.assign(nu, 0.0);
+        K_UU.zero();
+        WLumped.assign(n, 0.0);
+        for (int ip = 0; ip < nip; ip++) {
+            double Tip;
+            if (md.HasT) { Tip = 0; for (int a = 0; a < n; a++) Tip += ST.Nv(ip, a) * T[a]; }
+            else Tip = md.opt.T_default;
+            M.SetTemperature(Tip, md.opt.porosity);
+            double phi = 0, phi_Old = 0;
+            for (int a = 0; a < n; a++) { phi += Sph.Nv(ip, a) * Phase[a]; phi_Old += Sph.Nv(ip, a) * PhaseOld[a]; }
+            double Dam, dDam, nn1;
+            Material::Damage(md.pf.DamFunction, phi, md.pf.k0, md.pf.p, Dam, dDam, nn1);
+            ConstructBMat(B, Su, ip, dim);
+            V6 Strain, StrainOld, Strain_Pl, Strain_Pl_Old;
+            for (int r = 0; r < 6; r++) {
+                double s = 0, so = 0;
+                for (int j = 0; j < nu; j++) { s += B(r, j) * U[j]; so += B(r, j) * UOld[j]; }
+                Strain[r] = s; StrainOld[r] = so;
+            }
+            size_t gi = (size_t)el * nip + ip;
+            for (int r = 0; r < 6; r++) { Strain_Pl_Old[r] = md.PSOld[r * NIPT + gi]; Strain_Pl[r] = md.PS[r * NIPT + gi]; }
+            double hist[4] = {0, 0, 0, 0};
+            for (int k = 0; k < M.HistSize; k++) hist[k] = md.MH[k * NIPT + gi];
+            M6 DPlast;
+            double PlasticDiss, DamPlasticDiss;
+            int iters; bool maxit, called;
+            M.UpdatePlasticStrains(DPlast, Strain_Pl, Strain, Strain_Pl_Old, StrainOld, phi, phi_Old, dt, hist, PlasticDiss,
+                                   DamPlasticDiss, iters, maxit, called);
+            if (called) { st.calls++; st.iters += iters; st.maxit += maxit ? 1 : 0; }
+            for (int k = 0; k < M.HistSize; k++) md.MH[k * NIPT + gi] = hist[k];
+            for (int r = 0; r < 6; r++) md.PS[r * NIPT + gi] = Strain_Pl[r];
+            md.VD[gi] = md.VDOld[gi] + PlasticDiss * dt;
+            md.DVD[gi] = md.DVDOld[gi] + DamPlasticDiss * dt;
+            for (int r = 0; r < 6; r++) Strain[r] = Strain[r] - Strain_Pl[r];
+            M6 D_Dam, D_UnDam;
+            V6 S_Dam, S_Undam;
+            M.EnergySplit(D_Dam, D_UnDam, S_Dam, S_Undam, Strain);
+            double w = Su.w[ip];
+            // F_U += w*B^T*(Dam*S_Dam + S_Undam)
+            V6 sv;
+            for (int r = 0; r < 6; r++) sv[r] = Dam * S_Dam[r] + S_Undam[r];
+            for (int j = 0; j < nu; j++) {
+                double t = 0;
+                for (int r = 0; r < 6; r++) t += (w * B(r, j)) * sv[r];
+                F_U[j] += t;
+            }
+            // K_UU += ((w*B^T) * (Dam*D_Dam + D_UnDam)) * DPlast * B   (Eigen left-to-right)
+            M6 Sm;
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) Sm[i][j] = Dam * D_Dam[i][j] + D_UnDam[i][j];
+            Mat X1(nu, 6), X2(nu, 6);
+            for (int a = 0; a < nu; a++)
+                for (int j = 0; j < 6; j++) {
+                    double t = 0;
+                    for (int r = 0; r < 6; r++) t += (w * B(r, a)) * Sm[r][j];
+                    X1(a, j) = t;
+                }
+            for (int a = 0; a < nu; a++)
+                for (int j = 0; j < 6; j++) {
+                    double t = 0;
+                    for (int r = 0; r < 6; r++) t += X1(a, r) * DPlast[r][j];
+                    X2(a, j) = t;
+                }
+            for (int b = 0; b < nu; b++)
+                for (int a = 0; a < nu; a++) {
+                    double t = 0;
+                    for (int r = 0; r < 6; r++) t += X2(a, r) * B(r, b);
+                    K_UU(a, b) += t;
+                }
+            for (int a = 0; a < n; a++) WLumped[a] += w * Su.Nv(ip, a);
+        }
+        for (int a = 0; a < n; a++) {
+            int ny = a + n, nz = a + 2 * n;
+            if (md.opt.Inertia) {
+                double DamI, d1, d2;
+                Material::Damage(md.pf.DensityDamage, Phase[a], md.pf.k0, md.pf.p, DamI, d1, d2);
+                F_U[a] += WLumped[a] * DamI * M.Density * ddU[a];
+                K_UU(a, a) += WLumped[a] * DamI * M.Density * ddXdt2;
+                F_U[ny] += WLumped[a] * DamI * M.Density * ddU[ny];
+                K_UU(ny, ny) += WLumped[a] * DamI * M.Density * ddXdt2;
+                if (dim == 3) {
+                    F_U[nz] += WLumped[a] * DamI * M.Density * ddU[nz];
+                    K_UU(nz, nz) += WLumped[a] * DamI * M.Density * ddXdt2;
+                }
+            }
+            if (M.Damping > 0.0) {
+                F_U[a] += WLumped[a] * M.Damping * dU[a];
+                K_UU(a, a) += WLumped[a] * M.Damping * dXdt;
+                F_U[ny] += WLumped[a] * M.Damping * dU[ny];
+                K_UU(ny, ny) += WLumped[a] * M.Damping * dXdt;
+                if (dim == 3) {
+                    F_U[nz] += WLumped[a] * M.Damping * dU[nz];
+                    K_UU(nz, nz) += WLumped[a] * M.Damping * dXdt;
+                }
+            }
+            double DamG, d1, d2;
+            Material::Damage(md.pf.GravityDamage, Phase[a], md.pf.k0, md.pf.p, DamG, d1, d2);
+            F_U[a] += -WLumped[a] * DamG * M.Density * md.g[0];
+            F_U[ny] += -WLumped[a] * DamG * M.Density * md.g[1];
+            if (dim == 3) F_U[nz] += -WLumped[a] * DamG * M.Density * md.g[2];
+        }
+        sink.vec(dofsU, F_U);
+        sink.mat(dofsU, dofsU, K_UU);
+    }
+}
+
+// ---- PhaseFieldDamage::Assemble_Phase (PhaseField.cpp:485-731), Hist/None irreversibility
+static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats& st) {
+    const int n = c.n, nip = c.nip, dim = c.dim, nu = dim * n;
+    Material M = md.M;
+    Shapes Su, Sph, ST;
+    Su.resize(nip, n, dim); Sph.resize(nip, n, dim); ST.resize(nip, n, dim);
+    Mat B(6, nu);
+    std::vector<int64_t> El(n), dofsU(nu), dofsPhase(n), dofsT(n, 0);  // dofsT never filled (:517)
+    std::vector<int> fph = {md.field_phase}, fT = {md.field_T};
+    Vec U(nu), dU(nu), Phase(n), dPhase(n), PhaseOld(n), T(n, 0.0);
+    Vec Dist(n), DistCap(n), F_Ph(n), WLumped(n), LumpedDriving(n);
+    Mat dDist(n, n), dDistCap(n, n), K_PhPh(n, n);
+    const int su = md.Step_u, sph = md.Step_phase;
+    double dt = c.dt;
+    const size_t NIPT = (size_t)c.NE * nip;
+    for (int64_t el = e0; el < e1; el++) {
+        getShapeGrads(c, el, Su);
+        getShapeGrads(c, el, Sph);
+        elem_nodes(c, el, El);
+        getDofForNodes(c, El, md.fields_u, dofsU);
+        getDofForNodes(c, El, fph, dofsPhase);
+        for (int i = 0; i < nu; i++) { U[i] = c.state[IFE_STATE][su][dofsU[i]]; dU[i] = c.state[IFE_DSTATE][su][dofsU[i]]; }
+        for (int a = 0; a < n; a++) {
+            Phase[a] = c.state[IFE_STATE][sph][dofsPhase[a]];
+            dPhase[a] = c.state[IFE_DSTATE][sph][dofsPhase[a]];
+            PhaseOld[a] = c.state[IFE_STATE_OLD][sph][dofsPhase[a]];
+        }
+        if (md.HasT) {
+            getShapeGrads(c, el, ST);
+            if (!md.opt.phase_T_from_dof0) getDofForNodes(c, El, fT, dofsT);
+            for (int a = 0; a < n; a++) T[a] = c.state[IFE_STATE][md.Step_T][dofsT[a]];
+        }
+        F_Ph.assign(n, 0.0);
+        K_PhPh.zero();
+        WLumped.assign(n, 0.0);
+        LumpedDriving.assign(n, 0.0);
+        for (int ip = 0; ip < nip; ip++) {
+            double Tip;
+            if (md.HasT) { Tip = 0; for (int a = 0; a < n; a++) Tip += ST.Nv(ip, a) * T[a]; }
+            else Tip = md.opt.T_default;
+            M.SetTemperature(Tip, md.opt.porosity);
+            double phi = 0;
+            for (int a = 0; a < n; a++) phi += Sph.Nv(ip, a) * Phase[a];
+            double Dam, dDam, ddDam, DamI, dDamI, ddDamI, DamG, dDamG, ddDamG;
+            Material::Damage(md.pf.DamFunction, phi, md.pf.k0, md.pf.p, Dam, dDam, ddDam);
+            Material::Damage(md.pf.DensityDamage, phi, md.pf.k0, md.pf.p, DamI, dDamI, ddDamI);
+            Material::Damage(md.pf.GravityDamage, phi, md.pf.k0, md.pf.p, DamG, dDamG, ddDamG);
+            ConstructBMat(B, Su, ip, dim);
+            size_t gi = (size_t)el * nip + ip;
+            V6 Strain;
+            for (int r = 0; r < 6; r++) {
+                double s = 0;
+                for (int j = 0; j < nu; j++) s += B(r, j) * U[j];
+                Strain[r] = s - md.PS[r * NIPT + gi];
+            }
+            M6 D_Dam, D_UnDam;
+            V6 S_Dam, S_Undam;
+            double Energy = M.EnergySplit(D_Dam, D_UnDam, S_Dam, S_Undam, Strain);
+            if (md.opt.Irreversible == IFE_IRR_HIST) {  // strict '>' (:631-640)
+                if (md.EnergyOld[gi] > Energy) { md.Energy[gi] = md.EnergyOld[gi]; Energy = md.EnergyOld[gi]; }
+                else md.Energy[gi] = Energy;
+            } else {
+                md.Energy[gi] = Energy;
+            }
+            double gamma = GetDistributionFunction(md.pf.DistributionFunction, M.pf_l, dim, Dist, dDist, Sph, ip, Phase.data());
+            if (md.opt.LumpedPFCapacity) {
+                double u1, u2;
+                GetCapacityOnly(md.pf.DistributionFunction, M.pf_l, DistCap, dDistCap, u1, u2, Sph, ip, phi);
+            } else {
+                DistCap.assign(n, 0.0);
+                dDistCap.zero();
+            }
+            double w = Su.w[ip];
+            st.lcrack += gamma * w;
+            for (int a = 0; a < n; a++) F_Ph[a] += w * M.Gc * (Dist[a] - DistCap[a]);
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) K_PhPh(a, b) += w * M.Gc * (dDist(a, b) - dDistCap(a, b));
+            if (md.opt.LumpedDriving == 0) {
+                for (int a = 0; a < n; a++) F_Ph[a] += w * dDam * Sph.Nv(ip, a) * Energy;
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++) K_PhPh(a, b) += w * ddDam * Sph.Nv(ip, a) * Sph.Nv(ip, b) * Energy;
+            } else {
+                for (int a = 0; a < n; a++) LumpedDriving[a] += w * Sph.Nv(ip, a) * Energy;
+            }
+            if (md.pf.ViscoDrivingFactor > 1.0e-9) {
+                double DamV, dDamV, ddDamV;
+                double PlasticDiss = md.pf.ViscoDrivingFactor * md.DVD[gi];
+                Material::Damage(md.pf.ViscoDriving, phi, md.pf.k0, md.pf.p, DamV, dDamV, ddDamV);
+                for (int a = 0; a < n; a++) F_Ph[a] += w * dDamV * Sph.Nv(ip, a) * PlasticDiss;
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++) K_PhPh(a, b) += w * ddDamV * Sph.Nv(ip, a) * Sph.Nv(ip, b) * PlasticDiss;
+            }
+            if (md.opt.Inertia && md.pf.ConsistentDegradation) {
+                double v2 = 0;
+                for (int d = 0; d < dim; d++) {
+                    double vd = 0;
+                    for (int a = 0; a < n; a++) vd += Su.Nv(ip, a) * dU[d * n + a];
+                    v2 += vd * vd;
+                }
+                for (int a = 0; a < n; a++) F_Ph[a] += -w * 0.5 * M.Density * v2 * Sph.Nv(ip, a) * dDamI;
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++)
+                        K_PhPh(a, b) += -w * 0.5 * M.Density * v2 * Sph.Nv(ip, a) * Sph.Nv(ip, b) * ddDamI;
+            }
+            if (md.pf.ConsistentDegradation) {
+                double gu = 0;
+                for (int d = 0; d < dim; d++) {
+                    double ud = 0;
+                    for (int a = 0; a < n; a++) ud += Su.Nv(ip, a) * U[d * n + a];
+                    gu += md.g[d] * ud;
+                }
+                for (int a = 0; a < n; a++) F_Ph[a] += -w * M.Density * gu * Sph.Nv(ip, a) * dDamG;
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++) K_PhPh(a, b) += -w * M.Density * gu * Sph.Nv(ip, a) * Sph.Nv(ip, b) * ddDamG;
+            }
+            for (int a = 0; a < n; a++) WLumped[a] += w * Sph.Nv(ip, a);
+        }
+        for (int a = 0; a < n; a++) {
+            if (md.opt.LumpedDriving) {
+                double D0, dD0, ddD0;
+                Material::Damage(md.pf.DamFunction, Phase[a], md.pf.k0, md.pf.p, D0, dD0, ddD0);
+                F_Ph[a] += LumpedDriving[a] * dD0;
+                K_PhPh(a, a) += LumpedDriving[a] * ddD0;
+            }
+            if (md.opt.LumpedPFCapacity) {
+                double LD, dLD;
+                GetCapacityOnly(md.pf.DistributionFunction, M.pf_l, DistCap, dDistCap, LD, dLD, Sph, 0, Phase[a]);
+                F_Ph[a] += WLumped[a] * M.Gc * LD;
+                K_PhPh(a, a) += WLumped[a] * M.Gc * dLD;
+            }
+            F_Ph[a] += WLumped[a] * M.pf_visc / dt * (Phase[a] - PhaseOld[a]);
+            K_PhPh(a, a) += WLumped[a] * (M.pf_visc / dt + md.opt.ViscousStab * M.Gc / M.pf_l);
+        }
+        // EnforceIrreversible with Hist/None: no nodal terms (PhaseFieldUtil.cpp:350-370)
+        sink.vec(dofsPhase, F_Ph);
+        sink.mat(dofsPhase, dofsPhase, K_PhPh);
+    }
+}
+
+// ---- PhaseFieldDamage::Assemble_T (PhaseField.cpp:733-768) ------------------------
+static void Assemble_T_PF(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats&) {
+    const int n = c.n, nip = c.nip, dim = c.dim;
+    Shapes S;
+    S.resize(nip, n, dim);
+    std::vector<int64_t> El(n), dofsT(n);
+    std::vector<int> fT = {md.field_T};
+    Vec F_T(n);
+    const size_t NIPT = (size_t)c.NE * nip;
+    (void)NIPT;
+    for (int64_t el = e0; el < e1; el++) {
+        getShapeGrads(c, el, S);
+        elem_nodes(c, el, El);
+        getDofForNodes(c, El, fT, dofsT);
+        F_T.assign(n, 0.0);
+        for (int ip = 0; ip < nip; ip++) {
+            size_t gi = (size_t)el * nip + ip;
+            double HeatingRate = (md.VD[gi] - md.VDOld[gi]) / c.dt;
+            for (int a = 0; a < n; a++) F_T[a] += -S.w[ip] * S.Nv(ip, a) * HeatingRate;
+        }
+        sink.vec(dofsT, F_T);
+    }
+}
+
+// ---- ThermalModel::Assemble (Models/Thermal/ThermalModel.cpp:72-136) ----------------
+static void Assemble_Thermal(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats&) {
+    const int n = c.n, nip = c.nip, dim = c.dim;
+    const Material& M = md.M;
+    Shapes S;
+    S.resize(nip, n, dim);
+    std::vector<int64_t> El(n), dofsT(n);
+    std::vector<int> fT = {md.field_T};
+    Vec T(n), dT(n), F_T(n), WLumped(n);
+    Mat K_TT(n, n);
+    double dXdt = c.du_dt;
+    for (int64_t el = e0; el < e1; el++) {
+        getShapeGrads(c, el, S);
+        elem_nodes(c, el, El);
+        getDofForNodes(c, El, fT, dofsT);
+        for (int a = 0; a < n; a++) {
+            T[a] = c.state[IFE_STATE][md.Step_T][dofsT[a]];
+            dT[a] = c.state[IFE_DSTATE][md.Step_T][dofsT[a]];
+        }
+        F_T.assign(n, 0.0);
+        K_TT.zero();
+        WLumped.assign(n, 0.0);
+        for (int ip = 0; ip < nip; ip++) {
+            double w = S.w[ip];
+            // F_T += w*k*G^T*G*T  (Eigen: ((w*k)*G^T*G)*T)
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) {
+                    double gg = 0;
+                    for (int cc = 0; cc < dim; cc++) gg += S.Gv(ip, cc, a) * S.Gv(ip, cc, b);
+                    K_TT(a, b) += w * M.k * gg;
+                }
+            double GT[3] = {0, 0, 0};
+            for (int cc = 0; cc < dim; cc++) {
+                double t = 0;
+                for (int a = 0; a < n; a++) t += S.Gv(ip, cc, a) * T[a];
+                GT[cc] = t;
+            }
+            for (int a = 0; a < n; a++) {
+                double t = 0;
+                for (int cc = 0; cc < dim; cc++) t += S.Gv(ip, cc, a) * GT[cc];
+                F_T[a] += w * M.k * t;
+            }
+            for (int a = 0; a < n; a++) WLumped[a] += w * S.Nv(ip, a);
+        }
+        for (int a = 0; a < n; a++) {
+            F_T[a] += WLumped[a] * M.Density * M.cp * dT[a];
+            K_TT(a, a) += WLumped[a] * M.Density * M.cp * dXdt;
+        }
+        sink.vec(dofsT, F_T);
+        sink.mat(dofsT, dofsT, K_TT);
+    }
+}
+
+// BaseModel::Assemble dispatch for one model (PhaseField.cpp:291-309, ThermalModel.cpp:72)
+static void model_assemble(Ctx& c, Model& md, Sink& sink, int step, int64_t e0, int64_t e1, Stats& st) {
+    if (md.type == MODEL_PHASEFIELD) {
+        if (step == md.Step_u) Assemble_U(c, md, sink, e0, e1, st);
+        if (step == md.Step_phase) Assemble_Phase(c, md, sink, e0, e1, st);
+        if (step == md.Step_T && md.opt.ViscousHeating) Assemble_T_PF(c, md, sink, e0, e1, st);
+    } else {
+        if (step == md.Step_T) Assemble_Thermal(c, md, sink, e0, e1, st);
+    }
+}
+
+}  // namespace ifo
+
+// ==================================================================================
+// C interface for the test harness
+// ==================================================================================
+using namespace ifo;
+
+extern "C" {
+
+typedef struct {
+    int32_t type;  // 0 = Fracture/PhaseField, 1 = Thermal/ThermalModel
+    ife_material mat;
+    ife_phasefield_opts pf;
+    ife_model_opts opts;
+} ifo_model_desc;
+
+static thread_local std::string g_err;
+const char* ifo_last_error(void) { return g_err.c_str(); }
+
+void ifo_bezier_basis(int p, double x, double* N, double* G, double* G2) { BezierBasis(p, x, N, G, G2); }
+
+int ifo_ip_scheme_1d(int n, double* w, double* x) {
+    try {
+        std::vector<double> ww, xx;
+        getIPScheme1D(ww, xx, n);
+        for (int i = 0; i < n; i++) { w[i] = ww[i]; x[i] = xx[i]; }
+        return 0;
+    } catch (std::exception& ex) { g_err = ex.what(); return 1; }
+}
+
+// One element's getShapeGrads: X (n x dim), C (n x n row-major C[j][k]), W (n, 3D)
+int ifo_shape_grads(int dim, int ip1d, const double* X, const double* C, const double* W, double* N, double* G, double* w) {
+    try {
+        ElemType et;
+        et.init(dim, ip1d);
+        Shapes S;
+        S.resize(et.nip, et.n, dim);
+        if (dim == 2) shapeGrads_B2_2(et, X, C, S);
+        else shapeGrads_NURBS3_Cube2(et, X, C, W, S);
+        std::memcpy(N, S.N.data(), S.N.size() * 8);
+        std::memcpy(G, S.G.data(), S.G.size() * 8);
+        std::memcpy(w, S.w.data(), S.w.size() * 8);
+        return 0;
+    } catch (std::exception& ex) { g_err = ex.what(); return 1; }
+}
+
+// Cox-de Boor B-spline basis (all n_basis functions) at u; used to pin the Bezier
+// extraction operators independently of the extraction algorithm.
+void ifo_bspline_basis(const double* knots, int n_knots, int p, double u, double* out) {
+    int nb = n_knots - p - 1;
+    std::vector<double> Nk(n_knots - 1, 0.0);
+    double umax = knots[n_knots - 1];
+    for (int i = 0; i < n_knots - 1; i++) {
+        bool in = (u >= knots[i] && u < knots[i + 1]);
+        if (u == umax && knots[i] < knots[i + 1] && knots[i + 1] == umax) in = true;
+        Nk[i] = in ? 1.0 : 0.0;
+    }
+    for (int k = 1; k <= p; k++) {
+        for (int i = 0; i < n_knots - 1 - k; i++) {
+            double a = 0, b = 0;
+            double d1 = knots[i + k] - knots[i], d2 = knots[i + k + 1] - knots[i + 1];
+            if (d1 > 0) a = (u - knots[i]) / d1 * Nk[i];
+            if (d2 > 0) b = (knots[i + k + 1] - u) / d2 * Nk[i + 1];
+            Nk[i] = a + b;
+        }
+    }
+    for (int i = 0; i < nb; i++) out[i] = Nk[i];
+}
+
+// Single material-point kernels for unit tests
+void ifo_damage(int func, double phi, double k0, double p, double* out3) {
+    Material::Damage(func, phi, k0, p, out3[0], out3[1], out3[2]);
+}
+
+// return map at one point: strain(6), strain_pl (in: guess, out), strain_pl_old(6),
+// hist(1, in/out); out DMat (36 row-major), returns dissipation rate; iters out
+double ifo_return_map_viscous(const ife_material* mat, double T, double poros, double damage, double dt, const double* strain,
+                              double* strain_pl, const double* strain_pl_old, double* hist, double* DMat, int* iters) {
+    Material M;
+    M.init(*mat, nullptr);
+    M.SetTemperature(T, poros);
+    M6 Dm;
+    int it; bool mx;
+    double d = M.return_mapping_Viscous(Dm, strain_pl, strain, strain_pl_old, damage, dt, hist, it, mx);
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) DMat[i * 6 + j] = Dm[i][j];
+    *iters = it;
+    return d;
+}
+
+double ifo_energy_split(const ife_material* mat, int consistent, double T, double poros, const double* strain, double* D_Dam,
+                        double* D_UnDam, double* S_Dam, double* S_Undam) {
+    Material M;
+    ife_phasefield_opts pf{};
+    pf.ConsistentEnergy = consistent;
+    M.init(*mat, &pf);
+    M.SetTemperature(T, poros);
+    M6 a, b;
+    V6 c, d;
+    double E = M.EnergySplit(a, b, c, d, strain);
+    for (int i = 0; i < 6; i++) {
+        S_Dam[i] = c[i]; S_Undam[i] = d[i];
+        for (int j = 0; j < 6; j++) { D_Dam[i * 6 + j] = a[i][j]; D_UnDam[i * 6 + j] = b[i][j]; }
+    }
+    return E;
+}
+
+void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_models, const ifo_model_desc* models) {
+    try {
+        Ctx* c = new Ctx();
+        c->dim = mesh->dim; c->ip1d = mesh->ipcount1D;
+        c->n = mesh->nodes_per_elem;
+        c->NN = mesh->n_nodes; c->NE = mesh->n_elems;
+        if (!((c->dim == 2 && c->n == 9) || (c->dim == 3 && c->n == 27))) throw std::invalid_argument("element type");
+        c->et.init(c->dim, c->ip1d);
+        c->nip = c->et.nip;
+        c->X.assign(mesh->coords, mesh->coords + c->NN * c->dim);
+        c->W.assign(c->NN, 1.0);
+        if (mesh->weights) c->W.assign(mesh->weights, mesh->weights + c->NN);
+        c->elems.assign(mesh->elems, mesh->elems + c->NE * c->n);
+        c->C.assign(mesh->extraction, mesh->extraction + (size_t)c->NE * c->n * c->n);
+        c->nfields = dofs->n_fields;
+        c->nsteps = 0;
+        for (int i = 0; i < dofs->n_fields; i++) {
+            c->fname.push_back(dofs->names[i]);
+            c->fstep.push_back(dofs->steps[i]);
+            c->nsteps = std::max(c->nsteps, dofs->steps[i] + 1);
+        }
+        // models: Setup + init (PhaseField.cpp:33-101,167-283; ThermalModel.cpp:31-66)
+        std::vector<std::vector<char>> hasdof(c->nfields, std::vector<char>(c->NN, 0));
+        std::vector<char> innode(c->NN, 0);
+        for (int64_t i = 0; i < c->NE * c->n; i++) innode[c->elems[i]] = 1;  // GetUniqueNodes
+        for (int m = 0; m < n_models; m++) {
+            Model md;
+            md.type = models[m].type;
+            md.pf = models[m].pf;
+            md.opt = models[m].opts;
+            md.M.init(models[m].mat, md.type == MODEL_PHASEFIELD ? &md.pf : nullptr);
+            auto need = [&](const char* nm) {
+                int f = c->field_index(nm);
+                if (f < 0) throw std::invalid_argument(std::string("missing dof ") + nm);
+                for (int64_t v = 0; v < c->NN; v++)
+                    if (innode[v]) hasdof[f][v] = 1;
+                return f;
+            };
+            if (md.type == MODEL_PHASEFIELD) {
+                md.fields_u.push_back(need("ux"));
+                md.fields_u.push_back(need("uy"));
+                if (c->dim == 3) md.fields_u.push_back(need("uz"));
+                for (int f : md.fields_u)
+                    if (c->fstep[f] != c->fstep[md.fields_u[0]]) throw std::invalid_argument("u fields in different steps");
+                md.Step_u = c->fstep[md.fields_u[0]];
+                md.field_phase = need("phase");
+                md.Step_phase = c->fstep[md.field_phase];
+                md.HasT = c->field_index("T") >= 0;
+                md.Step_T = 99;
+                if (md.HasT) { md.field_T = need("T"); md.Step_T = c->fstep[md.field_T]; }
+                if (md.opt.Gravity) {
+                    if (c->dim == 3) { md.g[0] = 0; md.g[1] = 0; md.g[2] = -9.81; }
+                    else { md.g[0] = 0; md.g[1] = -9.81; md.g[2] = 0; }
+                }
+                if (!(md.opt.Irreversible == IFE_IRR_HIST || md.opt.Irreversible == IFE_IRR_NONE))
+                    throw std::invalid_argument("irreversibility method not supported by the oracle");
+                size_t NIPT = (size_t)c->NE * c->nip;
+                md.Energy.assign(NIPT, 0); md.EnergyOld.assign(NIPT, 0);
+                md.PS.assign(6 * NIPT, 0); md.PSOld.assign(6 * NIPT, 0);
+                md.VD.assign(NIPT, 0); md.VDOld.assign(NIPT, 0);
+                md.DVD.assign(NIPT, 0); md.DVDOld.assign(NIPT, 0);
+                md.MH.assign((size_t)std::max(md.M.HistSize, 1) * NIPT, 0);
+                md.MHOld.assign((size_t)std::max(md.M.HistSize, 1) * NIPT, 0);
+            } else {
+                md.field_T = need("T");
+                md.Step_T = c->fstep[md.field_T];
+            }
+            c->models.push_back(std::move(md));
+        }
+        // numbering (DofSpace.cpp:486-510): per step, per field in DofNames order, nodes ascending
+        c->dofOf.assign(c->nfields, std::vector<int64_t>(c->NN, -1));
+        c->nrows.assign(c->nsteps, 0);
+        for (int s = 0; s < c->nsteps; s++) {
+            int64_t counter = 0;
+            for (int f = 0; f < c->nfields; f++) {
+                if (c->fstep[f] != s) continue;
+                for (int64_t v = 0; v < c->NN; v++)
+                    if (hasdof[f][v]) c->dofOf[f][v] = counter++;
+            }
+            c->nrows[s] = counter;
+        }
+        for (int k = 0; k < 4; k++) {
+            c->state[k].resize(c->nsteps);
+            for (int s = 0; s < c->nsteps; s++) c->state[k][s].assign(c->nrows[s], 0.0);
+        }
+        // pattern: union of every element block the first assembly inserts (physics.cpp:70-78),
+        // built as COO -> per-row counting sort -> sort/unique per row
+        c->row_ptr.resize(c->nsteps);
+        c->col.resize(c->nsteps);
+        for (int s = 0; s < c->nsteps; s++) {
+            // blocks: (rows, cols) dof lists per element and model
+            auto for_blocks = [&](const std::function<void(const std::vector<int64_t>&)>& fn) {
+                std::vector<int64_t> El(c->n), d(c->n * c->dim), dph(c->n), dT(c->n);
+                for (auto& md : c->models) {
+                    for (int64_t el = 0; el < c->NE; el++) {
+                        elem_nodes(*c, el, El);
+                        if (md.type == MODEL_PHASEFIELD) {
+                            if (s == md.Step_u) { getDofForNodes(*c, El, md.fields_u, d); fn(d); }
+                            if (s == md.Step_phase) { std::vector<int> f1 = {md.field_phase}; getDofForNodes(*c, El, f1, dph); fn(dph); }
+                        } else if (s == md.Step_T) {
+                            std::vector<int> f1 = {md.field_T};
+                            getDofForNodes(*c, El, f1, dT);
+                            fn(dT);
+                        }
+                    }
+                }
+            };
+            const int64_t nr = c->nrows[s];
+            std::vector<int64_t> cnt(nr + 1, 0);
+            for_blocks([&](const std::vector<int64_t>& dofs) {
+                for (int64_t r : dofs) cnt[r + 1] += (int64_t)dofs.size();
+            });
+            for (int64_t r = 0; r < nr; r++) cnt[r + 1] += cnt[r];
+            std::vector<int32_t> coo(cnt[nr]);
+            std::vector<int64_t> fillp(cnt.begin(), cnt.end() - 1);
+            for_blocks([&](const std::vector<int64_t>& dofs) {
+                for (int64_t r : dofs)
+                    for (int64_t cc : dofs) coo[fillp[r]++] = (int32_t)cc;
+            });
+            auto& rp = c->row_ptr[s];
+            auto& cl = c->col[s];
+            rp.assign(nr + 1, 0);
+            std::vector<int64_t> len(nr, 0);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1024)
+#endif
+            for (int64_t r = 0; r < nr; r++) {
+                auto b = coo.begin() + cnt[r], e = coo.begin() + cnt[r + 1];
+                std::sort(b, e);
+                len[r] = std::unique(b, e) - b;
+            }
+            for (int64_t r = 0; r < nr; r++) rp[r + 1] = rp[r] + len[r];
+            cl.resize(rp[nr]);
+            for (int64_t r = 0; r < nr; r++) std::copy(coo.begin() + cnt[r], coo.begin() + cnt[r] + len[r], cl.begin() + rp[r]);
+        }
+        return c;
+    } catch (std::exception& ex) {
+        g_err = ex.what();
+        return nullptr;
+    }
+}
+
+void ifo_destroy(void* h) { delete (Ctx*)h; }
+
+int ifo_num_steps(void* h) { return ((Ctx*)h)->nsteps; }
+int64_t ifo_num_rows(void* h, int s) { return ((Ctx*)h)->nrows[s]; }
+int64_t ifo_nnz(void* h, int s) { return (int64_t)((Ctx*)h)->col[s].size(); }
+void ifo_pattern(void* h, int s, int64_t* row_ptr, int32_t* col) {
+    Ctx* c = (Ctx*)h;
+    std::copy(c->row_ptr[s].begin(), c->row_ptr[s].end(), row_ptr);
+    std::copy(c->col[s].begin(), c->col[s].end(), col);
+}
+void ifo_dof_numbers(void* h, int field, int64_t* out) {
+    Ctx* c = (Ctx*)h;
+    std::copy(c->dofOf[field].begin(), c->dofOf[field].end(), out);
+}
+double* ifo_state(void* h, int step, int kind) { return ((Ctx*)h)->state[kind][step].data(); }
+void ifo_set_time(void* h, double t, double dt, double du_dt, double ddu_dt) {
+    Ctx* c = (Ctx*)h;
+    c->t = t; c->dt = dt; c->du_dt = du_dt; c->ddu_dt = ddu_dt;
+}
+
+double* ifo_history(void* h, int model, const char* name, int64_t* ncomp) {
+    Ctx* c = (Ctx*)h;
+    Model& md = c->models[model];
+    std::string s(name);
+    size_t NIPT = (size_t)c->NE * c->nip;
+    std::vector<double>* v = nullptr;
+    if (s == "Energy") v = &md.Energy;
+    else if (s == "EnergyOld") v = &md.EnergyOld;
+    else if (s == "PlasticStrain") v = &md.PS;
+    else if (s == "PlasticStrainOld") v = &md.PSOld;
+    else if (s == "ViscDiss") v = &md.VD;
+    else if (s == "ViscDissOld") v = &md.VDOld;
+    else if (s == "DamViscDiss") v = &md.DVD;
+    else if (s == "DamViscDissOld") v = &md.DVDOld;
+    else if (s == "MatHist") v = &md.MH;
+    else if (s == "MatHistOld") v = &md.MHOld;
+    if (!v || v->empty()) { *ncomp = 0; return nullptr; }
+    *ncomp = (int64_t)(v->size() / NIPT);
+    return v->data();
+}
+
+// Physics::Assemble(step) (physics.cpp:255-303) in element order, single thread.
+int ifo_assemble(void* h, int step, double* K_vals, double* f) {
+    Ctx* c = (Ctx*)h;
+    try {
+        std::fill(K_vals, K_vals + c->col[step].size(), 0.0);
+        std::fill(f, f + c->nrows[step], 0.0);
+        Sink sink{c, step, false, nullptr, K_vals, f};
+        for (auto& md : c->models) {
+            Stats st;
+            model_assemble(*c, md, sink, step, 0, c->NE, st);
+            c->rm_calls += st.calls; c->rm_iters += st.iters; c->rm_maxit += st.maxit;
+            if (md.type == MODEL_PHASEFIELD && step == md.Step_phase) md.LCrack = st.lcrack;
+        }
+        return 0;
+    } catch (std::exception& ex) { g_err = ex.what(); return 1; }
+}
+
+void ifo_stats(void* h, int64_t* out3, double* lcrack) {
+    Ctx* c = (Ctx*)h;
+    out3[0] = c->rm_calls; out3[1] = c->rm_iters; out3[2] = c->rm_maxit;
+    *lcrack = 0;
+    for (auto& md : c->models)
+        if (md.type == MODEL_PHASEFIELD) *lcrack = md.LCrack;
+}
+void ifo_reset_stats(void* h) {
+    Ctx* c = (Ctx*)h;
+    c->rm_calls = c->rm_iters = c->rm_maxit = 0;
+}
+
+// CPU baseline: the steps in `steps` over elements [e0,e1), nthreads OpenMP threads, each
+// thread playing one reference MPI rank on a contiguous element range and adding into
+// the shared K/f with atomic adds.  K/f are zeroed first (as Physics::Assemble does).
+// Returns wall seconds of the element loops (zeroing excluded), or -1 on error.
+double ifo_time_assembly(void* h, int nsteps_run, const int* steps, int64_t e0, int64_t e1, int nthreads,
+                         double* const* K_vals, double* const* f) {
+    Ctx* c = (Ctx*)h;
+    try {
+        for (int si = 0; si < nsteps_run; si++) {
+            std::fill(K_vals[si], K_vals[si] + c->col[steps[si]].size(), 0.0);
+            std::fill(f[si], f[si] + c->nrows[steps[si]], 0.0);
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        for (int si = 0; si < nsteps_run; si++) {
+            int step = steps[si];
+            for (auto& md : c->models) {
+                std::string err;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+                {
+                    int tid = 0, nt = 1;
+#ifdef _OPENMP
+                    tid = omp_get_thread_num();
+                    nt = omp_get_num_threads();
+#endif
+                    int64_t cnt = e1 - e0;
+                    int64_t a = e0 + cnt * tid / nt, b = e0 + cnt * (tid + 1) / nt;
+                    Sink sink{c, step, false, nullptr, K_vals[si], f[si]};
+                    sink.atomic_add = nt > 1;
+                    Stats st;
+                    try {
+                        model_assemble(*c, md, sink, step, a, b, st);
+                    } catch (std::exception& ex) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+                        err = ex.what();
+                    }
+                }
+                if (!err.empty()) throw std::runtime_error(err);
+            }
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        return std::chrono::duration<double>(t1 - t0).count();
+    } catch (std::exception& ex) { g_err = ex.what(); return -1.0; }
+}
+
+int ifo_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+"""CPU tests of the C-ABI library: it loads, exports every declared entry point, rejects
+bad input with the right status codes, and its host logic (numbering, pattern,
+scatter colouring) matches the oracle bit for bit.  No kernel runs here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ice_fem_2025a_amd import abi
+from ice_fem_2025a_amd.config import make_problem
+from ice_fem_2025a_amd.engine import Physics
+from oracle.oracle import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "ife.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ife_[A-Za-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.load_library()
+    decl = declared_functions()
+    assert len(decl) >= 20
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert set(decl) == set(abi.EXPORTED)
+    assert lib.ife_api_version() == 1
+
+
+def test_unknown_model_is_rejected():
+    prob = make_problem("quad2d_le_at2", nel=(2, 2))
+    lib = abi.load_library()
+    ph = Physics.__new__(Physics)
+    keep = []
+    from ice_fem_2025a_amd.engine import _mesh_desc, _dofs_desc
+    md = _mesh_desc(prob.mesh, keep)
+    dd = _dofs_desc(prob.dof_names, prob.dof_steps, keep)
+    h = C.c_void_p()
+    assert lib.ife_physics_create(C.byref(md), C.byref(dd), C.byref(h)) == abi.IFE_OK
+    m = prob.models[0]
+    rc = lib.ife_physics_add_model(h, b"Fracture/NoSuchModel", b"x", 0, C.byref(m.material), C.byref(m.pf),
+                                   C.byref(m.opts), None)
+    assert rc == 6  # IFE_ERR_NOT_FOUND, like CreateModel's invalid_argument
+    assert b"has not been defined" in lib.ife_last_error()
+    # setup before models/plan ordering errors
+    assert lib.ife_physics_plan(h) == 5  # IFE_ERR_STATE: no models
+    bad = abi.Material.from_buffer_copy(m.material)
+    bad.Rheology = abi.RHEOLOGY["DamViscoElasticPlastic"]
+    assert lib.ife_physics_add_model(h, b"Fracture/PhaseField", b"x", 0, C.byref(bad), C.byref(m.pf),
+                                     C.byref(m.opts), None) == 2  # unsupported
+    lib.ife_physics_destroy(h)
+
+
+def test_null_and_bad_mesh_rejected():
+    lib = abi.load_library()
+    h = C.c_void_p()
+    assert lib.ife_physics_create(None, None, C.byref(h)) == 1  # IFE_ERR_INVALID
+    prob = make_problem("quad2d_le_at2", nel=(2, 2))
+    keep = []
+    from ice_fem_2025a_amd.engine import _mesh_desc, _dofs_desc
+    md = _mesh_desc(prob.mesh, keep)
+    md.ipcount1D = 2
+    dd = _dofs_desc(prob.dof_names, prob.dof_steps, keep)
+    assert lib.ife_physics_create(C.byref(md), C.byref(dd), C.byref(h)) == 2
+
+
+def test_dense_extraction_is_factored_and_non_kronecker_rejected():
+    prob = make_problem("hex3d_glen_at2", nel=(2, 2, 3))
+    lib = abi.load_library()
+    from oracle.oracle import mesh_desc
+    from ice_fem_2025a_amd.engine import _dofs_desc
+    md, keep = mesh_desc(prob.mesh, dense=True)
+    dd = _dofs_desc(prob.dof_names, prob.dof_steps, keep)
+    h = C.c_void_p()
+    assert lib.ife_physics_create(C.byref(md), C.byref(dd), C.byref(h)) == abi.IFE_OK
+    lib.ife_physics_destroy(h)
+    ext = np.ascontiguousarray(prob.mesh.dense_extraction())
+    ext[1, 3, 5] += 0.25
+    keep.append(ext)
+    md.extraction = ext.ctypes.data_as(C.POINTER(C.c_double))
+    assert lib.ife_physics_create(C.byref(md), C.byref(dd), C.byref(h)) == 2
+    assert b"Kronecker" in lib.ife_last_error()
+
+
+@pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (5, 3)), ("quad2d_le_at2", (1, 1)),
+                                      ("hex3d_glen_at2", (3, 2, 4)), ("hex3d_glen_at2", (1, 1, 1))])
+def test_numbering_and_pattern_match_oracle(name, nel):
+    prob = make_problem(name, nel=nel)
+    e = Physics(prob, device=False)
+    o = Oracle(prob)
+    for f in range(len(prob.dof_names)):
+        assert np.array_equal(e.dof_numbers(f), o.dof_numbers(f))
+    assert e.nsteps == o.nsteps
+    for s in range(o.nsteps):
+        r1, c1 = e.pattern(s)
+        r2, c2 = o.pattern(s)
+        assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+        # sorted, unique columns per row
+        for r in range(len(r1) - 1):
+            row = c1[r1[r]:r1[r + 1]]
+            assert np.all(np.diff(row) > 0)
+
+
+def test_numbering_is_field_blocked():
+    prob = make_problem("hex3d_glen_at2", nel=(2, 2, 2))
+    e = Physics(prob, device=False)
+    NN = prob.mesh.n_nodes
+    ux, uy, uz = (e.dof_numbers(i) for i in range(3))
+    assert np.array_equal(ux, np.arange(NN))
+    assert np.array_equal(uy, NN + np.arange(NN))
+    assert np.array_equal(uz, 2 * NN + np.arange(NN))
+    assert np.array_equal(e.dof_numbers(3), np.arange(NN))  # phase: own step
+    assert np.array_equal(e.dof_numbers(4), np.arange(NN))  # T: own step
+
+
+@pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (7, 5)), ("hex3d_glen_at2", (4, 3, 5))])
+def test_coloring_is_a_valid_partition(name, nel):
+    prob = make_problem(name, nel=nel)
+    e = Physics(prob, device=False)
+    ptr, el = e.coloring()
+    assert sorted(el.tolist()) == list(range(prob.mesh.n_elems))
+    for c in range(len(ptr) - 1):
+        members = el[ptr[c]:ptr[c + 1]]
+        nodes = prob.mesh.elems[members].ravel()
+        assert len(np.unique(nodes)) == len(nodes), f"colour {c} shares a node"
+        assert np.all(np.diff(members) > 0)
+    assert len(ptr) - 1 == 3 ** prob.mesh.dim
