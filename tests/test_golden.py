@@ -1,0 +1,74 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/gen_golden.py from the
+oracle): the inputs must regenerate bit for bit, the oracle must reproduce its frozen
+outputs (CPU), and the HIP engine must match them (GPU)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from ice_fem_2025a_amd import abi
+from ice_fem_2025a_amd.config import make_problem
+from parity_util import RTOL_FP64, csr_err, vec_err
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FILES = sorted(glob.glob(os.path.join(HERE, "*.npz")))
+HIST = ["PlasticStrain", "MatHist", "ViscDiss", "DamViscDiss", "Energy"]
+
+
+def _case(path):
+    base = os.path.basename(path)[:-4]
+    name, nel = base.rsplit("_", 1)
+    return name, tuple(int(v) for v in nel.split("x"))
+
+
+def _problem(path):
+    name, nel = _case(path)
+    prob = make_problem(name, nel=nel)
+    g = np.load(path)
+    assert np.array_equal(g["coords"], prob.mesh.coords)
+    assert np.array_equal(g["elems"], prob.mesh.elems)
+    assert np.array_equal(g["extraction_1d"], prob.mesh.extraction_1d)
+    for (kind, fname), v in prob.node_state.items():
+        assert np.array_equal(g[f"in_state_{kind}_{fname}"], v)
+    for (model, hname), v in prob.history.items():
+        assert np.array_equal(g[f"in_hist_{model}_{hname}"], v)
+    return prob, g
+
+
+def test_fixtures_exist():
+    assert len(FILES) >= 4
+
+
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_oracle_reproduces_golden(path):
+    from oracle.oracle import Oracle
+    prob, g = _problem(path)
+    o = Oracle(prob)
+    for s in range(o.nsteps):
+        rp, col = o.pattern(s)
+        assert np.array_equal(rp, g[f"row_ptr_{s}"]) and np.array_equal(col, g[f"col_{s}"])
+        K, f = o.assemble(s)
+        assert csr_err(K, g[f"K_{s}"], rp) <= 1e-13 and vec_err(f, g[f"f_{s}"]) <= 1e-13
+    assert o.stats()["return_map_iterations"] == int(g["return_map_iterations"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_engine_matches_golden(require_gpu, path):
+    from ice_fem_2025a_amd.engine import Physics
+    print(f"[golden] {os.path.basename(path)}", flush=True)
+    prob, g = _problem(path)
+    e = Physics(prob)
+    for s in range(e.nsteps):
+        rp, col = e.pattern(s)
+        assert np.array_equal(rp, g[f"row_ptr_{s}"]) and np.array_equal(col, g[f"col_{s}"])
+        e.assemble(s)
+        K, f = e.Kf(s)
+        assert csr_err(K, g[f"K_{s}"], rp) <= RTOL_FP64, f"step {s}"
+        assert vec_err(f, g[f"f_{s}"]) <= RTOL_FP64, f"step {s}"
+    if "out_hist_Energy" in g:
+        for h in HIST:
+            ref = g[f"out_hist_{h}"]
+            got = e.history(0, h)[:ref.shape[0]]
+            assert np.max(np.abs(got - ref)) <= 1e-12 * max(np.max(np.abs(ref)), 1e-300), h
