@@ -37,5 +37,22 @@ def build_engine(verbose=False):
     return OUT
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "ife_driver.cpp")
+DRIVER = os.path.join(OUT_DIR, "ife_driver")
+
+
+def build_driver():
+    """The C++ host driver (plain g++ over include/ife_model.hpp and the C-ABI)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
+           "-I/opt/rocm/include", DRIVER_SRC, "-o", DRIVER, "-L" + OUT_DIR, "-life", "-L/opt/rocm/lib",
+           "-lamdhip64", "-Wl,-rpath," + OUT_DIR, "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the C++ host driver")
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build_engine(verbose=True))
+    print(build_driver())

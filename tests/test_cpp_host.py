@@ -1,0 +1,86 @@
+"""The C++ host wrapper (include/ife_model.hpp) drives the engine like IceFEM's Physics
+drives its models; its K/f must match the oracle.  The driver binary is built by
+__graft_entry__.build() (g++ over the C-ABI)."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from ice_fem_2025a_amd import build as B
+from ice_fem_2025a_amd.config import make_problem
+from parity_util import RTOL_FP64, csr_err, vec_err
+
+
+def write_problem(prob, path, dof_numbers):
+    m = prob.mesh
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<iqq", m.dim, m.n_nodes, m.n_elems))
+        for a in (m.coords, m.weights):
+            fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        fh.write(np.ascontiguousarray(m.elems, dtype=np.int64).tobytes())
+        fh.write(np.ascontiguousarray(m.extraction_1d, dtype=np.float64).tobytes())
+        fh.write(struct.pack("<i", len(prob.dof_names)))
+        for n, s in zip(prob.dof_names, prob.dof_steps):
+            fh.write(struct.pack("<i", len(n)) + n.encode() + struct.pack("<i", s))
+        fh.write(struct.pack("<i", len(prob.models)))
+        from ice_fem_2025a_amd import abi
+        for md in prob.models:
+            for s in (md.type_name, md.name):
+                fh.write(struct.pack("<i", len(s)) + s.encode())
+            fh.write(bytes(md.material))
+            fh.write(bytes(md.pf if md.pf is not None else abi.PhaseFieldOpts()))
+            fh.write(bytes(md.opts if md.opts is not None else abi.ModelOpts()))
+        fh.write(struct.pack("<dddd", prob.t, prob.dt, prob.du_dt, prob.ddu_dt))
+        vecs = {}
+        for (kind, fname), vals in prob.node_state.items():
+            f = prob.dof_names.index(fname)
+            step = prob.dof_steps[f]
+            d = dof_numbers[f]
+            n_rows = max(dof_numbers[g].max() + 1 for g in range(len(prob.dof_names)) if prob.dof_steps[g] == step)
+            v = vecs.setdefault((step, kind), np.zeros(n_rows))
+            v[d[d >= 0]] = vals[d >= 0]
+        fh.write(struct.pack("<i", len(vecs)))
+        for (step, kind), v in vecs.items():
+            fh.write(struct.pack("<iiq", step, kind, v.size) + v.tobytes())
+        fh.write(struct.pack("<i", len(prob.history)))
+        for (model, name), v in prob.history.items():
+            v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+            fh.write(struct.pack("<i", model) + struct.pack("<i", len(name)) + name.encode() +
+                     struct.pack("<q", v.size) + v.tobytes())
+
+
+def read_output(path):
+    out = []
+    with open(path, "rb") as fh:
+        (ns,) = struct.unpack("<i", fh.read(4))
+        for _ in range(ns):
+            (nnz,) = struct.unpack("<q", fh.read(8))
+            K = np.frombuffer(fh.read(8 * nnz), dtype=np.float64)
+            (nr,) = struct.unpack("<q", fh.read(8))
+            f = np.frombuffer(fh.read(8 * nr), dtype=np.float64)
+            out.append((K, f))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (6, 4)), ("hex3d_glen_at2", (3, 4, 3))])
+def test_cpp_host_driver_matches_oracle(require_gpu, tmp_path, name, nel):
+    from oracle.oracle import Oracle
+    print(f"[cpp driver] {name} {nel}", flush=True)
+    assert os.path.exists(B.DRIVER), "C++ driver not built (run __graft_entry__.build())"
+    prob = make_problem(name, nel=nel)
+    o = Oracle(prob)
+    dof_numbers = [o.dof_numbers(f) for f in range(len(prob.dof_names))]
+    pin, pout = tmp_path / "problem.bin", tmp_path / "out.bin"
+    write_problem(prob, pin, dof_numbers)
+    r = subprocess.run([B.DRIVER, str(pin), str(pout)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    assert "unknown model rejected" in r.stdout
+    res = read_output(pout)
+    for s, (K, f) in enumerate(res):
+        Ko, fo = o.assemble(s)
+        rp, _ = o.pattern(s)
+        assert csr_err(K, Ko, rp) <= RTOL_FP64 and vec_err(f, fo) <= RTOL_FP64, f"step {s}"
