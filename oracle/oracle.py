@@ -155,6 +155,7 @@ class Oracle:
         return np.ctypeslib.as_array(p, shape=(n,))
 
     def history(self, model, name):
+        """A writable view into the oracle's history array (valid while this Oracle lives)."""
         nc = C.c_int64()
         p = self.L.ifo_history(self.h, model, name.encode(), C.byref(nc))
         if not p:

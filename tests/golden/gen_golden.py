@@ -38,7 +38,7 @@ def make(name, nel):
         out[f"row_ptr_{s}"], out[f"col_{s}"], out[f"K_{s}"], out[f"f_{s}"] = rp, col, K, f
     if prob.models[0].type_name == "Fracture/PhaseField":
         for h in HIST:
-            out[f"out_hist_{h}"] = o.history(0, h)
+            out[f"out_hist_{h}"] = o.history(0, h).copy()
     st = o.stats()
     out["return_map_iterations"] = np.array(st["return_map_iterations"])
     return out

@@ -51,6 +51,11 @@ def test_oracle_reproduces_golden(path):
         K, f = o.assemble(s)
         assert csr_err(K, g[f"K_{s}"], rp) <= 1e-13 and vec_err(f, g[f"f_{s}"]) <= 1e-13
     assert o.stats()["return_map_iterations"] == int(g["return_map_iterations"])
+    if "out_hist_Energy" in g:
+        for h in HIST:
+            ref = g[f"out_hist_{h}"]
+            assert np.all(np.isfinite(ref)), h
+            assert np.max(np.abs(o.history(0, h) - ref)) <= 1e-13 * max(np.max(np.abs(ref)), 1e-300), h
 
 
 @pytest.mark.gpu
