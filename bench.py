@@ -134,10 +134,10 @@ def main():
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        rate, ne, secs = cpu_baseline(args.workload, (20, 20, 20) if args.workload.startswith("hex3d") else None,
+        rate, ne, secs = cpu_baseline(args.workload, (30, 30, 30) if args.workload.startswith("hex3d") else None,
                                       threads)
         out["cpu_baseline"] = {"value": rate, "unit": "elements/s", "cores": threads, "kind": "port",
-                               "sample": f"{ne} elements (20x20x20 block of the same workload), all three steps, "
+                               "sample": f"{ne} elements (30x30x30 block of the same workload), all three steps, "
                                          f"{secs:.1f}s wall on {threads} threads"}
         out["gpu_over_cpu"] = value / rate
     print(json.dumps(out), flush=True)
