@@ -152,7 +152,7 @@ struct ife_physics {
     std::vector<Step> steps;
     std::vector<int> fpos;          // field -> position within its step (-1 unused)
     // device
-    DevBuf<double> d_nodes, d_ext, d_lcrack;
+    DevBuf<double> d_nodes, d_ext, d_lcrack, d_ipb;
     DevBuf<int32_t> d_enode, d_erank, d_colors;
     DevBuf<uint8_t> d_loc;
     DevBuf<int64_t> d_nbr_ptr;
@@ -164,12 +164,13 @@ struct ife_physics {
     // launch timing
     bool profiling = false;
     struct Rec { hipEvent_t a, b; int64_t elems; };
-    std::vector<Rec> recs;
+    std::vector<Rec> recs, mat_recs;
     std::vector<hipEvent_t> event_pool;
 
     ~ife_physics() {
         for (auto* m : models) delete m;
         for (auto& r : recs) { event_pool.push_back(r.a); event_pool.push_back(r.b); }
+        for (auto& r : mat_recs) { event_pool.push_back(r.a); event_pool.push_back(r.b); }
         for (auto e : event_pool) (void)hipEventDestroy(e);
     }
     hipEvent_t get_event() {
@@ -420,6 +421,7 @@ void setup_device(ife_physics* ph) {
     ph->d_colors.upload(ph->color_elems.data(), ph->color_elems.size());
     ph->d_status.alloc(1);
     HIPCHK(hipMemset(ph->d_status.p, 0, sizeof(Status)));
+    ph->d_ipb.alloc((size_t)IPB_NF * ph->NE * ph->nip);
     ph->d_lcrack.alloc(ph->NE);
     HIPCHK(hipMemset(ph->d_lcrack.p, 0, ph->NE * sizeof(double)));
     for (auto& st : ph->steps) {
@@ -521,6 +523,7 @@ KParams base_params(ife_physics* ph) {
     p.NIPT = ph->NE * ph->nip;
     p.st = ph->d_status.p;
     p.lcrack_elem = ph->d_lcrack.p;
+    p.ipb = ph->d_ipb.p;
     p.mixed = ph->precision == IFE_PREC_MIXED;
     return p;
 }
@@ -559,6 +562,16 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
     ph->last_had_phase = p.do_ph;
     HIPCHK(launch_zero_status(ph->d_status.p, s));
     if (!(p.do_u || p.do_ph || p.do_Tpf || p.do_th)) return;
+    // material update over all IPs, then the coloured element launches
+    if (ph->profiling) {
+        ife_physics::Rec r{ph->get_event(), ph->get_event(), 0};
+        HIPCHK(hipEventRecord(r.a, s));
+        HIPCHK(launch_material(p, ph->dim, s));
+        HIPCHK(hipEventRecord(r.b, s));
+        ph->mat_recs.push_back(r);
+    } else {
+        HIPCHK(launch_material(p, ph->dim, s));
+    }
     const int ncol = (int)ph->color_ptr.size() - 1;
     for (int c = 0; c < ncol; c++) {
         p.elist = ph->d_colors.p + ph->color_ptr[c];
@@ -858,6 +871,18 @@ int ife_physics_timings(ife_physics* ph, ife_timings* out) {
             ph->event_pool.push_back(r.b);
         }
         ph->recs.clear();
+        out->material_ms_total = 0;
+        out->material_launches = 0;
+        for (auto& r : ph->mat_recs) {
+            HIPCHK(hipEventSynchronize(r.b));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            out->material_launches++;
+            out->material_ms_total += ms;
+            ph->event_pool.push_back(r.a);
+            ph->event_pool.push_back(r.b);
+        }
+        ph->mat_recs.clear();
     });
 }
 

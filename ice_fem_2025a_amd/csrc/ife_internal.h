@@ -45,6 +45,18 @@ struct StepIO {
     int nf;              // fields in the step (row blocks)
 };
 
+// Per-IP coefficients written by the material kernel and contracted by the element
+// kernel, layout [field][e*nip + ip]:
+//   M    w*(Dam*D_Dam + D_UnDam)*DPlast (6x6, row-major)    SIG  w*(Dam*S_Dam + S_UnDam)
+//   PHA  coefficient of N N^T in K_phiphi                     PHB  coefficient of G^T G
+//   PHFN coefficient of N in F_phi                            PHFG coefficient of G^T (3)
+//   LDRIVE w*Energy (lumped driving)   GW gamma*w (crack length)   GC Gc at the IP
+//   THB  w*k   THFN -w*(heating rate)   THFG w*k*grad T (3)
+enum {
+    IPB_M = 0, IPB_SIG = 36, IPB_PHA = 42, IPB_PHB = 43, IPB_PHFN = 44, IPB_PHFG = 45, IPB_LDRIVE = 48,
+    IPB_THB = 49, IPB_THFN = 50, IPB_THFG = 51, IPB_GW = 54, IPB_GC = 55, IPB_NF = 56
+};
+
 struct Status {
     unsigned long long elements;
     unsigned long long rm_calls;
@@ -84,11 +96,13 @@ struct KParams {
     int64_t NIPT;
     Status* st;
     double* lcrack_elem;     // [e]
+    double* ipb;             // per-IP coefficients [IPB_NF][NIPT]
     int mixed;               // tangent contractions in FP32
 };
 
 // Launchers (ife_kernels.hip)
 hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s);
+hipError_t launch_material(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 hipError_t launch_zero_status(Status* st, hipStream_t s);

@@ -1,13 +1,17 @@
 // ife_kernels.hip -- element residual/tangent assembly kernels for gfx950 (MI355X).
 //
-// One workgroup (256 lanes, 4 wavefronts) assembles one element of one colour: the
-// element's geometry, extraction operators and nodal state are gathered into LDS,
-// shape functions are built there once and shared by every staggered step, the
-// per-IP material update (return map, energy split, history max) runs one IP per lane,
-// and the element tangents are register-tiled contractions over (IP, gradient
-// component).  Element contributions are added straight into the CSR values: elements
-// of one colour share no node, so no two workgroups of a launch touch the same CSR row
-// and the read-modify-write needs no atomics (deterministic, bit-reproducible order).
+// Two kernels per assembly:
+//   material_kernel  one lane per integration point over all elements: shape functions
+//                    at the IP, strains, the Glen return map, energy split, history max and
+//                    the crack-density / heating terms; writes per-IP coefficients.  The
+//                    data-dependent Newton loop runs here, where many waves hide its latency.
+//   assemble_kernel  one workgroup (256 lanes) per element of one colour: geometry and
+//                    extraction gathered into LDS, shape functions built there once and shared
+//                    by every staggered step, register-tiled contractions over (IP, gradient
+//                    component), and the element contributions added straight into the CSR
+//                    values.  Elements of one colour share no node, so no two workgroups of a
+//                    launch touch the same CSR row and the read-modify-write needs no atomics
+//                    (deterministic, bit-reproducible order).
 //
 // Reference semantics (paths relative to the IceFEM source tree):
 //   shape functions  mesh/ElementTypes/Bezier/B2_2.cpp:72-88 (2D, transposed-Jacobian form)
@@ -152,14 +156,13 @@ __device__ double yield_ve(const double S[6], double dF[6]) {
 
 // return_mapping_getKF (SolidMaterial.cpp:278-301).  K = [[I, a],[b^T, -1]] with
 // a = dF (ddF is zeroed in the reference, :289) and b = fac*Damage*D*dF.
-__device__ void getKF(const MatIP& m, double nC, double dt, double Damage, const double sol[7], const double eps_t[6],
+__device__ void getKF(const MatIP& m, double nC, double PreFac, double Damage, const double sol[7], const double eps_t[6],
                       const double eps_old[6], double f[7], double a[6], double b[6]) {
     double S[6];
     Dmul(m, sol, S);
     double F = yield_ve(S, a);
     for (int i = 0; i < 6; i++) f[i] = sol[i] - eps_t[i] + eps_old[i] + sol[6] * a[i];
     const double sc = 1.0e6;
-    double PreFac = m.ACreep * dt * pow(sc, nC);
     f[6] = PreFac * pow(F / sc, nC) - sol[6];
     double fac = nC * PreFac / sc * pow(F / sc, nC - 1.0);
     double Da[6], da[6];
@@ -180,13 +183,16 @@ __device__ __forceinline__ void solveK(const double a[6], const double b[6], con
 __device__ __forceinline__ double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); }
 
 // return_mapping_Viscous (SolidMaterial.cpp:200-269).  Returns the dissipation rate.
-__device__ double return_map_viscous(const MatIP& m, double nC, double dt, double Damage, double DP[6][6], double eps_pl[6],
-                                     const double eps_t[6], const double eps_pl_old[6], double& hist0, int& iters,
-                                     int& maxit) {
+// DPlast = (K^{-1})(0:5,0:5) = I - a b^T * inv at the final iterate; returned as (a, b, inv)
+__device__ double return_map_viscous(const MatIP& m, double nC, double dt, double Damage, double ra[6], double rb[6],
+                                     double& rinv, double eps_pl[6], const double eps_t[6], const double eps_pl_old[6],
+                                     double& hist0, int& iters, int& maxit) {
     double sol[7], f[7], fOld[7], dsol[7], a[6], b[6];
     for (int i = 0; i < 6; i++) sol[i] = eps_t[i] - eps_pl[i];
     sol[6] = hist0;
-    getKF(m, nC, dt, Damage, sol, eps_t, eps_pl_old, f, a, b);
+    // PreFac = ACreep*dt*sc^n (return_mapping_getKF :294), the same value on every call
+    const double PreFac = m.ACreep * dt * pow(1.0e6, nC);
+    getKF(m, nC, PreFac, Damage, sol, eps_t, eps_pl_old, f, a, b);
     double e0 = 0;
     for (int i = 0; i < 7; i++) e0 += f[i] * f[i];
     e0 = fmax(e0, 1e-12);
@@ -197,7 +203,7 @@ __device__ double return_map_viscous(const MatIP& m, double nC, double dt, doubl
         double x[7];
         solveK(a, b, f, x);
         for (int i = 0; i < 7; i++) { dsol[i] = -x[i]; sol[i] += dsol[i]; fOld[i] = f[i]; }
-        getKF(m, nC, dt, Damage, sol, eps_t, eps_pl_old, f, a, b);
+        getKF(m, nC, PreFac, Damage, sol, eps_t, eps_pl_old, f, a, b);
         double ls1 = 0, ls2 = 0;
         for (int i = 0; i < 7; i++) { ls1 += f[i] * dsol[i]; ls2 += fOld[i] * dsol[i]; }
         double ls;
@@ -208,7 +214,7 @@ __device__ double return_map_viscous(const MatIP& m, double nC, double dt, doubl
             ls = fmin(fmax(ls, 0.2), 1.0);
         }
         for (int i = 0; i < 7; i++) sol[i] = sol[i] - (1.0 - ls) * dsol[i];
-        getKF(m, nC, dt, Damage, sol, eps_t, eps_pl_old, f, a, b);
+        getKF(m, nC, PreFac, Damage, sol, eps_t, eps_pl_old, f, a, b);
         double e = 0;
         for (int i = 0; i < 7; i++) e += f[i] * f[i];
         if (e < 1.0e-16 || e / e0 < 1.0e-9) break;
@@ -219,12 +225,10 @@ __device__ double return_map_viscous(const MatIP& m, double nC, double dt, doubl
     Dmul(m, sol, S);
     yield_ve(S, dF);
     for (int i = 0; i < 6; i++) eps_pl[i] = eps_pl_old[i] + sol[6] * dF[i];
-    // DPlast = (K^{-1})(0:5,0:5) = I - a b^T / (1 + b^T a) at the final iterate
     double bta = 0;
     for (int i = 0; i < 6; i++) bta += b[i] * a[i];
-    double inv = 1.0 / (1.0 + bta);
-    for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++) DP[i][j] = (i == j ? 1.0 : 0.0) - a[i] * b[j] * inv;
+    rinv = 1.0 / (1.0 + bta);
+    for (int i = 0; i < 6; i++) { ra[i] = a[i]; rb[i] = b[i]; }
     double Diss = 0;
     for (int i = 0; i < 6; i++) Diss += S[i] * (eps_pl[i] - eps_pl_old[i]);
     Diss = Damage * Diss;
@@ -345,43 +349,421 @@ __device__ void cap_coeffs(int method, double l, double phi, double& dN, double&
     }
 }
 
+// energy split folded with the degradation: S = Dam*D_Dam + D_UnDam, sig = Dam*S_Dam + S_UnDam
+__device__ double energy_split_combined(const MatConst& mc, const MatIP& m, const double eps[6], double Dam, double S[6][6],
+                                        double sig[6]) {
+    double DD[6][6], DU[6][6], SD[6], SU[6];
+    double E = energy_split(mc, m, eps, DD, DU, SD, SU);
+    for (int i = 0; i < 6; i++) {
+        sig[i] = Dam * SD[i] + SU[i];
+        for (int j = 0; j < 6; j++) S[i][j] = Dam * DD[i][j] + DU[i][j];
+    }
+    return E;
+}
+
+// ---------------------------------------------------------------------------------------
+// Material-point kernel: one lane per integration point, EPB elements per workgroup with
+// their nodal data staged in LDS.  Recomputes the IP's shape functions from the element
+// geometry, then runs the u-step material update (return map, energy split), the phase
+// step's energy/history/crack-density terms and the T-step source terms, and writes the
+// per-IP coefficients the element kernel contracts (ife::IPB_* layout, [field][e*nip+ip]).
+// ---------------------------------------------------------------------------------------
+template <int DIM>
+struct MatSmem {
+    static constexpr int N = ET<DIM>::N, EPB = NT / ET<DIM>::NIP;
+    double X[EPB][N][4];
+    double U[EPB][DIM][N], Uo[EPB][DIM][N], Ph[EPB][N], PhO[EPB][N], T[EPB][N];
+    double C1[EPB][DIM][9];
+    unsigned cnt[4];
+};
+
+template <int DIM>
+__global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
+    using E = ET<DIM>;
+    constexpr int N = E::N, NIP = E::NIP, EPB = NT / NIP;
+    __shared__ MatSmem<DIM> sm;
+    const int tid = threadIdx.x;
+    const MatConst& mc = p.mc;
+    const int64_t e0 = (int64_t)blockIdx.x * EPB;
+    if (tid < 4) sm.cnt[tid] = 0;
+    // ---- stage the block's elements ----------------------------------------------
+    for (int i = tid; i < EPB * N; i += NT) {
+        int le = i / N, a = i % N;
+        int64_t el = e0 + le;
+        if (el >= p.NE) continue;
+        int v = p.enode[el * N + a];
+        int r = p.erank[el * N + a];
+        const double* nd = p.nodes + (int64_t)v * 4;
+        for (int c = 0; c < 4; c++) sm.X[le][a][c] = nd[c];
+        if (p.has_pf) {
+            for (int d = 0; d < DIM; d++) {
+                int64_t dd = (int64_t)p.fpos_u[d] * p.Ng + r;
+                sm.U[le][d][a] = p.su.x[dd];
+                sm.Uo[le][d][a] = p.su.xold[dd];
+            }
+            int64_t dp = (int64_t)p.fpos_ph * p.Ng + r;
+            sm.Ph[le][a] = p.sph.x[dp];
+            sm.PhO[le][a] = p.sph.xold[dp];
+        }
+        if (p.has_T) sm.T[le][a] = p.sT.x[(int64_t)p.fpos_T * p.Ng + r];
+    }
+    for (int i = tid; i < EPB * DIM * 9; i += NT) {
+        int le = i / (DIM * 9), k = i % (DIM * 9);
+        int64_t el = e0 + le;
+        if (el < p.NE) (&sm.C1[le][0][0])[k] = p.ext[el * DIM * 9 + k];
+    }
+    __syncthreads();
+    const int le = tid / NIP, ip = tid % NIP;
+    const int64_t el = e0 + le;
+    unsigned my_calls = 0, my_iters = 0, my_maxit = 0, my_bad = 0;
+    if (le < EPB && el < p.NE) {
+        const int64_t gi = el * NIP + ip;
+        const int64_t S = p.NIPT;
+        int q[3] = {ip % 3, (ip / 3) % 3, ip / 9};
+        // 1D B-spline values/derivatives at this IP: b = C_d * Bernstein(xi_q)
+        double b1[DIM][3], g1[DIM][3];
+        for (int d = 0; d < DIM; d++)
+            for (int a = 0; a < 3; a++) {
+                double s = 0, sd = 0;
+                for (int k = 0; k < 3; k++) {
+                    s += sm.C1[le][d][a * 3 + k] * p.B1[q[d]][k];
+                    sd += sm.C1[le][d][a * 3 + k] * p.dB1[q[d]][k];
+                }
+                b1[d][a] = s;
+                g1[d][a] = sd;
+            }
+        auto cbgb = [&](int a, double& cb, double gb[DIM]) {
+            int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+            if (DIM == 2) {
+                cb = b1[0][ax] * b1[1][ay];
+                gb[0] = g1[0][ax] * b1[1][ay];
+                gb[1] = b1[0][ax] * g1[1][ay];
+            } else {
+                cb = b1[0][ax] * b1[1][ay] * b1[DIM - 1][az];
+                gb[0] = g1[0][ax] * b1[1][ay] * b1[DIM - 1][az];
+                gb[1] = b1[0][ax] * g1[1][ay] * b1[DIM - 1][az];
+                gb[DIM - 1] = b1[0][ax] * b1[1][ay] * g1[DIM - 1][az];
+            }
+        };
+        // rational weights (3D): wFun = CB.W, dW = GB^T W (NURBS3_Cube2.cpp:96-103)
+        double wF = 1.0, dW[DIM];
+        for (int c = 0; c < DIM; c++) dW[c] = 0;
+        if (DIM == 3) {
+            wF = 0;
+#pragma unroll 1
+            for (int a = 0; a < N; a++) {
+                double cb, gb[DIM];
+                cbgb(a, cb, gb);
+                double Wa = sm.X[le][a][3];
+                wF += cb * Wa;
+                for (int c = 0; c < DIM; c++) dW[c] += gb[c] * Wa;
+            }
+        }
+        auto nshape = [&](int a, double& Na, double gx[DIM]) {  // N and GXi (3D) / GB (2D)
+            double cb, gb[DIM];
+            cbgb(a, cb, gb);
+            if (DIM == 3) {
+                double Wa = sm.X[le][a][3];
+                Na = Wa * cb / wF;
+                for (int c = 0; c < DIM; c++) gx[c] = Wa * (gb[c] / wF - cb * dW[c] / wF / wF);
+            } else {
+                Na = cb;
+                for (int c = 0; c < DIM; c++) gx[c] = gb[c];
+            }
+        };
+        double J[DIM][DIM];
+        for (int c = 0; c < DIM; c++)
+            for (int d = 0; d < DIM; d++) J[c][d] = 0;
+#pragma unroll 1
+        for (int a = 0; a < N; a++) {
+            double Na, gx[DIM];
+            nshape(a, Na, gx);
+            for (int c = 0; c < DIM; c++)
+                for (int d = 0; d < DIM; d++) J[c][d] += gx[c] * sm.X[le][a][d];
+        }
+        double Ji[DIM][DIM], w;
+        if (DIM == 3) {
+            double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+            double c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+            double c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+            double det = J[0][0] * c00 + J[0][1] * c10 + J[0][2] * c20;
+            double id = 1.0 / det;
+            Ji[0][0] = c00 * id; Ji[1][0] = c10 * id; Ji[2][0] = c20 * id;
+            Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+            Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+            Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+            Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+            Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+            Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+            double dt3 = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[1][0] * (J[0][1] * J[2][2] - J[0][2] * J[2][1]) +
+                         J[2][0] * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+            w = p.wq[q[0]] * p.wq[q[1]] * p.wq[q[2]] * fabs(dt3);
+        } else {
+            // B2_2: J = (GB X)^T, then inv(J) applied to GB (B2_2.cpp:79-85)
+            double Jt[2][2] = {{J[0][0], J[1][0]}, {J[0][1], J[1][1]}};
+            double det = Jt[0][0] * Jt[1][1] - Jt[0][1] * Jt[1][0];
+            double id = 1.0 / det;
+            Ji[0][0] = Jt[1][1] * id;
+            Ji[1][0] = -Jt[1][0] * id;
+            Ji[0][1] = -Jt[0][1] * id;
+            Ji[1][1] = Jt[0][0] * id;
+            w = p.wq[q[0]] * p.wq[q[1]] * fabs(det);
+        }
+        // field values and gradients at the IP
+        double gu[DIM][DIM], guo[DIM][DIM], gphi[DIM], gT[DIM];
+        double phi = 0, phio = 0, Tu = 0, Tph = 0;
+        for (int c = 0; c < DIM; c++) {
+            gphi[c] = 0; gT[c] = 0;
+            for (int d = 0; d < DIM; d++) { gu[c][d] = 0; guo[c][d] = 0; }
+        }
+        const double T0 = p.has_T ? p.sT.x[0] : 0.0;  // Assemble_Phase reads T dof 0 for every node
+#pragma unroll 1
+        for (int a = 0; a < N; a++) {
+            double Na, gx[DIM], G[DIM];
+            nshape(a, Na, gx);
+            for (int c = 0; c < DIM; c++) {
+                double s = 0;
+                for (int d = 0; d < DIM; d++) s += Ji[c][d] * gx[d];
+                G[c] = s;
+            }
+            if (p.has_pf) {
+                phi += Na * sm.Ph[le][a];
+                phio += Na * sm.PhO[le][a];
+                for (int c = 0; c < DIM; c++) gphi[c] += G[c] * sm.Ph[le][a];
+                for (int al = 0; al < DIM; al++)
+                    for (int c = 0; c < DIM; c++) {
+                        gu[al][c] += G[c] * sm.U[le][al][a];
+                        guo[al][c] += G[c] * sm.Uo[le][al][a];
+                    }
+            }
+            if (p.has_T) {
+                Tu += Na * sm.T[le][a];
+                Tph += Na * (mc.phase_T_from_dof0 ? T0 : sm.T[le][a]);
+                for (int c = 0; c < DIM; c++) gT[c] += G[c] * sm.T[le][a];
+            }
+        }
+        if (!p.has_T) Tu = Tph = mc.T_default;
+        double eps[6], epso[6], eps_pl[6];
+        for (int r = 0; r < 6; r++) { eps[r] = 0; epso[r] = 0; }
+        for (int al = 0; al < DIM; al++)
+            for (int c = 0; c < DIM; c++) { eps[vrow<DIM>(al, c)] += gu[al][c]; epso[vrow<DIM>(al, c)] += guo[al][c]; }
+        if (p.has_pf && (p.do_u || p.do_ph))
+            for (int r = 0; r < 6; r++) eps_pl[r] = p.PS[r * S + gi];
+        MatIP m;
+        double VD = 0;
+        if (p.has_pf && p.do_u) {
+            set_temperature(mc, Tu, mc.porosity, m);
+            double Dam, dDam, ddDam;
+            damage(mc.DamFunction, phi, mc.k0, mc.p, Dam, dDam, ddDam);
+            double epo[6];
+            for (int r = 0; r < 6; r++) epo[r] = p.PSOld[r * S + gi];
+            double ra[6] = {0, 0, 0, 0, 0, 0}, rb[6] = {0, 0, 0, 0, 0, 0}, rinv = 0.0;
+            double diss = 0, ddiss = 0;
+            int iters = 0, mx = 0;
+            bool called = false;
+            double hist0 = mc.HistSize > 0 ? p.MH[gi] : 0.0;
+            const int rh = mc.Rheology;
+            bool elastic = true;  // DPlast = I
+            if (rh == IFE_RHEO_VE || rh == IFE_RHEO_DVE) {
+                if (!(phi > mc.PhiViscLim)) {
+                    double dmg = 1.0, u1, u2;
+                    if (rh == IFE_RHEO_DVE) damage(mc.DamFunction, phi, mc.k0, mc.p, dmg, u1, u2);
+                    called = true;
+                    elastic = false;
+                    diss = return_map_viscous(m, mc.nCreep, p.dt, dmg, ra, rb, rinv, eps_pl, eps, epo, hist0, iters, mx);
+                }
+            } else if (rh == IFE_RHEO_VE_EXPL || rh == IFE_RHEO_DVE_EXPL) {
+                if (!(phio > mc.PhiViscLim)) {
+                    double dmg = 1.0, u1, u2;
+                    if (rh == IFE_RHEO_DVE_EXPL) damage(mc.DamFunction, phio, mc.k0, mc.p, dmg, u1, u2);
+                    called = true;
+                    diss = return_map_viscous(m, mc.nCreep, p.dt, dmg, ra, rb, rinv, eps_pl, epso, epo, hist0, iters, mx);
+                }  // explicit variants keep DPlast = I
+            }
+            if (!called)
+                for (int r = 0; r < 6; r++) eps_pl[r] = 1.0 * epo[r];
+            if (called) {
+                my_calls++;
+                my_iters += iters;
+                my_maxit += mx;
+            }
+            if (mc.HistSize > 0) p.MH[gi] = hist0;
+            for (int r = 0; r < 6; r++) p.PS[r * S + gi] = eps_pl[r];
+            VD = p.VDOld[gi] + diss * p.dt;
+            p.VD[gi] = VD;
+            p.DVD[gi] = p.DVDOld[gi] + ddiss * p.dt;
+            double ee[6];
+            for (int r = 0; r < 6; r++) ee[r] = eps[r] - eps_pl[r];
+            double Sm[6][6], sig[6];
+            energy_split_combined(mc, m, ee, Dam, Sm, sig);
+            for (int r = 0; r < 6; r++) {
+                double v = w * sig[r];
+                my_bad |= !isfinite(v);
+                p.ipb[(int64_t)(IPB_SIG + r) * S + gi] = v;
+            }
+            // Mt = w * S * DPlast = w * (S - (S a) b^T inv)
+            for (int i = 0; i < 6; i++) {
+                double sa = 0;
+                if (!elastic)
+                    for (int k = 0; k < 6; k++) sa += Sm[i][k] * ra[k];
+                for (int j = 0; j < 6; j++) {
+                    double v = elastic ? Sm[i][j] : (Sm[i][j] - sa * rb[j] * rinv);
+                    p.ipb[(int64_t)(IPB_M + i * 6 + j) * S + gi] = w * v;
+                }
+            }
+        } else if (p.has_pf && p.do_Tpf) {
+            VD = p.VD[gi];
+        }
+        if (p.has_pf && p.do_ph) {
+            set_temperature(mc, Tph, mc.porosity, m);
+            double Dam, dDam, ddDam, DamI, dDamI, ddDamI, DamG, dDamG, ddDamG;
+            damage(mc.DamFunction, phi, mc.k0, mc.p, Dam, dDam, ddDam);
+            damage(mc.DensityDamage, phi, mc.k0, mc.p, DamI, dDamI, ddDamI);
+            damage(mc.GravityDamage, phi, mc.k0, mc.p, DamG, dDamG, ddDamG);
+            double ee[6];
+            for (int r = 0; r < 6; r++) ee[r] = eps[r] - eps_pl[r];
+            double Sm[6][6], sig[6];
+            double Energy = energy_split_combined(mc, m, ee, 1.0, Sm, sig);
+            if (mc.Irreversible == IFE_IRR_HIST) {  // strict '>' (PhaseField.cpp:631-640)
+                double Ho = p.EOld[gi];
+                if (Ho > Energy) { p.E[gi] = Ho; Energy = Ho; }
+                else p.E[gi] = Energy;
+            } else {
+                p.E[gi] = Energy;
+            }
+            double gg = 0;
+            for (int c = 0; c < DIM; c++) gg += gphi[c] * gphi[c];
+            double dN, cNN, cGG, gamma;
+            dist_coeffs(mc.Distribution, mc.l, phi, gg, dN, cNN, cGG, gamma);
+            double capN = 0, capNN = 0;
+            if (mc.LumpedCap) {
+                double u1, u2;
+                cap_coeffs(mc.Distribution, mc.l, phi, capN, capNN, u1, u2);
+            }
+            double A = w * m.Gc * (cNN - capNN), FN = w * m.Gc * (dN - capN), LDR = 0;
+            if (!mc.LumpedDrive) {
+                FN += w * dDam * Energy;
+                A += w * ddDam * Energy;
+            } else {
+                LDR = w * Energy;
+            }
+            if (mc.ViscoDrivingFactor > 1.0e-9) {
+                double DamV, dDamV, ddDamV;
+                double PD = mc.ViscoDrivingFactor * p.DVD[gi];
+                damage(mc.ViscoDriving, phi, mc.k0, mc.p, DamV, dDamV, ddDamV);
+                FN += w * dDamV * PD;
+                A += w * ddDamV * PD;
+            }
+            if (mc.Inertia && mc.ConsistentDegradation) {
+                double v2 = 0;
+                for (int d = 0; d < DIM; d++) {
+                    double vd = 0;
+                    for (int a = 0; a < N; a++) {
+                        double Na, gx[DIM];
+                        nshape(a, Na, gx);
+                        vd += Na * p.su.dx[(int64_t)p.fpos_u[d] * p.Ng + p.erank[el * N + a]];
+                    }
+                    v2 += vd * vd;
+                }
+                FN += -w * 0.5 * mc.Density * v2 * dDamI;
+                A += -w * 0.5 * mc.Density * v2 * ddDamI;
+            }
+            if (mc.ConsistentDegradation) {
+                double gu_ = 0;
+                for (int d = 0; d < DIM; d++) {
+                    double ud = 0;
+                    for (int a = 0; a < N; a++) {
+                        double Na, gx[DIM];
+                        nshape(a, Na, gx);
+                        ud += Na * sm.U[le][d][a];
+                    }
+                    gu_ += mc.g[d] * ud;
+                }
+                FN += -w * mc.Density * gu_ * dDamG;
+                A += -w * mc.Density * gu_ * ddDamG;
+            }
+            p.ipb[(int64_t)IPB_PHA * S + gi] = A;
+            p.ipb[(int64_t)IPB_PHB * S + gi] = w * m.Gc * cGG;
+            p.ipb[(int64_t)IPB_PHFN * S + gi] = FN;
+            for (int c = 0; c < DIM; c++) p.ipb[(int64_t)(IPB_PHFG + c) * S + gi] = w * m.Gc * cGG * gphi[c];
+            p.ipb[(int64_t)IPB_LDRIVE * S + gi] = LDR;
+            p.ipb[(int64_t)IPB_GW * S + gi] = gamma * w;
+            p.ipb[(int64_t)IPB_GC * S + gi] = m.Gc;
+            my_bad |= !isfinite(A) || !isfinite(FN);
+        }
+        if (p.do_Tpf || p.do_th) {
+            double fn = 0;
+            if (p.has_pf && p.do_Tpf) fn = -w * ((VD - p.VDOld[gi]) / p.dt);
+            p.ipb[(int64_t)IPB_THFN * S + gi] = fn;
+            p.ipb[(int64_t)IPB_THB * S + gi] = p.do_th ? w * mc.k : 0.0;
+            for (int c = 0; c < DIM; c++) p.ipb[(int64_t)(IPB_THFG + c) * S + gi] = p.do_th ? w * mc.k * gT[c] : 0.0;
+        }
+    }
+    if (my_calls) {
+        atomicAdd(&sm.cnt[0], my_calls);
+        atomicAdd(&sm.cnt[1], my_iters);
+        if (my_maxit) atomicAdd(&sm.cnt[2], my_maxit);
+    }
+    if (my_bad) atomicOr(&sm.cnt[3], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        if (sm.cnt[0]) {
+            atomicAdd(&p.st->rm_calls, (unsigned long long)sm.cnt[0]);
+            atomicAdd(&p.st->rm_iters, (unsigned long long)sm.cnt[1]);
+        }
+        if (sm.cnt[2]) atomicAdd(&p.st->rm_maxit, (unsigned long long)sm.cnt[2]);
+        if (sm.cnt[3]) atomicOr(&p.st->nonfinite, 1u);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Element kernel: one workgroup per element of one colour.  Shape functions in LDS,
+// per-IP coefficients from the material kernel, register-tiled contractions, and the
+// coloured read-modify-write into the CSR values and residuals.
+// ---------------------------------------------------------------------------------------
 template <int DIM>
 struct Smem {
     static constexpr int N = ET<DIM>::N, NIP = ET<DIM>::NIP, NU = ET<DIM>::NU, CH = ET<DIM>::CH;
     double X[N][4];            // x, y, z, W
-    double b1[DIM][3][3];      // 1D B-spline values  [dir][gauss pt][fn]
-    double db1[DIM][3][3];     // 1D derivatives
     double Nsh[NIP][N];        // N (first CB)
     double G[NIP][DIM][N];     // G (first GB, then GXi)
-    double red[NIP][4];        // wFun, dW
-    double Jinv[NIP][DIM][DIM];
     double w[NIP];
-    double U[DIM][N], Uo[DIM][N], dU[DIM][N], ddU[DIM][N];
-    double Ph[N], PhO[N], T[N], dT[N];
-    double Mt[NIP][6][6];      // w*(Dam*D_Dam + D_UnDam)*DPlast
+    union {
+        struct {
+            double b1[DIM][3][3];      // 1D B-spline values  [dir][gauss pt][fn]
+            double db1[DIM][3][3];     // 1D derivatives
+            double red[NIP][4];        // wFun, dW
+            double Jinv[NIP][DIM][DIM];
+        } shp;
+        struct {
+            double Mt[CH][6][6];       // chunk of w*(Dam*D_Dam + D_UnDam)*DPlast
+            double Y[CH][6][NU];
+        } ku;
+    };
+    double dU[DIM][N], ddU[DIM][N];
+    double Ph[N], PhO[N], dT[N];
     double sig[NIP][6];        // w*(Dam*S_Dam + S_UnDam)
-    double phA[NIP], phB[NIP], phFN[NIP], phFG[NIP][DIM];
+    double phA[NIP], phB[NIP], phFN[NIP], phFG[NIP][DIM], ldrive[NIP], gw[NIP];
     double thB[NIP], thFN[NIP], thFG[NIP][DIM];
-    double ldrive[NIP];        // w*Energy (lumped driving)
-    double Y[CH][6][NU];
-    double kdu[DIM][N];        // nodal diagonal additions to K_UU
+    double kdu[N];             // nodal diagonal additions to K_UU
     double WL[N];
     double GcLast;
     int64_t rowst[N];          // nbr_ptr[rank]
     int deg[N];
     int rank[N];
     uint8_t loc[N][N];
-    unsigned int cnt_calls, cnt_iters, cnt_maxit, nonfinite;
+    unsigned int nonfinite;
 };
 
 template <int DIM>
-__global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
+__global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
     using E = ET<DIM>;
     constexpr int N = E::N, NIP = E::NIP, NU = E::NU;
     __shared__ Smem<DIM> sm;
     const int tid = threadIdx.x;
     const int64_t el = p.elist[blockIdx.x];
     const MatConst& mc = p.mc;
+    const int64_t S = p.NIPT, gi0 = el * NIP;
 
     // ---- gather -------------------------------------------------------------------
     if (tid < N) {
@@ -396,8 +778,6 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
         if (p.has_pf) {
             for (int a = 0; a < DIM; a++) {
                 int64_t d = (int64_t)p.fpos_u[a] * p.Ng + r;
-                sm.U[a][tid] = p.su.x[d];
-                sm.Uo[a][tid] = p.su.xold[d];
                 sm.dU[a][tid] = p.su.dx[d];
                 sm.ddU[a][tid] = p.su.ddx[d];
             }
@@ -405,23 +785,37 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
             sm.Ph[tid] = p.sph.x[dp];
             sm.PhO[tid] = p.sph.xold[dp];
         }
-        if (p.has_T) {
-            int64_t dt_ = (int64_t)p.fpos_T * p.Ng + r;
-            sm.T[tid] = p.sT.x[dt_];
-            sm.dT[tid] = p.sT.dx[dt_];
-        }
+        if (p.has_T) sm.dT[tid] = p.sT.dx[(int64_t)p.fpos_T * p.Ng + r];
     }
-    if (tid == 0) { sm.cnt_calls = 0; sm.cnt_iters = 0; sm.cnt_maxit = 0; sm.nonfinite = 0; }
+    if (tid == 0) sm.nonfinite = 0;
     for (int i = tid; i < N * N; i += NT) (&sm.loc[0][0])[i] = p.loc[el * N * N + i];
-    // 1D extraction applied to the Bernstein tables: b = C_d * B(xi_q)
     if (tid < DIM * 9) {
         int d = tid / 9, q = (tid / 3) % 3, a = tid % 3;
         const double* C = p.ext + (el * DIM + d) * 9;
         double s = 0, sd = 0;
         for (int k = 0; k < 3; k++) { s += C[a * 3 + k] * p.B1[q][k]; sd += C[a * 3 + k] * p.dB1[q][k]; }
-        sm.b1[d][q][a] = s;
-        sm.db1[d][q][a] = sd;
+        sm.shp.b1[d][q][a] = s;
+        sm.shp.db1[d][q][a] = sd;
     }
+    // per-IP coefficients of this element (contiguous per field)
+    if (p.do_u)
+        for (int i = tid; i < NIP * 6; i += NT) sm.sig[i / 6][i % 6] = p.ipb[(int64_t)(IPB_SIG + i % 6) * S + gi0 + i / 6];
+    if (p.do_ph)
+        for (int i = tid; i < NIP; i += NT) {
+            sm.phA[i] = p.ipb[(int64_t)IPB_PHA * S + gi0 + i];
+            sm.phB[i] = p.ipb[(int64_t)IPB_PHB * S + gi0 + i];
+            sm.phFN[i] = p.ipb[(int64_t)IPB_PHFN * S + gi0 + i];
+            for (int c = 0; c < DIM; c++) sm.phFG[i][c] = p.ipb[(int64_t)(IPB_PHFG + c) * S + gi0 + i];
+            sm.ldrive[i] = p.ipb[(int64_t)IPB_LDRIVE * S + gi0 + i];
+            sm.gw[i] = p.ipb[(int64_t)IPB_GW * S + gi0 + i];
+            if (i == NIP - 1) sm.GcLast = p.ipb[(int64_t)IPB_GC * S + gi0 + i];
+        }
+    if (p.do_Tpf || p.do_th)
+        for (int i = tid; i < NIP; i += NT) {
+            sm.thB[i] = p.ipb[(int64_t)IPB_THB * S + gi0 + i];
+            sm.thFN[i] = p.ipb[(int64_t)IPB_THFN * S + gi0 + i];
+            for (int c = 0; c < DIM; c++) sm.thFG[i][c] = p.ipb[(int64_t)(IPB_THFG + c) * S + gi0 + i];
+        }
     __syncthreads();
 
     // ---- shape functions (CB, GB via the tensor-product extraction) ---------------
@@ -429,14 +823,14 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
         int ip = i / N, a = i % N;
         int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9;
         int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
-        double bx = sm.b1[0][qx][ax], by = sm.b1[1][qy][ay];
-        double gx = sm.db1[0][qx][ax], gy = sm.db1[1][qy][ay];
+        double bx = sm.shp.b1[0][qx][ax], by = sm.shp.b1[1][qy][ay];
+        double gx = sm.shp.db1[0][qx][ax], gy = sm.shp.db1[1][qy][ay];
         if (DIM == 2) {
             sm.Nsh[ip][a] = bx * by;
             sm.G[ip][0][a] = gx * by;
             sm.G[ip][1][a] = bx * gy;
         } else {
-            double bz = sm.b1[DIM - 1][qz][az], gz = sm.db1[DIM - 1][qz][az];
+            double bz = sm.shp.b1[DIM - 1][qz][az], gz = sm.shp.db1[DIM - 1][qz][az];
             sm.Nsh[ip][a] = bx * by * bz;
             sm.G[ip][0][a] = gx * by * bz;
             sm.G[ip][1][a] = bx * gy * bz;
@@ -445,29 +839,27 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
     }
     __syncthreads();
     if (DIM == 3) {
-        // wFun = CB.W, dW = GB^T W (NURBS3_Cube2.cpp:96-103)
         if (tid < NIP * 4) {
             int ip = tid / 4, q = tid % 4;
             double s = 0;
             if (q == 0) for (int a = 0; a < N; a++) s += sm.Nsh[ip][a] * sm.X[a][3];
             else for (int a = 0; a < N; a++) s += sm.G[ip][q - 1][a] * sm.X[a][3];
-            sm.red[ip][q] = s;
+            sm.shp.red[ip][q] = s;
         }
         __syncthreads();
         for (int i = tid; i < NIP * N; i += NT) {
             int ip = i / N, a = i % N;
-            double wF = sm.red[ip][0], Wa = sm.X[a][3], cb = sm.Nsh[ip][a];
+            double wF = sm.shp.red[ip][0], Wa = sm.X[a][3], cb = sm.Nsh[ip][a];
             sm.Nsh[ip][a] = Wa * cb / wF;
-            for (int c = 0; c < DIM; c++) sm.G[ip][c][a] = Wa * (sm.G[ip][c][a] / wF - cb * sm.red[ip][c + 1] / wF / wF);
+            for (int c = 0; c < DIM; c++) sm.G[ip][c][a] = Wa * (sm.G[ip][c][a] / wF - cb * sm.shp.red[ip][c + 1] / wF / wF);
         }
         __syncthreads();
     }
-    // Jacobian: 3D J(c,d) = sum_a GXi(a,c) X(a,d); 2D J0(c,d) = sum_a GB(c,a) X(a,d), J = J0^T
     if (tid < NIP * DIM * DIM) {
         int ip = tid / (DIM * DIM), c = (tid / DIM) % DIM, d = tid % DIM;
         double s = 0;
         for (int a = 0; a < N; a++) s += sm.G[ip][c][a] * sm.X[a][d];
-        sm.Jinv[ip][c][d] = s;  // raw J (3D) / J0 (2D) for now
+        sm.shp.Jinv[ip][c][d] = s;
     }
     __syncthreads();
     if (tid < NIP) {
@@ -475,7 +867,7 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
         if (DIM == 3) {
             double J[3][3];
             for (int c = 0; c < 3; c++)
-                for (int d = 0; d < 3; d++) J[c][d] = sm.Jinv[ip][c][d];
+                for (int d = 0; d < 3; d++) J[c][d] = sm.shp.Jinv[ip][c][d];
             double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
             double c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
             double c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
@@ -494,17 +886,17 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
             int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9;
             sm.w[ip] = p.wq[qx] * p.wq[qy] * p.wq[qz] * fabs(dt3);
             for (int c = 0; c < 3; c++)
-                for (int d = 0; d < 3; d++) sm.Jinv[ip][c][d] = Ji[c][d];
+                for (int d = 0; d < 3; d++) sm.shp.Jinv[ip][c][d] = Ji[c][d];
         } else {
-            double J[2][2] = {{sm.Jinv[ip][0][0], sm.Jinv[ip][1][0]}, {sm.Jinv[ip][0][1], sm.Jinv[ip][1][1]}};
+            double J[2][2] = {{sm.shp.Jinv[ip][0][0], sm.shp.Jinv[ip][1][0]}, {sm.shp.Jinv[ip][0][1], sm.shp.Jinv[ip][1][1]}};
             double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
             double id = 1.0 / det;
             int qx = ip % 3, qy = ip / 3;
             sm.w[ip] = p.wq[qx] * p.wq[qy] * fabs(det);
-            sm.Jinv[ip][0][0] = J[1][1] * id;
-            sm.Jinv[ip][1][0] = -J[1][0] * id;
-            sm.Jinv[ip][0][1] = -J[0][1] * id;
-            sm.Jinv[ip][1][1] = J[0][0] * id;
+            sm.shp.Jinv[ip][0][0] = J[1][1] * id;
+            sm.shp.Jinv[ip][1][0] = -J[1][0] * id;
+            sm.shp.Jinv[ip][0][1] = -J[0][1] * id;
+            sm.shp.Jinv[ip][1][1] = J[0][0] * id;
         }
     }
     __syncthreads();
@@ -514,205 +906,14 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
         for (int c = 0; c < DIM; c++) g[c] = sm.G[ip][c][a];
         for (int c = 0; c < DIM; c++) {
             double s = 0;
-            for (int d = 0; d < DIM; d++) s += sm.Jinv[ip][c][d] * g[d];
+            for (int d = 0; d < DIM; d++) s += sm.shp.Jinv[ip][c][d] * g[d];
             sm.G[ip][c][a] = s;
-        }
-    }
-    __syncthreads();
-
-    // ---- per-IP material update (one lane per IP) ---------------------------------
-    if (tid < NIP) {
-        const int ip = tid;
-        const int64_t gi = el * NIP + ip;
-        const double w = sm.w[ip];
-        double phi = 0, phio = 0, Tu = 0, Tph = 0;
-        for (int a = 0; a < N; a++) {
-            double Na = sm.Nsh[ip][a];
-            phi += Na * sm.Ph[a];
-            phio += Na * sm.PhO[a];
-        }
-        if (p.has_T) {
-            double T0 = p.sT.x[0];  // Assemble_Phase reads global T dof 0 for every node
-            for (int a = 0; a < N; a++) {
-                Tu += sm.Nsh[ip][a] * sm.T[a];
-                Tph += sm.Nsh[ip][a] * (mc.phase_T_from_dof0 ? T0 : sm.T[a]);
-            }
-        } else {
-            Tu = Tph = mc.T_default;
-        }
-        double eps[6], epso[6], eps_pl[6];
-        bool need_eps = p.do_u || p.do_ph;
-        if (p.has_pf && need_eps) {
-            double gu[DIM][DIM], guo[DIM][DIM];  // gu[alpha][c] = d u_alpha / d x_c
-            for (int a = 0; a < DIM; a++)
-                for (int c = 0; c < DIM; c++) {
-                    double s = 0, so = 0;
-                    for (int b = 0; b < N; b++) { s += sm.G[ip][c][b] * sm.U[a][b]; so += sm.G[ip][c][b] * sm.Uo[a][b]; }
-                    gu[a][c] = s; guo[a][c] = so;
-                }
-            for (int r = 0; r < 6; r++) { eps[r] = 0; epso[r] = 0; }
-            for (int a = 0; a < DIM; a++)
-                for (int c = 0; c < DIM; c++) { eps[vrow<DIM>(a, c)] += gu[a][c]; epso[vrow<DIM>(a, c)] += guo[a][c]; }
-            for (int r = 0; r < 6; r++) eps_pl[r] = p.PS[r * p.NIPT + gi];
-        }
-        MatIP m;
-        double VD = 0;
-        if (p.has_pf && p.do_u) {
-            set_temperature(mc, Tu, mc.porosity, m);
-            double Dam, dDam, ddDam;
-            damage(mc.DamFunction, phi, mc.k0, mc.p, Dam, dDam, ddDam);
-            double DP[6][6];
-            double epo[6];
-            for (int r = 0; r < 6; r++) epo[r] = p.PSOld[r * p.NIPT + gi];
-            double diss = 0, ddiss = 0;
-            int iters = 0, mx = 0;
-            bool called = false;
-            double hist0 = mc.HistSize > 0 ? p.MH[gi] : 0.0;
-            int rh = mc.Rheology;
-            auto ident = [&]() {
-                for (int i = 0; i < 6; i++)
-                    for (int j = 0; j < 6; j++) DP[i][j] = (i == j) ? 1.0 : 0.0;
-                for (int i = 0; i < 6; i++) eps_pl[i] = 1.0 * epo[i];
-            };
-            if (rh == IFE_RHEO_LE) {
-                ident();
-            } else if (rh == IFE_RHEO_VE || rh == IFE_RHEO_DVE) {
-                if (phi > mc.PhiViscLim) ident();
-                else {
-                    double dmg = 1.0, u1, u2;
-                    if (rh == IFE_RHEO_DVE) damage(mc.DamFunction, phi, mc.k0, mc.p, dmg, u1, u2);
-                    called = true;
-                    diss = return_map_viscous(m, mc.nCreep, p.dt, dmg, DP, eps_pl, eps, epo, hist0, iters, mx);
-                }
-            } else {  // VE_expl / dVE_expl
-                if (phio > mc.PhiViscLim) ident();
-                else {
-                    double dmg = 1.0, u1, u2;
-                    if (rh == IFE_RHEO_DVE_EXPL) damage(mc.DamFunction, phio, mc.k0, mc.p, dmg, u1, u2);
-                    called = true;
-                    diss = return_map_viscous(m, mc.nCreep, p.dt, dmg, DP, eps_pl, epso, epo, hist0, iters, mx);
-                    for (int i = 0; i < 6; i++)
-                        for (int j = 0; j < 6; j++) DP[i][j] = (i == j) ? 1.0 : 0.0;
-                }
-            }
-            if (called) {
-                atomicAdd(&sm.cnt_calls, 1u);
-                atomicAdd(&sm.cnt_iters, (unsigned)iters);
-                if (mx) atomicAdd(&sm.cnt_maxit, 1u);
-            }
-            if (mc.HistSize > 0) p.MH[gi] = hist0;
-            for (int r = 0; r < 6; r++) p.PS[r * p.NIPT + gi] = eps_pl[r];
-            VD = p.VDOld[gi] + diss * p.dt;
-            p.VD[gi] = VD;
-            p.DVD[gi] = p.DVDOld[gi] + ddiss * p.dt;
-            double ee[6];
-            for (int r = 0; r < 6; r++) ee[r] = eps[r] - eps_pl[r];
-            double DD[6][6], DU[6][6], SD[6], SU[6];
-            energy_split(mc, m, ee, DD, DU, SD, SU);
-            for (int r = 0; r < 6; r++) sm.sig[ip][r] = w * (Dam * SD[r] + SU[r]);
-            for (int i = 0; i < 6; i++)
-                for (int j = 0; j < 6; j++) {
-                    double s = 0;
-                    for (int k = 0; k < 6; k++) s += (Dam * DD[i][k] + DU[i][k]) * DP[k][j];
-                    sm.Mt[ip][i][j] = w * s;
-                }
-        } else if (p.has_pf && p.do_Tpf) {
-            VD = p.VD[gi];
-        }
-        if (p.has_pf && p.do_ph) {
-            set_temperature(mc, Tph, mc.porosity, m);
-            double Dam, dDam, ddDam, DamI, dDamI, ddDamI, DamG, dDamG, ddDamG;
-            damage(mc.DamFunction, phi, mc.k0, mc.p, Dam, dDam, ddDam);
-            damage(mc.DensityDamage, phi, mc.k0, mc.p, DamI, dDamI, ddDamI);
-            damage(mc.GravityDamage, phi, mc.k0, mc.p, DamG, dDamG, ddDamG);
-            double ee[6];
-            for (int r = 0; r < 6; r++) ee[r] = eps[r] - eps_pl[r];
-            double DD[6][6], DU[6][6], SD[6], SU[6];
-            double Energy = energy_split(mc, m, ee, DD, DU, SD, SU);
-            if (mc.Irreversible == IFE_IRR_HIST) {
-                double Ho = p.EOld[gi];
-                if (Ho > Energy) { p.E[gi] = Ho; Energy = Ho; }
-                else p.E[gi] = Energy;
-            } else {
-                p.E[gi] = Energy;
-            }
-            double gphi[DIM], gg = 0;
-            for (int c = 0; c < DIM; c++) {
-                double s = 0;
-                for (int a = 0; a < N; a++) s += sm.G[ip][c][a] * sm.Ph[a];
-                gphi[c] = s;
-                gg += s * s;
-            }
-            double dN, cNN, cGG, gamma;
-            dist_coeffs(mc.Distribution, mc.l, phi, gg, dN, cNN, cGG, gamma);
-            double capN = 0, capNN = 0;
-            if (mc.LumpedCap) {
-                double u1, u2;
-                cap_coeffs(mc.Distribution, mc.l, phi, capN, capNN, u1, u2);
-            }
-            double A = w * m.Gc * (cNN - capNN), FN = w * m.Gc * (dN - capN);
-            if (!mc.LumpedDrive) {
-                FN += w * dDam * Energy;
-                A += w * ddDam * Energy;
-                sm.ldrive[ip] = 0;
-            } else {
-                sm.ldrive[ip] = w * Energy;
-            }
-            if (mc.ViscoDrivingFactor > 1.0e-9) {
-                double DamV, dDamV, ddDamV;
-                double PD = mc.ViscoDrivingFactor * p.DVD[gi];
-                damage(mc.ViscoDriving, phi, mc.k0, mc.p, DamV, dDamV, ddDamV);
-                FN += w * dDamV * PD;
-                A += w * ddDamV * PD;
-            }
-            if (mc.Inertia && mc.ConsistentDegradation) {
-                double v2 = 0;
-                for (int d = 0; d < DIM; d++) {
-                    double vd = 0;
-                    for (int a = 0; a < N; a++) vd += sm.Nsh[ip][a] * sm.dU[d][a];
-                    v2 += vd * vd;
-                }
-                FN += -w * 0.5 * mc.Density * v2 * dDamI;
-                A += -w * 0.5 * mc.Density * v2 * ddDamI;
-            }
-            if (mc.ConsistentDegradation) {
-                double gu = 0;
-                for (int d = 0; d < DIM; d++) {
-                    double ud = 0;
-                    for (int a = 0; a < N; a++) ud += sm.Nsh[ip][a] * sm.U[d][a];
-                    gu += mc.g[d] * ud;
-                }
-                FN += -w * mc.Density * gu * dDamG;
-                A += -w * mc.Density * gu * ddDamG;
-            }
-            sm.phA[ip] = A;
-            sm.phB[ip] = w * m.Gc * cGG;
-            sm.phFN[ip] = FN;
-            for (int c = 0; c < DIM; c++) sm.phFG[ip][c] = w * m.Gc * cGG * gphi[c];
-            if (ip == NIP - 1) sm.GcLast = m.Gc;
-            sm.red[ip][0] = gamma * w;
-        }
-        if (p.do_Tpf || p.do_th) {
-            double fn = 0;
-            if (p.has_pf && p.do_Tpf) fn = -w * ((VD - p.VDOld[gi]) / p.dt);
-            sm.thFN[ip] = fn;
-            if (p.do_th) {
-                sm.thB[ip] = w * mc.k;
-                for (int c = 0; c < DIM; c++) {
-                    double s = 0;
-                    for (int a = 0; a < N; a++) s += sm.G[ip][c][a] * sm.T[a];
-                    sm.thFG[ip][c] = w * mc.k * s;
-                }
-            } else {
-                sm.thB[ip] = 0;
-                for (int c = 0; c < DIM; c++) sm.thFG[ip][c] = 0;
-            }
         }
     }
     __syncthreads();
     if (p.has_pf && p.do_ph && tid == 0) {
         double s = 0;
-        for (int ip = 0; ip < NIP; ip++) s += sm.red[ip][0];
+        for (int ip = 0; ip < NIP; ip++) s += sm.gw[ip];
         p.lcrack_elem[el] = s;
     }
     // lumped weights (the same for the u, phase and T groups: one element group)
@@ -726,7 +927,7 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
             damage(mc.DensityDamage, sm.Ph[tid], mc.k0, mc.p, DamI, d1, d2);
             if (mc.Inertia) kd += s * DamI * mc.Density * p.ddu_dt;
             if (mc.Damping > 0.0) kd += s * mc.Damping * p.du_dt;
-            for (int a = 0; a < DIM; a++) sm.kdu[a][tid] = kd;
+            sm.kdu[tid] = kd;
         }
     }
     __syncthreads();
@@ -761,22 +962,26 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
         const int al = pair / DIM, be = pair % DIM;
         const int I0 = (q / (N / TJ)) * TI, J0 = (q % (N / TJ)) * TJ;
         for (int ip0 = 0; ip0 < NIP; ip0 += CH) {
+            for (int i = tid; i < CH * 36; i += NT)
+                (&sm.ku.Mt[0][0][0])[i] = p.ipb[(int64_t)(IPB_M + i % 36) * S + gi0 + ip0 + i / 36];
+            __syncthreads();
             for (int i = tid; i < CH * 6 * NU; i += NT) {
                 int qq = i / (6 * NU), rem = i % (6 * NU), r = rem / NU, col = rem % NU;
                 int b2 = col / N, j = col % N, ip = ip0 + qq;
                 double s = 0;
-                for (int d = 0; d < DIM; d++) s += sm.Mt[ip][r][vrow<DIM>(b2, d)] * sm.G[ip][d][j];
-                sm.Y[qq][r][col] = s;
+                for (int d = 0; d < DIM; d++) s += sm.ku.Mt[qq][r][vrow<DIM>(b2, d)] * sm.G[ip][d][j];
+                sm.ku.Y[qq][r][col] = s;
             }
             __syncthreads();
             if (tid < E::NTK) {
+#pragma unroll 1
                 for (int qq = 0; qq < CH; qq++) {
                     int ip = ip0 + qq;
                     for (int c = 0; c < DIM; c++) {
                         int r = vrow<DIM>(al, c);
                         double av[TI], bv[TJ];
                         for (int i = 0; i < TI; i++) av[i] = sm.G[ip][c][I0 + i];
-                        for (int j = 0; j < TJ; j++) bv[j] = sm.Y[qq][r][be * N + J0 + j];
+                        for (int j = 0; j < TJ; j++) bv[j] = sm.ku.Y[qq][r][be * N + J0 + j];
                         for (int i = 0; i < TI; i++)
                             for (int j = 0; j < TJ; j++) acc[i][j] += av[i] * bv[j];
                     }
@@ -793,7 +998,7 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
                 for (int j = 0; j < TJ; j++) {
                     int b = J0 + j;
                     double v = acc[i][j];
-                    if (al == be && a == b) v += sm.kdu[al][a];
+                    if (al == be && a == b) v += sm.kdu[a];
                     bad |= !isfinite(v);
                     p.su.K[base + sm.loc[a][b]] += v;
                 }
@@ -831,6 +1036,7 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
         for (int i = tid; i < N * N; i += NT) {
             int a = i / N, b = i % N;
             double s = 0;
+#pragma unroll 3
             for (int ip = 0; ip < NIP; ip++) {
                 double gg = 0;
                 for (int c = 0; c < DIM; c++) gg += sm.G[ip][c][a] * sm.G[ip][c][b];
@@ -875,6 +1081,7 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
             for (int i = tid; i < N * N; i += NT) {
                 int a = i / N, b = i % N;
                 double s = 0;
+#pragma unroll 3
                 for (int ip = 0; ip < NIP; ip++) {
                     double gg = 0;
                     for (int c = 0; c < DIM; c++) gg += sm.G[ip][c][a] * sm.G[ip][c][b];
@@ -891,11 +1098,6 @@ __global__ __launch_bounds__(NT) void assemble_kernel(KParams p) {
     __syncthreads();
     if (tid == 0) {
         atomicAdd(&p.st->elements, 1ull);
-        if (sm.cnt_calls) {
-            atomicAdd(&p.st->rm_calls, (unsigned long long)sm.cnt_calls);
-            atomicAdd(&p.st->rm_iters, (unsigned long long)sm.cnt_iters);
-        }
-        if (sm.cnt_maxit) atomicAdd(&p.st->rm_maxit, (unsigned long long)sm.cnt_maxit);
         if (sm.nonfinite) atomicOr(&p.st->nonfinite, 1u);
     }
 }
@@ -925,6 +1127,15 @@ hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
     dim3 grid((unsigned)p.nlist), block(NT);
     if (dim == 2) hipLaunchKernelGGL(assemble_kernel<2>, grid, block, 0, s, p);
     else hipLaunchKernelGGL(assemble_kernel<3>, grid, block, 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
+    const int nip = dim == 2 ? 9 : 27, epb = NT / nip;
+    int64_t blocks = (p.NE + epb - 1) / epb;
+    if (blocks <= 0) return hipSuccess;
+    if (dim == 2) hipLaunchKernelGGL(material_kernel<2>, dim3((unsigned)blocks), dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL(material_kernel<3>, dim3((unsigned)blocks), dim3(NT), 0, s, p);
     return hipGetLastError();
 }
 

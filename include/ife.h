@@ -175,6 +175,8 @@ typedef struct {
     double kernel_ms_total;    /* sum of their durations */
     double kernel_ms_max;
     int64_t elements;          /* elements those launches processed */
+    int64_t material_launches; /* material-point (return map) kernel launches */
+    double material_ms_total;
 } ife_timings;
 
 typedef struct ife_physics ife_physics;
