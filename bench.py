@@ -30,9 +30,16 @@ PEAK_HBM_GBS = 8000.0
 
 
 def flops_per_element(kbar):
-    # structural count per 3D quadratic hex: shape functions, K_UU, D_eff, f_U/strains/
-    # energy split, phase step, thermal = 1.83 MFLOP; return map 27 x (2.4k*kbar + 1.2k)
+    """Algorithmic FLOP per 3D quadratic hex for one K+f assembly of all steps:
+    shape functions 200.8k, K_UU structural 1181.0k, D_eff 13.6k, f_U/strains/energy split
+    44k, phase step 227k, thermal 127k (1.83M in total) plus the return map,
+    27 x (2.4k x mean iterations + 1.2k)."""
     return 1.83e6 + 65.0e3 * kbar
+
+
+# Of those, the element kernel performs the tangent/residual contractions: K_UU 1181.0k,
+# phase-step contractions 227k, thermal 127k and the f_U contraction (~20k of the 44k).
+ELEMENT_KERNEL_FLOP_3D = 1181.0e3 + 227.0e3 + 127.0e3 + 20.0e3
 
 
 def cpu_baseline(workload, sample_nel, threads):
@@ -43,6 +50,19 @@ def cpu_baseline(workload, sample_nel, threads):
     E = prob.mesh.n_elems
     t = o.time_assembly(list(range(o.nsteps)), 0, E, threads)
     return E / t, E, t
+
+
+def traffic_from_profile(precision):
+    """HBM bytes per element-kernel launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/), or None if there is none."""
+    path = os.path.join(ROOT, "profiles", f"assemble_kernel_hex3d_glen_at2_{precision}.md")
+    try:
+        for line in open(path):
+            if line.startswith("HBM bytes per launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -106,7 +126,8 @@ def main():
     avg_launch_ms = tim["kernel_ms_total"] / max(tim["launches"], 1)
     elems_per_launch = tim["elements"] / max(tim["launches"], 1)
     fpe = flops_per_element(kbar)
-    achieved_tflops = fpe * elems_per_launch / (avg_launch_ms / 1e3) / 1e12
+    achieved_tflops = ELEMENT_KERNEL_FLOP_3D * elems_per_launch / (avg_launch_ms / 1e3) / 1e12
+    step_tflops = fpe * E / (ms_per_step / 1e3) / 1e12
     peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
 
     out = {
@@ -126,10 +147,15 @@ def main():
                    "dofs_per_node": "u(3)+phi+T", "precision": args.precision,
                    "step": "ResetStep + fused K+f assembly of the u, phase and T steps"},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / peak, "traffic": None,
-                     "kernel": "assemble_kernel<3>", "avg_launch_ms": avg_launch_ms,
+                     "frac": achieved_tflops / peak, "traffic": traffic_from_profile(args.precision),
+                     "kernel": "assemble_kernel<3> (element contractions)", "avg_launch_ms": avg_launch_ms,
                      "launches_per_step": tim["launches"] / args.steps,
-                     "flop_per_element": fpe, "mean_return_map_iterations": kbar},
+                     "flop_per_element": ELEMENT_KERNEL_FLOP_3D},
+        "kernels_ms_per_step": {"material_kernel": tim["material_ms_total"] / args.steps,
+                                "assemble_kernel": tim["kernel_ms_total"] / args.steps,
+                                "gather_kernel": tim["gather_ms_total"] / args.steps},
+        "whole_step": {"flop_per_element": fpe, "achieved_tflops": step_tflops,
+                       "mean_return_map_iterations": kbar},
         "wall_s": wall,
     }
     if not args.no_cpu_baseline:

@@ -70,7 +70,8 @@ class AssemblyStatus(C.Structure):
 
 class Timings(C.Structure):
     _fields_ = [("launches", C.c_int64), ("kernel_ms_total", C.c_double), ("kernel_ms_max", C.c_double),
-                ("elements", C.c_int64), ("material_launches", C.c_int64), ("material_ms_total", C.c_double)]
+                ("elements", C.c_int64), ("material_launches", C.c_int64), ("material_ms_total", C.c_double),
+                ("gather_ms_total", C.c_double)]
 
 
 # exported symbols and their prototypes: (name, restype, argtypes)

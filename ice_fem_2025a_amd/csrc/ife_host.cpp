@@ -149,10 +149,16 @@ struct ife_physics {
     std::vector<uint8_t> loc;       // [e][n][n]
     std::vector<int64_t> color_ptr;
     std::vector<int32_t> color_elems;
+    std::vector<int64_t> n2e_ptr;   // node (rank) -> adjacent elements
+    std::vector<int32_t> n2e_elem;
+    std::vector<uint8_t> n2e_loc;
     std::vector<Step> steps;
     std::vector<int> fpos;          // field -> position within its step (-1 unused)
     // device
-    DevBuf<double> d_nodes, d_ext, d_lcrack, d_ipb;
+    DevBuf<double> d_nodes, d_ext, d_lcrack, d_ipb, d_ebuf;
+    DevBuf<int64_t> d_n2e_ptr;
+    DevBuf<int32_t> d_n2e_elem;
+    DevBuf<uint8_t> d_n2e_loc;
     DevBuf<int32_t> d_enode, d_erank, d_colors;
     DevBuf<uint8_t> d_loc;
     DevBuf<int64_t> d_nbr_ptr;
@@ -164,13 +170,14 @@ struct ife_physics {
     // launch timing
     bool profiling = false;
     struct Rec { hipEvent_t a, b; int64_t elems; };
-    std::vector<Rec> recs, mat_recs;
+    std::vector<Rec> recs, mat_recs, gat_recs;
     std::vector<hipEvent_t> event_pool;
 
     ~ife_physics() {
         for (auto* m : models) delete m;
         for (auto& r : recs) { event_pool.push_back(r.a); event_pool.push_back(r.b); }
         for (auto& r : mat_recs) { event_pool.push_back(r.a); event_pool.push_back(r.b); }
+        for (auto& r : gat_recs) { event_pool.push_back(r.a); event_pool.push_back(r.b); }
         for (auto e : event_pool) (void)hipEventDestroy(e);
     }
     hipEvent_t get_event() {
@@ -310,11 +317,20 @@ void plan(ife_physics* ph) {
     for (int64_t i = 0; i < ph->NE * n; i++) n2e_ptr[ph->erank[i] + 1]++;
     for (int64_t v = 0; v < Ng; v++) n2e_ptr[v + 1] += n2e_ptr[v];
     std::vector<int64_t> n2e(n2e_ptr[Ng]);
+    ph->n2e_elem.resize(n2e_ptr[Ng]);
+    ph->n2e_loc.resize(n2e_ptr[Ng]);
     {
+        // elements in ascending index per node: the order the gather adds them in
         std::vector<int64_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
         for (int64_t e = 0; e < ph->NE; e++)
-            for (int a = 0; a < n; a++) n2e[fill[ph->erank[e * n + a]]++] = e;
+            for (int a = 0; a < n; a++) {
+                int64_t k = fill[ph->erank[e * n + a]]++;
+                n2e[k] = e;
+                ph->n2e_elem[k] = (int32_t)e;
+                ph->n2e_loc[k] = (uint8_t)a;
+            }
     }
+    ph->n2e_ptr = n2e_ptr;
     std::vector<std::vector<int32_t>> lists(Ng);
     parallel_for(Ng, [&](int64_t v0, int64_t v1) {
         std::vector<int64_t> mark(Ng, -1);
@@ -338,7 +354,7 @@ void plan(ife_physics* ph) {
     lists.shrink_to_fit();
     int64_t maxdeg = 0;
     for (int64_t v = 0; v < Ng; v++) maxdeg = std::max(maxdeg, ph->nbr_ptr[v + 1] - ph->nbr_ptr[v]);
-    if (maxdeg > 255) THROW(IFE_ERR_UNSUPPORTED, "node degree above 255 (8-bit scatter slots)");
+    if (maxdeg > 128) THROW(IFE_ERR_UNSUPPORTED, "node degree above 128 (gather rows are staged in LDS)");
     // ---- per-step CSR: row (fp, v) = [g*Ng + nbr(v) for g in fields] --------------
     const int64_t nnz_node = ph->nbr_ptr[Ng];
     for (int s = 0; s < ph->nsteps; s++) {
@@ -422,6 +438,10 @@ void setup_device(ife_physics* ph) {
     ph->d_status.alloc(1);
     HIPCHK(hipMemset(ph->d_status.p, 0, sizeof(Status)));
     ph->d_ipb.alloc((size_t)IPB_NF * ph->NE * ph->nip);
+    ph->d_ebuf.alloc((size_t)ph->NE * ph->n * record_size(ph->dim));
+    ph->d_n2e_ptr.upload(ph->n2e_ptr.data(), ph->n2e_ptr.size());
+    ph->d_n2e_elem.upload(ph->n2e_elem.data(), ph->n2e_elem.size());
+    ph->d_n2e_loc.upload(ph->n2e_loc.data(), ph->n2e_loc.size());
     ph->d_lcrack.alloc(ph->NE);
     HIPCHK(hipMemset(ph->d_lcrack.p, 0, ph->NE * sizeof(double)));
     for (auto& st : ph->steps) {
@@ -524,6 +544,10 @@ KParams base_params(ife_physics* ph) {
     p.st = ph->d_status.p;
     p.lcrack_elem = ph->d_lcrack.p;
     p.ipb = ph->d_ipb.p;
+    p.ebuf = ph->d_ebuf.p;
+    p.n2e_ptr = ph->d_n2e_ptr.p;
+    p.n2e_elem = ph->d_n2e_elem.p;
+    p.n2e_loc = ph->d_n2e_loc.p;
     p.mixed = ph->precision == IFE_PREC_MIXED;
     return p;
 }
@@ -545,16 +569,14 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
         int st = which[i];
         double* K = Kv ? Kv[i] : nullptr;
         double* f = fv ? fv[i] : nullptr;
-        Step& S = ph->steps[st];
-        if (!K) K = S.K.p;
-        if (!f) f = S.f.p;
+        if (!K) K = ph->steps[st].K.p;
+        if (!f) f = ph->steps[st].f.p;
         if (pfm && pfm->Step_u == st) { p.su.K = K; p.su.f = f; }
         if (pfm && pfm->Step_phase == st) { p.sph.K = K; p.sph.f = f; }
         if (p.has_T && ph->fstep[thm ? thm->field_T : pfm->field_T] == st) { p.sT.K = K; p.sT.f = f; }
-        // Physics::Assemble: VecZeroEntries(f), MatZeroEntries(K) (physics.cpp:260-262)
-        HIPCHK(launch_fill(K, (int64_t)S.col.size(), 0.0, s));
-        HIPCHK(launch_fill(f, S.nrows, 0.0, s));
     }
+    // Physics::Assemble zeroes f and K and the models add into them (physics.cpp:260-284);
+    // here the gather kernel writes every entry of every node row exactly once instead.
     p.do_u = pfm && in(pfm->Step_u);
     p.do_ph = pfm && in(pfm->Step_phase);
     p.do_Tpf = pfm && pfm->HasT && pfm->opt.ViscousHeating && in(pfm->Step_T);
@@ -562,30 +584,24 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
     ph->last_had_phase = p.do_ph;
     HIPCHK(launch_zero_status(ph->d_status.p, s));
     if (!(p.do_u || p.do_ph || p.do_Tpf || p.do_th)) return;
-    // material update over all IPs, then the coloured element launches
-    if (ph->profiling) {
-        ife_physics::Rec r{ph->get_event(), ph->get_event(), 0};
-        HIPCHK(hipEventRecord(r.a, s));
-        HIPCHK(launch_material(p, ph->dim, s));
-        HIPCHK(hipEventRecord(r.b, s));
-        ph->mat_recs.push_back(r);
-    } else {
-        HIPCHK(launch_material(p, ph->dim, s));
-    }
-    const int ncol = (int)ph->color_ptr.size() - 1;
-    for (int c = 0; c < ncol; c++) {
-        p.elist = ph->d_colors.p + ph->color_ptr[c];
-        p.nlist = ph->color_ptr[c + 1] - ph->color_ptr[c];
-        if (ph->profiling && p.nlist > 0) {
-            ife_physics::Rec r{ph->get_event(), ph->get_event(), p.nlist};
+    // material update over all IPs, element records for all elements in one launch, then
+    // the per-node gather into the CSR rows
+    p.elist = nullptr;
+    p.nlist = ph->NE;
+    auto timed = [&](std::vector<ife_physics::Rec>& into, int64_t elems, const std::function<hipError_t()>& launch) {
+        if (ph->profiling) {
+            ife_physics::Rec r{ph->get_event(), ph->get_event(), elems};
             HIPCHK(hipEventRecord(r.a, s));
-            HIPCHK(launch_assemble(p, ph->dim, s));
+            HIPCHK(launch());
             HIPCHK(hipEventRecord(r.b, s));
-            ph->recs.push_back(r);
+            into.push_back(r);
         } else {
-            HIPCHK(launch_assemble(p, ph->dim, s));
+            HIPCHK(launch());
         }
-    }
+    };
+    timed(ph->mat_recs, 0, [&] { return launch_material(p, ph->dim, s); });
+    timed(ph->recs, ph->NE, [&] { return launch_assemble(p, ph->dim, s); });
+    timed(ph->gat_recs, 0, [&] { return launch_gather(p, ph->dim, s); });
 }
 
 DevBuf<double>* history_buf(Model* m, const std::string& nm, int64_t& ncomp) {
@@ -883,6 +899,16 @@ int ife_physics_timings(ife_physics* ph, ife_timings* out) {
             ph->event_pool.push_back(r.b);
         }
         ph->mat_recs.clear();
+        out->gather_ms_total = 0;
+        for (auto& r : ph->gat_recs) {
+            HIPCHK(hipEventSynchronize(r.b));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            out->gather_ms_total += ms;
+            ph->event_pool.push_back(r.a);
+            ph->event_pool.push_back(r.b);
+        }
+        ph->gat_recs.clear();
     });
 }
 

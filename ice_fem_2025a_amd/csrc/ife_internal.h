@@ -97,12 +97,18 @@ struct KParams {
     Status* st;
     double* lcrack_elem;     // [e]
     double* ipb;             // per-IP coefficients [IPB_NF][NIPT]
+    double* ebuf;            // element records [e][a][record] (rows of node a of the element)
+    const int64_t* n2e_ptr;  // node (group rank) -> adjacent elements, ascending
+    const int32_t* n2e_elem;
+    const uint8_t* n2e_loc;  // local index of the node in that element
     int mixed;               // tangent contractions in FP32
 };
 
 // Launchers (ife_kernels.hip)
 hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_material(const KParams& p, int dim, hipStream_t s);
+hipError_t launch_gather(const KParams& p, int dim, hipStream_t s);
+int record_size(int dim);
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 hipError_t launch_zero_status(Status* st, hipStream_t s);

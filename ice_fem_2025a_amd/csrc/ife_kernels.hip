@@ -45,6 +45,14 @@ struct ET {
     static constexpr int CH = 3;                     // IPs per Y chunk
 };
 
+// Element-buffer record of one (element, local node a): the rows of node a of the element
+// tangents and residuals, [ K_UU(alpha,beta,b) | F_U(alpha) | K_pp(b) | F_p | K_TT(b) | F_T ]
+template <int DIM>
+struct Rec {
+    static constexpr int N = DIM == 2 ? 9 : 27;
+    static constexpr int KU = 0, FU = DIM * DIM * N, KP = FU + DIM, FP = KP + N, KT = FP + 1, FT = KT + N, SIZE = FT + 1;
+};
+
 // Voigt row (xx,yy,zz,xy,yz,xz) holding d u_alpha / d x_c (ConstructBMat, utility/math.cpp:3-26)
 template <int DIM>
 __device__ __forceinline__ int vrow(int a, int c) {
@@ -748,10 +756,7 @@ struct Smem {
     double kdu[N];             // nodal diagonal additions to K_UU
     double WL[N];
     double GcLast;
-    int64_t rowst[N];          // nbr_ptr[rank]
-    int deg[N];
     int rank[N];
-    uint8_t loc[N][N];
     unsigned int nonfinite;
 };
 
@@ -761,7 +766,9 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
     constexpr int N = E::N, NIP = E::NIP, NU = E::NU;
     __shared__ Smem<DIM> sm;
     const int tid = threadIdx.x;
-    const int64_t el = p.elist[blockIdx.x];
+    const int64_t el = p.elist ? (int64_t)p.elist[blockIdx.x] : (int64_t)blockIdx.x;
+    using RC = Rec<DIM>;
+    double* const rec = p.ebuf + el * N * RC::SIZE;  // this element's records
     const MatConst& mc = p.mc;
     const int64_t S = p.NIPT, gi0 = el * NIP;
 
@@ -772,9 +779,6 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
         sm.rank[tid] = r;
         const double* nd = p.nodes + (int64_t)v * 4;
         sm.X[tid][0] = nd[0]; sm.X[tid][1] = nd[1]; sm.X[tid][2] = nd[2]; sm.X[tid][3] = nd[3];
-        int64_t r0 = p.nbr_ptr[r], r1 = p.nbr_ptr[r + 1];
-        sm.rowst[tid] = r0;
-        sm.deg[tid] = (int)(r1 - r0);
         if (p.has_pf) {
             for (int a = 0; a < DIM; a++) {
                 int64_t d = (int64_t)p.fpos_u[a] * p.Ng + r;
@@ -788,7 +792,6 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
         if (p.has_T) sm.dT[tid] = p.sT.dx[(int64_t)p.fpos_T * p.Ng + r];
     }
     if (tid == 0) sm.nonfinite = 0;
-    for (int i = tid; i < N * N; i += NT) (&sm.loc[0][0])[i] = p.loc[el * N * N + i];
     if (tid < DIM * 9) {
         int d = tid / 9, q = (tid / 3) % 3, a = tid % 3;
         const double* C = p.ext + (el * DIM + d) * 9;
@@ -951,7 +954,7 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
             damage(mc.GravityDamage, sm.Ph[a], mc.k0, mc.p, DamG, d1, d2);
             s += -WLa * DamG * mc.Density * mc.g[al];
             if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-            p.su.f[(int64_t)p.fpos_u[al] * p.Ng + sm.rank[a]] += s;
+            rec[a * RC::SIZE + RC::FU + al] = s;
         }
         // K_UU = sum_ip B^T Mt B via Y = Mt B in IP chunks
         constexpr int TI = E::TI, TJ = E::TJ, TPP = E::TPP, CH = E::CH;
@@ -993,14 +996,13 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
             unsigned bad = 0;
             for (int i = 0; i < TI; i++) {
                 int a = I0 + i;
-                int64_t base = (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * sm.rowst[a] +
-                               (int64_t)p.fpos_u[be] * sm.deg[a];
+                double* out = rec + a * RC::SIZE + RC::KU + (al * DIM + be) * N + J0;
                 for (int j = 0; j < TJ; j++) {
                     int b = J0 + j;
                     double v = acc[i][j];
                     if (al == be && a == b) v += sm.kdu[a];
                     bad |= !isfinite(v);
-                    p.su.K[base + sm.loc[a][b]] += v;
+                    out[j] = v;
                 }
             }
             if (bad) atomicOr(&sm.nonfinite, 1u);
@@ -1031,7 +1033,7 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
             }
             s += WLa * mc.pf_visc / p.dt * (sm.Ph[a] - sm.PhO[a]);
             if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-            p.sph.f[(int64_t)p.fpos_ph * p.Ng + sm.rank[a]] += s;
+            rec[a * RC::SIZE + RC::FP] = s;
         }
         for (int i = tid; i < N * N; i += NT) {
             int a = i / N, b = i % N;
@@ -1059,9 +1061,7 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
                 s += WLa * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
             }
             if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-            int64_t pos = (int64_t)p.fpos_ph * p.sph.M + (int64_t)p.sph.nf * sm.rowst[a] +
-                          (int64_t)p.fpos_ph * sm.deg[a] + sm.loc[a][b];
-            p.sph.K[pos] += s;
+            rec[a * RC::SIZE + RC::KP + b] = s;
         }
     }
 
@@ -1075,7 +1075,7 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
             }
             if (p.do_th) s += sm.WL[a] * mc.Density_th * mc.cp * sm.dT[a];
             if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-            p.sT.f[(int64_t)p.fpos_Tth * p.Ng + sm.rank[a]] += s;
+            rec[a * RC::SIZE + RC::FT] = s;
         }
         if (p.do_th) {
             for (int i = tid; i < N * N; i += NT) {
@@ -1089,9 +1089,7 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
                 }
                 if (a == b) s += sm.WL[a] * mc.Density_th * mc.cp * p.du_dt;
                 if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-                int64_t pos = (int64_t)p.fpos_Tth * p.sT.M + (int64_t)p.sT.nf * sm.rowst[a] +
-                              (int64_t)p.fpos_Tth * sm.deg[a] + sm.loc[a][b];
-                p.sT.K[pos] += s;
+                rec[a * RC::SIZE + RC::KT + b] = s;
             }
         }
     }
@@ -1099,6 +1097,81 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
     if (tid == 0) {
         atomicAdd(&p.st->elements, 1ull);
         if (sm.nonfinite) atomicOr(&p.st->nonfinite, 1u);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Gather kernel: one wavefront per node (group rank r).  Adds the records of the node's
+// adjacent elements, in increasing element index (the order in which the reference's
+// element loop adds them through MatSetValues/VecSetValues), into LDS copies of the
+// node's CSR rows, then writes each row once, contiguously.  Every CSR entry of the
+// pattern and every residual entry belongs to exactly one node, so the rows are
+// overwritten rather than zeroed and accumulated (Physics::Assemble's zero + add).
+// ---------------------------------------------------------------------------------------
+constexpr int GW = 64;      // lanes per gather workgroup
+constexpr int MAXDEG = 128; // neighbour slots per row held in LDS
+
+template <int DIM>
+__global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
+    using RC = Rec<DIM>;
+    constexpr int N = RC::N;
+    __shared__ double ku[DIM][DIM][MAXDEG];
+    __shared__ double kp[MAXDEG], kt[MAXDEG];
+    __shared__ double fu[DIM], fp[1], ft[1];
+    const int lane = threadIdx.x;
+    const int64_t r = blockIdx.x;
+    const int64_t r0 = p.nbr_ptr[r];
+    const int deg = (int)(p.nbr_ptr[r + 1] - r0);
+    const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
+    for (int i = lane; i < DIM * DIM * MAXDEG; i += GW) (&ku[0][0][0])[i] = 0.0;
+    for (int i = lane; i < MAXDEG; i += GW) { kp[i] = 0.0; kt[i] = 0.0; }
+    if (lane < DIM) fu[lane] = 0.0;
+    if (lane == 0) { fp[0] = 0.0; ft[0] = 0.0; }
+    __syncthreads();
+    for (int64_t k = p.n2e_ptr[r]; k < p.n2e_ptr[r + 1]; k++) {
+        const int64_t e = p.n2e_elem[k];
+        const int a = p.n2e_loc[k];
+        const double* rec = p.ebuf + (e * N + a) * RC::SIZE;
+        const uint8_t* lc = p.loc + (e * N + a) * N;
+        for (int t = lane; t < RC::SIZE; t += GW) {
+            if (t < RC::FU) {
+                if (du) {
+                    int ab = t / N, b = t % N;
+                    (&ku[0][0][0])[ab * MAXDEG + lc[b]] += rec[t];
+                }
+            } else if (t < RC::KP) {
+                if (du) fu[t - RC::FU] += rec[t];
+            } else if (t < RC::FP) {
+                if (dph) kp[lc[t - RC::KP]] += rec[t];
+            } else if (t == RC::FP) {
+                if (dph) fp[0] += rec[t];
+            } else if (t < RC::FT) {
+                if (p.do_th) kt[lc[t - RC::KT]] += rec[t];
+            } else {
+                if (dT) ft[0] += rec[t];
+            }
+        }
+        __syncthreads();
+    }
+    if (du) {
+        for (int al = 0; al < DIM; al++) {
+            double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * r0;
+            for (int be = 0; be < DIM; be++)
+                for (int i = lane; i < deg; i += GW) K[(int64_t)p.fpos_u[be] * deg + i] = ku[al][be][i];
+        }
+        if (lane < DIM) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = fu[lane];
+    }
+    if (dph) {
+        double* K = p.sph.K + (int64_t)p.fpos_ph * p.sph.M + (int64_t)p.sph.nf * r0 + (int64_t)p.fpos_ph * deg;
+        for (int i = lane; i < deg; i += GW) K[i] = kp[i];
+        if (lane == 0) p.sph.f[(int64_t)p.fpos_ph * p.Ng + r] = fp[0];
+    }
+    if (dT) {
+        if (p.do_th) {
+            double* K = p.sT.K + (int64_t)p.fpos_Tth * p.sT.M + (int64_t)p.sT.nf * r0 + (int64_t)p.fpos_Tth * deg;
+            for (int i = lane; i < deg; i += GW) K[i] = kt[i];
+        }
+        if (lane == 0) p.sT.f[(int64_t)p.fpos_Tth * p.Ng + r] = ft[0];
     }
 }
 
@@ -1129,6 +1202,15 @@ hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
     else hipLaunchKernelGGL(assemble_kernel<3>, grid, block, 0, s, p);
     return hipGetLastError();
 }
+
+hipError_t launch_gather(const KParams& p, int dim, hipStream_t s) {
+    if (p.Ng <= 0) return hipSuccess;
+    if (dim == 2) hipLaunchKernelGGL(gather_kernel<2>, dim3((unsigned)p.Ng), dim3(GW), 0, s, p);
+    else hipLaunchKernelGGL(gather_kernel<3>, dim3((unsigned)p.Ng), dim3(GW), 0, s, p);
+    return hipGetLastError();
+}
+
+int record_size(int dim) { return dim == 2 ? Rec<2>::SIZE : Rec<3>::SIZE; }
 
 hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
     const int nip = dim == 2 ? 9 : 27, epb = NT / nip;

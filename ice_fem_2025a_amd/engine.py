@@ -192,7 +192,7 @@ class Physics:
         abi.check(self.L.ife_physics_timings(self.h, C.byref(t)))
         return {"launches": t.launches, "kernel_ms_total": t.kernel_ms_total, "kernel_ms_max": t.kernel_ms_max,
                 "elements": t.elements, "material_launches": t.material_launches,
-                "material_ms_total": t.material_ms_total}
+                "material_ms_total": t.material_ms_total, "gather_ms_total": t.gather_ms_total}
 
     def commit(self, model, commit_type=1, stream=None):
         abi.check(self.L.ife_model_commit(self.h, model, commit_type, stream.handle if stream else None))

@@ -167,7 +167,7 @@ typedef struct {
     double  LCrack;                 /* sum gamma*w of the last phase assembly (PhaseField.cpp:650) */
 } ife_assembly_status;
 
-/* Kernel timing of assemble calls (HIP events around every element-kernel launch),
+/* Kernel timing of assemble calls (HIP events around every kernel launch),
  * the engine's counterpart of the per-model "Runtime: min .. max" report of
  * Physics::Assemble (physics.cpp:266-284). */
 typedef struct {
@@ -177,6 +177,7 @@ typedef struct {
     int64_t elements;          /* elements those launches processed */
     int64_t material_launches; /* material-point (return map) kernel launches */
     double material_ms_total;
+    double gather_ms_total;    /* per-node CSR gather kernel */
 } ife_timings;
 
 typedef struct ife_physics ife_physics;
