@@ -5,13 +5,14 @@
 //                    at the IP, strains, the Glen return map, energy split, history max and
 //                    the crack-density / heating terms; writes per-IP coefficients.  The
 //                    data-dependent Newton loop runs here, where many waves hide its latency.
-//   assemble_kernel  one workgroup (256 lanes) per element of one colour: geometry and
+//   assemble_kernel  one workgroup (256 lanes, 4 wavefronts) per element: geometry and
 //                    extraction gathered into LDS, shape functions built there once and shared
-//                    by every staggered step, register-tiled contractions over (IP, gradient
-//                    component), and the element contributions added straight into the CSR
-//                    values.  Elements of one colour share no node, so no two workgroups of a
-//                    launch touch the same CSR row and the read-modify-write needs no atomics
-//                    (deterministic, bit-reproducible order).
+//                    by every staggered step, the element tangents as 16x16 MFMA tiles
+//                    contracted over (IP, gradient component), and the element's node rows
+//                    written contiguously to an element-record buffer.
+//   gather_kernel    one wavefront per node: adds the records of the node's elements in
+//                    ascending element order and writes the node's CSR rows once (no atomics,
+//                    no colouring, deterministic).
 //
 // Reference semantics (paths relative to the IceFEM source tree):
 //   shape functions  mesh/ElementTypes/Bezier/B2_2.cpp:72-88 (2D, transposed-Jacobian form)
@@ -37,12 +38,6 @@ struct ET {
     static constexpr int N = DIM == 2 ? 9 : 27;  // nodes per element (quadratic)
     static constexpr int NIP = N;                // 3 Gauss points per direction
     static constexpr int NU = DIM * N;           // displacement dofs per element
-    // K_UU register tile per lane: TI rows (nodes) x TJ cols (nodes) of one (alpha,beta) block
-    static constexpr int TI = DIM == 2 ? 1 : 3;
-    static constexpr int TJ = DIM == 2 ? 3 : 9;
-    static constexpr int TPP = (N / TI) * (N / TJ);  // tiles per (alpha,beta) block
-    static constexpr int NTK = TPP * DIM * DIM;      // lanes busy on K_UU
-    static constexpr int CH = 3;                     // IPs per Y chunk
 };
 
 // Element-buffer record of one (element, local node a): the rows of node a of the element
@@ -725,43 +720,148 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Element kernel: one workgroup per element of one colour.  Shape functions in LDS,
-// per-IP coefficients from the material kernel, register-tiled contractions, and the
-// coloured read-modify-write into the CSR values and residuals.
+// Element kernel: one workgroup per element.  Shape functions in LDS, per-IP coefficients
+// from the material kernel, MFMA contractions, node-row records out.
 // ---------------------------------------------------------------------------------------
 template <int DIM>
 struct Smem {
-    static constexpr int N = ET<DIM>::N, NIP = ET<DIM>::NIP, NU = ET<DIM>::NU, CH = ET<DIM>::CH;
+    static constexpr int N = ET<DIM>::N, NIP = ET<DIM>::NIP, NU = ET<DIM>::NU;
     double X[N][4];            // x, y, z, W
     double Nsh[NIP][N];        // N (first CB)
     double G[NIP][DIM][N];     // G (first GB, then GXi)
     double w[NIP];
-    union {
-        struct {
-            double b1[DIM][3][3];      // 1D B-spline values  [dir][gauss pt][fn]
-            double db1[DIM][3][3];     // 1D derivatives
-            double red[NIP][4];        // wFun, dW
-            double Jinv[NIP][DIM][DIM];
-        } shp;
-        struct {
-            double Mt[CH][6][6];       // chunk of w*(Dam*D_Dam + D_UnDam)*DPlast
-            double Y[CH][6][NU];
-        } ku;
-    };
+    struct {
+        double b1[DIM][3][3];      // 1D B-spline values  [dir][gauss pt][fn]
+        double db1[DIM][3][3];     // 1D derivatives
+        double red[NIP][4];        // wFun, dW
+        double Jinv[NIP][DIM][DIM];
+    } shp;
+    double Mt[NIP][6][6];      // w*(Dam*D_Dam + D_UnDam)*DPlast per IP
     double dU[DIM][N], ddU[DIM][N];
     double Ph[N], PhO[N], dT[N];
     double sig[NIP][6];        // w*(Dam*S_Dam + S_UnDam)
     double phA[NIP], phB[NIP], phFN[NIP], phFG[NIP][DIM], ldrive[NIP], gw[NIP];
     double thB[NIP], thFN[NIP], thFG[NIP][DIM];
-    double kdu[N];             // nodal diagonal additions to K_UU
+    double kdu[N], kdp[N], kdt[N];  // nodal diagonal additions to K_UU, K_pp, K_TT
     double WL[N];
     double GcLast;
     int rank[N];
     unsigned int nonfinite;
 };
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// K_UU = sum_ip B^T Mt B on MFMA.  For each displacement component alpha the block row
+// K[(alpha,i)][(beta,j)] = sum_{k=(ip,c)} G[ip][c][i] * Y[k][(beta,j)] with
+// Y[(ip,c)][(beta,j)] = sum_d Mt[ip][vrow(alpha,c)][vrow(beta,d)] G[ip][d][j]: an
+// (N x DIM*NIP) x (DIM*NIP x DIM*N) product done as 16x16 tiles of
+// v_mfma_f64_16x16x4f64 (A[i][k]: lane%16, lane/16; B[k][j]: lane/16, lane%16; D[i][j]:
+// i = lane/16 + 4*reg, j = lane%16).  B fragments are formed on the fly from Mt and G.
 template <int DIM>
-__global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
+__device__ void kuu_mfma(const Smem<DIM>& sm, double* rec, int tid, unsigned& bad) {
+    using E = ET<DIM>;
+    using RC = Rec<DIM>;
+    constexpr int N = E::N, NIP = E::NIP, NU = E::NU;
+    constexpr int MT = (N + 15) / 16, NTL = (NU + 15) / 16, KS = (DIM * NIP + 3) / 4;
+    constexpr int TILES = DIM * NTL * MT, NW = NT / 64, TPW = (TILES + NW - 1) / NW;
+    const int lane = tid & 63, w = tid >> 6, lk = lane >> 4, l16 = lane & 15;
+    dbl4 acc[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; u++) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int ks = 0; ks < KS; ks++) {
+        const int k = ks * 4 + lk;
+        const bool kv = k < DIM * NIP;
+        const int ip = kv ? k / DIM : 0, c = kv ? k % DIM : 0;
+        double a[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) {
+            const int i = mt * 16 + l16;
+            a[mt] = (kv && i < N) ? sm.G[ip][c][i] : 0.0;
+        }
+        int lastB = -1;
+        double b = 0.0;
+#pragma unroll
+        for (int u = 0; u < TPW; u++) {
+            const int t = w * TPW + u;
+            if (t < TILES) {
+                const int mt = t % MT, an = t / MT, al = an / NTL, nt = an % NTL;
+                if (an != lastB) {
+                    lastB = an;
+                    const int n = nt * 16 + l16;
+                    b = 0.0;
+                    if (kv && n < NU) {
+                        const int be = n / N, j = n % N, r = vrow<DIM>(al, c);
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) b += sm.Mt[ip][r][vrow<DIM>(be, d)] * sm.G[ip][d][j];
+                    }
+                }
+                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b, acc[u], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TPW; u++) {
+        const int t = w * TPW + u;
+        if (t >= TILES) break;
+        const int mt = t % MT, an = t / MT, al = an / NTL, nt = an % NTL;
+        const int n = nt * 16 + l16;
+        if (n >= NU) continue;
+        const int be = n / N, j = n % N;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = mt * 16 + lk + 4 * v;
+            if (i < N) {
+                double val = acc[u][v];
+                if (al == be && i == j) val += sm.kdu[i];
+                bad |= !isfinite(val);
+                rec[i * RC::SIZE + RC::KU + (al * DIM + be) * N + j] = val;
+            }
+        }
+    }
+}
+
+// K_pp = sum_ip phA N N^T + phB G^T G and K_TT = sum_ip thB G^T G as one shared-A
+// product over k = (ip, q), q = 0 -> N, q = 1.. -> d/dx_{q-1}, on MFMA.
+template <int DIM>
+__device__ void kpt_mfma(const Smem<DIM>& sm, double* rec, int tid, bool dph, bool dth, unsigned& bad) {
+    using E = ET<DIM>;
+    using RC = Rec<DIM>;
+    constexpr int N = E::N, NIP = E::NIP, Q = DIM + 1, KD = NIP * Q, KS = (KD + 3) / 4;
+    constexpr int MT = (N + 15) / 16, TILES = MT * MT;
+    const int lane = tid & 63, w = tid >> 6, lk = lane >> 4, l16 = lane & 15;
+    if (w >= TILES) return;  // wave-uniform
+    const int mt = w / MT, nt = w % MT;
+    const int i = mt * 16 + l16, j = nt * 16 + l16;
+    dbl4 accp = {0.0, 0.0, 0.0, 0.0}, acct = {0.0, 0.0, 0.0, 0.0};
+    for (int ks = 0; ks < KS; ks++) {
+        const int k = ks * 4 + lk;
+        const bool kv = k < KD;
+        const int ip = kv ? k / Q : 0, q = kv ? k % Q : 0;
+        const double xa = (kv && i < N) ? (q == 0 ? sm.Nsh[ip][i] : sm.G[ip][q - 1][i]) : 0.0;
+        const double xb = (kv && j < N) ? (q == 0 ? sm.Nsh[ip][j] : sm.G[ip][q - 1][j]) : 0.0;
+        if (dph) accp = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, (q == 0 ? sm.phA[ip] : sm.phB[ip]) * xb, accp, 0, 0, 0);
+        if (dth) acct = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, (q == 0 ? 0.0 : sm.thB[ip]) * xb, acct, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int ii = mt * 16 + lk + 4 * v, jj = nt * 16 + l16;
+        if (ii < N && jj < N) {
+            if (dph) {
+                double val = accp[v] + (ii == jj ? sm.kdp[ii] : 0.0);
+                bad |= !isfinite(val);
+                rec[ii * RC::SIZE + RC::KP + jj] = val;
+            }
+            if (dth) {
+                double val = acct[v] + (ii == jj ? sm.kdt[ii] : 0.0);
+                bad |= !isfinite(val);
+                rec[ii * RC::SIZE + RC::KT + jj] = val;
+            }
+        }
+    }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
     using E = ET<DIM>;
     constexpr int N = E::N, NIP = E::NIP, NU = E::NU;
     __shared__ Smem<DIM> sm;
@@ -801,8 +901,11 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
         sm.shp.db1[d][q][a] = sd;
     }
     // per-IP coefficients of this element (contiguous per field)
-    if (p.do_u)
+    if (p.do_u) {
         for (int i = tid; i < NIP * 6; i += NT) sm.sig[i / 6][i % 6] = p.ipb[(int64_t)(IPB_SIG + i % 6) * S + gi0 + i / 6];
+        for (int i = tid; i < NIP * 36; i += NT)
+            (&sm.Mt[0][0][0])[(i % NIP) * 36 + i / NIP] = p.ipb[(int64_t)(IPB_M + i / NIP) * S + gi0 + i % NIP];
+    }
     if (p.do_ph)
         for (int i = tid; i < NIP; i += NT) {
             sm.phA[i] = p.ipb[(int64_t)IPB_PHA * S + gi0 + i];
@@ -932,6 +1035,25 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
             if (mc.Damping > 0.0) kd += s * mc.Damping * p.du_dt;
             sm.kdu[tid] = kd;
         }
+        if (p.has_pf && p.do_ph) {
+            const double GcL = sm.GcLast;
+            double kd = 0;
+            if (mc.LumpedDrive) {
+                double ld = 0;
+                for (int ip = 0; ip < NIP; ip++) ld += sm.ldrive[ip] * sm.Nsh[ip][tid];
+                double D0, dD0, ddD0;
+                damage(mc.DamFunction, sm.Ph[tid], mc.k0, mc.p, D0, dD0, ddD0);
+                kd += ld * ddD0;
+            }
+            if (mc.LumpedCap) {
+                double u1, u2, LD, dLD;
+                cap_coeffs(mc.Distribution, mc.l, sm.Ph[tid], u1, u2, LD, dLD);
+                kd += s * GcL * dLD;
+            }
+            kd += s * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
+            sm.kdp[tid] = kd;
+        }
+        if (p.do_th) sm.kdt[tid] = s * mc.Density_th * mc.cp * p.du_dt;
     }
     __syncthreads();
 
@@ -956,57 +1078,9 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
             if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
             rec[a * RC::SIZE + RC::FU + al] = s;
         }
-        // K_UU = sum_ip B^T Mt B via Y = Mt B in IP chunks
-        constexpr int TI = E::TI, TJ = E::TJ, TPP = E::TPP, CH = E::CH;
-        double acc[TI][TJ];
-        for (int i = 0; i < TI; i++)
-            for (int j = 0; j < TJ; j++) acc[i][j] = 0;
-        const int pair = tid / TPP, q = tid % TPP;
-        const int al = pair / DIM, be = pair % DIM;
-        const int I0 = (q / (N / TJ)) * TI, J0 = (q % (N / TJ)) * TJ;
-        for (int ip0 = 0; ip0 < NIP; ip0 += CH) {
-            for (int i = tid; i < CH * 36; i += NT)
-                (&sm.ku.Mt[0][0][0])[i] = p.ipb[(int64_t)(IPB_M + i % 36) * S + gi0 + ip0 + i / 36];
-            __syncthreads();
-            for (int i = tid; i < CH * 6 * NU; i += NT) {
-                int qq = i / (6 * NU), rem = i % (6 * NU), r = rem / NU, col = rem % NU;
-                int b2 = col / N, j = col % N, ip = ip0 + qq;
-                double s = 0;
-                for (int d = 0; d < DIM; d++) s += sm.ku.Mt[qq][r][vrow<DIM>(b2, d)] * sm.G[ip][d][j];
-                sm.ku.Y[qq][r][col] = s;
-            }
-            __syncthreads();
-            if (tid < E::NTK) {
-#pragma unroll 1
-                for (int qq = 0; qq < CH; qq++) {
-                    int ip = ip0 + qq;
-                    for (int c = 0; c < DIM; c++) {
-                        int r = vrow<DIM>(al, c);
-                        double av[TI], bv[TJ];
-                        for (int i = 0; i < TI; i++) av[i] = sm.G[ip][c][I0 + i];
-                        for (int j = 0; j < TJ; j++) bv[j] = sm.ku.Y[qq][r][be * N + J0 + j];
-                        for (int i = 0; i < TI; i++)
-                            for (int j = 0; j < TJ; j++) acc[i][j] += av[i] * bv[j];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        if (tid < E::NTK) {
-            unsigned bad = 0;
-            for (int i = 0; i < TI; i++) {
-                int a = I0 + i;
-                double* out = rec + a * RC::SIZE + RC::KU + (al * DIM + be) * N + J0;
-                for (int j = 0; j < TJ; j++) {
-                    int b = J0 + j;
-                    double v = acc[i][j];
-                    if (al == be && a == b) v += sm.kdu[a];
-                    bad |= !isfinite(v);
-                    out[j] = v;
-                }
-            }
-            if (bad) atomicOr(&sm.nonfinite, 1u);
-        }
+        unsigned bad = 0;
+        kuu_mfma<DIM>(sm, rec, tid, bad);
+        if (bad) atomicOr(&sm.nonfinite, 1u);
     }
 
     // ---- phase step: F_phi, K_phiphi ------------------------------------------------
@@ -1035,34 +1109,6 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
             if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
             rec[a * RC::SIZE + RC::FP] = s;
         }
-        for (int i = tid; i < N * N; i += NT) {
-            int a = i / N, b = i % N;
-            double s = 0;
-#pragma unroll 3
-            for (int ip = 0; ip < NIP; ip++) {
-                double gg = 0;
-                for (int c = 0; c < DIM; c++) gg += sm.G[ip][c][a] * sm.G[ip][c][b];
-                s += sm.phA[ip] * (sm.Nsh[ip][a] * sm.Nsh[ip][b]) + sm.phB[ip] * gg;
-            }
-            if (a == b) {
-                double WLa = sm.WL[a];
-                if (mc.LumpedDrive) {
-                    double ld = 0;
-                    for (int ip = 0; ip < NIP; ip++) ld += sm.ldrive[ip] * sm.Nsh[ip][a];
-                    double D0, dD0, ddD0;
-                    damage(mc.DamFunction, sm.Ph[a], mc.k0, mc.p, D0, dD0, ddD0);
-                    s += ld * ddD0;
-                }
-                if (mc.LumpedCap) {
-                    double u1, u2, LD, dLD;
-                    cap_coeffs(mc.Distribution, mc.l, sm.Ph[a], u1, u2, LD, dLD);
-                    s += WLa * GcL * dLD;
-                }
-                s += WLa * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
-            }
-            if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-            rec[a * RC::SIZE + RC::KP + b] = s;
-        }
     }
 
     // ---- T step: F_T (phase-field heating + thermal), K_TT -------------------------
@@ -1077,21 +1123,11 @@ __global__ __launch_bounds__(NT, 2) void assemble_kernel(KParams p) {
             if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
             rec[a * RC::SIZE + RC::FT] = s;
         }
-        if (p.do_th) {
-            for (int i = tid; i < N * N; i += NT) {
-                int a = i / N, b = i % N;
-                double s = 0;
-#pragma unroll 3
-                for (int ip = 0; ip < NIP; ip++) {
-                    double gg = 0;
-                    for (int c = 0; c < DIM; c++) gg += sm.G[ip][c][a] * sm.G[ip][c][b];
-                    s += sm.thB[ip] * gg;
-                }
-                if (a == b) s += sm.WL[a] * mc.Density_th * mc.cp * p.du_dt;
-                if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-                rec[a * RC::SIZE + RC::KT + b] = s;
-            }
-        }
+    }
+    if ((p.has_pf && p.do_ph) || p.do_th) {
+        unsigned bad = 0;
+        kpt_mfma<DIM>(sm, rec, tid, p.has_pf && p.do_ph, p.do_th, bad);
+        if (bad) atomicOr(&sm.nonfinite, 1u);
     }
     __syncthreads();
     if (tid == 0) {
