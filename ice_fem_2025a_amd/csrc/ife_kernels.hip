@@ -56,8 +56,7 @@ __device__ __forceinline__ int vrow(int a, int c) {
         return a == 0 ? (c == 0 ? 0 : 3) : (c == 0 ? 3 : 1);
     } else {
         // x: 0,3,5   y: 3,1,4   z: 5,4,2
-        const int t[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
-        return t[a][c];
+        return a == c ? a : (a + c == 1 ? 3 : (a + c == 3 ? 4 : 5));
     }
 }
 
@@ -743,6 +742,7 @@ struct Smem {
     double phA[NIP], phB[NIP], phFN[NIP], phFG[NIP][DIM], ldrive[NIP], gw[NIP];
     double thB[NIP], thFN[NIP], thFG[NIP][DIM];
     double kdu[N], kdp[N], kdt[N];  // nodal diagonal additions to K_UU, K_pp, K_TT
+    double fnu[DIM][N], fnp[N], fnt[N];  // nodal (lumped) additions to F_U, F_p, F_T
     double WL[N];
     double GcLast;
     int rank[N];
@@ -751,113 +751,182 @@ struct Smem {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// K_UU = sum_ip B^T Mt B on MFMA.  For each displacement component alpha the block row
-// K[(alpha,i)][(beta,j)] = sum_{k=(ip,c)} G[ip][c][i] * Y[k][(beta,j)] with
-// Y[(ip,c)][(beta,j)] = sum_d Mt[ip][vrow(alpha,c)][vrow(beta,d)] G[ip][d][j]: an
-// (N x DIM*NIP) x (DIM*NIP x DIM*N) product done as 16x16 tiles of
-// v_mfma_f64_16x16x4f64 (A[i][k]: lane%16, lane/16; B[k][j]: lane/16, lane%16; D[i][j]:
-// i = lane/16 + 4*reg, j = lane%16).  B fragments are formed on the fly from Mt and G.
+// K_UU = sum_ip B^T Mt B on MFMA, one wavefront per displacement component alpha.  The
+// block row K[(alpha,i)][(beta,j)] = sum_{c,ip} G[ip][c][i] * Y_c[ip][(beta,j)] with
+// Y_c[ip][(beta,j)] = sum_d Mt[ip][vrow(alpha,c)][vrow(beta,d)] G[ip][d][j].  The contraction
+// runs over c outermost (so the Voigt row vrow(alpha,c) is wave-uniform) and over the IPs in
+// steps of four; every beta block is a 32-column (16 in 2D) group of 16x16 tiles of
+// v_mfma_f64_16x16x4f64 (A[i][k]: i = lane%16, k = lane/16; B[k][j]: k = lane/16, j = lane%16;
+// D[i][j]: i = lane/16 + 4*reg, j = lane%16).  The first padding column j = N of beta block 0
+// carries F_U[(alpha,i)] = sum_{c,ip} G[ip][c][i] sig[ip][vrow(alpha,c)] through the same A.
 template <int DIM>
-__device__ void kuu_mfma(const Smem<DIM>& sm, double* rec, int tid, unsigned& bad) {
+__device__ void kuu_mfma(const Smem<DIM>& sm, double* rec, const int al, const int lane, unsigned& bad) {
     using E = ET<DIM>;
     using RC = Rec<DIM>;
-    constexpr int N = E::N, NIP = E::NIP, NU = E::NU;
-    constexpr int MT = (N + 15) / 16, NTL = (NU + 15) / 16, KS = (DIM * NIP + 3) / 4;
-    constexpr int TILES = DIM * NTL * MT, NW = NT / 64, TPW = (TILES + NW - 1) / NW;
-    const int lane = tid & 63, w = tid >> 6, lk = lane >> 4, l16 = lane & 15;
-    dbl4 acc[TPW];
+    constexpr int N = E::N, NIP = E::NIP, MT = (N + 15) / 16, KSTEP = (NIP + 3) / 4, FCOL = N;
+    static_assert(FCOL < MT * 16, "no padding column for the residual");
+    const int lk = lane >> 4, l16 = lane & 15;
+    dbl4 acc[DIM][MT][MT];
 #pragma unroll
-    for (int u = 0; u < TPW; u++) acc[u] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int ks = 0; ks < KS; ks++) {
-        const int k = ks * 4 + lk;
-        const bool kv = k < DIM * NIP;
-        const int ip = kv ? k / DIM : 0, c = kv ? k % DIM : 0;
-        double a[MT];
+    for (int be = 0; be < DIM; be++)
 #pragma unroll
-        for (int mt = 0; mt < MT; mt++) {
-            const int i = mt * 16 + l16;
-            a[mt] = (kv && i < N) ? sm.G[ip][c][i] : 0.0;
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int jt = 0; jt < MT; jt++) acc[be][mt][jt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < DIM; c++) {
+        const int r = vrow<DIM>(al, c);
+#pragma unroll 1
+        for (int s = 0; s < KSTEP; s++) {
+            const int ip0 = s * 4 + lk;
+            const bool ipv = ip0 < NIP;
+            const int ip = ipv ? ip0 : NIP - 1;
+            double a[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) {
+                const int i = mt * 16 + l16;
+                const double g = sm.G[ip][c][i < N ? i : N - 1];
+                a[mt] = (ipv && i < N) ? g : 0.0;
+            }
+            double m[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) m[q] = sm.Mt[ip][r][q];
+            const double fs = sm.sig[ip][r];
+#pragma unroll
+            for (int jt = 0; jt < MT; jt++) {
+                const int j = jt * 16 + l16;
+                const bool jv = ipv && j < N;
+                double g[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) g[d] = sm.G[ip][d][j < N ? j : N - 1];
+#pragma unroll
+                for (int be = 0; be < DIM; be++) {
+                    double bb = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) bb += m[vrow<DIM>(be, d)] * g[d];
+                    bb = jv ? bb : 0.0;
+                    if (be == 0 && jt == MT - 1) bb = (ipv && j == FCOL) ? fs : bb;
+#pragma unroll
+                    for (int mt = 0; mt < MT; mt++)
+                        acc[be][mt][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bb, acc[be][mt][jt], 0, 0, 0);
+                }
+            }
         }
-        int lastB = -1;
-        double b = 0.0;
+    }
 #pragma unroll
-        for (int u = 0; u < TPW; u++) {
-            const int t = w * TPW + u;
-            if (t < TILES) {
-                const int mt = t % MT, an = t / MT, al = an / NTL, nt = an % NTL;
-                if (an != lastB) {
-                    lastB = an;
-                    const int n = nt * 16 + l16;
-                    b = 0.0;
-                    if (kv && n < NU) {
-                        const int be = n / N, j = n % N, r = vrow<DIM>(al, c);
+    for (int be = 0; be < DIM; be++)
 #pragma unroll
-                        for (int d = 0; d < DIM; d++) b += sm.Mt[ip][r][vrow<DIM>(be, d)] * sm.G[ip][d][j];
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int jt = 0; jt < MT; jt++) {
+                const int j = jt * 16 + l16;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = mt * 16 + lk + 4 * v;
+                    if (i >= N) continue;
+                    if (j < N) {
+                        double val = acc[be][mt][jt][v];
+                        if (al == be && i == j) val += sm.kdu[i];
+                        bad |= !isfinite(val);
+                        rec[i * RC::SIZE + RC::KU + (al * DIM + be) * N + j] = val;
+                    } else if (be == 0 && j == FCOL) {
+                        double val = acc[be][mt][jt][v] + sm.fnu[al][i];
+                        bad |= !isfinite(val);
+                        rec[i * RC::SIZE + RC::FU + al] = val;
                     }
                 }
-                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], b, acc[u], 0, 0, 0);
             }
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < TPW; u++) {
-        const int t = w * TPW + u;
-        if (t >= TILES) break;
-        const int mt = t % MT, an = t / MT, al = an / NTL, nt = an % NTL;
-        const int n = nt * 16 + l16;
-        if (n >= NU) continue;
-        const int be = n / N, j = n % N;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = mt * 16 + lk + 4 * v;
-            if (i < N) {
-                double val = acc[u][v];
-                if (al == be && i == j) val += sm.kdu[i];
-                bad |= !isfinite(val);
-                rec[i * RC::SIZE + RC::KU + (al * DIM + be) * N + j] = val;
-            }
-        }
-    }
 }
 
-// K_pp = sum_ip phA N N^T + phB G^T G and K_TT = sum_ip thB G^T G as one shared-A
-// product over k = (ip, q), q = 0 -> N, q = 1.. -> d/dx_{q-1}, on MFMA.
+// K_pp = sum_ip phA N N^T + phB G^T G and K_TT = sum_ip thB G^T G on MFMA in one wavefront,
+// contracted over k = (q, ip): q = 0 -> N, q = 1.. -> d/dx_{q-1}, q outermost.  The padding
+// column j = N carries F_p = sum_ip phFN N + phFG . G and F_T = sum_ip thFN N + thFG . G.
 template <int DIM>
-__device__ void kpt_mfma(const Smem<DIM>& sm, double* rec, int tid, bool dph, bool dth, unsigned& bad) {
+__device__ void kpt_mfma(const Smem<DIM>& sm, double* rec, const int lane, const bool dph, const bool dT,
+                         const bool dth, unsigned& bad) {
     using E = ET<DIM>;
     using RC = Rec<DIM>;
-    constexpr int N = E::N, NIP = E::NIP, Q = DIM + 1, KD = NIP * Q, KS = (KD + 3) / 4;
-    constexpr int MT = (N + 15) / 16, TILES = MT * MT;
-    const int lane = tid & 63, w = tid >> 6, lk = lane >> 4, l16 = lane & 15;
-    if (w >= TILES) return;  // wave-uniform
-    const int mt = w / MT, nt = w % MT;
-    const int i = mt * 16 + l16, j = nt * 16 + l16;
-    dbl4 accp = {0.0, 0.0, 0.0, 0.0}, acct = {0.0, 0.0, 0.0, 0.0};
-    for (int ks = 0; ks < KS; ks++) {
-        const int k = ks * 4 + lk;
-        const bool kv = k < KD;
-        const int ip = kv ? k / Q : 0, q = kv ? k % Q : 0;
-        const double xa = (kv && i < N) ? (q == 0 ? sm.Nsh[ip][i] : sm.G[ip][q - 1][i]) : 0.0;
-        const double xb = (kv && j < N) ? (q == 0 ? sm.Nsh[ip][j] : sm.G[ip][q - 1][j]) : 0.0;
-        if (dph) accp = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, (q == 0 ? sm.phA[ip] : sm.phB[ip]) * xb, accp, 0, 0, 0);
-        if (dth) acct = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, (q == 0 ? 0.0 : sm.thB[ip]) * xb, acct, 0, 0, 0);
-    }
+    constexpr int N = E::N, NIP = E::NIP, MT = (N + 15) / 16, KSTEP = (NIP + 3) / 4, FCOL = N;
+    const int lk = lane >> 4, l16 = lane & 15;
+    dbl4 accp[MT][MT], acct[MT][MT];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        const int ii = mt * 16 + lk + 4 * v, jj = nt * 16 + l16;
-        if (ii < N && jj < N) {
-            if (dph) {
-                double val = accp[v] + (ii == jj ? sm.kdp[ii] : 0.0);
-                bad |= !isfinite(val);
-                rec[ii * RC::SIZE + RC::KP + jj] = val;
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int nt = 0; nt < MT; nt++) {
+            accp[mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
+            acct[mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
+#pragma unroll
+    for (int q = 0; q <= DIM; q++) {
+#pragma unroll 1
+        for (int s = 0; s < KSTEP; s++) {
+            const int ip0 = s * 4 + lk;
+            const bool ipv = ip0 < NIP;
+            const int ip = ipv ? ip0 : NIP - 1;
+            const double* X = q == 0 ? &sm.Nsh[ip][0] : &sm.G[ip][q > 0 ? q - 1 : 0][0];
+            double a[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) {
+                const int i = mt * 16 + l16;
+                const double x = X[i < N ? i : N - 1];
+                a[mt] = (ipv && i < N) ? x : 0.0;
             }
-            if (dth) {
-                double val = acct[v] + (ii == jj ? sm.kdt[ii] : 0.0);
-                bad |= !isfinite(val);
-                rec[ii * RC::SIZE + RC::KT + jj] = val;
+            const double cp = q == 0 ? sm.phA[ip] : sm.phB[ip];
+            const double fp = q == 0 ? sm.phFN[ip] : sm.phFG[ip][q > 0 ? q - 1 : 0];
+            const double ct = q == 0 ? 0.0 : sm.thB[ip];
+            const double ft = q == 0 ? sm.thFN[ip] : sm.thFG[ip][q > 0 ? q - 1 : 0];
+#pragma unroll
+            for (int nt = 0; nt < MT; nt++) {
+                const int j = nt * 16 + l16;
+                const double x = X[j < N ? j : N - 1];
+                double bp = j < N ? cp * x : (j == FCOL ? fp : 0.0);
+                double bt = j < N ? ct * x : (j == FCOL ? ft : 0.0);
+                bp = ipv ? bp : 0.0;
+                bt = ipv ? bt : 0.0;
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++) {
+                    if (dph) accp[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bp, accp[mt][nt], 0, 0, 0);
+                    // q = 0 feeds K_TT nothing: only the tile holding the residual column
+                    if (dT && (q > 0 || nt == MT - 1))
+                        acct[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bt, acct[mt][nt], 0, 0, 0);
+                }
             }
         }
     }
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int nt = 0; nt < MT; nt++) {
+            const int j = nt * 16 + l16;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = mt * 16 + lk + 4 * v;
+                if (i >= N) continue;
+                if (j < N) {
+                    if (dph) {
+                        double val = accp[mt][nt][v] + (i == j ? sm.kdp[i] : 0.0);
+                        bad |= !isfinite(val);
+                        rec[i * RC::SIZE + RC::KP + j] = val;
+                    }
+                    if (dth) {
+                        double val = acct[mt][nt][v] + (i == j ? sm.kdt[i] : 0.0);
+                        bad |= !isfinite(val);
+                        rec[i * RC::SIZE + RC::KT + j] = val;
+                    }
+                } else if (j == FCOL) {
+                    if (dph) {
+                        double val = accp[mt][nt][v] + sm.fnp[i];
+                        bad |= !isfinite(val);
+                        rec[i * RC::SIZE + RC::FP] = val;
+                    }
+                    if (dT) {
+                        double val = acct[mt][nt][v] + sm.fnt[i];
+                        bad |= !isfinite(val);
+                        rec[i * RC::SIZE + RC::FT] = val;
+                    }
+                }
+            }
+        }
 }
 
 template <int DIM>
@@ -1022,111 +1091,70 @@ __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
         for (int ip = 0; ip < NIP; ip++) s += sm.gw[ip];
         p.lcrack_elem[el] = s;
     }
-    // lumped weights (the same for the u, phase and T groups: one element group)
+    // lumped weights (the same for the u, phase and T groups: one element group) and the
+    // nodal (lumped) tangent and residual terms
     if (tid < N) {
+        const int a = tid;
         double s = 0;
-        for (int ip = 0; ip < NIP; ip++) s += sm.w[ip] * sm.Nsh[ip][tid];
-        sm.WL[tid] = s;
+        for (int ip = 0; ip < NIP; ip++) s += sm.w[ip] * sm.Nsh[ip][a];
+        sm.WL[a] = s;
         if (p.has_pf && p.do_u) {
+            // inertia, damping and gravity (PhaseField.cpp:436-477)
+            double DamI, DamG, d1, d2;
+            damage(mc.DensityDamage, sm.Ph[a], mc.k0, mc.p, DamI, d1, d2);
+            damage(mc.GravityDamage, sm.Ph[a], mc.k0, mc.p, DamG, d1, d2);
             double kd = 0;
-            double DamI, d1, d2;
-            damage(mc.DensityDamage, sm.Ph[tid], mc.k0, mc.p, DamI, d1, d2);
             if (mc.Inertia) kd += s * DamI * mc.Density * p.ddu_dt;
             if (mc.Damping > 0.0) kd += s * mc.Damping * p.du_dt;
-            sm.kdu[tid] = kd;
+            sm.kdu[a] = kd;
+            for (int al = 0; al < DIM; al++) {
+                double f = 0;
+                if (mc.Inertia) f += s * DamI * mc.Density * sm.ddU[al][a];
+                if (mc.Damping > 0.0) f += s * mc.Damping * sm.dU[al][a];
+                f += -s * DamG * mc.Density * mc.g[al];
+                sm.fnu[al][a] = f;
+            }
         }
         if (p.has_pf && p.do_ph) {
+            // lumped driving / capacity, phase viscosity and stabilisation (PhaseField.cpp:693-711)
             const double GcL = sm.GcLast;
-            double kd = 0;
-            if (mc.LumpedDrive) {
-                double ld = 0;
-                for (int ip = 0; ip < NIP; ip++) ld += sm.ldrive[ip] * sm.Nsh[ip][tid];
-                double D0, dD0, ddD0;
-                damage(mc.DamFunction, sm.Ph[tid], mc.k0, mc.p, D0, dD0, ddD0);
-                kd += ld * ddD0;
-            }
-            if (mc.LumpedCap) {
-                double u1, u2, LD, dLD;
-                cap_coeffs(mc.Distribution, mc.l, sm.Ph[tid], u1, u2, LD, dLD);
-                kd += s * GcL * dLD;
-            }
-            kd += s * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
-            sm.kdp[tid] = kd;
-        }
-        if (p.do_th) sm.kdt[tid] = s * mc.Density_th * mc.cp * p.du_dt;
-    }
-    __syncthreads();
-
-    // ---- u step: F_U and K_UU ------------------------------------------------------
-    if (p.has_pf && p.do_u) {
-        // F_U[(alpha,a)] = sum_ip sum_c G[ip][c][a] sig[ip][vrow(alpha,c)] + nodal terms
-        for (int i = tid; i < NU; i += NT) {
-            int al = i / N, a = i % N;
-            double s = 0;
-            for (int ip = 0; ip < NIP; ip++)
-                for (int c = 0; c < DIM; c++) s += sm.G[ip][c][a] * sm.sig[ip][vrow<DIM>(al, c)];
-            double WLa = sm.WL[a];
-            if (mc.Inertia) {
-                double DamI, d1, d2;
-                damage(mc.DensityDamage, sm.Ph[a], mc.k0, mc.p, DamI, d1, d2);
-                s += WLa * DamI * mc.Density * sm.ddU[al][a];
-            }
-            if (mc.Damping > 0.0) s += WLa * mc.Damping * sm.dU[al][a];
-            double DamG, d1, d2;
-            damage(mc.GravityDamage, sm.Ph[a], mc.k0, mc.p, DamG, d1, d2);
-            s += -WLa * DamG * mc.Density * mc.g[al];
-            if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-            rec[a * RC::SIZE + RC::FU + al] = s;
-        }
-        unsigned bad = 0;
-        kuu_mfma<DIM>(sm, rec, tid, bad);
-        if (bad) atomicOr(&sm.nonfinite, 1u);
-    }
-
-    // ---- phase step: F_phi, K_phiphi ------------------------------------------------
-    if (p.has_pf && p.do_ph) {
-        const double GcL = sm.GcLast;
-        for (int a = tid; a < N; a += NT) {
-            double s = 0;
-            for (int ip = 0; ip < NIP; ip++) {
-                s += sm.phFN[ip] * sm.Nsh[ip][a];
-                for (int c = 0; c < DIM; c++) s += sm.G[ip][c][a] * sm.phFG[ip][c];
-            }
-            double WLa = sm.WL[a];
+            double kd = 0, f = 0;
             if (mc.LumpedDrive) {
                 double ld = 0;
                 for (int ip = 0; ip < NIP; ip++) ld += sm.ldrive[ip] * sm.Nsh[ip][a];
                 double D0, dD0, ddD0;
                 damage(mc.DamFunction, sm.Ph[a], mc.k0, mc.p, D0, dD0, ddD0);
-                s += ld * dD0;
+                kd += ld * ddD0;
+                f += ld * dD0;
             }
             if (mc.LumpedCap) {
                 double u1, u2, LD, dLD;
                 cap_coeffs(mc.Distribution, mc.l, sm.Ph[a], u1, u2, LD, dLD);
-                s += WLa * GcL * LD;
+                kd += s * GcL * dLD;
+                f += s * GcL * LD;
             }
-            s += WLa * mc.pf_visc / p.dt * (sm.Ph[a] - sm.PhO[a]);
-            if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-            rec[a * RC::SIZE + RC::FP] = s;
+            kd += s * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
+            f += s * mc.pf_visc / p.dt * (sm.Ph[a] - sm.PhO[a]);
+            sm.kdp[a] = kd;
+            sm.fnp[a] = f;
         }
+        // heat capacity (ThermalModel.cpp:128-129)
+        sm.kdt[a] = p.do_th ? s * mc.Density_th * mc.cp * p.du_dt : 0.0;
+        sm.fnt[a] = p.do_th ? s * mc.Density_th * mc.cp * sm.dT[a] : 0.0;
     }
+    __syncthreads();
 
-    // ---- T step: F_T (phase-field heating + thermal), K_TT -------------------------
-    if (p.do_Tpf || p.do_th) {
-        for (int a = tid; a < N; a += NT) {
-            double s = 0;
-            for (int ip = 0; ip < NIP; ip++) {
-                s += sm.thFN[ip] * sm.Nsh[ip][a];
-                for (int c = 0; c < DIM; c++) s += sm.G[ip][c][a] * sm.thFG[ip][c];
-            }
-            if (p.do_th) s += sm.WL[a] * mc.Density_th * mc.cp * sm.dT[a];
-            if (!isfinite(s)) atomicOr(&sm.nonfinite, 1u);
-            rec[a * RC::SIZE + RC::FT] = s;
-        }
-    }
-    if ((p.has_pf && p.do_ph) || p.do_th) {
+    // ---- tangents and residuals: waves 0..DIM-1 K_UU/F_U (alpha = wave), wave DIM the
+    // phase and T steps ---------------------------------------------------------------
+    {
+        const int wv = tid >> 6, lane = tid & 63;
+        const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
         unsigned bad = 0;
-        kpt_mfma<DIM>(sm, rec, tid, p.has_pf && p.do_ph, p.do_th, bad);
+        if (wv < DIM) {
+            if (du) kuu_mfma<DIM>(sm, rec, wv, lane, bad);
+        } else if (wv == DIM) {
+            if (dph || dT) kpt_mfma<DIM>(sm, rec, lane, dph, dT, p.do_th, bad);
+        }
         if (bad) atomicOr(&sm.nonfinite, 1u);
     }
     __syncthreads();
@@ -1143,71 +1171,90 @@ __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
 // node's CSR rows, then writes each row once, contiguously.  Every CSR entry of the
 // pattern and every residual entry belongs to exactly one node, so the rows are
 // overwritten rather than zeroed and accumulated (Physics::Assemble's zero + add).
+//
+// The workgroup is a single wavefront, whose LDS instructions execute in issue order, so
+// record k+1's read-add-write of a slot sees record k's: no barrier is needed between
+// records, and the loads of PF records are issued before any of them is added.
 // ---------------------------------------------------------------------------------------
-constexpr int GW = 64;      // lanes per gather workgroup
+constexpr int GW = 64;      // lanes per gather workgroup (one wavefront)
 constexpr int MAXDEG = 128; // neighbour slots per row held in LDS
+constexpr int GPF = 4;      // records loaded ahead
 
 template <int DIM>
 __global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
     using RC = Rec<DIM>;
-    constexpr int N = RC::N;
-    __shared__ double ku[DIM][DIM][MAXDEG];
-    __shared__ double kp[MAXDEG], kt[MAXDEG];
-    __shared__ double fu[DIM], fp[1], ft[1];
+    constexpr int N = RC::N, PER = (RC::SIZE + GW - 1) / GW;
+    // LDS row accumulators: K_UU [DIM*DIM][MAXDEG], K_pp [MAXDEG], K_TT [MAXDEG], f_U, f_p, f_T
+    constexpr int OKP = DIM * DIM * MAXDEG, OKT = OKP + MAXDEG, OF = OKT + MAXDEG, LEN = OF + DIM + 2;
+    __shared__ double acc[LEN];
     const int lane = threadIdx.x;
     const int64_t r = blockIdx.x;
     const int64_t r0 = p.nbr_ptr[r];
     const int deg = (int)(p.nbr_ptr[r + 1] - r0);
     const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
-    for (int i = lane; i < DIM * DIM * MAXDEG; i += GW) (&ku[0][0][0])[i] = 0.0;
-    for (int i = lane; i < MAXDEG; i += GW) { kp[i] = 0.0; kt[i] = 0.0; }
-    if (lane < DIM) fu[lane] = 0.0;
-    if (lane == 0) { fp[0] = 0.0; ft[0] = 0.0; }
+    for (int i = lane; i < LEN; i += GW) acc[i] = 0.0;
+    // per lane: record entry t -> LDS base slot, index into the record's column-slot row
+    // (-1: fixed slot), active flag
+    int base[PER], lb[PER];
+    bool act[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int t = lane + u * GW;
+        base[u] = 0; lb[u] = -1; act[u] = false;
+        if (t < RC::FU) { base[u] = (t / N) * MAXDEG; lb[u] = t % N; act[u] = du; }
+        else if (t < RC::KP) { base[u] = OF + (t - RC::FU); act[u] = du; }
+        else if (t < RC::FP) { base[u] = OKP; lb[u] = t - RC::KP; act[u] = dph; }
+        else if (t == RC::FP) { base[u] = OF + DIM; act[u] = dph; }
+        else if (t < RC::FT) { base[u] = OKT; lb[u] = t - RC::KT; act[u] = p.do_th; }
+        else if (t == RC::FT) { base[u] = OF + DIM + 1; act[u] = dT; }
+    }
     __syncthreads();
-    for (int64_t k = p.n2e_ptr[r]; k < p.n2e_ptr[r + 1]; k++) {
-        const int64_t e = p.n2e_elem[k];
-        const int a = p.n2e_loc[k];
-        const double* rec = p.ebuf + (e * N + a) * RC::SIZE;
-        const uint8_t* lc = p.loc + (e * N + a) * N;
-        for (int t = lane; t < RC::SIZE; t += GW) {
-            if (t < RC::FU) {
-                if (du) {
-                    int ab = t / N, b = t % N;
-                    (&ku[0][0][0])[ab * MAXDEG + lc[b]] += rec[t];
-                }
-            } else if (t < RC::KP) {
-                if (du) fu[t - RC::FU] += rec[t];
-            } else if (t < RC::FP) {
-                if (dph) kp[lc[t - RC::KP]] += rec[t];
-            } else if (t == RC::FP) {
-                if (dph) fp[0] += rec[t];
-            } else if (t < RC::FT) {
-                if (p.do_th) kt[lc[t - RC::KT]] += rec[t];
-            } else {
-                if (dT) ft[0] += rec[t];
+    const int64_t k0 = p.n2e_ptr[r], k1 = p.n2e_ptr[r + 1];
+    for (int64_t kb = k0; kb < k1; kb += GPF) {
+        double v[GPF][PER];
+        int slot[GPF][PER];
+#pragma unroll
+        for (int q = 0; q < GPF; q++) {
+            const int64_t k = kb + q < k1 ? kb + q : k1 - 1;
+            const int64_t e = p.n2e_elem[k];
+            const int a = p.n2e_loc[k];
+            const double* rec = p.ebuf + (e * N + a) * RC::SIZE;
+            const uint8_t* lc = p.loc + (e * N + a) * N;
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                v[q][u] = act[u] ? rec[lane + u * GW] : 0.0;
+                slot[q][u] = base[u] + (act[u] && lb[u] >= 0 ? (int)lc[lb[u]] : 0);
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < GPF; q++) {
+            if (kb + q < k1) {
+#pragma unroll
+                for (int u = 0; u < PER; u++)
+                    if (act[u]) acc[slot[q][u]] += v[q][u];
+            }
+        }
     }
+    __syncthreads();
     if (du) {
         for (int al = 0; al < DIM; al++) {
             double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * r0;
             for (int be = 0; be < DIM; be++)
-                for (int i = lane; i < deg; i += GW) K[(int64_t)p.fpos_u[be] * deg + i] = ku[al][be][i];
+                for (int i = lane; i < deg; i += GW) K[(int64_t)p.fpos_u[be] * deg + i] = acc[(al * DIM + be) * MAXDEG + i];
         }
-        if (lane < DIM) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = fu[lane];
+        if (lane < DIM) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = acc[OF + lane];
     }
     if (dph) {
         double* K = p.sph.K + (int64_t)p.fpos_ph * p.sph.M + (int64_t)p.sph.nf * r0 + (int64_t)p.fpos_ph * deg;
-        for (int i = lane; i < deg; i += GW) K[i] = kp[i];
-        if (lane == 0) p.sph.f[(int64_t)p.fpos_ph * p.Ng + r] = fp[0];
+        for (int i = lane; i < deg; i += GW) K[i] = acc[OKP + i];
+        if (lane == 0) p.sph.f[(int64_t)p.fpos_ph * p.Ng + r] = acc[OF + DIM];
     }
     if (dT) {
         if (p.do_th) {
             double* K = p.sT.K + (int64_t)p.fpos_Tth * p.sT.M + (int64_t)p.sT.nf * r0 + (int64_t)p.fpos_Tth * deg;
-            for (int i = lane; i < deg; i += GW) K[i] = kt[i];
+            for (int i = lane; i < deg; i += GW) K[i] = acc[OKT + i];
         }
-        if (lane == 0) p.sT.f[(int64_t)p.fpos_Tth * p.Ng + r] = ft[0];
+        if (lane == 0) p.sT.f[(int64_t)p.fpos_Tth * p.Ng + r] = acc[OF + DIM + 1];
     }
 }
 
