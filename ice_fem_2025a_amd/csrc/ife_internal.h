@@ -52,9 +52,13 @@ struct StepIO {
 //   PHFN coefficient of N in F_phi                            PHFG coefficient of G^T (3)
 //   LDRIVE w*Energy (lumped driving)   GW gamma*w (crack length)   GC Gc at the IP
 //   THB  w*k   THFN -w*(heating rate)   THFG w*k*grad T (3)
+//   geometry at the IP, for G = W_a (JI . GB_a - CB_a DW) and N = W_a CB_a WF (3D rational basis;
+//   2D: G = JI . GB_a, N = CB_a): JI = J^-1 / wFun (row-major DIM x DIM), WF = 1 / wFun,
+//   DW = J^-1 dW / wFun^2 (3), W the IP weight
 enum {
     IPB_M = 0, IPB_SIG = 36, IPB_PHA = 42, IPB_PHB = 43, IPB_PHFN = 44, IPB_PHFG = 45, IPB_LDRIVE = 48,
-    IPB_THB = 49, IPB_THFN = 50, IPB_THFG = 51, IPB_GW = 54, IPB_GC = 55, IPB_NF = 56
+    IPB_THB = 49, IPB_THFN = 50, IPB_THFG = 51, IPB_GW = 54, IPB_GC = 55,
+    IPB_JI = 56, IPB_WF = 65, IPB_DW = 66, IPB_W = 69, IPB_NF = 70
 };
 
 struct Status {
@@ -96,7 +100,7 @@ struct KParams {
     int64_t NIPT;
     Status* st;
     double* lcrack_elem;     // [e]
-    double* ipb;             // per-IP coefficients [IPB_NF][NIPT]
+    double* ipb;             // per-IP coefficients [e][IPB_NF][nip]
     double* ebuf;            // element records [e][a][record] (rows of node a of the element)
     const int64_t* n2e_ptr;  // node (group rank) -> adjacent elements, ascending
     const int32_t* n2e_elem;

@@ -420,6 +420,7 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
     unsigned my_calls = 0, my_iters = 0, my_maxit = 0, my_bad = 0;
     if (le < EPB && el < p.NE) {
         const int64_t gi = el * NIP + ip;
+        double* const ipo = p.ipb + el * IPB_NF * NIP + ip;  // [e][field][ip]
         const int64_t S = p.NIPT;
         int q[3] = {ip % 3, (ip / 3) % 3, ip / 9};
         // 1D B-spline values/derivatives at this IP: b = C_d * Bernstein(xi_q)
@@ -511,6 +512,21 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             Ji[1][1] = Jt[0][0] * id;
             w = p.wq[q[0]] * p.wq[q[1]] * fabs(det);
         }
+        // geometry for the element kernel's shape functions, folded so that it needs no
+        // division: G_a = W_a (J^-1 GB_a / wFun - CB_a J^-1 dW / wFun^2), N_a = W_a CB_a / wFun
+        {
+            const double rw = DIM == 3 ? 1.0 / wF : 1.0;
+            for (int c = 0; c < DIM; c++) {
+                double q = 0;
+                for (int d = 0; d < DIM; d++) {
+                    ipo[(IPB_JI + c * DIM + d) * NIP] = Ji[c][d] * rw;
+                    q += Ji[c][d] * dW[d];
+                }
+                ipo[(IPB_DW + c) * NIP] = q * rw * rw;
+            }
+            ipo[IPB_WF * NIP] = rw;
+            ipo[IPB_W * NIP] = w;
+        }
         // field values and gradients at the IP
         double gu[DIM][DIM], guo[DIM][DIM], gphi[DIM], gT[DIM];
         double phi = 0, phio = 0, Tu = 0, Tph = 0;
@@ -601,7 +617,7 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             for (int r = 0; r < 6; r++) {
                 double v = w * sig[r];
                 my_bad |= !isfinite(v);
-                p.ipb[(int64_t)(IPB_SIG + r) * S + gi] = v;
+                ipo[(IPB_SIG + r) * NIP] = v;
             }
             // Mt = w * S * DPlast = w * (S - (S a) b^T inv)
             for (int i = 0; i < 6; i++) {
@@ -610,7 +626,7 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                     for (int k = 0; k < 6; k++) sa += Sm[i][k] * ra[k];
                 for (int j = 0; j < 6; j++) {
                     double v = elastic ? Sm[i][j] : (Sm[i][j] - sa * rb[j] * rinv);
-                    p.ipb[(int64_t)(IPB_M + i * 6 + j) * S + gi] = w * v;
+                    ipo[(IPB_M + i * 6 + j) * NIP] = w * v;
                 }
             }
         } else if (p.has_pf && p.do_Tpf) {
@@ -684,21 +700,21 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 FN += -w * mc.Density * gu_ * dDamG;
                 A += -w * mc.Density * gu_ * ddDamG;
             }
-            p.ipb[(int64_t)IPB_PHA * S + gi] = A;
-            p.ipb[(int64_t)IPB_PHB * S + gi] = w * m.Gc * cGG;
-            p.ipb[(int64_t)IPB_PHFN * S + gi] = FN;
-            for (int c = 0; c < DIM; c++) p.ipb[(int64_t)(IPB_PHFG + c) * S + gi] = w * m.Gc * cGG * gphi[c];
-            p.ipb[(int64_t)IPB_LDRIVE * S + gi] = LDR;
-            p.ipb[(int64_t)IPB_GW * S + gi] = gamma * w;
-            p.ipb[(int64_t)IPB_GC * S + gi] = m.Gc;
+            ipo[(IPB_PHA) * NIP] = A;
+            ipo[(IPB_PHB) * NIP] = w * m.Gc * cGG;
+            ipo[(IPB_PHFN) * NIP] = FN;
+            for (int c = 0; c < DIM; c++) ipo[(IPB_PHFG + c) * NIP] = w * m.Gc * cGG * gphi[c];
+            ipo[(IPB_LDRIVE) * NIP] = LDR;
+            ipo[(IPB_GW) * NIP] = gamma * w;
+            ipo[(IPB_GC) * NIP] = m.Gc;
             my_bad |= !isfinite(A) || !isfinite(FN);
         }
         if (p.do_Tpf || p.do_th) {
             double fn = 0;
             if (p.has_pf && p.do_Tpf) fn = -w * ((VD - p.VDOld[gi]) / p.dt);
-            p.ipb[(int64_t)IPB_THFN * S + gi] = fn;
-            p.ipb[(int64_t)IPB_THB * S + gi] = p.do_th ? w * mc.k : 0.0;
-            for (int c = 0; c < DIM; c++) p.ipb[(int64_t)(IPB_THFG + c) * S + gi] = p.do_th ? w * mc.k * gT[c] : 0.0;
+            ipo[(IPB_THFN) * NIP] = fn;
+            ipo[(IPB_THB) * NIP] = p.do_th ? w * mc.k : 0.0;
+            for (int c = 0; c < DIM; c++) ipo[(IPB_THFG + c) * NIP] = p.do_th ? w * mc.k * gT[c] : 0.0;
         }
     }
     if (my_calls) {
@@ -709,6 +725,8 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
     if (my_bad) atomicOr(&sm.cnt[3], 1u);
     __syncthreads();
     if (tid == 0) {
+        const int64_t ne = p.NE - e0 < EPB ? p.NE - e0 : EPB;
+        atomicAdd(&p.st->elements, (unsigned long long)ne);
         if (sm.cnt[0]) {
             atomicAdd(&p.st->rm_calls, (unsigned long long)sm.cnt[0]);
             atomicAdd(&p.st->rm_iters, (unsigned long long)sm.cnt[1]);
@@ -724,88 +742,72 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
 // ---------------------------------------------------------------------------------------
 template <int DIM>
 struct Smem {
-    static constexpr int N = ET<DIM>::N, NIP = ET<DIM>::NIP, NU = ET<DIM>::NU;
+    static constexpr int N = ET<DIM>::N, NIP = ET<DIM>::NIP;
+    static constexpr int NP = (N + 15) / 16 * 16;    // nodes padded to whole 16-wide tiles
+    static constexpr int IPP = (NIP + 3) / 4 * 4;    // IPs padded to whole 4-deep k steps
     double X[N][4];            // x, y, z, W
-    double Nsh[NIP][N];        // N (first CB)
-    double G[NIP][DIM][N];     // G (first GB, then GXi)
-    double w[NIP];
+    double Nsh[IPP][NP];       // N, zero beyond (NIP, N)
+    double G[IPP][DIM][NP];    // dN/dx, zero beyond (NIP, N)
     struct {
         double b1[DIM][3][3];      // 1D B-spline values  [dir][gauss pt][fn]
         double db1[DIM][3][3];     // 1D derivatives
-        double red[NIP][4];        // wFun, dW
-        double Jinv[NIP][DIM][DIM];
     } shp;
-    double Mt[NIP][6][6];      // w*(Dam*D_Dam + D_UnDam)*DPlast per IP
     double dU[DIM][N], ddU[DIM][N];
     double Ph[N], PhO[N], dT[N];
-    double sig[NIP][6];        // w*(Dam*S_Dam + S_UnDam)
-    double phA[NIP], phB[NIP], phFN[NIP], phFG[NIP][DIM], ldrive[NIP], gw[NIP];
-    double thB[NIP], thFN[NIP], thFG[NIP][DIM];
-    double kdu[N], kdp[N], kdt[N];  // nodal diagonal additions to K_UU, K_pp, K_TT
+    double ipc[IPB_NF][IPP];   // per-IP coefficients from the material kernel (ife::IPB_*), zero beyond NIP
+    double kdu[N], kdp[N], kdt[N];       // nodal diagonal additions to K_UU, K_pp, K_TT
     double fnu[DIM][N], fnp[N], fnt[N];  // nodal (lumped) additions to F_U, F_p, F_T
-    double WL[N];
-    double GcLast;
-    int rank[N];
-    unsigned int nonfinite;
 };
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// K_UU = sum_ip B^T Mt B on MFMA, one wavefront per displacement component alpha.  The
-// block row K[(alpha,i)][(beta,j)] = sum_{c,ip} G[ip][c][i] * Y_c[ip][(beta,j)] with
-// Y_c[ip][(beta,j)] = sum_d Mt[ip][vrow(alpha,c)][vrow(beta,d)] G[ip][d][j].  The contraction
-// runs over c outermost (so the Voigt row vrow(alpha,c) is wave-uniform) and over the IPs in
-// steps of four; every beta block is a 32-column (16 in 2D) group of 16x16 tiles of
-// v_mfma_f64_16x16x4f64 (A[i][k]: i = lane%16, k = lane/16; B[k][j]: k = lane/16, j = lane%16;
-// D[i][j]: i = lane/16 + 4*reg, j = lane%16).  The first padding column j = N of beta block 0
-// carries F_U[(alpha,i)] = sum_{c,ip} G[ip][c][i] sig[ip][vrow(alpha,c)] through the same A.
-template <int DIM>
-__device__ void kuu_mfma(const Smem<DIM>& sm, double* rec, const int al, const int lane, unsigned& bad) {
-    using E = ET<DIM>;
-    using RC = Rec<DIM>;
-    constexpr int N = E::N, NIP = E::NIP, MT = (N + 15) / 16, KSTEP = (NIP + 3) / 4, FCOL = N;
-    static_assert(FCOL < MT * 16, "no padding column for the residual");
-    const int lk = lane >> 4, l16 = lane & 15;
-    dbl4 acc[DIM][MT][MT];
+// MFMA operand layout of v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16 k, B[k][j] from
+// lane j + 16 k, D[i][j] in lane j + 16 (i % 4), register i / 4.
+
+// K_UU block row of displacement component alpha = sum_ip B^T Mt B.  K[(alpha,i)][(beta,j)] =
+// sum_{c,ip} G[ip][c][i] * Y_c[ip][(beta,j)] with Y_c[ip][(beta,j)] =
+// sum_d Mt[ip][vrow(alpha,c)][vrow(beta,d)] G[ip][d][j], contracted with c outermost (so the
+// Voigt row vrow(alpha,c) is uniform over the wavefront) and the IPs in steps of four.  Each
+// beta block is MT x MT 16x16 tiles.  The first padding column j = N of beta block 0 carries
+// F_U[(alpha,i)] = sum_{c,ip} G[ip][c][i] sig[ip][vrow(alpha,c)] through the same A operand.
+template <int DIM, int MT>
+__device__ __forceinline__ void kuu_loop(const Smem<DIM>& sm, const int al, const int lk, const int l16,
+                                         dbl4 (&acc)[DIM][MT][MT]) {
+    using SM = Smem<DIM>;
+    constexpr int N = SM::N, IPP = SM::IPP, KSTEP = IPP / 4, FCOL = N;
 #pragma unroll
-    for (int be = 0; be < DIM; be++)
+    for (int x = 0; x < DIM; x++)
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-            for (int jt = 0; jt < MT; jt++) acc[be][mt][jt] = dbl4{0.0, 0.0, 0.0, 0.0};
+            for (int nt = 0; nt < MT; nt++) acc[x][mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
         const int r = vrow<DIM>(al, c);
+        const double* const Mrow = &sm.ipc[IPB_M + r * 6][0];
+        const double* const Srow = &sm.ipc[IPB_SIG + r][0];
 #pragma unroll 1
         for (int s = 0; s < KSTEP; s++) {
-            const int ip0 = s * 4 + lk;
-            const bool ipv = ip0 < NIP;
-            const int ip = ipv ? ip0 : NIP - 1;
+            const int ip = s * 4 + lk;
             double a[MT];
 #pragma unroll
-            for (int mt = 0; mt < MT; mt++) {
-                const int i = mt * 16 + l16;
-                const double g = sm.G[ip][c][i < N ? i : N - 1];
-                a[mt] = (ipv && i < N) ? g : 0.0;
-            }
+            for (int mt = 0; mt < MT; mt++) a[mt] = sm.G[ip][c][mt * 16 + l16];
             double m[6];
 #pragma unroll
-            for (int q = 0; q < 6; q++) m[q] = sm.Mt[ip][r][q];
-            const double fs = sm.sig[ip][r];
+            for (int q = 0; q < 6; q++) m[q] = Mrow[q * IPP + ip];
+            const double fs = Srow[ip];
 #pragma unroll
             for (int jt = 0; jt < MT; jt++) {
                 const int j = jt * 16 + l16;
-                const bool jv = ipv && j < N;
                 double g[DIM];
 #pragma unroll
-                for (int d = 0; d < DIM; d++) g[d] = sm.G[ip][d][j < N ? j : N - 1];
+                for (int d = 0; d < DIM; d++) g[d] = sm.G[ip][d][j];
 #pragma unroll
                 for (int be = 0; be < DIM; be++) {
                     double bb = 0.0;
 #pragma unroll
                     for (int d = 0; d < DIM; d++) bb += m[vrow<DIM>(be, d)] * g[d];
-                    bb = jv ? bb : 0.0;
-                    if (be == 0 && jt == MT - 1) bb = (ipv && j == FCOL) ? fs : bb;
+                    if (be == 0 && jt == MT - 1) bb = j == FCOL ? fs : bb;
 #pragma unroll
                     for (int mt = 0; mt < MT; mt++)
                         acc[be][mt][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bb, acc[be][mt][jt], 0, 0, 0);
@@ -813,6 +815,13 @@ __device__ void kuu_mfma(const Smem<DIM>& sm, double* rec, const int al, const i
             }
         }
     }
+}
+
+template <int DIM, int MT>
+__device__ __forceinline__ void kuu_store(const Smem<DIM>& sm, double* rec, const int al, const int lk, const int l16,
+                                          const dbl4 (&acc)[DIM][MT][MT], double& chk) {
+    using RC = Rec<DIM>;
+    constexpr int N = Smem<DIM>::N, FCOL = N;
 #pragma unroll
     for (int be = 0; be < DIM; be++)
 #pragma unroll
@@ -827,72 +836,73 @@ __device__ void kuu_mfma(const Smem<DIM>& sm, double* rec, const int al, const i
                     if (j < N) {
                         double val = acc[be][mt][jt][v];
                         if (al == be && i == j) val += sm.kdu[i];
-                        bad |= !isfinite(val);
+                        chk = fma(val, 0.0, chk);
                         rec[i * RC::SIZE + RC::KU + (al * DIM + be) * N + j] = val;
                     } else if (be == 0 && j == FCOL) {
                         double val = acc[be][mt][jt][v] + sm.fnu[al][i];
-                        bad |= !isfinite(val);
+                        chk = fma(val, 0.0, chk);
                         rec[i * RC::SIZE + RC::FU + al] = val;
                     }
                 }
             }
 }
 
-// K_pp = sum_ip phA N N^T + phB G^T G and K_TT = sum_ip thB G^T G on MFMA in one wavefront,
-// contracted over k = (q, ip): q = 0 -> N, q = 1.. -> d/dx_{q-1}, q outermost.  The padding
-// column j = N carries F_p = sum_ip phFN N + phFG . G and F_T = sum_ip thFN N + thFG . G.
-template <int DIM>
-__device__ void kpt_mfma(const Smem<DIM>& sm, double* rec, const int lane, const bool dph, const bool dT,
-                         const bool dth, unsigned& bad) {
-    using E = ET<DIM>;
-    using RC = Rec<DIM>;
-    constexpr int N = E::N, NIP = E::NIP, MT = (N + 15) / 16, KSTEP = (NIP + 3) / 4, FCOL = N;
-    const int lk = lane >> 4, l16 = lane & 15;
-    dbl4 accp[MT][MT], acct[MT][MT];
+// K_pp = sum_ip phA N N^T + phB G^T G (acc[0]) and K_TT = sum_ip thB G^T G (acc[1]) in one
+// wavefront, contracted over k = (q, ip): q = 0 -> N, q = 1.. -> d/dx_{q-1}, q outermost.
+// The padding column j = N carries F_p = sum_ip phFN N + phFG . G and F_T = sum_ip thFN N +
+// thFG . G.
+template <int DIM, int MT>
+__device__ __forceinline__ void kpt_loop(const Smem<DIM>& sm, const int lk, const int l16, const bool dph,
+                                         const bool dT, dbl4 (&acc)[DIM][MT][MT]) {
+    using SM = Smem<DIM>;
+    constexpr int N = SM::N, IPP = SM::IPP, KSTEP = IPP / 4, FCOL = N;
 #pragma unroll
-    for (int mt = 0; mt < MT; mt++)
+    for (int x = 0; x < DIM; x++)
 #pragma unroll
-        for (int nt = 0; nt < MT; nt++) {
-            accp[mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
-            acct[mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
-        }
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int nt = 0; nt < MT; nt++) acc[x][mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q <= DIM; q++) {
+        const int qg = q > 0 ? q - 1 : 0;
+        const double* const cpr = &sm.ipc[q == 0 ? IPB_PHA : IPB_PHB][0];
+        const double* const fpr = &sm.ipc[q == 0 ? IPB_PHFN : IPB_PHFG + qg][0];
+        const double* const ctr = &sm.ipc[IPB_THB][0];
+        const double* const ftr = &sm.ipc[q == 0 ? IPB_THFN : IPB_THFG + qg][0];
 #pragma unroll 1
         for (int s = 0; s < KSTEP; s++) {
-            const int ip0 = s * 4 + lk;
-            const bool ipv = ip0 < NIP;
-            const int ip = ipv ? ip0 : NIP - 1;
-            const double* X = q == 0 ? &sm.Nsh[ip][0] : &sm.G[ip][q > 0 ? q - 1 : 0][0];
+            const int ip = s * 4 + lk;
+            const double* const X = q == 0 ? &sm.Nsh[ip][0] : &sm.G[ip][qg][0];
             double a[MT];
 #pragma unroll
-            for (int mt = 0; mt < MT; mt++) {
-                const int i = mt * 16 + l16;
-                const double x = X[i < N ? i : N - 1];
-                a[mt] = (ipv && i < N) ? x : 0.0;
-            }
-            const double cp = q == 0 ? sm.phA[ip] : sm.phB[ip];
-            const double fp = q == 0 ? sm.phFN[ip] : sm.phFG[ip][q > 0 ? q - 1 : 0];
-            const double ct = q == 0 ? 0.0 : sm.thB[ip];
-            const double ft = q == 0 ? sm.thFN[ip] : sm.thFG[ip][q > 0 ? q - 1 : 0];
+            for (int mt = 0; mt < MT; mt++) a[mt] = X[mt * 16 + l16];
+            const double cp = cpr[ip], fp = fpr[ip], ct = q == 0 ? 0.0 : ctr[ip], ft = ftr[ip];
 #pragma unroll
             for (int nt = 0; nt < MT; nt++) {
                 const int j = nt * 16 + l16;
-                const double x = X[j < N ? j : N - 1];
-                double bp = j < N ? cp * x : (j == FCOL ? fp : 0.0);
-                double bt = j < N ? ct * x : (j == FCOL ? ft : 0.0);
-                bp = ipv ? bp : 0.0;
-                bt = ipv ? bt : 0.0;
+                const double x = X[j];
+                double bp = cp * x, bt = ct * x;
+                if (nt == MT - 1) {
+                    bp = j == FCOL ? fp : bp;
+                    bt = j == FCOL ? ft : bt;
+                }
 #pragma unroll
                 for (int mt = 0; mt < MT; mt++) {
-                    if (dph) accp[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bp, accp[mt][nt], 0, 0, 0);
+                    if (dph) acc[0][mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bp, acc[0][mt][nt], 0, 0, 0);
                     // q = 0 feeds K_TT nothing: only the tile holding the residual column
                     if (dT && (q > 0 || nt == MT - 1))
-                        acct[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bt, acct[mt][nt], 0, 0, 0);
+                        acc[1][mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bt, acc[1][mt][nt], 0, 0, 0);
                 }
             }
         }
     }
+}
+
+template <int DIM, int MT>
+__device__ __forceinline__ void kpt_store(const Smem<DIM>& sm, double* rec, const int lk, const int l16, const bool dph,
+                                          const bool dT, const bool dth, const dbl4 (&acc)[DIM][MT][MT], double& chk) {
+    using RC = Rec<DIM>;
+    constexpr int N = Smem<DIM>::N, FCOL = N;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
@@ -904,24 +914,24 @@ __device__ void kpt_mfma(const Smem<DIM>& sm, double* rec, const int lane, const
                 if (i >= N) continue;
                 if (j < N) {
                     if (dph) {
-                        double val = accp[mt][nt][v] + (i == j ? sm.kdp[i] : 0.0);
-                        bad |= !isfinite(val);
+                        double val = acc[0][mt][nt][v] + (i == j ? sm.kdp[i] : 0.0);
+                        chk = fma(val, 0.0, chk);
                         rec[i * RC::SIZE + RC::KP + j] = val;
                     }
                     if (dth) {
-                        double val = acct[mt][nt][v] + (i == j ? sm.kdt[i] : 0.0);
-                        bad |= !isfinite(val);
+                        double val = acc[1][mt][nt][v] + (i == j ? sm.kdt[i] : 0.0);
+                        chk = fma(val, 0.0, chk);
                         rec[i * RC::SIZE + RC::KT + j] = val;
                     }
                 } else if (j == FCOL) {
                     if (dph) {
-                        double val = accp[mt][nt][v] + sm.fnp[i];
-                        bad |= !isfinite(val);
+                        double val = acc[0][mt][nt][v] + sm.fnp[i];
+                        chk = fma(val, 0.0, chk);
                         rec[i * RC::SIZE + RC::FP] = val;
                     }
                     if (dT) {
-                        double val = acct[mt][nt][v] + sm.fnt[i];
-                        bad |= !isfinite(val);
+                        double val = acc[1][mt][nt][v] + sm.fnt[i];
+                        chk = fma(val, 0.0, chk);
                         rec[i * RC::SIZE + RC::FT] = val;
                     }
                 }
@@ -929,23 +939,84 @@ __device__ void kpt_mfma(const Smem<DIM>& sm, double* rec, const int lane, const
         }
 }
 
+// Nodal (lumped) terms of node a: lumped weight WL = sum_ip w N_a, inertia, damping and
+// gravity (PhaseField.cpp:436-477), lumped driving / capacity, phase viscosity and
+// stabilisation (PhaseField.cpp:693-711), heat capacity (ThermalModel.cpp:128-129).
+template <int DIM>
+__device__ __forceinline__ void nodal_terms(Smem<DIM>& sm, const KParams& p, const int a, const bool du, const bool dph,
+                                            const bool dth) {
+    constexpr int NIP = Smem<DIM>::NIP;
+    const MatConst& mc = p.mc;
+    double s = 0;
+    for (int ip = 0; ip < NIP; ip++) s += sm.ipc[IPB_W][ip] * sm.Nsh[ip][a];
+    if (du) {
+        double DamI, DamG, d1, d2;
+        damage(mc.DensityDamage, sm.Ph[a], mc.k0, mc.p, DamI, d1, d2);
+        damage(mc.GravityDamage, sm.Ph[a], mc.k0, mc.p, DamG, d1, d2);
+        double kd = 0;
+        if (mc.Inertia) kd += s * DamI * mc.Density * p.ddu_dt;
+        if (mc.Damping > 0.0) kd += s * mc.Damping * p.du_dt;
+        sm.kdu[a] = kd;
+        for (int al = 0; al < DIM; al++) {
+            double f = 0;
+            if (mc.Inertia) f += s * DamI * mc.Density * sm.ddU[al][a];
+            if (mc.Damping > 0.0) f += s * mc.Damping * sm.dU[al][a];
+            f += -s * DamG * mc.Density * mc.g[al];
+            sm.fnu[al][a] = f;
+        }
+    }
+    if (dph) {
+        const double GcL = sm.ipc[IPB_GC][NIP - 1];
+        double kd = 0, f = 0;
+        if (mc.LumpedDrive) {
+            double ld = 0;
+            for (int ip = 0; ip < NIP; ip++) ld += sm.ipc[IPB_LDRIVE][ip] * sm.Nsh[ip][a];
+            double D0, dD0, ddD0;
+            damage(mc.DamFunction, sm.Ph[a], mc.k0, mc.p, D0, dD0, ddD0);
+            kd += ld * ddD0;
+            f += ld * dD0;
+        }
+        if (mc.LumpedCap) {
+            double u1, u2, LD, dLD;
+            cap_coeffs(mc.Distribution, mc.l, sm.Ph[a], u1, u2, LD, dLD);
+            kd += s * GcL * dLD;
+            f += s * GcL * LD;
+        }
+        kd += s * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
+        f += s * mc.pf_visc / p.dt * (sm.Ph[a] - sm.PhO[a]);
+        sm.kdp[a] = kd;
+        sm.fnp[a] = f;
+    }
+    sm.kdt[a] = dth ? s * mc.Density_th * mc.cp * p.du_dt : 0.0;
+    sm.fnt[a] = dth ? s * mc.Density_th * mc.cp * sm.dT[a] : 0.0;
+}
+
 template <int DIM>
 __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
     using E = ET<DIM>;
-    constexpr int N = E::N, NIP = E::NIP, NU = E::NU;
-    __shared__ Smem<DIM> sm;
+    using SM = Smem<DIM>;
+    using RC = Rec<DIM>;
+    constexpr int N = E::N, NIP = E::NIP, NP = SM::NP, IPP = SM::IPP, MT = NP / 16;
+    __shared__ SM sm;
     const int tid = threadIdx.x;
     const int64_t el = p.elist ? (int64_t)p.elist[blockIdx.x] : (int64_t)blockIdx.x;
-    using RC = Rec<DIM>;
     double* const rec = p.ebuf + el * N * RC::SIZE;  // this element's records
-    const MatConst& mc = p.mc;
-    const int64_t S = p.NIPT, gi0 = el * NIP;
+    // per-IP coefficients of this element: one contiguous [field][ip] block, loads issued first
+    constexpr int IPC_NV = IPB_NF * NIP, IPC_PER = (IPC_NV + NT - 1) / NT;
+    double ipv[IPC_PER];
+    {
+        const double* const src = p.ipb + el * IPC_NV;
+#pragma unroll
+        for (int k = 0; k < IPC_PER; k++) {
+            const int t = tid + k * NT;
+            ipv[k] = t < IPC_NV ? src[t] : 0.0;
+        }
+    }
 
     // ---- gather -------------------------------------------------------------------
     if (tid < N) {
         int v = p.enode[el * N + tid];
         int r = p.erank[el * N + tid];
-        sm.rank[tid] = r;
         const double* nd = p.nodes + (int64_t)v * 4;
         sm.X[tid][0] = nd[0]; sm.X[tid][1] = nd[1]; sm.X[tid][2] = nd[2]; sm.X[tid][3] = nd[3];
         if (p.has_pf) {
@@ -960,7 +1031,6 @@ __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
         }
         if (p.has_T) sm.dT[tid] = p.sT.dx[(int64_t)p.fpos_T * p.Ng + r];
     }
-    if (tid == 0) sm.nonfinite = 0;
     if (tid < DIM * 9) {
         int d = tid / 9, q = (tid / 3) % 3, a = tid % 3;
         const double* C = p.ext + (el * DIM + d) * 9;
@@ -969,199 +1039,80 @@ __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
         sm.shp.b1[d][q][a] = s;
         sm.shp.db1[d][q][a] = sd;
     }
-    // per-IP coefficients of this element (contiguous per field)
-    if (p.do_u) {
-        for (int i = tid; i < NIP * 6; i += NT) sm.sig[i / 6][i % 6] = p.ipb[(int64_t)(IPB_SIG + i % 6) * S + gi0 + i / 6];
-        for (int i = tid; i < NIP * 36; i += NT)
-            (&sm.Mt[0][0][0])[(i % NIP) * 36 + i / NIP] = p.ipb[(int64_t)(IPB_M + i / NIP) * S + gi0 + i % NIP];
+#pragma unroll
+    for (int k = 0; k < IPC_PER; k++) {
+        const int t = tid + k * NT;
+        if (t < IPC_NV) sm.ipc[t / NIP][t % NIP] = ipv[k];
     }
-    if (p.do_ph)
-        for (int i = tid; i < NIP; i += NT) {
-            sm.phA[i] = p.ipb[(int64_t)IPB_PHA * S + gi0 + i];
-            sm.phB[i] = p.ipb[(int64_t)IPB_PHB * S + gi0 + i];
-            sm.phFN[i] = p.ipb[(int64_t)IPB_PHFN * S + gi0 + i];
-            for (int c = 0; c < DIM; c++) sm.phFG[i][c] = p.ipb[(int64_t)(IPB_PHFG + c) * S + gi0 + i];
-            sm.ldrive[i] = p.ipb[(int64_t)IPB_LDRIVE * S + gi0 + i];
-            sm.gw[i] = p.ipb[(int64_t)IPB_GW * S + gi0 + i];
-            if (i == NIP - 1) sm.GcLast = p.ipb[(int64_t)IPB_GC * S + gi0 + i];
-        }
-    if (p.do_Tpf || p.do_th)
-        for (int i = tid; i < NIP; i += NT) {
-            sm.thB[i] = p.ipb[(int64_t)IPB_THB * S + gi0 + i];
-            sm.thFN[i] = p.ipb[(int64_t)IPB_THFN * S + gi0 + i];
-            for (int c = 0; c < DIM; c++) sm.thFG[i][c] = p.ipb[(int64_t)(IPB_THFG + c) * S + gi0 + i];
-        }
+    for (int t = tid; t < IPB_NF * (IPP - NIP); t += NT) sm.ipc[t / (IPP - NIP)][NIP + t % (IPP - NIP)] = 0.0;
     __syncthreads();
 
-    // ---- shape functions (CB, GB via the tensor-product extraction) ---------------
-    for (int i = tid; i < NIP * N; i += NT) {
-        int ip = i / N, a = i % N;
-        int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9;
-        int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
-        double bx = sm.shp.b1[0][qx][ax], by = sm.shp.b1[1][qy][ay];
-        double gx = sm.shp.db1[0][qx][ax], gy = sm.shp.db1[1][qy][ay];
+    // ---- shape functions ------------------------------------------------------------
+    // One elementwise pass over (ip, node): CB, GB from the tensor-product 1D factors, the
+    // rational basis N = W CB / wFun, GXi = W (GB / wFun - CB dW / wFun^2) (3D,
+    // NURBS3_Cube2.cpp:96-112) and G = J^-1 GXi (B2_2.cpp:79-85 in 2D: N = CB, GXi = GB,
+    // J^-1 in the transposed form), with J^-1 / wFun, 1 / wFun and J^-1 dW / wFun^2 taken
+    // from the material kernel, which forms them at the same IP.  Padding entries are zeroed.
+    for (int i = tid; i < IPP * NP; i += NT) {
+        const int ip = i / NP, a = i % NP;
+        if (ip >= NIP || a >= N) {
+            sm.Nsh[ip][a] = 0.0;
+            for (int c = 0; c < DIM; c++) sm.G[ip][c][a] = 0.0;
+            continue;
+        }
+        const int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9;
+        const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+        const double bx = sm.shp.b1[0][qx][ax], by = sm.shp.b1[1][qy][ay];
+        const double gx = sm.shp.db1[0][qx][ax], gy = sm.shp.db1[1][qy][ay];
+        double cb, gb[DIM];
         if (DIM == 2) {
-            sm.Nsh[ip][a] = bx * by;
-            sm.G[ip][0][a] = gx * by;
-            sm.G[ip][1][a] = bx * gy;
+            cb = bx * by;
+            gb[0] = gx * by;
+            gb[1] = bx * gy;
         } else {
-            double bz = sm.shp.b1[DIM - 1][qz][az], gz = sm.shp.db1[DIM - 1][qz][az];
-            sm.Nsh[ip][a] = bx * by * bz;
-            sm.G[ip][0][a] = gx * by * bz;
-            sm.G[ip][1][a] = bx * gy * bz;
-            sm.G[ip][DIM - 1][a] = bx * by * gz;
+            const double bz = sm.shp.b1[DIM - 1][qz][az], gz = sm.shp.db1[DIM - 1][qz][az];
+            cb = bx * by * bz;
+            gb[0] = gx * by * bz;
+            gb[1] = bx * gy * bz;
+            gb[DIM - 1] = bx * by * gz;
         }
-    }
-    __syncthreads();
-    if (DIM == 3) {
-        if (tid < NIP * 4) {
-            int ip = tid / 4, q = tid % 4;
-            double s = 0;
-            if (q == 0) for (int a = 0; a < N; a++) s += sm.Nsh[ip][a] * sm.X[a][3];
-            else for (int a = 0; a < N; a++) s += sm.G[ip][q - 1][a] * sm.X[a][3];
-            sm.shp.red[ip][q] = s;
-        }
-        __syncthreads();
-        for (int i = tid; i < NIP * N; i += NT) {
-            int ip = i / N, a = i % N;
-            double wF = sm.shp.red[ip][0], Wa = sm.X[a][3], cb = sm.Nsh[ip][a];
-            sm.Nsh[ip][a] = Wa * cb / wF;
-            for (int c = 0; c < DIM; c++) sm.G[ip][c][a] = Wa * (sm.G[ip][c][a] / wF - cb * sm.shp.red[ip][c + 1] / wF / wF);
-        }
-        __syncthreads();
-    }
-    if (tid < NIP * DIM * DIM) {
-        int ip = tid / (DIM * DIM), c = (tid / DIM) % DIM, d = tid % DIM;
-        double s = 0;
-        for (int a = 0; a < N; a++) s += sm.G[ip][c][a] * sm.X[a][d];
-        sm.shp.Jinv[ip][c][d] = s;
-    }
-    __syncthreads();
-    if (tid < NIP) {
-        int ip = tid;
-        if (DIM == 3) {
-            double J[3][3];
-            for (int c = 0; c < 3; c++)
-                for (int d = 0; d < 3; d++) J[c][d] = sm.shp.Jinv[ip][c][d];
-            double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-            double c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-            double c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-            double det = J[0][0] * c00 + J[0][1] * c10 + J[0][2] * c20;
-            double id = 1.0 / det;
-            double Ji[3][3];
-            Ji[0][0] = c00 * id; Ji[1][0] = c10 * id; Ji[2][0] = c20 * id;
-            Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-            Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-            Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-            Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-            Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-            Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-            double dt3 = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[1][0] * (J[0][1] * J[2][2] - J[0][2] * J[2][1]) +
-                         J[2][0] * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
-            int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9;
-            sm.w[ip] = p.wq[qx] * p.wq[qy] * p.wq[qz] * fabs(dt3);
-            for (int c = 0; c < 3; c++)
-                for (int d = 0; d < 3; d++) sm.shp.Jinv[ip][c][d] = Ji[c][d];
-        } else {
-            double J[2][2] = {{sm.shp.Jinv[ip][0][0], sm.shp.Jinv[ip][1][0]}, {sm.shp.Jinv[ip][0][1], sm.shp.Jinv[ip][1][1]}};
-            double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-            double id = 1.0 / det;
-            int qx = ip % 3, qy = ip / 3;
-            sm.w[ip] = p.wq[qx] * p.wq[qy] * fabs(det);
-            sm.shp.Jinv[ip][0][0] = J[1][1] * id;
-            sm.shp.Jinv[ip][1][0] = -J[1][0] * id;
-            sm.shp.Jinv[ip][0][1] = -J[0][1] * id;
-            sm.shp.Jinv[ip][1][1] = J[0][0] * id;
-        }
-    }
-    __syncthreads();
-    for (int i = tid; i < NIP * N; i += NT) {
-        int ip = i / N, a = i % N;
-        double g[DIM];
-        for (int c = 0; c < DIM; c++) g[c] = sm.G[ip][c][a];
+        const double Wa = DIM == 3 ? sm.X[a][3] : 1.0;
+        sm.Nsh[ip][a] = DIM == 3 ? Wa * cb * sm.ipc[IPB_WF][ip] : cb;
         for (int c = 0; c < DIM; c++) {
             double s = 0;
-            for (int d = 0; d < DIM; d++) s += sm.shp.Jinv[ip][c][d] * g[d];
-            sm.G[ip][c][a] = s;
+            for (int d = 0; d < DIM; d++) s += sm.ipc[IPB_JI + c * DIM + d][ip] * gb[d];
+            sm.G[ip][c][a] = DIM == 3 ? Wa * (s - cb * sm.ipc[IPB_DW + c][ip]) : s;
         }
-    }
-    __syncthreads();
-    if (p.has_pf && p.do_ph && tid == 0) {
-        double s = 0;
-        for (int ip = 0; ip < NIP; ip++) s += sm.gw[ip];
-        p.lcrack_elem[el] = s;
-    }
-    // lumped weights (the same for the u, phase and T groups: one element group) and the
-    // nodal (lumped) tangent and residual terms
-    if (tid < N) {
-        const int a = tid;
-        double s = 0;
-        for (int ip = 0; ip < NIP; ip++) s += sm.w[ip] * sm.Nsh[ip][a];
-        sm.WL[a] = s;
-        if (p.has_pf && p.do_u) {
-            // inertia, damping and gravity (PhaseField.cpp:436-477)
-            double DamI, DamG, d1, d2;
-            damage(mc.DensityDamage, sm.Ph[a], mc.k0, mc.p, DamI, d1, d2);
-            damage(mc.GravityDamage, sm.Ph[a], mc.k0, mc.p, DamG, d1, d2);
-            double kd = 0;
-            if (mc.Inertia) kd += s * DamI * mc.Density * p.ddu_dt;
-            if (mc.Damping > 0.0) kd += s * mc.Damping * p.du_dt;
-            sm.kdu[a] = kd;
-            for (int al = 0; al < DIM; al++) {
-                double f = 0;
-                if (mc.Inertia) f += s * DamI * mc.Density * sm.ddU[al][a];
-                if (mc.Damping > 0.0) f += s * mc.Damping * sm.dU[al][a];
-                f += -s * DamG * mc.Density * mc.g[al];
-                sm.fnu[al][a] = f;
-            }
-        }
-        if (p.has_pf && p.do_ph) {
-            // lumped driving / capacity, phase viscosity and stabilisation (PhaseField.cpp:693-711)
-            const double GcL = sm.GcLast;
-            double kd = 0, f = 0;
-            if (mc.LumpedDrive) {
-                double ld = 0;
-                for (int ip = 0; ip < NIP; ip++) ld += sm.ldrive[ip] * sm.Nsh[ip][a];
-                double D0, dD0, ddD0;
-                damage(mc.DamFunction, sm.Ph[a], mc.k0, mc.p, D0, dD0, ddD0);
-                kd += ld * ddD0;
-                f += ld * dD0;
-            }
-            if (mc.LumpedCap) {
-                double u1, u2, LD, dLD;
-                cap_coeffs(mc.Distribution, mc.l, sm.Ph[a], u1, u2, LD, dLD);
-                kd += s * GcL * dLD;
-                f += s * GcL * LD;
-            }
-            kd += s * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
-            f += s * mc.pf_visc / p.dt * (sm.Ph[a] - sm.PhO[a]);
-            sm.kdp[a] = kd;
-            sm.fnp[a] = f;
-        }
-        // heat capacity (ThermalModel.cpp:128-129)
-        sm.kdt[a] = p.do_th ? s * mc.Density_th * mc.cp * p.du_dt : 0.0;
-        sm.fnt[a] = p.do_th ? s * mc.Density_th * mc.cp * sm.dT[a] : 0.0;
     }
     __syncthreads();
 
-    // ---- tangents and residuals: waves 0..DIM-1 K_UU/F_U (alpha = wave), wave DIM the
-    // phase and T steps ---------------------------------------------------------------
-    {
-        const int wv = tid >> 6, lane = tid & 63;
-        const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
-        unsigned bad = 0;
-        if (wv < DIM) {
-            if (du) kuu_mfma<DIM>(sm, rec, wv, lane, bad);
-        } else if (wv == DIM) {
-            if (dph || dT) kpt_mfma<DIM>(sm, rec, lane, dph, dT, p.do_th, bad);
+    // ---- tangents and residuals ----------------------------------------------------
+    // Waves 0..DIM-1: K_UU/F_U of alpha = wave.  Wave DIM: the nodal terms, then K_pp/F_p
+    // and K_TT/F_T.  Accumulators stay in registers across the barrier that publishes the
+    // nodal terms to every wave's write-back.
+    const int wv = tid >> 6, lane = tid & 63, lk = lane >> 4, l16 = lane & 15;
+    const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
+    dbl4 acc[DIM][MT][MT];  // zeroed by the contraction loops
+    if (wv < DIM) {
+        if (du) kuu_loop<DIM, MT>(sm, wv, lk, l16, acc);
+    } else if (wv == DIM) {
+        if (lane < N) nodal_terms<DIM>(sm, p, lane, du, dph, p.do_th);
+        if (dph && lane == 0) {
+            double s = 0;
+            for (int ip = 0; ip < NIP; ip++) s += sm.ipc[IPB_GW][ip];
+            p.lcrack_elem[el] = s;
         }
-        if (bad) atomicOr(&sm.nonfinite, 1u);
+        if (dph || dT) kpt_loop<DIM, MT>(sm, lk, l16, dph, dT, acc);
     }
     __syncthreads();
-    if (tid == 0) {
-        atomicAdd(&p.st->elements, 1ull);
-        if (sm.nonfinite) atomicOr(&p.st->nonfinite, 1u);
+    double chk = 0.0;  // stays 0 unless a stored value is NaN or infinite
+    if (wv < DIM) {
+        if (du) kuu_store<DIM, MT>(sm, rec, wv, lk, l16, acc, chk);
+    } else if (wv == DIM) {
+        if (dph || dT) kpt_store<DIM, MT>(sm, rec, lk, l16, dph, dT, p.do_th, acc, chk);
     }
+    // no closing barrier: a workgroup retires as soon as its waves have issued their stores
+    if (chk != 0.0) atomicOr(&p.st->nonfinite, 1u);
 }
 
 // ---------------------------------------------------------------------------------------

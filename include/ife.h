@@ -159,7 +159,7 @@ typedef struct {
 
 /* Per-launch report (kernel status word + cheap reductions). */
 typedef struct {
-    int64_t elements_assembled;
+    int64_t elements_assembled;     /* elements whose IPs the material pass updated */
     int64_t return_map_calls;
     int64_t return_map_iterations;  /* sum over IPs: mean = iterations / calls */
     int64_t return_map_maxit_hits;  /* reference logs these (SolidMaterial.cpp:246-250) */
