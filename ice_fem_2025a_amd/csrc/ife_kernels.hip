@@ -740,14 +740,14 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
 // Element kernel: one workgroup per element.  Shape functions in LDS, per-IP coefficients
 // from the material kernel, MFMA contractions, node-row records out.
 // ---------------------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, typename T>
 struct Smem {
     static constexpr int N = ET<DIM>::N, NIP = ET<DIM>::NIP;
     static constexpr int NP = (N + 15) / 16 * 16;    // nodes padded to whole 16-wide tiles
     static constexpr int IPP = (NIP + 3) / 4 * 4;    // IPs padded to whole 4-deep k steps
     double X[N][4];            // x, y, z, W
-    double Nsh[IPP][NP];       // N, zero beyond (NIP, N)
-    double G[IPP][DIM][NP];    // dN/dx, zero beyond (NIP, N)
+    T Nsh[IPP][NP];            // N, zero beyond (NIP, N)
+    T G[IPP][DIM][NP];         // dN/dx, zero beyond (NIP, N)
     struct {
         double b1[DIM][3][3];      // 1D B-spline values  [dir][gauss pt][fn]
         double db1[DIM][3][3];     // 1D derivatives
@@ -760,9 +760,25 @@ struct Smem {
 };
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
-// MFMA operand layout of v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16 k, B[k][j] from
-// lane j + 16 k, D[i][j] in lane j + 16 (i % 4), register i / 4.
+// 16x16x4 MFMA of the tile type.  Operands: A[i][k] from lane i + 16 k, B[k][j] from lane
+// j + 16 k.  Result D[i][j]: FP64 (v_mfma_f64_16x16x4f64) in lane j + 16 (i % 4), register
+// i / 4; FP32 (v_mfma_f32_16x16x4f32) in lane j + 16 (i / 4), register i % 4.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef dbl4 V;
+    static __device__ __forceinline__ V op(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+    typedef flt4 V;
+    static __device__ __forceinline__ V op(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
 
 // K_UU block row of displacement component alpha = sum_ip B^T Mt B.  K[(alpha,i)][(beta,j)] =
 // sum_{c,ip} G[ip][c][i] * Y_c[ip][(beta,j)] with Y_c[ip][(beta,j)] =
@@ -770,17 +786,18 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // Voigt row vrow(alpha,c) is uniform over the wavefront) and the IPs in steps of four.  Each
 // beta block is MT x MT 16x16 tiles.  The first padding column j = N of beta block 0 carries
 // F_U[(alpha,i)] = sum_{c,ip} G[ip][c][i] sig[ip][vrow(alpha,c)] through the same A operand.
-template <int DIM, int MT>
-__device__ __forceinline__ void kuu_loop(const Smem<DIM>& sm, const int al, const int lk, const int l16,
-                                         dbl4 (&acc)[DIM][MT][MT]) {
-    using SM = Smem<DIM>;
+template <int DIM, int MT, typename T>
+__device__ __forceinline__ void kuu_loop(const Smem<DIM, T>& sm, const int al, const int lk, const int l16,
+                                         typename Mfma<T>::V (&acc)[DIM][MT][MT]) {
+    using SM = Smem<DIM, T>;
+    typedef typename Mfma<T>::V V;
     constexpr int N = SM::N, IPP = SM::IPP, KSTEP = IPP / 4, FCOL = N;
 #pragma unroll
     for (int x = 0; x < DIM; x++)
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-            for (int nt = 0; nt < MT; nt++) acc[x][mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
+            for (int nt = 0; nt < MT; nt++) acc[x][mt][nt] = V{0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
         const int r = vrow<DIM>(al, c);
@@ -789,39 +806,38 @@ __device__ __forceinline__ void kuu_loop(const Smem<DIM>& sm, const int al, cons
 #pragma unroll 1
         for (int s = 0; s < KSTEP; s++) {
             const int ip = s * 4 + lk;
-            double a[MT];
+            T a[MT];
 #pragma unroll
             for (int mt = 0; mt < MT; mt++) a[mt] = sm.G[ip][c][mt * 16 + l16];
-            double m[6];
+            T m[6];
 #pragma unroll
-            for (int q = 0; q < 6; q++) m[q] = Mrow[q * IPP + ip];
-            const double fs = Srow[ip];
+            for (int q = 0; q < 6; q++) m[q] = (T)Mrow[q * IPP + ip];
+            const T fs = (T)Srow[ip];
 #pragma unroll
             for (int jt = 0; jt < MT; jt++) {
                 const int j = jt * 16 + l16;
-                double g[DIM];
+                T g[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; d++) g[d] = sm.G[ip][d][j];
 #pragma unroll
                 for (int be = 0; be < DIM; be++) {
-                    double bb = 0.0;
+                    T bb = 0;
 #pragma unroll
                     for (int d = 0; d < DIM; d++) bb += m[vrow<DIM>(be, d)] * g[d];
                     if (be == 0 && jt == MT - 1) bb = j == FCOL ? fs : bb;
 #pragma unroll
-                    for (int mt = 0; mt < MT; mt++)
-                        acc[be][mt][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bb, acc[be][mt][jt], 0, 0, 0);
+                    for (int mt = 0; mt < MT; mt++) acc[be][mt][jt] = Mfma<T>::op(a[mt], bb, acc[be][mt][jt]);
                 }
             }
         }
     }
 }
 
-template <int DIM, int MT>
-__device__ __forceinline__ void kuu_store(const Smem<DIM>& sm, double* rec, const int al, const int lk, const int l16,
-                                          const dbl4 (&acc)[DIM][MT][MT], double& chk) {
+template <int DIM, int MT, typename T, typename R>
+__device__ __forceinline__ void kuu_store(const Smem<DIM, T>& sm, R* rec, const int al, const int lk, const int l16,
+                                          const typename Mfma<T>::V (&acc)[DIM][MT][MT], double& chk) {
     using RC = Rec<DIM>;
-    constexpr int N = Smem<DIM>::N, FCOL = N;
+    constexpr int N = Smem<DIM, T>::N, FCOL = N;
 #pragma unroll
     for (int be = 0; be < DIM; be++)
 #pragma unroll
@@ -831,17 +847,17 @@ __device__ __forceinline__ void kuu_store(const Smem<DIM>& sm, double* rec, cons
                 const int j = jt * 16 + l16;
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int i = mt * 16 + lk + 4 * v;
+                    const int i = mt * 16 + Mfma<T>::row(lk, v);
                     if (i >= N) continue;
                     if (j < N) {
                         double val = acc[be][mt][jt][v];
                         if (al == be && i == j) val += sm.kdu[i];
                         chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::KU + (al * DIM + be) * N + j] = val;
+                        rec[i * RC::SIZE + RC::KU + (al * DIM + be) * N + j] = (R)val;
                     } else if (be == 0 && j == FCOL) {
-                        double val = acc[be][mt][jt][v] + sm.fnu[al][i];
+                        double val = (double)acc[be][mt][jt][v] + sm.fnu[al][i];
                         chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::FU + al] = val;
+                        rec[i * RC::SIZE + RC::FU + al] = (R)val;
                     }
                 }
             }
@@ -851,17 +867,18 @@ __device__ __forceinline__ void kuu_store(const Smem<DIM>& sm, double* rec, cons
 // wavefront, contracted over k = (q, ip): q = 0 -> N, q = 1.. -> d/dx_{q-1}, q outermost.
 // The padding column j = N carries F_p = sum_ip phFN N + phFG . G and F_T = sum_ip thFN N +
 // thFG . G.
-template <int DIM, int MT>
-__device__ __forceinline__ void kpt_loop(const Smem<DIM>& sm, const int lk, const int l16, const bool dph,
-                                         const bool dT, dbl4 (&acc)[DIM][MT][MT]) {
-    using SM = Smem<DIM>;
+template <int DIM, int MT, typename T>
+__device__ __forceinline__ void kpt_loop(const Smem<DIM, T>& sm, const int lk, const int l16, const bool dph,
+                                         const bool dT, typename Mfma<T>::V (&acc)[DIM][MT][MT]) {
+    using SM = Smem<DIM, T>;
+    typedef typename Mfma<T>::V V;
     constexpr int N = SM::N, IPP = SM::IPP, KSTEP = IPP / 4, FCOL = N;
 #pragma unroll
     for (int x = 0; x < DIM; x++)
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-            for (int nt = 0; nt < MT; nt++) acc[x][mt][nt] = dbl4{0.0, 0.0, 0.0, 0.0};
+            for (int nt = 0; nt < MT; nt++) acc[x][mt][nt] = V{0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q <= DIM; q++) {
         const int qg = q > 0 ? q - 1 : 0;
@@ -872,37 +889,37 @@ __device__ __forceinline__ void kpt_loop(const Smem<DIM>& sm, const int lk, cons
 #pragma unroll 1
         for (int s = 0; s < KSTEP; s++) {
             const int ip = s * 4 + lk;
-            const double* const X = q == 0 ? &sm.Nsh[ip][0] : &sm.G[ip][qg][0];
-            double a[MT];
+            const T* const X = q == 0 ? &sm.Nsh[ip][0] : &sm.G[ip][qg][0];
+            T a[MT];
 #pragma unroll
             for (int mt = 0; mt < MT; mt++) a[mt] = X[mt * 16 + l16];
-            const double cp = cpr[ip], fp = fpr[ip], ct = q == 0 ? 0.0 : ctr[ip], ft = ftr[ip];
+            const T cp = (T)cpr[ip], fp = (T)fpr[ip], ct = q == 0 ? (T)0 : (T)ctr[ip], ft = (T)ftr[ip];
 #pragma unroll
             for (int nt = 0; nt < MT; nt++) {
                 const int j = nt * 16 + l16;
-                const double x = X[j];
-                double bp = cp * x, bt = ct * x;
+                const T x = X[j];
+                T bp = cp * x, bt = ct * x;
                 if (nt == MT - 1) {
                     bp = j == FCOL ? fp : bp;
                     bt = j == FCOL ? ft : bt;
                 }
 #pragma unroll
                 for (int mt = 0; mt < MT; mt++) {
-                    if (dph) acc[0][mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bp, acc[0][mt][nt], 0, 0, 0);
+                    if (dph) acc[0][mt][nt] = Mfma<T>::op(a[mt], bp, acc[0][mt][nt]);
                     // q = 0 feeds K_TT nothing: only the tile holding the residual column
-                    if (dT && (q > 0 || nt == MT - 1))
-                        acc[1][mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[mt], bt, acc[1][mt][nt], 0, 0, 0);
+                    if (dT && (q > 0 || nt == MT - 1)) acc[1][mt][nt] = Mfma<T>::op(a[mt], bt, acc[1][mt][nt]);
                 }
             }
         }
     }
 }
 
-template <int DIM, int MT>
-__device__ __forceinline__ void kpt_store(const Smem<DIM>& sm, double* rec, const int lk, const int l16, const bool dph,
-                                          const bool dT, const bool dth, const dbl4 (&acc)[DIM][MT][MT], double& chk) {
+template <int DIM, int MT, typename T, typename R>
+__device__ __forceinline__ void kpt_store(const Smem<DIM, T>& sm, R* rec, const int lk, const int l16, const bool dph,
+                                          const bool dT, const bool dth, const typename Mfma<T>::V (&acc)[DIM][MT][MT],
+                                          double& chk) {
     using RC = Rec<DIM>;
-    constexpr int N = Smem<DIM>::N, FCOL = N;
+    constexpr int N = Smem<DIM, T>::N, FCOL = N;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
@@ -910,29 +927,29 @@ __device__ __forceinline__ void kpt_store(const Smem<DIM>& sm, double* rec, cons
             const int j = nt * 16 + l16;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int i = mt * 16 + lk + 4 * v;
+                const int i = mt * 16 + Mfma<T>::row(lk, v);
                 if (i >= N) continue;
                 if (j < N) {
                     if (dph) {
-                        double val = acc[0][mt][nt][v] + (i == j ? sm.kdp[i] : 0.0);
+                        double val = (double)acc[0][mt][nt][v] + (i == j ? sm.kdp[i] : 0.0);
                         chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::KP + j] = val;
+                        rec[i * RC::SIZE + RC::KP + j] = (R)val;
                     }
                     if (dth) {
-                        double val = acc[1][mt][nt][v] + (i == j ? sm.kdt[i] : 0.0);
+                        double val = (double)acc[1][mt][nt][v] + (i == j ? sm.kdt[i] : 0.0);
                         chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::KT + j] = val;
+                        rec[i * RC::SIZE + RC::KT + j] = (R)val;
                     }
                 } else if (j == FCOL) {
                     if (dph) {
-                        double val = acc[0][mt][nt][v] + sm.fnp[i];
+                        double val = (double)acc[0][mt][nt][v] + sm.fnp[i];
                         chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::FP] = val;
+                        rec[i * RC::SIZE + RC::FP] = (R)val;
                     }
                     if (dT) {
-                        double val = acc[1][mt][nt][v] + sm.fnt[i];
+                        double val = (double)acc[1][mt][nt][v] + sm.fnt[i];
                         chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::FT] = val;
+                        rec[i * RC::SIZE + RC::FT] = (R)val;
                     }
                 }
             }
@@ -942,13 +959,13 @@ __device__ __forceinline__ void kpt_store(const Smem<DIM>& sm, double* rec, cons
 // Nodal (lumped) terms of node a: lumped weight WL = sum_ip w N_a, inertia, damping and
 // gravity (PhaseField.cpp:436-477), lumped driving / capacity, phase viscosity and
 // stabilisation (PhaseField.cpp:693-711), heat capacity (ThermalModel.cpp:128-129).
-template <int DIM>
-__device__ __forceinline__ void nodal_terms(Smem<DIM>& sm, const KParams& p, const int a, const bool du, const bool dph,
+template <int DIM, typename T>
+__device__ __forceinline__ void nodal_terms(Smem<DIM, T>& sm, const KParams& p, const int a, const bool du, const bool dph,
                                             const bool dth) {
-    constexpr int NIP = Smem<DIM>::NIP;
+    constexpr int NIP = Smem<DIM, T>::NIP;
     const MatConst& mc = p.mc;
     double s = 0;
-    for (int ip = 0; ip < NIP; ip++) s += sm.ipc[IPB_W][ip] * sm.Nsh[ip][a];
+    for (int ip = 0; ip < NIP; ip++) s += sm.ipc[IPB_W][ip] * (double)sm.Nsh[ip][a];
     if (du) {
         double DamI, DamG, d1, d2;
         damage(mc.DensityDamage, sm.Ph[a], mc.k0, mc.p, DamI, d1, d2);
@@ -970,7 +987,7 @@ __device__ __forceinline__ void nodal_terms(Smem<DIM>& sm, const KParams& p, con
         double kd = 0, f = 0;
         if (mc.LumpedDrive) {
             double ld = 0;
-            for (int ip = 0; ip < NIP; ip++) ld += sm.ipc[IPB_LDRIVE][ip] * sm.Nsh[ip][a];
+            for (int ip = 0; ip < NIP; ip++) ld += sm.ipc[IPB_LDRIVE][ip] * (double)sm.Nsh[ip][a];
             double D0, dD0, ddD0;
             damage(mc.DamFunction, sm.Ph[a], mc.k0, mc.p, D0, dD0, ddD0);
             kd += ld * ddD0;
@@ -991,16 +1008,18 @@ __device__ __forceinline__ void nodal_terms(Smem<DIM>& sm, const KParams& p, con
     sm.fnt[a] = dth ? s * mc.Density_th * mc.cp * sm.dT[a] : 0.0;
 }
 
-template <int DIM>
-__global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
+// T: the tile type of the element contractions (double: FP64 mode; float: mixed mode, FP32
+// MFMA); R: the element record type.
+template <int DIM, typename T, typename R>
+__global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void assemble_kernel(KParams p) {
     using E = ET<DIM>;
-    using SM = Smem<DIM>;
+    using SM = Smem<DIM, T>;
     using RC = Rec<DIM>;
     constexpr int N = E::N, NIP = E::NIP, NP = SM::NP, IPP = SM::IPP, MT = NP / 16;
     __shared__ SM sm;
     const int tid = threadIdx.x;
     const int64_t el = p.elist ? (int64_t)p.elist[blockIdx.x] : (int64_t)blockIdx.x;
-    double* const rec = p.ebuf + el * N * RC::SIZE;  // this element's records
+    R* const rec = (R*)p.ebuf + el * N * RC::SIZE;  // this element's records
     // per-IP coefficients of this element: one contiguous [field][ip] block, loads issued first
     constexpr int IPC_NV = IPB_NF * NIP, IPC_PER = (IPC_NV + NT - 1) / NT;
     double ipv[IPC_PER];
@@ -1056,8 +1075,8 @@ __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
     for (int i = tid; i < IPP * NP; i += NT) {
         const int ip = i / NP, a = i % NP;
         if (ip >= NIP || a >= N) {
-            sm.Nsh[ip][a] = 0.0;
-            for (int c = 0; c < DIM; c++) sm.G[ip][c][a] = 0.0;
+            sm.Nsh[ip][a] = 0;
+            for (int c = 0; c < DIM; c++) sm.G[ip][c][a] = 0;
             continue;
         }
         const int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9;
@@ -1077,11 +1096,11 @@ __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
             gb[DIM - 1] = bx * by * gz;
         }
         const double Wa = DIM == 3 ? sm.X[a][3] : 1.0;
-        sm.Nsh[ip][a] = DIM == 3 ? Wa * cb * sm.ipc[IPB_WF][ip] : cb;
+        sm.Nsh[ip][a] = (T)(DIM == 3 ? Wa * cb * sm.ipc[IPB_WF][ip] : cb);
         for (int c = 0; c < DIM; c++) {
             double s = 0;
             for (int d = 0; d < DIM; d++) s += sm.ipc[IPB_JI + c * DIM + d][ip] * gb[d];
-            sm.G[ip][c][a] = DIM == 3 ? Wa * (s - cb * sm.ipc[IPB_DW + c][ip]) : s;
+            sm.G[ip][c][a] = (T)(DIM == 3 ? Wa * (s - cb * sm.ipc[IPB_DW + c][ip]) : s);
         }
     }
     __syncthreads();
@@ -1092,24 +1111,24 @@ __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
     // nodal terms to every wave's write-back.
     const int wv = tid >> 6, lane = tid & 63, lk = lane >> 4, l16 = lane & 15;
     const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
-    dbl4 acc[DIM][MT][MT];  // zeroed by the contraction loops
+    typename Mfma<T>::V acc[DIM][MT][MT];  // zeroed by the contraction loops
     if (wv < DIM) {
-        if (du) kuu_loop<DIM, MT>(sm, wv, lk, l16, acc);
+        if (du) kuu_loop<DIM, MT, T>(sm, wv, lk, l16, acc);
     } else if (wv == DIM) {
-        if (lane < N) nodal_terms<DIM>(sm, p, lane, du, dph, p.do_th);
+        if (lane < N) nodal_terms<DIM, T>(sm, p, lane, du, dph, p.do_th);
         if (dph && lane == 0) {
             double s = 0;
             for (int ip = 0; ip < NIP; ip++) s += sm.ipc[IPB_GW][ip];
             p.lcrack_elem[el] = s;
         }
-        if (dph || dT) kpt_loop<DIM, MT>(sm, lk, l16, dph, dT, acc);
+        if (dph || dT) kpt_loop<DIM, MT, T>(sm, lk, l16, dph, dT, acc);
     }
     __syncthreads();
     double chk = 0.0;  // stays 0 unless a stored value is NaN or infinite
     if (wv < DIM) {
-        if (du) kuu_store<DIM, MT>(sm, rec, wv, lk, l16, acc, chk);
+        if (du) kuu_store<DIM, MT, T, R>(sm, rec, wv, lk, l16, acc, chk);
     } else if (wv == DIM) {
-        if (dph || dT) kpt_store<DIM, MT>(sm, rec, lk, l16, dph, dT, p.do_th, acc, chk);
+        if (dph || dT) kpt_store<DIM, MT, T, R>(sm, rec, lk, l16, dph, dT, p.do_th, acc, chk);
     }
     // no closing barrier: a workgroup retires as soon as its waves have issued their stores
     if (chk != 0.0) atomicOr(&p.st->nonfinite, 1u);
@@ -1124,17 +1143,41 @@ __global__ __launch_bounds__(NT, 3) void assemble_kernel(KParams p) {
 // overwritten rather than zeroed and accumulated (Physics::Assemble's zero + add).
 //
 // The workgroup is a single wavefront, whose LDS instructions execute in issue order, so
-// record k+1's read-add-write of a slot sees record k's: no barrier is needed between
-// records, and the loads of PF records are issued before any of them is added.
+// the adds of record k+1 into a slot follow those of record k: no barrier is needed
+// between records, and the loads of GPF records are issued before any of them is added.
+// The adds are LDS atomics without return (no read-add-write latency chain); within one
+// record the lanes' slots are distinct.
 // ---------------------------------------------------------------------------------------
 constexpr int GW = 64;      // lanes per gather workgroup (one wavefront)
 constexpr int MAXDEG = 128; // neighbour slots per row held in LDS
-constexpr int GPF = 4;      // records loaded ahead
+constexpr int GPF = 8;      // records loaded ahead
 
-template <int DIM>
+// W consecutive record entries per lane load; one entry per lane keeps the LDS adds of a
+// load on consecutive entries (conflict-free)
+template <typename T>
+struct RecLoad;
+template <>
+struct RecLoad<double> {
+    static constexpr int W = 1;
+    typedef double V;
+    static __device__ __forceinline__ V zero() { return 0.0; }
+    static __device__ __forceinline__ double get(V x, int) { return x; }
+};
+template <>
+struct RecLoad<float> {
+    static constexpr int W = 1;
+    typedef float V;
+    static __device__ __forceinline__ V zero() { return 0.f; }
+    static __device__ __forceinline__ double get(V x, int) { return (double)x; }
+};
+
+// T: element record type (double, or float in mixed mode); the rows accumulate in FP64.
+template <int DIM, typename T>
 __global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
     using RC = Rec<DIM>;
-    constexpr int N = RC::N, PER = (RC::SIZE + GW - 1) / GW;
+    typedef typename RecLoad<T>::V V;
+    constexpr int N = RC::N, W = RecLoad<T>::W, PER = (RC::SIZE + GW * W - 1) / (GW * W);
+    static_assert(RC::SIZE % W == 0, "record length must be a whole number of loads");
     // LDS row accumulators: K_UU [DIM*DIM][MAXDEG], K_pp [MAXDEG], K_TT [MAXDEG], f_U, f_p, f_T
     constexpr int OKP = DIM * DIM * MAXDEG, OKT = OKP + MAXDEG, OF = OKT + MAXDEG, LEN = OF + DIM + 2;
     __shared__ double acc[LEN];
@@ -1144,45 +1187,62 @@ __global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
     const int deg = (int)(p.nbr_ptr[r + 1] - r0);
     const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
     for (int i = lane; i < LEN; i += GW) acc[i] = 0.0;
-    // per lane: record entry t -> LDS base slot, index into the record's column-slot row
-    // (-1: fixed slot), active flag
-    int base[PER], lb[PER];
-    bool act[PER];
+    // per lane: record entries t = (lane + GW u) W + w -> LDS base slot, index into the
+    // record's column-slot row (-1: fixed slot), active flag
+    int base[PER][W], lb[PER][W];
+    bool act[PER][W], ld[PER];
 #pragma unroll
     for (int u = 0; u < PER; u++) {
-        const int t = lane + u * GW;
-        base[u] = 0; lb[u] = -1; act[u] = false;
-        if (t < RC::FU) { base[u] = (t / N) * MAXDEG; lb[u] = t % N; act[u] = du; }
-        else if (t < RC::KP) { base[u] = OF + (t - RC::FU); act[u] = du; }
-        else if (t < RC::FP) { base[u] = OKP; lb[u] = t - RC::KP; act[u] = dph; }
-        else if (t == RC::FP) { base[u] = OF + DIM; act[u] = dph; }
-        else if (t < RC::FT) { base[u] = OKT; lb[u] = t - RC::KT; act[u] = p.do_th; }
-        else if (t == RC::FT) { base[u] = OF + DIM + 1; act[u] = dT; }
+        ld[u] = false;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const int t = (lane + u * GW) * W + w;
+            int bs = 0, l = -1;
+            bool ac = false;
+            if (t < RC::FU) { bs = (t / N) * MAXDEG; l = t % N; ac = du; }
+            else if (t < RC::KP) { bs = OF + (t - RC::FU); ac = du; }
+            else if (t < RC::FP) { bs = OKP; l = t - RC::KP; ac = dph; }
+            else if (t == RC::FP) { bs = OF + DIM; ac = dph; }
+            else if (t < RC::FT) { bs = OKT; l = t - RC::KT; ac = p.do_th; }
+            else if (t == RC::FT) { bs = OF + DIM + 1; ac = dT; }
+            base[u][w] = bs; lb[u][w] = l; act[u][w] = ac;
+            ld[u] = ld[u] || ac;
+        }
     }
     __syncthreads();
     const int64_t k0 = p.n2e_ptr[r], k1 = p.n2e_ptr[r + 1];
-    for (int64_t kb = k0; kb < k1; kb += GPF) {
-        double v[GPF][PER];
-        int slot[GPF][PER];
+    // the node's (element, local index) list, GW entries at a time, held one per lane and
+    // read back with readlane: the record loads of a batch need no other memory round trip
+    for (int64_t c0 = k0; c0 < k1; c0 += GW) {
+        const int nc = (int)(k1 - c0 < GW ? k1 - c0 : GW);
+        const int my_e = lane < nc ? p.n2e_elem[c0 + lane] : 0;
+        const int my_a = lane < nc ? (int)p.n2e_loc[c0 + lane] : 0;
+        for (int kb = 0; kb < nc; kb += GPF) {
+            V x[GPF][PER];
+            int lcv[GPF];
 #pragma unroll
-        for (int q = 0; q < GPF; q++) {
-            const int64_t k = kb + q < k1 ? kb + q : k1 - 1;
-            const int64_t e = p.n2e_elem[k];
-            const int a = p.n2e_loc[k];
-            const double* rec = p.ebuf + (e * N + a) * RC::SIZE;
-            const uint8_t* lc = p.loc + (e * N + a) * N;
+            for (int q = 0; q < GPF; q++) {
+                const int k = kb + q < nc ? kb + q : nc - 1;
+                const int64_t e = __builtin_amdgcn_readlane(my_e, k);
+                const int a = __builtin_amdgcn_readlane(my_a, k);
+                const V* rec = (const V*)((const T*)p.ebuf + (e * N + a) * RC::SIZE);
+                // the record's column-slot row: one byte per lane < N, read across lanes below
+                lcv[q] = lane < N ? (int)p.loc[(e * N + a) * N + lane] : 0;
 #pragma unroll
-            for (int u = 0; u < PER; u++) {
-                v[q][u] = act[u] ? rec[lane + u * GW] : 0.0;
-                slot[q][u] = base[u] + (act[u] && lb[u] >= 0 ? (int)lc[lb[u]] : 0);
+                for (int u = 0; u < PER; u++) x[q][u] = ld[u] ? rec[lane + u * GW] : RecLoad<T>::zero();
             }
-        }
 #pragma unroll
-        for (int q = 0; q < GPF; q++) {
-            if (kb + q < k1) {
+            for (int q = 0; q < GPF; q++) {
+                if (kb + q < nc) {
 #pragma unroll
-                for (int u = 0; u < PER; u++)
-                    if (act[u]) acc[slot[q][u]] += v[q][u];
+                    for (int u = 0; u < PER; u++)
+#pragma unroll
+                        for (int w = 0; w < W; w++) {
+                            const int sl = __shfl(lcv[q], lb[u][w] >= 0 ? lb[u][w] : 0);
+                            if (act[u][w])  // ds_add_f64, no return
+                                atomicAdd(&acc[base[u][w] + (lb[u][w] >= 0 ? sl : 0)], RecLoad<T>::get(x[q][u], w));
+                        }
+                }
             }
         }
     }
@@ -1232,15 +1292,22 @@ __global__ void copy_kernel(double* d, const double* s, int64_t n) {
 hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
     if (p.nlist <= 0) return hipSuccess;
     dim3 grid((unsigned)p.nlist), block(NT);
-    if (dim == 2) hipLaunchKernelGGL(assemble_kernel<2>, grid, block, 0, s, p);
-    else hipLaunchKernelGGL(assemble_kernel<3>, grid, block, 0, s, p);
+    // Element records stay FP64 in both modes: the gather reads FP32 records no faster.
+    if (p.mixed) {
+        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, float, double>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((assemble_kernel<3, float, double>), grid, block, 0, s, p);
+    } else {
+        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, double, double>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((assemble_kernel<3, double, double>), grid, block, 0, s, p);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_gather(const KParams& p, int dim, hipStream_t s) {
     if (p.Ng <= 0) return hipSuccess;
-    if (dim == 2) hipLaunchKernelGGL(gather_kernel<2>, dim3((unsigned)p.Ng), dim3(GW), 0, s, p);
-    else hipLaunchKernelGGL(gather_kernel<3>, dim3((unsigned)p.Ng), dim3(GW), 0, s, p);
+    const dim3 grid((unsigned)p.Ng), block(GW);
+    if (dim == 2) hipLaunchKernelGGL((gather_kernel<2, double>), grid, block, 0, s, p);
+    else hipLaunchKernelGGL((gather_kernel<3, double>), grid, block, 0, s, p);
     return hipGetLastError();
 }
 
