@@ -156,6 +156,16 @@ __device__ double yield_ve(const double S[6], double dF[6]) {
     return F;
 }
 
+// x^(n-1) for x >= 0: repeated products for the small integer exponents of Glen's law
+__device__ __forceinline__ double pow_nm1(double x, double n) {
+    const double e = n - 1.0;
+    if (e == 0.0) return 1.0;
+    if (e == 1.0) return x;
+    if (e == 2.0) return x * x;
+    if (e == 3.0) return (x * x) * x;
+    return pow(x, e);
+}
+
 // return_mapping_getKF (SolidMaterial.cpp:278-301).  K = [[I, a],[b^T, -1]] with
 // a = dF (ddF is zeroed in the reference, :289) and b = fac*Damage*D*dF.
 __device__ void getKF(const MatIP& m, double nC, double PreFac, double Damage, const double sol[7], const double eps_t[6],
@@ -165,8 +175,10 @@ __device__ void getKF(const MatIP& m, double nC, double PreFac, double Damage, c
     double F = yield_ve(S, a);
     for (int i = 0; i < 6; i++) f[i] = sol[i] - eps_t[i] + eps_old[i] + sol[6] * a[i];
     const double sc = 1.0e6;
-    f[6] = PreFac * pow(F / sc, nC) - sol[6];
-    double fac = nC * PreFac / sc * pow(F / sc, nC - 1.0);
+    // (F/sc)^n and n (F/sc)^(n-1) from one power: x^n = x^(n-1) x
+    const double x = F / sc, pm1 = pow_nm1(x, nC);
+    f[6] = PreFac * (pm1 * x) - sol[6];
+    double fac = nC * PreFac / sc * pm1;
     double Da[6], da[6];
     for (int i = 0; i < 6; i++) da[i] = Damage * a[i];
     Dmul(m, da, Da);  // D symmetric: (Damage dF^T D)_j = (D (Damage dF))_j
@@ -448,42 +460,55 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 gb[DIM - 1] = b1[0][ax] * b1[1][ay] * g1[DIM - 1][az];
             }
         };
-        // rational weights (3D): wFun = CB.W, dW = GB^T W (NURBS3_Cube2.cpp:96-103)
-        double wF = 1.0, dW[DIM];
-        for (int c = 0; c < DIM; c++) dW[c] = 0;
-        if (DIM == 3) {
-            wF = 0;
-#pragma unroll 1
-            for (int a = 0; a < N; a++) {
-                double cb, gb[DIM];
-                cbgb(a, cb, gb);
-                double Wa = sm.X[le][a][3];
-                wF += cb * Wa;
-                for (int c = 0; c < DIM; c++) dW[c] += gb[c] * Wa;
-            }
+        // One pass over the nodes: W-weighted sums of CB and GB against each nodal field f,
+        // SC(f) = sum_a W_a CB_a f_a and SG_c(f) = sum_a W_a GB_ca f_a (2D: W = 1).  With
+        // f = 1 they are wFun and dW (NURBS3_Cube2.cpp:96-103); the rational basis then gives
+        // sum_a N_a f_a = SC(f) / wFun and sum_a GXi_ca f_a = SG_c(f) / wFun - dW_c SC(f) / wFun^2
+        // (NURBS3_Cube2.cpp:104-106), f = X_d gives J (:107-108), and field gradients are
+        // J^-1 applied to the GXi sums (:110-112).  2D (B2_2.cpp:72-88): N = CB, GXi = GB.
+        enum { F_ONE = 0, F_X = 1, F_PH = F_X + DIM, F_PHO, F_U, F_UO = F_U + DIM, F_T = F_UO + DIM, NFLD };
+        double SC[NFLD], SG[DIM][NFLD];
+        for (int k = 0; k < NFLD; k++) {
+            SC[k] = 0;
+            for (int c = 0; c < DIM; c++) SG[c][k] = 0;
         }
-        auto nshape = [&](int a, double& Na, double gx[DIM]) {  // N and GXi (3D) / GB (2D)
+#pragma unroll 1
+        for (int a = 0; a < N; a++) {
             double cb, gb[DIM];
             cbgb(a, cb, gb);
-            if (DIM == 3) {
-                double Wa = sm.X[le][a][3];
-                Na = Wa * cb / wF;
-                for (int c = 0; c < DIM; c++) gx[c] = Wa * (gb[c] / wF - cb * dW[c] / wF / wF);
-            } else {
-                Na = cb;
-                for (int c = 0; c < DIM; c++) gx[c] = gb[c];
+            const double Wa = DIM == 3 ? sm.X[le][a][3] : 1.0;
+            const double wc = Wa * cb;
+            double wg[DIM];
+            for (int c = 0; c < DIM; c++) wg[c] = Wa * gb[c];
+            double fv[NFLD];
+            fv[F_ONE] = 1.0;
+            for (int d = 0; d < DIM; d++) fv[F_X + d] = sm.X[le][a][d];
+            fv[F_PH] = sm.Ph[le][a];
+            fv[F_PHO] = sm.PhO[le][a];
+            for (int d = 0; d < DIM; d++) {
+                fv[F_U + d] = sm.U[le][d][a];
+                fv[F_UO + d] = sm.Uo[le][d][a];
             }
+            fv[F_T] = sm.T[le][a];
+            for (int k = 0; k < NFLD; k++) {
+                SC[k] += wc * fv[k];
+                for (int c = 0; c < DIM; c++) SG[c][k] += wg[c] * fv[k];
+            }
+        }
+        const double wF = DIM == 3 ? SC[F_ONE] : 1.0;
+        const double rw = 1.0 / wF;
+        double dW[DIM];
+        for (int c = 0; c < DIM; c++) dW[c] = DIM == 3 ? SG[c][F_ONE] : 0.0;
+        auto val = [&](int k) { return DIM == 3 ? SC[k] * rw : SC[k]; };
+        auto rgrad = [&](int k, int c) { return DIM == 3 ? SG[c][k] * rw - dW[c] * SC[k] * rw * rw : SG[c][k]; };
+        auto nshape = [&](int a, double& Na) {  // N_a, for the rarely used per-node sums below
+            double cb, gb[DIM];
+            cbgb(a, cb, gb);
+            Na = DIM == 3 ? sm.X[le][a][3] * cb * rw : cb;
         };
         double J[DIM][DIM];
         for (int c = 0; c < DIM; c++)
-            for (int d = 0; d < DIM; d++) J[c][d] = 0;
-#pragma unroll 1
-        for (int a = 0; a < N; a++) {
-            double Na, gx[DIM];
-            nshape(a, Na, gx);
-            for (int c = 0; c < DIM; c++)
-                for (int d = 0; d < DIM; d++) J[c][d] += gx[c] * sm.X[le][a][d];
-        }
+            for (int d = 0; d < DIM; d++) J[c][d] = rgrad(F_X + d, c);
         double Ji[DIM][DIM], w;
         if (DIM == 3) {
             double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
@@ -529,36 +554,33 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
         }
         // field values and gradients at the IP
         double gu[DIM][DIM], guo[DIM][DIM], gphi[DIM], gT[DIM];
-        double phi = 0, phio = 0, Tu = 0, Tph = 0;
-        for (int c = 0; c < DIM; c++) {
-            gphi[c] = 0; gT[c] = 0;
-            for (int d = 0; d < DIM; d++) { gu[c][d] = 0; guo[c][d] = 0; }
-        }
-        const double T0 = p.has_T ? p.sT.x[0] : 0.0;  // Assemble_Phase reads T dof 0 for every node
-#pragma unroll 1
-        for (int a = 0; a < N; a++) {
-            double Na, gx[DIM], G[DIM];
-            nshape(a, Na, gx);
+        auto grad = [&](int k, double g[DIM]) {
+            double r[DIM];
+            for (int d = 0; d < DIM; d++) r[d] = rgrad(k, d);
             for (int c = 0; c < DIM; c++) {
-                double s = 0;
-                for (int d = 0; d < DIM; d++) s += Ji[c][d] * gx[d];
-                G[c] = s;
+                double t = 0;
+                for (int d = 0; d < DIM; d++) t += Ji[c][d] * r[d];
+                g[c] = t;
             }
-            if (p.has_pf) {
-                phi += Na * sm.Ph[le][a];
-                phio += Na * sm.PhO[le][a];
-                for (int c = 0; c < DIM; c++) gphi[c] += G[c] * sm.Ph[le][a];
-                for (int al = 0; al < DIM; al++)
-                    for (int c = 0; c < DIM; c++) {
-                        gu[al][c] += G[c] * sm.U[le][al][a];
-                        guo[al][c] += G[c] * sm.Uo[le][al][a];
-                    }
+        };
+        const double T0 = p.has_T ? p.sT.x[0] : 0.0;  // Assemble_Phase reads T dof 0 for every node
+        double phi = 0, phio = 0, Tu = 0, Tph = 0;
+        for (int c = 0; c < DIM; c++) { gphi[c] = 0; gT[c] = 0; }
+        for (int al = 0; al < DIM; al++)
+            for (int c = 0; c < DIM; c++) { gu[al][c] = 0; guo[al][c] = 0; }
+        if (p.has_pf) {
+            phi = val(F_PH);
+            phio = val(F_PHO);
+            grad(F_PH, gphi);
+            for (int al = 0; al < DIM; al++) {
+                grad(F_U + al, gu[al]);
+                grad(F_UO + al, guo[al]);
             }
-            if (p.has_T) {
-                Tu += Na * sm.T[le][a];
-                Tph += Na * (mc.phase_T_from_dof0 ? T0 : sm.T[le][a]);
-                for (int c = 0; c < DIM; c++) gT[c] += G[c] * sm.T[le][a];
-            }
+        }
+        if (p.has_T) {
+            Tu = val(F_T);
+            Tph = mc.phase_T_from_dof0 ? T0 * val(F_ONE) : Tu;
+            grad(F_T, gT);
         }
         if (!p.has_T) Tu = Tph = mc.T_default;
         double eps[6], epso[6], eps_pl[6];
@@ -677,8 +699,8 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 for (int d = 0; d < DIM; d++) {
                     double vd = 0;
                     for (int a = 0; a < N; a++) {
-                        double Na, gx[DIM];
-                        nshape(a, Na, gx);
+                        double Na;
+                        nshape(a, Na);
                         vd += Na * p.su.dx[(int64_t)p.fpos_u[d] * p.Ng + p.erank[el * N + a]];
                     }
                     v2 += vd * vd;
@@ -691,8 +713,8 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 for (int d = 0; d < DIM; d++) {
                     double ud = 0;
                     for (int a = 0; a < N; a++) {
-                        double Na, gx[DIM];
-                        nshape(a, Na, gx);
+                        double Na;
+                        nshape(a, Na);
                         ud += Na * sm.U[le][d][a];
                     }
                     gu_ += mc.g[d] * ud;
