@@ -8,6 +8,7 @@ PhaseField.cpp:191), so an IceFEM input file's material/model blocks translate 1
 Workloads (the BASELINE configurations, named by what they are):
   "quad2d_le_at2"   64x32 B2_2 quads on 0.1x0.05 m, u(2)+phi, linear elastic + AT2
   "hex3d_glen_at2"  40x40x80 NURBS3_Cube2 hexes, u(3)+phi+T, Glen viscous + AT2 + thermal
+  "hex3d_vep_at2"   16x16x16 hexes, as hex3d_glen_at2 with the visco-elasto-plastic rheology
 Sizes can be scaled down (nel=...) for parity tests; the physics stays the same.
 """
 import copy
@@ -76,6 +77,17 @@ WORKLOADS = {
     },
 }
 WORKLOADS["hex3d_glen_at2"]["properties"]["Ice"]["SolidProperties"]["Damping"] = 1.0e6
+
+# Visco-elasto-plastic variant of the 3D workload (SolidMaterial::return_mapping_VEP): Glen
+# creep plus a Drucker-Prager yield surface (friction 20 deg, dilatancy 10 deg) with linear
+# hardening and plastic viscosity; sy low enough that a share of the IPs yields.
+WORKLOADS["hex3d_vep_at2"] = copy.deepcopy(WORKLOADS["hex3d_glen_at2"])
+WORKLOADS["hex3d_vep_at2"]["index"] = 5
+WORKLOADS["hex3d_vep_at2"]["mesh"]["nel"] = [16, 16, 16]
+WORKLOADS["hex3d_vep_at2"]["mesh"]["lengths"] = [0.04, 0.04, 0.04]
+WORKLOADS["hex3d_vep_at2"]["properties"]["Ice"]["SolidProperties"].update(
+    {"Rheology": "ViscoElasticPlastic", "sy": 5.0e5, "F_deg": 20.0, "G_deg": 10.0, "hardening": 1.0e8,
+     "pl_visc": 1.0e9})
 
 
 def get_inputs(name, nel=None):

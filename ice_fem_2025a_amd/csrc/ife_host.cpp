@@ -241,6 +241,14 @@ MatConst make_matconst(const ife_physics* ph, const Model* pfm, const Model* thm
     mc.l = m.l; mc.Gc_in = m.Gc; mc.pf_visc = m.Viscosity;
     mc.Split = m.Split;
     mc.TRef = m.TRef; mc.TOffset = m.TOffset; mc.QCreep = m.QCreep; mc.ft0 = m.ft0; mc.ftdeg = m.ftdeg;
+    mc.sy0 = m.sy; mc.sydeg = m.sydeg;
+    if (m.Rheology == IFE_RHEO_VEP || m.Rheology == IFE_RHEO_DVEP) {  // SolidMaterial.cpp:49-64
+        const double ya = M_PI / 180.0 * m.F_deg, pa = M_PI / 180.0 * m.G_deg;
+        mc.yield_alpha = 6.0 * std::sin(ya) / (3.0 - std::sin(ya));
+        mc.pot_alpha = 6.0 * std::sin(pa) / (3.0 - std::sin(pa));
+        mc.hardening = m.hardening;
+        mc.pl_visc = m.pl_visc;
+    }
     // the thermal model reads its own material block (ThermalModel.cpp:61-62)
     if (thm) { mc.k = thm->mat.ThermalConductivity; mc.cp = thm->mat.HeatCapacity; mc.Density_th = thm->mat.Density; }
     if (pfm) {
@@ -708,9 +716,11 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             m->HasT = ph->field_index("T") >= 0;  // physics->dofspace->hasDofType("T")
             if (m->HasT) { m->field_T = need("T"); m->Step_T = ph->fstep[m->field_T]; }
             const int r = mat->Rheology;
-            if (!(r == IFE_RHEO_LE || r == IFE_RHEO_VE || r == IFE_RHEO_DVE || r == IFE_RHEO_VE_EXPL || r == IFE_RHEO_DVE_EXPL))
-                THROW(IFE_ERR_UNSUPPORTED, "rheology not implemented by the engine (LE, VE, dVE, VE_expl, dVE_expl are)");
-            m->HistSize = (r == IFE_RHEO_LE) ? 0 : 1;
+            // SolidMaterial::UpdatePlasticStrains has no dVEP case and throws (SolidMaterial.cpp:188-191)
+            if (!(r == IFE_RHEO_LE || r == IFE_RHEO_VE || r == IFE_RHEO_DVE || r == IFE_RHEO_VE_EXPL || r == IFE_RHEO_DVE_EXPL ||
+                  r == IFE_RHEO_VEP))
+                THROW(IFE_ERR_UNSUPPORTED, "rheology not available for Fracture/PhaseField (LE, VE, dVE, VE_expl, dVE_expl, VEP are)");
+            m->HistSize = (r == IFE_RHEO_LE) ? 0 : (r == IFE_RHEO_VEP ? 4 : 1);  // SolidMaterial.cpp:26-52
             if (!(mat->Split == IFE_SPLIT_NONE || mat->Split == IFE_SPLIT_VOLSTRAINS || mat->Split == IFE_SPLIT_VOLSTRESS))
                 THROW(IFE_ERR_UNSUPPORTED, "energy split not implemented by the engine (None, VolStrains, VolStress are)");
             int dfn = pf->DistributionFunction;

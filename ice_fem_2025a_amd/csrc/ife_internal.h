@@ -18,6 +18,9 @@ struct MatConst {
     double l, Gc_in, pf_visc;
     int Split, ConsistentEnergy;
     double TRef, TOffset, QCreep, ft0, ftdeg;
+    // visco-elasto-plastic (SolidMaterial.cpp:49-64): sy at TRef, its T slope, Drucker-Prager
+    // coefficients 6 sin / (3 - sin) of the friction / dilatancy angles, hardening, plastic viscosity
+    double sy0, sydeg, yield_alpha, pot_alpha, hardening, pl_visc;
     double k, cp, Density_th;
     // PF utility options
     int DamFunction, DensityDamage, GravityDamage, ViscoDriving;

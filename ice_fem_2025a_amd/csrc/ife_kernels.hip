@@ -61,7 +61,7 @@ __device__ __forceinline__ int vrow(int a, int c) {
 }
 
 struct MatIP {
-    double dam, Young, Shear, Lame, Bulk, ACreep, Gc;
+    double dam, Young, Shear, Lame, Bulk, ACreep, Gc, sy;
     double Dll, Dlo, Dss;  // D = dam*D0: diag normal, off-diag normal, shear
 };
 
@@ -86,6 +86,7 @@ __device__ void set_temperature(const MatConst& mc, double T, double poros, MatI
     m.ACreep = mc.A0 * exp(ExpTerm);
     double ft = dam * (mc.ft0 - mc.ftdeg * (TAbs - mc.TRef));
     m.Gc = 2.0 * ft * ft / m.Young * mc.l;
+    m.sy = dam * (mc.sy0 - mc.sydeg * (TAbs - mc.TRef));
 }
 
 __device__ __forceinline__ void Dmul(const MatIP& m, const double v[6], double o[6]) {
@@ -250,6 +251,240 @@ __device__ double return_map_viscous(const MatIP& m, double nC, double dt, doubl
     return fabs(Diss) / dt;
 }
 
+
+// ---- visco-elasto-plastic return map (SolidMaterial.cpp:303-521) -----------------------
+// Voigt products with P = I - 1/3 i i^T and J2Mat_stress (P with 2 on the shear diagonal)
+__device__ __forceinline__ void Pmul(const double s[6], double o[6], double shear) {
+    const double d = 1.0 - 1.0 / 3.0, q = 0.0 - 1.0 / 3.0;
+    o[0] = d * s[0] + q * s[1] + q * s[2];
+    o[1] = q * s[0] + d * s[1] + q * s[2];
+    o[2] = q * s[0] + q * s[1] + d * s[2];
+    o[3] = shear * s[3];
+    o[4] = shear * s[4];
+    o[5] = shear * s[5];
+}
+__device__ __forceinline__ double Pij(int i, int j, double shear) {
+    if (i < 3 && j < 3) return i == j ? 1.0 - 1.0 / 3.0 : 0.0 - 1.0 / 3.0;
+    return i == j ? shear : 0.0;
+}
+
+// A x = B for an M x M system and NR right-hand sides, Gaussian elimination with partial
+// pivoting (largest magnitude, first on ties); A and B are overwritten, B by the solution.
+template <int M, int NR>
+__device__ void gauss_pp(double (&A)[M][M], double (&B)[M][NR]) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        int piv = k;
+        double amax = fabs(A[k][k]);
+#pragma unroll
+        for (int r = k + 1; r < M; r++) {
+            const double v = fabs(A[r][k]);
+            if (v > amax) { amax = v; piv = r; }
+        }
+#pragma unroll
+        for (int r = k + 1; r < M; r++) {
+            const bool sw = r == piv;
+#pragma unroll
+            for (int c = k; c < M; c++) {
+                const double t = A[k][c];
+                A[k][c] = sw ? A[r][c] : t;
+                A[r][c] = sw ? t : A[r][c];
+            }
+#pragma unroll
+            for (int c = 0; c < NR; c++) {
+                const double t = B[k][c];
+                B[k][c] = sw ? B[r][c] : t;
+                B[r][c] = sw ? t : B[r][c];
+            }
+        }
+        const double inv = 1.0 / A[k][k];
+#pragma unroll
+        for (int r = k + 1; r < M; r++) {
+            const double f = A[r][k] * inv;
+#pragma unroll
+            for (int c = k + 1; c < M; c++) A[r][c] -= f * A[k][c];
+#pragma unroll
+            for (int c = 0; c < NR; c++) B[r][c] -= f * B[k][c];
+        }
+    }
+#pragma unroll
+    for (int k = M - 1; k >= 0; k--)
+#pragma unroll
+        for (int c = 0; c < NR; c++) {
+            double t = B[k][c];
+#pragma unroll
+            for (int j = k + 1; j < M; j++) t -= A[k][j] * B[j][c];
+            B[k][c] = t / A[k][k];
+        }
+}
+
+struct VEPConst {
+    double ya, pa, hard, plv, sy;
+};
+
+// SolidMaterial::YieldFunction (SolidMaterial.cpp:447-475), F and dF
+__device__ double yield_vep(const VEPConst& v, const double S[6], const double TS[6], double dF[6]) {
+    double dJ2[6];
+    Pmul(S, dJ2, 2.0);
+    double J2 = 0;
+    for (int i = 0; i < 6; i++) J2 += S[i] * dJ2[i];
+    J2 = 0.5 * J2;
+    const double I1 = S[0] + S[1] + S[2];
+    const bool apex = v.ya > 0.0 && (TS[0] + TS[1] + TS[2] > v.sy * 3.0 / v.ya);
+    if (!apex) {
+        const double c = 0.5 * sqrt(3.0) / sqrt(J2 + 1.0e-6);
+        for (int i = 0; i < 6; i++) dF[i] = c * dJ2[i] + v.ya / 3.0 * (i < 3 ? 1.0 : 0.0);
+        return sqrt(3.0 * J2 + 1.0e-6) + v.ya / 3.0 * I1 - v.sy;
+    }
+    for (int i = 0; i < 6; i++) dF[i] = v.ya / 3.0 * (i < 3 ? 1.0 : 0.0);
+    return v.ya / 3.0 * I1 - v.sy;
+}
+
+// SolidMaterial::PotentialFunction (SolidMaterial.cpp:494-521): dG (normalised) and ddG
+__device__ void potential_vep(const VEPConst& v, const MatIP& m, const double S[6], const double TS[6], double dG[6],
+                              double ddG[6][6]) {
+    double dJ2[6];
+    Pmul(S, dJ2, 2.0);
+    double J2 = 0;
+    for (int i = 0; i < 6; i++) J2 += S[i] * dJ2[i];
+    J2 = 0.5 * J2;
+    const bool apex = v.ya > 0.0 && (TS[0] + TS[1] + TS[2] > v.sy * 3.0 / v.ya);
+    if (!apex) {
+        const double c = 0.5 * sqrt(3.0) / sqrt(J2 + 1.0e-6);
+        const double c2 = 0.25 * sqrt(3.0) * pow(J2 + 1.0e-6, -1.5);
+        for (int i = 0; i < 6; i++) dG[i] = c * dJ2[i] + v.pa / 3.0 * (i < 3 ? 1.0 : 0.0);
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) ddG[i][j] = c * Pij(i, j, 2.0) - c2 * dJ2[i] * dJ2[j];
+    } else {
+        // D^-1 (TS - 3 sy / alpha i): isotropic inverse, normal block (I - b/(a+2b) 11^T)/(a-b)
+        double ds[6];
+        for (int i = 0; i < 6; i++) ds[i] = TS[i] - 3.0 * v.sy / v.ya * (i < 3 ? 1.0 : 0.0);
+        const double a = m.Dll, b = m.Dlo, tr = ds[0] + ds[1] + ds[2];
+        for (int i = 0; i < 3; i++) dG[i] = (ds[i] - b / (a + 2.0 * b) * tr) / (a - b);
+        for (int i = 3; i < 6; i++) dG[i] = ds[i] / m.Dss;
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) ddG[i][j] = 0.0;
+    }
+    double z = 0;
+    for (int i = 0; i < 6; i++) z += dG[i] * dG[i];
+    if (z > 0.0) {
+        const double nrm = sqrt(z);
+        for (int i = 0; i < 6; i++) dG[i] /= nrm;
+    }
+}
+
+// SolidMaterial::return_mapping_VEP_getKF (SolidMaterial.cpp:359-406), Damage = 1
+__device__ void getKF_vep(const VEPConst& v, const MatIP& m, double nC, double dt, double h0, const double sol[8],
+                          const double et[6], const double eo[6], double K[8][8], double f[8]) {
+    double S[6], SD[6], TS[6], tmp[6], dF[6], dG[6], ddG[6][6];
+    Dmul(m, sol, S);
+    Pmul(S, SD, 1.0);
+    for (int i = 0; i < 6; i++) tmp[i] = et[i] - eo[i] - sol[6] * SD[i];
+    Dmul(m, tmp, TS);
+    const double FT = yield_vep(v, TS, TS, dF);
+    const double F = yield_vep(v, S, TS, dF);
+    potential_vep(v, m, S, TS, dG, ddG);
+    double h = h0 + v.hard * sqrt(sol[7] * sol[7]);
+    double dh = v.hard * sgn(sol[7]);
+    if (h < -0.9 * v.sy) { h = -0.9 * v.sy; dh = -0.0; }
+    for (int i = 0; i < 6; i++) f[i] = sol[i] - et[i] + eo[i] + sol[6] * SD[i] + sol[7] * dG[i];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double a = 0, b = 0;
+            for (int k = 0; k < 6; k++) {
+                const double dkj = Dij(m, k, j);
+                a += (sol[6] * Pij(i, k, 1.0)) * dkj;
+                b += (sol[7] * ddG[i][k]) * dkj;
+            }
+            K[i][j] = ((i == j ? 1.0 : 0.0) + a) + b;
+        }
+    for (int i = 0; i < 6; i++) { K[i][6] = SD[i]; K[i][7] = dG[i]; }
+    double ss = 0;
+    for (int i = 0; i < 6; i++) ss += SD[i] * SD[i];
+    f[6] = sol[6] - m.ACreep * dt * pow(ss, (nC - 1.0) / 2.0);
+    const double c6 = -m.ACreep * dt * pow(ss, (nC - 3.0) / 2.0) * (nC - 1.0) / 2.0 * 2.0;
+    double r[6], rr[6];
+    for (int i = 0; i < 6; i++) r[i] = c6 * SD[i];
+    Pmul(r, rr, 1.0);  // P symmetric: (r^T P)_j = (P r)_j
+    double KD[6];
+    Dmul(m, rr, KD);   // D symmetric
+    for (int j = 0; j < 6; j++) K[6][j] = KD[j];
+    K[6][6] = 1.0;
+    K[6][7] = 0.0;
+    if (FT - h0 < 0.0) {
+        f[7] = -(v.plv + 1.0e0) / dt * sol[7];
+        for (int j = 0; j < 7; j++) K[7][j] = 0.0;
+        K[7][7] = -(v.plv + 1.0e0) / dt;
+    } else {
+        f[7] = F - h - v.plv / dt * sqrt(sol[7] * sol[7] + 1.0e-12);
+        double dFD[6];
+        Dmul(m, dF, dFD);
+        for (int j = 0; j < 6; j++) K[7][j] = dFD[j];
+        K[7][6] = 0.0;
+        K[7][7] = -v.plv / dt * sgn(sol[7]) - dh;
+    }
+}
+
+// SolidMaterial::return_mapping_VEP (SolidMaterial.cpp:303-357) + _updateStrain (:408-445).
+// hist[0..2] in/out (the initial guess is hist[1], hist[2]); returns DMat = (K^-1)(0:5,0:5).
+__device__ void return_map_vep(const VEPConst& v, const MatIP& m, double nC, double dt, double eps_pl[6], const double et[6],
+                               const double eo[6], double hist[3], double h0, double DMat[6][6], double& diss,
+                               double& ddiss, int& iters, int& maxit) {
+    double sol[8], f[8], fOld[8], dsol[8], K[8][8];
+    for (int i = 0; i < 6; i++) sol[i] = et[i] - eps_pl[i];
+    sol[6] = hist[1];
+    sol[7] = hist[2];
+    getKF_vep(v, m, nC, dt, h0, sol, et, eo, K, f);
+    double e0 = 0;
+    for (int i = 0; i < 8; i++) e0 += f[i] * f[i];
+    e0 = fmax(e0, 1e-12);
+    int it = 0;
+    maxit = 0;
+    while (true) {
+        it += 1;
+        double x[8][1];
+        for (int i = 0; i < 8; i++) x[i][0] = f[i];
+        gauss_pp<8, 1>(K, x);  // K is rebuilt below
+        for (int i = 0; i < 8; i++) { dsol[i] = -x[i][0]; sol[i] += dsol[i]; fOld[i] = f[i]; }
+        getKF_vep(v, m, nC, dt, h0, sol, et, eo, K, f);
+        double ls1 = 0, ls2 = 0;
+        for (int i = 0; i < 8; i++) { ls1 += f[i] * dsol[i]; ls2 += fOld[i] * dsol[i]; }
+        double ls = -(ls2) / (fabs(ls1 - ls2) + 1e-12) * sgn(ls1 - ls2);
+        ls = fmin(fmax(ls, 0.2), 1.0);
+        for (int i = 0; i < 8; i++) sol[i] = sol[i] - (1.0 - ls) * dsol[i];
+        getKF_vep(v, m, nC, dt, h0, sol, et, eo, K, f);
+        double e = 0;
+        for (int i = 0; i < 8; i++) e += f[i] * f[i];
+        if (e < 1.0e-16 || e / e0 < 1.0e-8) break;
+        if (it > 250) { maxit = 1; break; }
+    }
+    iters = it;
+    double S[6], SD[6], TS[6], tmp[6], dG[6], ddG[6][6];
+    Dmul(m, sol, S);
+    Pmul(S, SD, 1.0);
+    for (int i = 0; i < 6; i++) tmp[i] = et[i] - eo[i] - sol[6] * SD[i];
+    Dmul(m, tmp, TS);
+    potential_vep(v, m, S, TS, dG, ddG);
+    double DV = 0, DP = 0;
+    for (int i = 0; i < 6; i++) {
+        const double sve = sol[6] * SD[i], spl = sol[7] * dG[i];
+        eps_pl[i] = (eo[i] + sve) + spl;
+        DV += 1.0 / dt * fabs(sve * S[i]);
+        DP += 1.0 / dt * fabs(spl * S[i]);
+    }
+    hist[0] = h0 + v.hard * sqrt(sol[7] * sol[7]);
+    hist[1] = sol[6];
+    hist[2] = sol[7];
+    diss = DV + DP;
+    ddiss = DP;
+    double B[8][6];
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 6; j++) B[i][j] = i == j ? 1.0 : 0.0;
+    gauss_pp<8, 6>(K, B);
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) DMat[i][j] = B[i][j];
+}
+
 // PhaseFieldMaterial::EnergySplit (PhaseFieldMaterial.cpp:47-94) for None, VolStrains
 // (:194-215) and VolStress (:97-117).  Returns the driving energy.
 __device__ double energy_split(const MatConst& mc, const MatIP& m, const double eps[6], double DD[6][6], double DU[6][6],
@@ -391,7 +626,9 @@ struct MatSmem {
     unsigned cnt[4];
 };
 
-template <int DIM>
+// VEP: the visco-elasto-plastic rheology, a separate instantiation (its 8x8 Newton solve
+// needs far more registers than the Glen map).
+template <int DIM, bool VEP>
 __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
     using E = ET<DIM>;
     constexpr int N = E::N, NIP = E::NIP, EPB = NT / NIP;
@@ -604,7 +841,18 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             double hist0 = mc.HistSize > 0 ? p.MH[gi] : 0.0;
             const int rh = mc.Rheology;
             bool elastic = true;  // DPlast = I
-            if (rh == IFE_RHEO_VE || rh == IFE_RHEO_DVE) {
+            double DM[6][6];      // DPlast of the VEP map
+            if constexpr (VEP) {
+                // phiOld against PhiViscLim, damage 1 (SolidMaterial.cpp:178-186)
+                if (!(phio > mc.PhiViscLim)) {
+                    const VEPConst vc{mc.yield_alpha, mc.pot_alpha, mc.hardening, mc.pl_visc, m.sy};
+                    double hist[3] = {p.MH[gi], p.MH[S + gi], p.MH[2 * S + gi]};
+                    called = true;
+                    elastic = false;
+                    return_map_vep(vc, m, mc.nCreep, p.dt, eps_pl, eps, epo, hist, p.MHOld[gi], DM, diss, ddiss, iters, mx);
+                    for (int k = 0; k < 3; k++) p.MH[k * S + gi] = hist[k];
+                }
+            } else if (rh == IFE_RHEO_VE || rh == IFE_RHEO_DVE) {
                 if (!(phi > mc.PhiViscLim)) {
                     double dmg = 1.0, u1, u2;
                     if (rh == IFE_RHEO_DVE) damage(mc.DamFunction, phi, mc.k0, mc.p, dmg, u1, u2);
@@ -627,7 +875,7 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 my_iters += iters;
                 my_maxit += mx;
             }
-            if (mc.HistSize > 0) p.MH[gi] = hist0;
+            if (!VEP && mc.HistSize > 0) p.MH[gi] = hist0;
             for (int r = 0; r < 6; r++) p.PS[r * S + gi] = eps_pl[r];
             VD = p.VDOld[gi] + diss * p.dt;
             p.VD[gi] = VD;
@@ -641,7 +889,15 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 my_bad |= !isfinite(v);
                 ipo[(IPB_SIG + r) * NIP] = v;
             }
-            // Mt = w * S * DPlast = w * (S - (S a) b^T inv)
+            // Mt = w * S * DPlast = w * (S - (S a) b^T inv)  (VEP: the full DPlast)
+            if (VEP && !elastic) {
+                for (int i = 0; i < 6; i++)
+                    for (int j = 0; j < 6; j++) {
+                        double t = 0;
+                        for (int k = 0; k < 6; k++) t += Sm[i][k] * DM[k][j];
+                        ipo[(IPB_M + i * 6 + j) * NIP] = w * t;
+                    }
+            } else
             for (int i = 0; i < 6; i++) {
                 double sa = 0;
                 if (!elastic)
@@ -1339,8 +1595,14 @@ hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
     const int nip = dim == 2 ? 9 : 27, epb = NT / nip;
     int64_t blocks = (p.NE + epb - 1) / epb;
     if (blocks <= 0) return hipSuccess;
-    if (dim == 2) hipLaunchKernelGGL(material_kernel<2>, dim3((unsigned)blocks), dim3(NT), 0, s, p);
-    else hipLaunchKernelGGL(material_kernel<3>, dim3((unsigned)blocks), dim3(NT), 0, s, p);
+    const bool vep = p.mc.Rheology == IFE_RHEO_VEP;
+    if (vep) {
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, true>), dim3((unsigned)blocks), dim3(NT), 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, true>), dim3((unsigned)blocks), dim3(NT), 0, s, p);
+    } else {
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false>), dim3((unsigned)blocks), dim3(NT), 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, false>), dim3((unsigned)blocks), dim3(NT), 0, s, p);
+    }
     return hipGetLastError();
 }
 

@@ -354,6 +354,9 @@ struct Material {
     // PF utility used by dVE (SolidMaterial.cpp:35-46)
     int DegFunction;
     double k0, p;
+    // visco-elasto-plastic parameters (SolidMaterial.cpp:49-64) and P = I - 1/3 i i^T (:92)
+    double yield_alpha = 0, pot_alpha = 0, hardening = 0, pl_visc = 0;
+    double Pdev[6][6];
 
     void init(const ife_material& m, const ife_phasefield_opts* pf) {
         in = m;
@@ -387,6 +390,16 @@ struct Material {
                 J2Mat_stress[i][j] = (i == j ? 1.0 : 0.0) - 1.0 / 3.0 * ii;
             }
         J2Mat_stress[3][3] = 2.0; J2Mat_stress[4][4] = 2.0; J2Mat_stress[5][5] = 2.0;
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) Pdev[i][j] = (i == j ? 1.0 : 0.0) - 1.0 / 3.0 * ((i < 3 && j < 3) ? 1.0 : 0.0);
+        if (Rheology == IFE_RHEO_VEP || Rheology == IFE_RHEO_DVEP) {
+            yield_alpha = M_PI / 180.0 * m.F_deg;
+            yield_alpha = 6.0 * std::sin(yield_alpha) / (3.0 - std::sin(yield_alpha));
+            pot_alpha = M_PI / 180.0 * m.G_deg;
+            pot_alpha = 6.0 * std::sin(pot_alpha) / (3.0 - std::sin(pot_alpha));
+            hardening = m.hardening;
+            pl_visc = m.pl_visc;
+        }
         pf_l = m.l; Gc = m.Gc; pf_visc = m.Viscosity;
         StressSplit = m.Split;
         ConsistentEnergy = pf ? (pf->ConsistentEnergy != 0) : true;
@@ -561,10 +574,219 @@ struct Material {
         return Diss;
     }
 
+
+    // ---- visco-elasto-plastic return map (SolidMaterial.cpp:303-521) -------------------
+    void Dmat(const double* x, double Damage, V6 out) const {  // (Damage*D)*x
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += (Damage * D[i][j]) * x[j];
+            out[i] = t;
+        }
+    }
+    // SolidMaterial::YieldFunction (SolidMaterial.cpp:447-475); ddF is unused by the map
+    double YieldFunction(const V6 Stress, const V6 TrialStress, V6 dF) const {
+        double I1 = Stress[0] + Stress[1] + Stress[2];
+        V6 dJ2;
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += J2Mat_stress[i][j] * Stress[j];
+            dJ2[i] = t;
+        }
+        double J2 = 0;
+        for (int i = 0; i < 6; i++) J2 += Stress[i] * dJ2[i];
+        J2 = 0.5 * J2;
+        bool Apex = false;
+        if (yield_alpha > 0.0) {
+            double I1Apex = sy * 3.0 / yield_alpha;
+            if (TrialStress[0] + TrialStress[1] + TrialStress[2] > I1Apex) Apex = true;
+        }
+        double F;
+        if (Apex == false) {
+            F = std::sqrt(3.0 * J2 + 1.0e-6) + yield_alpha / 3.0 * I1 - sy;
+            double c = 0.5 * std::sqrt(3.0) / std::sqrt(J2 + 1.0e-6);
+            for (int i = 0; i < 6; i++) dF[i] = c * dJ2[i] + yield_alpha / 3.0 * (i < 3 ? 1.0 : 0.0);
+        } else {
+            F = yield_alpha / 3.0 * I1 - sy;
+            for (int i = 0; i < 6; i++) dF[i] = yield_alpha / 3.0 * (i < 3 ? 1.0 : 0.0);
+        }
+        return F;
+    }
+    // SolidMaterial::PotentialFunction (SolidMaterial.cpp:494-521), dG normalised
+    void PotentialFunction(const V6 Stress, const V6 TrialStress, V6 dG, M6 ddG) const {
+        V6 dJ2;
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += J2Mat_stress[i][j] * Stress[j];
+            dJ2[i] = t;
+        }
+        double J2 = 0;
+        for (int i = 0; i < 6; i++) J2 += Stress[i] * dJ2[i];
+        J2 = 0.5 * J2;
+        bool Apex = false;
+        if (yield_alpha > 0.0) {
+            double I1Apex = sy * 3.0 / yield_alpha;
+            if (TrialStress[0] + TrialStress[1] + TrialStress[2] > I1Apex) Apex = true;
+        }
+        if (Apex == false) {
+            double c = 0.5 * std::sqrt(3.0) / std::sqrt(J2 + 1.0e-6);
+            double c2 = 0.25 * std::sqrt(3.0) * std::pow(J2 + 1.0e-6, -1.5);
+            for (int i = 0; i < 6; i++) dG[i] = c * dJ2[i] + pot_alpha / 3.0 * (i < 3 ? 1.0 : 0.0);
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) ddG[i][j] = c * J2Mat_stress[i][j] - c2 * dJ2[i] * dJ2[j];
+        } else {
+            V6 dStress;
+            for (int i = 0; i < 6; i++) dStress[i] = TrialStress[i] - 3.0 * sy / yield_alpha * (i < 3 ? 1.0 : 0.0);
+            double Dinv[36];
+            lu_inverse(6, &D[0][0], Dinv);
+            for (int i = 0; i < 6; i++) {
+                double t = 0;
+                for (int j = 0; j < 6; j++) t += Dinv[i * 6 + j] * dStress[j];
+                dG[i] = t;
+            }
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) ddG[i][j] = 0.0;
+        }
+        double z = 0;
+        for (int i = 0; i < 6; i++) z += dG[i] * dG[i];
+        if (z > 0.0) {
+            double nrm = std::sqrt(z);
+            for (int i = 0; i < 6; i++) dG[i] /= nrm;
+        }
+    }
+    // SolidMaterial::return_mapping_VEP_getKF (SolidMaterial.cpp:359-406), row-major 8x8 K
+    void getKF_VEP(double K[64], double f[8], const double sol[8], const V6 strain_total, const V6 strain_vep_old, double dt,
+                   double Damage, const double* histOld) const {
+        for (int i = 0; i < 64; i++) K[i] = 0;
+        for (int i = 0; i < 8; i++) f[i] = 0;
+        V6 Stress, SDev, StressTrial, tmp;
+        Dmat(sol, Damage, Stress);
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += Pdev[i][j] * Stress[j];
+            SDev[i] = t;
+        }
+        for (int i = 0; i < 6; i++) tmp[i] = strain_total[i] - strain_vep_old[i] - sol[6] * SDev[i];
+        Dmat(tmp, Damage, StressTrial);
+        V6 dF, dG;
+        M6 ddG;
+        double FTrial = YieldFunction(StressTrial, StressTrial, dF);
+        double F = YieldFunction(Stress, StressTrial, dF);
+        PotentialFunction(Stress, StressTrial, dG, ddG);
+        double h = histOld[0] + hardening * std::sqrt(sol[7] * sol[7]);
+        double dh = hardening * sgn(sol[7]);
+        if (h < -0.9 * sy) { h = -0.9 * sy; dh = -0.0; }
+        for (int i = 0; i < 6; i++) f[i] = sol[i] - strain_total[i] + strain_vep_old[i] + sol[6] * SDev[i] + sol[7] * dG[i];
+        // I + (Damage*sol6*P)*D + (Damage*sol7*ddG)*D
+        const double s6 = Damage * sol[6], s7 = Damage * sol[7];
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) {
+                double a = 0, b = 0;
+                for (int k = 0; k < 6; k++) { a += (s6 * Pdev[i][k]) * D[k][j]; b += (s7 * ddG[i][k]) * D[k][j]; }
+                K[i * 8 + j] = ((i == j ? 1.0 : 0.0) + a) + b;
+            }
+        for (int i = 0; i < 6; i++) { K[i * 8 + 6] = SDev[i]; K[i * 8 + 7] = dG[i]; }
+        double ss = 0;
+        for (int i = 0; i < 6; i++) ss += SDev[i] * SDev[i];
+        f[6] = sol[6] - ACreep * dt * std::pow(ss, (nCreep - 1.0) / 2.0);
+        K[6 * 8 + 6] = 1.0;
+        double c6 = -ACreep * dt * std::pow(ss, (nCreep - 3.0) / 2.0) * (nCreep - 1.0) / 2.0 * 2.0 * Damage;
+        V6 r;
+        for (int j = 0; j < 6; j++) {
+            double t = 0;
+            for (int k = 0; k < 6; k++) t += (c6 * SDev[k]) * Pdev[k][j];
+            r[j] = t;
+        }
+        for (int j = 0; j < 6; j++) {
+            double t = 0;
+            for (int k = 0; k < 6; k++) t += r[k] * D[k][j];
+            K[6 * 8 + j] = t;
+        }
+        if (FTrial - histOld[0] < 0.0) {
+            f[7] = -(pl_visc + 1.0e0) / dt * sol[7];
+            K[7 * 8 + 7] = -(pl_visc + 1.0e0) / dt;
+        } else {
+            f[7] = F - h - pl_visc / dt * std::sqrt(sol[7] * sol[7] + 1.0e-12);
+            for (int j = 0; j < 6; j++) {
+                double t = 0;
+                for (int k = 0; k < 6; k++) t += (Damage * dF[k]) * D[k][j];
+                K[7 * 8 + j] = t;
+            }
+            K[7 * 8 + 7] = -pl_visc / dt * sgn(sol[7]) - dh;
+        }
+    }
+    // SolidMaterial::return_mapping_VEP (SolidMaterial.cpp:303-357) + _updateStrain (:408-445)
+    void return_mapping_VEP(M6 DMat, V6 Strain_Pl, const V6 Strain, const V6 Strain_Pl_Old, double dt, double Damage,
+                            double* hist, const double* histOld, double& DissRate, double& DamDissRate, int& iters,
+                            bool& maxit) const {
+        double K[64], Kinv[64], f[8], fOld[8], sol[8], dsol[8];
+        for (int i = 0; i < 6; i++) sol[i] = Strain[i] - Strain_Pl[i];
+        sol[6] = hist[1];
+        sol[7] = hist[2];
+        getKF_VEP(K, f, sol, Strain, Strain_Pl_Old, dt, Damage, histOld);
+        double e0 = 0;
+        for (int i = 0; i < 8; i++) e0 += f[i] * f[i];
+        e0 = std::max(e0, 1e-12);
+        bool conv = false;
+        unsigned it = 0;
+        maxit = false;
+        while (conv == false) {
+            it += 1;
+            lu_inverse(8, K, Kinv);
+            for (int i = 0; i < 8; i++) {
+                double t = 0;
+                for (int j = 0; j < 8; j++) t += -Kinv[i * 8 + j] * f[j];
+                dsol[i] = t;
+            }
+            for (int i = 0; i < 8; i++) sol[i] += dsol[i];
+            for (int i = 0; i < 8; i++) fOld[i] = f[i];
+            getKF_VEP(K, f, sol, Strain, Strain_Pl_Old, dt, Damage, histOld);
+            double ls1 = 0, ls2 = 0;
+            for (int i = 0; i < 8; i++) { ls1 += f[i] * dsol[i]; ls2 += fOld[i] * dsol[i]; }
+            double lineSearch = -(ls2) / (std::fabs(ls1 - ls2) + 1e-12) * sgn(ls1 - ls2);
+            lineSearch = std::min(std::max(lineSearch, 0.2), 1.0);
+            for (int i = 0; i < 8; i++) sol[i] = sol[i] - (1.0 - lineSearch) * dsol[i];
+            getKF_VEP(K, f, sol, Strain, Strain_Pl_Old, dt, Damage, histOld);
+            double e = 0;
+            for (int i = 0; i < 8; i++) e += f[i] * f[i];
+            if (e < 1.0e-16 || e / e0 < 1.0e-8) conv = true;
+            if (it > 250) { conv = true; maxit = true; }
+        }
+        iters = (int)it;
+        // updateStrain
+        V6 Stress, SDev, StressTrial, tmp, dG;
+        M6 ddG;
+        Dmat(sol, Damage, Stress);
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += Pdev[i][j] * Stress[j];
+            SDev[i] = t;
+        }
+        for (int i = 0; i < 6; i++) tmp[i] = Strain[i] - Strain_Pl_Old[i] - sol[6] * SDev[i];
+        Dmat(tmp, Damage, StressTrial);
+        PotentialFunction(Stress, StressTrial, dG, ddG);
+        double DissV = 0.0, DissP = 0.0;
+        for (int i = 0; i < 6; i++) Strain_Pl[i] = Strain_Pl_Old[i];
+        V6 SVE, SPL;
+        for (int i = 0; i < 6; i++) { SVE[i] = sol[6] * SDev[i]; Strain_Pl[i] += SVE[i]; }
+        for (int i = 0; i < 6; i++) { SPL[i] = sol[7] * dG[i]; Strain_Pl[i] += SPL[i]; }
+        for (int i = 0; i < 6; i++) {
+            DissV += 1.0 / dt * std::fabs(SVE[i] * Stress[i]);
+            DissP += 1.0 / dt * std::fabs(SPL[i] * Stress[i]);
+        }
+        hist[0] = histOld[0] + hardening * std::sqrt(sol[7] * sol[7]);
+        hist[1] = sol[6];
+        hist[2] = sol[7];
+        DissRate = DissV + DissP;
+        DamDissRate = DissP;
+        lu_inverse(8, K, Kinv);
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) DMat[i][j] = Kinv[i * 8 + j];
+    }
+
     // SolidMaterial::UpdatePlasticStrains (SolidMaterial.cpp:123-192)
     void UpdatePlasticStrains(M6 DMat, V6 Strain_Pl, const V6 Strain, const V6 Strain_Pl_Old, const V6 StrainOld, double phi,
-                              double phiOld, double dt, double* hist, double& DissRate, double& DamDissRate, int& iters,
-                              bool& maxit, bool& called) const {
+                              double phiOld, double dt, double* hist, const double* histOld, double& DissRate,
+                              double& DamDissRate, int& iters, bool& maxit, bool& called) const {
         DissRate = 0.0; DamDissRate = 0.0;
         double damage = 1.0, u1, u2;
         iters = 0; maxit = false; called = false;
@@ -604,6 +826,14 @@ struct Material {
                     DissRate = return_mapping_Viscous(DMat, Strain_Pl, StrainOld, Strain_Pl_Old, damage, dt, hist, iters, maxit);
                     for (int i = 0; i < 6; i++)
                         for (int j = 0; j < 6; j++) DMat[i][j] = (i == j) ? 1.0 : 0.0;
+                }
+                break;
+            case IFE_RHEO_VEP:  // damage stays 1 (the degradation call is commented out, :183)
+                if (phiOld > PhiViscLim) identity();
+                else {
+                    called = true;
+                    return_mapping_VEP(DMat, Strain_Pl, Strain, Strain_Pl_Old, dt, 1.0, hist, histOld, DissRate, DamDissRate,
+                                       iters, maxit);
                 }
                 break;
             default: throw std::invalid_argument("rheology not supported by the oracle");
@@ -978,13 +1208,13 @@ static void Assemble_U(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, St
             }
             size_t gi = (size_t)el * nip + ip;
             for (int r = 0; r < 6; r++) { Strain_Pl_Old[r] = md.PSOld[r * NIPT + gi]; Strain_Pl[r] = md.PS[r * NIPT + gi]; }
-            double hist[4] = {0, 0, 0, 0};
-            for (int k = 0; k < M.HistSize; k++) hist[k] = md.MH[k * NIPT + gi];
+            double hist[4] = {0, 0, 0, 0}, histOld[4] = {0, 0, 0, 0};
+            for (int k = 0; k < M.HistSize; k++) { hist[k] = md.MH[k * NIPT + gi]; histOld[k] = md.MHOld[k * NIPT + gi]; }
             M6 DPlast;
             double PlasticDiss, DamPlasticDiss;
             int iters; bool maxit, called;
-            M.UpdatePlasticStrains(DPlast, Strain_Pl, Strain, Strain_Pl_Old, StrainOld, phi, phi_Old, dt, hist, PlasticDiss,
-                                   DamPlasticDiss, iters, maxit, called);
+            M.UpdatePlasticStrains(DPlast, Strain_Pl, Strain, Strain_Pl_Old, StrainOld, phi, phi_Old, dt, hist, histOld,
+                                   PlasticDiss, DamPlasticDiss, iters, maxit, called);
             if (called) { st.calls++; st.iters += iters; st.maxit += maxit ? 1 : 0; }
             for (int k = 0; k < M.HistSize; k++) md.MH[k * NIPT + gi] = hist[k];
             for (int r = 0; r < 6; r++) md.PS[r * NIPT + gi] = Strain_Pl[r];
@@ -1378,6 +1608,24 @@ double ifo_return_map_viscous(const ife_material* mat, double T, double poros, d
         for (int j = 0; j < 6; j++) DMat[i * 6 + j] = Dm[i][j];
     *iters = it;
     return d;
+}
+
+// SolidMaterial::return_mapping_VEP at temperature T: hist_io[0..2] in/out (initial guess
+// hist[1], hist[2]), hist_old[0] the hardening history; diss2 = {DissRate, DamDissRate}.
+void ifo_return_map_vep(const ife_material* mat, double T, double poros, double dt, const double* strain, double* strain_pl,
+                        const double* strain_pl_old, double* hist_io, const double* hist_old, double* DMat, double* diss2,
+                        int* iters) {
+    Material M;
+    M.init(*mat, nullptr);
+    M.SetTemperature(T, poros);
+    M6 Dm;
+    int it; bool mx;
+    double h[4] = {hist_io[0], hist_io[1], hist_io[2], 0.0}, ho[4] = {hist_old[0], 0, 0, 0};
+    M.return_mapping_VEP(Dm, strain_pl, strain, strain_pl_old, dt, 1.0, h, ho, diss2[0], diss2[1], it, mx);
+    for (int k = 0; k < 3; k++) hist_io[k] = h[k];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) DMat[i * 6 + j] = Dm[i][j];
+    *iters = it;
 }
 
 double ifo_energy_split(const ife_material* mat, int consistent, double T, double poros, const double* strain, double* D_Dam,

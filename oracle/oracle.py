@@ -43,6 +43,8 @@ def load():
         "ifo_damage": (None, [C.c_int, C.c_double, C.c_double, C.c_double, dp]),
         "ifo_return_map_viscous": (C.c_double, [C.POINTER(abi.Material), C.c_double, C.c_double, C.c_double,
                                                 C.c_double, dp, dp, dp, dp, dp, C.POINTER(C.c_int)]),
+        "ifo_return_map_vep": (None, [C.POINTER(abi.Material), C.c_double, C.c_double, C.c_double, dp, dp, dp, dp,
+                                      dp, dp, dp, C.POINTER(C.c_int)]),
         "ifo_energy_split": (C.c_double, [C.POINTER(abi.Material), C.c_int, C.c_double, C.c_double, dp, dp, dp,
                                           dp, dp]),
         "ifo_create": (P, [C.POINTER(abi.MeshDesc), C.POINTER(abi.DofsDesc), C.c_int, C.POINTER(ModelDesc)]),

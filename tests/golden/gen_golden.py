@@ -16,7 +16,7 @@ from ice_fem_2025a_amd.config import make_problem  # noqa: E402
 from oracle.oracle import Oracle  # noqa: E402
 
 CASES = [("quad2d_le_at2", (2, 2)), ("quad2d_le_at2", (3, 2)), ("hex3d_glen_at2", (2, 2, 2)),
-         ("hex3d_glen_at2", (1, 2, 3))]
+         ("hex3d_glen_at2", (1, 2, 3)), ("hex3d_vep_at2", (2, 2, 2))]
 HIST = ["PlasticStrain", "MatHist", "ViscDiss", "DamViscDiss", "Energy"]
 
 
@@ -45,7 +45,10 @@ def make(name, nel):
 
 
 if __name__ == "__main__":
+    only = sys.argv[1:]  # optional: names of the fixtures to (re)write
     for name, nel in CASES:
+        if only and fixture_name(name, nel) not in only:
+            continue
         d = make(name, nel)
         p = os.path.join(HERE, fixture_name(name, nel))
         np.savez_compressed(p, **d)
