@@ -46,7 +46,8 @@ class Material(C.Structure):
                                           "Viscosity")] + \
                [("Split", C.c_int32)] + \
                [(n, C.c_double) for n in ("TRef", "TOffset", "QCreep", "ft0", "ftdeg", "sydeg",
-                                          "ThermalConductivity", "HeatCapacity")]
+                                          "ThermalConductivity", "HeatCapacity", "B_DP", "eRef", "cReduce",
+                                          "StarConvexGamma")]
 
 
 class PhaseFieldOpts(C.Structure):

@@ -129,6 +129,8 @@ def material_struct(inputs, mat_name):
     s.TRef, s.TOffset, s.QCreep = td.get("TRef", TREF), td.get("TOffset", 0.0), td.get("QCreep", 0.0)
     s.ft0, s.ftdeg, s.sydeg = td.get("ft0", 0.0), td.get("ftdeg", 0.0), td.get("sydeg", 0.0)
     s.ThermalConductivity, s.HeatCapacity = tp.get("ThermalConductivity", 0.0), tp.get("HeatCapacity", 0.0)
+    s.B_DP, s.eRef, s.cReduce = fp.get("B_DP", 0.0), fp.get("eRef", 0.0), fp.get("cReduce", 0.0)
+    s.StarConvexGamma = fp.get("StarConvexGamma", 0.0)
     return s
 
 

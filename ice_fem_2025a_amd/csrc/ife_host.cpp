@@ -242,6 +242,7 @@ MatConst make_matconst(const ife_physics* ph, const Model* pfm, const Model* thm
     mc.Split = m.Split;
     mc.TRef = m.TRef; mc.TOffset = m.TOffset; mc.QCreep = m.QCreep; mc.ft0 = m.ft0; mc.ftdeg = m.ftdeg;
     mc.sy0 = m.sy; mc.sydeg = m.sydeg;
+    mc.B_dp = m.B_DP; mc.Ice_eRef = m.eRef; mc.Ice_cReduce = m.cReduce; mc.StarConvexGamma = m.StarConvexGamma;
     if (m.Rheology == IFE_RHEO_VEP || m.Rheology == IFE_RHEO_DVEP) {  // SolidMaterial.cpp:49-64
         const double ya = M_PI / 180.0 * m.F_deg, pa = M_PI / 180.0 * m.G_deg;
         mc.yield_alpha = 6.0 * std::sin(ya) / (3.0 - std::sin(ya));
@@ -721,8 +722,11 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
                   r == IFE_RHEO_VEP))
                 THROW(IFE_ERR_UNSUPPORTED, "rheology not available for Fracture/PhaseField (LE, VE, dVE, VE_expl, dVE_expl, VEP are)");
             m->HistSize = (r == IFE_RHEO_LE) ? 0 : (r == IFE_RHEO_VEP ? 4 : 1);  // SolidMaterial.cpp:26-52
-            if (!(mat->Split == IFE_SPLIT_NONE || mat->Split == IFE_SPLIT_VOLSTRAINS || mat->Split == IFE_SPLIT_VOLSTRESS))
-                THROW(IFE_ERR_UNSUPPORTED, "energy split not implemented by the engine (None, VolStrains, VolStress are)");
+            if (mat->Split < IFE_SPLIT_NONE || mat->Split > IFE_SPLIT_STARCONVEX)
+                THROW(IFE_ERR_INVALID, "unknown energy split");
+            // the reference throws for this combination (PhaseFieldMaterial.cpp:150-152)
+            if (mat->Split == IFE_SPLIT_SPECSTRESS && pf->ConsistentEnergy)
+                THROW(IFE_ERR_UNSUPPORTED, "SpecStress split with ConsistentEnergy is not defined by the reference");
             int dfn = pf->DistributionFunction;
             if (!(dfn == IFE_DIST_LINEAR || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR2 || dfn == IFE_DIST_QUADRATIC ||
                   dfn == IFE_DIST_LOCAL))

@@ -21,6 +21,8 @@ struct MatConst {
     // visco-elasto-plastic (SolidMaterial.cpp:49-64): sy at TRef, its T slope, Drucker-Prager
     // coefficients 6 sin / (3 - sin) of the friction / dilatancy angles, hardening, plastic viscosity
     double sy0, sydeg, yield_alpha, pot_alpha, hardening, pl_visc;
+    // DruckerPrager / ICE / StarConvex split parameters (PhaseFieldMaterial.cpp:22-31)
+    double B_dp, Ice_eRef, Ice_cReduce, StarConvexGamma;
     double k, cp, Density_th;
     // PF utility options
     int DamFunction, DensityDamage, GravityDamage, ViscoDriving;

@@ -485,8 +485,223 @@ __device__ void return_map_vep(const VEPConst& v, const MatIP& m, double nC, dou
         for (int j = 0; j < 6; j++) DMat[i][j] = B[i][j];
 }
 
-// PhaseFieldMaterial::EnergySplit (PhaseFieldMaterial.cpp:47-94) for None, VolStrains
-// (:194-215) and VolStress (:97-117).  Returns the driving energy.
+
+// ---- further energy splits (PhaseFieldMaterial.cpp:119-408) --------------------------------
+// PhaseFieldMaterial::GetEigenValues2D (:410-458): plane-strain principal strains, zz first
+__device__ void eigen2d(const double su[6], double s[3], double ds[3][6], double dds[3][6][6]) {
+    double si[6];
+    for (int i = 0; i < 6; i++) si[i] = 1.0 * su[i];
+    for (int ic = 0; ic < 3; ic++)
+        if (fabs(si[ic]) < 1.0e-6) si[ic] = (ic + 1.0) * 1.0e-6;
+    for (int ic = 0; ic < 3; ic++) {
+        for (int i = 0; i < 6; i++) {
+            ds[ic][i] = 0;
+            for (int j = 0; j < 6; j++) dds[ic][i][j] = 0;
+        }
+        if (ic == 0) {
+            s[0] = su[2];
+            ds[0][2] = 1.0;
+            continue;
+        }
+        const double sg = 2.0 * ic - 3.0, e = 2.220446049250313e-16;  // DBL_EPSILON
+        const double st = si[0] * si[0] + si[1] * si[1] - 2.0 * si[0] * si[1] + 4 * si[3] * si[3];
+        const double stu = su[0] * su[0] + su[1] * su[1] - 2.0 * su[0] * su[1] + 4 * su[3] * su[3];
+        s[ic] = 0.5 * (su[0] + su[1] + sg * pow(stu, 0.5));
+        const double r1 = pow(st + e, -0.5), r3 = pow(st + e, -1.5);
+        ds[ic][0] = 0.5 + 0.5 * sg * (si[0] - si[1]) * r1;
+        ds[ic][1] = 0.5 + 0.5 * sg * (-si[0] + si[1]) * r1;
+        ds[ic][3] = 2.0 * sg * si[3] * r1;
+        const double c1 = 2.0 * si[3] * (si[0] - si[1]) * r3;
+        const double c2 = 2.0 * si[3] * si[3] * r3;
+        const double c3 = 2.0 * (si[0] - si[1]) * (si[0] - si[1]) * r3;
+        dds[ic][0][0] = sg * c2;  dds[ic][0][1] = -sg * c2; dds[ic][0][3] = -sg * c1;
+        dds[ic][1][0] = -sg * c2; dds[ic][1][1] = sg * c2;  dds[ic][1][3] = sg * c1;
+        dds[ic][3][0] = -sg * c1; dds[ic][3][1] = sg * c1;  dds[ic][3][3] = sg * c3;
+    }
+}
+
+// symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations
+__device__ void eig3(const double A[3][3], double lam[3], double V[3][3]) {
+    double a[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) { a[i][j] = A[i][j]; V[i][j] = i == j ? 1.0 : 0.0; }
+    for (int sweep = 0; sweep < 32; sweep++) {
+        const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        const double dg = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (off <= 1e-40 * dg || off == 0.0) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                if (a[p][q] == 0.0) continue;
+                const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < 3; k++) {
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - sn * akq;
+                    a[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - sn * aqk;
+                    a[q][k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; k++) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - sn * vkq;
+                    V[k][q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < 3; i++) lam[i] = a[i][i];
+}
+
+// SpecStrainSplit (:247-297)
+__device__ double split_spec_strain(const MatIP& m, const double eps[6], double DD[6][6], double DU[6][6], double SD[6],
+                                    double SU[6]) {
+    const double VC[6] = {1, 1, 1, 0.5, 0.5, 0.5};
+    double E = 0, pri[3], dp[3][6], ddp[3][6][6], sv[6];
+    for (int i = 0; i < 6; i++) sv[i] = VC[i] * eps[i];
+    eigen2d(sv, pri, dp, ddp);
+    const double p0 = pri[0] + pri[1] + pri[2];
+    double pd1[6];
+    for (int i = 0; i < 6; i++) pd1[i] = dp[0][i] + dp[1][i] + dp[2][i];
+    if (p0 >= 0) E += 0.5 * m.Lame * pow(p0, 2);
+    double* Sv = p0 >= 0 ? SD : SU;
+    double (*Dv)[6] = p0 >= 0 ? DD : DU;
+    for (int i = 0; i < 6; i++) {
+        Sv[i] += m.Lame * p0 * pd1[i];
+        for (int j = 0; j < 6; j++)
+            Dv[i][j] += m.Lame * (p0 * (ddp[0][i][j] + ddp[1][i][j] + ddp[2][i][j]) + pd1[i] * pd1[j]);
+    }
+    for (int ic = 0; ic < 3; ic++) {
+        double* Sc = pri[ic] >= 0 ? SD : SU;
+        double (*Dc)[6] = pri[ic] >= 0 ? DD : DU;
+        if (pri[ic] >= 0) E += m.Shear * pow(pri[ic], 2);
+        for (int i = 0; i < 6; i++) {
+            Sc[i] += m.Shear * (2 * pri[ic] * dp[ic][i]);
+            for (int j = 0; j < 6; j++) Dc[i][j] += m.Shear * (2 * pri[ic] * ddp[ic][i][j] + 2 * dp[ic][i] * dp[ic][j]);
+        }
+    }
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            DD[i][j] = VC[i] * DD[i][j] * VC[j];
+            DU[i][j] = VC[i] * DU[i][j] * VC[j];
+        }
+    for (int i = 0; i < 6; i++) {
+        double a = 0, b = 0;
+        for (int j = 0; j < 6; j++) { a += DD[i][j] * eps[j]; b += DU[i][j] * eps[j]; }
+        SD[i] = a;
+        SU[i] = b;
+    }
+    return E;
+}
+
+// SpecStressSplit (:119-192): energy of the positive part of the stress (ConsistentEnergy off)
+__device__ double split_spec_stress(const MatIP& m, const double eps[6]) {
+    double sn[6];
+    Dmul(m, eps, sn);
+    for (int i = 0; i < 4; i++)
+        if (sn[i] * sn[i] < 1.0e-9) sn[i] = (i + 1) * 1e-9;
+    const double A[3][3] = {{sn[0], sn[3], sn[5]}, {sn[3], sn[1], sn[4]}, {sn[5], sn[4], sn[2]}};
+    double lam[3], V[3][3], P[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    eig3(A, lam, V);
+    for (int k = 0; k < 3; k++)
+        if (lam[k] > 0)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) P[i][j] += V[i][k] * lam[k] * V[j][k];
+    const double sv[6] = {P[0][0], P[1][1], P[2][2], 0.5 * (P[0][1] + P[1][0]), 0.5 * (P[1][2] + P[2][1]), 0.5 * (P[0][2] + P[2][0])};
+    double t = 0;
+    for (int i = 0; i < 6; i++) t += sv[i] * eps[i];
+    return 0.5 * t;
+}
+
+// DruckerPragerSplit (:299-354)
+__device__ double split_drucker_prager(const MatConst& mc, const MatIP& m, const double eps[6], double DD[6][6],
+                                       double DU[6][6], double SD[6], double SU[6]) {
+    const double B = mc.B_dp, K = m.Bulk, G = m.Shear;
+    const double I1 = eps[0] + eps[1] + eps[2];
+    double dJ2[6];
+    Pmul(eps, dJ2, 0.5);  // eps^T J2Mat_strain (symmetric; shears 2/4)
+    double J2 = 0;
+    for (int j = 0; j < 6; j++) J2 += (0.5 * dJ2[j]) * eps[j];
+    const double sq = sqrt(J2 + 1.0e-16);
+    double dsq[6];
+    for (int i = 0; i < 6; i++) dsq[i] = 0.5 / sq * dJ2[i];
+    auto ddsq = [&](int i, int j) { return 0.5 / sq * Pij(i, j, 0.5) - 0.25 / sq / sq / sq * dJ2[i] * dJ2[j]; };
+    auto full = [&](double S[6], double Dm[6][6]) {
+        for (int i = 0; i < 6; i++) {
+            const double ii = i < 3 ? 1.0 : 0.0;
+            S[i] = K * I1 * ii + 2.0 * G * dJ2[i];
+            for (int j = 0; j < 6; j++) Dm[i][j] = K * ii * (j < 3 ? 1.0 : 0.0) + 2.0 * G * Pij(i, j, 0.5);
+        }
+    };
+    if (-6.0 * B * sqrt(J2) < I1) {
+        full(SD, DD);
+        return 0.5 * K * pow(I1, 2) + 2.0 * G * J2;
+    }
+    if (2 * G * sqrt(J2) < 3 * B * K * I1) {
+        full(SU, DU);
+        return 0.0;
+    }
+    double pre = 1.0 / (18.0 * B * B * K + 2.0 * G);
+    double q = -3.0 * B * K * I1 + 2 * G * sq, dq[6];
+    for (int i = 0; i < 6; i++) dq[i] = -3.0 * B * K * (i < 3 ? 1.0 : 0.0) + 2.0 * G * dsq[i];
+    const double E = pre * pow(q, 2);
+    for (int i = 0; i < 6; i++) {
+        SD[i] = pre * 2.0 * q * dq[i];
+        for (int j = 0; j < 6; j++) DD[i][j] = pre * (2.0 * q * (2.0 * G * ddsq(i, j)) + 2.0 * dq[i] * dq[j]);
+    }
+    pre = K * G / (18.0 * B * B * K + 2.0 * G);
+    q = I1 + 6 * B * sq;
+    for (int i = 0; i < 6; i++) dq[i] = (i < 3 ? 1.0 : 0.0) + 6 * B * dsq[i];
+    for (int i = 0; i < 6; i++) {
+        SU[i] = pre * 2.0 * q * dq[i];
+        for (int j = 0; j < 6; j++) DU[i][j] = pre * (2.0 * q * (6 * B * ddsq(i, j)) + 2.0 * dq[i] * dq[j]);
+    }
+    return E;
+}
+
+// IceSplit (:356-408) and StarConvexSplit (:217-245)
+__device__ double split_vol_dev(const MatConst& mc, const MatIP& m, const double eps[6], bool ice, double DD[6][6],
+                                double DU[6][6], double SD[6], double SU[6]) {
+    const double VC[6] = {1, 1, 1, 0.5, 0.5, 0.5}, VDD[6] = {1, 1, 1, 2, 2, 2};
+    const double K = m.Bulk, G = m.Shear, g = ice ? 0.0 : mc.StarConvexGamma;
+    const double p = eps[0] + eps[1] + eps[2];
+    double dev[6];
+    for (int i = 0; i < 6; i++) dev[i] = VC[i] * eps[i] - (i < 3 ? 1.0 / 3.0 * p : 0.0);
+    const double pp = p >= 0 ? p : 0.0, pm = p >= 0 ? 0.0 : p, dpp = p >= 0 ? 1.0 : 0.0, dpm = p >= 0 ? 0.0 : 1.0;
+    double E = 0.5 * K * (ice ? pp * pp : (pp * pp - g * pm * pm));
+    for (int i = 0; i < 3; i++) {
+        SD[i] += K * (ice ? pp * dpp : (pp * dpp - g * pm * dpm));
+        SU[i] += (ice ? 1.0 : (1.0 + g)) * K * pm * dpm;
+        for (int j = 0; j < 3; j++) {
+            DD[i][j] += K * (ice ? dpp * dpp : (dpp * dpp - g * dpm * dpm));
+            DU[i][j] += (ice ? 1.0 : (1.0 + g)) * K * dpm * dpm;
+        }
+    }
+    double t = 0;
+    for (int i = 0; i < 6; i++) t += dev[i] * VDD[i] * dev[i];
+    E += G * t;
+    for (int i = 0; i < 6; i++) SD[i] += 2 * G * VC[i] * VDD[i] * dev[i];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) DD[i][j] += 2 * G * VC[i] * VDD[i] * VC[i] * Pij(i, j, 1.0);
+    if (ice) {
+        E += -0.5 * K * mc.Ice_cReduce * pm * pm / (mc.Ice_eRef * mc.Ice_eRef + pm * pm);
+        for (int i = 0; i < 6; i++) {
+            double a = 0, b = 0;
+            for (int j = 0; j < 6; j++) { a += DD[i][j] * eps[j]; b += DU[i][j] * eps[j]; }
+            SD[i] = a;
+            SU[i] = b;
+        }
+        if (E < 0.0) E = 0.0;
+        if (E > 1.0e12 || isnan(E)) E = 1.0e12;
+        if (E < -1.0e12) E = -1.0e12;
+    }
+    return E;
+}
+
+// PhaseFieldMaterial::EnergySplit (PhaseFieldMaterial.cpp:47-94): None, VolStrains
+// (:194-215), VolStress (:97-117) and the splits above.  Returns the driving energy.
 __device__ double energy_split(const MatConst& mc, const MatIP& m, const double eps[6], double DD[6][6], double DU[6][6],
                                double SD[6], double SU[6]) {
     double E = 0;
@@ -525,6 +740,14 @@ __device__ double energy_split(const MatConst& mc, const MatIP& m, const double 
                 double P = (i == j ? 1.0 : 0.0) - ((i < 3 && j < 3) ? 1.0 / 3.0 : 0.0);
                 DD[i][j] += 2 * m.Shear * VC[i] * VDD[i] * VC[i] * P;
             }
+    } else if (mc.Split == IFE_SPLIT_SPECSTRAINS) {
+        E = split_spec_strain(m, eps, DD, DU, SD, SU);
+    } else if (mc.Split == IFE_SPLIT_SPECSTRESS) {
+        E = split_spec_stress(m, eps);  // stress and tangent come from the override below
+    } else if (mc.Split == IFE_SPLIT_DRUCKERPRAGER) {
+        E = split_drucker_prager(mc, m, eps, DD, DU, SD, SU);
+    } else if (mc.Split == IFE_SPLIT_ICE || mc.Split == IFE_SPLIT_STARCONVEX) {
+        E = split_vol_dev(mc, m, eps, mc.Split == IFE_SPLIT_ICE, DD, DU, SD, SU);
     } else {  // VolStress
         double S[6];
         Dmul(m, eps, S);

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define IFE_API_VERSION 1
+#define IFE_API_VERSION 2
 
 typedef enum {
     IFE_OK = 0,
@@ -129,6 +129,9 @@ typedef struct {
     double TRef, TOffset, QCreep, ft0, ftdeg, sydeg;
     /* ThermalProperties */
     double ThermalConductivity, HeatCapacity;
+    /* FractureProperties of the DruckerPrager, ICE and StarConvex splits
+     * (PhaseFieldMaterial.cpp:22-31): B_DP, eRef, cReduce, StarConvexGamma */
+    double B_DP, eRef, cReduce, StarConvexGamma;
 } ife_material;
 
 /* properties.PhaseField.* (PhaseFieldUtil.cpp:4-39, PhaseField.cpp:196-267,

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -346,6 +347,8 @@ struct Material {
     double pf_l, Gc, pf_visc;
     int StressSplit;
     bool ConsistentEnergy;
+    double B_dp, Ice_eRef, Ice_cReduce, StarConvexGamma;
+    double J2Mat_strain[6][6];  // SolidMaterial.cpp:102-106
     // IceMaterial.cpp:5-45
     double TRef, TOffset, A0, QCreep, ft0, ftdeg, sy, sy0, sydeg, Young0;
     const double R = 8.31446261815324;  // BaseMaterial.h:26
@@ -402,6 +405,10 @@ struct Material {
         }
         pf_l = m.l; Gc = m.Gc; pf_visc = m.Viscosity;
         StressSplit = m.Split;
+        B_dp = m.B_DP; Ice_eRef = m.eRef; Ice_cReduce = m.cReduce; StarConvexGamma = m.StarConvexGamma;
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) J2Mat_strain[i][j] = (i == j ? 1.0 : 0.0) - 1.0 / 3.0 * ((i < 3 && j < 3) ? 1.0 : 0.0);
+        J2Mat_strain[3][3] = 2.0 / 4.0; J2Mat_strain[4][4] = 2.0 / 4.0; J2Mat_strain[5][5] = 2.0 / 4.0;
         ConsistentEnergy = pf ? (pf->ConsistentEnergy != 0) : true;
         TOffset = m.TOffset; TRef = m.TRef;
         A0 = ACreep; QCreep = m.QCreep; ft0 = m.ft0; ftdeg = m.ftdeg;
@@ -840,8 +847,249 @@ struct Material {
         }
     }
 
-    // PhaseFieldMaterial::EnergySplit (PhaseFieldMaterial.cpp:47-94) with VolStress
-    // (:97-117) and VolStrains (:194-215).
+
+    // PhaseFieldMaterial::GetEigenValues2D (PhaseFieldMaterial.cpp:410-458): plane-strain
+    // principal strains (zz first, then the in-plane pair) and their derivatives
+    static void GetEigenValues2D(const V6 s_unf, double s[3], V6 ds[3], M6 dds[3]) {
+        V6 s_in;
+        for (int i = 0; i < 6; i++) s_in[i] = 1.0 * s_unf[i];
+        for (int ic = 0; ic < 3; ic++)
+            if (std::fabs(s_in[ic]) < 1.0e-6) s_in[ic] = (ic + 1.0) * 1.0e-6;
+        for (int ic = 0; ic < 3; ic++) {
+            for (int i = 0; i < 6; i++) {
+                ds[ic][i] = 0;
+                for (int j = 0; j < 6; j++) dds[ic][i][j] = 0;
+            }
+            if (ic == 0) {
+                s[ic] = s_unf[2];
+                ds[ic][2] = 1.0;
+            } else {
+                double sg = 2.0 * ic - 3.0;
+                double st = s_in[0] * s_in[0] + s_in[1] * s_in[1] - 2.0 * s_in[0] * s_in[1] + 4 * s_in[3] * s_in[3];
+                double stu = s_unf[0] * s_unf[0] + s_unf[1] * s_unf[1] - 2.0 * s_unf[0] * s_unf[1] + 4 * s_unf[3] * s_unf[3];
+                s[ic] = 0.5 * (s_unf[0] + s_unf[1] + sg * std::pow(stu, 0.5));
+                const double e = DBL_EPSILON;
+                ds[ic][0] = 0.5 + 0.5 * sg * (s_in[0] - s_in[1]) * std::pow(st + e, -0.5);
+                ds[ic][1] = 0.5 + 0.5 * sg * (-s_in[0] + s_in[1]) * std::pow(st + e, -0.5);
+                ds[ic][3] = 2.0 * sg * s_in[3] * std::pow(st + e, -0.5);
+                double c1 = 2.0 * s_in[3] * (s_in[0] - s_in[1]) * std::pow(st + e, -1.5);
+                double c2 = 2.0 * s_in[3] * s_in[3] * std::pow(st + e, -1.5);
+                double c3 = 2.0 * (s_in[0] - s_in[1]) * (s_in[0] - s_in[1]) * std::pow(st + e, -1.5);
+                dds[ic][0][0] = sg * c2;  dds[ic][0][1] = -sg * c2; dds[ic][0][3] = -sg * c1;
+                dds[ic][1][0] = -sg * c2; dds[ic][1][1] = sg * c2;  dds[ic][1][3] = sg * c1;
+                dds[ic][3][0] = -sg * c1; dds[ic][3][1] = sg * c1;  dds[ic][3][3] = sg * c3;
+            }
+        }
+    }
+
+    // symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (stands in for
+    // Eigen::SelfAdjointEigenSolver; the positive-part projector sum it feeds is unique)
+    static void eig3(const double A[3][3], double lam[3], double V[3][3]) {
+        double a[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) { a[i][j] = A[i][j]; V[i][j] = i == j ? 1.0 : 0.0; }
+        for (int sweep = 0; sweep < 32; sweep++) {
+            double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+            double dg = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+            if (off <= 1e-40 * dg || off == 0.0) break;
+            for (int p = 0; p < 2; p++)
+                for (int q = p + 1; q < 3; q++) {
+                    if (a[p][q] == 0.0) continue;
+                    double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                    double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                    double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                    for (int k = 0; k < 3; k++) {  // A <- A J
+                        double akp = a[k][p], akq = a[k][q];
+                        a[k][p] = c * akp - sn * akq;
+                        a[k][q] = sn * akp + c * akq;
+                    }
+                    for (int k = 0; k < 3; k++) {  // A <- J^T A
+                        double apk = a[p][k], aqk = a[q][k];
+                        a[p][k] = c * apk - sn * aqk;
+                        a[q][k] = sn * apk + c * aqk;
+                    }
+                    for (int k = 0; k < 3; k++) {
+                        double vkp = V[k][p], vkq = V[k][q];
+                        V[k][p] = c * vkp - sn * vkq;
+                        V[k][q] = sn * vkp + c * vkq;
+                    }
+                }
+        }
+        for (int i = 0; i < 3; i++) lam[i] = a[i][i];
+    }
+
+    double SpecStrainSplit(M6 D_Dam, M6 D_UnDam, V6 S_Dam, V6 S_Undam, const V6 Strain) const {  // :247-297
+        double Energy = 0.0;
+        double Pri[3];
+        V6 dPri[3];
+        M6 d2Pri[3];
+        V6 SV;
+        for (int i = 0; i < 6; i++) SV[i] = VoightCorrect[i] * Strain[i];
+        GetEigenValues2D(SV, Pri, dPri, d2Pri);
+        double p0 = Pri[0] + Pri[1] + Pri[2];
+        V6 pd1;
+        M6 pd2;
+        for (int i = 0; i < 6; i++) {
+            pd1[i] = dPri[0][i] + dPri[1][i] + dPri[2][i];
+            for (int j = 0; j < 6; j++) pd2[i][j] = d2Pri[0][i][j] + d2Pri[1][i][j] + d2Pri[2][i][j];
+        }
+        double* Sv = p0 >= 0 ? S_Dam : S_Undam;
+        double (*Dm)[6] = p0 >= 0 ? D_Dam : D_UnDam;
+        if (p0 >= 0) Energy += 0.5 * Lame * std::pow(p0, 2);
+        for (int i = 0; i < 6; i++) {
+            Sv[i] += Lame * p0 * pd1[i];
+            for (int j = 0; j < 6; j++) Dm[i][j] += Lame * (p0 * pd2[i][j] + pd1[i] * pd1[j]);
+        }
+        for (int ic = 0; ic < 3; ic++) {
+            V6 d2;
+            M6 dd2;
+            for (int i = 0; i < 6; i++) {
+                d2[i] = 2 * Pri[ic] * dPri[ic][i];
+                for (int j = 0; j < 6; j++) dd2[i][j] = 2 * Pri[ic] * d2Pri[ic][i][j] + 2 * dPri[ic][i] * dPri[ic][j];
+            }
+            double* Sc = Pri[ic] >= 0 ? S_Dam : S_Undam;
+            double (*Dc)[6] = Pri[ic] >= 0 ? D_Dam : D_UnDam;
+            if (Pri[ic] >= 0) Energy += Shear * std::pow(Pri[ic], 2);
+            for (int i = 0; i < 6; i++) {
+                Sc[i] += Shear * d2[i];
+                for (int j = 0; j < 6; j++) Dc[i][j] += Shear * dd2[i][j];
+            }
+        }
+        for (int i = 0; i < 6; i++) {
+            for (int j = 0; j < 6; j++) {
+                D_Dam[i][j] = VoightCorrect[i] * D_Dam[i][j] * VoightCorrect[j];
+                D_UnDam[i][j] = VoightCorrect[i] * D_UnDam[i][j] * VoightCorrect[j];
+            }
+        }
+        for (int i = 0; i < 6; i++) {  // stresses from the tangents (PhaseFieldMaterial.cpp:293-294)
+            double a = 0, b = 0;
+            for (int j = 0; j < 6; j++) { a += D_Dam[i][j] * Strain[j]; b += D_UnDam[i][j] * Strain[j]; }
+            S_Dam[i] = a;
+            S_Undam[i] = b;
+        }
+        return Energy;
+    }
+
+    double SpecStressSplit(const V6 Strain) const {  // :119-192, energy only (ConsistentEnergy false)
+        V6 Sn;
+        for (int i = 0; i < 6; i++) {
+            double t = 0;
+            for (int j = 0; j < 6; j++) t += D[i][j] * Strain[j];
+            Sn[i] = t;
+        }
+        for (int i = 0; i < 4; i++)
+            if (Sn[i] * Sn[i] < 1.0e-9) Sn[i] = (i + 1) * 1e-9;
+        double A[3][3] = {{Sn[0], Sn[3], Sn[5]}, {Sn[3], Sn[1], Sn[4]}, {Sn[5], Sn[4], Sn[2]}};
+        double lam[3], V[3][3];
+        eig3(A, lam, V);
+        double P[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int k = 0; k < 3; k++)
+            if (lam[k] > 0)
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) P[i][j] += V[i][k] * lam[k] * V[j][k];
+        V6 SV = {P[0][0], P[1][1], P[2][2], 0.5 * (P[0][1] + P[1][0]), 0.5 * (P[1][2] + P[2][1]), 0.5 * (P[0][2] + P[2][0])};
+        double t = 0;
+        for (int i = 0; i < 6; i++) t += SV[i] * Strain[i];
+        return 0.5 * t;
+    }
+
+    double DruckerPragerSplit(M6 D_Dam, M6 D_UnDam, V6 S_Dam, V6 S_Undam, const V6 Strain) const {  // :299-354
+        double Energy = 0.0;
+        double I1 = Strain[0] + Strain[1] + Strain[2];
+        V6 dI1 = {1, 1, 1, 0, 0, 0}, dJ2;
+        double J2 = 0;
+        for (int j = 0; j < 6; j++) {
+            double t = 0;
+            for (int i = 0; i < 6; i++) t += Strain[i] * J2Mat_strain[i][j];
+            dJ2[j] = t;
+        }
+        for (int j = 0; j < 6; j++) J2 += (0.5 * dJ2[j]) * Strain[j];
+        double sqrtJ2 = std::sqrt(J2 + 1.0e-16);
+        V6 dsq;
+        M6 ddsq;
+        for (int i = 0; i < 6; i++) dsq[i] = 0.5 / sqrtJ2 * dJ2[i];
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++)
+                ddsq[i][j] = 0.5 / sqrtJ2 * J2Mat_strain[i][j] - 0.25 / sqrtJ2 / sqrtJ2 / sqrtJ2 * dJ2[i] * dJ2[j];
+        auto full = [&](V6 S, M6 Dm) {
+            for (int i = 0; i < 6; i++) {
+                S[i] = Bulk * I1 * dI1[i] + 2.0 * Shear * dJ2[i];
+                for (int j = 0; j < 6; j++) Dm[i][j] = Bulk * dI1[i] * dI1[j] + 2.0 * Shear * J2Mat_strain[i][j];
+            }
+        };
+        if (-6.0 * B_dp * std::sqrt(J2) < I1) {
+            Energy = 0.5 * Bulk * std::pow(I1, 2) + 2.0 * Shear * J2;
+            full(S_Dam, D_Dam);
+        } else if (2 * Shear * std::sqrt(J2) < 3 * B_dp * Bulk * I1) {
+            Energy = 0.0;
+            full(S_Undam, D_UnDam);
+        } else {
+            double prefac = 1.0 / (18.0 * B_dp * B_dp * Bulk + 2.0 * Shear);
+            double q = -3.0 * B_dp * Bulk * I1 + 2 * Shear * sqrtJ2;
+            V6 dq;
+            for (int i = 0; i < 6; i++) dq[i] = -3.0 * B_dp * Bulk * dI1[i] + 2.0 * Shear * dsq[i];
+            Energy = prefac * std::pow(q, 2);
+            for (int i = 0; i < 6; i++) {
+                S_Dam[i] = prefac * 2.0 * q * dq[i];
+                for (int j = 0; j < 6; j++) D_Dam[i][j] = prefac * (2.0 * q * (2.0 * Shear * ddsq[i][j]) + 2.0 * dq[i] * dq[j]);
+            }
+            prefac = Bulk * Shear / (18.0 * B_dp * B_dp * Bulk + 2.0 * Shear);
+            q = I1 + 6 * B_dp * sqrtJ2;
+            for (int i = 0; i < 6; i++) dq[i] = dI1[i] + 6 * B_dp * dsq[i];
+            for (int i = 0; i < 6; i++) {
+                S_Undam[i] = prefac * 2.0 * q * dq[i];
+                for (int j = 0; j < 6; j++) D_UnDam[i][j] = prefac * (2.0 * q * (6 * B_dp * ddsq[i][j]) + 2.0 * dq[i] * dq[j]);
+            }
+        }
+        return Energy;
+    }
+
+    // IceSplit (:356-408) and StarConvexSplit (:217-245): volumetric part split on the sign of
+    // the volumetric strain, deviatoric part always damageable
+    double VolDevSplit(M6 D_Dam, M6 D_UnDam, V6 S_Dam, V6 S_Undam, const V6 Strain, bool ice) const {
+        double Energy = 0.0;
+        double p = Strain[0] + Strain[1] + Strain[2];
+        V6 Dev;
+        for (int i = 0; i < 6; i++) Dev[i] = VoightCorrect[i] * Strain[i] - (i < 3 ? 1.0 / 3.0 * p : 0.0);
+        double pplus, pmin, dpplus, dpmin;
+        if (p >= 0) { pplus = p; dpplus = 1.0; pmin = 0.0; dpmin = 0.0; }
+        else { pplus = 0.0; dpplus = 0.0; pmin = p; dpmin = 1.0; }
+        const double g = ice ? 0.0 : StarConvexGamma;
+        Energy += 0.5 * Bulk * (ice ? pplus * pplus : (pplus * pplus - g * pmin * pmin));
+        for (int i = 0; i < 3; i++) {
+            S_Dam[i] += Bulk * (ice ? pplus * dpplus : (pplus * dpplus - g * pmin * dpmin));
+            S_Undam[i] += (ice ? 1.0 : (1.0 + g)) * Bulk * pmin * dpmin;
+            for (int j = 0; j < 3; j++) {
+                D_Dam[i][j] += Bulk * (ice ? dpplus * dpplus : (dpplus * dpplus - g * dpmin * dpmin));
+                D_UnDam[i][j] += (ice ? 1.0 : (1.0 + g)) * Bulk * dpmin * dpmin;
+            }
+        }
+        double t = 0;
+        for (int i = 0; i < 6; i++) t += Dev[i] * VoightDoubleDot[i] * Dev[i];
+        Energy += Shear * t;
+        for (int i = 0; i < 6; i++) S_Dam[i] += 2 * Shear * VoightCorrect[i] * VoightDoubleDot[i] * Dev[i];
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) {
+                double P = (i == j ? 1.0 : 0.0) - ((i < 3 && j < 3) ? 1.0 / 3.0 : 0.0);
+                D_Dam[i][j] += 2 * Shear * VoightCorrect[i] * VoightDoubleDot[i] * VoightCorrect[i] * P;
+            }
+        if (ice) {
+            Energy += -0.5 * Bulk * Ice_cReduce * pmin * pmin / (Ice_eRef * Ice_eRef + pmin * pmin);
+            for (int i = 0; i < 6; i++) {  // stresses from the tangents (:391-392)
+                double a = 0, b = 0;
+                for (int j = 0; j < 6; j++) { a += D_Dam[i][j] * Strain[j]; b += D_UnDam[i][j] * Strain[j]; }
+                S_Dam[i] = a;
+                S_Undam[i] = b;
+            }
+            if (Energy < 0.0) Energy = 0.0;
+            if (Energy > 1.0e12 || std::isnan(Energy)) Energy = 1.0e12;
+            if (Energy < -1.0e12) Energy = -1.0e12;
+        }
+        return Energy;
+    }
+
+    // PhaseFieldMaterial::EnergySplit (PhaseFieldMaterial.cpp:47-94): VolStress (:97-117),
+    // VolStrains (:194-215), SpecStrains, SpecStress, DruckerPrager, ICE, StarConvex.
     double EnergySplit(M6 D_Dam, M6 D_UnDam, V6 S_Dam, V6 S_Undam, const V6 Strain) const {
         double Energy = 0;
         for (int i = 0; i < 6; i++) {
@@ -917,6 +1165,15 @@ struct Material {
                 for (int i = 0; i < 6; i++) t += S_Dam[i] * Strain[i];
                 Energy = 0.5 * t;
             } break;
+            case IFE_SPLIT_SPECSTRAINS: Energy = SpecStrainSplit(D_Dam, D_UnDam, S_Dam, S_Undam, Strain); break;
+            case IFE_SPLIT_SPECSTRESS:
+                if (ConsistentEnergy)
+                    throw std::invalid_argument("Spectral Stress split not implemented when using a consistent split");
+                Energy = SpecStressSplit(Strain);
+                break;
+            case IFE_SPLIT_DRUCKERPRAGER: Energy = DruckerPragerSplit(D_Dam, D_UnDam, S_Dam, S_Undam, Strain); break;
+            case IFE_SPLIT_ICE: Energy = VolDevSplit(D_Dam, D_UnDam, S_Dam, S_Undam, Strain, true); break;
+            case IFE_SPLIT_STARCONVEX: Energy = VolDevSplit(D_Dam, D_UnDam, S_Dam, S_Undam, Strain, false); break;
             default: throw std::invalid_argument("energy split not supported by the oracle");
         }
         if (ConsistentEnergy == false) {
@@ -1637,7 +1894,13 @@ double ifo_energy_split(const ife_material* mat, int consistent, double T, doubl
     M.SetTemperature(T, poros);
     M6 a, b;
     V6 c, d;
-    double E = M.EnergySplit(a, b, c, d, strain);
+    double E;
+    try {
+        E = M.EnergySplit(a, b, c, d, strain);
+    } catch (std::exception& ex) {  // the reference throws (e.g. SpecStress with ConsistentEnergy)
+        g_err = ex.what();
+        return std::nan("");
+    }
     for (int i = 0; i < 6; i++) {
         S_Dam[i] = c[i]; S_Undam[i] = d[i];
         for (int j = 0; j < 6; j++) { D_Dam[i * 6 + j] = a[i][j]; D_UnDam[i * 6 + j] = b[i][j]; }

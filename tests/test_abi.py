@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     for name in decl:
         assert hasattr(lib, name), name
     assert set(decl) == set(abi.EXPORTED)
-    assert lib.ife_api_version() == 1
+    assert lib.ife_api_version() == 2
 
 
 def test_unknown_model_is_rejected():
@@ -52,7 +52,14 @@ def test_unknown_model_is_rejected():
     bad = abi.Material.from_buffer_copy(m.material)
     bad.Rheology = abi.RHEOLOGY["DamViscoElasticPlastic"]
     assert lib.ife_physics_add_model(h, b"Fracture/PhaseField", b"x", 0, C.byref(bad), C.byref(m.pf),
-                                     C.byref(m.opts), None) == 2  # unsupported
+                                     C.byref(m.opts), None) == 2  # unsupported (the reference throws too)
+    # SpecStress with a consistent energy split throws in the reference (PhaseFieldMaterial.cpp:150-152)
+    bad = abi.Material.from_buffer_copy(m.material)
+    bad.Split = abi.SPLIT["SpecStress"]
+    pf = abi.PhaseFieldOpts.from_buffer_copy(m.pf)
+    pf.ConsistentEnergy = 1
+    assert lib.ife_physics_add_model(h, b"Fracture/PhaseField", b"x", 0, C.byref(bad), C.byref(pf),
+                                     C.byref(m.opts), None) == 2
     lib.ife_physics_destroy(h)
 
 
