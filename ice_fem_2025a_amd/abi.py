@@ -104,6 +104,10 @@ _PROTOS = [
                                     C.POINTER(C.c_int64)]),
     ("ife_model_commit", C.c_int, [_P, C.c_int32, C.c_int32, _P]),
     ("ife_model_reset_step", C.c_int, [_P, C.c_int32, _P]),
+    ("ife_physics_adjoint", C.c_int, [_P, C.POINTER(_P), _P, _P, _P, _P]),
+    ("ife_physics_spmv", C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
+    ("ife_model_set_element_params", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("ife_model_set_param", C.c_int, [_P, C.c_int32, C.c_char_p, C.c_double]),
     ("ife_last_error", C.c_char_p, []),
     ("ife_api_version", C.c_int, []),
 ]

@@ -9,6 +9,9 @@ Workloads (the BASELINE configurations, named by what they are):
   "quad2d_le_at2"   64x32 B2_2 quads on 0.1x0.05 m, u(2)+phi, linear elastic + AT2
   "hex3d_glen_at2"  40x40x80 NURBS3_Cube2 hexes, u(3)+phi+T, Glen viscous + AT2 + thermal
   "hex3d_vep_at2"   16x16x16 hexes, as hex3d_glen_at2 with the visco-elasto-plastic rheology
+  "hex3d_adjoint"   32x32x32 hexes on a 0.08 m cube, as hex3d_glen_at2, with per-element Gc
+                    log-normal(0, 0.3) J/m^2 and Glen A log-uniform in [1e-25, 1e-23] Pa^-3 s^-1
+                    (the adjoint-sensitivity workload; cotangents from make_cotangent)
 Sizes can be scaled down (nel=...) for parity tests; the physics stays the same.
 """
 import copy
@@ -88,6 +91,14 @@ WORKLOADS["hex3d_vep_at2"]["mesh"]["lengths"] = [0.04, 0.04, 0.04]
 WORKLOADS["hex3d_vep_at2"]["properties"]["Ice"]["SolidProperties"].update(
     {"Rheology": "ViscoElasticPlastic", "sy": 5.0e5, "F_deg": 20.0, "G_deg": 10.0, "hardening": 1.0e8,
      "pl_visc": 1.0e9})
+
+
+# Adjoint-sensitivity workload: the 3D Glen/AT2/thermal physics on a 32^3 block with
+# per-element Gc and A drawn after the state.
+WORKLOADS["hex3d_adjoint"] = copy.deepcopy(WORKLOADS["hex3d_glen_at2"])
+WORKLOADS["hex3d_adjoint"]["index"] = 2
+WORKLOADS["hex3d_adjoint"]["mesh"]["nel"] = [32, 32, 32]
+WORKLOADS["hex3d_adjoint"]["mesh"]["lengths"] = [0.08, 0.08, 0.08]
 
 
 def get_inputs(name, nel=None):
@@ -205,6 +216,7 @@ class Problem:
     models: list
     node_state: dict = field(default_factory=dict)   # (kind, field) -> (n_nodes,) array
     history: dict = field(default_factory=dict)      # (model, name) -> (ncomp, E*nip) array
+    elem_params: dict = field(default_factory=dict)  # model -> {"Gc": (E,), "A": (E,)}
     t: float = 0.0
     dt: float = 1.0
     du_dt: float = 1.0
@@ -268,4 +280,13 @@ def make_problem(name, nel=None, seed=None):
         prob.node_state[(abi.IFE_DSTATE, "T")] = rng.uniform(-1e-2, 1e-2, NN)
         prob.history[(0, "EnergyOld")] = rng.uniform(0.0, 5.0e3, (1, E * nip))
         prob.history[(0, "Energy")] = prob.history[(0, "EnergyOld")].copy()
+    if name == "hex3d_adjoint":
+        prob.elem_params[0] = {"Gc": np.exp(rng.normal(0.0, 0.3, E)),
+                               "A": np.exp(rng.uniform(np.log(1e-25), np.log(1e-23), E))}
     return prob
+
+
+def make_cotangent(n_rows, seed=20261203):
+    """Seeded N(0, 1) cotangent over the dofs of every step: n_rows[s] -> {s: vector}."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    return {s: rng.normal(0.0, 1.0, n) for s, n in enumerate(n_rows)}

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -108,6 +109,12 @@ struct Model {
     int HistSize = 0;
     // device history [comp][E*nip]
     DevBuf<double> E, EOld, PS, PSOld, VD, VDOld, DVD, DVDOld, MH, MHOld;
+    // per-element parameters (allocated at setup, used once set) and scalar overrides
+    DevBuf<double> Gc_e, A_e;
+    bool use_Gc_e = false, use_A_e = false;
+    double n_override = std::numeric_limits<double>::quiet_NaN();
+    double rm_eabs = -1, rm_erel = -1;
+    int rm_maxit = -1, exact_tangent = 0;
 };
 
 struct Step {
@@ -238,6 +245,12 @@ MatConst make_matconst(const ife_physics* ph, const Model* pfm, const Model* thm
     mc.Rheology = m.Rheology;
     mc.HistSize = pfm ? pfm->HistSize : 0;
     mc.A0 = m.A; mc.nCreep = m.n; mc.PhiViscLim = m.PhiViscLim;
+    mc.rm_eabs = -1; mc.rm_erel = -1; mc.rm_maxit = -1; mc.exact_tangent = 0;
+    if (pfm) {
+        if (!std::isnan(pfm->n_override)) mc.nCreep = pfm->n_override;
+        mc.rm_eabs = pfm->rm_eabs; mc.rm_erel = pfm->rm_erel; mc.rm_maxit = pfm->rm_maxit;
+        mc.exact_tangent = pfm->exact_tangent;
+    }
     mc.l = m.l; mc.Gc_in = m.Gc; mc.pf_visc = m.Viscosity;
     mc.Split = m.Split;
     mc.TRef = m.TRef; mc.TOffset = m.TOffset; mc.QCreep = m.QCreep; mc.ft0 = m.ft0; mc.ftdeg = m.ftdeg;
@@ -475,6 +488,8 @@ void setup_device(ife_physics* ph) {
         m->PSOld.alloc(6 * NIPT);
         m->MH.alloc(hs * NIPT);
         m->MHOld.alloc(hs * NIPT);
+        m->Gc_e.alloc(ph->NE);
+        m->A_e.alloc(ph->NE);
         DevBuf<double>* all[] = {&m->E, &m->EOld, &m->VD, &m->VDOld, &m->DVD, &m->DVDOld, &m->PS, &m->PSOld, &m->MH, &m->MHOld};
         for (auto* b : all) HIPCHK(hipMemset(b->p, 0, b->n * sizeof(double)));
     }
@@ -540,6 +555,8 @@ KParams base_params(ife_physics* ph) {
         p.E = pfm->E.p; p.EOld = pfm->EOld.p; p.PS = pfm->PS.p; p.PSOld = pfm->PSOld.p;
         p.VD = pfm->VD.p; p.VDOld = pfm->VDOld.p; p.DVD = pfm->DVD.p; p.DVDOld = pfm->DVDOld.p;
         p.MH = pfm->MH.p; p.MHOld = pfm->MHOld.p;
+        p.Gc_e = pfm->use_Gc_e ? pfm->Gc_e.p : nullptr;
+        p.A_e = pfm->use_A_e ? pfm->A_e.p : nullptr;
     }
     int fT = -1;
     if (thm) fT = thm->field_T;
@@ -611,6 +628,40 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
     timed(ph->mat_recs, 0, [&] { return launch_material(p, ph->dim, s); });
     timed(ph->recs, ph->NE, [&] { return launch_assemble(p, ph->dim, s); });
     timed(ph->gat_recs, 0, [&] { return launch_gather(p, ph->dim, s); });
+}
+
+// Adjoint of the full assembly's residuals (u, phase and T steps of ife_physics_assemble_all)
+// with respect to the per-element Gc, the per-element creep prefactor A and the Glen exponent
+// n, contracted with the cotangents lam[step].  Recomputes every IP's state from the current
+// state vectors and the committed history; writes no history.
+void run_adjoint(ife_physics* ph, const double* const* lam, double* g_Gc, double* g_A, double* g_n, hipStream_t s) {
+    if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+    Model* pfm = ph->pf_model();
+    if (!pfm) THROW(IFE_ERR_STATE, "the adjoint needs a Fracture/PhaseField model");
+    const ife_material& mat = pfm->mat;
+    const int rh = mat.Rheology;
+    if (rh != IFE_RHEO_LE && rh != IFE_RHEO_VE && rh != IFE_RHEO_DVE)
+        THROW(IFE_ERR_UNSUPPORTED, "adjoint: rheology must be LinearElastic, ViscoElastic or DamViscoElastic");
+    if (mat.Split != IFE_SPLIT_NONE && mat.Split != IFE_SPLIT_VOLSTRAINS && mat.Split != IFE_SPLIT_VOLSTRESS)
+        THROW(IFE_ERR_UNSUPPORTED, "adjoint: energy split must be None, VolStrains or VolStress");
+    if (pfm->opt.LumpedDriving || pfm->opt.LumpedPFCapacity)
+        THROW(IFE_ERR_UNSUPPORTED, "adjoint: lumped driving / lumped capacity are not differentiated");
+    if (pfm->pf.ViscoDrivingFactor > 1.0e-9) THROW(IFE_ERR_UNSUPPORTED, "adjoint: viscous driving is not differentiated");
+    require_ready(ph);
+    KParams p = base_params(ph);
+    p.do_u = 1;
+    p.do_ph = 1;
+    p.do_Tpf = pfm->HasT && pfm->opt.ViscousHeating;
+    p.do_th = 0;
+    auto pick = [&](int st) -> const double* { return (lam && st >= 0 && st < ph->nsteps) ? lam[st] : nullptr; };
+    p.lam_u = pick(pfm->Step_u);
+    p.lam_ph = pick(pfm->Step_phase);
+    p.lam_T = p.do_Tpf ? pick(pfm->Step_T) : nullptr;
+    p.g_Gc = g_Gc;
+    p.g_A = g_A;
+    p.g_n = g_n;
+    HIPCHK(launch_zero_status(ph->d_status.p, s));
+    HIPCHK(launch_adjoint(p, ph->dim, s));
 }
 
 DevBuf<double>* history_buf(Model* m, const std::string& nm, int64_t& ncomp) {
@@ -847,6 +898,23 @@ int ife_physics_assemble_all(ife_physics* ph, double* const* K_vals, double* con
     return guard([&] { run(ph, true, 0, K_vals, f, (hipStream_t)stream); });
 }
 
+int ife_physics_adjoint(ife_physics* ph, const double* const* lambda, double* g_Gc, double* g_A, double* g_n,
+                        void* stream) {
+    return guard([&] { run_adjoint(ph, lambda, g_Gc, g_A, g_n, (hipStream_t)stream); });
+}
+
+int ife_physics_spmv(ife_physics* ph, int32_t step, const double* K_vals, const double* x, double* y, void* stream) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (!x || !y) THROW(IFE_ERR_INVALID, "null argument");
+        Step& st = ph->steps[step];
+        if (!st.has_matrix) THROW(IFE_ERR_STATE, "step has no matrix");
+        const double* K = K_vals ? K_vals : st.K.p;
+        HIPCHK(launch_spmv(st.d_row_ptr.p, st.d_col.p, K, x, y, st.nrows, (hipStream_t)stream));
+    });
+}
+
 int ife_physics_Kf(ife_physics* ph, int32_t step, double** K_vals, double** f) {
     return guard([&] {
         require_ready(ph);
@@ -957,6 +1025,35 @@ static int copy_history(ife_physics* ph, int32_t model, bool commit, void* strea
             if (commit) HIPCHK(launch_copy(old->p, cur->p, (int64_t)cur->n, s));
             else HIPCHK(launch_copy(cur->p, old->p, (int64_t)cur->n, s));
         }
+    });
+}
+
+int ife_model_set_element_params(ife_physics* ph, int32_t model, const double* Gc_e, const double* A_e) {
+    return guard([&] {
+        require_ready(ph);
+        if (model < 0 || model >= (int)ph->models.size()) THROW(IFE_ERR_INVALID, "bad model index");
+        Model* m = ph->models[model];
+        if (m->type != MT_PHASEFIELD) THROW(IFE_ERR_INVALID, "per-element parameters belong to a Fracture/PhaseField model");
+        if (Gc_e) HIPCHK(hipMemcpy(m->Gc_e.p, Gc_e, ph->NE * sizeof(double), hipMemcpyHostToDevice));
+        if (A_e) HIPCHK(hipMemcpy(m->A_e.p, A_e, ph->NE * sizeof(double), hipMemcpyHostToDevice));
+        m->use_Gc_e = Gc_e != nullptr;
+        m->use_A_e = A_e != nullptr;
+    });
+}
+
+int ife_model_set_param(ife_physics* ph, int32_t model, const char* name, double value) {
+    return guard([&] {
+        if (!ph || !name) THROW(IFE_ERR_INVALID, "null argument");
+        if (model < 0 || model >= (int)ph->models.size()) THROW(IFE_ERR_INVALID, "bad model index");
+        Model* m = ph->models[model];
+        if (m->type != MT_PHASEFIELD) THROW(IFE_ERR_INVALID, "parameters belong to a Fracture/PhaseField model");
+        std::string nm(name);
+        if (nm == "n") m->n_override = value;
+        else if (nm == "rm_abs_tol") m->rm_eabs = value;
+        else if (nm == "rm_rel_tol") m->rm_erel = value;
+        else if (nm == "rm_max_iterations") m->rm_maxit = (int)value;
+        else if (nm == "exact_tangent") m->exact_tangent = value != 0.0;
+        else THROW(IFE_ERR_NOT_FOUND, "unknown parameter " + nm);
     });
 }
 

@@ -23,6 +23,11 @@ struct MatConst {
     double sy0, sydeg, yield_alpha, pot_alpha, hardening, pl_visc;
     // DruckerPrager / ICE / StarConvex split parameters (PhaseFieldMaterial.cpp:22-31)
     double B_dp, Ice_eRef, Ice_cReduce, StarConvexGamma;
+    // return-map stop rule (e < rm_eabs or e/e0 < rm_erel, at most rm_maxit iterations; a
+    // negative value keeps the reference's) and the tangent choice: exact_tangent = 1 forms
+    // DPlast from the return-map Jacobian including d2F/dsigma2 (the consistent tangent)
+    double rm_eabs, rm_erel;
+    int rm_maxit, exact_tangent;
     double k, cp, Density_th;
     // PF utility options
     int DamFunction, DensityDamage, GravityDamage, ViscoDriving;
@@ -106,6 +111,12 @@ struct KParams {
     Status* st;
     double* lcrack_elem;     // [e]
     double* ipb;             // per-IP coefficients [e][IPB_NF][nip]
+    const double* Gc_e;      // per-element Gc of the phase step (null: SetTemperature's)
+    const double* A_e;       // per-element creep prefactor A0 (null: the material's)
+    // adjoint: cotangents of the u, phase and T step residuals (global DOF order of each step)
+    // and per-element outputs d(lambda^T r)/d(Gc_e), /d(A_e) and the element's share of /dn
+    const double *lam_u, *lam_ph, *lam_T;
+    double *g_Gc, *g_A, *g_n;
     double* ebuf;            // element records [e][a][record] (rows of node a of the element)
     const int64_t* n2e_ptr;  // node (group rank) -> adjacent elements, ascending
     const int32_t* n2e_elem;
@@ -116,6 +127,9 @@ struct KParams {
 // Launchers (ife_kernels.hip)
 hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_material(const KParams& p, int dim, hipStream_t s);
+hipError_t launch_adjoint(const KParams& p, int dim, hipStream_t s);
+hipError_t launch_spmv(const int64_t* row_ptr, const int32_t* col, const double* val, const double* x, double* y,
+                       int64_t n_rows, hipStream_t s);
 hipError_t launch_gather(const KParams& p, int dim, hipStream_t s);
 int record_size(int dim);
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);

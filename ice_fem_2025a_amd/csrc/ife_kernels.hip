@@ -66,7 +66,8 @@ struct MatIP {
 };
 
 // IceMaterial::SetTemperature (IceMaterial.cpp:49-84)
-__device__ void set_temperature(const MatConst& mc, double T, double poros, MatIP& m) {
+// A0: the creep prefactor at TRef (the material's, or the element's when set)
+__device__ void set_temperature(const MatConst& mc, double T, double poros, MatIP& m, double A0) {
     const double R = 8.31446261815324;
     double dam = 1 - poros;
     double TAbs = T + mc.TOffset;
@@ -83,7 +84,7 @@ __device__ void set_temperature(const MatConst& mc, double T, double poros, MatI
     m.Dlo = dam * (Lame0 * 1.0);
     m.Dss = dam * Shear0;
     double ExpTerm = -mc.QCreep / R * (1 / TAbs - 1 / mc.TRef);
-    m.ACreep = mc.A0 * exp(ExpTerm);
+    m.ACreep = A0 * exp(ExpTerm);
     double ft = dam * (mc.ft0 - mc.ftdeg * (TAbs - mc.TRef));
     m.Gc = 2.0 * ft * ft / m.Young * mc.l;
     m.sy = dam * (mc.sy0 - mc.sydeg * (TAbs - mc.TRef));
@@ -157,6 +158,16 @@ __device__ double yield_ve(const double S[6], double dF[6]) {
     return F;
 }
 
+// return-map stop rule: e < eabs or e/e0 < erel, at most maxit iterations
+struct RMStop {
+    double eabs, erel;
+    int maxit;
+};
+__device__ __forceinline__ RMStop rm_stop(const MatConst& mc, bool vep) {
+    return RMStop{mc.rm_eabs >= 0 ? mc.rm_eabs : 1.0e-16, mc.rm_erel >= 0 ? mc.rm_erel : (vep ? 1.0e-8 : 1.0e-9),
+                  mc.rm_maxit > 0 ? mc.rm_maxit : 250};
+}
+
 // x^(n-1) for x >= 0: repeated products for the small integer exponents of Glen's law
 __device__ __forceinline__ double pow_nm1(double x, double n) {
     const double e = n - 1.0;
@@ -199,9 +210,10 @@ __device__ __forceinline__ double sgn(double x) { return x > 0 ? 1.0 : (x < 0 ? 
 
 // return_mapping_Viscous (SolidMaterial.cpp:200-269).  Returns the dissipation rate.
 // DPlast = (K^{-1})(0:5,0:5) = I - a b^T * inv at the final iterate; returned as (a, b, inv)
-__device__ double return_map_viscous(const MatIP& m, double nC, double dt, double Damage, double ra[6], double rb[6],
+__device__ double return_map_viscous(const RMStop& stop, const MatIP& m, double nC, double dt, double Damage, double ra[6],
+                                     double rb[6],
                                      double& rinv, double eps_pl[6], const double eps_t[6], const double eps_pl_old[6],
-                                     double& hist0, int& iters, int& maxit) {
+                                     double& hist0, int& iters, int& maxit, double* xsol) {
     double sol[7], f[7], fOld[7], dsol[7], a[6], b[6];
     for (int i = 0; i < 6; i++) sol[i] = eps_t[i] - eps_pl[i];
     sol[6] = hist0;
@@ -232,8 +244,8 @@ __device__ double return_map_viscous(const MatIP& m, double nC, double dt, doubl
         getKF(m, nC, PreFac, Damage, sol, eps_t, eps_pl_old, f, a, b);
         double e = 0;
         for (int i = 0; i < 7; i++) e += f[i] * f[i];
-        if (e < 1.0e-16 || e / e0 < 1.0e-9) break;
-        if (it > 250) { maxit = 1; break; }
+        if (e < stop.eabs || e / e0 < stop.erel) break;
+        if (it > stop.maxit) { maxit = 1; break; }
     }
     iters = it;
     double S[6], dF[6];
@@ -248,7 +260,50 @@ __device__ double return_map_viscous(const MatIP& m, double nC, double dt, doubl
     for (int i = 0; i < 6; i++) Diss += S[i] * (eps_pl[i] - eps_pl_old[i]);
     Diss = Damage * Diss;
     hist0 = sol[6];
+    if (xsol)
+        for (int i = 0; i < 7; i++) xsol[i] = sol[i];
     return fabs(Diss) / dt;
+}
+
+// Exact Jacobian of the Glen return-map residual (return_mapping_getKF's f, SolidMaterial.cpp
+// :278-301) in x = (elastic strain, dlambda): [[I + dl d2F D, dF], [n PreFac/sc (F/sc)^(n-1)
+// (D dF)^T, -1]], with d2F the derivative of the reference's dF (which the reference zeroes)
+// and without the Damage factor the reference puts in its K (f itself has none).
+__device__ void glen_jacobian_exact(const MatIP& m, double nC, double PreFac, const double x[7], double J[7][7]) {
+    const double sc = 1.0e6, eps = 1.0e-12;
+    double S[6], s[6], dJ2[6];
+    Dmul(m, x, S);
+    for (int i = 0; i < 6; i++) s[i] = S[i] / sc;
+    const double d = 1.0 - 1.0 / 3.0, o = 0.0 - 1.0 / 3.0;
+    dJ2[0] = d * s[0] + o * s[1] + o * s[2];
+    dJ2[1] = o * s[0] + d * s[1] + o * s[2];
+    dJ2[2] = o * s[0] + o * s[1] + d * s[2];
+    for (int i = 3; i < 6; i++) dJ2[i] = 2.0 * s[i];
+    double J2 = 0;
+    for (int i = 0; i < 6; i++) J2 += s[i] * dJ2[i];
+    J2 *= 0.5;
+    const double F = sc * sqrt(2.0 * J2), c = 0.5 * sqrt(2.0) / sqrt(J2 + eps);
+    double dF[6];
+    for (int i = 0; i < 6; i++) dF[i] = c * dJ2[i];
+    // d2F = (c / sc) (J2Mat - 0.5 / (J2 + eps) dJ2 dJ2^T); J[0:6][0:6] = I + dl d2F D
+    double H[6][6];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            const double jm = (i < 3 && j < 3) ? (i == j ? d : o) : (i == j ? 2.0 : 0.0);
+            H[i][j] = c / sc * (jm - 0.5 / (J2 + eps) * dJ2[i] * dJ2[j]);
+        }
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double t = 0;
+            for (int k = 0; k < 6; k++) t += H[i][k] * Dij(m, k, j);
+            J[i][j] = (i == j ? 1.0 : 0.0) + x[6] * t;
+        }
+    for (int i = 0; i < 6; i++) J[i][6] = dF[i];
+    double DdF[6];
+    Dmul(m, dF, DdF);
+    const double fac = nC * PreFac / sc * pow_nm1(F / sc, nC);
+    for (int j = 0; j < 6; j++) J[6][j] = fac * DdF[j];
+    J[6][6] = -1.0;
 }
 
 
@@ -427,7 +482,8 @@ __device__ void getKF_vep(const VEPConst& v, const MatIP& m, double nC, double d
 
 // SolidMaterial::return_mapping_VEP (SolidMaterial.cpp:303-357) + _updateStrain (:408-445).
 // hist[0..2] in/out (the initial guess is hist[1], hist[2]); returns DMat = (K^-1)(0:5,0:5).
-__device__ void return_map_vep(const VEPConst& v, const MatIP& m, double nC, double dt, double eps_pl[6], const double et[6],
+__device__ void return_map_vep(const RMStop& stop, const VEPConst& v, const MatIP& m, double nC, double dt, double eps_pl[6],
+                               const double et[6],
                                const double eo[6], double hist[3], double h0, double DMat[6][6], double& diss,
                                double& ddiss, int& iters, int& maxit) {
     double sol[8], f[8], fOld[8], dsol[8], K[8][8];
@@ -455,8 +511,8 @@ __device__ void return_map_vep(const VEPConst& v, const MatIP& m, double nC, dou
         getKF_vep(v, m, nC, dt, h0, sol, et, eo, K, f);
         double e = 0;
         for (int i = 0; i < 8; i++) e += f[i] * f[i];
-        if (e < 1.0e-16 || e / e0 < 1.0e-8) break;
-        if (it > 250) { maxit = 1; break; }
+        if (e < stop.eabs || e / e0 < stop.erel) break;
+        if (it > stop.maxit) { maxit = 1; break; }
     }
     iters = it;
     double S[6], SD[6], TS[6], tmp[6], dG[6], ddG[6][6];
@@ -840,22 +896,32 @@ __device__ double energy_split_combined(const MatConst& mc, const MatIP& m, cons
 // step's energy/history/crack-density terms and the T-step source terms, and writes the
 // per-IP coefficients the element kernel contracts (ife::IPB_* layout, [field][e*nip+ip]).
 // ---------------------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, bool ADJ>
 struct MatSmem {
-    static constexpr int N = ET<DIM>::N, EPB = NT / ET<DIM>::NIP;
+    static constexpr int N = ET<DIM>::N, NIP = ET<DIM>::NIP, EPB = NT / ET<DIM>::NIP, EA = ADJ ? EPB : 1;
     double X[EPB][N][4];
     double U[EPB][DIM][N], Uo[EPB][DIM][N], Ph[EPB][N], PhO[EPB][N], T[EPB][N];
     double C1[EPB][DIM][9];
     unsigned cnt[4];
+    double L[EA][DIM + 2][N];   // adjoint: cotangent of u (DIM), phase, T at the nodes
+    double GI[EA][NIP][3];      // adjoint: per-IP contributions to the Gc, A and n gradients
 };
 
 // VEP: the visco-elasto-plastic rheology, a separate instantiation (its 8x8 Newton solve
 // needs far more registers than the Glen map).
-template <int DIM, bool VEP>
+// EXACT: DPlast of the Glen map from its exact Jacobian (the consistent tangent), likewise a
+// separate instantiation.
+// ADJ: the adjoint pass.  Recomputes every IP's state (return map, split energy, history
+// max) without writing history or coefficients, and contracts the cotangent lambda with the
+// derivatives of the residuals: d/dGc_e of the phase residual (linear in Gc), and d/dA_e,
+// d/dn through the converged Glen map by implicit differentiation with its exact Jacobian
+// (u residual via the stress, phase residual via the history energy, T residual via the
+// viscous heating).
+template <int DIM, bool VEP, bool EXACT, bool ADJ>
 __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
     using E = ET<DIM>;
     constexpr int N = E::N, NIP = E::NIP, EPB = NT / NIP;
-    __shared__ MatSmem<DIM> sm;
+    __shared__ MatSmem<DIM, ADJ> sm;
     const int tid = threadIdx.x;
     const MatConst& mc = p.mc;
     const int64_t e0 = (int64_t)blockIdx.x * EPB;
@@ -880,6 +946,11 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             sm.PhO[le][a] = p.sph.xold[dp];
         }
         if (p.has_T) sm.T[le][a] = p.sT.x[(int64_t)p.fpos_T * p.Ng + r];
+        if constexpr (ADJ) {
+            for (int d = 0; d < DIM; d++) sm.L[le][d][a] = p.lam_u ? p.lam_u[(int64_t)p.fpos_u[d] * p.Ng + r] : 0.0;
+            sm.L[le][DIM][a] = p.lam_ph ? p.lam_ph[(int64_t)p.fpos_ph * p.Ng + r] : 0.0;
+            sm.L[le][DIM + 1][a] = p.lam_T ? p.lam_T[(int64_t)p.fpos_T * p.Ng + r] : 0.0;
+        }
     }
     for (int i = tid; i < EPB * DIM * 9; i += NT) {
         int le = i / (DIM * 9), k = i % (DIM * 9);
@@ -926,7 +997,8 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
         // sum_a N_a f_a = SC(f) / wFun and sum_a GXi_ca f_a = SG_c(f) / wFun - dW_c SC(f) / wFun^2
         // (NURBS3_Cube2.cpp:104-106), f = X_d gives J (:107-108), and field gradients are
         // J^-1 applied to the GXi sums (:110-112).  2D (B2_2.cpp:72-88): N = CB, GXi = GB.
-        enum { F_ONE = 0, F_X = 1, F_PH = F_X + DIM, F_PHO, F_U, F_UO = F_U + DIM, F_T = F_UO + DIM, NFLD };
+        enum { F_ONE = 0, F_X = 1, F_PH = F_X + DIM, F_PHO, F_U, F_UO = F_U + DIM, F_T = F_UO + DIM, F_L = F_T + 1,
+               NFLD = ADJ ? F_L + DIM + 2 : F_L };
         double SC[NFLD], SG[DIM][NFLD];
         for (int k = 0; k < NFLD; k++) {
             SC[k] = 0;
@@ -950,6 +1022,8 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 fv[F_UO + d] = sm.Uo[le][d][a];
             }
             fv[F_T] = sm.T[le][a];
+            if constexpr (ADJ)
+                for (int k = 0; k < DIM + 2; k++) fv[F_L + k] = sm.L[le][k][a];
             for (int k = 0; k < NFLD; k++) {
                 SC[k] += wc * fv[k];
                 for (int c = 0; c < DIM; c++) SG[c][k] += wg[c] * fv[k];
@@ -999,7 +1073,7 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
         }
         // geometry for the element kernel's shape functions, folded so that it needs no
         // division: G_a = W_a (J^-1 GB_a / wFun - CB_a J^-1 dW / wFun^2), N_a = W_a CB_a / wFun
-        {
+        if constexpr (!ADJ) {
             const double rw = DIM == 3 ? 1.0 / wF : 1.0;
             for (int c = 0; c < DIM; c++) {
                 double q = 0;
@@ -1051,8 +1125,11 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             for (int r = 0; r < 6; r++) eps_pl[r] = p.PS[r * S + gi];
         MatIP m;
         double VD = 0;
+        // adjoint: d eps_p / d(A_e, n) and d(dissipation rate) / d(A_e, n) at this IP
+        bool adj_on = false;
+        double adj_dep[2][6], adj_ddiss[2] = {0.0, 0.0}, gpart[3] = {0.0, 0.0, 0.0};
         if (p.has_pf && p.do_u) {
-            set_temperature(mc, Tu, mc.porosity, m);
+            set_temperature(mc, Tu, mc.porosity, m, p.A_e ? p.A_e[el] : mc.A0);
             double Dam, dDam, ddDam;
             damage(mc.DamFunction, phi, mc.k0, mc.p, Dam, dDam, ddDam);
             double epo[6];
@@ -1064,7 +1141,8 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             double hist0 = mc.HistSize > 0 ? p.MH[gi] : 0.0;
             const int rh = mc.Rheology;
             bool elastic = true;  // DPlast = I
-            double DM[6][6];      // DPlast of the VEP map
+            double DM[6][6];      // DPlast of the VEP map / the exact Glen tangent
+            double xs[7];         // converged return-map state (elastic strain, dlambda)
             if constexpr (VEP) {
                 // phiOld against PhiViscLim, damage 1 (SolidMaterial.cpp:178-186)
                 if (!(phio > mc.PhiViscLim)) {
@@ -1072,8 +1150,9 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                     double hist[3] = {p.MH[gi], p.MH[S + gi], p.MH[2 * S + gi]};
                     called = true;
                     elastic = false;
-                    return_map_vep(vc, m, mc.nCreep, p.dt, eps_pl, eps, epo, hist, p.MHOld[gi], DM, diss, ddiss, iters, mx);
-                    for (int k = 0; k < 3; k++) p.MH[k * S + gi] = hist[k];
+                    return_map_vep(rm_stop(mc, true), vc, m, mc.nCreep, p.dt, eps_pl, eps, epo, hist, p.MHOld[gi], DM, diss, ddiss, iters, mx);
+                    if constexpr (!ADJ)
+                        for (int k = 0; k < 3; k++) p.MH[k * S + gi] = hist[k];
                 }
             } else if (rh == IFE_RHEO_VE || rh == IFE_RHEO_DVE) {
                 if (!(phi > mc.PhiViscLim)) {
@@ -1081,14 +1160,51 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                     if (rh == IFE_RHEO_DVE) damage(mc.DamFunction, phi, mc.k0, mc.p, dmg, u1, u2);
                     called = true;
                     elastic = false;
-                    diss = return_map_viscous(m, mc.nCreep, p.dt, dmg, ra, rb, rinv, eps_pl, eps, epo, hist0, iters, mx);
+                    diss = return_map_viscous(rm_stop(mc, false), m, mc.nCreep, p.dt, dmg, ra, rb, rinv, eps_pl, eps, epo, hist0,
+                                              iters, mx, xs);
+                    if constexpr (ADJ) {
+                        // implicit differentiation of f(x; A, n) = 0: dx/dp = -J^-1 df/dp, df/dp
+                        // only in f6 = PreFac (F/sc)^n - dl, PreFac = ACreep dt sc^n, ACreep = A e^(..)
+                        const double sc = 1.0e6, PreFac = m.ACreep * p.dt * pow(sc, mc.nCreep);
+                        double J7[7][7], R[7][2], S6[6], tmp[6];
+                        glen_jacobian_exact(m, mc.nCreep, PreFac, xs, J7);
+                        Dmul(m, xs, S6);
+                        const double F = yield_ve(S6, tmp), Fn = pow(F / sc, mc.nCreep);
+                        const double Aused = p.A_e ? p.A_e[el] : mc.A0;
+                        for (int i = 0; i < 7; i++) { R[i][0] = 0.0; R[i][1] = 0.0; }
+                        R[6][0] = PreFac / Aused * Fn;
+                        R[6][1] = F > 0.0 ? PreFac * Fn * log(F) : 0.0;
+                        gauss_pp<7, 2>(J7, R);
+                        // eps_p = eps - x(0:6) at the solution: d eps_p = R(0:6)
+                        double X = 0;
+                        for (int i = 0; i < 6; i++) X += S6[i] * (eps_pl[i] - epo[i]);
+                        const double sx = (dmg * X) > 0 ? 1.0 : ((dmg * X) < 0 ? -1.0 : 0.0);
+                        for (int k = 0; k < 2; k++) {
+                            double dx[6], Ddx[6], dX = 0;
+                            for (int i = 0; i < 6; i++) { adj_dep[k][i] = R[i][k]; dx[i] = -R[i][k]; }
+                            Dmul(m, dx, Ddx);
+                            for (int i = 0; i < 6; i++) dX += Ddx[i] * (eps_pl[i] - epo[i]) + S6[i] * adj_dep[k][i];
+                            adj_ddiss[k] = sx * dmg / p.dt * dX;
+                        }
+                        adj_on = true;
+                    }
+                    if constexpr (EXACT) {
+                        double J7[7][7], B[7][6];
+                        glen_jacobian_exact(m, mc.nCreep, m.ACreep * p.dt * pow(1.0e6, mc.nCreep), xs, J7);
+                        for (int i = 0; i < 7; i++)
+                            for (int j = 0; j < 6; j++) B[i][j] = i == j ? 1.0 : 0.0;
+                        gauss_pp<7, 6>(J7, B);
+                        for (int i = 0; i < 6; i++)
+                            for (int j = 0; j < 6; j++) DM[i][j] = B[i][j];
+                    }
                 }
             } else if (rh == IFE_RHEO_VE_EXPL || rh == IFE_RHEO_DVE_EXPL) {
                 if (!(phio > mc.PhiViscLim)) {
                     double dmg = 1.0, u1, u2;
                     if (rh == IFE_RHEO_DVE_EXPL) damage(mc.DamFunction, phio, mc.k0, mc.p, dmg, u1, u2);
                     called = true;
-                    diss = return_map_viscous(m, mc.nCreep, p.dt, dmg, ra, rb, rinv, eps_pl, epso, epo, hist0, iters, mx);
+                    diss = return_map_viscous(rm_stop(mc, false), m, mc.nCreep, p.dt, dmg, ra, rb, rinv, eps_pl, epso, epo,
+                                              hist0, iters, mx, nullptr);
                 }  // explicit variants keep DPlast = I
             }
             if (!called)
@@ -1098,22 +1214,47 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 my_iters += iters;
                 my_maxit += mx;
             }
-            if (!VEP && mc.HistSize > 0) p.MH[gi] = hist0;
-            for (int r = 0; r < 6; r++) p.PS[r * S + gi] = eps_pl[r];
             VD = p.VDOld[gi] + diss * p.dt;
-            p.VD[gi] = VD;
-            p.DVD[gi] = p.DVDOld[gi] + ddiss * p.dt;
+            if constexpr (!ADJ) {
+                if (!VEP && mc.HistSize > 0) p.MH[gi] = hist0;
+                for (int r = 0; r < 6; r++) p.PS[r * S + gi] = eps_pl[r];
+                p.VD[gi] = VD;
+                p.DVD[gi] = p.DVDOld[gi] + ddiss * p.dt;
+            }
             double ee[6];
             for (int r = 0; r < 6; r++) ee[r] = eps[r] - eps_pl[r];
             double Sm[6][6], sig[6];
             energy_split_combined(mc, m, ee, Dam, Sm, sig);
+            if constexpr (ADJ) {
+                if (adj_on) {
+                    // u residual: lambda_u^T dF_U = -w eps(lambda_u) . Sm d eps_p; T residual (viscous
+                    // heating, F_T = -w N (VD - VDOld)/dt): -w (N lambda_T) d(diss)
+                    double el6[6] = {0, 0, 0, 0, 0, 0}, gl[DIM];
+                    for (int al = 0; al < DIM; al++) {
+                        grad(F_L + al, gl);
+                        for (int c = 0; c < DIM; c++) el6[vrow<DIM>(al, c)] += gl[c];
+                    }
+                    const double lT = p.do_Tpf ? val(F_L + DIM + 1) : 0.0;
+                    for (int k = 0; k < 2; k++) {
+                        double t = 0;
+                        for (int i = 0; i < 6; i++) {
+                            double ds = 0;
+                            for (int j = 0; j < 6; j++) ds += Sm[i][j] * adj_dep[k][j];
+                            t += el6[i] * ds;
+                        }
+                        gpart[1 + k] += -w * t - w * lT * adj_ddiss[k];
+                    }
+                }
+            }
+            if constexpr (!ADJ)
             for (int r = 0; r < 6; r++) {
                 double v = w * sig[r];
                 my_bad |= !isfinite(v);
                 ipo[(IPB_SIG + r) * NIP] = v;
             }
             // Mt = w * S * DPlast = w * (S - (S a) b^T inv)  (VEP: the full DPlast)
-            if (VEP && !elastic) {
+            if (ADJ) {
+            } else if ((VEP || EXACT) && !elastic) {
                 for (int i = 0; i < 6; i++)
                     for (int j = 0; j < 6; j++) {
                         double t = 0;
@@ -1134,7 +1275,8 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             VD = p.VD[gi];
         }
         if (p.has_pf && p.do_ph) {
-            set_temperature(mc, Tph, mc.porosity, m);
+            set_temperature(mc, Tph, mc.porosity, m, p.A_e ? p.A_e[el] : mc.A0);
+            if (p.Gc_e) m.Gc = p.Gc_e[el];  // the element's Gc replaces SetTemperature's
             double Dam, dDam, ddDam, DamI, dDamI, ddDamI, DamG, dDamG, ddDamG;
             damage(mc.DamFunction, phi, mc.k0, mc.p, Dam, dDam, ddDam);
             damage(mc.DensityDamage, phi, mc.k0, mc.p, DamI, dDamI, ddDamI);
@@ -1143,13 +1285,12 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             for (int r = 0; r < 6; r++) ee[r] = eps[r] - eps_pl[r];
             double Sm[6][6], sig[6];
             double Energy = energy_split_combined(mc, m, ee, 1.0, Sm, sig);
+            bool fromE = true;  // the history takes the current energy
             if (mc.Irreversible == IFE_IRR_HIST) {  // strict '>' (PhaseField.cpp:631-640)
                 double Ho = p.EOld[gi];
-                if (Ho > Energy) { p.E[gi] = Ho; Energy = Ho; }
-                else p.E[gi] = Energy;
-            } else {
-                p.E[gi] = Energy;
+                if (Ho > Energy) { Energy = Ho; fromE = false; }
             }
+            if constexpr (!ADJ) p.E[gi] = Energy;
             double gg = 0;
             for (int c = 0; c < DIM; c++) gg += gphi[c] * gphi[c];
             double dN, cNN, cGG, gamma;
@@ -1158,6 +1299,28 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             if (mc.LumpedCap) {
                 double u1, u2;
                 cap_coeffs(mc.Distribution, mc.l, phi, capN, capNN, u1, u2);
+            }
+            if constexpr (ADJ) {
+                // phase residual: Gc multiplies w (dN - capN) N and w cGG grad(phi) . G
+                double gl[DIM];
+                grad(F_L + DIM, gl);
+                double t = 0;
+                for (int c = 0; c < DIM; c++) t += gphi[c] * gl[c];
+                const double lph = val(F_L + DIM);
+                gpart[0] += w * (dN - capN) * lph + w * cGG * t;
+                // driving term w dD H N with H = Psi+(eps - eps_p) when the history takes it:
+                // dH = -(dPsi+/d eps_e) . d eps_p
+                if (adj_on && fromE && !mc.LumpedDrive) {
+                    MatConst mcs = mc;
+                    mcs.ConsistentEnergy = 1;
+                    double DD[6][6], DU[6][6], SD[6], SU[6];
+                    energy_split(mcs, m, ee, DD, DU, SD, SU);
+                    for (int k = 0; k < 2; k++) {
+                        double dH = 0;
+                        for (int i = 0; i < 6; i++) dH -= SD[i] * adj_dep[k][i];
+                        gpart[1 + k] += w * dDam * lph * dH;
+                    }
+                }
             }
             double A = w * m.Gc * (cNN - capNN), FN = w * m.Gc * (dN - capN), LDR = 0;
             if (!mc.LumpedDrive) {
@@ -1201,21 +1364,29 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                 FN += -w * mc.Density * gu_ * dDamG;
                 A += -w * mc.Density * gu_ * ddDamG;
             }
-            ipo[(IPB_PHA) * NIP] = A;
-            ipo[(IPB_PHB) * NIP] = w * m.Gc * cGG;
-            ipo[(IPB_PHFN) * NIP] = FN;
-            for (int c = 0; c < DIM; c++) ipo[(IPB_PHFG + c) * NIP] = w * m.Gc * cGG * gphi[c];
-            ipo[(IPB_LDRIVE) * NIP] = LDR;
-            ipo[(IPB_GW) * NIP] = gamma * w;
-            ipo[(IPB_GC) * NIP] = m.Gc;
+            if constexpr (!ADJ) {
+                ipo[(IPB_PHA) * NIP] = A;
+                ipo[(IPB_PHB) * NIP] = w * m.Gc * cGG;
+                ipo[(IPB_PHFN) * NIP] = FN;
+                for (int c = 0; c < DIM; c++) ipo[(IPB_PHFG + c) * NIP] = w * m.Gc * cGG * gphi[c];
+                ipo[(IPB_LDRIVE) * NIP] = LDR;
+                ipo[(IPB_GW) * NIP] = gamma * w;
+                ipo[(IPB_GC) * NIP] = m.Gc;
+            }
             my_bad |= !isfinite(A) || !isfinite(FN);
         }
-        if (p.do_Tpf || p.do_th) {
+        if (!ADJ && (p.do_Tpf || p.do_th)) {
             double fn = 0;
             if (p.has_pf && p.do_Tpf) fn = -w * ((VD - p.VDOld[gi]) / p.dt);
             ipo[(IPB_THFN) * NIP] = fn;
             ipo[(IPB_THB) * NIP] = p.do_th ? w * mc.k : 0.0;
             for (int c = 0; c < DIM; c++) ipo[(IPB_THFG + c) * NIP] = p.do_th ? w * mc.k * gT[c] : 0.0;
+        }
+        if constexpr (ADJ) {
+            for (int k = 0; k < 3; k++) {
+                sm.GI[le][ip][k] = gpart[k];
+                my_bad |= !isfinite(gpart[k]);
+            }
         }
     }
     if (my_calls) {
@@ -1225,6 +1396,20 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
     }
     if (my_bad) atomicOr(&sm.cnt[3], 1u);
     __syncthreads();
+    if constexpr (ADJ) {
+        // per-element sums over the IPs in IP order (deterministic)
+        if (tid < EPB && e0 + tid < p.NE) {
+            const int64_t ea = e0 + tid;
+            double g[3] = {0.0, 0.0, 0.0};
+            for (int i = 0; i < NIP; i++)
+                for (int k = 0; k < 3; k++) g[k] += sm.GI[tid][i][k];
+            if (p.g_Gc) p.g_Gc[ea] = g[0];
+            if (p.g_A) p.g_A[ea] = g[1];
+            if (p.g_n) p.g_n[ea] = g[2];
+        }
+        if (tid == 0 && sm.cnt[3]) atomicOr(&p.st->nonfinite, 1u);
+        return;
+    }
     if (tid == 0) {
         const int64_t ne = p.NE - e0 < EPB ? p.NE - e0 : EPB;
         atomicAdd(&p.st->elements, (unsigned long long)ne);
@@ -1782,6 +1967,20 @@ __global__ void zero_status_kernel(Status* s) {
     }
 }
 
+// y = K x over the step's CSR matrix: 4 rows per 256-lane workgroup, one wave per row, lanes
+// stride the row's columns and a fixed butterfly sums them (the same order on every call)
+__global__ __launch_bounds__(256) void spmv_kernel(const int64_t* rp, const int32_t* col, const double* val,
+                                                    const double* x, double* y, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + threadIdx.x / GW;
+    const int lane = threadIdx.x % GW;
+    if (r >= n) return;
+    const int64_t b = rp[r], e = rp[r + 1];
+    double t = 0.0;
+    for (int64_t k = b + lane; k < e; k += GW) t += val[k] * x[col[k]];
+    for (int o = GW / 2; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if (lane == 0) y[r] = t;
+}
+
 __global__ void copy_kernel(double* d, const double* s, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1819,13 +2018,36 @@ hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
     int64_t blocks = (p.NE + epb - 1) / epb;
     if (blocks <= 0) return hipSuccess;
     const bool vep = p.mc.Rheology == IFE_RHEO_VEP;
+    const bool exact = !vep && p.mc.exact_tangent;
+    const dim3 g((unsigned)blocks), b(NT);
     if (vep) {
-        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, true>), dim3((unsigned)blocks), dim3(NT), 0, s, p);
-        else hipLaunchKernelGGL((material_kernel<3, true>), dim3((unsigned)blocks), dim3(NT), 0, s, p);
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, true, false, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, true, false, false>), g, b, 0, s, p);
+    } else if (exact) {
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, true, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, false, true, false>), g, b, 0, s, p);
     } else {
-        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false>), dim3((unsigned)blocks), dim3(NT), 0, s, p);
-        else hipLaunchKernelGGL((material_kernel<3, false>), dim3((unsigned)blocks), dim3(NT), 0, s, p);
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, false, false, false>), g, b, 0, s, p);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint(const KParams& p, int dim, hipStream_t s) {
+    const int nip = dim == 2 ? 9 : 27, epb = NT / nip;
+    int64_t blocks = (p.NE + epb - 1) / epb;
+    if (blocks <= 0) return hipSuccess;
+    const dim3 g((unsigned)blocks), b(NT);
+    if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, true>), g, b, 0, s, p);
+    else hipLaunchKernelGGL((material_kernel<3, false, false, true>), g, b, 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_spmv(const int64_t* row_ptr, const int32_t* col, const double* val, const double* x, double* y,
+                       int64_t n_rows, hipStream_t s) {
+    if (n_rows <= 0) return hipSuccess;
+    const int64_t blocks = (n_rows + 3) / 4;
+    hipLaunchKernelGGL(spmv_kernel, dim3((unsigned)blocks), dim3(256), 0, s, row_ptr, col, val, x, y, n_rows);
     return hipGetLastError();
 }
 

@@ -149,6 +149,8 @@ class Physics:
             full = np.zeros((nc, npc))
             full[:vals.shape[0]] = vals
             hip.to_device(p, np.ascontiguousarray(full))
+        for model, prm in prob.elem_params.items():
+            self.set_element_params(model, prm.get("Gc"), prm.get("A"))
 
     def history(self, model, name):
         p, nc, npc = self.history_ptr(model, name)
@@ -176,6 +178,73 @@ class Physics:
         if f.size:
             hip.from_device(f, fp.value)
         return K, f
+
+    # ---- parameters and differentiation ------------------------------------------------
+    def set_element_params(self, model, Gc_e=None, A_e=None):
+        """Per-element Gc / creep prefactor A of a Fracture/PhaseField model (None: the material's)."""
+        ne = self.prob.mesh.n_elems
+        args = []
+        for a in (Gc_e, A_e):
+            if a is None:
+                args.append(None)
+            else:
+                a = np.ascontiguousarray(a, dtype=np.float64).reshape(ne)
+                args.append(a)
+        abi.check(self.L.ife_model_set_element_params(
+            self.h, model, *[None if a is None else a.ctypes.data_as(C.POINTER(C.c_double)) for a in args]))
+
+    def set_param(self, model, name, value):
+        abi.check(self.L.ife_model_set_param(self.h, model, name.encode(), float(value)))
+
+    def adjoint_device(self, lam_ptrs, g_Gc=None, g_A=None, g_n=None, stream=None):
+        """ife_physics_adjoint on device pointers: lam_ptrs[step] (or None), outputs n_elems each."""
+        arr = (C.c_void_p * self.nsteps)(*[p if p else None for p in lam_ptrs])
+        abi.check(self.L.ife_physics_adjoint(self.h, arr, g_Gc, g_A, g_n, stream.handle if stream else None))
+
+    def adjoint(self, lam):
+        """Host convenience: lam = {step: vector}; returns (g_Gc, g_A, g_n per element)."""
+        bufs = [hip.DeviceArray.from_host(lam[s]) if s in lam else None for s in range(self.nsteps)]
+        ne = self.prob.mesh.n_elems
+        out = [hip.DeviceArray(ne) for _ in range(3)]
+        self.adjoint_device([b.ptr if b is not None else None for b in bufs], *[o.ptr for o in out])
+        return tuple(o.get() for o in out)
+
+    def spmv(self, step, x):
+        """y = K x with the step's last assembled K (host vectors in, host vector out)."""
+        dx = hip.DeviceArray.from_host(x)
+        dy = hip.DeviceArray(self.n_rows(step))
+        abi.check(self.L.ife_physics_spmv(self.h, step, None, dx.ptr, dy.ptr, None))
+        return dy.get()
+
+    def get_state(self, step, kind):
+        out = np.empty(self.n_rows(step))
+        hip.synchronize()
+        hip.from_device(out, self.state_ptr(step, kind))
+        return out
+
+    def set_state(self, step, kind, vec):
+        v = np.ascontiguousarray(vec, dtype=np.float64)
+        assert v.size == self.n_rows(step)
+        hip.synchronize()
+        hip.to_device(self.state_ptr(step, kind), v)
+
+    def residuals(self):
+        """assemble_all, then f of every step concatenated (host)."""
+        self.assemble_all()
+        hip.synchronize()
+        out = []
+        for s in range(self.nsteps):
+            f = np.empty(self.n_rows(s))
+            if f.size:
+                hip.from_device(f, self.Kf_ptrs(s)[1])
+            out.append(f)
+        return np.concatenate(out)
+
+    def Kf_ptrs(self, step):
+        """Device pointers of the engine-owned K values and f of a step."""
+        kp, fp = C.c_void_p(), C.c_void_p()
+        abi.check(self.L.ife_physics_Kf(self.h, step, C.byref(kp), C.byref(fp)))
+        return kp.value, fp.value
 
     def status(self, stream=None):
         st = abi.AssemblyStatus()

@@ -5,6 +5,8 @@ buffers and to time kernels on the engine's own stream."""
 import ctypes as C
 import os
 
+import numpy as np
+
 _hip = None
 
 
@@ -38,6 +40,12 @@ def lib():
         h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         h.hipEventElapsedTime.restype = C.c_int
         h.hipEventDestroy.argtypes = [C.c_void_p]
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipMalloc.restype = C.c_int
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipFree.restype = C.c_int
+        h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        h.hipMemcpyAsync.restype = C.c_int
         h.hipGetErrorString.argtypes = [C.c_int]
         h.hipGetErrorString.restype = C.c_char_p
     return _hip
@@ -64,6 +72,43 @@ def to_device(dev_ptr, arr):
 
 def from_device(arr, dev_ptr):
     _chk(lib().hipMemcpy(arr.ctypes.data, dev_ptr, arr.nbytes, D2H), "hipMemcpy D2H")
+
+
+def copy_d2d(dst_ptr, src_ptr, nbytes, stream=None):
+    _chk(lib().hipMemcpyAsync(dst_ptr, src_ptr, nbytes, D2D, stream), "hipMemcpyAsync D2D")
+
+
+class DeviceArray:
+    """A float64 device buffer owned by the harness (hipMalloc / hipFree)."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = C.c_void_p()
+        _chk(lib().hipMalloc(C.byref(p), max(self.n, 1) * 8), "hipMalloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        d = cls(a.size)
+        if a.size:
+            to_device(d.ptr, a)
+        return d
+
+    def get(self):
+        out = np.empty(self.n)
+        synchronize()
+        if self.n:
+            from_device(out, self.ptr)
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
 
 
 def synchronize():

@@ -269,6 +269,30 @@ int ife_physics_Kf(ife_physics* ph, int32_t step, double** K_vals, double** f);
 /* Synchronises `stream` and returns the status of the last assemble call. */
 int ife_physics_status(ife_physics* ph, void* stream, ife_assembly_status* out);
 
+/* ---- differentiation (no counterpart in the reference, which has no sensitivities) ---- */
+
+/* Adjoint of the full assembly.  With r(p) the residuals ife_physics_assemble_all
+ * writes (u, phase and T steps) at the current state and committed history, and p the
+ * per-element Gc, the per-element Glen prefactor A (the creep constant at TRef) and the
+ * Glen exponent n, writes g_Gc[e] = d(lambda^T r)/dGc_e, g_A[e] = d(lambda^T r)/dA_e and
+ * g_n[e] = element e's share of d(lambda^T r)/dn (sum over e for the derivative).
+ * lambda[s] is a device vector of n_rows(s) doubles in the step's global dof order (NULL:
+ * zero); g_* are device arrays of n_elems doubles (NULL: not wanted).  The quadrature-point
+ * state (return map, split energy, history max) is recomputed, nothing is written to the
+ * history.  d/dA and d/dn follow the converged Glen return map by implicit differentiation
+ * with its exact Jacobian.  Supported: rheology LE / VE / dVE, splits None / VolStrains /
+ * VolStress, no lumped driving or capacity, no viscous driving (else IFE_ERR_UNSUPPORTED).
+ * Asynchronous on `stream`; ife_physics_status reports non-finite values. */
+int ife_physics_adjoint(ife_physics* ph, const double* const* lambda, double* g_Gc, double* g_A,
+                        double* g_n, void* stream);
+
+/* y = K x over a step's CSR pattern (device vectors of n_rows doubles).  K_vals NULL:
+ * the engine-owned values of the last assembly.  With the exact tangent on
+ * (ife_model_set_param "exact_tangent") this is the Jacobian-vector product of the step's
+ * residual. */
+int ife_physics_spmv(ife_physics* ph, int32_t step, const double* K_vals, const double* x,
+                     double* y, void* stream);
+
 /* Enable/disable per-launch event timing (off by default). */
 int ife_physics_set_profiling(ife_physics* ph, int32_t enable);
 
@@ -289,6 +313,23 @@ int ife_model_commit(ife_physics* ph, int32_t model, int32_t commit_type, void* 
 
 /* BaseModel::ResetStep (PhaseField.cpp:147-163): old -> current. */
 int ife_model_reset_step(ife_physics* ph, int32_t model, void* stream);
+
+/* Per-element material parameters of a Fracture/PhaseField model (no counterpart in the
+ * reference, where Gc follows SetTemperature and A is one material constant,
+ * IceMaterial.cpp:49-84).  Gc_e replaces the phase step's Gc of element e; A_e replaces the
+ * creep prefactor A of element e before the Arrhenius factor.  Host arrays of n_elems
+ * doubles, copied; NULL returns that parameter to the material's. */
+int ife_model_set_element_params(ife_physics* ph, int32_t model, const double* Gc_e,
+                                 const double* A_e);
+
+/* Scalar overrides of a Fracture/PhaseField model (no counterpart in the reference):
+ * "n" the Glen exponent (SolidProperties.n), "rm_abs_tol" / "rm_rel_tol" /
+ * "rm_max_iterations" the return-map stop rule (the reference's 1e-16 / 1e-9 (1e-8 for
+ * the visco-elasto-plastic map) / 250, SolidMaterial.cpp:200-269), "exact_tangent" (1: the
+ * consistent tangent of the Glen map, including the second derivative of the yield
+ * function that the reference drops, SolidMaterial.cpp:289).  A negative tolerance or
+ * non-positive iteration count restores the reference's value. */
+int ife_model_set_param(ife_physics* ph, int32_t model, const char* name, double value);
 
 const char* ife_last_error(void);
 int ife_api_version(void);

@@ -360,6 +360,17 @@ struct Material {
     // visco-elasto-plastic parameters (SolidMaterial.cpp:49-64) and P = I - 1/3 i i^T (:92)
     double yield_alpha = 0, pot_alpha = 0, hardening = 0, pl_visc = 0;
     double Pdev[6][6];
+    // return-map stop rule: e < rm_eabs or e/e0 < rm_erel, at most rm_maxit iterations;
+    // negative values keep the reference's (SolidMaterial.cpp:243-244, 344-345)
+    double rm_eabs = -1, rm_erel = -1;
+    int rm_maxit = -1;
+    bool rm_stop(double e, double e0, unsigned it, bool vep, bool& conv, bool& maxit) const {
+        const double ea = rm_eabs >= 0 ? rm_eabs : 1.0e-16, er = rm_erel >= 0 ? rm_erel : (vep ? 1.0e-8 : 1.0e-9);
+        const unsigned mi = rm_maxit > 0 ? (unsigned)rm_maxit : 250u;
+        if (e < ea || e / e0 < er) conv = true;
+        if (it > mi) { conv = true; maxit = true; }
+        return conv;
+    }
 
     void init(const ife_material& m, const ife_phasefield_opts* pf) {
         in = m;
@@ -557,8 +568,7 @@ struct Material {
             getKF(K, f, sol, Strain, Strain_Pl_Old, Damage, dt);
             double e = 0;
             for (int i = 0; i < 7; i++) e += f[i] * f[i];
-            if (e < 1.0e-16 || e / e0 < 1.0e-9) conv = true;
-            if (it > 250) { conv = true; maxit = true; }
+            rm_stop(e, e0, it, false, conv, maxit);
         }
         iters = (int)it;
         V6 Stress;
@@ -755,8 +765,7 @@ struct Material {
             getKF_VEP(K, f, sol, Strain, Strain_Pl_Old, dt, Damage, histOld);
             double e = 0;
             for (int i = 0; i < 8; i++) e += f[i] * f[i];
-            if (e < 1.0e-16 || e / e0 < 1.0e-8) conv = true;
-            if (it > 250) { conv = true; maxit = true; }
+            rm_stop(e, e0, it, true, conv, maxit);
         }
         iters = (int)it;
         // updateStrain
@@ -1287,6 +1296,9 @@ struct Model {
     double g[3] = {0, 0, 0};
     // history [comp][elem][ip]
     std::vector<double> Energy, EnergyOld, PS, PSOld, VD, VDOld, DVD, DVDOld, MH, MHOld;
+    // per-element parameters (empty: the material's): Gc replaces SetTemperature's Gc in the
+    // phase step, A replaces A0 in the creep law
+    std::vector<double> Gc_e, A_e;
     double LCrack = 0;
 };
 
@@ -1424,6 +1436,7 @@ static void Assemble_U(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, St
     double ddXdt2 = c.ddu_dt, dXdt = c.du_dt, dt = c.dt;
     const size_t NIPT = (size_t)c.NE * nip;
     for (int64_t el = e0; el < e1; el++) {
+        if (!md.A_e.empty()) M.A0 = md.A_e[el];
         getShapeGrads(c, el, Su);
         getShapeGrads(c, el, Sph);
         elem_nodes(c, el, El);
@@ -1591,6 +1604,7 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
             if (md.HasT) { Tip = 0; for (int a = 0; a < n; a++) Tip += ST.Nv(ip, a) * T[a]; }
             else Tip = md.opt.T_default;
             M.SetTemperature(Tip, md.opt.porosity);
+            if (!md.Gc_e.empty()) M.Gc = md.Gc_e[el];
             double phi = 0;
             for (int a = 0; a < n; a++) phi += Sph.Nv(ip, a) * Phase[a];
             double Dam, dDam, ddDam, DamI, dDamI, ddDamI, DamG, dDamG, ddDamG;
@@ -2092,6 +2106,31 @@ double* ifo_history(void* h, int model, const char* name, int64_t* ncomp) {
 }
 
 // Physics::Assemble(step) (physics.cpp:255-303) in element order, single thread.
+// Per-element Gc and creep A of model `model` (arrays of n_elems, copied; NULL clears).
+int ifo_set_element_params(void* h, int model, const double* Gc_e, const double* A_e) {
+    Ctx* c = (Ctx*)h;
+    if (model < 0 || model >= (int)c->models.size()) return -1;
+    Model& md = c->models[model];
+    if (Gc_e) md.Gc_e.assign(Gc_e, Gc_e + c->NE); else md.Gc_e.clear();
+    if (A_e) md.A_e.assign(A_e, A_e + c->NE); else md.A_e.clear();
+    return 0;
+}
+
+// Scalar model parameters: "n" (creep exponent), "rm_abs_tol", "rm_rel_tol",
+// "rm_max_iterations" (return-map stop rule; negative restores the reference's).
+int ifo_set_param(void* h, int model, const char* name, double value) {
+    Ctx* c = (Ctx*)h;
+    if (model < 0 || model >= (int)c->models.size()) return -1;
+    Material& M = c->models[model].M;
+    std::string nm(name);
+    if (nm == "n") M.nCreep = value;
+    else if (nm == "rm_abs_tol") M.rm_eabs = value;
+    else if (nm == "rm_rel_tol") M.rm_erel = value;
+    else if (nm == "rm_max_iterations") M.rm_maxit = (int)value;
+    else return -2;
+    return 0;
+}
+
 int ifo_assemble(void* h, int step, double* K_vals, double* f) {
     Ctx* c = (Ctx*)h;
     try {

@@ -43,6 +43,8 @@ def load():
         "ifo_damage": (None, [C.c_int, C.c_double, C.c_double, C.c_double, dp]),
         "ifo_return_map_viscous": (C.c_double, [C.POINTER(abi.Material), C.c_double, C.c_double, C.c_double,
                                                 C.c_double, dp, dp, dp, dp, dp, C.POINTER(C.c_int)]),
+        "ifo_set_element_params": (C.c_int, [P, C.c_int, dp, dp]),
+        "ifo_set_param": (C.c_int, [P, C.c_int, C.c_char_p, C.c_double]),
         "ifo_return_map_vep": (None, [C.POINTER(abi.Material), C.c_double, C.c_double, C.c_double, dp, dp, dp, dp,
                                       dp, dp, dp, C.POINTER(C.c_int)]),
         "ifo_energy_split": (C.c_double, [C.POINTER(abi.Material), C.c_int, C.c_double, C.c_double, dp, dp, dp,
@@ -176,6 +178,8 @@ class Oracle:
         for (model, name), vals in prob.history.items():
             h = self.history(model, name)
             h[:vals.shape[0]] = vals
+        for model, prm in prob.elem_params.items():
+            self.set_element_params(model, prm.get("Gc"), prm.get("A"))
 
     def pattern(self, step):
         n, nnz = self.n_rows(step), self.L.ifo_nnz(self.h, step)
@@ -192,6 +196,32 @@ class Oracle:
         if self.L.ifo_assemble(self.h, step, _dp(K), _dp(f)) != 0:
             raise RuntimeError("oracle: " + self.L.ifo_last_error().decode())
         return K, f
+
+    def set_element_params(self, model, Gc_e=None, A_e=None):
+        """Per-element Gc / creep prefactor A of a Fracture/PhaseField model (None: the material's)."""
+        ne = self.prob.mesh.n_elems
+        g = None if Gc_e is None else np.ascontiguousarray(Gc_e, dtype=np.float64).reshape(ne)
+        a = None if A_e is None else np.ascontiguousarray(A_e, dtype=np.float64).reshape(ne)
+        rc = self.L.ifo_set_element_params(self.h, model, _dp(g) if g is not None else None,
+                                           _dp(a) if a is not None else None)
+        if rc != 0:
+            raise RuntimeError(f"oracle: set_element_params failed ({rc})")
+
+    def set_param(self, model, name, value):
+        if self.L.ifo_set_param(self.h, model, name.encode(), float(value)) != 0:
+            raise RuntimeError(f"oracle: unknown parameter {name}")
+
+    def reset_step(self, model):
+        """BaseModel::ResetStep (PhaseField.cpp:147-163): every history array old -> current."""
+        for cur in ("Energy", "PlasticStrain", "ViscDiss", "DamViscDiss", "MatHist"):
+            h, ho = self.history(model, cur), self.history(model, cur + "Old")
+            if h is not None and ho is not None:
+                h[:] = ho
+
+    def residuals(self, steps=None):
+        """f of every step (assembled in step order, as one staggered pass), concatenated."""
+        steps = range(self.nsteps) if steps is None else steps
+        return np.concatenate([self.assemble(s)[1] for s in steps])
 
     def stats(self):
         o = np.zeros(3, dtype=np.int64)

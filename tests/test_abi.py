@@ -137,3 +137,41 @@ def test_coloring_is_a_valid_partition(name, nel):
         assert len(np.unique(nodes)) == len(nodes), f"colour {c} shares a node"
         assert np.all(np.diff(members) > 0)
     assert len(ptr) - 1 == 3 ** prob.mesh.dim
+
+
+def test_adjoint_and_spmv_argument_checks():
+    """ife_physics_adjoint rejects options it does not differentiate before touching the
+    device; both entry points need a set-up physics."""
+    lib = abi.load_library()
+    assert lib.ife_physics_adjoint(None, None, None, None, None, None) == 1
+    assert lib.ife_physics_spmv(None, 0, None, None, None, None) == 1
+    e = Physics(make_problem("hex3d_vep_at2", nel=(1, 1, 1)), device=False)
+    assert lib.ife_physics_adjoint(e.h, None, None, None, None, None) == 2  # VEP rheology
+    assert b"rheology" in lib.ife_last_error()
+    e = Physics(make_problem("hex3d_adjoint", nel=(1, 1, 1)), device=False)
+    assert lib.ife_physics_adjoint(e.h, None, None, None, None, None) == 5  # not set up
+    assert lib.ife_physics_spmv(e.h, 0, None, None, None, None) == 5
+    prob = make_problem("quad2d_le_at2", nel=(1, 1))
+    m = prob.models[0]
+    m.opts.LumpedDriving = 1
+    e = Physics(prob, device=False)
+    assert lib.ife_physics_adjoint(e.h, None, None, None, None, None) == 2
+
+
+def test_element_params_and_param_names_checked():
+    lib = abi.load_library()
+    e = Physics(make_problem("hex3d_adjoint", nel=(1, 1, 1)), device=False)
+    assert lib.ife_model_set_param(e.h, 0, b"no_such_parameter", 1.0) == 6  # not found
+    assert lib.ife_model_set_param(e.h, 1, b"n", 3.0) == 1                  # thermal model
+    assert lib.ife_model_set_param(e.h, 0, b"n", 3.0) == abi.IFE_OK
+    assert lib.ife_model_set_element_params(e.h, 0, None, None) == 5       # needs set-up
+
+
+def test_adjoint_workload_element_params_are_seeded():
+    prob = make_problem("hex3d_adjoint")
+    Gc, A = prob.elem_params[0]["Gc"], prob.elem_params[0]["A"]
+    assert Gc.shape == A.shape == (32 ** 3,)
+    assert abs(np.mean(np.log(Gc))) < 0.01 and abs(np.std(np.log(Gc)) - 0.3) < 0.01
+    assert A.min() >= 1e-25 and A.max() <= 1e-23
+    again = make_problem("hex3d_adjoint", nel=(2, 2, 2))
+    assert again.elem_params[0]["Gc"].shape == (8,)
