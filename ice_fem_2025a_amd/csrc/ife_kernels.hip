@@ -757,7 +757,11 @@ __device__ double split_vol_dev(const MatConst& mc, const MatIP& m, const double
 }
 
 // PhaseFieldMaterial::EnergySplit (PhaseFieldMaterial.cpp:47-94): None, VolStrains
-// (:194-215), VolStress (:97-117) and the splits above.  Returns the driving energy.
+// (:194-215), VolStress (:97-117) and, with FULL, the splits above.  Returns the driving
+// energy.  FULL = false compiles only the three closed-form splits (the caller guarantees
+// mc.Split is one of them), which keeps their instantiations free of the eigen-solver
+// splits' register and stack demand.
+template <bool FULL = true>
 __device__ double energy_split(const MatConst& mc, const MatIP& m, const double eps[6], double DD[6][6], double DU[6][6],
                                double SD[6], double SU[6]) {
     double E = 0;
@@ -796,13 +800,13 @@ __device__ double energy_split(const MatConst& mc, const MatIP& m, const double 
                 double P = (i == j ? 1.0 : 0.0) - ((i < 3 && j < 3) ? 1.0 / 3.0 : 0.0);
                 DD[i][j] += 2 * m.Shear * VC[i] * VDD[i] * VC[i] * P;
             }
-    } else if (mc.Split == IFE_SPLIT_SPECSTRAINS) {
+    } else if (FULL && mc.Split == IFE_SPLIT_SPECSTRAINS) {
         E = split_spec_strain(m, eps, DD, DU, SD, SU);
-    } else if (mc.Split == IFE_SPLIT_SPECSTRESS) {
+    } else if (FULL && mc.Split == IFE_SPLIT_SPECSTRESS) {
         E = split_spec_stress(m, eps);  // stress and tangent come from the override below
-    } else if (mc.Split == IFE_SPLIT_DRUCKERPRAGER) {
+    } else if (FULL && mc.Split == IFE_SPLIT_DRUCKERPRAGER) {
         E = split_drucker_prager(mc, m, eps, DD, DU, SD, SU);
-    } else if (mc.Split == IFE_SPLIT_ICE || mc.Split == IFE_SPLIT_STARCONVEX) {
+    } else if (FULL && (mc.Split == IFE_SPLIT_ICE || mc.Split == IFE_SPLIT_STARCONVEX)) {
         E = split_vol_dev(mc, m, eps, mc.Split == IFE_SPLIT_ICE, DD, DU, SD, SU);
     } else {  // VolStress
         double S[6];
@@ -878,10 +882,11 @@ __device__ void cap_coeffs(int method, double l, double phi, double& dN, double&
 }
 
 // energy split folded with the degradation: S = Dam*D_Dam + D_UnDam, sig = Dam*S_Dam + S_UnDam
+template <bool FULL = true>
 __device__ double energy_split_combined(const MatConst& mc, const MatIP& m, const double eps[6], double Dam, double S[6][6],
                                         double sig[6]) {
     double DD[6][6], DU[6][6], SD[6], SU[6];
-    double E = energy_split(mc, m, eps, DD, DU, SD, SU);
+    double E = energy_split<FULL>(mc, m, eps, DD, DU, SD, SU);
     for (int i = 0; i < 6; i++) {
         sig[i] = Dam * SD[i] + SU[i];
         for (int j = 0; j < 6; j++) S[i][j] = Dam * DD[i][j] + DU[i][j];
@@ -917,7 +922,8 @@ struct MatSmem {
 // d/dn through the converged Glen map by implicit differentiation with its exact Jacobian
 // (u residual via the stress, phase residual via the history energy, T residual via the
 // viscous heating).
-template <int DIM, bool VEP, bool EXACT, bool ADJ>
+// FULL: every energy split; false: None / VolStrains / VolStress only.
+template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL>
 __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
     using E = ET<DIM>;
     constexpr int N = E::N, NIP = E::NIP, EPB = NT / NIP;
@@ -1224,7 +1230,7 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             double ee[6];
             for (int r = 0; r < 6; r++) ee[r] = eps[r] - eps_pl[r];
             double Sm[6][6], sig[6];
-            energy_split_combined(mc, m, ee, Dam, Sm, sig);
+            energy_split_combined<FULL>(mc, m, ee, Dam, Sm, sig);
             if constexpr (ADJ) {
                 if (adj_on) {
                     // u residual: lambda_u^T dF_U = -w eps(lambda_u) . Sm d eps_p; T residual (viscous
@@ -1284,7 +1290,7 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
             double ee[6];
             for (int r = 0; r < 6; r++) ee[r] = eps[r] - eps_pl[r];
             double Sm[6][6], sig[6];
-            double Energy = energy_split_combined(mc, m, ee, 1.0, Sm, sig);
+            double Energy = energy_split_combined<FULL>(mc, m, ee, 1.0, Sm, sig);
             bool fromE = true;  // the history takes the current energy
             if (mc.Irreversible == IFE_IRR_HIST) {  // strict '>' (PhaseField.cpp:631-640)
                 double Ho = p.EOld[gi];
@@ -1314,7 +1320,7 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
                     MatConst mcs = mc;
                     mcs.ConsistentEnergy = 1;
                     double DD[6][6], DU[6][6], SD[6], SU[6];
-                    energy_split(mcs, m, ee, DD, DU, SD, SU);
+                    energy_split<FULL>(mcs, m, ee, DD, DU, SD, SU);
                     for (int k = 0; k < 2; k++) {
                         double dH = 0;
                         for (int i = 0; i < 6; i++) dH -= SD[i] * adj_dep[k][i];
@@ -2019,16 +2025,21 @@ hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
     const bool vep = p.mc.Rheology == IFE_RHEO_VEP;
     const bool exact = !vep && p.mc.exact_tangent;
+    const int sp = p.mc.Split;
+    const bool full = !(sp == IFE_SPLIT_NONE || sp == IFE_SPLIT_VOLSTRAINS || sp == IFE_SPLIT_VOLSTRESS);
     const dim3 g((unsigned)blocks), b(NT);
     if (vep) {
-        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, true, false, false>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((material_kernel<3, true, false, false>), g, b, 0, s, p);
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, true, false, false, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, true, false, false, true>), g, b, 0, s, p);
     } else if (exact) {
-        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, true, false>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((material_kernel<3, false, true, false>), g, b, 0, s, p);
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, true, false, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, false, true, false, true>), g, b, 0, s, p);
+    } else if (full) {
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, false, false, false, true>), g, b, 0, s, p);
     } else {
-        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((material_kernel<3, false, false, false>), g, b, 0, s, p);
+        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((material_kernel<3, false, false, false, false>), g, b, 0, s, p);
     }
     return hipGetLastError();
 }
@@ -2038,8 +2049,9 @@ hipError_t launch_adjoint(const KParams& p, int dim, hipStream_t s) {
     int64_t blocks = (p.NE + epb - 1) / epb;
     if (blocks <= 0) return hipSuccess;
     const dim3 g((unsigned)blocks), b(NT);
-    if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, true>), g, b, 0, s, p);
-    else hipLaunchKernelGGL((material_kernel<3, false, false, true>), g, b, 0, s, p);
+    // the host admits None / VolStrains / VolStress splits only
+    if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, true, false>), g, b, 0, s, p);
+    else hipLaunchKernelGGL((material_kernel<3, false, false, true, false>), g, b, 0, s, p);
     return hipGetLastError();
 }
 
