@@ -13,7 +13,16 @@ Also reported:
                divided by the mean kernel-launch duration measured with HIP events on
                the engine's stream over the timed region;
   cpu_baseline the FP64 oracle (a restatement of the reference element loop) timed on
-               this host's cores over a bounded element sample.
+               this host's cores over a bounded element sample;
+  mixed        the same step in mixed precision (FP32-MFMA element contractions) and its
+               speed-up over FP64.
+
+Other workloads (--workload):
+  quad2d_batch   64 independent 2D domains (u(2)+phi+T, 2k..60k elements each, ~1M in
+                 total) in FP32-MFMA mode, assembled by one autotuned batched launch per
+                 kernel captured in a hipGraph; a step is one graph replay.
+  hex3d_adjoint  32^3 hexes with per-element Gc and A: the adjoint pass
+                 (d(lambda^T r)/d(Gc_e, A_e, n)) after one assembly; a step is one adjoint.
 """
 import argparse
 import json
@@ -65,32 +74,11 @@ def traffic_from_profile(precision):
     return None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--workload", default="hex3d_glen_at2")
-    ap.add_argument("--nel", type=int, nargs="*", default=None, help="override element counts")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on one MI355X)",
-              file=sys.stderr)
-        sys.exit(2)
-
+def measure_single(prob, precision, steps, warmup):
+    """Time `steps` ResetStep + assemble_all steps of one domain on one stream."""
     from ice_fem_2025a_amd import hip
-    from ice_fem_2025a_amd.config import make_problem
     from ice_fem_2025a_amd.engine import Physics
-
-    t0 = time.time()
-    prob = make_problem(args.workload, nel=args.nel)
-    ph = Physics(prob, precision=args.precision)
-    print(f"# setup {time.time() - t0:.1f}s: {prob.mesh.n_elems} elements, "
-          f"{sum(ph.n_rows(s) for s in range(ph.nsteps))} dofs, "
-          f"{sum(ph.nnz(s) for s in range(ph.nsteps))} nnz", file=sys.stderr, flush=True)
+    ph = Physics(prob, precision=precision)
     stream = hip.Stream()
     E = prob.mesh.n_elems
 
@@ -98,20 +86,19 @@ def main():
         ph.reset_step(0, stream)
         ph.assemble_all(stream)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     stream.synchronize()
     st = ph.status(stream)
     assert st["nonfinite"] == 0, "non-finite values in the assembly"
     kbar = st["return_map_iterations"] / max(st["return_map_calls"], 1)
-
     ph.set_profiling(True)
     ph.timings()  # clear
     hip.synchronize()
     ev0, ev1 = hip.Event(), hip.Event()
     t_wall = time.perf_counter()
     ev0.record(stream)
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
     ev1.record(stream)
     stream.synchronize()
@@ -120,8 +107,193 @@ def main():
     ms_total = ev0.elapsed_ms(ev1)
     tim = ph.timings()
     ph.set_profiling(False)
-    ms_per_step = ms_total / args.steps
+    dofs = sum(ph.n_rows(s) for s in range(ph.nsteps))
+    nnz = sum(ph.nnz(s) for s in range(ph.nsteps))
+    ph.close()
+    ms_per_step = ms_total / steps
+    return {"ms_per_step": ms_per_step, "value": E / (ms_per_step / 1e3), "kbar": kbar, "tim": tim, "wall": wall,
+            "dofs": dofs, "nnz": nnz}
+
+
+def bench_batch(args):
+    """64 independent 2D domains, one batched assembly per step (hipGraph replay)."""
+    from ice_fem_2025a_amd import hip
+    from ice_fem_2025a_amd.config import batch_sizes, make_batch
+    from ice_fem_2025a_amd.engine import Batch, Physics
+    t0 = time.time()
+    probs = make_batch()
+    doms = [Physics(p, precision=args.precision) for p in probs]
+    E = sum(p.mesh.n_elems for p in probs)
+    b = Batch(doms)
+    print(f"# setup {time.time() - t0:.1f}s: {len(doms)} domains, {E} elements", file=sys.stderr, flush=True)
+    stream = hip.Stream()
+    tuned = b.tune(stream)
+    b.capture()
+    for _ in range(args.warmup):
+        b.launch(stream)
+    stream.synchronize()
+    for d in doms:
+        st = d.status()
+        assert st["nonfinite"] == 0 and st["elements"] == d.prob.mesh.n_elems
+    ev0, ev1 = hip.Event(), hip.Event()
+    t_wall = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        b.launch(stream)
+    ev1.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms_per_step = ev0.elapsed_ms(ev1) / args.steps
+    # the same work as one assemble_all per domain (four launches each), for comparison
+    ev0.record(stream)
+    for _ in range(args.steps):
+        for d in doms:
+            d.assemble_all(stream)
+    ev1.record(stream)
+    stream.synchronize()
+    ms_unbatched = ev0.elapsed_ms(ev1) / args.steps
     value = E / (ms_per_step / 1e3)
+    nnz = sum(d.nnz(s) for d in doms for s in range(d.nsteps))
+    dofs = sum(d.n_rows(s) for d in doms for s in range(d.nsteps))
+    # algorithmic HBM bytes of one assembly: CSR values and f written once, geometry
+    # (X 16 B/node), 1D extraction factors (144 B/element), nodal state read (U, phase,
+    # phase_old, T, dT: 40 B/node), history H_old read and H written (16 B/IP)
+    nodes = sum(p.mesh.n_nodes for p in probs)
+    alg_bytes = 8 * nnz + 8 * dofs + 16 * nodes + 144 * E + 40 * nodes + 16 * 9 * E
+    gbs = alg_bytes / (ms_per_step / 1e3) / 1e9
+    out = {
+        "metric": "elements assembled/sec (K+f), batched 2D Q2 Bezier quad domains, FP32-MFMA",
+        "value": value, "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-mfma+f64",
+        "data": "synthetic (seeded Philox sizes and states on generated B-spline patches)",
+        "config": {"workload": "quad2d_batch", "domains": len(doms), "elements": E, "dofs": dofs, "nnz": nnz,
+                   "elements_per_domain": [p.mesh.n_elems for p in probs][:64],
+                   "grids": [list(x) for x in batch_sizes()], "precision": args.precision,
+                   "step": "one hipGraph replay: status reset + batched material, element and gather kernels"},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                     "traffic": None, "kernel": "whole batched step (algorithmic bytes)",
+                     "algorithmic_bytes_per_step": alg_bytes},
+        "tuning": tuned,
+        "unbatched_ms_per_step": ms_unbatched,
+        "batched_speedup": ms_unbatched / ms_per_step,
+        "wall_s": wall,
+    }
+    if not args.no_cpu_baseline:
+        from oracle.oracle import Oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ne = secs = 0.0
+        for p in probs:  # whole domains until ~20k elements
+            o = Oracle(p)
+            secs += o.time_assembly(list(range(o.nsteps)), 0, p.mesh.n_elems, threads)
+            ne += p.mesh.n_elems
+            if ne >= 20000:
+                break
+        out["cpu_baseline"] = {"value": ne / secs, "unit": "elements/s", "cores": threads, "kind": "port",
+                               "sample": f"{int(ne)} elements (the first domains of the batch), all three steps, "
+                                         f"{secs:.1f}s on {threads} threads"}
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """Adjoint pass on the 32^3 per-element-parameter workload."""
+    import numpy as np
+
+    from ice_fem_2025a_amd import hip
+    from ice_fem_2025a_amd.config import make_cotangent, make_problem
+    from ice_fem_2025a_amd.engine import Physics
+    prob = make_problem("hex3d_adjoint", nel=args.nel)
+    ph = Physics(prob, precision=args.precision)
+    E = prob.mesh.n_elems
+    lam = make_cotangent([ph.n_rows(s) for s in range(ph.nsteps)])
+    bufs = [hip.DeviceArray.from_host(lam[s]) for s in range(ph.nsteps)]
+    outb = [hip.DeviceArray(E) for _ in range(3)]
+    stream = hip.Stream()
+    ph.reset_step(0, stream)
+    ph.assemble_all(stream)
+
+    def step():
+        ph.adjoint_device([b.ptr for b in bufs], *[o.ptr for o in outb], stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    assert ph.status(stream)["nonfinite"] == 0
+    ev0, ev1 = hip.Event(), hip.Event()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    stream.synchronize()
+    ms = ev0.elapsed_ms(ev1) / args.steps
+    ev0.record(stream)
+    for _ in range(args.steps):
+        ph.reset_step(0, stream)
+        ph.assemble_all(stream)
+    ev1.record(stream)
+    stream.synchronize()
+    ms_fwd = ev0.elapsed_ms(ev1) / args.steps
+    g = [o.get() for o in outb]
+    # algorithmic bytes of the adjoint pass: geometry and 1D extraction, nodal state and
+    # cotangents (U, U_old, phase, phase_old, T, lambda_u, lambda_phase, lambda_T), the
+    # per-IP history read (eps_p old + guess, hist, E_old, VD_old: 15 doubles) and three
+    # outputs per element
+    nodes = prob.mesh.n_nodes
+    alg = 32 * nodes + 216 * E + 8 * 13 * nodes + 8 * 15 * 27 * E + 24 * E
+    gbs = alg / (ms / 1e3) / 1e9
+    out = {
+        "metric": "adjoint gradient d(lambda^T r)/d(Gc_e, A_e, n): elements/s of the adjoint pass",
+        "value": E / (ms / 1e3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Philox states, log-normal Gc_e, log-uniform A_e, N(0,1) cotangent)",
+        "config": {"workload": "hex3d_adjoint", "elements": E, "mesh": list(prob.inputs["mesh"]["nel"]),
+                   "precision_of_forward": args.precision,
+                   "step": "one adjoint pass (state recomputed per IP, implicit Glen-map derivatives)"},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                     "traffic": None, "kernel": "material_kernel<3, adjoint>", "algorithmic_bytes": alg},
+        "forward_ms_per_assembly": ms_fwd,
+        "adjoint_over_forward": ms / ms_fwd,
+        "gradient_norms": {"Gc": float(np.linalg.norm(g[0])), "A": float(np.linalg.norm(g[1])),
+                           "n": float(np.sum(g[2]))},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--workload", default="hex3d_glen_at2")
+    ap.add_argument("--nel", type=int, nargs="*", default=None, help="override element counts")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    args.precision_given = args.precision is not None
+    args.precision = args.precision or "fp64"
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    if args.workload == "quad2d_batch":
+        if args.precision == "fp64" and not args.precision_given:
+            args.precision = "mixed"
+        return bench_batch(args)
+    if args.workload == "hex3d_adjoint":
+        return bench_adjoint(args)
+
+    from ice_fem_2025a_amd.config import make_problem
+
+    t0 = time.time()
+    prob = make_problem(args.workload, nel=args.nel)
+    E = prob.mesh.n_elems
+    m = measure_single(prob, args.precision, args.steps, args.warmup)
+    print(f"# setup+run {time.time() - t0:.1f}s: {E} elements, {m['dofs']} dofs, {m['nnz']} nnz",
+          file=sys.stderr, flush=True)
+    kbar, tim, wall, ms_per_step, value = m["kbar"], m["tim"], m["wall"], m["ms_per_step"], m["value"]
 
     avg_launch_ms = tim["kernel_ms_total"] / max(tim["launches"], 1)
     elems_per_launch = tim["elements"] / max(tim["launches"], 1)
@@ -158,6 +330,14 @@ def main():
                        "mean_return_map_iterations": kbar},
         "wall_s": wall,
     }
+    if args.precision == "fp64" and not args.precision_given:
+        mx = measure_single(prob, "mixed", args.steps, args.warmup)
+        out["mixed"] = {"value": mx["value"], "ms_per_step": mx["ms_per_step"],
+                        "speedup_vs_fp64": ms_per_step / mx["ms_per_step"],
+                        "kernels_ms_per_step": {"material_kernel": mx["tim"]["material_ms_total"] / args.steps,
+                                                "assemble_kernel": mx["tim"]["kernel_ms_total"] / args.steps,
+                                                "gather_kernel": mx["tim"]["gather_ms_total"] / args.steps},
+                        "error_bound_vs_fp64_oracle": "1e-6 relative (row/step-scaled), tests/test_gpu_mixed.py"}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         rate, ne, secs = cpu_baseline(args.workload, (30, 30, 30) if args.workload.startswith("hex3d") else None,

@@ -77,6 +77,13 @@ class Timings(C.Structure):
 
 # exported symbols and their prototypes: (name, restype, argtypes)
 _P = C.c_void_p
+class BatchTuning(C.Structure):
+    _fields_ = [("grid_material", C.c_int64), ("grid_assemble", C.c_int64), ("grid_gather", C.c_int64),
+                ("lanes_assemble", C.c_int32), ("candidates_timed", C.c_int32), ("n_domains", C.c_int32),
+                ("blocks_material", C.c_int64), ("elements", C.c_int64), ("nodes", C.c_int64),
+                ("ms_material", C.c_double), ("ms_assemble", C.c_double), ("ms_gather", C.c_double)]
+
+
 _PROTOS = [
     ("ife_physics_create", C.c_int, [C.POINTER(MeshDesc), C.POINTER(DofsDesc), C.POINTER(_P)]),
     ("ife_physics_add_model", C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(Material),
@@ -108,6 +115,15 @@ _PROTOS = [
     ("ife_physics_spmv", C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
     ("ife_model_set_element_params", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("ife_model_set_param", C.c_int, [_P, C.c_int32, C.c_char_p, C.c_double]),
+    ("ife_batch_create", C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
+    ("ife_batch_update", C.c_int, [_P]),
+    ("ife_batch_assemble", C.c_int, [_P, _P]),
+    ("ife_batch_capture", C.c_int, [_P]),
+    ("ife_batch_launch", C.c_int, [_P, _P]),
+    ("ife_batch_tune", C.c_int, [_P, _P, C.POINTER(BatchTuning)]),
+    ("ife_batch_set_shape", C.c_int, [_P, C.POINTER(BatchTuning)]),
+    ("ife_batch_get_shape", C.c_int, [_P, C.POINTER(BatchTuning)]),
+    ("ife_batch_destroy", None, [_P]),
     ("ife_last_error", C.c_char_p, []),
     ("ife_api_version", C.c_int, []),
 ]

@@ -9,6 +9,8 @@ Workloads (the BASELINE configurations, named by what they are):
   "quad2d_le_at2"   64x32 B2_2 quads on 0.1x0.05 m, u(2)+phi, linear elastic + AT2
   "hex3d_glen_at2"  40x40x80 NURBS3_Cube2 hexes, u(3)+phi+T, Glen viscous + AT2 + thermal
   "hex3d_vep_at2"   16x16x16 hexes, as hex3d_glen_at2 with the visco-elasto-plastic rheology
+  "quad2d_thermal"  one domain of the batched 2D workload: as quad2d_le_at2 plus T and
+                    Thermal/ThermalModel (u(2)+phi+T); make_batch draws 64 of them
   "hex3d_adjoint"   32x32x32 hexes on a 0.08 m cube, as hex3d_glen_at2, with per-element Gc
                     log-normal(0, 0.3) J/m^2 and Glen A log-uniform in [1e-25, 1e-23] Pa^-3 s^-1
                     (the adjoint-sensitivity workload; cotangents from make_cotangent)
@@ -92,6 +94,16 @@ WORKLOADS["hex3d_vep_at2"]["properties"]["Ice"]["SolidProperties"].update(
     {"Rheology": "ViscoElasticPlastic", "sy": 5.0e5, "F_deg": 20.0, "G_deg": 10.0, "hardening": 1.0e8,
      "pl_visc": 1.0e9})
 
+
+# One domain of the batched 2D workload: the 2D LE/AT2 physics plus temperature and the
+# thermal model (dofs u(2)+phi+T).
+WORKLOADS["quad2d_thermal"] = copy.deepcopy(WORKLOADS["quad2d_le_at2"])
+WORKLOADS["quad2d_thermal"]["index"] = 3
+WORKLOADS["quad2d_thermal"]["Dofs"] = {"DofNames": ["ux", "uy", "phase", "T"], "DofStep": [0, 0, 1, 2]}
+WORKLOADS["quad2d_thermal"]["Models"]["Names"] = ["PhaseField1", "Thermal"]
+WORKLOADS["quad2d_thermal"]["Models"]["PhaseField1"]["ElementGroup_T"] = "internal"
+WORKLOADS["quad2d_thermal"]["Models"]["Thermal"] = {"Name": "Thermal/ThermalModel", "ElementGroup_T": "internal",
+                                                    "Material": "Ice"}
 
 # Adjoint-sensitivity workload: the 3D Glen/AT2/thermal physics on a 32^3 block with
 # per-element Gc and A drawn after the state.
@@ -242,10 +254,12 @@ def build_models(inputs):
     return out
 
 
-def make_problem(name, nel=None, seed=None):
+def make_problem(name, nel=None, seed=None, h=None):
     """Build a seeded workload.  Draws use numpy's Philox with seed 20261203 + index,
     per node (node order) or per IP ([element][ip] order)."""
     inp = get_inputs(name, nel)
+    if h is not None:  # fixed element size instead of the workload's box
+        inp["mesh"]["lengths"] = [h * n for n in inp["mesh"]["nel"]]
     ms = inp["mesh"]
     mesh = make_patch(ms["nel"], ms["lengths"])
     dim = mesh.dim
@@ -257,12 +271,15 @@ def make_problem(name, nel=None, seed=None):
     NN, E, nip = mesh.n_nodes, mesh.n_elems, prob.nip
     ucomp = ["ux", "uy", "uz"][:dim]
     L = ms["lengths"][0] * (1.0 if nel is None else WORKLOADS[name]["mesh"]["nel"][0] / ms["nel"][0])
-    if WORKLOADS[name]["index"] == 0:
+    if WORKLOADS[name]["index"] in (0, 3):
         # U ~ U(-1e-3 L, 1e-3 L) per dof, phi ~ U[0, 0.9), phi_old = phi, H_old = 0
         for c in ucomp:
             prob.node_state[(abi.IFE_STATE, c)] = rng.uniform(-1e-3 * L, 1e-3 * L, NN)
         prob.node_state[(abi.IFE_STATE, "phase")] = rng.uniform(0.0, 0.9, NN)
         prob.node_state[(abi.IFE_STATE_OLD, "phase")] = prob.node_state[(abi.IFE_STATE, "phase")].copy()
+        if "T" in prob.dof_names:  # T as in the 3D workload: U[250, 273] K, dT ~ U(-1e-2, 1e-2)
+            prob.node_state[(abi.IFE_STATE, "T")] = rng.uniform(250.0, 273.0, NN)
+            prob.node_state[(abi.IFE_DSTATE, "T")] = rng.uniform(-1e-2, 1e-2, NN)
     else:
         # T ~ U[250,273] K; displacement increments with per-node strain scale
         # log-uniform in [1e-6, 1e-3] times the element size; U_old = U - dU
@@ -284,6 +301,32 @@ def make_problem(name, nel=None, seed=None):
         prob.elem_params[0] = {"Gc": np.exp(rng.normal(0.0, 0.3, E)),
                                "A": np.exp(rng.uniform(np.log(1e-25), np.log(1e-23), E))}
     return prob
+
+
+def batch_sizes(n_domains=64, nel_range=(2000, 60000), seed=20261203 + 3):
+    """Element grids of the batched 2D workload: nel ~ log-U[nel_range], aspect ratio
+    nx/ny ~ U[1, 4], nx = round(sqrt(nel r)), ny = max(1, round(nel / nx))."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    nel = np.exp(rng.uniform(np.log(nel_range[0]), np.log(nel_range[1]), n_domains))
+    ratio = rng.uniform(1.0, 4.0, n_domains)
+    out = []
+    for ne, r in zip(nel, ratio):
+        nx = max(1, int(round(np.sqrt(ne * r))))
+        ny = max(1, int(round(ne / nx)))
+        out.append((nx, ny))
+    return out
+
+
+def make_batch(n_domains=64, nel_range=(2000, 60000), seed=20261203 + 3):
+    """The batched 2D workload: independent quad2d_thermal domains with seeded sizes; the
+    element size is the single-domain workload's (0.1 m / 64), domain d's state is drawn
+    with seed + 1000 (d + 1)."""
+    h = WORKLOADS["quad2d_thermal"]["mesh"]["lengths"][0] / WORKLOADS["quad2d_thermal"]["mesh"]["nel"][0]
+    probs = []
+    for d, (nx, ny) in enumerate(batch_sizes(n_domains, nel_range, seed)):
+        p = make_problem("quad2d_thermal", nel=(nx, ny), seed=seed + 1000 * (d + 1), h=h)
+        probs.append(p)
+    return probs
 
 
 def make_cotangent(n_rows, seed=20261203):

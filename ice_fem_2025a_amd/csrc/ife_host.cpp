@@ -578,6 +578,21 @@ KParams base_params(ife_physics* ph) {
     return p;
 }
 
+// KParams of one assembly of every step into the engine-owned K/f (what run(all) launches)
+KParams all_steps_params(ife_physics* ph) {
+    require_ready(ph);
+    KParams p = base_params(ph);
+    Model* pfm = ph->pf_model();
+    Model* thm = ph->th_model();
+    p.do_u = pfm != nullptr;
+    p.do_ph = pfm != nullptr;
+    p.do_Tpf = pfm && pfm->HasT && pfm->opt.ViscousHeating;
+    p.do_th = thm != nullptr;
+    p.elist = nullptr;
+    p.nlist = ph->NE;
+    return p;
+}
+
 void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* fv, hipStream_t s) {
     require_ready(ph);
     KParams p = base_params(ph);
@@ -677,6 +692,156 @@ DevBuf<double>* history_buf(Model* m, const std::string& nm, int64_t& ncomp) {
     if (nm == "MatHist") { ncomp = std::max(m->HistSize, 1); return &m->MH; }
     if (nm == "MatHistOld") { ncomp = std::max(m->HistSize, 1); return &m->MHOld; }
     return nullptr;
+}
+
+}  // namespace
+
+// ==================================================================================
+// Batches of independent domains
+// ==================================================================================
+struct ife_batch {
+    std::vector<ife_physics*> doms;
+    int dim = 0, mixed = 0, vep = 0, exact = 0, full = 0;
+    DevBuf<KParams> dP;
+    DevBuf<int64_t> bp[3];      // block prefix sums: material, element, gather kernels
+    int64_t tot[3] = {0, 0, 0};
+    int64_t grid[3] = {0, 0, 0};  // workgroups per launch (0: one per block)
+    int nta = 256;
+    hipStream_t cap = nullptr;    // capture stream (default priority)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~ife_batch() {
+        drop_graph();
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+    BatchArgs args() const {
+        BatchArgs a{};
+        a.P = dP.p;
+        a.nd = (int)doms.size();
+        a.dim = dim; a.mixed = mixed; a.vep = vep; a.exact = exact; a.full = full;
+        for (int k = 0; k < 3; k++) { a.bptr[k] = bp[k].p; a.tot[k] = tot[k]; }
+        return a;
+    }
+};
+
+namespace {
+
+// (Re)read every domain's parameters (time, material constants, precision) into the
+// device parameter array; the block tables depend only on the meshes.
+void batch_update(ife_batch* b) {
+    std::vector<KParams> P;
+    int vep = -1, exact = -1, full = 0, mixed = -1;
+    for (auto* ph : b->doms) {
+        KParams p = all_steps_params(ph);
+        const int v = p.mc.Rheology == IFE_RHEO_VEP;
+        const int x = !v && p.mc.exact_tangent;
+        const int sp = p.mc.Split;
+        full |= !(sp == IFE_SPLIT_NONE || sp == IFE_SPLIT_VOLSTRAINS || sp == IFE_SPLIT_VOLSTRESS);
+        if (vep < 0) { vep = v; exact = x; mixed = p.mixed; }
+        if (v != vep || x != exact)
+            THROW(IFE_ERR_UNSUPPORTED, "batch: domains need the same return map (Glen / visco-elasto-plastic, tangent)");
+        if (p.mixed != mixed) THROW(IFE_ERR_UNSUPPORTED, "batch: domains need the same precision");
+        P.push_back(p);
+    }
+    b->vep = vep; b->exact = exact; b->full = full; b->mixed = mixed;
+    b->dP.upload(P.data(), P.size());
+    b->drop_graph();  // kernel arguments point at dP, which upload may have moved
+}
+
+void batch_enqueue(ife_batch* b, hipStream_t s) {
+    const BatchArgs a = b->args();
+    HIPCHK(launch_batch_zero(a, s));
+    HIPCHK(launch_batch_material(a, b->grid[0], s));
+    HIPCHK(launch_batch_assemble(a, b->grid[1], b->nta, s));
+    HIPCHK(launch_batch_gather(a, b->grid[2], s));
+    for (auto* ph : b->doms) ph->last_had_phase = ph->pf_model() != nullptr;
+}
+
+void batch_capture(ife_batch* b) {
+    b->drop_graph();
+    if (!b->cap) HIPCHK(hipStreamCreate(&b->cap));
+    HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    try {
+        batch_enqueue(b, b->cap);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->cap, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(b->cap, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+
+// Launch-shape autotuner: per kernel, the median of three timed launches of every
+// candidate shape (after one warm-up launch) on the current inputs; the fastest is kept.
+void batch_tune(ife_batch* b, hipStream_t s, ife_batch_tuning* out) {
+    batch_enqueue(b, s);  // every kernel's inputs in place
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int timed = 0;
+    auto time_it = [&](const std::function<hipError_t()>& launch) {
+        HIPCHK(launch());
+        float t[3];
+        for (int r = 0; r < 3; r++) {
+            HIPCHK(hipEventRecord(e0, s));
+            HIPCHK(launch());
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
+        }
+        std::sort(t, t + 3);
+        timed++;
+        return (double)t[1];
+    };
+    auto candidates = [&](int64_t total) {
+        std::vector<int64_t> c = {0};
+        for (int64_t g = 256; g < total && g <= 256 * 64; g *= 2) c.push_back(g);
+        return c;
+    };
+    const BatchArgs a = b->args();
+    double best[3] = {1e300, 1e300, 1e300};
+    try {
+        for (int64_t g : candidates(b->tot[0])) {
+            double ms = time_it([&] { return launch_batch_material(a, g, s); });
+            if (ms < best[0]) { best[0] = ms; b->grid[0] = g; }
+        }
+        std::vector<int> ntas = {256};
+        if (b->dim == 2) ntas.push_back(192);
+        for (int nt : ntas)
+            for (int64_t g : candidates(b->tot[1])) {
+                double ms = time_it([&] { return launch_batch_assemble(a, g, nt, s); });
+                if (ms < best[1]) { best[1] = ms; b->grid[1] = g; b->nta = nt; }
+            }
+        for (int64_t g : candidates(b->tot[2])) {
+            double ms = time_it([&] { return launch_batch_gather(a, g, s); });
+            if (ms < best[2]) { best[2] = ms; b->grid[2] = g; }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->drop_graph();
+    if (out) {
+        out->grid_material = b->grid[0];
+        out->grid_assemble = b->grid[1];
+        out->grid_gather = b->grid[2];
+        out->lanes_assemble = b->nta;
+        out->candidates_timed = timed;
+        out->ms_material = best[0];
+        out->ms_assemble = best[1];
+        out->ms_gather = best[2];
+    }
 }
 
 }  // namespace
@@ -1063,5 +1228,101 @@ int ife_model_commit(ife_physics* ph, int32_t model, int32_t commit_type, void* 
 }
 
 int ife_model_reset_step(ife_physics* ph, int32_t model, void* stream) { return copy_history(ph, model, false, stream); }
+
+int ife_batch_create(ife_physics* const* domains, int32_t n_domains, ife_batch** out) {
+    return guard([&] {
+        if (!out || (!domains && n_domains > 0) || n_domains <= 0) THROW(IFE_ERR_INVALID, "batch: no domains");
+        *out = nullptr;
+        auto b = std::make_unique<ife_batch>();
+        std::vector<int64_t> pre[3];
+        for (int k = 0; k < 3; k++) pre[k].push_back(0);
+        for (int32_t d = 0; d < n_domains; d++) {
+            ife_physics* ph = domains[d];
+            if (!ph) THROW(IFE_ERR_INVALID, "batch: null domain");
+            require_ready(ph);
+            if (d == 0) b->dim = ph->dim;
+            if (ph->dim != b->dim) THROW(IFE_ERR_UNSUPPORTED, "batch: domains need the same dimension");
+            b->doms.push_back(ph);
+            pre[0].push_back(pre[0].back() + blocks_material(ph->dim, ph->NE));
+            pre[1].push_back(pre[1].back() + ph->NE);
+            pre[2].push_back(pre[2].back() + ph->Ng);
+        }
+        for (int k = 0; k < 3; k++) {
+            b->bp[k].upload(pre[k].data(), pre[k].size());
+            b->tot[k] = pre[k].back();
+        }
+        batch_update(b.get());
+        *out = b.release();
+    });
+}
+
+int ife_batch_update(ife_batch* b) {
+    return guard([&] {
+        if (!b) THROW(IFE_ERR_INVALID, "null batch");
+        batch_update(b);
+    });
+}
+
+int ife_batch_assemble(ife_batch* b, void* stream) {
+    return guard([&] {
+        if (!b) THROW(IFE_ERR_INVALID, "null batch");
+        batch_enqueue(b, (hipStream_t)stream);
+    });
+}
+
+int ife_batch_capture(ife_batch* b) {
+    return guard([&] {
+        if (!b) THROW(IFE_ERR_INVALID, "null batch");
+        batch_capture(b);
+    });
+}
+
+int ife_batch_launch(ife_batch* b, void* stream) {
+    return guard([&] {
+        if (!b) THROW(IFE_ERR_INVALID, "null batch");
+        if (!b->exec) batch_capture(b);
+        HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+        for (auto* ph : b->doms) ph->last_had_phase = ph->pf_model() != nullptr;
+    });
+}
+
+int ife_batch_tune(ife_batch* b, void* stream, ife_batch_tuning* out) {
+    return guard([&] {
+        if (!b) THROW(IFE_ERR_INVALID, "null batch");
+        batch_tune(b, (hipStream_t)stream, out);
+    });
+}
+
+int ife_batch_set_shape(ife_batch* b, const ife_batch_tuning* shape) {
+    return guard([&] {
+        if (!b || !shape) THROW(IFE_ERR_INVALID, "null argument");
+        if (shape->grid_material < 0 || shape->grid_assemble < 0 || shape->grid_gather < 0)
+            THROW(IFE_ERR_INVALID, "batch: negative grid");
+        if (!(shape->lanes_assemble == 256 || (shape->lanes_assemble == 192 && b->dim == 2)))
+            THROW(IFE_ERR_INVALID, "batch: element kernel lanes must be 256 (or 192 in 2D)");
+        b->grid[0] = shape->grid_material;
+        b->grid[1] = shape->grid_assemble;
+        b->grid[2] = shape->grid_gather;
+        b->nta = shape->lanes_assemble;
+        b->drop_graph();
+    });
+}
+
+int ife_batch_get_shape(const ife_batch* b, ife_batch_tuning* out) {
+    return guard([&] {
+        if (!b || !out) THROW(IFE_ERR_INVALID, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        out->grid_material = b->grid[0];
+        out->grid_assemble = b->grid[1];
+        out->grid_gather = b->grid[2];
+        out->lanes_assemble = b->nta;
+        out->n_domains = (int32_t)b->doms.size();
+        out->blocks_material = b->tot[0];
+        out->elements = b->tot[1];
+        out->nodes = b->tot[2];
+    });
+}
+
+void ife_batch_destroy(ife_batch* b) { delete b; }
 
 }  // extern "C"

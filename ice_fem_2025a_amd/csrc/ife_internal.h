@@ -124,6 +124,17 @@ struct KParams {
     int mixed;               // tangent contractions in FP32
 };
 
+// One batched launch over nd domains: P[d] is domain d's KParams (device memory), bptr[k]
+// (device, nd + 1 entries) the prefix sums of the domains' block counts for kernel k
+// (0 material, 1 element, 2 gather) and tot[k] their totals.  All domains share dim,
+// precision and material-kernel variant.
+struct BatchArgs {
+    const KParams* P;
+    int nd, dim, mixed, vep, exact, full;
+    const int64_t* bptr[3];
+    int64_t tot[3];
+};
+
 // Launchers (ife_kernels.hip)
 hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_material(const KParams& p, int dim, hipStream_t s);
@@ -135,5 +146,12 @@ int record_size(int dim);
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 hipError_t launch_zero_status(Status* st, hipStream_t s);
+// batched forms; grid = number of workgroups launched (0 or >= the total: one per block,
+// else persistent workgroups striding over the blocks); nta = lanes of the element kernel
+hipError_t launch_batch_zero(const BatchArgs& a, hipStream_t s);
+hipError_t launch_batch_material(const BatchArgs& a, int64_t grid, hipStream_t s);
+hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipStream_t s);
+hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s);
+int blocks_material(int dim, int64_t n_elems);
 
 }  // namespace ife

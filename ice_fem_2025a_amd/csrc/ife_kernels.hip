@@ -923,14 +923,15 @@ struct MatSmem {
 // (u residual via the stress, phase residual via the history energy, T residual via the
 // viscous heating).
 // FULL: every energy split; false: None / VolStrains / VolStress only.
+// blk: the workgroup's block of EPB elements within the domain p.
 template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL>
-__global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
+__device__ __forceinline__ void material_body(const KParams& p, const int64_t blk) {
     using E = ET<DIM>;
     constexpr int N = E::N, NIP = E::NIP, EPB = NT / NIP;
     __shared__ MatSmem<DIM, ADJ> sm;
     const int tid = threadIdx.x;
     const MatConst& mc = p.mc;
-    const int64_t e0 = (int64_t)blockIdx.x * EPB;
+    const int64_t e0 = blk * EPB;
     if (tid < 4) sm.cnt[tid] = 0;
     // ---- stage the block's elements ----------------------------------------------
     for (int i = tid; i < EPB * N; i += NT) {
@@ -1428,6 +1429,37 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
     }
 }
 
+// The domain of flattened block b of a batch: the last d with bptr[d] <= b (bptr ascending,
+// bptr[0] = 0, bptr[nd] = total).  Every lane computes the same value.
+__device__ __forceinline__ int batch_domain(const int64_t* __restrict__ bptr, int nd, int64_t b) {
+    int lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (bptr[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL>
+__global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
+    material_body<DIM, VEP, EXACT, ADJ, FULL>(p, blockIdx.x);
+}
+
+// Batched form: the blocks of all domains in one launch.  gridDim.x may be below the total
+// (persistent workgroups striding over the blocks); the barrier at the top of each item
+// keeps the next item's LDS staging behind the previous item's last LDS reads.
+template <int DIM, bool VEP, bool EXACT, bool FULL>
+__global__ __launch_bounds__(NT, 2) void material_batch(const KParams* __restrict__ P, const int64_t* __restrict__ bptr,
+                                                        int nd) {
+    const int64_t total = bptr[nd];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int d = batch_domain(bptr, nd, b);
+        __syncthreads();
+        material_body<DIM, VEP, EXACT, false, FULL>(P[d], b - bptr[d]);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Element kernel: one workgroup per element.  Shape functions in LDS, per-IP coefficients
 // from the material kernel, MFMA contractions, node-row records out.
@@ -1702,24 +1734,26 @@ __device__ __forceinline__ void nodal_terms(Smem<DIM, T>& sm, const KParams& p, 
 
 // T: the tile type of the element contractions (double: FP64 mode; float: mixed mode, FP32
 // MFMA); R: the element record type.
-template <int DIM, typename T, typename R>
-__global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void assemble_kernel(KParams p) {
+// NTA: lanes per workgroup, 64 (DIM + 1) or 256 (one idle wave in 2D).
+template <int DIM, typename T, typename R, int NTA>
+__device__ __forceinline__ void assemble_body(const KParams& p, const int64_t blk) {
     using E = ET<DIM>;
     using SM = Smem<DIM, T>;
     using RC = Rec<DIM>;
     constexpr int N = E::N, NIP = E::NIP, NP = SM::NP, IPP = SM::IPP, MT = NP / 16;
+    static_assert(NTA >= 64 * (DIM + 1) && NTA <= 256 && NTA % 64 == 0, "one wave per role");
     __shared__ SM sm;
     const int tid = threadIdx.x;
-    const int64_t el = p.elist ? (int64_t)p.elist[blockIdx.x] : (int64_t)blockIdx.x;
+    const int64_t el = p.elist ? (int64_t)p.elist[blk] : blk;
     R* const rec = (R*)p.ebuf + el * N * RC::SIZE;  // this element's records
     // per-IP coefficients of this element: one contiguous [field][ip] block, loads issued first
-    constexpr int IPC_NV = IPB_NF * NIP, IPC_PER = (IPC_NV + NT - 1) / NT;
+    constexpr int IPC_NV = IPB_NF * NIP, IPC_PER = (IPC_NV + NTA - 1) / NTA;
     double ipv[IPC_PER];
     {
         const double* const src = p.ipb + el * IPC_NV;
 #pragma unroll
         for (int k = 0; k < IPC_PER; k++) {
-            const int t = tid + k * NT;
+            const int t = tid + k * NTA;
             ipv[k] = t < IPC_NV ? src[t] : 0.0;
         }
     }
@@ -1752,10 +1786,10 @@ __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void assemble_kernel(KP
     }
 #pragma unroll
     for (int k = 0; k < IPC_PER; k++) {
-        const int t = tid + k * NT;
+        const int t = tid + k * NTA;
         if (t < IPC_NV) sm.ipc[t / NIP][t % NIP] = ipv[k];
     }
-    for (int t = tid; t < IPB_NF * (IPP - NIP); t += NT) sm.ipc[t / (IPP - NIP)][NIP + t % (IPP - NIP)] = 0.0;
+    for (int t = tid; t < IPB_NF * (IPP - NIP); t += NTA) sm.ipc[t / (IPP - NIP)][NIP + t % (IPP - NIP)] = 0.0;
     __syncthreads();
 
     // ---- shape functions ------------------------------------------------------------
@@ -1764,7 +1798,7 @@ __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void assemble_kernel(KP
     // NURBS3_Cube2.cpp:96-112) and G = J^-1 GXi (B2_2.cpp:79-85 in 2D: N = CB, GXi = GB,
     // J^-1 in the transposed form), with J^-1 / wFun, 1 / wFun and J^-1 dW / wFun^2 taken
     // from the material kernel, which forms them at the same IP.  Padding entries are zeroed.
-    for (int i = tid; i < IPP * NP; i += NT) {
+    for (int i = tid; i < IPP * NP; i += NTA) {
         const int ip = i / NP, a = i % NP;
         if (ip >= NIP || a >= N) {
             sm.Nsh[ip][a] = 0;
@@ -1826,6 +1860,22 @@ __global__ __launch_bounds__(NT, sizeof(T) == 8 ? 3 : 4) void assemble_kernel(KP
     if (chk != 0.0) atomicOr(&p.st->nonfinite, 1u);
 }
 
+template <int DIM, typename T, typename R, int NTA>
+__global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_kernel(KParams p) {
+    assemble_body<DIM, T, R, NTA>(p, blockIdx.x);
+}
+
+template <int DIM, typename T, typename R, int NTA>
+__global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_batch(const KParams* __restrict__ P,
+                                                                               const int64_t* __restrict__ bptr, int nd) {
+    const int64_t total = bptr[nd];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int d = batch_domain(bptr, nd, b);
+        __syncthreads();
+        assemble_body<DIM, T, R, NTA>(P[d], b - bptr[d]);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Gather kernel: one wavefront per node (group rank r).  Adds the records of the node's
 // adjacent elements, in increasing element index (the order in which the reference's
@@ -1865,7 +1915,7 @@ struct RecLoad<float> {
 
 // T: element record type (double, or float in mixed mode); the rows accumulate in FP64.
 template <int DIM, typename T>
-__global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
+__device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk) {
     using RC = Rec<DIM>;
     typedef typename RecLoad<T>::V V;
     constexpr int N = RC::N, W = RecLoad<T>::W, PER = (RC::SIZE + GW * W - 1) / (GW * W);
@@ -1874,7 +1924,7 @@ __global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
     constexpr int OKP = DIM * DIM * MAXDEG, OKT = OKP + MAXDEG, OF = OKT + MAXDEG, LEN = OF + DIM + 2;
     __shared__ double acc[LEN];
     const int lane = threadIdx.x;
-    const int64_t r = blockIdx.x;
+    const int64_t r = blk;
     const int64_t r0 = p.nbr_ptr[r];
     const int deg = (int)(p.nbr_ptr[r + 1] - r0);
     const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
@@ -1961,6 +2011,22 @@ __global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
     }
 }
 
+template <int DIM, typename T>
+__global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
+    gather_body<DIM, T>(p, blockIdx.x);
+}
+
+template <int DIM, typename T>
+__global__ __launch_bounds__(GW) void gather_batch(const KParams* __restrict__ P, const int64_t* __restrict__ bptr,
+                                                   int nd) {
+    const int64_t total = bptr[nd];
+    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const int d = batch_domain(bptr, nd, b);
+        __syncthreads();
+        gather_body<DIM, T>(P[d], b - bptr[d]);
+    }
+}
+
 __global__ void fill_kernel(double* x, int64_t n, double v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1969,6 +2035,14 @@ __global__ void fill_kernel(double* x, int64_t n, double v) {
 
 __global__ void zero_status_kernel(Status* s) {
     if (threadIdx.x == 0) {
+        s->elements = 0; s->rm_calls = 0; s->rm_iters = 0; s->rm_maxit = 0; s->nonfinite = 0;
+    }
+}
+
+__global__ void zero_status_batch(const KParams* __restrict__ P, int nd) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < nd) {
+        Status* s = P[d].st;
         s->elements = 0; s->rm_calls = 0; s->rm_iters = 0; s->rm_maxit = 0; s->nonfinite = 0;
     }
 }
@@ -2000,11 +2074,11 @@ hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
     dim3 grid((unsigned)p.nlist), block(NT);
     // Element records stay FP64 in both modes: the gather reads FP32 records no faster.
     if (p.mixed) {
-        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, float, double>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((assemble_kernel<3, float, double>), grid, block, 0, s, p);
+        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, float, double, NT>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((assemble_kernel<3, float, double, NT>), grid, block, 0, s, p);
     } else {
-        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, double, double>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((assemble_kernel<3, double, double>), grid, block, 0, s, p);
+        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, double, double, NT>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((assemble_kernel<3, double, double, NT>), grid, block, 0, s, p);
     }
     return hipGetLastError();
 }
@@ -2081,6 +2155,63 @@ hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s)
 
 hipError_t launch_zero_status(Status* st, hipStream_t s) {
     hipLaunchKernelGGL(zero_status_kernel, dim3(1), dim3(64), 0, s, st);
+    return hipGetLastError();
+}
+
+int blocks_material(int dim, int64_t n_elems) {
+    const int epb = NT / (dim == 2 ? 9 : 27);
+    return (int)((n_elems + epb - 1) / epb);
+}
+
+namespace {
+dim3 batch_grid(int64_t grid, int64_t total) { return dim3((unsigned)(grid > 0 && grid < total ? grid : total)); }
+}  // namespace
+
+hipError_t launch_batch_zero(const BatchArgs& a, hipStream_t s) {
+    if (a.nd <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_status_batch, dim3((a.nd + 63) / 64), dim3(64), 0, s, a.P, a.nd);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_material(const BatchArgs& a, int64_t grid, hipStream_t s) {
+    if (a.tot[0] <= 0) return hipSuccess;
+    const dim3 g = batch_grid(grid, a.tot[0]), b(NT);
+    const int64_t* bp = a.bptr[0];
+#define IFE_MB(D, V, X, F) hipLaunchKernelGGL((material_batch<D, V, X, F>), g, b, 0, s, a.P, bp, a.nd)
+    if (a.vep) { if (a.dim == 2) IFE_MB(2, true, false, true); else IFE_MB(3, true, false, true); }
+    else if (a.exact) { if (a.dim == 2) IFE_MB(2, false, true, true); else IFE_MB(3, false, true, true); }
+    else if (a.full) { if (a.dim == 2) IFE_MB(2, false, false, true); else IFE_MB(3, false, false, true); }
+    else { if (a.dim == 2) IFE_MB(2, false, false, false); else IFE_MB(3, false, false, false); }
+#undef IFE_MB
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipStream_t s) {
+    if (a.tot[1] <= 0) return hipSuccess;
+    const dim3 g = batch_grid(grid, a.tot[1]);
+    const int64_t* bp = a.bptr[1];
+    if (a.dim == 3 || nta != 192) {
+        const dim3 b(NT);
+        if (a.mixed) {
+            if (a.dim == 2) hipLaunchKernelGGL((assemble_batch<2, float, double, NT>), g, b, 0, s, a.P, bp, a.nd);
+            else hipLaunchKernelGGL((assemble_batch<3, float, double, NT>), g, b, 0, s, a.P, bp, a.nd);
+        } else {
+            if (a.dim == 2) hipLaunchKernelGGL((assemble_batch<2, double, double, NT>), g, b, 0, s, a.P, bp, a.nd);
+            else hipLaunchKernelGGL((assemble_batch<3, double, double, NT>), g, b, 0, s, a.P, bp, a.nd);
+        }
+    } else {  // 2D with three waves: no idle wave
+        const dim3 b(192);
+        if (a.mixed) hipLaunchKernelGGL((assemble_batch<2, float, double, 192>), g, b, 0, s, a.P, bp, a.nd);
+        else hipLaunchKernelGGL((assemble_batch<2, double, double, 192>), g, b, 0, s, a.P, bp, a.nd);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s) {
+    if (a.tot[2] <= 0) return hipSuccess;
+    const dim3 g = batch_grid(grid, a.tot[2]), b(GW);
+    if (a.dim == 2) hipLaunchKernelGGL((gather_batch<2, double>), g, b, 0, s, a.P, a.bptr[2], a.nd);
+    else hipLaunchKernelGGL((gather_batch<3, double>), g, b, 0, s, a.P, a.bptr[2], a.nd);
     return hipGetLastError();
 }
 

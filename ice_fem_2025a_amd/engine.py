@@ -268,3 +268,57 @@ class Physics:
 
     def reset_step(self, model, stream=None):
         abi.check(self.L.ife_model_reset_step(self.h, model, stream.handle if stream else None))
+
+
+class Batch:
+    """Independent domains assembled together (ife_batch_*): one batched launch per kernel
+    over all domains, optionally captured in a hipGraph and replayed per Newton iteration.
+    The Physics objects must stay alive while the batch is used."""
+
+    def __init__(self, domains):
+        self.L = abi.load_library()
+        self.domains = list(domains)
+        arr = (C.c_void_p * len(self.domains))(*[d.h.value for d in self.domains])
+        h = C.c_void_p()
+        abi.check(self.L.ife_batch_create(arr, len(self.domains), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ife_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def update(self):
+        abi.check(self.L.ife_batch_update(self.h))
+
+    def assemble(self, stream=None):
+        abi.check(self.L.ife_batch_assemble(self.h, stream.handle if stream else None))
+
+    def capture(self):
+        abi.check(self.L.ife_batch_capture(self.h))
+
+    def launch(self, stream=None):
+        abi.check(self.L.ife_batch_launch(self.h, stream.handle if stream else None))
+
+    @staticmethod
+    def _todict(t):
+        return {k: getattr(t, k) for k, _ in abi.BatchTuning._fields_}
+
+    def tune(self, stream=None):
+        t = abi.BatchTuning()
+        abi.check(self.L.ife_batch_tune(self.h, stream.handle if stream else None, C.byref(t)))
+        return self._todict(t)
+
+    def shape(self):
+        t = abi.BatchTuning()
+        abi.check(self.L.ife_batch_get_shape(self.h, C.byref(t)))
+        return self._todict(t)
+
+    def set_shape(self, grid_material=0, grid_assemble=0, grid_gather=0, lanes_assemble=256):
+        t = abi.BatchTuning()
+        t.grid_material, t.grid_assemble, t.grid_gather = grid_material, grid_assemble, grid_gather
+        t.lanes_assemble = lanes_assemble
+        abi.check(self.L.ife_batch_set_shape(self.h, C.byref(t)))

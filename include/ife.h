@@ -185,6 +185,25 @@ typedef struct {
 
 typedef struct ife_physics ife_physics;
 
+/* A batch of independent domains assembled together (see ife_batch_create). */
+typedef struct ife_batch ife_batch;
+
+/* Launch shape of a batch and what the autotuner measured.  grid_* = workgroups per
+ * launch of the material, element and gather kernels (0: one workgroup per block of
+ * work; otherwise persistent workgroups that stride over the blocks of all domains);
+ * lanes_assemble = lanes per element workgroup (256, or 192 in 2D: one wavefront per
+ * role, no idle wavefront). */
+typedef struct {
+    int64_t grid_material, grid_assemble, grid_gather;
+    int32_t lanes_assemble;
+    int32_t candidates_timed;    /* shapes the tuner timed */
+    int32_t n_domains;
+    int64_t blocks_material;     /* work blocks of the three kernels over the batch */
+    int64_t elements;
+    int64_t nodes;
+    double ms_material, ms_assemble, ms_gather;  /* tuner: median time of the chosen shapes */
+} ife_batch_tuning;
+
 /* ---- life cycle ---------------------------------------------------------------- */
 
 /* Physics::Physics (physics.cpp:15-46) with the mesh already loaded and DofSpace
@@ -330,6 +349,42 @@ int ife_model_set_element_params(ife_physics* ph, int32_t model, const double* G
  * function that the reference drops, SolidMaterial.cpp:289).  A negative tolerance or
  * non-positive iteration count restores the reference's value. */
 int ife_model_set_param(ife_physics* ph, int32_t model, const char* name, double value);
+
+/* ---- batches of independent domains (no counterpart in the reference, which runs one
+ * mesh per process) ------------------------------------------------------------------
+ * Every domain is a set-up ife_physics (same dimension, precision and return map).  One
+ * batched assembly runs, for all domains, the work of ife_physics_assemble_all: four
+ * launches (status reset, material, element, gather kernels) whose workgroups map to
+ * (domain, block) through prefix sums of the domains' block counts.  ife_batch_capture
+ * records them into one hipGraph, replayed by ife_batch_launch once per Newton
+ * iteration.  The domains must outlive the batch; each domain's results land in its own
+ * engine-owned K/f, status and history, exactly as ife_physics_assemble_all writes them. */
+int ife_batch_create(ife_physics* const* domains, int32_t n_domains, ife_batch** out);
+
+/* Re-read the domains' time, parameters and precision (after ife_physics_set_time,
+ * ife_model_set_param, ...).  Drops a captured graph. */
+int ife_batch_update(ife_batch* b);
+
+/* One batched assembly, launched directly on `stream`. */
+int ife_batch_assemble(ife_batch* b, void* stream);
+
+/* Capture one batched assembly into the batch's hipGraph (replacing a previous one). */
+int ife_batch_capture(ife_batch* b);
+
+/* Replay the captured graph on `stream` (captures first if needed). */
+int ife_batch_launch(ife_batch* b, void* stream);
+
+/* Autotuner: times every candidate launch shape of each kernel on the current inputs
+ * (grid = one workgroup per block, or 256, 512, ... persistent workgroups; element
+ * kernel with 256 or, in 2D, 192 lanes) and keeps the fastest.  Runs assemblies on
+ * `stream` and synchronises it; drops a captured graph. */
+int ife_batch_tune(ife_batch* b, void* stream, ife_batch_tuning* out);
+
+/* Set / read the launch shape explicitly. */
+int ife_batch_set_shape(ife_batch* b, const ife_batch_tuning* shape);
+int ife_batch_get_shape(const ife_batch* b, ife_batch_tuning* out);
+
+void ife_batch_destroy(ife_batch* b);
 
 const char* ife_last_error(void);
 int ife_api_version(void);

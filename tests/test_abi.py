@@ -175,3 +175,25 @@ def test_adjoint_workload_element_params_are_seeded():
     assert A.min() >= 1e-25 and A.max() <= 1e-23
     again = make_problem("hex3d_adjoint", nel=(2, 2, 2))
     assert again.elem_params[0]["Gc"].shape == (8,)
+
+
+def test_batch_argument_checks():
+    lib = abi.load_library()
+    out = C.c_void_p()
+    assert lib.ife_batch_create(None, 0, C.byref(out)) == 1
+    e = Physics(make_problem("quad2d_thermal", nel=(2, 2)), device=False)
+    arr = (C.c_void_p * 1)(e.h.value)
+    assert lib.ife_batch_create(arr, 1, C.byref(out)) == 5  # domains must be set up
+    assert lib.ife_batch_assemble(None, None) == 1
+    assert lib.ife_batch_tune(None, None, None) == 1
+
+
+def test_batch_workload_sizes_are_seeded():
+    from ice_fem_2025a_amd.config import batch_sizes
+    sz = batch_sizes()
+    assert len(sz) == 64
+    nel = np.array([x * y for x, y in sz])
+    ratio = np.array([x / y for x, y in sz])
+    assert nel.min() >= 1900 and nel.max() <= 62000 and 0.9e6 < nel.sum() < 1.1e6
+    assert ratio.min() >= 0.8 and ratio.max() <= 4.5
+    assert batch_sizes() == sz
