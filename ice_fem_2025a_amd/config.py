@@ -11,6 +11,10 @@ Workloads (the BASELINE configurations, named by what they are):
   "hex3d_vep_at2"   16x16x16 hexes, as hex3d_glen_at2 with the visco-elasto-plastic rheology
   "quad2d_thermal"  one domain of the batched 2D workload: as quad2d_le_at2 plus T and
                     Thermal/ThermalModel (u(2)+phi+T); make_batch draws 64 of them
+  "hex3d_coss"      200x50x60 hexes (0.5 x 0.125 x 0.15 m), Fracture/CossPhaseField: u(3) +
+                    micro-rotations w(3) + phi + T, damaged visco-elasto-plastic Cosserat ice
+                    (TriAxial.json material, Density 917, TOffset 0), SpecStress energy,
+                    gravity, viscous heating, thermal model
   "hex3d_adjoint"   32x32x32 hexes on a 0.08 m cube, as hex3d_glen_at2, with per-element Gc
                     log-normal(0, 0.3) J/m^2 and Glen A log-uniform in [1e-25, 1e-23] Pa^-3 s^-1
                     (the adjoint-sensitivity workload; cotangents from make_cotangent)
@@ -95,6 +99,37 @@ WORKLOADS["hex3d_vep_at2"]["properties"]["Ice"]["SolidProperties"].update(
      "pl_visc": 1.0e9})
 
 
+# Cosserat workload: the material block of TestCases/TriAxialCompression/TriAxial.json
+# (:28-63) with Density 917 and TOffset 0 (T in K, as the other 3D workloads), Irreversible
+# Hist, gravity and viscous heating on, the AT2 crack density of the other workloads.
+WORKLOADS["hex3d_coss"] = {
+    "index": 4,
+    "mesh": {"dim": 3, "ipcount1D": 3, "nel": [200, 50, 60], "lengths": [0.5, 0.125, 0.15]},
+    "properties": {
+        "PhaseField": {"DamFunction": "Quadratic", "DensityDamage": "Step2", "GravityDamage": "Step2",
+                       "ViscoDriving": "None", "ViscoDrivingFactor": 0.0, "ConsistentEnergy": False,
+                       "ConsistentDegradation": False, "DistributionFunction": "Quadratic", "k0": 1.0e-12},
+        "Ice": {
+            "TemperatureDependent": {"TemperatureDependent": True, "TRef": TREF, "TOffset": 0.0, "QCreep": 150.0e3,
+                                     "ft0": 0.2e6, "ftdeg": 4.0e4, "sydeg": 0.5e6},
+            "SolidProperties": {"Rheology": "DamViscoElasticPlastic", "Density": 917.0, "Young": 5.0e9, "Poisson": 0.35,
+                                "G_coss": 1.8e9, "l_coss": 2.5e-3, "Damping": 1.0e6, "A": 1.0e-24, "PhiViscLim": 0.2,
+                                "n": 3.0, "sy": 9.5e6, "hardening": -0.9, "hard_eRef": 2.0e-4, "pl_visc": 1.0e3},
+            "FractureProperties": {"Split": "SpecStress", "l": 1.5e-3, "Gc": 4.0, "Viscosity": 1.0e-1},
+            "ThermalProperties": {"ThermalConductivity": 2.0, "HeatCapacity": 2.1e3},
+        },
+        "TimeDiscretisation": {"Scheme": "BDF2"},
+    },
+    "Dofs": {"DofNames": ["ux", "uy", "uz", "ox", "oy", "oz", "phase", "T"], "DofStep": [0, 0, 0, 0, 0, 0, 1, 2]},
+    "Models": {"Names": ["PhaseField1", "Thermal"],
+               "PhaseField1": {"Name": "Fracture/CossPhaseField", "ElementGroup_u": "internal",
+                               "ElementGroup_phase": "internal", "ElementGroup_T": "internal", "Material": "Ice",
+                               "ViscousStab": 0.0, "Irreversible": "Hist", "Intertia": False, "Gravity": True,
+                               "LumpedPFCapacity": False, "LumpedDriving": False, "ViscousHeating": True},
+               "Thermal": {"Name": "Thermal/ThermalModel", "ElementGroup_T": "internal", "Material": "Ice"}},
+    "dt": 1.0,
+}
+
 # One domain of the batched 2D workload: the 2D LE/AT2 physics plus temperature and the
 # thermal model (dofs u(2)+phi+T).
 WORKLOADS["quad2d_thermal"] = copy.deepcopy(WORKLOADS["quad2d_le_at2"])
@@ -157,15 +192,16 @@ def material_struct(inputs, mat_name):
     return s
 
 
-def phasefield_struct(inputs):
-    """properties.PhaseField.* -> ife_phasefield_opts (PhaseFieldUtil.cpp:4-39, PhaseField.cpp:196-267)."""
+def phasefield_struct(inputs, cosserat=False):
+    """properties.PhaseField.* -> ife_phasefield_opts (PhaseFieldUtil.cpp:4-39, PhaseField.cpp:196-267,
+    CossPhaseField.cpp:246-266)."""
     pf = inputs["properties"].get("PhaseField", {})
     s = abi.PhaseFieldOpts()
     s.DamFunction = abi.DAMAGE[pf.get("DamFunction", "Quadratic")]
-    # the reference reads "DensityDamage" for both the inertia and the gravity
-    # degradation (PhaseField.cpp:240-254)
+    # Fracture/PhaseField reads "DensityDamage" for both the inertia and the gravity
+    # degradation (PhaseField.cpp:240-254); Fracture/CossPhaseField reads "GravityDamage"
     s.DensityDamage = abi.DAMAGE[pf.get("DensityDamage", "None")]
-    s.GravityDamage = abi.DAMAGE[pf.get("DensityDamage", "None")]
+    s.GravityDamage = abi.DAMAGE[pf.get("GravityDamage" if cosserat else "DensityDamage", "None")]
     s.ViscoDriving = abi.DAMAGE[pf.get("ViscoDriving", "None")]
     s.ViscoDrivingFactor = pf.get("ViscoDrivingFactor", 0.0)
     s.k0, s.p = pf.get("k0", 0.0), pf.get("p", 2.0)
@@ -245,8 +281,9 @@ def build_models(inputs):
         md = inputs["Models"][nm]
         tn = md["Name"]
         mat = material_struct(inputs, md["Material"])
-        if tn == "Fracture/PhaseField":
-            out.append(ModelSpec(tn, nm, mat, phasefield_struct(inputs), model_opts_struct(inputs, nm)))
+        if tn in ("Fracture/PhaseField", "Fracture/CossPhaseField"):
+            out.append(ModelSpec(tn, nm, mat, phasefield_struct(inputs, tn == "Fracture/CossPhaseField"),
+                                 model_opts_struct(inputs, nm)))
         elif tn == "Thermal/ThermalModel":
             out.append(ModelSpec(tn, nm, mat, None, None))
         else:
@@ -291,12 +328,19 @@ def make_problem(name, nel=None, seed=None, h=None):
             prob.node_state[(abi.IFE_STATE, c)] = uold + du
             prob.node_state[(abi.IFE_STATE_OLD, c)] = uold
             prob.node_state[(abi.IFE_DSTATE, c)] = du / dt
-        prob.node_state[(abi.IFE_STATE, "phase")] = rng.uniform(0.0, 0.9, NN)
+        # the Cosserat workload's PhiViscLim is 0.2: phi ~ U[0, 0.4) there, so that about half
+        # of the IPs run the visco-plastic map (U[0, 0.9) elsewhere)
+        phi_hi = 0.4 if "ox" in prob.dof_names else 0.9
+        prob.node_state[(abi.IFE_STATE, "phase")] = rng.uniform(0.0, phi_hi, NN)
         prob.node_state[(abi.IFE_STATE_OLD, "phase")] = prob.node_state[(abi.IFE_STATE, "phase")].copy()
         prob.node_state[(abi.IFE_STATE, "T")] = rng.uniform(250.0, 273.0, NN)
         prob.node_state[(abi.IFE_DSTATE, "T")] = rng.uniform(-1e-2, 1e-2, NN)
         prob.history[(0, "EnergyOld")] = rng.uniform(0.0, 5.0e3, (1, E * nip))
         prob.history[(0, "Energy")] = prob.history[(0, "EnergyOld")].copy()
+        if "ox" in prob.dof_names:  # micro-rotations: log-uniform magnitudes in [1e-6, 1e-3] rad
+            for c in ("ox", "oy", "oz"):
+                mag = np.exp(rng.uniform(np.log(1e-6), np.log(1e-3), NN))
+                prob.node_state[(abi.IFE_STATE, c)] = mag * rng.choice([-1.0, 1.0], NN)
     if name == "hex3d_adjoint":
         prob.elem_params[0] = {"Gc": np.exp(rng.normal(0.0, 0.3, E)),
                                "A": np.exp(rng.uniform(np.log(1e-25), np.log(1e-23), E))}

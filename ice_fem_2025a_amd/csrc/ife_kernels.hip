@@ -1429,9 +1429,16 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
     }
 }
 
+// Batch tables are read through the constant address space: the loads are then known to
+// be invariant, and with uniform addresses they become scalar loads into SGPRs, as the
+// single-domain kernels' by-value KParams are.  (Through a generic pointer the compiler
+// keeps the parameters in VGPRs and spills.)
+typedef const KParams __attribute__((address_space(4))) KParamsC;
+typedef const int64_t __attribute__((address_space(4))) I64C;
+
 // The domain of flattened block b of a batch: the last d with bptr[d] <= b (bptr ascending,
 // bptr[0] = 0, bptr[nd] = total).  Every lane computes the same value.
-__device__ __forceinline__ int batch_domain(const int64_t* __restrict__ bptr, int nd, int64_t b) {
+__device__ __forceinline__ int batch_domain(I64C* bptr, int nd, int64_t b) {
     int lo = 0, hi = nd;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -1446,17 +1453,26 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
     material_body<DIM, VEP, EXACT, ADJ, FULL>(p, blockIdx.x);
 }
 
-// Batched form: the blocks of all domains in one launch.  gridDim.x may be below the total
-// (persistent workgroups striding over the blocks); the barrier at the top of each item
-// keeps the next item's LDS staging behind the previous item's last LDS reads.
-template <int DIM, bool VEP, bool EXACT, bool FULL>
-__global__ __launch_bounds__(NT, 2) void material_batch(const KParams* __restrict__ P, const int64_t* __restrict__ bptr,
-                                                        int nd) {
-    const int64_t total = bptr[nd];
-    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+// Batched form: the blocks of all domains in one launch.  PERSIST = false: one workgroup per
+// block (gridDim.x = the total).  PERSIST = true: gridDim.x persistent workgroups stride
+// over the blocks, with a barrier at the top of each item that keeps the next item's LDS
+// staging behind the previous item's last LDS reads.  (The loop costs registers, so both
+// forms exist and the launch-shape tuner picks.)
+template <int DIM, bool VEP, bool EXACT, bool FULL, bool PERSIST>
+__global__ __launch_bounds__(NT, 2) void material_batch(const KParams* P_, const int64_t* bptr_, int nd) {
+    KParamsC* P = (KParamsC*)P_;
+    I64C* bptr = (I64C*)bptr_;
+    if constexpr (!PERSIST) {
+        const int64_t b = blockIdx.x;
         const int d = batch_domain(bptr, nd, b);
-        __syncthreads();
-        material_body<DIM, VEP, EXACT, false, FULL>(P[d], b - bptr[d]);
+        material_body<DIM, VEP, EXACT, false, FULL>((const KParams&)P[d], b - bptr[d]);
+    } else {
+        const int64_t total = bptr[nd];
+        for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+            const int d = batch_domain(bptr, nd, b);
+            __syncthreads();
+            material_body<DIM, VEP, EXACT, false, FULL>((const KParams&)P[d], b - bptr[d]);
+        }
     }
 }
 
@@ -1865,14 +1881,22 @@ __global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_kernel(K
     assemble_body<DIM, T, R, NTA>(p, blockIdx.x);
 }
 
-template <int DIM, typename T, typename R, int NTA>
-__global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_batch(const KParams* __restrict__ P,
-                                                                               const int64_t* __restrict__ bptr, int nd) {
-    const int64_t total = bptr[nd];
-    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+template <int DIM, typename T, typename R, int NTA, bool PERSIST>
+__global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_batch(const KParams* P_, const int64_t* bptr_,
+                                                                               int nd) {
+    KParamsC* P = (KParamsC*)P_;
+    I64C* bptr = (I64C*)bptr_;
+    if constexpr (!PERSIST) {
+        const int64_t b = blockIdx.x;
         const int d = batch_domain(bptr, nd, b);
-        __syncthreads();
-        assemble_body<DIM, T, R, NTA>(P[d], b - bptr[d]);
+        assemble_body<DIM, T, R, NTA>((const KParams&)P[d], b - bptr[d]);
+    } else {
+        const int64_t total = bptr[nd];
+        for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+            const int d = batch_domain(bptr, nd, b);
+            __syncthreads();
+            assemble_body<DIM, T, R, NTA>((const KParams&)P[d], b - bptr[d]);
+        }
     }
 }
 
@@ -2016,14 +2040,21 @@ __global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
     gather_body<DIM, T>(p, blockIdx.x);
 }
 
-template <int DIM, typename T>
-__global__ __launch_bounds__(GW) void gather_batch(const KParams* __restrict__ P, const int64_t* __restrict__ bptr,
-                                                   int nd) {
-    const int64_t total = bptr[nd];
-    for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+template <int DIM, typename T, bool PERSIST>
+__global__ __launch_bounds__(GW) void gather_batch(const KParams* P_, const int64_t* bptr_, int nd) {
+    KParamsC* P = (KParamsC*)P_;
+    I64C* bptr = (I64C*)bptr_;
+    if constexpr (!PERSIST) {
+        const int64_t b = blockIdx.x;
         const int d = batch_domain(bptr, nd, b);
-        __syncthreads();
-        gather_body<DIM, T>(P[d], b - bptr[d]);
+        gather_body<DIM, T>((const KParams&)P[d], b - bptr[d]);
+    } else {
+        const int64_t total = bptr[nd];
+        for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+            const int d = batch_domain(bptr, nd, b);
+            __syncthreads();
+            gather_body<DIM, T>((const KParams&)P[d], b - bptr[d]);
+        }
     }
 }
 
@@ -2164,7 +2195,8 @@ int blocks_material(int dim, int64_t n_elems) {
 }
 
 namespace {
-dim3 batch_grid(int64_t grid, int64_t total) { return dim3((unsigned)(grid > 0 && grid < total ? grid : total)); }
+bool persistent(int64_t grid, int64_t total) { return grid > 0 && grid < total; }
+dim3 batch_grid(int64_t grid, int64_t total) { return dim3((unsigned)(persistent(grid, total) ? grid : total)); }
 }  // namespace
 
 hipError_t launch_batch_zero(const BatchArgs& a, hipStream_t s) {
@@ -2177,7 +2209,12 @@ hipError_t launch_batch_material(const BatchArgs& a, int64_t grid, hipStream_t s
     if (a.tot[0] <= 0) return hipSuccess;
     const dim3 g = batch_grid(grid, a.tot[0]), b(NT);
     const int64_t* bp = a.bptr[0];
-#define IFE_MB(D, V, X, F) hipLaunchKernelGGL((material_batch<D, V, X, F>), g, b, 0, s, a.P, bp, a.nd)
+    const bool ps = persistent(grid, a.tot[0]);
+#define IFE_MB(D, V, X, F)                                                                          \
+    do {                                                                                            \
+        if (ps) hipLaunchKernelGGL((material_batch<D, V, X, F, true>), g, b, 0, s, a.P, bp, a.nd);  \
+        else hipLaunchKernelGGL((material_batch<D, V, X, F, false>), g, b, 0, s, a.P, bp, a.nd);    \
+    } while (0)
     if (a.vep) { if (a.dim == 2) IFE_MB(2, true, false, true); else IFE_MB(3, true, false, true); }
     else if (a.exact) { if (a.dim == 2) IFE_MB(2, false, true, true); else IFE_MB(3, false, true, true); }
     else if (a.full) { if (a.dim == 2) IFE_MB(2, false, false, true); else IFE_MB(3, false, false, true); }
@@ -2190,28 +2227,34 @@ hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipS
     if (a.tot[1] <= 0) return hipSuccess;
     const dim3 g = batch_grid(grid, a.tot[1]);
     const int64_t* bp = a.bptr[1];
+    const bool ps = persistent(grid, a.tot[1]);
+#define IFE_AB(D, T, L)                                                                               \
+    do {                                                                                              \
+        if (ps) hipLaunchKernelGGL((assemble_batch<D, T, double, L, true>), g, dim3(L), 0, s, a.P, bp, a.nd); \
+        else hipLaunchKernelGGL((assemble_batch<D, T, double, L, false>), g, dim3(L), 0, s, a.P, bp, a.nd);  \
+    } while (0)
     if (a.dim == 3 || nta != 192) {
-        const dim3 b(NT);
-        if (a.mixed) {
-            if (a.dim == 2) hipLaunchKernelGGL((assemble_batch<2, float, double, NT>), g, b, 0, s, a.P, bp, a.nd);
-            else hipLaunchKernelGGL((assemble_batch<3, float, double, NT>), g, b, 0, s, a.P, bp, a.nd);
-        } else {
-            if (a.dim == 2) hipLaunchKernelGGL((assemble_batch<2, double, double, NT>), g, b, 0, s, a.P, bp, a.nd);
-            else hipLaunchKernelGGL((assemble_batch<3, double, double, NT>), g, b, 0, s, a.P, bp, a.nd);
-        }
+        if (a.mixed) { if (a.dim == 2) IFE_AB(2, float, NT); else IFE_AB(3, float, NT); }
+        else { if (a.dim == 2) IFE_AB(2, double, NT); else IFE_AB(3, double, NT); }
     } else {  // 2D with three waves: no idle wave
-        const dim3 b(192);
-        if (a.mixed) hipLaunchKernelGGL((assemble_batch<2, float, double, 192>), g, b, 0, s, a.P, bp, a.nd);
-        else hipLaunchKernelGGL((assemble_batch<2, double, double, 192>), g, b, 0, s, a.P, bp, a.nd);
+        if (a.mixed) IFE_AB(2, float, 192);
+        else IFE_AB(2, double, 192);
     }
+#undef IFE_AB
     return hipGetLastError();
 }
 
 hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s) {
     if (a.tot[2] <= 0) return hipSuccess;
     const dim3 g = batch_grid(grid, a.tot[2]), b(GW);
-    if (a.dim == 2) hipLaunchKernelGGL((gather_batch<2, double>), g, b, 0, s, a.P, a.bptr[2], a.nd);
-    else hipLaunchKernelGGL((gather_batch<3, double>), g, b, 0, s, a.P, a.bptr[2], a.nd);
+    const bool ps = persistent(grid, a.tot[2]);
+    if (a.dim == 2) {
+        if (ps) hipLaunchKernelGGL((gather_batch<2, double, true>), g, b, 0, s, a.P, a.bptr[2], a.nd);
+        else hipLaunchKernelGGL((gather_batch<2, double, false>), g, b, 0, s, a.P, a.bptr[2], a.nd);
+    } else {
+        if (ps) hipLaunchKernelGGL((gather_batch<3, double, true>), g, b, 0, s, a.P, a.bptr[2], a.nd);
+        else hipLaunchKernelGGL((gather_batch<3, double, false>), g, b, 0, s, a.P, a.bptr[2], a.nd);
+    }
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cfloat>
+#include <limits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -354,6 +355,12 @@ struct Material {
     const double R = 8.31446261815324;  // BaseMaterial.h:26
     // ThermalMaterial.cpp:5-12
     double k, cp;
+    // CosseratMaterial.cpp:3-103 (18 components: xx yy zz xy yx yz zy xz zx strains, then
+    // the nine curvatures); D_coss = dam * D_coss0 (IceMaterial.cpp:66)
+    double l_coss = 0, G_coss = 0, G_coss0 = 0, Hardening_e_ref = 0;
+    double Dc[18][18], Dc0[18][18], J2c[18][18];
+    static constexpr double coss_sc = 1.0e6, coss_scale1 = 1.0e3, coss_scale2 = -1.0e-6, coss_scale3 = -1.0e-6,
+                            coss_eps = 1.0e-12;  // CosseratMaterial.h:53-58
     // PF utility used by dVE (SolidMaterial.cpp:35-46)
     int DegFunction;
     double k0, p;
@@ -427,7 +434,45 @@ struct Material {
         std::memcpy(D0, D, sizeof(D));
         Young0 = Young;
         k = m.ThermalConductivity; cp = m.HeatCapacity;
+        init_coss(m);
         SetTemperature(TRef, 0.0);  // IceMaterial.cpp:43
+    }
+
+    // CosseratMaterial::CosseratMaterial (CosseratMaterial.cpp:3-103), with the Lame and
+    // Shear moduli of the input Young's modulus
+    void init_coss(const ife_material& m) {
+        l_coss = m.l_coss;
+        G_coss = m.G_coss;
+        G_coss0 = G_coss;
+        Hardening_e_ref = m.hard_eRef;
+        for (int i = 0; i < 18; i++)
+            for (int j = 0; j < 18; j++) { Dc0[i][j] = 0; J2c[i][j] = 0; }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                Dc0[i][j] = Lame + (i == j ? 2.0 * Shear : 0.0);
+                J2c[i][j] = i == j ? 2.0 / 3.0 : -1.0 / 3.0;
+            }
+        const double a1 = 0.25, a2 = 0.25, a3 = 0.5;
+        for (int k = 0; k < 3; k++) {  // the pairs (3,4), (5,6), (7,8)
+            const int i = 3 + 2 * k, j = 4 + 2 * k;
+            Dc0[i][i] = Shear + G_coss; Dc0[j][j] = Shear + G_coss;
+            Dc0[i][j] = Shear - G_coss; Dc0[j][i] = Shear - G_coss;
+        }
+        // J2Mat_Coss couples 3-6, 4-7, 5-8 (CosseratMaterial.cpp:46-58)
+        for (int k = 0; k < 3; k++) {
+            const int i = 3 + k, j = 6 + k;
+            J2c[i][i] = 2 * a1; J2c[j][j] = 2 * a1;
+            J2c[i][j] = 2 * a2; J2c[j][i] = 2 * a2;
+        }
+        for (int i = 9; i < 18; i++) { Dc0[i][i] = 2.0 * Shear; J2c[i][i] = 2 * a3; }
+        std::memcpy(Dc, Dc0, sizeof(Dc));
+    }
+    void Dc_mul(const double* x, double Damage, double* out) const {  // (Damage*D_coss)*x
+        for (int i = 0; i < 18; i++) {
+            double t = 0;
+            for (int j = 0; j < 18; j++) t += (Damage * Dc[i][j]) * x[j];
+            out[i] = t;
+        }
     }
 
     // IceMaterial::SetTemperature (IceMaterial.cpp:49-84)
@@ -441,6 +486,9 @@ struct Material {
         Bulk = Lame + 2.0 / 3.0 * Shear;
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 6; j++) D[i][j] = dam * D0[i][j];
+        G_coss = G_coss0 * dam;
+        for (int i = 0; i < 18; i++)
+            for (int j = 0; j < 18; j++) Dc[i][j] = dam * Dc0[i][j];
         double ExpTerm = -QCreep / R * (1 / TAbs - 1 / TRef);
         ACreep = A0 * std::exp(ExpTerm);
         double ft = dam * (ft0 - ftdeg * (TAbs - TRef));
@@ -1194,6 +1242,267 @@ struct Material {
         }
         return Energy;
     }
+
+    // ---- Cosserat material (CosseratMaterial.cpp) ---------------------------------------
+    typedef double C18[18];
+    typedef double C18x18[18][18];
+
+    // CosseratMaterial::SpecStressSplit energy (:371-442): the positive part of the
+    // symmetrised macro stress (entries below 1e-9 in magnitude replaced by (i+1) 1e-9)
+    // against the macro strain, clamped at zero
+    double CossSpecStressEnergy(const double* Strain) const {
+        C18 Stress;
+        Dc_mul(Strain, 1.0, Stress);
+        double sn[9];
+        for (int i = 0; i < 9; i++) {
+            sn[i] = Stress[i];
+            if (std::fabs(sn[i]) < 1.0e-9) sn[i] = (i + 1) * 1.0e-9;
+        }
+        double SM[3][3] = {{sn[0], sn[3], sn[7]}, {sn[4], sn[1], sn[5]}, {sn[8], sn[6], sn[2]}};
+        double A[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) A[i][j] = 0.5 * (SM[i][j] + SM[j][i]);
+        double lam[3], V[3][3];
+        eig3(A, lam, V);
+        double P[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double t = 0;
+                for (int k = 0; k < 3; k++)
+                    if (lam[k] > 0) t += V[i][k] * lam[k] * V[j][k];
+                P[i][j] = t;
+            }
+        const double sv[9] = {P[0][0], P[1][1], P[2][2], P[0][1], P[1][0], P[1][2], P[2][1], P[0][2], P[2][0]};
+        double E = 0;
+        for (int i = 0; i < 9; i++) E += sv[i] * Strain[i];
+        E *= 0.5;
+        return std::max(0.0, E);
+    }
+
+    // CosseratMaterial::EnergySplit (:109-152): NoSplit and SpecStress (consistent energy
+    // off); the stress and tangent are D_coss eps and D_coss whenever the energy is not
+    // consistent
+    double CossEnergySplit(C18x18 D_Dam, C18x18 D_UnDam, C18 S_Dam, C18 S_Undam, const double* Strain) const {
+        double Energy = 0;
+        for (int i = 0; i < 18; i++) {
+            S_Dam[i] = 0; S_Undam[i] = 0;
+            for (int j = 0; j < 18; j++) { D_Dam[i][j] = 0; D_UnDam[i][j] = 0; }
+        }
+        if (StressSplit == IFE_SPLIT_NONE) {
+            for (int i = 0; i < 18; i++)
+                for (int j = 0; j < 18; j++) D_Dam[i][j] = Dc[i][j];
+            Dc_mul(Strain, 1.0, S_Dam);
+            for (int i = 0; i < 18; i++) Energy += S_Dam[i] * Strain[i];
+            Energy *= 0.5;
+        } else if (StressSplit == IFE_SPLIT_SPECSTRESS) {
+            if (ConsistentEnergy) throw std::invalid_argument("Cosserat SpecStress split with a consistent energy not supported");
+            Energy = CossSpecStressEnergy(Strain);
+        } else {
+            throw std::invalid_argument("Cosserat energy split not supported");
+        }
+        if (ConsistentEnergy == false) {
+            Dc_mul(Strain, 1.0, S_Dam);
+            for (int i = 0; i < 18; i++) {
+                S_Undam[i] = 0;
+                for (int j = 0; j < 18; j++) { D_Dam[i][j] = Dc[i][j]; D_UnDam[i][j] = 0; }
+            }
+        }
+        return Energy;
+    }
+
+    // J2 of the scaled stress with J2Mat_Coss: returns J2, fills dJ2
+    double coss_J2(const double* Stress, double* dJ2) const {
+        double s[18];
+        for (int i = 0; i < 18; i++) s[i] = Stress[i] / coss_sc;
+        double J2 = 0;
+        for (int i = 0; i < 18; i++) {
+            double t = 0;
+            for (int j = 0; j < 18; j++) t += J2c[i][j] * s[j];
+            dJ2[i] = t;
+            J2 += s[i] * t;
+        }
+        return 0.5 * J2;
+    }
+    // d2 of sqrt(k J2) type functions: c/sc (J2Mat - 0.5/(J2+eps) dJ2 dJ2^T), c = 0.5 sqrt(3)/sqrt(J2+eps)
+    void coss_dd(double J2, const double* dJ2, C18x18 dd) const {
+        const double c = 0.5 / coss_sc * std::sqrt(3.0) / std::sqrt(J2 + coss_eps);
+        for (int i = 0; i < 18; i++)
+            for (int j = 0; j < 18; j++) dd[i][j] = c * (J2c[i][j] - 0.5 / (J2 + coss_eps) * dJ2[i] * dJ2[j]);
+    }
+    // YieldFunction (:771-799, apex branch disabled): F = sc sqrt(3 J2) - sy
+    double CossYield(const double* Stress, double* dF, C18x18 ddF) const {
+        double dJ2[18];
+        double J2 = coss_J2(Stress, dJ2);
+        double F = coss_sc * std::sqrt(3.0 * J2) - sy;
+        for (int i = 0; i < 18; i++) dF[i] = 0.5 * std::sqrt(3.0) / std::sqrt(J2 + coss_eps) * dJ2[i];
+        coss_dd(J2, dJ2, ddF);
+        return F;
+    }
+    // YieldFunctionVE (:801-814): F = sc sqrt(2 J2), dF with sqrt(2), ddF with sqrt(3) as written
+    double CossYieldVE(const double* Stress, double* dF, C18x18 ddF) const {
+        double dJ2[18];
+        double J2 = coss_J2(Stress, dJ2);
+        double F = coss_sc * std::sqrt(2.0 * J2);
+        for (int i = 0; i < 18; i++) dF[i] = 0.5 * std::sqrt(2.0) / std::sqrt(J2 + coss_eps) * dJ2[i];
+        coss_dd(J2, dJ2, ddF);
+        return F;
+    }
+    // PotentialFunction (:816-845, apex branch disabled)
+    void CossPotential(const double* Stress, double* dG, C18x18 ddG) const {
+        double dJ2[18];
+        double J2 = coss_J2(Stress, dJ2);
+        for (int i = 0; i < 18; i++) dG[i] = 0.5 * std::sqrt(3.0) / std::sqrt(J2 + coss_eps) * dJ2[i];
+        coss_dd(J2, dJ2, ddG);
+    }
+
+    // return_mapping_VEP_getKF (:671-728), row-major 20x20 K.  Explicit hardening.
+    void CossGetKF(double* K, double* f, const double* sol, const double* strain_total, const double* strain_vep_old, double dt,
+                   double Damage, const double* histOld) const {
+        for (int i = 0; i < 400; i++) K[i] = 0;
+        for (int i = 0; i < 20; i++) f[i] = 0;
+        C18 Stress, dF, dG, dF_VE, StressTrial, tmp;
+        C18x18 ddF, ddG, ddF_VE;
+        Dc_mul(sol, Damage, Stress);
+        double F_VE = CossYieldVE(Stress, dF_VE, ddF_VE);
+        for (int i = 0; i < 18; i++) tmp[i] = strain_total[i] - strain_vep_old[i] - sol[18] * dF_VE[i];
+        Dc_mul(tmp, Damage, StressTrial);
+        double FTrial = CossYield(StressTrial, dF, ddF);
+        double F = CossYield(Stress, dF, ddF);
+        CossPotential(Stress, dG, ddG);
+        const double include_dh = 0.0;  // ExplicitHardening = true
+        const double e_eq = histOld[0] + include_dh * std::sqrt(sol[19] * sol[19]);
+        const double h = hardening * sy * (1.0 - std::exp(-e_eq * l_coss / Hardening_e_ref));
+        const double hOld = hardening * sy * (1.0 - std::exp(-histOld[0] * l_coss / Hardening_e_ref));
+        const double dh = include_dh * hardening * sy / Hardening_e_ref * std::exp(-e_eq * l_coss / Hardening_e_ref) * sgn(sol[19]);
+        const double s1 = coss_scale1, s2 = coss_scale2, s3 = coss_scale3;
+        for (int i = 0; i < 18; i++)
+            f[i] = s1 * (sol[i] - strain_total[i] + strain_vep_old[i] + sol[18] * dF_VE[i] + sol[19] * dG[i]);
+        // K(0:18,0:18) = scale1 (I + Damage (sol18 ddF_VE + sol19 ddG) D_coss)
+        for (int i = 0; i < 18; i++)
+            for (int j = 0; j < 18; j++) {
+                double t = 0;
+                for (int k = 0; k < 18; k++) t += Damage * (sol[18] * ddF_VE[i][k] + sol[19] * ddG[i][k]) * Dc[k][j];
+                K[i * 20 + j] = s1 * ((i == j ? 1.0 : 0.0) + t);
+            }
+        for (int i = 0; i < 18; i++) { K[i * 20 + 18] = s1 * dF_VE[i]; K[i * 20 + 19] = s1 * dG[i]; }
+        const double PreFac = ACreep * dt * std::pow(coss_sc, nCreep);
+        f[18] = s2 * (PreFac * std::pow(F_VE / coss_sc, nCreep) - sol[18]);
+        K[18 * 20 + 18] = -s2;
+        const double fac = nCreep * PreFac / coss_sc * std::pow(F_VE / coss_sc, nCreep - 1.0);
+        for (int j = 0; j < 18; j++) {
+            double t = 0;
+            for (int i = 0; i < 18; i++) t += Damage * dF_VE[i] * Dc[i][j];
+            K[18 * 20 + j] = s2 * fac * t;
+        }
+        if (FTrial - hOld < 0.0) {
+            f[19] = -s3 * (l_coss * pl_visc + 1.0e-6) / dt * sol[19];
+            K[19 * 20 + 19] = -s3 * (l_coss * pl_visc + 1.0e-6) / dt;
+        } else {
+            f[19] = s3 * (F - h - l_coss * pl_visc / dt * sol[19]);
+            for (int j = 0; j < 18; j++) {
+                double t = 0;
+                for (int i = 0; i < 18; i++) t += Damage * dF[i] * Dc[i][j];
+                K[19 * 20 + j] = s3 * t;
+            }
+            K[19 * 20 + 19] = s3 * (-l_coss * pl_visc / dt - dh);
+        }
+    }
+
+    // return_mapping_VEP (:508-575) and _updateStrain (:730-769).  Stop rule e < 1e-6 or
+    // e/e0 < 1e-5 (overridable), at most 500 iterations; a NaN or a residual grown 1e6-fold
+    // after 25 iterations ends the iteration where it is (the backup map is disabled there).
+    void CossReturnMapVEP(C18x18 DMat, double* Strain_Pl, const double* Strain, const double* Strain_Pl_Old, double dt,
+                          double Damage, double* hist, const double* histOld, double& DissRate, double& DamDissRate,
+                          int& iters, bool& maxit) const {
+        double K[400], Kinv[400], f[20], fOld[20], sol[20], dsol[20];
+        for (int i = 0; i < 18; i++) sol[i] = Strain[i] - Strain_Pl[i];
+        sol[18] = hist[1];
+        sol[19] = hist[2];
+        CossGetKF(K, f, sol, Strain, Strain_Pl_Old, dt, Damage, histOld);
+        double e0 = 0;
+        for (int i = 0; i < 20; i++) e0 += f[i] * f[i];
+        e0 = std::max(e0, 1e-12);
+        const double ea = rm_eabs >= 0 ? rm_eabs : 1.0e-6, er = rm_erel >= 0 ? rm_erel : 1.0e-5;
+        const unsigned mi = rm_maxit > 0 ? (unsigned)rm_maxit : 500u;
+        bool conv = false;
+        unsigned it = 0;
+        maxit = false;
+        while (!conv) {
+            it += 1;
+            lu_inverse(20, K, Kinv);
+            for (int i = 0; i < 20; i++) {
+                double t = 0;
+                for (int j = 0; j < 20; j++) t += Kinv[i * 20 + j] * f[j];
+                dsol[i] = -t;
+            }
+            for (int i = 0; i < 20; i++) { sol[i] += dsol[i]; fOld[i] = f[i]; }
+            CossGetKF(K, f, sol, Strain, Strain_Pl_Old, dt, Damage, histOld);
+            double ls1 = 0, ls2 = 0;
+            for (int i = 0; i < 20; i++) { ls1 += f[i] * dsol[i]; ls2 += fOld[i] * dsol[i]; }
+            double ls;
+            if ((ls1 - ls2) == 0.0) ls = 1.0;
+            else {
+                ls = -(ls2) / (ls1 - ls2);
+                ls = std::min(std::max(ls, 0.01), 1.0);
+            }
+            for (int i = 0; i < 20; i++) sol[i] = sol[i] - (1.0 - ls) * dsol[i];
+            CossGetKF(K, f, sol, Strain, Strain_Pl_Old, dt, Damage, histOld);
+            double e = 0;
+            for (int i = 0; i < 20; i++) e += f[i] * f[i];
+            if (e < ea || e / e0 < er) conv = true;
+            if (it > mi) { maxit = true; conv = true; }
+            if (std::isnan(e) || std::isnan(e0) || (std::fabs(e / e0) > 1e6 && it > 25)) conv = true;
+        }
+        iters = (int)it;
+        // updateStrain
+        C18 Stress, dG, dF_VE, tmp, StressTrial;
+        C18x18 ddG, ddF_VE;
+        Dc_mul(sol, Damage, Stress);
+        CossYieldVE(Stress, dF_VE, ddF_VE);
+        for (int i = 0; i < 18; i++) tmp[i] = Strain[i] - Strain_Pl_Old[i] - sol[18] * dF_VE[i];
+        Dc_mul(tmp, Damage, StressTrial);
+        CossPotential(Stress, dG, ddG);
+        double DissV = 0, DissP = 0;
+        for (int i = 0; i < 18; i++) {
+            const double ve = sol[18] * dF_VE[i], pl = sol[19] * dG[i];
+            Strain_Pl[i] = Strain_Pl_Old[i] + ve + pl;
+            DissV += 1.0 / dt * std::fabs(ve * Stress[i]);
+            DissP += 1.0 / dt * std::fabs(pl * Stress[i]);
+        }
+        hist[0] = histOld[0] + std::sqrt(sol[19] * sol[19]);
+        hist[1] = sol[18];
+        hist[2] = sol[19];
+        hist[3] = histOld[3] + std::sqrt(sol[18] * sol[18]);
+        DissRate = DissV + DissP;
+        DamDissRate = DissP;
+        lu_inverse(20, K, Kinv);
+        for (int i = 0; i < 18; i++)
+            for (int j = 0; j < 18; j++) DMat[i][j] = coss_scale1 * Kinv[i * 20 + j];
+    }
+
+    // CosseratMaterial::UpdatePlasticStrains (:451-506): LE, VEP, dVEP
+    void CossUpdatePlasticStrains(C18x18 DMat, double* Strain_Pl, const double* Strain, const double* Strain_Pl_Old, double phi,
+                                  double dt, double* hist, const double* histOld, double& DissRate, double& DamDissRate,
+                                  int& iters, bool& maxit, bool& called) const {
+        DissRate = 0.0; DamDissRate = 0.0; called = false; iters = 0; maxit = false;
+        auto identity = [&] {
+            for (int i = 0; i < 18; i++) {
+                Strain_Pl[i] = 1.0 * Strain_Pl_Old[i];
+                for (int j = 0; j < 18; j++) DMat[i][j] = i == j ? 1.0 : 0.0;
+            }
+        };
+        if (Rheology == IFE_RHEO_LE) { identity(); return; }
+        if (Rheology != IFE_RHEO_VEP && Rheology != IFE_RHEO_DVEP)
+            throw std::invalid_argument("Rheology type not defined in CosseratMaterial.cpp,");
+        if (phi > PhiViscLim) { identity(); return; }
+        double damage = 1.0, u1, u2;
+        if (Rheology == IFE_RHEO_DVEP) {
+            double pc = std::min(1.0, std::max(0.0, phi));
+            Damage(DegFunction, pc, k0, p, damage, u1, u2);
+        }
+        called = true;
+        CossReturnMapVEP(DMat, Strain_Pl, Strain, Strain_Pl_Old, dt, damage, hist, histOld, DissRate, DamDissRate, iters, maxit);
+    }
 };
 
 // PhaseFieldUtility::GetDistributionFunction (PhaseFieldUtil.cpp:185-221), non-HO forms.
@@ -1281,7 +1590,8 @@ static void GetCapacityOnly(int method, double l, Vec& Dist, Mat& dDist, double&
 // --------------------------------------------------------------------------------
 // Problem: mesh, numbering, pattern, state, history, models
 // --------------------------------------------------------------------------------
-enum { MODEL_PHASEFIELD = 0, MODEL_THERMAL = 1 };
+enum { MODEL_PHASEFIELD = 0, MODEL_THERMAL = 1, MODEL_COSS = 2 };
+static bool is_pf(int type) { return type == MODEL_PHASEFIELD || type == MODEL_COSS; }
 
 struct Model {
     int type = 0;
@@ -1704,6 +2014,292 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
 }
 
 // ---- PhaseFieldDamage::Assemble_T (PhaseField.cpp:733-768) ------------------------
+// ---- CossPhaseField (Models/Fracture/PhaseField/CossPhaseField.cpp), 3D ----------------
+// ConstructBMatCoss (:1168-1223): rows xx yy zz xy yx yz zy xz zx (strains), then nine
+// curvature rows scaled by l_coss; columns field-blocked [ux uy uz ox oy oz]
+static void ConstructBMatCoss(Mat& B, const Shapes& S, int ip, double l) {
+    const int n = S.n;
+    B.zero();
+    for (int a = 0; a < n; a++) {
+        const double N = S.Nv(ip, a), g0 = S.Gv(ip, 0, a), g1 = S.Gv(ip, 1, a), g2 = S.Gv(ip, 2, a);
+        B(0, a) = g0; B(1, n + a) = g1; B(2, 2 * n + a) = g2;
+        B(3, n + a) = g0; B(3, 5 * n + a) = -N;
+        B(4, a) = g1; B(4, 5 * n + a) = N;
+        B(5, 2 * n + a) = g1; B(5, 3 * n + a) = -N;
+        B(6, n + a) = g2; B(6, 3 * n + a) = N;
+        B(7, 2 * n + a) = g0; B(7, 4 * n + a) = -N;
+        B(8, a) = g2; B(8, 4 * n + a) = N;
+        B(9, 3 * n + a) = l * g0; B(10, 4 * n + a) = l * g1; B(11, 5 * n + a) = l * g2;
+        B(12, 3 * n + a) = l * g0; B(13, 4 * n + a) = l * g1;
+        B(14, 4 * n + a) = l * g1; B(15, 5 * n + a) = l * g2;
+        B(16, 3 * n + a) = l * g0; B(17, 5 * n + a) = l * g2;
+    }
+}
+
+// CossPhaseField::Assemble_U (:337-534)
+static void Assemble_U_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats& st) {
+    const int n = c.n, nip = c.nip, nu = 6 * n;
+    Material M = md.M;
+    Shapes Su, Sph, ST;
+    Su.resize(nip, n, 3); Sph.resize(nip, n, 3); ST.resize(nip, n, 3);
+    Mat B(18, nu);
+    std::vector<int64_t> El(n), dofsU(nu), dofsPhase(n), dofsT(n);
+    std::vector<int> fph = {md.field_phase}, fT = {md.field_T};
+    Vec U(nu), dU(nu), ddU(nu), UOld(nu), Phase(n), PhaseOld(n), T(n, 0.0);
+    Vec F_U(nu), WLumpedI(n), WLumpedG(n);
+    Mat K_UU(nu, nu), X1(nu, 18), X2(nu, 18);
+    const int su = md.Step_u, sph = md.Step_phase;
+    const double ddXdt2 = c.ddu_dt, dXdt = c.du_dt, dt = c.dt;
+    const size_t NIPT = (size_t)c.NE * nip;
+    for (int64_t el = e0; el < e1; el++) {
+        if (!md.A_e.empty()) M.A0 = md.A_e[el];
+        getShapeGrads(c, el, Su);
+        getShapeGrads(c, el, Sph);
+        elem_nodes(c, el, El);
+        getDofForNodes(c, El, md.fields_u, dofsU);
+        getDofForNodes(c, El, fph, dofsPhase);
+        for (int i = 0; i < nu; i++) {
+            U[i] = c.state[IFE_STATE][su][dofsU[i]];
+            UOld[i] = c.state[IFE_STATE_OLD][su][dofsU[i]];
+            dU[i] = c.state[IFE_DSTATE][su][dofsU[i]];
+            ddU[i] = c.state[IFE_DDSTATE][su][dofsU[i]];
+        }
+        for (int a = 0; a < n; a++) {
+            Phase[a] = c.state[IFE_STATE][sph][dofsPhase[a]];
+            PhaseOld[a] = c.state[IFE_STATE_OLD][sph][dofsPhase[a]];
+        }
+        if (md.HasT) {
+            getShapeGrads(c, el, ST);
+            getDofForNodes(c, El, fT, dofsT);
+            for (int a = 0; a < n; a++) T[a] = c.state[IFE_STATE][md.Step_T][dofsT[a]];
+        }
+        F_U.assign(nu, 0.0);
+        K_UU.zero();
+        WLumpedI.assign(n, 0.0);
+        WLumpedG.assign(n, 0.0);
+        for (int ip = 0; ip < nip; ip++) {
+            const double poros = 0.0;
+            double Tip;
+            if (md.HasT) { Tip = 0; for (int a = 0; a < n; a++) Tip += ST.Nv(ip, a) * T[a]; }
+            else Tip = md.opt.T_default;
+            M.SetTemperature(Tip, poros);
+            double phi = 0, phi_Old = 0;
+            for (int a = 0; a < n; a++) { phi += Sph.Nv(ip, a) * Phase[a]; phi_Old += Sph.Nv(ip, a) * PhaseOld[a]; }
+            (void)phi_Old;
+            double Dam, dDam, nn1, DamI, DamG;
+            Material::Damage(md.pf.DamFunction, phi, md.pf.k0, md.pf.p, Dam, dDam, nn1);
+            Material::Damage(md.pf.DensityDamage, phi, md.pf.k0, md.pf.p, DamI, dDam, nn1);
+            Material::Damage(md.pf.GravityDamage, phi, md.pf.k0, md.pf.p, DamG, dDam, nn1);
+            ConstructBMatCoss(B, Su, ip, M.l_coss);
+            double Strain[18], Strain_Pl[18], Strain_Pl_Old[18];
+            for (int r = 0; r < 18; r++) {
+                double s_ = 0;
+                for (int j = 0; j < nu; j++) s_ += B(r, j) * U[j];
+                Strain[r] = s_;
+            }
+            size_t gi = (size_t)el * nip + ip;
+            for (int r = 0; r < 18; r++) { Strain_Pl_Old[r] = md.PSOld[r * NIPT + gi]; Strain_Pl[r] = md.PS[r * NIPT + gi]; }
+            double hist[4] = {0, 0, 0, 0}, histOld[4] = {0, 0, 0, 0};
+            for (int k = 0; k < M.HistSize; k++) { hist[k] = md.MH[k * NIPT + gi]; histOld[k] = md.MHOld[k * NIPT + gi]; }
+            double DPlast[18][18];
+            double PlasticDiss, DamPlasticDiss;
+            int iters; bool maxit, called;
+            M.CossUpdatePlasticStrains(DPlast, Strain_Pl, Strain, Strain_Pl_Old, phi, dt, hist, histOld, PlasticDiss,
+                                       DamPlasticDiss, iters, maxit, called);
+            if (called) { st.calls++; st.iters += iters; st.maxit += maxit ? 1 : 0; }
+            for (int k = 0; k < M.HistSize; k++) md.MH[k * NIPT + gi] = hist[k];
+            for (int r = 0; r < 18; r++) md.PS[r * NIPT + gi] = Strain_Pl[r];
+            md.VD[gi] = md.VDOld[gi] + PlasticDiss * dt;
+            md.DVD[gi] = md.DVDOld[gi] + DamPlasticDiss * dt;
+            for (int r = 0; r < 18; r++) Strain[r] = Strain[r] - Strain_Pl[r];
+            double D_Dam[18][18], D_UnDam[18][18], S_Dam[18], S_Undam[18];
+            M.CossEnergySplit(D_Dam, D_UnDam, S_Dam, S_Undam, Strain);
+            const double w = Su.w[ip];
+            double sv[18];
+            for (int r = 0; r < 18; r++) sv[r] = Dam * S_Dam[r] + S_Undam[r];
+            for (int j = 0; j < nu; j++) {
+                double t = 0;
+                for (int r = 0; r < 18; r++) t += (w * B(r, j)) * sv[r];
+                F_U[j] += t;
+            }
+            // K_UU += ((w B^T) (Dam D_Dam + D_UnDam)) DPlast B  (Eigen left-to-right)
+            for (int a = 0; a < nu; a++)
+                for (int j = 0; j < 18; j++) {
+                    double t = 0;
+                    for (int r = 0; r < 18; r++) t += (w * B(r, a)) * (Dam * D_Dam[r][j] + D_UnDam[r][j]);
+                    X1(a, j) = t;
+                }
+            for (int a = 0; a < nu; a++)
+                for (int j = 0; j < 18; j++) {
+                    double t = 0;
+                    for (int r = 0; r < 18; r++) t += X1(a, r) * DPlast[r][j];
+                    X2(a, j) = t;
+                }
+            for (int b = 0; b < nu; b++)
+                for (int a = 0; a < nu; a++) {
+                    double t = 0;
+                    for (int r = 0; r < 18; r++) t += X2(a, r) * B(r, b);
+                    K_UU(a, b) += t;
+                }
+            for (int a = 0; a < n; a++) {
+                WLumpedI[a] += w * Su.Nv(ip, a) * DamI;
+                WLumpedG[a] += w * Su.Nv(ip, a) * (1.0 - poros) * DamG;
+            }
+        }
+        const double l2 = M.l_coss * M.l_coss;
+        for (int a = 0; a < n; a++) {
+            const int ny = a + n, nz = a + 2 * n, ox = a + 3 * n, oy = a + 4 * n, oz = a + 5 * n;
+            if (md.opt.Inertia) {
+                const double rhoCoss = 2 * l2 * M.Density / (1 + M.Poisson);
+                F_U[a] += WLumpedI[a] * M.Density * ddU[a];
+                K_UU(a, a) += WLumpedI[a] * M.Density * ddXdt2;
+                F_U[ny] += WLumpedI[a] * M.Density * ddU[ny];
+                K_UU(ny, ny) += WLumpedI[a] * M.Density * ddXdt2;
+                F_U[oz] += WLumpedI[a] * rhoCoss * ddU[oz];
+                K_UU(oz, oz) += WLumpedI[a] * rhoCoss * ddXdt2;
+                F_U[nz] += WLumpedI[a] * M.Density * ddU[nz];
+                K_UU(nz, nz) += WLumpedI[a] * M.Density * ddXdt2;
+                F_U[ox] += WLumpedI[a] * rhoCoss * ddU[ox];
+                K_UU(ox, ox) += WLumpedI[a] * M.Density * ddXdt2;  // Density, as written (:503)
+                F_U[oy] += WLumpedI[a] * rhoCoss * ddU[oy];
+                K_UU(oy, oy) += WLumpedI[a] * M.Density * ddXdt2;  // (:506)
+            }
+            if (M.Damping > 0.0) {
+                F_U[a] += WLumpedI[a] * M.Damping * dU[a];
+                K_UU(a, a) += WLumpedI[a] * M.Damping * dXdt;
+                F_U[ny] += WLumpedI[a] * M.Damping * dU[ny];
+                K_UU(ny, ny) += WLumpedI[a] * M.Damping * dXdt;
+                F_U[oz] += WLumpedI[a] * 2 * l2 * M.Damping * dU[oz];
+                K_UU(oz, oz) += WLumpedI[a] * 2 * l2 * M.Damping * dXdt;
+                F_U[nz] += WLumpedI[a] * M.Damping * dU[nz];
+                K_UU(nz, nz) += WLumpedI[a] * M.Damping * dXdt;
+                F_U[ox] += WLumpedI[a] * 2 * l2 * M.Damping * dU[ox];
+                K_UU(ox, ox) += WLumpedI[a] * 2 * l2 * M.Damping * dXdt;
+                F_U[oy] += WLumpedI[a] * 2 * l2 * M.Damping * dU[oy];
+                K_UU(oy, oy) += WLumpedI[a] * 2 * l2 * M.Damping * dXdt;
+            }
+            F_U[a] += -WLumpedG[a] * M.Density * md.g[0];
+            F_U[ny] += -WLumpedG[a] * M.Density * md.g[1];
+            F_U[nz] += -WLumpedG[a] * M.Density * md.g[2];
+        }
+        sink.vec(dofsU, F_U);
+        sink.mat(dofsU, dofsU, K_UU);
+    }
+}
+
+// CossPhaseField::Assemble_Phase (:536-769), Hist/None irreversibility, non-HO distributions
+static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats& st) {
+    const int n = c.n, nip = c.nip, nu = 6 * n;
+    Material M = md.M;
+    Shapes Su, Sph, ST;
+    Su.resize(nip, n, 3); Sph.resize(nip, n, 3); ST.resize(nip, n, 3);
+    Mat B(18, nu);
+    std::vector<int64_t> El(n), dofsU(nu), dofsPhase(n), dofsT(n, 0);  // dofsT never filled (:575)
+    std::vector<int> fph = {md.field_phase};
+    Vec U(nu), dU(nu), Phase(n), PhaseOld(n), T(n, 0.0);
+    Vec Dist(n), F_Ph(n), WLumped(n);
+    Mat dDist(n, n), K_PhPh(n, n);
+    const int su = md.Step_u, sph = md.Step_phase;
+    const double dt = c.dt, t = c.t;
+    const size_t NIPT = (size_t)c.NE * nip;
+    for (int64_t el = e0; el < e1; el++) {
+        getShapeGrads(c, el, Su);
+        getShapeGrads(c, el, Sph);
+        elem_nodes(c, el, El);
+        getDofForNodes(c, El, md.fields_u, dofsU);
+        getDofForNodes(c, El, fph, dofsPhase);
+        for (int i = 0; i < nu; i++) { U[i] = c.state[IFE_STATE][su][dofsU[i]]; dU[i] = c.state[IFE_DSTATE][su][dofsU[i]]; }
+        for (int a = 0; a < n; a++) {
+            Phase[a] = c.state[IFE_STATE][sph][dofsPhase[a]];
+            PhaseOld[a] = c.state[IFE_STATE_OLD][sph][dofsPhase[a]];
+        }
+        if (md.HasT) {
+            getShapeGrads(c, el, ST);
+            for (int a = 0; a < n; a++) T[a] = c.state[IFE_STATE][md.Step_T][dofsT[a]];
+        }
+        F_Ph.assign(n, 0.0);
+        K_PhPh.zero();
+        WLumped.assign(n, 0.0);
+        for (int ip = 0; ip < nip; ip++) {
+            double Tip;
+            if (md.HasT) { Tip = 0; for (int a = 0; a < n; a++) Tip += ST.Nv(ip, a) * T[a]; }
+            else Tip = md.opt.T_default;
+            M.SetTemperature(Tip, 0.0);
+            if (!md.Gc_e.empty()) M.Gc = md.Gc_e[el];
+            double phi = 0;
+            for (int a = 0; a < n; a++) phi += Sph.Nv(ip, a) * Phase[a];
+            double Dam, dDam, ddDam, DamI, dDamI, ddDamI, DamG, dDamG, ddDamG;
+            Material::Damage(md.pf.DamFunction, phi, md.pf.k0, md.pf.p, Dam, dDam, ddDam);
+            Material::Damage(md.pf.DensityDamage, phi, md.pf.k0, md.pf.p, DamI, dDamI, ddDamI);
+            Material::Damage(md.pf.GravityDamage, phi, md.pf.k0, md.pf.p, DamG, dDamG, ddDamG);
+            ConstructBMatCoss(B, Su, ip, M.l_coss);
+            size_t gi = (size_t)el * nip + ip;
+            double Strain[18];
+            for (int r = 0; r < 18; r++) {
+                double s_ = 0;
+                for (int j = 0; j < nu; j++) s_ += B(r, j) * U[j];
+                Strain[r] = s_ - md.PS[r * NIPT + gi];
+            }
+            double D_Dam[18][18], D_UnDam[18][18], S_Dam[18], S_Undam[18];
+            double Energy = M.CossEnergySplit(D_Dam, D_UnDam, S_Dam, S_Undam, Strain);
+            if (t < 0.0) Energy = 0.0;  // initialisation period (:682-684)
+            if (md.opt.Irreversible == IFE_IRR_HIST) {
+                if (md.EnergyOld[gi] > Energy) { md.Energy[gi] = md.EnergyOld[gi]; Energy = md.EnergyOld[gi]; }
+                else md.Energy[gi] = Energy;
+            } else {
+                md.Energy[gi] = Energy;
+            }
+            double gamma = GetDistributionFunction(md.pf.DistributionFunction, M.pf_l, 3, Dist, dDist, Sph, ip, Phase.data());
+            double w = Su.w[ip];
+            st.lcrack += gamma * w;
+            for (int a = 0; a < n; a++) F_Ph[a] += w * M.Gc * Dist[a];
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) K_PhPh(a, b) += w * M.Gc * dDist(a, b);
+            for (int a = 0; a < n; a++) F_Ph[a] += w * dDam * Sph.Nv(ip, a) * Energy;
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) K_PhPh(a, b) += w * ddDam * Sph.Nv(ip, a) * Sph.Nv(ip, b) * Energy;
+            if (md.pf.ViscoDrivingFactor > 1.0e-9) {
+                double DamV, dDamV, ddDamV;
+                double PlasticDiss = md.pf.ViscoDrivingFactor * md.DVD[gi];
+                Material::Damage(md.pf.ViscoDriving, phi, md.pf.k0, md.pf.p, DamV, dDamV, ddDamV);
+                for (int a = 0; a < n; a++) F_Ph[a] += w * dDamV * Sph.Nv(ip, a) * PlasticDiss;
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++) K_PhPh(a, b) += w * ddDamV * Sph.Nv(ip, a) * Sph.Nv(ip, b) * PlasticDiss;
+            }
+            if (md.opt.Inertia && md.pf.ConsistentDegradation) {
+                double v2 = 0;  // (N_uu dU)^T (N_uu dU), six components
+                for (int d = 0; d < 6; d++) {
+                    double vd = 0;
+                    for (int a = 0; a < n; a++) vd += Su.Nv(ip, a) * dU[d * n + a];
+                    v2 += vd * vd;
+                }
+                for (int a = 0; a < n; a++) F_Ph[a] += -w * 0.5 * M.Density * v2 * Sph.Nv(ip, a) * dDamI;
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++)
+                        K_PhPh(a, b) += -w * 0.5 * M.Density * v2 * Sph.Nv(ip, a) * Sph.Nv(ip, b) * ddDamI;
+            }
+            if (md.pf.ConsistentDegradation) {
+                double gu = 0;
+                for (int d = 0; d < 3; d++) {
+                    double ud = 0;
+                    for (int a = 0; a < n; a++) ud += Su.Nv(ip, a) * U[d * n + a];
+                    gu += md.g[d] * ud;
+                }
+                for (int a = 0; a < n; a++) F_Ph[a] += -w * M.Density * gu * Sph.Nv(ip, a) * dDamG;
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++) K_PhPh(a, b) += -w * M.Density * gu * Sph.Nv(ip, a) * Sph.Nv(ip, b) * ddDamG;
+            }
+            for (int a = 0; a < n; a++) WLumped[a] += w * Sph.Nv(ip, a);
+        }
+        for (int a = 0; a < n; a++) {
+            F_Ph[a] += WLumped[a] * M.pf_visc / dt * (Phase[a] - PhaseOld[a]);
+            K_PhPh(a, a) += WLumped[a] * (M.pf_visc / dt + md.opt.ViscousStab * M.Gc / M.pf_l);
+        }
+        sink.vec(dofsPhase, F_Ph);
+        sink.mat(dofsPhase, dofsPhase, K_PhPh);
+    }
+}
+
 static void Assemble_T_PF(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats&) {
     const int n = c.n, nip = c.nip, dim = c.dim;
     Shapes S;
@@ -1785,6 +2381,10 @@ static void model_assemble(Ctx& c, Model& md, Sink& sink, int step, int64_t e0, 
     if (md.type == MODEL_PHASEFIELD) {
         if (step == md.Step_u) Assemble_U(c, md, sink, e0, e1, st);
         if (step == md.Step_phase) Assemble_Phase(c, md, sink, e0, e1, st);
+        if (step == md.Step_T && md.opt.ViscousHeating) Assemble_T_PF(c, md, sink, e0, e1, st);
+    } else if (md.type == MODEL_COSS) {  // CossPhaseField::Assemble (:317-335)
+        if (step == md.Step_u) Assemble_U_Coss(c, md, sink, e0, e1, st);
+        if (step == md.Step_phase) Assemble_Phase_Coss(c, md, sink, e0, e1, st);
         if (step == md.Step_T && md.opt.ViscousHeating) Assemble_T_PF(c, md, sink, e0, e1, st);
     } else {
         if (step == md.Step_T) Assemble_Thermal(c, md, sink, e0, e1, st);
@@ -1922,6 +2522,58 @@ double ifo_energy_split(const ife_material* mat, int consistent, double T, doubl
     return E;
 }
 
+// Cosserat material pieces at temperature T (porosity 0), for the pinning tests.
+// which: 0 YieldFunction, 1 YieldFunctionVE, 2 PotentialFunction (F = 0); ddF row-major 18x18.
+double ifo_coss_yield(const ife_material* mat, double T, int which, const double* stress, double* dF, double* ddF) {
+    Material M;
+    M.init(*mat, nullptr);
+    M.SetTemperature(T, 0.0);
+    double dd[18][18], F = 0;
+    if (which == 0) F = M.CossYield(stress, dF, dd);
+    else if (which == 1) F = M.CossYieldVE(stress, dF, dd);
+    else M.CossPotential(stress, dF, dd);
+    for (int i = 0; i < 18; i++)
+        for (int j = 0; j < 18; j++) ddF[i * 18 + j] = dd[i][j];
+    return F;
+}
+
+double ifo_coss_energy_split(const ife_material* mat, const ife_phasefield_opts* pf, double T, const double* strain,
+                             double* D_Dam, double* D_UnDam, double* S_Dam, double* S_Undam) {
+    try {
+        Material M;
+        M.init(*mat, pf);
+        M.SetTemperature(T, 0.0);
+        double Dd[18][18], Du[18][18];
+        double E = M.CossEnergySplit(Dd, Du, S_Dam, S_Undam, strain);
+        for (int i = 0; i < 18; i++)
+            for (int j = 0; j < 18; j++) { D_Dam[i * 18 + j] = Dd[i][j]; D_UnDam[i * 18 + j] = Du[i][j]; }
+        return E;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return std::numeric_limits<double>::quiet_NaN();
+    }
+}
+
+// CosseratMaterial::return_mapping_VEP at temperature T: strain_pl (in: initial guess,
+// out), hist (4, in/out), hist_old (4), DMat (18x18 out), diss2 = {DissRate, DamDissRate}.
+// eabs / erel / maxit: the stop rule (negative: the reference's 1e-6 / 1e-5 / 500).
+void ifo_coss_return_map(const ife_material* mat, const ife_phasefield_opts* pf, double T, double dt, double damage,
+                         const double* strain, double* strain_pl, const double* strain_pl_old, double* hist,
+                         const double* hist_old, double* DMat, double* diss2, int* iters, double eabs, double erel,
+                         int maxit) {
+    Material M;
+    M.init(*mat, pf);
+    M.SetTemperature(T, 0.0);
+    M.rm_eabs = eabs;
+    M.rm_erel = erel;
+    M.rm_maxit = maxit;
+    double Dm[18][18];
+    bool mx;
+    M.CossReturnMapVEP(Dm, strain_pl, strain, strain_pl_old, dt, damage, hist, hist_old, diss2[0], diss2[1], *iters, mx);
+    for (int i = 0; i < 18; i++)
+        for (int j = 0; j < 18; j++) DMat[i * 18 + j] = Dm[i][j];
+}
+
 void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_models, const ifo_model_desc* models) {
     try {
         Ctx* c = new Ctx();
@@ -1952,7 +2604,7 @@ void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_mod
             md.type = models[m].type;
             md.pf = models[m].pf;
             md.opt = models[m].opts;
-            md.M.init(models[m].mat, md.type == MODEL_PHASEFIELD ? &md.pf : nullptr);
+            md.M.init(models[m].mat, is_pf(md.type) ? &md.pf : nullptr);
             auto need = [&](const char* nm) {
                 int f = c->field_index(nm);
                 if (f < 0) throw std::invalid_argument(std::string("missing dof ") + nm);
@@ -1960,10 +2612,13 @@ void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_mod
                     if (innode[v]) hasdof[f][v] = 1;
                 return f;
             };
-            if (md.type == MODEL_PHASEFIELD) {
+            if (is_pf(md.type)) {
+                if (md.type == MODEL_COSS && c->dim != 3) throw std::invalid_argument("Cosserat model: 3D only");
                 md.fields_u.push_back(need("ux"));
                 md.fields_u.push_back(need("uy"));
                 if (c->dim == 3) md.fields_u.push_back(need("uz"));
+                if (md.type == MODEL_COSS)
+                    for (const char* nm : {"ox", "oy", "oz"}) md.fields_u.push_back(need(nm));
                 for (int f : md.fields_u)
                     if (c->fstep[f] != c->fstep[md.fields_u[0]]) throw std::invalid_argument("u fields in different steps");
                 md.Step_u = c->fstep[md.fields_u[0]];
@@ -1980,7 +2635,8 @@ void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_mod
                     throw std::invalid_argument("irreversibility method not supported by the oracle");
                 size_t NIPT = (size_t)c->NE * c->nip;
                 md.Energy.assign(NIPT, 0); md.EnergyOld.assign(NIPT, 0);
-                md.PS.assign(6 * NIPT, 0); md.PSOld.assign(6 * NIPT, 0);
+                const size_t nps = md.type == MODEL_COSS ? 18 : 6;
+                md.PS.assign(nps * NIPT, 0); md.PSOld.assign(nps * NIPT, 0);
                 md.VD.assign(NIPT, 0); md.VDOld.assign(NIPT, 0);
                 md.DVD.assign(NIPT, 0); md.DVDOld.assign(NIPT, 0);
                 md.MH.assign((size_t)std::max(md.M.HistSize, 1) * NIPT, 0);
@@ -2014,11 +2670,12 @@ void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_mod
         for (int s = 0; s < c->nsteps; s++) {
             // blocks: (rows, cols) dof lists per element and model
             auto for_blocks = [&](const std::function<void(const std::vector<int64_t>&)>& fn) {
-                std::vector<int64_t> El(c->n), d(c->n * c->dim), dph(c->n), dT(c->n);
+                std::vector<int64_t> El(c->n), dph(c->n), dT(c->n);
                 for (auto& md : c->models) {
                     for (int64_t el = 0; el < c->NE; el++) {
                         elem_nodes(*c, el, El);
-                        if (md.type == MODEL_PHASEFIELD) {
+                        if (is_pf(md.type)) {
+                            std::vector<int64_t> d(c->n * md.fields_u.size());
                             if (s == md.Step_u) { getDofForNodes(*c, El, md.fields_u, d); fn(d); }
                             if (s == md.Step_phase) { std::vector<int> f1 = {md.field_phase}; getDofForNodes(*c, El, f1, dph); fn(dph); }
                         } else if (s == md.Step_T) {
@@ -2141,7 +2798,7 @@ int ifo_assemble(void* h, int step, double* K_vals, double* f) {
             Stats st;
             model_assemble(*c, md, sink, step, 0, c->NE, st);
             c->rm_calls += st.calls; c->rm_iters += st.iters; c->rm_maxit += st.maxit;
-            if (md.type == MODEL_PHASEFIELD && step == md.Step_phase) md.LCrack = st.lcrack;
+            if (is_pf(md.type) && step == md.Step_phase) md.LCrack = st.lcrack;
         }
         return 0;
     } catch (std::exception& ex) { g_err = ex.what(); return 1; }
@@ -2152,7 +2809,7 @@ void ifo_stats(void* h, int64_t* out3, double* lcrack) {
     out3[0] = c->rm_calls; out3[1] = c->rm_iters; out3[2] = c->rm_maxit;
     *lcrack = 0;
     for (auto& md : c->models)
-        if (md.type == MODEL_PHASEFIELD) *lcrack = md.LCrack;
+        if (is_pf(md.type)) *lcrack = md.LCrack;
 }
 void ifo_reset_stats(void* h) {
     Ctx* c = (Ctx*)h;

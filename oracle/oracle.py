@@ -47,6 +47,12 @@ def load():
         "ifo_set_param": (C.c_int, [P, C.c_int, C.c_char_p, C.c_double]),
         "ifo_return_map_vep": (None, [C.POINTER(abi.Material), C.c_double, C.c_double, C.c_double, dp, dp, dp, dp,
                                       dp, dp, dp, C.POINTER(C.c_int)]),
+        "ifo_coss_yield": (C.c_double, [C.POINTER(abi.Material), C.c_double, C.c_int, dp, dp, dp]),
+        "ifo_coss_energy_split": (C.c_double, [C.POINTER(abi.Material), C.POINTER(abi.PhaseFieldOpts), C.c_double, dp,
+                                               dp, dp, dp, dp]),
+        "ifo_coss_return_map": (None, [C.POINTER(abi.Material), C.POINTER(abi.PhaseFieldOpts), C.c_double, C.c_double,
+                                       C.c_double, dp, dp, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.c_double,
+                                       C.c_double, C.c_int]),
         "ifo_energy_split": (C.c_double, [C.POINTER(abi.Material), C.c_int, C.c_double, C.c_double, dp, dp, dp,
                                           dp, dp]),
         "ifo_create": (P, [C.POINTER(abi.MeshDesc), C.POINTER(abi.DofsDesc), C.c_int, C.POINTER(ModelDesc)]),
@@ -123,7 +129,7 @@ class Oracle:
         dd = dofs_desc(prob.dof_names, prob.dof_steps, keep)
         arr = (ModelDesc * len(prob.models))()
         for i, m in enumerate(prob.models):
-            arr[i].type = 0 if m.type_name == "Fracture/PhaseField" else 1
+            arr[i].type = {"Fracture/PhaseField": 0, "Thermal/ThermalModel": 1, "Fracture/CossPhaseField": 2}[m.type_name]
             arr[i].mat = m.material
             if m.pf is not None:
                 arr[i].pf = m.pf
