@@ -104,13 +104,16 @@ struct Model {
     ife_phasefield_opts pf;
     ife_model_opts opt;
     int Step_u = -1, Step_phase = -1, Step_T = 99;
-    int field_u[3] = {-1, -1, -1}, field_phase = -1, field_T = -1;
+    int field_u[6] = {-1, -1, -1, -1, -1, -1}, field_phase = -1, field_T = -1;
+    int nfu = 0;        // u-step fields: dim, or 6 for the Cosserat model (u and micro-rotations)
+    bool coss = false;  // Fracture/CossPhaseField
     bool HasT = false;
     int HistSize = 0;
     // device history [comp][E*nip]
     DevBuf<double> E, EOld, PS, PSOld, VD, VDOld, DVD, DVDOld, MH, MHOld;
     // per-element parameters (allocated at setup, used once set) and scalar overrides
     DevBuf<double> Gc_e, A_e;
+    DevBuf<double> cossb;  // Cosserat per-IP tangent/stress coefficients [e][COSS_NF][nip]
     bool use_Gc_e = false, use_A_e = false;
     double n_override = std::numeric_limits<double>::quiet_NaN();
     double rm_eabs = -1, rm_erel = -1;
@@ -256,6 +259,7 @@ MatConst make_matconst(const ife_physics* ph, const Model* pfm, const Model* thm
     mc.TRef = m.TRef; mc.TOffset = m.TOffset; mc.QCreep = m.QCreep; mc.ft0 = m.ft0; mc.ftdeg = m.ftdeg;
     mc.sy0 = m.sy; mc.sydeg = m.sydeg;
     mc.B_dp = m.B_DP; mc.Ice_eRef = m.eRef; mc.Ice_cReduce = m.cReduce; mc.StarConvexGamma = m.StarConvexGamma;
+    mc.l_coss = m.l_coss; mc.G_coss0 = m.G_coss; mc.hard_eRef = m.hard_eRef;  // CosseratMaterial.cpp:5-10
     if (m.Rheology == IFE_RHEO_VEP || m.Rheology == IFE_RHEO_DVEP) {  // SolidMaterial.cpp:49-64
         const double ya = M_PI / 180.0 * m.F_deg, pa = M_PI / 180.0 * m.G_deg;
         mc.yield_alpha = 6.0 * std::sin(ya) / (3.0 - std::sin(ya));
@@ -327,7 +331,7 @@ void plan(ife_physics* ph) {
         };
         if (m->type == MT_PHASEFIELD) {
             std::vector<int> fu;
-            for (int d = 0; d < ph->dim; d++) fu.push_back(m->field_u[d]);
+            for (int d = 0; d < m->nfu; d++) fu.push_back(m->field_u[d]);
             need_exact(m->Step_u, fu);
             need_exact(m->Step_phase, {m->field_phase});
         } else {
@@ -484,8 +488,10 @@ void setup_device(ife_physics* ph) {
         size_t hs = (size_t)std::max(m->HistSize, 1);
         DevBuf<double>* one[] = {&m->E, &m->EOld, &m->VD, &m->VDOld, &m->DVD, &m->DVDOld};
         for (auto* b : one) b->alloc(NIPT);
-        m->PS.alloc(6 * NIPT);
-        m->PSOld.alloc(6 * NIPT);
+        const size_t nps = m->coss ? 18 : 6;
+        m->PS.alloc(nps * NIPT);
+        m->PSOld.alloc(nps * NIPT);
+        if (m->coss) m->cossb.alloc((size_t)COSS_NF * NIPT);
         m->MH.alloc(hs * NIPT);
         m->MHOld.alloc(hs * NIPT);
         m->Gc_e.alloc(ph->NE);
@@ -550,7 +556,9 @@ KParams base_params(ife_physics* ph) {
     if (pfm) {
         p.su = stepio(pfm->Step_u);
         p.sph = stepio(pfm->Step_phase);
-        for (int d = 0; d < ph->dim; d++) p.fpos_u[d] = ph->fpos[pfm->field_u[d]];
+        for (int d = 0; d < pfm->nfu; d++) p.fpos_u[d] = ph->fpos[pfm->field_u[d]];
+        p.coss = pfm->coss;
+        p.cossb = pfm->cossb.p;
         p.fpos_ph = ph->fpos[pfm->field_phase];
         p.E = pfm->E.p; p.EOld = pfm->EOld.p; p.PS = pfm->PS.p; p.PSOld = pfm->PSOld.p;
         p.VD = pfm->VD.p; p.VDOld = pfm->VDOld.p; p.DVD = pfm->DVD.p; p.DVDOld = pfm->DVDOld.p;
@@ -640,6 +648,30 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
             HIPCHK(launch());
         }
     };
+    if (pfm && pfm->coss) {
+        // Cosserat: the material kernel updates every IP; the u step's 162-dof element
+        // blocks go straight into the zeroed CSR values colour by colour; the phase and T
+        // steps use the shared element and gather kernels
+        timed(ph->mat_recs, 0, [&] { return launch_coss_material(p, s); });
+        if (p.do_u) {
+            const Step& su = ph->steps[pfm->Step_u];
+            HIPCHK(launch_fill(p.su.K, (int64_t)su.col.size(), 0.0, s));
+            HIPCHK(launch_fill(p.su.f, su.nrows, 0.0, s));
+            for (size_t c = 0; c + 1 < ph->color_ptr.size(); c++) {
+                KParams q = p;
+                q.elist = ph->d_colors.p + ph->color_ptr[c];
+                q.nlist = ph->color_ptr[c + 1] - ph->color_ptr[c];
+                timed(ph->recs, q.nlist, [&] { return launch_coss_assemble(q, s); });
+            }
+        }
+        KParams q = p;
+        q.do_u = 0;
+        if (q.do_ph || q.do_Tpf || q.do_th) {
+            HIPCHK(launch_assemble(q, ph->dim, s));
+            timed(ph->gat_recs, 0, [&] { return launch_gather(q, ph->dim, s); });
+        }
+        return;
+    }
     timed(ph->mat_recs, 0, [&] { return launch_material(p, ph->dim, s); });
     timed(ph->recs, ph->NE, [&] { return launch_assemble(p, ph->dim, s); });
     timed(ph->gat_recs, 0, [&] { return launch_gather(p, ph->dim, s); });
@@ -687,8 +719,8 @@ DevBuf<double>* history_buf(Model* m, const std::string& nm, int64_t& ncomp) {
     if (nm == "ViscDissOld") return &m->VDOld;
     if (nm == "DamViscDiss") return &m->DVD;
     if (nm == "DamViscDissOld") return &m->DVDOld;
-    if (nm == "PlasticStrain") { ncomp = 6; return &m->PS; }
-    if (nm == "PlasticStrainOld") { ncomp = 6; return &m->PSOld; }
+    if (nm == "PlasticStrain") { ncomp = m->coss ? 18 : 6; return &m->PS; }
+    if (nm == "PlasticStrainOld") { ncomp = m->coss ? 18 : 6; return &m->PSOld; }
     if (nm == "MatHist") { ncomp = std::max(m->HistSize, 1); return &m->MH; }
     if (nm == "MatHistOld") { ncomp = std::max(m->HistSize, 1); return &m->MHOld; }
     return nullptr;
@@ -923,6 +955,7 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             m->pf = *pf;
             m->opt = *opts;
             const char* un[3] = {"ux", "uy", "uz"};
+            m->nfu = ph->dim;
             for (int d = 0; d < ph->dim; d++) m->field_u[d] = need(un[d]);
             for (int d = 1; d < ph->dim; d++)
                 if (ph->fstep[m->field_u[d]] != ph->fstep[m->field_u[0]])
@@ -954,6 +987,42 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             int dams[] = {pf->DamFunction, pf->DensityDamage, pf->GravityDamage, pf->ViscoDriving};
             for (int d : dams)
                 if (d < IFE_DAM_NONE || d > IFE_DAM_PORDER) THROW(IFE_ERR_INVALID, "unknown damage function");
+        } else if (tn == "Fracture/CossPhaseField") {
+            // CossPhaseField::Setup (CossPhaseField.cpp:168-280)
+            if (ph->pf_model()) THROW(IFE_ERR_UNSUPPORTED, "one phase-field model per physics");
+            if (!pf || !opts) THROW(IFE_ERR_INVALID, "phase-field options required");
+            if (ph->dim != 3) THROW(IFE_ERR_UNSUPPORTED, "Fracture/CossPhaseField is implemented for 3D meshes");
+            m->type = MT_PHASEFIELD;
+            m->coss = true;
+            m->pf = *pf;
+            m->opt = *opts;
+            m->opt.LumpedPFCapacity = 0;  // CossPhaseField has no lumped capacity / driving forms
+            m->opt.LumpedDriving = 0;
+            const char* un[6] = {"ux", "uy", "uz", "ox", "oy", "oz"};
+            m->nfu = 6;
+            for (int d = 0; d < 6; d++) m->field_u[d] = need(un[d]);
+            for (int d = 1; d < 6; d++)
+                if (ph->fstep[m->field_u[d]] != ph->fstep[m->field_u[0]])
+                    THROW(IFE_ERR_INVALID, "Fracture/CossPhaseField requires the u and w fields in one solver step");
+            m->Step_u = ph->fstep[m->field_u[0]];
+            m->field_phase = need("phase");
+            m->Step_phase = ph->fstep[m->field_phase];
+            m->HasT = ph->field_index("T") >= 0;
+            if (m->HasT) { m->field_T = need("T"); m->Step_T = ph->fstep[m->field_T]; }
+            const int r = mat->Rheology;
+            // CosseratMaterial::UpdatePlasticStrains: LE, VEP, dVEP (CosseratMaterial.cpp:451-506)
+            if (!(r == IFE_RHEO_LE || r == IFE_RHEO_VEP || r == IFE_RHEO_DVEP))
+                THROW(IFE_ERR_UNSUPPORTED, "rheology not defined for a Cosserat material (LE, VEP, dVEP are)");
+            m->HistSize = r == IFE_RHEO_LE ? 0 : 4;
+            if (!(mat->Split == IFE_SPLIT_NONE || (mat->Split == IFE_SPLIT_SPECSTRESS && !pf->ConsistentEnergy)))
+                THROW(IFE_ERR_UNSUPPORTED, "Cosserat energy split: NoSplit, or SpecStress without a consistent energy");
+            int dfn = pf->DistributionFunction;
+            if (!(dfn == IFE_DIST_LINEAR || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR2 || dfn == IFE_DIST_QUADRATIC ||
+                  dfn == IFE_DIST_LOCAL))
+                THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
+            if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE))
+                THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist and None are)");
+            if (mat->l_coss <= 0.0) THROW(IFE_ERR_INVALID, "l_coss must be positive");
         } else if (tn == "Thermal/ThermalModel") {
             if (ph->th_model()) THROW(IFE_ERR_UNSUPPORTED, "one Thermal/ThermalModel per physics");
             m->type = MT_THERMAL;

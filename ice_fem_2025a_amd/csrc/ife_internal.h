@@ -28,6 +28,10 @@ struct MatConst {
     // DPlast from the return-map Jacobian including d2F/dsigma2 (the consistent tangent)
     double rm_eabs, rm_erel;
     int rm_maxit, exact_tangent;
+    // Cosserat material (CosseratMaterial.cpp:5-10): internal length, couple modulus at
+    // porosity 0 (scaled by 1 - porosity like D_coss, IceMaterial.cpp:64-66), hardening
+    // reference strain
+    double l_coss, G_coss0, hard_eRef;
     double k, cp, Density_th;
     // PF utility options
     int DamFunction, DensityDamage, GravityDamage, ViscoDriving;
@@ -102,7 +106,7 @@ struct KParams {
     int do_u, do_ph, do_Tpf, do_th;
     int has_pf, has_th, has_T;
     StepIO su, sph, sT;
-    int fpos_u[3];           // position of ux,uy[,uz] among the u step's fields
+    int fpos_u[6];           // position of ux,uy[,uz][,ox,oy,oz] among the u step's fields
     int fpos_ph, fpos_T;
     int fpos_Tth;            // T position in its step (thermal model)
     // history [comp][e*nip + ip]
@@ -122,7 +126,12 @@ struct KParams {
     const int32_t* n2e_elem;
     const uint8_t* n2e_loc;  // local index of the node in that element
     int mixed;               // tangent contractions in FP32
+    // Cosserat model: per-IP coefficients [e][COSS_NF][nip]: w (Dam D_coss DPlast) row-major
+    // 18x18, w sigma (18), w DamI, w DamG (lumped inertia and gravity weights)
+    int coss;
+    double* cossb;
 };
+enum { COSS_C = 0, COSS_SIG = 324, COSS_WI = 342, COSS_WG = 343, COSS_NF = 344 };
 
 // One batched launch over nd domains: P[d] is domain d's KParams (device memory), bptr[k]
 // (device, nd + 1 entries) the prefix sums of the domains' block counts for kernel k
@@ -142,6 +151,10 @@ hipError_t launch_adjoint(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_spmv(const int64_t* row_ptr, const int32_t* col, const double* val, const double* x, double* y,
                        int64_t n_rows, hipStream_t s);
 hipError_t launch_gather(const KParams& p, int dim, hipStream_t s);
+// Cosserat: material update of every IP (u, phase, T coefficients), then the u-step element
+// kernel over one colour of elements (adds into the zeroed CSR values and f)
+hipError_t launch_coss_material(const KParams& p, hipStream_t s);
+hipError_t launch_coss_assemble(const KParams& p, hipStream_t s);
 int record_size(int dim);
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);

@@ -2098,6 +2098,683 @@ __global__ void copy_kernel(double* d, const double* s, int64_t n) {
     for (; i < n; i += stride) d[i] = s[i];
 }
 
+// ---------------------------------------------------------------------------------------
+// Cosserat model (Fracture/CossPhaseField, 3D): 18-component strain (xx yy zz xy yx yz zy xz
+// zx, then the nine curvatures), u-step dofs [ux uy uz ox oy oz] x 27 nodes.
+// ---------------------------------------------------------------------------------------
+// D_coss = dam * D_coss0 (CosseratMaterial.cpp:68-102, IceMaterial.cpp:66), D_coss0 from the
+// moduli of the input Young's modulus: normal block lambda + 2 mu / lambda, the pairs
+// (xy,yx) (yz,zy) (xz,zx) mu +- G_c, curvatures 2 mu.
+struct CossD {
+    double dl, dlo, sp, sm, s2;
+};
+__device__ __forceinline__ CossD coss_D(const MatConst& mc, double dam) {
+    const double Sh = mc.Young0 / (2.0 * (1 + mc.Poisson));
+    const double La = mc.Young0 * mc.Poisson / ((1.0 + mc.Poisson) * (1.0 - 2.0 * mc.Poisson));
+    return CossD{dam * (La + 2.0 * Sh), dam * La, dam * (Sh + mc.G_coss0), dam * (Sh - mc.G_coss0), dam * (2.0 * Sh)};
+}
+__device__ __forceinline__ double coss_Dij(const CossD& D, int i, int j) {
+    if (i < 3 && j < 3) return i == j ? D.dl : D.dlo;
+    if (i >= 9 || j >= 9) return i == j ? D.s2 : 0.0;
+    if (i < 3 || j < 3) return 0.0;
+    if (i == j) return D.sp;
+    return ((i - 3) >> 1) == ((j - 3) >> 1) ? D.sm : 0.0;  // (3,4) (5,6) (7,8)
+}
+// o = f D x
+__device__ __forceinline__ void coss_Dmul(const CossD& D, const double* x, double f, double* o) {
+    o[0] = f * (D.dl * x[0] + D.dlo * x[1] + D.dlo * x[2]);
+    o[1] = f * (D.dlo * x[0] + D.dl * x[1] + D.dlo * x[2]);
+    o[2] = f * (D.dlo * x[0] + D.dlo * x[1] + D.dl * x[2]);
+    for (int k = 3; k < 9; k += 2) {
+        o[k] = f * (D.sp * x[k] + D.sm * x[k + 1]);
+        o[k + 1] = f * (D.sm * x[k] + D.sp * x[k + 1]);
+    }
+    for (int k = 9; k < 18; k++) o[k] = f * (D.s2 * x[k]);
+}
+// J2Mat_Coss (CosseratMaterial.cpp:33-66)
+__device__ __forceinline__ double coss_J2ij(int i, int j) {
+    if (i < 3 && j < 3) return i == j ? 2.0 / 3.0 : -1.0 / 3.0;
+    if (i >= 9 || j >= 9) return i == j ? 1.0 : 0.0;
+    if (i < 3 || j < 3) return 0.0;
+    return (i - j == 0 || i - j == 3 || j - i == 3) ? 0.5 : 0.0;  // (3,6) (4,7) (5,8) and diagonal
+}
+// J2 of the scaled stress, dJ2 = J2Mat s
+__device__ double coss_J2(const double* S, double* dJ2) {
+    const double sc = 1.0e6;
+    double s[18];
+    for (int i = 0; i < 18; i++) s[i] = S[i] / sc;
+    dJ2[0] = 2.0 / 3.0 * s[0] - 1.0 / 3.0 * s[1] - 1.0 / 3.0 * s[2];
+    dJ2[1] = -1.0 / 3.0 * s[0] + 2.0 / 3.0 * s[1] - 1.0 / 3.0 * s[2];
+    dJ2[2] = -1.0 / 3.0 * s[0] - 1.0 / 3.0 * s[1] + 2.0 / 3.0 * s[2];
+    for (int k = 3; k < 6; k++) {
+        dJ2[k] = 0.5 * s[k] + 0.5 * s[k + 3];
+        dJ2[k + 3] = 0.5 * s[k] + 0.5 * s[k + 3];
+    }
+    for (int k = 9; k < 18; k++) dJ2[k] = s[k];
+    double J2 = 0;
+    for (int i = 0; i < 18; i++) J2 += s[i] * dJ2[i];
+    return 0.5 * J2;
+}
+
+// return_mapping_VEP_getKF (CosseratMaterial.cpp:671-728): K row-major 20x20 (scaled rows),
+// f; ddF_VE and ddG are the same function of the same stress, so the strain block is
+// scale1 (I + Damage (sol18 + sol19) ddG D_coss), with ddG = c/sc (J2Mat - dJ2 dJ2^T / (2 (J2+eps))).
+// Explicit hardening (h from the committed history).
+struct CossRM {
+    CossD D;
+    double sy, ACreep, nC, hard, hard_eRef, l, plv;
+};
+__device__ void coss_getKF(const CossRM& R, const double* sol, const double* et, const double* eo, double dt, double Damage,
+                           double h0, double (&K)[20][20], double (&f)[20]) {
+    const double sc = 1.0e6, eps = 1.0e-12, s1 = 1.0e3, s2 = -1.0e-6, s3 = -1.0e-6;
+    double S[18], dJ2[18], dFve[18], dG[18], tmp[18], ST[18], dJ2t[18];
+    coss_Dmul(R.D, sol, Damage, S);
+    const double J2 = coss_J2(S, dJ2);
+    const double rj = 1.0 / sqrt(J2 + eps);
+    const double Fve = sc * sqrt(2.0 * J2);
+    for (int i = 0; i < 18; i++) { dFve[i] = 0.5 * sqrt(2.0) * rj * dJ2[i]; dG[i] = 0.5 * sqrt(3.0) * rj * dJ2[i]; }
+    for (int i = 0; i < 18; i++) tmp[i] = et[i] - eo[i] - sol[18] * dFve[i];
+    coss_Dmul(R.D, tmp, Damage, ST);
+    const double J2t = coss_J2(ST, dJ2t);
+    const double FTrial = sc * sqrt(3.0 * J2t) - R.sy;
+    const double F = sc * sqrt(3.0 * J2) - R.sy;
+    const double hOld = R.hard * R.sy * (1.0 - exp(-h0 * R.l / R.hard_eRef));
+    const double h = hOld;  // e_eq = histOld[0] (explicit hardening)
+    for (int i = 0; i < 18; i++) f[i] = s1 * (sol[i] - et[i] + eo[i] + sol[18] * dFve[i] + sol[19] * dG[i]);
+    // strain block: H D with H = cH (J2Mat - b dJ2 dJ2^T): (J2Mat D)_ij - b dJ2_i (D dJ2)_j
+    const double cH = Damage * (sol[18] + sol[19]) * 0.5 / sc * sqrt(3.0) * rj, b = 0.5 / (J2 + eps);
+    double DdJ[18];
+    coss_Dmul(R.D, dJ2, 1.0, DdJ);
+#pragma unroll 1
+    for (int i = 0; i < 18; i++)
+#pragma unroll 1
+        for (int j = 0; j < 18; j++) {
+            double jd = 0;
+            for (int k = 0; k < 18; k++) jd += coss_J2ij(i, k) * coss_Dij(R.D, k, j);
+            K[i][j] = s1 * ((i == j ? 1.0 : 0.0) + cH * (jd - b * dJ2[i] * DdJ[j]));
+        }
+    for (int i = 0; i < 18; i++) { K[i][18] = s1 * dFve[i]; K[i][19] = s1 * dG[i]; }
+    const double PreFac = R.ACreep * dt * pow(sc, R.nC);
+    f[18] = s2 * (PreFac * pow(Fve / sc, R.nC) - sol[18]);
+    const double fac = R.nC * PreFac / sc * pow(Fve / sc, R.nC - 1.0);
+    double DdF[18];
+    coss_Dmul(R.D, dFve, Damage, DdF);
+    for (int j = 0; j < 18; j++) K[18][j] = s2 * fac * DdF[j];
+    K[18][18] = -s2;
+    K[18][19] = 0.0;
+    if (FTrial - hOld < 0.0) {
+        f[19] = -s3 * (R.l * R.plv + 1.0e-6) / dt * sol[19];
+        for (int j = 0; j < 19; j++) K[19][j] = 0.0;
+        K[19][19] = -s3 * (R.l * R.plv + 1.0e-6) / dt;
+    } else {
+        f[19] = s3 * (F - h - R.l * R.plv / dt * sol[19]);
+        double DdFp[18];
+        coss_Dmul(R.D, dG, Damage, DdFp);  // YieldFunction's dF equals dG at this stress
+        for (int j = 0; j < 18; j++) K[19][j] = s3 * DdFp[j];
+        K[19][18] = 0.0;
+        K[19][19] = s3 * (-R.l * R.plv / dt);
+    }
+}
+
+// dense 20x20 solve with partial pivoting, NR right-hand sides (loops kept rolled: the
+// matrices live in private memory); multipliers by division, updates and substitutions in
+// the order of an LU factorisation followed by forward and back substitution
+template <int NR>
+__device__ void lu_solve20(double (&A)[20][20], double (&B)[20][NR]) {
+#pragma unroll 1
+    for (int k = 0; k < 20; k++) {
+        int piv = k;
+        double amax = fabs(A[k][k]);
+        for (int r = k + 1; r < 20; r++)
+            if (fabs(A[r][k]) > amax) { amax = fabs(A[r][k]); piv = r; }
+        if (piv != k) {
+            for (int c = 0; c < 20; c++) { const double t = A[k][c]; A[k][c] = A[piv][c]; A[piv][c] = t; }
+            for (int c = 0; c < NR; c++) { const double t = B[k][c]; B[k][c] = B[piv][c]; B[piv][c] = t; }
+        }
+#pragma unroll 1
+        for (int r = k + 1; r < 20; r++) {
+            const double fr = A[r][k] / A[k][k];
+            for (int c = k + 1; c < 20; c++) A[r][c] -= fr * A[k][c];
+            for (int c = 0; c < NR; c++) B[r][c] -= fr * B[k][c];
+        }
+    }
+#pragma unroll 1
+    for (int k = 19; k >= 0; k--)
+        for (int c = 0; c < NR; c++) {
+            double t = B[k][c];
+            for (int j = k + 1; j < 20; j++) t -= A[k][j] * B[j][c];
+            B[k][c] = t / A[k][k];
+        }
+}
+
+// CosseratMaterial::return_mapping_VEP (:508-575) + _updateStrain (:730-769).  Returns the
+// iteration count; DM = scale1 (K^-1)(0:18,0:18) at the final iterate.
+__device__ int coss_return_map(const CossRM& R, const RMStop& stop, double dt, double Damage, const double* et,
+                               const double* eo, double* epl, double hist[4], const double h0[4], double (&DM)[18][18],
+                               double& diss, double& ddiss, int& mx) {
+    double sol[20], f[20], fOld[20], dsol[20];
+    double K[20][20];
+    for (int i = 0; i < 18; i++) sol[i] = et[i] - epl[i];
+    sol[18] = hist[1];
+    sol[19] = hist[2];
+    coss_getKF(R, sol, et, eo, dt, Damage, h0[0], K, f);
+    double e0 = 0;
+    for (int i = 0; i < 20; i++) e0 += f[i] * f[i];
+    e0 = fmax(e0, 1e-12);
+    int it = 0;
+    mx = 0;
+    double Ki[20][20];
+#pragma unroll 1
+    while (true) {
+        it++;
+        // dsol = -K^-1 f with the inverse formed first, as the reference does (:531)
+        for (int i = 0; i < 20; i++)
+            for (int j = 0; j < 20; j++) Ki[i][j] = i == j ? 1.0 : 0.0;
+        lu_solve20<20>(K, Ki);
+        for (int i = 0; i < 20; i++) {
+            double t = 0;
+            for (int j = 0; j < 20; j++) t += Ki[i][j] * f[j];
+            dsol[i] = -t;
+        }
+        for (int i = 0; i < 20; i++) { sol[i] += dsol[i]; fOld[i] = f[i]; }
+        coss_getKF(R, sol, et, eo, dt, Damage, h0[0], K, f);
+        double ls1 = 0, ls2 = 0;
+        for (int i = 0; i < 20; i++) { ls1 += f[i] * dsol[i]; ls2 += fOld[i] * dsol[i]; }
+        double ls;
+        if ((ls1 - ls2) == 0.0) ls = 1.0;
+        else ls = fmin(fmax(-(ls2) / (ls1 - ls2), 0.01), 1.0);
+        for (int i = 0; i < 20; i++) sol[i] = sol[i] - (1.0 - ls) * dsol[i];
+        coss_getKF(R, sol, et, eo, dt, Damage, h0[0], K, f);
+        double e = 0;
+        for (int i = 0; i < 20; i++) e += f[i] * f[i];
+        if (e < stop.eabs || e / e0 < stop.erel) break;
+        if (it > stop.maxit) { mx = 1; break; }
+        if (isnan(e) || isnan(e0) || (fabs(e / e0) > 1e6 && it > 25)) break;  // accepted as is (:562-568)
+    }
+    // updateStrain
+    double S[18], dJ2[18];
+    coss_Dmul(R.D, sol, Damage, S);
+    const double J2 = coss_J2(S, dJ2);
+    const double rj = 1.0 / sqrt(J2 + 1.0e-12);
+    double dv = 0, dp = 0;
+    for (int i = 0; i < 18; i++) {
+        const double ve = sol[18] * (0.5 * sqrt(2.0) * rj * dJ2[i]), pl = sol[19] * (0.5 * sqrt(3.0) * rj * dJ2[i]);
+        epl[i] = eo[i] + ve + pl;
+        dv += 1.0 / dt * fabs(ve * S[i]);
+        dp += 1.0 / dt * fabs(pl * S[i]);
+    }
+    hist[0] = h0[0] + sqrt(sol[19] * sol[19]);
+    hist[1] = sol[18];
+    hist[2] = sol[19];
+    hist[3] = h0[3] + sqrt(sol[18] * sol[18]);
+    diss = dv + dp;
+    ddiss = dp;
+    for (int i = 0; i < 20; i++)
+        for (int j = 0; j < 20; j++) Ki[i][j] = i == j ? 1.0 : 0.0;
+    lu_solve20<20>(K, Ki);
+    for (int i = 0; i < 18; i++)
+        for (int j = 0; j < 18; j++) DM[i][j] = 1.0e3 * Ki[i][j];
+    return it;
+}
+
+// CosseratMaterial::SpecStressSplit energy (:371-442): the positive part of the symmetrised
+// macro stress (|s_i| < 1e-9 replaced by (i+1) 1e-9) against the macro strain, at least 0
+__device__ double coss_spec_energy(const CossD& D, const double* ee) {
+    double S[18];
+    coss_Dmul(D, ee, 1.0, S);
+    double sn[9];
+    for (int i = 0; i < 9; i++) sn[i] = fabs(S[i]) < 1.0e-9 ? (i + 1) * 1.0e-9 : S[i];
+    const double SM[3][3] = {{sn[0], sn[3], sn[7]}, {sn[4], sn[1], sn[5]}, {sn[8], sn[6], sn[2]}};
+    double A[3][3], lam[3], V[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) A[i][j] = 0.5 * (SM[i][j] + SM[j][i]);
+    eig3(A, lam, V);
+    double P[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double t = 0;
+            for (int k = 0; k < 3; k++)
+                if (lam[k] > 0) t += V[i][k] * lam[k] * V[j][k];
+            P[i][j] = t;
+        }
+    const double sv[9] = {P[0][0], P[1][1], P[2][2], P[0][1], P[1][0], P[1][2], P[2][1], P[0][2], P[2][0]};
+    double E = 0;
+    for (int i = 0; i < 9; i++) E += sv[i] * ee[i];
+    return fmax(0.0, 0.5 * E);
+}
+
+struct CossSmem {
+    static constexpr int N = 27, NIP = 27, EPB = NT / 27;
+    double X[EPB][N][4];
+    double U[EPB][6][N], Uo[EPB][6][N], Ph[EPB][N], PhO[EPB][N], T[EPB][N];
+    double C1[EPB][3][9];
+    unsigned cnt[4];
+};
+
+// Material-point kernel of the Cosserat model: one lane per IP.  Shape functions and
+// geometry as the solid kernel forms them; the u step (CossPhaseField.cpp:388-461: 18-strain,
+// CosseratMaterial::UpdatePlasticStrains, energy split) writes w (Dam D_coss DPlast), w sigma
+// and the lumped weights to cossb; the phase step (:645-733) and the T step (:790-802) write
+// the same per-IP coefficients as the solid model for the shared element kernel.
+__global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
+    constexpr int N = 27, NIP = 27, EPB = CossSmem::EPB;
+    __shared__ CossSmem sm;
+    const int tid = threadIdx.x;
+    const MatConst& mc = p.mc;
+    const int64_t e0 = (int64_t)blockIdx.x * EPB;
+    if (tid < 4) sm.cnt[tid] = 0;
+    for (int i = tid; i < EPB * N; i += NT) {
+        const int le = i / N, a = i % N;
+        const int64_t el = e0 + le;
+        if (el >= p.NE) continue;
+        const int v = p.enode[el * N + a], r = p.erank[el * N + a];
+        const double* nd = p.nodes + (int64_t)v * 4;
+        for (int c = 0; c < 4; c++) sm.X[le][a][c] = nd[c];
+        for (int d = 0; d < 6; d++) {
+            const int64_t dd = (int64_t)p.fpos_u[d] * p.Ng + r;
+            sm.U[le][d][a] = p.su.x[dd];
+            sm.Uo[le][d][a] = p.su.xold[dd];
+        }
+        const int64_t dp = (int64_t)p.fpos_ph * p.Ng + r;
+        sm.Ph[le][a] = p.sph.x[dp];
+        sm.PhO[le][a] = p.sph.xold[dp];
+        sm.T[le][a] = p.has_T ? p.sT.x[(int64_t)p.fpos_T * p.Ng + r] : 0.0;
+    }
+    for (int i = tid; i < EPB * 27; i += NT) {
+        const int le = i / 27, k = i % 27;
+        const int64_t el = e0 + le;
+        if (el < p.NE) (&sm.C1[le][0][0])[k] = p.ext[el * 27 + k];
+    }
+    __syncthreads();
+    const int le = tid / NIP, ip = tid % NIP;
+    const int64_t el = e0 + le;
+    unsigned my_calls = 0, my_iters = 0, my_maxit = 0, my_bad = 0;
+    if (le < EPB && el < p.NE) {
+        const int64_t gi = el * NIP + ip, S = p.NIPT;
+        double* const ipo = p.ipb + el * IPB_NF * NIP + ip;
+        double* const cbo = p.cossb + el * COSS_NF * NIP + ip;
+        const int q[3] = {ip % 3, (ip / 3) % 3, ip / 9};
+        double b1[3][3], g1[3][3];
+        for (int d = 0; d < 3; d++)
+            for (int a = 0; a < 3; a++) {
+                double s = 0, sd = 0;
+                for (int k = 0; k < 3; k++) {
+                    s += sm.C1[le][d][a * 3 + k] * p.B1[q[d]][k];
+                    sd += sm.C1[le][d][a * 3 + k] * p.dB1[q[d]][k];
+                }
+                b1[d][a] = s;
+                g1[d][a] = sd;
+            }
+        // W-weighted sums against the nodal fields (see material_body)
+        enum { F_ONE = 0, F_X = 1, F_PH = 4, F_PHO, F_U, F_UO = F_U + 6, F_T = F_UO + 6, NFLD };
+        double SC[NFLD], SG[3][NFLD];
+        for (int k = 0; k < NFLD; k++) { SC[k] = 0; SG[0][k] = 0; SG[1][k] = 0; SG[2][k] = 0; }
+#pragma unroll 1
+        for (int a = 0; a < N; a++) {
+            const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+            const double cb = b1[0][ax] * b1[1][ay] * b1[2][az];
+            const double gb[3] = {g1[0][ax] * b1[1][ay] * b1[2][az], b1[0][ax] * g1[1][ay] * b1[2][az],
+                                  b1[0][ax] * b1[1][ay] * g1[2][az]};
+            const double Wa = sm.X[le][a][3];
+            double fv[NFLD];
+            fv[F_ONE] = 1.0;
+            for (int d = 0; d < 3; d++) fv[F_X + d] = sm.X[le][a][d];
+            fv[F_PH] = sm.Ph[le][a];
+            fv[F_PHO] = sm.PhO[le][a];
+            for (int d = 0; d < 6; d++) { fv[F_U + d] = sm.U[le][d][a]; fv[F_UO + d] = sm.Uo[le][d][a]; }
+            fv[F_T] = sm.T[le][a];
+            for (int k = 0; k < NFLD; k++) {
+                SC[k] += Wa * cb * fv[k];
+                for (int c = 0; c < 3; c++) SG[c][k] += Wa * gb[c] * fv[k];
+            }
+        }
+        const double wF = SC[F_ONE], rw = 1.0 / wF;
+        const double dW[3] = {SG[0][F_ONE], SG[1][F_ONE], SG[2][F_ONE]};
+        auto val = [&](int k) { return SC[k] * rw; };
+        auto rgrad = [&](int k, int c) { return SG[c][k] * rw - dW[c] * SC[k] * rw * rw; };
+        double J[3][3];
+        for (int c = 0; c < 3; c++)
+            for (int d = 0; d < 3; d++) J[c][d] = rgrad(F_X + d, c);
+        double Ji[3][3];
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                     c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c10 + J[0][2] * c20, id = 1.0 / det;
+        Ji[0][0] = c00 * id; Ji[1][0] = c10 * id; Ji[2][0] = c20 * id;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+        const double dt3 = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[1][0] * (J[0][1] * J[2][2] - J[0][2] * J[2][1]) +
+                           J[2][0] * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+        const double w = p.wq[q[0]] * p.wq[q[1]] * p.wq[q[2]] * fabs(dt3);
+        for (int c = 0; c < 3; c++) {
+            double qq = 0;
+            for (int d = 0; d < 3; d++) {
+                ipo[(IPB_JI + c * 3 + d) * NIP] = Ji[c][d] * rw;
+                qq += Ji[c][d] * dW[d];
+            }
+            ipo[(IPB_DW + c) * NIP] = qq * rw * rw;
+        }
+        ipo[IPB_WF * NIP] = rw;
+        ipo[IPB_W * NIP] = w;
+        auto grad = [&](int k, double g[3]) {
+            double r3[3];
+            for (int d = 0; d < 3; d++) r3[d] = rgrad(k, d);
+            for (int c = 0; c < 3; c++) g[c] = Ji[c][0] * r3[0] + Ji[c][1] * r3[1] + Ji[c][2] * r3[2];
+        };
+        const double phi = val(F_PH), phio = val(F_PHO);
+        double gphi[3], gT[3] = {0, 0, 0};
+        grad(F_PH, gphi);
+        const double Tu = p.has_T ? val(F_T) : mc.T_default;
+        if (p.has_T) grad(F_T, gT);
+        const double T0 = p.has_T ? p.sT.x[0] : 0.0;  // Assemble_Phase reads T dof 0 for every node (:575)
+        const double Tph = p.has_T ? T0 * val(F_ONE) : mc.T_default;
+        // 18-strain (ConstructBMatCoss): grad u, N w, l grad w
+        double gu[6][3], wv[3];
+        for (int a = 0; a < 6; a++) grad(F_U + a, gu[a]);
+        for (int k = 0; k < 3; k++) wv[k] = val(F_U + 3 + k);
+        const double lc = mc.l_coss;
+        double eps[18];
+        eps[0] = gu[0][0]; eps[1] = gu[1][1]; eps[2] = gu[2][2];
+        eps[3] = gu[1][0] - wv[2]; eps[4] = gu[0][1] + wv[2];
+        eps[5] = gu[2][1] - wv[0]; eps[6] = gu[1][2] + wv[0];
+        eps[7] = gu[2][0] - wv[1]; eps[8] = gu[0][2] + wv[1];
+        eps[9] = lc * gu[3][0]; eps[10] = lc * gu[4][1]; eps[11] = lc * gu[5][2];
+        eps[12] = lc * gu[3][0]; eps[13] = lc * gu[4][1]; eps[14] = lc * gu[4][1];
+        eps[15] = lc * gu[5][2]; eps[16] = lc * gu[3][0]; eps[17] = lc * gu[5][2];
+        double epl[18];
+        if (p.do_u || p.do_ph)
+            for (int r = 0; r < 18; r++) epl[r] = p.PS[r * S + gi];
+        MatIP m;
+        double VD = 0;
+        if (p.do_u) {
+            set_temperature(mc, Tu, 0.0, m, p.A_e ? p.A_e[el] : mc.A0);
+            const CossD D = coss_D(mc, m.dam);
+            double Dam, d1, d2, DamI, DamG;
+            damage(mc.DamFunction, phi, mc.k0, mc.p, Dam, d1, d2);
+            damage(mc.DensityDamage, phi, mc.k0, mc.p, DamI, d1, d2);
+            damage(mc.GravityDamage, phi, mc.k0, mc.p, DamG, d1, d2);
+            double epo[18];
+            for (int r = 0; r < 18; r++) epo[r] = p.PSOld[r * S + gi];
+            double DM[18][18];
+            double diss = 0, ddiss = 0;
+            bool elastic = true;
+            const int rh = mc.Rheology;
+            if ((rh == IFE_RHEO_VEP || rh == IFE_RHEO_DVEP) && !(phi > mc.PhiViscLim)) {
+                double dmg = 1.0;
+                if (rh == IFE_RHEO_DVEP) damage(mc.DamFunction, fmin(1.0, fmax(0.0, phi)), mc.k0, mc.p, dmg, d1, d2);
+                const CossRM R{D, m.sy, m.ACreep, mc.nCreep, mc.hardening, mc.hard_eRef, lc, mc.pl_visc};
+                double hist[4] = {p.MH[gi], p.MH[S + gi], p.MH[2 * S + gi], p.MH[3 * S + gi]};
+                const double h0[4] = {p.MHOld[gi], p.MHOld[S + gi], p.MHOld[2 * S + gi], p.MHOld[3 * S + gi]};
+                const RMStop stop{mc.rm_eabs >= 0 ? mc.rm_eabs : 1.0e-6, mc.rm_erel >= 0 ? mc.rm_erel : 1.0e-5,
+                                  mc.rm_maxit > 0 ? mc.rm_maxit : 500};
+                int mx = 0;
+                const int it = coss_return_map(R, stop, p.dt, dmg, eps, epo, epl, hist, h0, DM, diss, ddiss, mx);
+                for (int k = 0; k < 4; k++) p.MH[k * S + gi] = hist[k];
+                my_calls++;
+                my_iters += it;
+                my_maxit += mx;
+                elastic = false;
+            } else {
+                for (int r = 0; r < 18; r++) epl[r] = 1.0 * epo[r];
+            }
+            for (int r = 0; r < 18; r++) p.PS[r * S + gi] = epl[r];
+            VD = p.VDOld[gi] + diss * p.dt;
+            p.VD[gi] = VD;
+            p.DVD[gi] = p.DVDOld[gi] + ddiss * p.dt;
+            double ee[18], sig[18];
+            for (int r = 0; r < 18; r++) ee[r] = eps[r] - epl[r];
+            coss_Dmul(D, ee, 1.0, sig);  // energy not consistent: S_Dam = D_coss ee, S_Undam = 0
+            for (int r = 0; r < 18; r++) {
+                const double v = w * (Dam * sig[r]);
+                my_bad |= !isfinite(v);
+                cbo[(COSS_SIG + r) * NIP] = v;
+            }
+            // w (Dam D_coss) DPlast
+#pragma unroll 1
+            for (int j = 0; j < 18; j++) {
+                double col[18], o[18];
+                for (int k = 0; k < 18; k++) col[k] = elastic ? (k == j ? 1.0 : 0.0) : DM[k][j];
+                coss_Dmul(D, col, Dam, o);
+                for (int i = 0; i < 18; i++) cbo[(COSS_C + i * 18 + j) * NIP] = w * o[i];
+            }
+            cbo[COSS_WI * NIP] = w * DamI;
+            cbo[COSS_WG * NIP] = w * DamG;  // (1 - porosity) = 1
+        } else if (p.has_pf && p.do_Tpf) {
+            VD = p.VD[gi];
+        }
+        if (p.do_ph) {
+            set_temperature(mc, Tph, 0.0, m, p.A_e ? p.A_e[el] : mc.A0);
+            if (p.Gc_e) m.Gc = p.Gc_e[el];
+            const CossD D = coss_D(mc, m.dam);
+            double Dam, dDam, ddDam, DamI, dDamI, ddDamI, DamG, dDamG, ddDamG;
+            damage(mc.DamFunction, phi, mc.k0, mc.p, Dam, dDam, ddDam);
+            damage(mc.DensityDamage, phi, mc.k0, mc.p, DamI, dDamI, ddDamI);
+            damage(mc.GravityDamage, phi, mc.k0, mc.p, DamG, dDamG, ddDamG);
+            double ee[18];
+            for (int r = 0; r < 18; r++) ee[r] = eps[r] - epl[r];
+            double Energy;
+            if (mc.Split == IFE_SPLIT_NONE) {
+                double s18[18];
+                coss_Dmul(D, ee, 1.0, s18);
+                Energy = 0;
+                for (int r = 0; r < 18; r++) Energy += s18[r] * ee[r];
+                Energy *= 0.5;
+            } else {
+                Energy = coss_spec_energy(D, ee);
+            }
+            if (p.t < 0.0) Energy = 0.0;  // (:682-684)
+            if (mc.Irreversible == IFE_IRR_HIST) {
+                const double Ho = p.EOld[gi];
+                if (Ho > Energy) Energy = Ho;
+            }
+            p.E[gi] = Energy;
+            double gg = 0;
+            for (int c = 0; c < 3; c++) gg += gphi[c] * gphi[c];
+            double dN, cNN, cGG, gamma;
+            dist_coeffs(mc.Distribution, mc.l, phi, gg, dN, cNN, cGG, gamma);
+            double A = w * m.Gc * cNN, FN = w * m.Gc * dN;
+            FN += w * dDam * Energy;
+            A += w * ddDam * Energy;
+            if (mc.ViscoDrivingFactor > 1.0e-9) {
+                double DamV, dDamV, ddDamV;
+                const double PD = mc.ViscoDrivingFactor * p.DVD[gi];
+                damage(mc.ViscoDriving, phi, mc.k0, mc.p, DamV, dDamV, ddDamV);
+                FN += w * dDamV * PD;
+                A += w * ddDamV * PD;
+            }
+            if (mc.Inertia && mc.ConsistentDegradation) {  // six velocity components
+                double v2 = 0;
+                for (int d = 0; d < 6; d++) {
+                    double vd = 0;
+                    for (int a = 0; a < N; a++) {
+                        const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+                        const double Na = sm.X[le][a][3] * b1[0][ax] * b1[1][ay] * b1[2][az] * rw;
+                        vd += Na * p.su.dx[(int64_t)p.fpos_u[d] * p.Ng + p.erank[el * N + a]];
+                    }
+                    v2 += vd * vd;
+                }
+                FN += -w * 0.5 * mc.Density * v2 * dDamI;
+                A += -w * 0.5 * mc.Density * v2 * ddDamI;
+            }
+            if (mc.ConsistentDegradation) {
+                double gu_ = 0;
+                for (int d = 0; d < 3; d++) gu_ += mc.g[d] * val(F_U + d);
+                FN += -w * mc.Density * gu_ * dDamG;
+                A += -w * mc.Density * gu_ * ddDamG;
+            }
+            ipo[(IPB_PHA) * NIP] = A;
+            ipo[(IPB_PHB) * NIP] = w * m.Gc * cGG;
+            ipo[(IPB_PHFN) * NIP] = FN;
+            for (int c = 0; c < 3; c++) ipo[(IPB_PHFG + c) * NIP] = w * m.Gc * cGG * gphi[c];
+            ipo[(IPB_LDRIVE) * NIP] = 0.0;
+            ipo[(IPB_GW) * NIP] = gamma * w;
+            ipo[(IPB_GC) * NIP] = m.Gc;
+            my_bad |= !isfinite(A) || !isfinite(FN);
+        }
+        if (p.do_Tpf || p.do_th) {
+            double fn = 0;
+            if (p.do_Tpf) fn = -w * ((VD - p.VDOld[gi]) / p.dt);
+            ipo[(IPB_THFN) * NIP] = fn;
+            ipo[(IPB_THB) * NIP] = p.do_th ? w * mc.k : 0.0;
+            for (int c = 0; c < 3; c++) ipo[(IPB_THFG + c) * NIP] = p.do_th ? w * mc.k * gT[c] : 0.0;
+        }
+    }
+    if (my_calls) {
+        atomicAdd(&sm.cnt[0], my_calls);
+        atomicAdd(&sm.cnt[1], my_iters);
+        if (my_maxit) atomicAdd(&sm.cnt[2], my_maxit);
+    }
+    if (my_bad) atomicOr(&sm.cnt[3], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        const int64_t ne = p.NE - e0 < EPB ? p.NE - e0 : EPB;
+        atomicAdd(&p.st->elements, (unsigned long long)ne);
+        if (sm.cnt[0]) {
+            atomicAdd(&p.st->rm_calls, (unsigned long long)sm.cnt[0]);
+            atomicAdd(&p.st->rm_iters, (unsigned long long)sm.cnt[1]);
+        }
+        if (sm.cnt[2]) atomicAdd(&p.st->rm_maxit, (unsigned long long)sm.cnt[2]);
+        if (sm.cnt[3]) atomicOr(&p.st->nonfinite, 1u);
+    }
+}
+
+// B-matrix rows of each u-step field (ConstructBMatCoss, CossPhaseField.cpp:1184-1221):
+// (row, basis k: 0 N, 1..3 G_x..G_z, coefficient: 0 +1, 1 -1, 2 l_coss)
+__constant__ const signed char coss_nrow[6] = {3, 3, 3, 5, 5, 5};
+__constant__ const signed char coss_row[6][5] = {
+    {0, 4, 8, 0, 0}, {1, 3, 6, 0, 0}, {2, 5, 7, 0, 0}, {5, 6, 9, 12, 16}, {7, 8, 10, 13, 14}, {3, 4, 11, 15, 17}};
+__constant__ const signed char coss_bas[6][5] = {
+    {1, 2, 3, 0, 0}, {2, 1, 3, 0, 0}, {3, 2, 1, 0, 0}, {0, 0, 1, 1, 1}, {0, 0, 2, 2, 2}, {0, 0, 3, 3, 3}};
+__constant__ const signed char coss_cof[6][5] = {
+    {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {1, 0, 2, 2, 2}, {1, 0, 2, 2, 2}, {1, 0, 2, 2, 2}};
+
+// Cosserat u-step element kernel: one workgroup per element of one colour (no two elements
+// of a colour share a node, so the element adds into its rows of the zeroed CSR values and
+// f without atomics).  K_ab(alpha a, beta b) = sum_ip sum_kl Phi_k(a) M^ab_kl Phi_l(b) with
+// Phi = (N, G_x, G_y, G_z) and M^ab = P_a^T (w Dam D_coss DPlast) P_b, P_a the field's
+// B-matrix rows; one field pair at a time, its 4x4 blocks for all IPs staged in LDS.
+__global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
+    constexpr int N = 27, NIP = 27, NP = 28;
+    __shared__ double Phi[NIP][4][NP];
+    __shared__ double Mb[NIP][16];
+    __shared__ double Xs[N][4];
+    __shared__ double b1s[3][3][3], db1s[3][3][3];
+    __shared__ int32_t rk[N];
+    __shared__ int64_t rp0[N], rdeg[N];
+    const int tid = threadIdx.x;
+    const int64_t el = p.elist[blockIdx.x];
+    const double* const ipc = p.ipb + el * IPB_NF * NIP;
+    const double* const cb = p.cossb + el * COSS_NF * NIP;
+    const MatConst& mc = p.mc;
+    if (tid < N) {
+        const int v = p.enode[el * N + tid], r = p.erank[el * N + tid];
+        const double* nd = p.nodes + (int64_t)v * 4;
+        for (int c = 0; c < 4; c++) Xs[tid][c] = nd[c];
+        rk[tid] = r;
+        rp0[tid] = p.nbr_ptr[r];
+        rdeg[tid] = p.nbr_ptr[r + 1] - p.nbr_ptr[r];
+    }
+    if (tid < 27) {
+        const int d = tid / 9, q = (tid / 3) % 3, a = tid % 3;
+        const double* C = p.ext + (el * 3 + d) * 9;
+        double s = 0, sd = 0;
+        for (int k = 0; k < 3; k++) { s += C[a * 3 + k] * p.B1[q][k]; sd += C[a * 3 + k] * p.dB1[q][k]; }
+        b1s[d][q][a] = s;
+        db1s[d][q][a] = sd;
+    }
+    __syncthreads();
+    for (int i = tid; i < NIP * NP; i += NT) {
+        const int ip = i / NP, a = i % NP;
+        if (a >= N) { for (int k = 0; k < 4; k++) Phi[ip][k][a] = 0.0; continue; }
+        const int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9, ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+        const double bx = b1s[0][qx][ax], by = b1s[1][qy][ay], bz = b1s[2][qz][az];
+        const double gx = db1s[0][qx][ax], gy = db1s[1][qy][ay], gz = db1s[2][qz][az];
+        const double cbv = bx * by * bz, gb[3] = {gx * by * bz, bx * gy * bz, bx * by * gz};
+        const double Wa = Xs[a][3];
+        Phi[ip][0][a] = Wa * cbv * ipc[IPB_WF * NIP + ip];
+        for (int c = 0; c < 3; c++) {
+            double s = 0;
+            for (int d = 0; d < 3; d++) s += ipc[(IPB_JI + c * 3 + d) * NIP + ip] * gb[d];
+            Phi[ip][1 + c][a] = Wa * (s - cbv * ipc[(IPB_DW + c) * NIP + ip]);
+        }
+    }
+    __syncthreads();
+    const double lc = mc.l_coss;
+    auto cof = [&](int c) { return c == 0 ? 1.0 : (c == 1 ? -1.0 : lc); };
+    double bad = 0.0;
+#pragma unroll 1
+    for (int blk = 0; blk < 36; blk++) {
+        const int al = blk / 6, be = blk % 6;
+        for (int i = tid; i < NIP * 16; i += NT) {
+            const int ip = i / 16, k = (i / 4) % 4, l = i % 4;
+            double m = 0;
+            for (int x = 0; x < coss_nrow[al]; x++) {
+                if (coss_bas[al][x] != k) continue;
+                for (int y = 0; y < coss_nrow[be]; y++) {
+                    if (coss_bas[be][y] != l) continue;
+                    m += cof(coss_cof[al][x]) * cof(coss_cof[be][y]) *
+                         cb[(COSS_C + coss_row[al][x] * 18 + coss_row[be][y]) * NIP + ip];
+                }
+            }
+            Mb[ip][i % 16] = m;
+        }
+        __syncthreads();
+        for (int e = tid; e < N * N; e += NT) {
+            const int a = e / N, b = e % N;
+            double acc = 0;
+#pragma unroll 3
+            for (int ip = 0; ip < NIP; ip++) {
+                const double pb0 = Phi[ip][0][b], pb1 = Phi[ip][1][b], pb2 = Phi[ip][2][b], pb3 = Phi[ip][3][b];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const double t = Mb[ip][k * 4] * pb0 + Mb[ip][k * 4 + 1] * pb1 + Mb[ip][k * 4 + 2] * pb2 +
+                                     Mb[ip][k * 4 + 3] * pb3;
+                    acc += Phi[ip][k][a] * t;
+                }
+            }
+            if (al == be && a == b) {  // lumped inertia / damping on the diagonal (:474-520)
+                double WI = 0;
+                for (int ip = 0; ip < NIP; ip++) WI += cb[COSS_WI * NIP + ip] * Phi[ip][0][a];
+                const double l2 = lc * lc;
+                if (mc.Inertia) acc += WI * (al == 5 ? 2 * l2 * mc.Density / (1 + mc.Poisson) : mc.Density) * p.ddu_dt;
+                if (mc.Damping > 0.0) acc += WI * (al >= 3 ? 2 * l2 * mc.Damping : mc.Damping) * p.du_dt;
+            }
+            bad += acc - acc;
+            double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * rp0[a] +
+                        (int64_t)p.fpos_u[be] * rdeg[a] + p.loc[(el * N + a) * N + b];
+            *K += acc;
+        }
+        __syncthreads();
+    }
+    // F_U (:454-455) and the lumped nodal terms (:474-527)
+    for (int t = tid; t < 6 * N; t += NT) {
+        const int al = t / N, a = t % N;
+        double acc = 0;
+        for (int ip = 0; ip < NIP; ip++)
+            for (int x = 0; x < coss_nrow[al]; x++)
+                acc += cof(coss_cof[al][x]) * cb[(COSS_SIG + coss_row[al][x]) * NIP + ip] * Phi[ip][coss_bas[al][x]][a];
+        double WI = 0, WG = 0;
+        for (int ip = 0; ip < NIP; ip++) {
+            WI += cb[COSS_WI * NIP + ip] * Phi[ip][0][a];
+            WG += cb[COSS_WG * NIP + ip] * Phi[ip][0][a];
+        }
+        const int64_t dof = (int64_t)p.fpos_u[al] * p.Ng + rk[a];
+        const double l2 = lc * lc;
+        if (mc.Inertia) {
+            const double rhoC = 2 * l2 * mc.Density / (1 + mc.Poisson);
+            acc += WI * (al < 3 ? mc.Density : rhoC) * p.su.ddx[dof];
+        }
+        if (mc.Damping > 0.0) acc += WI * (al < 3 ? mc.Damping : 2 * l2 * mc.Damping) * p.su.dx[dof];
+        if (al < 3) acc += -WG * mc.Density * mc.g[al];
+        bad += acc - acc;
+        p.su.f[dof] += acc;
+    }
+    if (bad != 0.0) atomicOr(&p.st->nonfinite, 1u);
+}
+
 }  // namespace
 
 hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
@@ -2123,6 +2800,19 @@ hipError_t launch_gather(const KParams& p, int dim, hipStream_t s) {
 }
 
 int record_size(int dim) { return dim == 2 ? Rec<2>::SIZE : Rec<3>::SIZE; }
+
+hipError_t launch_coss_material(const KParams& p, hipStream_t s) {
+    const int64_t blocks = (p.NE + CossSmem::EPB - 1) / CossSmem::EPB;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(coss_material_kernel, dim3((unsigned)blocks), dim3(NT), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_coss_assemble(const KParams& p, hipStream_t s) {
+    if (p.nlist <= 0) return hipSuccess;
+    hipLaunchKernelGGL(coss_assemble_kernel, dim3((unsigned)p.nlist), dim3(NT), 0, s, p);
+    return hipGetLastError();
+}
 
 hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
     const int nip = dim == 2 ? 9 : 27, epb = NT / nip;

@@ -95,7 +95,8 @@ def test_dense_extraction_is_factored_and_non_kronecker_rejected():
 
 
 @pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (5, 3)), ("quad2d_le_at2", (1, 1)),
-                                      ("hex3d_glen_at2", (3, 2, 4)), ("hex3d_glen_at2", (1, 1, 1))])
+                                      ("hex3d_glen_at2", (3, 2, 4)), ("hex3d_glen_at2", (1, 1, 1)),
+                                      ("hex3d_coss", (3, 2, 4)), ("quad2d_thermal", (4, 3))])
 def test_numbering_and_pattern_match_oracle(name, nel):
     prob = make_problem(name, nel=nel)
     e = Physics(prob, device=False)
@@ -197,3 +198,27 @@ def test_batch_workload_sizes_are_seeded():
     assert nel.min() >= 1900 and nel.max() <= 62000 and 0.9e6 < nel.sum() < 1.1e6
     assert ratio.min() >= 0.8 and ratio.max() <= 4.5
     assert batch_sizes() == sz
+
+
+def test_cosserat_model_checks():
+    """Fracture/CossPhaseField: 3D only, LE / VEP / dVEP, NoSplit or non-consistent SpecStress."""
+    lib = abi.load_library()
+    from ice_fem_2025a_amd.engine import _mesh_desc, _dofs_desc
+    prob = make_problem("hex3d_coss", nel=(1, 1, 1))
+    keep = []
+    md = _mesh_desc(prob.mesh, keep)
+    dd = _dofs_desc(prob.dof_names, prob.dof_steps, keep)
+    m = prob.models[0]
+    for field, value, code in (("Rheology", abi.RHEOLOGY["ViscoElastic"], 2), ("Split", abi.SPLIT["VolStrains"], 2)):
+        h = C.c_void_p()
+        assert lib.ife_physics_create(C.byref(md), C.byref(dd), C.byref(h)) == abi.IFE_OK
+        bad = abi.Material.from_buffer_copy(m.material)
+        setattr(bad, field, value)
+        assert lib.ife_physics_add_model(h, b"Fracture/CossPhaseField", b"x", 0, C.byref(bad), C.byref(m.pf),
+                                         C.byref(m.opts), None) == code
+        lib.ife_physics_destroy(h)
+    h = C.c_void_p()
+    assert lib.ife_physics_create(C.byref(md), C.byref(dd), C.byref(h)) == abi.IFE_OK
+    assert lib.ife_physics_add_model(h, b"Fracture/CossPhaseField", b"x", 0, C.byref(m.material), C.byref(m.pf),
+                                     C.byref(m.opts), None) == abi.IFE_OK
+    lib.ife_physics_destroy(h)
