@@ -982,8 +982,9 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
                 THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
             if (opts->LumpedPFCapacity && !(dfn == IFE_DIST_QUADRATIC || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR))
                 THROW(IFE_ERR_UNSUPPORTED, "lumped capacity form not implemented for this distribution");
-            if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE))
-                THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist and None are)");
+            if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE ||
+                  opts->Irreversible == IFE_IRR_PENALTY))
+                THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist, None and Penalty are)");
             int dams[] = {pf->DamFunction, pf->DensityDamage, pf->GravityDamage, pf->ViscoDriving};
             for (int d : dams)
                 if (d < IFE_DAM_NONE || d > IFE_DAM_PORDER) THROW(IFE_ERR_INVALID, "unknown damage function");
@@ -1020,8 +1021,9 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             if (!(dfn == IFE_DIST_LINEAR || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR2 || dfn == IFE_DIST_QUADRATIC ||
                   dfn == IFE_DIST_LOCAL))
                 THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
-            if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE))
-                THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist and None are)");
+            if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE ||
+                  opts->Irreversible == IFE_IRR_PENALTY))
+                THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist, None and Penalty are)");
             if (mat->l_coss <= 0.0) THROW(IFE_ERR_INVALID, "l_coss must be positive");
         } else if (tn == "Thermal/ThermalModel") {
             if (ph->th_model()) THROW(IFE_ERR_UNSUPPORTED, "one Thermal/ThermalModel per physics");

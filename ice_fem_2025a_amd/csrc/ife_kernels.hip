@@ -1741,6 +1741,14 @@ __device__ __forceinline__ void nodal_terms(Smem<DIM, T>& sm, const KParams& p, 
         }
         kd += s * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
         f += s * mc.pf_visc / p.dt * (sm.Ph[a] - sm.PhO[a]);
+        if (mc.Irreversible == IFE_IRR_PENALTY) {
+            // PhaseFieldUtility::PenaltyIrr (PhaseFieldUtil.cpp:350-381): limits max(0, phi_old)
+            // (at most 1 - 1e-3) and 1.1, the input Gc, PenaltyFactor 1e6 (PhaseFieldUtil.h:75)
+            const double lo = fmin(1.0 - 1.0e-3, fmax(0.0, (double)sm.PhO[a])), hi = 1.0 + 1e-1, pen = 1.0e6;
+            const double ph = sm.Ph[a];
+            if (ph < lo) { f += s * mc.Gc_in * pen * (ph - lo); kd += s * mc.Gc_in * pen; }
+            if (ph > hi) { f += s * mc.Gc_in * pen * (ph - hi); kd += s * mc.Gc_in * pen; }
+        }
         sm.kdp[a] = kd;
         sm.fnp[a] = f;
     }

@@ -1873,7 +1873,27 @@ static void Assemble_U(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, St
     }
 }
 
-// ---- PhaseFieldDamage::Assemble_Phase (PhaseField.cpp:485-731), Hist/None irreversibility
+// PhaseFieldUtility::EnforceIrreversible (PhaseFieldUtil.cpp:350-370): no nodal terms with
+// Hist/None; PenaltyIrr (:372-381) with the limits min(1 - 1e-3, max(0, PhaseOld)) and 1.1,
+// the input Gc of the fracture properties and PenaltyFactor 1e6 (PhaseFieldUtil.h:75)
+static void EnforcePenalty(const Model& md, const Material& M, const Vec& Phase, const Vec& PhaseOld, const Vec& WLumped,
+                           Vec& F_Ph, Mat& K_PhPh) {
+    if (md.opt.Irreversible != IFE_IRR_PENALTY) return;
+    const double Gc = M.in.Gc, pen = 1.0e6;
+    for (size_t n = 0; n < Phase.size(); n++) {
+        const double lim0 = std::min(1.0 - 1.0e-3, std::max(0.0, PhaseOld[n])), lim1 = 1.0 + 1e-1;
+        if (Phase[n] < lim0) {
+            F_Ph[n] += WLumped[n] * Gc * pen * (Phase[n] - lim0);
+            K_PhPh(n, n) += WLumped[n] * Gc * pen;
+        }
+        if (Phase[n] > lim1) {
+            F_Ph[n] += WLumped[n] * Gc * pen * (Phase[n] - lim1);
+            K_PhPh(n, n) += WLumped[n] * Gc * pen;
+        }
+    }
+}
+
+// ---- PhaseFieldDamage::Assemble_Phase (PhaseField.cpp:485-731), Hist/None/Penalty irreversibility
 static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats& st) {
     const int n = c.n, nip = c.nip, dim = c.dim, nu = dim * n;
     Material M = md.M;
@@ -2007,7 +2027,7 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
             F_Ph[a] += WLumped[a] * M.pf_visc / dt * (Phase[a] - PhaseOld[a]);
             K_PhPh(a, a) += WLumped[a] * (M.pf_visc / dt + md.opt.ViscousStab * M.Gc / M.pf_l);
         }
-        // EnforceIrreversible with Hist/None: no nodal terms (PhaseFieldUtil.cpp:350-370)
+        EnforcePenalty(md, M, Phase, PhaseOld, WLumped, F_Ph, K_PhPh);
         sink.vec(dofsPhase, F_Ph);
         sink.mat(dofsPhase, dofsPhase, K_PhPh);
     }
@@ -2295,6 +2315,7 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
             F_Ph[a] += WLumped[a] * M.pf_visc / dt * (Phase[a] - PhaseOld[a]);
             K_PhPh(a, a) += WLumped[a] * (M.pf_visc / dt + md.opt.ViscousStab * M.Gc / M.pf_l);
         }
+        EnforcePenalty(md, M, Phase, PhaseOld, WLumped, F_Ph, K_PhPh);
         sink.vec(dofsPhase, F_Ph);
         sink.mat(dofsPhase, dofsPhase, K_PhPh);
     }
@@ -2631,7 +2652,8 @@ void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_mod
                     if (c->dim == 3) { md.g[0] = 0; md.g[1] = 0; md.g[2] = -9.81; }
                     else { md.g[0] = 0; md.g[1] = -9.81; md.g[2] = 0; }
                 }
-                if (!(md.opt.Irreversible == IFE_IRR_HIST || md.opt.Irreversible == IFE_IRR_NONE))
+                if (!(md.opt.Irreversible == IFE_IRR_HIST || md.opt.Irreversible == IFE_IRR_NONE ||
+                      md.opt.Irreversible == IFE_IRR_PENALTY))
                     throw std::invalid_argument("irreversibility method not supported by the oracle");
                 size_t NIPT = (size_t)c->NE * c->nip;
                 md.Energy.assign(NIPT, 0); md.EnergyOld.assign(NIPT, 0);
