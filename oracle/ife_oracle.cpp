@@ -183,6 +183,8 @@ struct ElemType {
     std::vector<double> w;      // nip  (IPScheme.cpp:9-56)
     std::vector<double> Nbase;  // nip x n
     std::vector<double> Gbase;  // nip x dim x n
+    std::vector<double> G2base; // nip x ng2 x n: 2D xx yy xy; 3D xx yy zz xy yz xz
+    int ng2 = 0;
 
     void init(int dim_, int ip1d) {  // B2_2.cpp:16-70, NURBS3_Cube2.cpp:17-87
         dim = dim_;
@@ -193,6 +195,8 @@ struct ElemType {
         w.assign(nip, 0);
         Nbase.assign((size_t)nip * n, 0);
         Gbase.assign((size_t)nip * dim * n, 0);
+        ng2 = dim == 2 ? 3 : 6;
+        G2base.assign((size_t)nip * ng2 * n, 0);
         int cnt = 0;
         int nk = (dim == 2) ? 1 : ip1d;
         for (int k = 0; k < nk; k++)
@@ -214,6 +218,19 @@ struct ElemType {
                                 Gbase[((size_t)cnt * dim + 0) * n + c] = (dim == 2) ? Gx[ii] * Ny[jj] : Gx[ii] * Ny[jj] * Nz[kk];
                                 Gbase[((size_t)cnt * dim + 1) * n + c] = (dim == 2) ? Nx[ii] * Gy[jj] : Nx[ii] * Gy[jj] * Nz[kk];
                                 if (dim == 3) Gbase[((size_t)cnt * dim + 2) * n + c] = Nx[ii] * Ny[jj] * Gz[kk];
+                                double* g2 = &G2base[(size_t)cnt * ng2 * n + c];
+                                if (dim == 2) {  // B2_2.cpp:42-48
+                                    g2[0 * n] = G2x[ii] * Ny[jj];
+                                    g2[1 * n] = Nx[ii] * G2y[jj];
+                                    g2[2 * n] = Gx[ii] * Gy[jj];
+                                } else {         // NURBS3_Cube2.cpp:46-58
+                                    g2[0 * n] = G2x[ii] * Ny[jj] * Nz[kk];
+                                    g2[1 * n] = Nx[ii] * G2y[jj] * Nz[kk];
+                                    g2[2 * n] = Nx[ii] * Ny[jj] * G2z[kk];
+                                    g2[3 * n] = Gx[ii] * Gy[jj] * Nz[kk];
+                                    g2[4 * n] = Nx[ii] * Gy[jj] * Gz[kk];
+                                    g2[5 * n] = Gx[ii] * Ny[jj] * Gz[kk];
+                                }
                                 c++;
                             }
                     cnt++;
@@ -223,17 +240,47 @@ struct ElemType {
 
 // Output of one getShapeGrads call: N[ip][a], G[ip][c][a], w[ip]
 struct Shapes {
-    int nip = 0, n = 0, dim = 0;
-    std::vector<double> N, G, w;
+    int nip = 0, n = 0, dim = 0, ng2 = 0;
+    std::vector<double> N, G, G2, w;  // G2 filled by getShapeGrads2 only
     void resize(int nip_, int n_, int dim_) {
         nip = nip_; n = n_; dim = dim_;
+        ng2 = dim == 2 ? 3 : 6;
         N.assign((size_t)nip * n, 0);
         G.assign((size_t)nip * dim * n, 0);
+        G2.assign((size_t)nip * ng2 * n, 0);
         w.assign(nip, 0);
     }
     double Nv(int ip, int a) const { return N[(size_t)ip * n + a]; }
     double Gv(int ip, int c, int a) const { return G[((size_t)ip * dim + c) * n + a]; }
+    double G2v(int ip, int r, int a) const { return G2[((size_t)ip * ng2 + r) * n + a]; }
 };
+
+// J2 of the second-derivative transformation (the reference's explicit formula):
+// 2D B2_2.cpp:102-113, 3D NURBS3_Cube2.cpp:169-207; J(xi, x) row-major
+static void J2matrix(int dim, const double J[3][3], double* J2 /* ng2 x ng2 row-major */) {
+    if (dim == 2) {
+        double (*M)[3] = (double (*)[3])J2;
+        M[0][0] = J[0][0] * J[0][0]; M[0][1] = J[0][1] * J[0][1]; M[0][2] = 2 * J[0][0] * J[0][1];
+        M[1][0] = J[1][0] * J[1][0]; M[1][1] = J[1][1] * J[1][1]; M[1][2] = 2 * J[1][0] * J[1][1];
+        M[2][0] = J[0][0] * J[1][0]; M[2][1] = J[0][1] * J[1][1]; M[2][2] = J[1][0] * J[0][1] + J[1][1] * J[0][0];
+        return;
+    }
+    double (*M)[6] = (double (*)[6])J2;
+    for (int a = 0; a < 3; a++) {
+        for (int b = 0; b < 3; b++) M[a][b] = J[a][b] * J[a][b];
+        M[a][3] = 2 * J[a][0] * J[a][1];
+        M[a][4] = 2 * J[a][1] * J[a][2];
+        M[a][5] = 2 * J[a][0] * J[a][2];
+    }
+    const int pr[3][2] = {{0, 1}, {1, 2}, {0, 2}};  // rows 3, 4, 5: (xi eta), (eta zeta), (xi zeta)
+    for (int r = 0; r < 3; r++) {
+        const int i = pr[r][0], k = pr[r][1];
+        for (int b = 0; b < 3; b++) M[3 + r][b] = J[i][b] * J[k][b];
+        M[3 + r][3] = J[i][0] * J[k][1] + J[k][0] * J[i][1];
+        M[3 + r][4] = J[i][1] * J[k][2] + J[k][1] * J[i][2];
+        M[3 + r][5] = J[i][0] * J[k][2] + J[k][0] * J[i][2];
+    }
+}
 
 // B2_2::getShapeGrads (B2_2.cpp:72-88): N = Nbase*C^T, GB = Gbase*C^T, J = (GB*X)^T,
 // G = inv(J)*GB  (the transposed-Jacobian form is reproduced as written).
@@ -266,6 +313,126 @@ static void shapeGrads_B2_2(const ElemType& et, const double* X, const double* C
         for (int j = 0; j < n; j++)
             for (int a = 0; a < 2; a++)
                 S.G[((size_t)ip * 2 + a) * n + j] = Ji[a][0] * GB[0][j] + Ji[a][1] * GB[1][j];
+    }
+}
+
+// B2_2::getShapeGrads2 (B2_2.cpp:90-121): J = GB X (not transposed), G = inv(J) GB,
+// G2 = inv(J2) (G2B - (G2B X) GB)
+static void shapeGrads2_B2_2(const ElemType& et, const double* X, const double* C, Shapes& S) {
+    const int n = et.n;
+    for (int ip = 0; ip < et.nip; ip++) {
+        double GB[2][9], G2B[3][9];
+        for (int j = 0; j < n; j++) {
+            double s = 0;
+            for (int k = 0; k < n; k++) s += et.Nbase[(size_t)ip * n + k] * C[j * n + k];
+            S.N[(size_t)ip * n + j] = s;
+            for (int c = 0; c < 2; c++) {
+                double t = 0;
+                for (int k = 0; k < n; k++) t += et.Gbase[((size_t)ip * 2 + c) * n + k] * C[j * n + k];
+                GB[c][j] = t;
+            }
+            for (int r = 0; r < 3; r++) {
+                double t = 0;
+                for (int k = 0; k < n; k++) t += et.G2base[((size_t)ip * 3 + r) * n + k] * C[j * n + k];
+                G2B[r][j] = t;
+            }
+        }
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, J2P[3][2];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                double t = 0;
+                for (int j = 0; j < n; j++) t += GB[a][j] * X[j * 2 + b];
+                J[a][b] = t;
+            }
+        for (int r = 0; r < 3; r++)
+            for (int b = 0; b < 2; b++) {
+                double t = 0;
+                for (int j = 0; j < n; j++) t += G2B[r][j] * X[j * 2 + b];
+                J2P[r][b] = t;
+            }
+        double J2[9], J2i[9];
+        J2matrix(2, J, J2);
+        lu_inverse(3, J2, J2i);
+        double Jd[2][2] = {{J[0][0], J[0][1]}, {J[1][0], J[1][1]}}, Ji[2][2];
+        inv2(Jd, Ji);
+        S.w[ip] = et.w[ip] * std::fabs(det2(Jd));
+        for (int j = 0; j < n; j++) {
+            for (int a = 0; a < 2; a++) S.G[((size_t)ip * 2 + a) * n + j] = Ji[a][0] * GB[0][j] + Ji[a][1] * GB[1][j];
+            double v[3];
+            for (int r = 0; r < 3; r++) v[r] = G2B[r][j] - (J2P[r][0] * GB[0][j] + J2P[r][1] * GB[1][j]);
+            for (int r = 0; r < 3; r++) S.G2[((size_t)ip * 3 + r) * n + j] = J2i[r * 3 + 0] * v[0] + J2i[r * 3 + 1] * v[1] + J2i[r * 3 + 2] * v[2];
+        }
+    }
+}
+
+// NURBS3_Cube2::getShapeGrads2 (NURBS3_Cube2.cpp:117-220): rational second derivatives
+// G2Xi = W (G2B/wFun - (dW_j GB_i + dW_i GB_j)/wFun^2 - dW2 CB/wFun^2 + 2 dW_i dW_j CB/wFun^3),
+// G2 = inv(J2) (G2Xi - (G2Xi^T X) GXi)
+static void shapeGrads2_NURBS3_Cube2(const ElemType& et, const double* X, const double* C, const double* W, Shapes& S) {
+    const int n = et.n;
+    const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
+    for (int ip = 0; ip < et.nip; ip++) {
+        double CB[27], GB[27][3], G2B[27][6];
+        for (int j = 0; j < n; j++) {
+            double s = 0;
+            for (int k = 0; k < n; k++) s += C[j * n + k] * et.Nbase[(size_t)ip * n + k];
+            CB[j] = s;
+            for (int c = 0; c < 3; c++) {
+                double t = 0;
+                for (int k = 0; k < n; k++) t += C[j * n + k] * et.Gbase[((size_t)ip * 3 + c) * n + k];
+                GB[j][c] = t;
+            }
+            for (int r = 0; r < 6; r++) {
+                double t = 0;
+                for (int k = 0; k < n; k++) t += C[j * n + k] * et.G2base[((size_t)ip * 6 + r) * n + k];
+                G2B[j][r] = t;
+            }
+        }
+        double wFun = 0, dW[3] = {0, 0, 0}, dW2[6] = {0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < n; j++) {
+            wFun += CB[j] * W[j];
+            for (int c = 0; c < 3; c++) dW[c] += GB[j][c] * W[j];
+            for (int r = 0; r < 6; r++) dW2[r] += G2B[j][r] * W[j];
+        }
+        double GXi[27][3], G2Xi[27][6];
+        for (int j = 0; j < n; j++) {
+            for (int c = 0; c < 3; c++) GXi[j][c] = W[j] * (GB[j][c] / wFun - CB[j] * dW[c] / wFun / wFun);
+            for (int r = 0; r < 6; r++) {
+                const int ii = ij[r][0], jj = ij[r][1];
+                G2Xi[j][r] = W[j] * (G2B[j][r] / wFun - dW[jj] * GB[j][ii] / wFun / wFun - dW[ii] * GB[j][jj] / wFun / wFun -
+                                     dW2[r] * CB[j] / wFun / wFun + 2 * dW[ii] * dW[jj] * CB[j] / wFun / wFun / wFun);
+            }
+        }
+        double J[3][3], J2P[6][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double t = 0;
+                for (int j = 0; j < n; j++) t += GXi[j][a] * X[j * 3 + b];
+                J[a][b] = t;
+            }
+        for (int r = 0; r < 6; r++)
+            for (int b = 0; b < 3; b++) {
+                double t = 0;
+                for (int j = 0; j < n; j++) t += G2Xi[j][r] * X[j * 3 + b];
+                J2P[r][b] = t;
+            }
+        double J2[36], J2i[36], Ji[3][3];
+        J2matrix(3, J, J2);
+        lu_inverse(6, J2, J2i);
+        inv3(J, Ji);
+        S.w[ip] = et.w[ip] * std::fabs(det3(J));
+        for (int j = 0; j < n; j++) {
+            S.N[(size_t)ip * n + j] = W[j] * CB[j] / wFun;
+            for (int a = 0; a < 3; a++)
+                S.G[((size_t)ip * 3 + a) * n + j] = Ji[a][0] * GXi[j][0] + Ji[a][1] * GXi[j][1] + Ji[a][2] * GXi[j][2];
+            double v[6];
+            for (int r = 0; r < 6; r++) v[r] = G2Xi[j][r] - (J2P[r][0] * GXi[j][0] + J2P[r][1] * GXi[j][1] + J2P[r][2] * GXi[j][2]);
+            for (int r = 0; r < 6; r++) {
+                double t = 0;
+                for (int k = 0; k < 6; k++) t += J2i[r * 6 + k] * v[k];
+                S.G2[((size_t)ip * 6 + r) * n + j] = t;
+            }
+        }
     }
 }
 
@@ -1556,6 +1723,48 @@ static double GetDistributionFunction(int method, double l, int dim, Vec& Dist, 
             for (int b = 0; b < n; b++)
                 for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / (l) * (2 * S.Nv(ip, a) * S.Nv(ip, b));
             break;
+        case IFE_DIST_HO_QUADRATIC:
+        case IFE_DIST_HO_CZM:
+        case IFE_DIST_HO_LINEAR:
+        case IFE_DIST_HO_LINEAR2: {
+            // PhaseFieldUtility::GetDistributionFunction with G2 (PhaseFieldUtil.cpp:223-280):
+            // DoubleCon = diag(1,1[,1],2[,2,2]) on the second-derivative components
+            const int ng = S.ng2;
+            double ddG[6] = {0, 0, 0, 0, 0, 0}, DC[6];
+            for (int r = 0; r < ng; r++) {
+                DC[r] = r < dim ? 1.0 : 2.0;
+                double t = 0;
+                for (int a = 0; a < n; a++) t += S.G2v(ip, r, a) * Phase[a];
+                ddG[r] = t;
+            }
+            double ddd = 0;
+            for (int r = 0; r < ng; r++) ddd += ddG[r] * DC[r] * ddG[r];
+            auto G2tD = [&](int a) { double t = 0; for (int r = 0; r < ng; r++) t += S.G2v(ip, r, a) * DC[r] * ddG[r]; return t; };
+            auto G2G2 = [&](int a, int b) { double t = 0; for (int r = 0; r < ng; r++) t += S.G2v(ip, r, a) * DC[r] * S.G2v(ip, r, b); return t; };
+            const double l4 = l * l * l * l;
+            if (method == IFE_DIST_HO_QUADRATIC || method == IFE_DIST_HO_CZM) {
+                const bool q = method == IFE_DIST_HO_QUADRATIC;
+                const double c = q ? 1.0 / (2.0 * l) : 1.0 / (3.020 * l);
+                gamma = c * ((q ? phi * phi : 2.0 * phi - phi * phi) + 0.5 * l * l * dGdG + l4 / 16.0 * ddd);
+                for (int a = 0; a < n; a++)
+                    Dist[a] = c * ((q ? 2.0 * S.Nv(ip, a) * phi : 2.0 * S.Nv(ip, a) * (1.0 - phi)) + l * l * GtdG(a) + l4 / 8.0 * G2tD(a));
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++)
+                        dDist(a, b) = c * ((q ? 2.0 : -2.0) * S.Nv(ip, a) * S.Nv(ip, b) + l * l * GtG(a, b) + l4 / 8.0 * G2G2(a, b));
+            } else if (method == IFE_DIST_HO_LINEAR) {
+                const double c = 1.0 / (19.2 * l);
+                gamma = c * (9.0 * phi + l4 * ddd);
+                for (int a = 0; a < n; a++) Dist[a] = c * (9.0 * S.Nv(ip, a) + 2.0 * l4 * G2tD(a));
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++) dDist(a, b) = c * (+2.0 * l4 * G2G2(a, b));
+            } else {
+                const double c = 1.0 / (4.0 * l);
+                gamma = c * (2.0 * phi + 0.5 * l * l * dGdG + l4 / 16.0 * ddd);
+                for (int a = 0; a < n; a++) Dist[a] = c * (2.0 * S.Nv(ip, a) + l * l * GtdG(a) + l4 / 8.0 * G2tD(a));
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++) dDist(a, b) = c * (l * l * GtG(a, b) + l4 / 8.0 * G2G2(a, b));
+            }
+        } break;
         default: throw std::invalid_argument("distribution function not supported by the oracle");
     }
     return gamma;
@@ -1580,6 +1789,28 @@ static void GetCapacityOnly(int method, double l, Vec& Dist, Mat& dDist, double&
             break;
         case IFE_DIST_LINEAR:
             for (int a = 0; a < n; a++) Dist[a] = 3.0 / (8.0 * l) * (S.Nv(ip, a));
+            dDist.zero();
+            DL = 3.0 / (8.0 * l); dDL = 0.0;
+            break;
+        case IFE_DIST_HO_QUADRATIC:  // PhaseFieldUtil.cpp:319-345
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (2.0 * l) * (2.0 * S.Nv(ip, a) * phi);
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / (2.0 * l) * (2.0 * S.Nv(ip, a) * S.Nv(ip, b));
+            DL = 1.0 / (l) * phi; dDL = 1.0 / l;
+            break;
+        case IFE_DIST_HO_CZM:
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (3.020 * l) * (2.0 * S.Nv(ip, a) * (1.0 - phi));
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / (3.020 * l) * (-2.0 * S.Nv(ip, a) * S.Nv(ip, b));
+            DL = 2.0 / (3.020 * l) * (1 - phi); dDL = -2.0 / (3.020 * l);
+            break;
+        case IFE_DIST_HO_LINEAR:
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (19.2 * l) * (9.0 * S.Nv(ip, a));
+            dDist.zero();
+            DL = 9.0 / (19.2 * l); dDL = 0.0;
+            break;
+        case IFE_DIST_HO_LINEAR2:
+            for (int a = 0; a < n; a++) Dist[a] = 3.0 / (8.0 * l) * S.Nv(ip, a);
             dDist.zero();
             DL = 3.0 / (8.0 * l); dDL = 0.0;
             break;
@@ -1656,6 +1887,26 @@ static void getShapeGrads(const Ctx& c, int64_t e, Shapes& S) {
     const double* Ce = &c.C[(size_t)e * n * n];
     if (c.dim == 2) shapeGrads_B2_2(c.et, Xe.data(), Ce, S);
     else shapeGrads_NURBS3_Cube2(c.et, Xe.data(), Ce, We.data(), S);
+}
+
+// Mesh::getShapeGrads with second derivatives (the variant the phase step calls when the
+// crack density needs them, PhaseField.cpp:528-532)
+static void getShapeGrads2(const Ctx& c, int64_t e, Shapes& S) {
+    const int n = c.n;
+    std::vector<double> Xe((size_t)n * c.dim), We(n);
+    for (int a = 0; a < n; a++) {
+        int64_t v = c.elems[(size_t)e * n + a];
+        for (int d = 0; d < c.dim; d++) Xe[(size_t)a * c.dim + d] = c.X[(size_t)v * c.dim + d];
+        We[a] = c.W[(size_t)v];
+    }
+    const double* Ce = &c.C[(size_t)e * n * n];
+    if (c.dim == 2) shapeGrads2_B2_2(c.et, Xe.data(), Ce, S);
+    else shapeGrads2_NURBS3_Cube2(c.et, Xe.data(), Ce, We.data(), S);
+}
+
+static bool needs_grad2(int method) {  // PhaseFieldUtil.cpp:18-22 (Custom is not supported)
+    return method == IFE_DIST_HO_LINEAR || method == IFE_DIST_HO_LINEAR2 || method == IFE_DIST_HO_CZM ||
+           method == IFE_DIST_HO_QUADRATIC;
 }
 
 // DofSpace::getDofForNodes (DofSpace.cpp:987-1008): out[i + j*n] = dof(node_i, field_j)
@@ -1910,7 +2161,8 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
     const size_t NIPT = (size_t)c.NE * nip;
     for (int64_t el = e0; el < e1; el++) {
         getShapeGrads(c, el, Su);
-        getShapeGrads(c, el, Sph);
+        if (needs_grad2(md.pf.DistributionFunction)) getShapeGrads2(c, el, Sph);  // PhaseField.cpp:528-532
+        else getShapeGrads(c, el, Sph);
         elem_nodes(c, el, El);
         getDofForNodes(c, El, md.fields_u, dofsU);
         getDofForNodes(c, El, fph, dofsPhase);
@@ -2224,7 +2476,8 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
     const size_t NIPT = (size_t)c.NE * nip;
     for (int64_t el = e0; el < e1; el++) {
         getShapeGrads(c, el, Su);
-        getShapeGrads(c, el, Sph);
+        if (needs_grad2(md.pf.DistributionFunction)) getShapeGrads2(c, el, Sph);  // PhaseField.cpp:528-532
+        else getShapeGrads(c, el, Sph);
         elem_nodes(c, el, El);
         getDofForNodes(c, El, md.fields_u, dofsU);
         getDofForNodes(c, El, fph, dofsPhase);
@@ -2541,6 +2794,46 @@ double ifo_energy_split(const ife_material* mat, int consistent, double T, doubl
         for (int j = 0; j < 6; j++) { D_Dam[i * 6 + j] = a[i][j]; D_UnDam[i * 6 + j] = b[i][j]; }
     }
     return E;
+}
+
+// Shape functions with second derivatives of element el (getShapeGrads2), for the pinning
+// tests: N (nip x n), G (nip x dim x n), G2 (nip x ng2 x n), w (nip)
+int ifo_shape_grads2(void* h, int64_t el, double* N, double* G, double* G2, double* w) {
+    Ctx* c = (Ctx*)h;
+    if (el < 0 || el >= c->NE) return -1;
+    Shapes S;
+    S.resize(c->nip, c->n, c->dim);
+    getShapeGrads2(*c, el, S);
+    std::copy(S.N.begin(), S.N.end(), N);
+    std::copy(S.G.begin(), S.G.end(), G);
+    std::copy(S.G2.begin(), S.G2.end(), G2);
+    std::copy(S.w.begin(), S.w.end(), w);
+    return 0;
+}
+
+// GetDistributionFunction of `method` at element el, IP ip for nodal phase values Phase
+// (n): Dist (n), dDist (n x n column-major); returns gamma (NaN on an unknown method)
+double ifo_distribution(void* h, int64_t el, int ip, int method, double l, const double* Phase, double* Dist,
+                        double* dDist) {
+    Ctx* c = (Ctx*)h;
+    Shapes S;
+    S.resize(c->nip, c->n, c->dim);
+    if (needs_grad2(method)) getShapeGrads2(*c, el, S);
+    else getShapeGrads(*c, el, S);
+    Vec D(c->n);
+    Mat dD(c->n, c->n);
+    double g;
+    try {
+        g = GetDistributionFunction(method, l, c->dim, D, dD, S, ip, Phase);
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return std::numeric_limits<double>::quiet_NaN();
+    }
+    for (int a = 0; a < c->n; a++) {
+        Dist[a] = D[a];
+        for (int b = 0; b < c->n; b++) dDist[a + b * c->n] = dD(a, b);
+    }
+    return g;
 }
 
 // Cosserat material pieces at temperature T (porosity 0), for the pinning tests.
