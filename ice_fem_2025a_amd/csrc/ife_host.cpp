@@ -503,12 +503,18 @@ void setup_device(ife_physics* ph) {
     ph->ready = true;
 }
 
+bool is_ho_distribution(int method) {  // PhaseFieldUtil.cpp:18-22
+    return method == IFE_DIST_HO_LINEAR || method == IFE_DIST_HO_LINEAR2 || method == IFE_DIST_HO_CZM ||
+           method == IFE_DIST_HO_QUADRATIC;
+}
+
 void require_ready(const ife_physics* ph) {
     if (!ph) THROW(IFE_ERR_INVALID, "null physics");
     if (!ph->ready) THROW(IFE_ERR_STATE, "ife_physics_setup has not been called");
 }
 
-// Bernstein basis of degree 2 at x (BaseElemType.cpp:197-216) and the 3-point Gauss
+// Bernstein basis of degree 2 at x with its first and second derivatives
+// (BaseElemType.cpp:197-216) and the 3-point Gauss
 // rule on [0,1] (IPScheme.cpp:62-109)
 void bernstein_tables(KParams& p) {
     const double xs[3] = {(-std::sqrt(3.0 / 5.0) + 1) / 2, (0.0 + 1) / 2, (std::sqrt(3.0 / 5.0) + 1) / 2};
@@ -522,6 +528,8 @@ void bernstein_tables(KParams& p) {
             p.B1[q][i] = pre * std::pow(1.0 - x, std::max(0, pdeg - i)) * std::pow(x, std::max(0, i));
             p.dB1[q][i] = pre * ((i - pdeg) * std::pow(1.0 - x, std::max(0, pdeg - i - 1)) * std::pow(x, std::max(0, i)) +
                                  i * std::pow(1.0 - x, pdeg - i) * std::pow(x, std::max(0, i - 1)));
+            // second derivatives of (1-x)^2, 2x(1-x), x^2
+            p.d2B1[q][i] = i == 1 ? -4.0 : 2.0;
         }
         p.wq[q] = ws[q];
     }
@@ -648,6 +656,9 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
             HIPCHK(launch());
         }
     };
+    // higher-order crack densities: their derivative parts go into the phase records
+    // between the element kernel and the gather
+    const bool ho = p.do_ph && is_ho_distribution(pfm->pf.DistributionFunction);
     if (pfm && pfm->coss) {
         // Cosserat: the material kernel updates every IP; the u step's 162-dof element
         // blocks go straight into the zeroed CSR values colour by colour; the phase and T
@@ -668,12 +679,14 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
         q.do_u = 0;
         if (q.do_ph || q.do_Tpf || q.do_th) {
             HIPCHK(launch_assemble(q, ph->dim, s));
+            if (ho) HIPCHK(launch_ho_phase(q, ph->dim, s));
             timed(ph->gat_recs, 0, [&] { return launch_gather(q, ph->dim, s); });
         }
         return;
     }
     timed(ph->mat_recs, 0, [&] { return launch_material(p, ph->dim, s); });
     timed(ph->recs, ph->NE, [&] { return launch_assemble(p, ph->dim, s); });
+    if (ho) HIPCHK(launch_ho_phase(p, ph->dim, s));
     timed(ph->gat_recs, 0, [&] { return launch_gather(p, ph->dim, s); });
 }
 
@@ -694,6 +707,8 @@ void run_adjoint(ife_physics* ph, const double* const* lam, double* g_Gc, double
     if (pfm->opt.LumpedDriving || pfm->opt.LumpedPFCapacity)
         THROW(IFE_ERR_UNSUPPORTED, "adjoint: lumped driving / lumped capacity are not differentiated");
     if (pfm->pf.ViscoDrivingFactor > 1.0e-9) THROW(IFE_ERR_UNSUPPORTED, "adjoint: viscous driving is not differentiated");
+    if (is_ho_distribution(pfm->pf.DistributionFunction))
+        THROW(IFE_ERR_UNSUPPORTED, "adjoint: the higher-order crack densities are not differentiated");
     require_ready(ph);
     KParams p = base_params(ph);
     p.do_u = 1;
@@ -779,6 +794,8 @@ void batch_update(ife_batch* b) {
         if (v != vep || x != exact)
             THROW(IFE_ERR_UNSUPPORTED, "batch: domains need the same return map (Glen / visco-elasto-plastic, tangent)");
         if (p.mixed != mixed) THROW(IFE_ERR_UNSUPPORTED, "batch: domains need the same precision");
+        if (p.has_pf && is_ho_distribution(p.mc.Distribution))
+            THROW(IFE_ERR_UNSUPPORTED, "batch: the higher-order crack densities run through ife_physics_assemble");
         P.push_back(p);
     }
     b->vep = vep; b->exact = exact; b->full = full; b->mixed = mixed;
@@ -977,11 +994,8 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             if (mat->Split == IFE_SPLIT_SPECSTRESS && pf->ConsistentEnergy)
                 THROW(IFE_ERR_UNSUPPORTED, "SpecStress split with ConsistentEnergy is not defined by the reference");
             int dfn = pf->DistributionFunction;
-            if (!(dfn == IFE_DIST_LINEAR || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR2 || dfn == IFE_DIST_QUADRATIC ||
-                  dfn == IFE_DIST_LOCAL))
+            if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_LOCAL)  // Custom needs a user callback
                 THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
-            if (opts->LumpedPFCapacity && !(dfn == IFE_DIST_QUADRATIC || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR))
-                THROW(IFE_ERR_UNSUPPORTED, "lumped capacity form not implemented for this distribution");
             if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE ||
                   opts->Irreversible == IFE_IRR_PENALTY))
                 THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist, None and Penalty are)");
@@ -1018,8 +1032,7 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             if (!(mat->Split == IFE_SPLIT_NONE || (mat->Split == IFE_SPLIT_SPECSTRESS && !pf->ConsistentEnergy)))
                 THROW(IFE_ERR_UNSUPPORTED, "Cosserat energy split: NoSplit, or SpecStress without a consistent energy");
             int dfn = pf->DistributionFunction;
-            if (!(dfn == IFE_DIST_LINEAR || dfn == IFE_DIST_CZM || dfn == IFE_DIST_LINEAR2 || dfn == IFE_DIST_QUADRATIC ||
-                  dfn == IFE_DIST_LOCAL))
+            if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_LOCAL)  // Custom needs a user callback
                 THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
             if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE ||
                   opts->Irreversible == IFE_IRR_PENALTY))

@@ -97,7 +97,7 @@ struct KParams {
     int64_t Ng;              // nodes in the group
     int64_t NE;
     // Bernstein values/derivatives at the 1D Gauss points (BezierBasis, ipcount1D = 3)
-    double B1[3][3], dB1[3][3], wq[3];
+    double B1[3][3], dB1[3][3], d2B1[3][3], wq[3];
     // physics
     MatConst mc;
     double t, dt, du_dt, ddu_dt;
@@ -151,6 +151,7 @@ hipError_t launch_adjoint(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_spmv(const int64_t* row_ptr, const int32_t* col, const double* val, const double* x, double* y,
                        int64_t n_rows, hipStream_t s);
 hipError_t launch_gather(const KParams& p, int dim, hipStream_t s);
+hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s);
 // Cosserat: material update of every IP (u, phase, T coefficients), then the u-step element
 // kernel over one colour of elements (adds into the zeroed CSR values and f)
 hipError_t launch_coss_material(const KParams& p, hipStream_t s);

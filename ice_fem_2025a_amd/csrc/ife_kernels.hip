@@ -840,7 +840,7 @@ __device__ double energy_split(const MatConst& mc, const MatIP& m, const double 
     return E;
 }
 
-// crack-density coefficients (PhaseFieldUtil.cpp:185-221, non-HO forms):
+// crack-density coefficients (PhaseFieldUtil.cpp:185-280; of the higher-order forms the N-parts):
 // Dist = dN*N + cGG*G^T grad(phi),  dDist = cNN*N N^T + cGG*G^T G,  gamma
 __device__ void dist_coeffs(int method, double l, double phi, double gg, double& dN, double& cNN, double& cGG,
                             double& gamma) {
@@ -862,10 +862,46 @@ __device__ void dist_coeffs(int method, double l, double phi, double gg, double&
             gamma = 1.0 / (4.0 * l) * (2.0 * phi + l * l * gg);
             dN = 1.0 / (4.0 * l) * 2.0; cNN = 0.0; cGG = 1.0 / (4.0 * l) * 2.0 * l * l;
             break;
+        // higher-order densities (PhaseFieldUtil.cpp:223-280): the N-parts only; their
+        // first- and second-derivative parts (ho_coeffs) go through ho_phase_kernel
+        case IFE_DIST_HO_QUADRATIC:
+            gamma = 1.0 / (2.0 * l) * (phi * phi);
+            dN = 1.0 / (2.0 * l) * 2.0 * phi; cNN = 1.0 / (2.0 * l) * 2.0; cGG = 0.0;
+            break;
+        case IFE_DIST_HO_CZM:
+            gamma = 1.0 / (3.020 * l) * (2.0 * phi - phi * phi);
+            dN = 1.0 / (3.020 * l) * 2.0 * (1.0 - phi); cNN = 1.0 / (3.020 * l) * (-2.0); cGG = 0.0;
+            break;
+        case IFE_DIST_HO_LINEAR:
+            gamma = 1.0 / (19.2 * l) * (9.0 * phi);
+            dN = 1.0 / (19.2 * l) * 9.0; cNN = 0.0; cGG = 0.0;
+            break;
+        case IFE_DIST_HO_LINEAR2:
+            gamma = 1.0 / (4.0 * l) * (2.0 * phi);
+            dN = 1.0 / (4.0 * l) * 2.0; cNN = 0.0; cGG = 0.0;
+            break;
         default:  // Local
             gamma = 1.0 / l * (phi * phi);
             dN = 1.0 / l * 2 * phi; cNN = 1.0 / l * 2; cGG = 0.0;
             break;
+    }
+}
+
+__device__ __forceinline__ bool is_ho(int method) {
+    return method == IFE_DIST_HO_QUADRATIC || method == IFE_DIST_HO_CZM || method == IFE_DIST_HO_LINEAR ||
+           method == IFE_DIST_HO_LINEAR2;
+}
+
+// Derivative parts of the higher-order densities (PhaseFieldUtil.cpp:236-278): dDist gains
+// kG G^T G + kG2 G2^T DC G2 and Dist the same applied to phi, with DC = diag(1,1[,1],2[,2,2]);
+// gamma gains kG/2 |grad phi|^2 + kG2/2 ddphi^T DC ddphi.
+__device__ __forceinline__ void ho_coeffs(int method, double l, double& kG, double& kG2) {
+    const double l2 = l * l, l4 = l2 * l2;
+    switch (method) {
+        case IFE_DIST_HO_QUADRATIC: kG = 1.0 / (2.0 * l) * l2; kG2 = 1.0 / (2.0 * l) * (l4 / 8.0); break;
+        case IFE_DIST_HO_CZM: kG = 1.0 / (3.020 * l) * l2; kG2 = 1.0 / (3.020 * l) * (l4 / 8.0); break;
+        case IFE_DIST_HO_LINEAR: kG = 0.0; kG2 = 1.0 / (19.2 * l) * (2.0 * l4); break;
+        default: kG = 1.0 / (4.0 * l) * l2; kG2 = 1.0 / (4.0 * l) * (l4 / 8.0); break;  // HO_Linear2
     }
 }
 
@@ -877,7 +913,15 @@ __device__ void cap_coeffs(int method, double l, double phi, double& dN, double&
             dN = 1.0 / (3.141592 * l) * 2.0 * (1.0 - phi); cNN = 1.0 / (3.141592 * l) * (-2.0);
             LD = 2.0 / (3.141592 * l) * (1.0 - phi); dLD = -2.0 / (3.141592 * l);
             break;
-        default: dN = 3.0 / (8.0 * l); cNN = 0.0; LD = 3.0 / (8.0 * l); dLD = 0.0; break;  // Linear
+        case IFE_DIST_HO_QUADRATIC: dN = 1.0 / (2.0 * l) * 2.0 * phi; cNN = 1.0 / (2.0 * l) * 2.0; LD = 1.0 / l * phi; dLD = 1.0 / l; break;
+        case IFE_DIST_HO_CZM:
+            dN = 1.0 / (3.020 * l) * 2.0 * (1.0 - phi); cNN = 1.0 / (3.020 * l) * (-2.0);
+            LD = 2.0 / (3.020 * l) * (1.0 - phi); dLD = -2.0 / (3.020 * l);
+            break;
+        case IFE_DIST_HO_LINEAR: dN = 1.0 / (19.2 * l) * 9.0; cNN = 0.0; LD = 9.0 / (19.2 * l); dLD = 0.0; break;
+        case IFE_DIST_LINEAR2: dN = 1.0 / (4.0 * l) * 2.0; cNN = 0.0; LD = 1.0 / (4.0 * l); dLD = 0.0; break;
+        case IFE_DIST_LOCAL: dN = 1.0 / l * 2.0 * phi; cNN = 1.0 / l * 2.0; LD = 2.0 / l * phi; dLD = 2.0 / l; break;
+        default: dN = 3.0 / (8.0 * l); cNN = 0.0; LD = 3.0 / (8.0 * l); dLD = 0.0; break;  // Linear, HO_Linear2
     }
 }
 
@@ -1909,6 +1953,261 @@ __global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_batch(co
 }
 
 // ---------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------
+// Higher-order crack densities: the first- and second-derivative parts of K_pp, F_p and the
+// crack length, with the shape derivatives of Mesh::getShapeGrads2 (B2_2.cpp:90-121,
+// NURBS3_Cube2.cpp:117-220) at every IP:
+//   J = GXi X (not transposed, also in 2D), G = J^-1 GXi,
+//   G2 = J2^-1 (G2Xi - (G2Xi X) GXi)      (GXi, not G, in the last product: as written there)
+// with J2 the chain-rule matrix of J (rows: xi-pairs, columns: x-pairs), the 3D rational basis
+//   GXi_i = W (GB_i / wFun - CB dW_i / wFun^2),
+//   G2Xi_ij = W (G2B_ij / wFun - (dW_j GB_i + dW_i GB_j + dW2_ij CB) / wFun^2
+//               + 2 dW_i dW_j CB / wFun^3)
+// and GXi = GB, G2Xi = G2B in 2D.  With Phi = (G, G2) the rows of one IP,
+//   K_pp += sum_ip Phi^T diag(cK) Phi,   F_p += sum_ip Phi^T cF,
+//   cK = w Gc (kG [DIM], kG2 DC [second-derivative rows]),  cF = cK * (Phi phi),
+//   crack length += sum_ip w (cK / (w Gc)) (Phi phi)^2 / 2.
+// The element kernel has written the N-parts into the element records; one workgroup per
+// element adds these parts to them, nine IPs at a time, before the gather.
+template <int DIM>
+struct HoSmem {
+    static constexpr int N = DIM == 2 ? 9 : 27, NIP = N, NG2 = DIM == 2 ? 3 : 6, NK = DIM + NG2, NR = 1 + DIM + NG2,
+                         CH = 9;
+    double X[N][4];
+    double ph[N];
+    double t1[3][DIM][3][3];  // C_d times the Bernstein value / first / second derivative [dir][gauss pt][fn]
+    double S[CH][NR][DIM + 1];  // sum_a W_a raw_m(a) f(a), f = 1, x, y(, z)
+    double Ji[CH][DIM][DIM], J2i[CH][NG2][NG2], Q[CH][NG2][DIM];
+    double rw[CH], dW[CH][DIM], dW2[CH][NG2];
+    double Phi[CH][NK][N];
+    double cK[CH][NK], cF[CH][NK], gw[CH][NK];
+};
+
+// the pair of directions of second-derivative row r: xx yy [zz] xy [yz xz]
+template <int DIM>
+__device__ __forceinline__ void ho_pair(int r, int& i, int& j) {
+    if (r < DIM) { i = j = r; }
+    else if (DIM == 2) { i = 0; j = 1; }
+    else { i = r == 4 ? 1 : 0; j = r == 3 ? 1 : 2; }
+}
+
+// raw tensor-product basis quantity m of node a at IP ip: m = 0 CB, 1.. GB, 1 + DIM.. G2B
+template <int DIM>
+__device__ __forceinline__ double ho_raw(const HoSmem<DIM>& sm, int ip, int a, int m) {
+    const int q[3] = {ip % 3, (ip / 3) % 3, ip / 9}, n[3] = {a % 3, (a / 3) % 3, a / 9};
+    int ord[3] = {0, 0, 0};
+    if (m >= 1 && m <= DIM) ord[m - 1] = 1;
+    else if (m > DIM) {
+        int i, j;
+        ho_pair<DIM>(m - 1 - DIM, i, j);
+        ord[i]++;
+        ord[j]++;
+    }
+    double v = 1.0;
+    for (int d = 0; d < DIM; d++) v *= sm.t1[ord[d]][d][q[d]][n[d]];
+    return v;
+}
+
+// Gauss-Jordan inverse with partial pivoting of a small dense matrix
+template <int M>
+__device__ void gj_inverse(double A[M][M], double Ai[M][M]) {
+    for (int i = 0; i < M; i++)
+        for (int j = 0; j < M; j++) Ai[i][j] = i == j ? 1.0 : 0.0;
+    for (int c = 0; c < M; c++) {
+        int piv = c;
+        for (int r = c + 1; r < M; r++)
+            if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        if (piv != c)
+            for (int j = 0; j < M; j++) {
+                double t = A[c][j]; A[c][j] = A[piv][j]; A[piv][j] = t;
+                t = Ai[c][j]; Ai[c][j] = Ai[piv][j]; Ai[piv][j] = t;
+            }
+        const double id = 1.0 / A[c][c];
+        for (int j = 0; j < M; j++) { A[c][j] *= id; Ai[c][j] *= id; }
+        for (int r = 0; r < M; r++) {
+            if (r == c) continue;
+            const double f = A[r][c];
+            for (int j = 0; j < M; j++) { A[r][j] -= f * A[c][j]; Ai[r][j] -= f * Ai[c][j]; }
+        }
+    }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
+    using H = HoSmem<DIM>;
+    using RC = Rec<DIM>;
+    constexpr int N = H::N, NIP = H::NIP, NG2 = H::NG2, NK = H::NK, NR = H::NR, CH = H::CH, NT = 256;
+    constexpr int NOUT = N * N + N, SLOTS = (NOUT + NT - 1) / NT;
+    __shared__ H sm;
+    const int tid = threadIdx.x;
+    const int64_t el = blockIdx.x;
+    if (tid < N) {
+        const int v = p.enode[el * N + tid], r = p.erank[el * N + tid];
+        const double* nd = p.nodes + (int64_t)v * 4;
+        for (int c = 0; c < 4; c++) sm.X[tid][c] = nd[c];
+        sm.ph[tid] = p.sph.x[(int64_t)p.fpos_ph * p.Ng + r];
+    }
+    if (tid < 3 * DIM * 9) {
+        const int kind = tid / (DIM * 9), d = (tid / 9) % DIM, q = (tid / 3) % 3, a = tid % 3;
+        const double* C = p.ext + (el * DIM + d) * 9;
+        const double* tab = kind == 0 ? p.B1[q] : (kind == 1 ? p.dB1[q] : p.d2B1[q]);
+        double s = 0;
+        for (int k = 0; k < 3; k++) s += C[a * 3 + k] * tab[k];
+        sm.t1[kind][d][q][a] = s;
+    }
+    double kG, kG2;
+    ho_coeffs(p.mc.Distribution, p.mc.l, kG, kG2);
+    const double* const ipc = p.ipb + el * IPB_NF * NIP;
+    double acc[SLOTS];
+    for (int k = 0; k < SLOTS; k++) acc[k] = 0.0;
+    double lc = 0.0;
+    __syncthreads();
+#pragma unroll 1
+    for (int ip0 = 0; ip0 < NIP; ip0 += CH) {
+        // W-weighted sums of the raw basis against 1 and the coordinates
+        for (int t = tid; t < CH * NR * (DIM + 1); t += NT) {
+            const int i = t / (NR * (DIM + 1)), m = (t / (DIM + 1)) % NR, b = t % (DIM + 1);
+            double s = 0;
+            for (int a = 0; a < N; a++) {
+                const double f = b == 0 ? 1.0 : sm.X[a][b - 1];
+                s += (DIM == 3 ? sm.X[a][3] : 1.0) * ho_raw<DIM>(sm, ip0 + i, a, m) * f;
+            }
+            sm.S[i][m][b] = s;
+        }
+        __syncthreads();
+        // per-IP geometry: J, J2P = G2Xi X, J2, their inverses and Q = J2^-1 J2P
+        if (tid < CH) {
+            const int i = tid;
+            double J[DIM][DIM], J2P[NG2][DIM];
+            double rw = 1.0, dW[DIM], dW2[NG2];
+            for (int c = 0; c < DIM; c++) dW[c] = DIM == 3 ? sm.S[i][1 + c][0] : 0.0;
+            for (int r = 0; r < NG2; r++) dW2[r] = DIM == 3 ? sm.S[i][1 + DIM + r][0] : 0.0;
+            if (DIM == 3) rw = 1.0 / sm.S[i][0][0];
+            for (int b = 0; b < DIM; b++) {
+                const double s0 = sm.S[i][0][1 + b];
+                for (int a = 0; a < DIM; a++)
+                    J[a][b] = DIM == 3 ? sm.S[i][1 + a][1 + b] * rw - dW[a] * s0 * rw * rw : sm.S[i][1 + a][1 + b];
+                for (int r = 0; r < NG2; r++) {
+                    int ii, jj;
+                    ho_pair<DIM>(r, ii, jj);
+                    J2P[r][b] = DIM == 3 ? sm.S[i][1 + DIM + r][1 + b] * rw -
+                                               (dW[jj] * sm.S[i][1 + ii][1 + b] + dW[ii] * sm.S[i][1 + jj][1 + b] + dW2[r] * s0) * rw * rw +
+                                               2.0 * dW[ii] * dW[jj] * s0 * rw * rw * rw
+                                         : sm.S[i][1 + DIM + r][1 + b];
+                }
+            }
+            double J2[NG2][NG2], J2i[NG2][NG2], Jc[DIM][DIM], Ji[DIM][DIM];
+            for (int r = 0; r < NG2; r++) {
+                int ii, kk;
+                ho_pair<DIM>(r, ii, kk);
+                for (int c = 0; c < NG2; c++) {
+                    int a, b;
+                    ho_pair<DIM>(c, a, b);
+                    J2[r][c] = a == b ? J[ii][a] * J[kk][a] : J[ii][a] * J[kk][b] + J[kk][a] * J[ii][b];
+                }
+            }
+            for (int a = 0; a < DIM; a++)
+                for (int b = 0; b < DIM; b++) Jc[a][b] = J[a][b];
+            gj_inverse<NG2>(J2, J2i);
+            gj_inverse<DIM>(Jc, Ji);
+            for (int a = 0; a < DIM; a++)
+                for (int b = 0; b < DIM; b++) sm.Ji[i][a][b] = Ji[a][b];
+            for (int r = 0; r < NG2; r++) {
+                for (int c = 0; c < NG2; c++) sm.J2i[i][r][c] = J2i[r][c];
+                for (int b = 0; b < DIM; b++) {
+                    double t = 0;
+                    for (int k = 0; k < NG2; k++) t += J2i[r][k] * J2P[k][b];
+                    sm.Q[i][r][b] = t;
+                }
+                sm.dW2[i][r] = dW2[r];
+            }
+            for (int c = 0; c < DIM; c++) sm.dW[i][c] = dW[c];
+            sm.rw[i] = rw;
+        }
+        __syncthreads();
+        // Phi = (G, G2) of every (IP, node)
+        for (int t = tid; t < CH * N; t += NT) {
+            const int i = t / N, a = t % N;
+            double raw[NR];
+            for (int m = 0; m < NR; m++) raw[m] = ho_raw<DIM>(sm, ip0 + i, a, m);
+            double GXi[DIM], G2Xi[NG2];
+            if (DIM == 3) {
+                const double Wa = sm.X[a][3], rw = sm.rw[i];
+                for (int c = 0; c < DIM; c++) GXi[c] = Wa * (raw[1 + c] * rw - raw[0] * sm.dW[i][c] * rw * rw);
+                for (int r = 0; r < NG2; r++) {
+                    int ii, jj;
+                    ho_pair<DIM>(r, ii, jj);
+                    G2Xi[r] = Wa * (raw[1 + DIM + r] * rw -
+                                    (sm.dW[i][jj] * raw[1 + ii] + sm.dW[i][ii] * raw[1 + jj] + sm.dW2[i][r] * raw[0]) * rw * rw +
+                                    2.0 * sm.dW[i][ii] * sm.dW[i][jj] * raw[0] * rw * rw * rw);
+                }
+            } else {
+                for (int c = 0; c < DIM; c++) GXi[c] = raw[1 + c];
+                for (int r = 0; r < NG2; r++) G2Xi[r] = raw[1 + DIM + r];
+            }
+            for (int c = 0; c < DIM; c++) {
+                double g = 0;
+                for (int d = 0; d < DIM; d++) g += sm.Ji[i][c][d] * GXi[d];
+                sm.Phi[i][c][a] = g;
+            }
+            for (int r = 0; r < NG2; r++) {
+                double g = 0;
+                for (int k = 0; k < NG2; k++) g += sm.J2i[i][r][k] * G2Xi[k];
+                for (int b = 0; b < DIM; b++) g -= sm.Q[i][r][b] * GXi[b];
+                sm.Phi[i][DIM + r][a] = g;
+            }
+        }
+        __syncthreads();
+        // coefficients: grad phi / second derivatives of phi at the IP
+        for (int t = tid; t < CH * NK; t += NT) {
+            const int i = t / NK, k = t % NK, ip = ip0 + i;
+            double s = 0;
+            for (int a = 0; a < N; a++) s += sm.Phi[i][k][a] * sm.ph[a];
+            const double kc = k < DIM ? kG : kG2 * (k - DIM < DIM ? 1.0 : 2.0);
+            const double w = ipc[IPB_W * NIP + ip], wgc = w * ipc[IPB_GC * NIP + ip];
+            sm.cK[i][k] = wgc * kc;
+            sm.cF[i][k] = wgc * kc * s;
+            sm.gw[i][k] = 0.5 * kc * s * s * w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SLOTS; k++) {
+            const int o = tid + k * NT;
+            if (o >= NOUT) continue;
+            double t = 0;
+            if (o < N * N) {
+                const int a = o / N, b = o % N;
+                for (int i = 0; i < CH; i++)
+                    for (int kk = 0; kk < NK; kk++) t += sm.Phi[i][kk][a] * sm.cK[i][kk] * sm.Phi[i][kk][b];
+            } else {
+                const int a = o - N * N;
+                for (int i = 0; i < CH; i++)
+                    for (int kk = 0; kk < NK; kk++) t += sm.Phi[i][kk][a] * sm.cF[i][kk];
+            }
+            acc[k] += t;
+        }
+        if (tid == 0)
+            for (int i = 0; i < CH; i++)
+                for (int kk = 0; kk < NK; kk++) lc += sm.gw[i][kk];
+        __syncthreads();  // the next chunk overwrites the shared arrays
+    }
+    double* const rec = p.ebuf + el * N * RC::SIZE;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++) {
+        const int o = tid + k * NT;
+        if (o >= NOUT) continue;
+        bad |= !isfinite(acc[k]);
+        if (o < N * N) rec[(o / N) * RC::SIZE + RC::KP + o % N] += acc[k];
+        else rec[(o - N * N) * RC::SIZE + RC::FP] += acc[k];
+    }
+    if (tid == 0) {
+        bad |= !isfinite(lc);
+        p.lcrack_elem[el] += lc;
+    }
+    if (bad) atomicOr(&p.st->nonfinite, 1u);
+}
+
 // Gather kernel: one wavefront per node (group rank r).  Adds the records of the node's
 // adjacent elements, in increasing element index (the order in which the reference's
 // element loop adds them through MatSetValues/VecSetValues), into LDS copies of the
@@ -2796,6 +3095,13 @@ hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
         if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, double, double, NT>), grid, block, 0, s, p);
         else hipLaunchKernelGGL((assemble_kernel<3, double, double, NT>), grid, block, 0, s, p);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s) {
+    if (p.NE == 0) return hipSuccess;
+    if (dim == 2) hipLaunchKernelGGL(ho_phase_kernel<2>, dim3((unsigned)p.NE), dim3(256), 0, s, p);
+    else hipLaunchKernelGGL(ho_phase_kernel<3>, dim3((unsigned)p.NE), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

@@ -1770,7 +1770,7 @@ static double GetDistributionFunction(int method, double l, int dim, Vec& Dist, 
     return gamma;
 }
 
-// PhaseFieldUtility::GetCapacityOnly (PhaseFieldUtil.cpp:282-348), non-HO forms
+// PhaseFieldUtility::GetCapacityOnly (PhaseFieldUtil.cpp:282-348)
 static void GetCapacityOnly(int method, double l, Vec& Dist, Mat& dDist, double& DL, double& dDL, const Shapes& S, int ip,
                             double phi) {
     const int n = S.n;
@@ -1791,6 +1791,17 @@ static void GetCapacityOnly(int method, double l, Vec& Dist, Mat& dDist, double&
             for (int a = 0; a < n; a++) Dist[a] = 3.0 / (8.0 * l) * (S.Nv(ip, a));
             dDist.zero();
             DL = 3.0 / (8.0 * l); dDL = 0.0;
+            break;
+        case IFE_DIST_LINEAR2:
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / (4.0 * l) * (2.0 * S.Nv(ip, a));
+            dDist.zero();
+            DL = 1.0 / (4.0 * l); dDL = 0.0;
+            break;
+        case IFE_DIST_LOCAL:
+            for (int a = 0; a < n; a++) Dist[a] = 1.0 / l * (2.0 * S.Nv(ip, a) * phi);
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / l * (2.0 * S.Nv(ip, a) * S.Nv(ip, b));
+            DL = 2.0 / l * phi; dDL = 2.0 / l;
             break;
         case IFE_DIST_HO_QUADRATIC:  // PhaseFieldUtil.cpp:319-345
             for (int a = 0; a < n; a++) Dist[a] = 1.0 / (2.0 * l) * (2.0 * S.Nv(ip, a) * phi);
