@@ -1955,7 +1955,7 @@ __global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_batch(co
 // ---------------------------------------------------------------------------------------
 // ---------------------------------------------------------------------------------------
 // Higher-order crack densities: the first- and second-derivative parts of K_pp, F_p and the
-// crack length, with the shape derivatives of Mesh::getShapeGrads2 (B2_2.cpp:90-121,
+// crack length, with the shape derivatives of Mesh::getShapeGrads2 (B2_2.cpp:90-125,
 // NURBS3_Cube2.cpp:117-220) at every IP:
 //   J = GXi X (not transposed, also in 2D), G = J^-1 GXi,
 //   G2 = J2^-1 (G2Xi - (G2Xi X) GXi)      (GXi, not G, in the last product: as written there)

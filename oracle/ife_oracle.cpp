@@ -316,7 +316,7 @@ static void shapeGrads_B2_2(const ElemType& et, const double* X, const double* C
     }
 }
 
-// B2_2::getShapeGrads2 (B2_2.cpp:90-121): J = GB X (not transposed), G = inv(J) GB,
+// B2_2::getShapeGrads2 (B2_2.cpp:90-125): J = GB X (not transposed), G = inv(J) GB,
 // G2 = inv(J2) (G2B - (G2B X) GB)
 static void shapeGrads2_B2_2(const ElemType& et, const double* X, const double* C, Shapes& S) {
     const int n = et.n;

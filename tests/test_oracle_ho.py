@@ -1,12 +1,12 @@
 """Second shape derivatives and the higher-order crack densities in the oracle
-(getShapeGrads2: B2_2.cpp:90-121, NURBS3_Cube2.cpp:117-220; GetDistributionFunction with
+(getShapeGrads2: B2_2.cpp:90-125, NURBS3_Cube2.cpp:117-220; GetDistributionFunction with
 G2: PhaseFieldUtil.cpp:223-280), pinned without the reference:
 - reproduction: on the generated meshes (unit weights, control points at the Greville
   abscissae, so every element map is affine) sum_a G2_a X_a = 0, and the B-spline
   coefficients of x^2 (u_{i+1} u_{i+2} per direction) and of x y (Greville abscissae
   products) give exactly 2 and 1 in the matching component and 0 elsewhere.  The
   reference subtracts J2Part2 times the parametric gradient GXi where the chain rule has
-  the physical gradient (NURBS3_Cube2.cpp:217, B2_2.cpp:118), which is exact only for affine
+  the physical gradient (NURBS3_Cube2.cpp:217, B2_2.cpp:122), which is exact only for affine
   element maps: with non-unit weights sum_a G2_a X_a is then not zero, and the oracle
   reproduces that as written (the partition of unity still holds);
 - partition of unity: sum_a G2_a = 0, and G reproduces the coordinates (G X = I);
