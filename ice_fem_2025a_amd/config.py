@@ -347,6 +347,32 @@ def make_problem(name, nel=None, seed=None, h=None):
     return prob
 
 
+def add_multipliers(prob, method, seed=20261203 + 7):
+    """Switch a problem's phase-field model to a Lagrange-multiplier irreversibility
+    (LagrangeMult, AugmentedLagrangeMult: field lMult_pf; AugmentedLagrangeMult2/3: lMult_pf
+    and lMult_pf2), with the multiplier fields placed after "phase" in its solver step as in
+    the shipped inputs (TestCases/IceCliffs/IceCliff.json:85-86).  Seeded states that visit
+    every branch of the multiplier terms: phi_old = phi + U(-0.1, 0.1), a tenth of the nodes
+    at phi ~ U[1, 1.2), lMult_pf ~ U(-0.5, 0.5), lMult_pf2 ~ U(0, 1)."""
+    m = prob.models[0]
+    m.opts.Irreversible = abi.IRREVERSIBLE[method]
+    names = ["lMult_pf"] + (["lMult_pf2"] if method in ("AugmentedLagrangeMult2", "AugmentedLagrangeMult3") else [])
+    ip = prob.dof_names.index("phase")
+    step = prob.dof_steps[ip]
+    prob.dof_names = prob.dof_names[:ip + 1] + names + prob.dof_names[ip + 1:]
+    prob.dof_steps = prob.dof_steps[:ip + 1] + [step] * len(names) + prob.dof_steps[ip + 1:]
+    rng = np.random.Generator(np.random.Philox(seed))
+    NN = prob.mesh.n_nodes
+    ph = prob.node_state[(abi.IFE_STATE, "phase")].copy()
+    hi = rng.uniform(0.0, 1.0, NN) < 0.1
+    ph[hi] = rng.uniform(1.0, 1.2, int(hi.sum()))
+    prob.node_state[(abi.IFE_STATE, "phase")] = ph
+    prob.node_state[(abi.IFE_STATE_OLD, "phase")] = ph + rng.uniform(-0.1, 0.1, NN)
+    for nm in names:  # lMult_pf2 acts at the upper limit 1.1: shifted up to reach it
+        prob.node_state[(abi.IFE_STATE, nm)] = rng.uniform(-0.5, 0.5, NN) + (0.5 if nm == "lMult_pf2" else 0.0)
+    return prob
+
+
 def batch_sizes(n_domains=64, nel_range=(2000, 60000), seed=20261203 + 3):
     """Element grids of the batched 2D workload: nel ~ log-U[nel_range], aspect ratio
     nx/ny ~ U[1, 4], nx = round(sqrt(nel r)), ny = max(1, round(nel / nx))."""

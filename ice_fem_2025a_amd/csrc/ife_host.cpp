@@ -105,6 +105,7 @@ struct Model {
     ife_model_opts opt;
     int Step_u = -1, Step_phase = -1, Step_T = 99;
     int field_u[6] = {-1, -1, -1, -1, -1, -1}, field_phase = -1, field_T = -1;
+    int field_L = -1, field_L2 = -1;  // multiplier fields of the phase step (lMult_pf, lMult_pf2)
     int nfu = 0;        // u-step fields: dim, or 6 for the Cosserat model (u and micro-rotations)
     bool coss = false;  // Fracture/CossPhaseField
     bool HasT = false;
@@ -125,7 +126,13 @@ struct Step {
     int nf = 0;
     std::vector<int> fields;      // used fields of the step, DofNames order
     bool has_matrix = false;
-    int64_t M = 0;                // nnz of one field's row block
+    int64_t M = 0;                // nnz of one field's row block (every field pair present)
+    // field pairs present: blk[fp][g] for row field fp and column field g.  Row block fp starts
+    // at rb[fp], holds cnt[fp] column blocks per node row, column field g at block cpos[fp][g]
+    std::vector<std::vector<char>> blk;
+    std::vector<int64_t> rb;
+    std::vector<int> cnt;
+    std::vector<std::vector<int>> cpos;
     std::vector<int64_t> row_ptr;
     std::vector<int32_t> col;
     DevBuf<int64_t> d_row_ptr;
@@ -333,7 +340,10 @@ void plan(ife_physics* ph) {
             std::vector<int> fu;
             for (int d = 0; d < m->nfu; d++) fu.push_back(m->field_u[d]);
             need_exact(m->Step_u, fu);
-            need_exact(m->Step_phase, {m->field_phase});
+            std::vector<int> fph = {m->field_phase};
+            for (int fl : {m->field_L, m->field_L2})
+                if (fl >= 0) fph.push_back(fl);
+            need_exact(m->Step_phase, fph);
         } else {
             need_exact(m->Step_T, {m->field_T});
         }
@@ -381,24 +391,46 @@ void plan(ife_physics* ph) {
     int64_t maxdeg = 0;
     for (int64_t v = 0; v < Ng; v++) maxdeg = std::max(maxdeg, ph->nbr_ptr[v + 1] - ph->nbr_ptr[v]);
     if (maxdeg > 128) THROW(IFE_ERR_UNSUPPORTED, "node degree above 128 (gather rows are staged in LDS)");
-    // ---- per-step CSR: row (fp, v) = [g*Ng + nbr(v) for g in fields] --------------
+    // ---- per-step CSR: row (fp, v) = [g*Ng + nbr(v) for the fields g paired with fp] ----
+    // Every field pair of a step is present except the two multipliers of AugmentedLagrangeMult2
+    // and 3, whose blocks the phase step never inserts (PhaseField.cpp:717-728)
     const int64_t nnz_node = ph->nbr_ptr[Ng];
     for (int s = 0; s < ph->nsteps; s++) {
         Step& st = ph->steps[s];
+        const int nf = st.nf;
+        st.blk.assign(nf, std::vector<char>(nf, 1));
+        for (auto* m : ph->models)
+            if (m->type == MT_PHASEFIELD && m->Step_phase == s && m->field_L >= 0 && m->field_L2 >= 0) {
+                st.blk[ph->fpos[m->field_L]][ph->fpos[m->field_L2]] = 0;
+                st.blk[ph->fpos[m->field_L2]][ph->fpos[m->field_L]] = 0;
+            }
+        st.rb.assign(nf, 0);
+        st.cnt.assign(nf, 0);
+        st.cpos.assign(nf, std::vector<int>(nf, -1));
+        int64_t off = 0;
+        for (int fp = 0; fp < nf; fp++) {
+            for (int g = 0; g < nf; g++)
+                if (st.blk[fp][g]) st.cpos[fp][g] = st.cnt[fp]++;
+            st.rb[fp] = off;
+            off += (int64_t)st.cnt[fp] * nnz_node;
+        }
         st.row_ptr.assign(st.nrows + 1, 0);
         if (!st.has_matrix) { st.M = 0; continue; }
-        const int nf = st.nf;
         st.M = (int64_t)nf * nnz_node;
         for (int fp = 0; fp < nf; fp++)
-            for (int64_t v = 0; v <= Ng; v++) st.row_ptr[(int64_t)fp * Ng + v] = fp * st.M + (int64_t)nf * ph->nbr_ptr[v];
-        st.col.resize((size_t)nf * st.M);
+            for (int64_t v = 0; v <= Ng; v++) st.row_ptr[(int64_t)fp * Ng + v] = st.rb[fp] + (int64_t)st.cnt[fp] * ph->nbr_ptr[v];
+        st.row_ptr[st.nrows] = off;
+        st.col.resize((size_t)off);
         parallel_for(Ng, [&](int64_t v0, int64_t v1) {
             for (int fp = 0; fp < nf; fp++)
                 for (int64_t v = v0; v < v1; v++) {
                     int64_t deg = ph->nbr_ptr[v + 1] - ph->nbr_ptr[v];
                     int32_t* out = st.col.data() + st.row_ptr[(int64_t)fp * Ng + v];
-                    for (int g = 0; g < nf; g++)
-                        for (int64_t k = 0; k < deg; k++) out[g * deg + k] = (int32_t)(g * Ng + ph->nbr[ph->nbr_ptr[v] + k]);
+                    for (int g = 0; g < nf; g++) {
+                        if (!st.blk[fp][g]) continue;
+                        const int c = st.cpos[fp][g];
+                        for (int64_t k = 0; k < deg; k++) out[c * deg + k] = (int32_t)(g * Ng + ph->nbr[ph->nbr_ptr[v] + k]);
+                    }
                 }
         });
     }
@@ -568,6 +600,20 @@ KParams base_params(ife_physics* ph) {
         p.coss = pfm->coss;
         p.cossb = pfm->cossb.p;
         p.fpos_ph = ph->fpos[pfm->field_phase];
+        // the phase step's rows of phi and of the multipliers (absent fields: -1)
+        const Step& sp = ph->steps[pfm->Step_phase];
+        const int lf[3] = {pfm->field_phase, pfm->field_L, pfm->field_L2};
+        p.lag = pfm->field_L >= 0 ? pfm->opt.Irreversible : 0;
+        for (int i = 0; i < 3; i++) {
+            const int fi = lf[i] >= 0 ? ph->fpos[lf[i]] : -1;
+            p.ph_rb[i] = fi >= 0 ? sp.rb[fi] : 0;
+            p.ph_cnt[i] = fi >= 0 ? sp.cnt[fi] : 0;
+            p.ph_fpos[i] = fi;
+            for (int j = 0; j < 3; j++) {
+                const int fj = lf[j] >= 0 ? ph->fpos[lf[j]] : -1;
+                p.ph_col[i][j] = fi >= 0 && fj >= 0 ? sp.cpos[fi][fj] : -1;
+            }
+        }
         p.E = pfm->E.p; p.EOld = pfm->EOld.p; p.PS = pfm->PS.p; p.PSOld = pfm->PSOld.p;
         p.VD = pfm->VD.p; p.VDOld = pfm->VDOld.p; p.DVD = pfm->DVD.p; p.DVDOld = pfm->DVDOld.p;
         p.MH = pfm->MH.p; p.MHOld = pfm->MHOld.p;
@@ -965,6 +1011,21 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             ph->fused[i] = 1;  // AddDofs on the group's unique nodes
             return i;
         };
+        // multiplier DOFs of the Lagrange irreversibility methods on the phase nodes, in the
+        // phase step (PhaseField.cpp:96-101,270-280; CossPhaseField.cpp:101-106,296-306)
+        auto add_multipliers = [&](Model* md) {
+            const int irr = md->opt.Irreversible;
+            if (irr < IFE_IRR_NONE || irr > IFE_IRR_AUGLAGR3) THROW(IFE_ERR_INVALID, "unknown irreversibility method");
+            if (irr < IFE_IRR_LAGRANGE) return;
+            md->field_L = need("lMult_pf");
+            if (ph->fstep[md->field_L] != md->Step_phase)
+                THROW(IFE_ERR_INVALID, "the phase field requires phase and lMult_pf to be in the same solver step");
+            if (irr == IFE_IRR_AUGLAGR2 || irr == IFE_IRR_AUGLAGR3) {
+                md->field_L2 = need("lMult_pf2");
+                if (ph->fstep[md->field_L2] != md->Step_phase)
+                    THROW(IFE_ERR_INVALID, "the phase field requires phase and lMult_pf2 to be in the same solver step");
+            }
+        };
         if (tn == "Fracture/PhaseField") {
             if (ph->pf_model()) THROW(IFE_ERR_UNSUPPORTED, "one Fracture/PhaseField model per physics");
             if (!pf || !opts) THROW(IFE_ERR_INVALID, "phase-field options required");
@@ -996,9 +1057,7 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             int dfn = pf->DistributionFunction;
             if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_LOCAL)  // Custom needs a user callback
                 THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
-            if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE ||
-                  opts->Irreversible == IFE_IRR_PENALTY))
-                THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist, None and Penalty are)");
+            add_multipliers(m.get());
             int dams[] = {pf->DamFunction, pf->DensityDamage, pf->GravityDamage, pf->ViscoDriving};
             for (int d : dams)
                 if (d < IFE_DAM_NONE || d > IFE_DAM_PORDER) THROW(IFE_ERR_INVALID, "unknown damage function");
@@ -1034,9 +1093,7 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             int dfn = pf->DistributionFunction;
             if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_LOCAL)  // Custom needs a user callback
                 THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
-            if (!(opts->Irreversible == IFE_IRR_HIST || opts->Irreversible == IFE_IRR_NONE ||
-                  opts->Irreversible == IFE_IRR_PENALTY))
-                THROW(IFE_ERR_UNSUPPORTED, "irreversibility method not implemented (Hist, None and Penalty are)");
+            add_multipliers(m.get());
             if (mat->l_coss <= 0.0) THROW(IFE_ERR_INVALID, "l_coss must be positive");
         } else if (tn == "Thermal/ThermalModel") {
             if (ph->th_model()) THROW(IFE_ERR_UNSUPPORTED, "one Thermal/ThermalModel per physics");

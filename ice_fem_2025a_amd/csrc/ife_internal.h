@@ -108,6 +108,13 @@ struct KParams {
     StepIO su, sph, sT;
     int fpos_u[6];           // position of ux,uy[,uz][,ox,oy,oz] among the u step's fields
     int fpos_ph, fpos_T;
+    // phase-step rows of (phi, lMult_pf, lMult_pf2): first-row offset of each field's row
+    // block, column blocks per node row, position of the column block of field j in row i
+    // (-1: absent) and the fields' positions in the step (-1: absent).  lag: the multiplier
+    // irreversibility method (IFE_IRR_LAGRANGE .. IFE_IRR_AUGLAGR3), 0 without multipliers.
+    int lag;
+    int64_t ph_rb[3];
+    int ph_cnt[3], ph_col[3][3], ph_fpos[3];
     int fpos_Tth;            // T position in its step (thermal model)
     // history [comp][e*nip + ip]
     double *E, *EOld, *PS, *PSOld, *VD, *VDOld, *DVD, *DVDOld, *MH, *MHOld;

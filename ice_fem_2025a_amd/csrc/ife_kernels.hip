@@ -45,7 +45,10 @@ struct ET {
 template <int DIM>
 struct Rec {
     static constexpr int N = DIM == 2 ? 9 : 27;
-    static constexpr int KU = 0, FU = DIM * DIM * N, KP = FU + DIM, FP = KP + N, KT = FP + 1, FT = KT + N, SIZE = FT + 1;
+    // WL: the element's lumped weight of the node (multiplier irreversibility); one pad
+    // entry keeps the length even
+    static constexpr int KU = 0, FU = DIM * DIM * N, KP = FU + DIM, FP = KP + N, KT = FP + 1, FT = KT + N, WL = FT + 1,
+                         SIZE = FT + 3;
 };
 
 // Voigt row (xx,yy,zz,xy,yz,xz) holding d u_alpha / d x_c (ConstructBMat, utility/math.cpp:3-26)
@@ -1744,8 +1747,8 @@ __device__ __forceinline__ void kpt_store(const Smem<DIM, T>& sm, R* rec, const 
 // gravity (PhaseField.cpp:436-477), lumped driving / capacity, phase viscosity and
 // stabilisation (PhaseField.cpp:693-711), heat capacity (ThermalModel.cpp:128-129).
 template <int DIM, typename T>
-__device__ __forceinline__ void nodal_terms(Smem<DIM, T>& sm, const KParams& p, const int a, const bool du, const bool dph,
-                                            const bool dth) {
+__device__ __forceinline__ double nodal_terms(Smem<DIM, T>& sm, const KParams& p, const int a, const bool du, const bool dph,
+                                              const bool dth) {
     constexpr int NIP = Smem<DIM, T>::NIP;
     const MatConst& mc = p.mc;
     double s = 0;
@@ -1798,6 +1801,7 @@ __device__ __forceinline__ void nodal_terms(Smem<DIM, T>& sm, const KParams& p, 
     }
     sm.kdt[a] = dth ? s * mc.Density_th * mc.cp * p.du_dt : 0.0;
     sm.fnt[a] = dth ? s * mc.Density_th * mc.cp * sm.dT[a] : 0.0;
+    return s;
 }
 
 // T: the tile type of the element contractions (double: FP64 mode; float: mixed mode, FP32
@@ -1909,7 +1913,10 @@ __device__ __forceinline__ void assemble_body(const KParams& p, const int64_t bl
     if (wv < DIM) {
         if (du) kuu_loop<DIM, MT, T>(sm, wv, lk, l16, acc);
     } else if (wv == DIM) {
-        if (lane < N) nodal_terms<DIM, T>(sm, p, lane, du, dph, p.do_th);
+        if (lane < N) {
+            const double wl = nodal_terms<DIM, T>(sm, p, lane, du, dph, p.do_th);
+            if (dph && p.lag) rec[lane * RC::SIZE + RC::WL] = (R)wl;
+        }
         if (dph && lane == 0) {
             double s = 0;
             for (int ip = 0; ip < NIP; ip++) s += sm.ipc[IPB_GW][ip];
@@ -1953,6 +1960,71 @@ __global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_batch(co
 }
 
 // ---------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------
+// Multiplier irreversibility of one node (PhaseFieldUtility::EnforceIrreversible,
+// PhaseFieldUtil.cpp:350-370): LagrangeMult (LagrMultIrr :383-398), AugmentedLagrangeMult
+// (AugLagrIrr :400-438), AugmentedLagrangeMult2 (AugLagrIrr2 :441-478); AugmentedLagrangeMult3
+// adds no terms.  Every term is an element's lumped weight of the node times nodal values, so
+// the node's terms are those with W, the lumped weights summed over its elements.  Limits
+// min(1 - 1e-3, max(0, phi_old)) and 1.1; Gc and l are the input fracture properties,
+// LagFact = 1 (PhaseFieldUtil.h:74).  kpp and fph add to K_phiphi(n,n) and F_phi(n); the rest
+// are the diagonal entries of the (phi,L), (L,phi), (L,L), (phi,L2), (L2,phi), (L2,L2) blocks
+// and the multiplier residuals.
+struct LagTerms {
+    double fph, kpp, kpl, klp, kll, fl, kpl2, kl2p, kl2l2, fl2;
+};
+
+__device__ LagTerms lagrange_terms(int method, double Gc, double l, double W, double ph, double pho, double L, double L2) {
+    LagTerms t{};
+    const double lo = fmin(1.0 - 1.0e-3, fmax(0.0, pho)), hi = 1.0 + 1e-1;
+    if (method == IFE_IRR_LAGRANGE) {
+        t.fph = W * L;
+        t.kpl = W;
+        if (ph >= 1.0) { t.fl = W * (ph - hi); t.klp = W; }          // L (phi - 1.1)
+        else if (ph < pho) { t.fl = W * (ph - lo); t.klp = W; }      // L <phi - phi_old>
+        else { t.fl = W * L; t.kll = W; }
+    } else if (method == IFE_IRR_AUGLAGR) {
+        const double c = Gc / l;  // / LagFact
+        t.fl = -W * c * L * (1.0 + 1.0e-10);
+        t.kll = -W * c * (1.0 + 1.0e-10);
+        t.fph = W * c * L;
+        t.kpl = W * c;
+        const double q = L + (ph - lo);
+        if (q <= 0) {
+            t.fph += W * c * (q - L);
+            t.kpp += W * c;
+            t.fl += W * c * q;
+            t.kll += W * c;
+            t.klp += W * c;
+        }
+    } else if (method == IFE_IRR_AUGLAGR2) {
+        const double c = Gc / l;
+        t.fl = -W * c * L * (1.0 - 1.0e-8);
+        t.kll = -W * c * (1.0 - 1.0e-8);
+        t.fl2 = -W * c * L2 * (1.0 - 1.0e-8);
+        t.kl2l2 = -W * c * (1.0 - 1.0e-8);
+        t.fph = W * c * L + W * c * L2;
+        t.kpl = W * c;
+        t.kpl2 = W * c;
+        const double q = L + (ph - lo), q2 = L2 + (ph - hi);
+        if (q <= 0) {  // lower limit
+            t.fph += W * c * (q - L);
+            t.kpp += W * c;
+            t.fl += W * c * q;
+            t.kll += W * c;
+            t.klp += W * c;
+        }
+        if (q2 >= 0) {  // upper limit
+            t.fph += W * c * (q2 - L2);
+            t.kpp += W * c;
+            t.fl2 += W * c * q2;
+            t.kl2l2 += W * c;
+            t.kl2p += W * c;
+        }
+    }
+    return t;
+}
+
 // ---------------------------------------------------------------------------------------
 // Higher-order crack densities: the first- and second-derivative parts of K_pp, F_p and the
 // crack length, with the shape derivatives of Mesh::getShapeGrads2 (B2_2.cpp:90-125,
@@ -2032,7 +2104,7 @@ __device__ void gj_inverse(double A[M][M], double Ai[M][M]) {
     }
 }
 
-template <int DIM>
+template <int DIM, typename R>
 __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
     using H = HoSmem<DIM>;
     using RC = Rec<DIM>;
@@ -2191,15 +2263,15 @@ __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
                 for (int kk = 0; kk < NK; kk++) lc += sm.gw[i][kk];
         __syncthreads();  // the next chunk overwrites the shared arrays
     }
-    double* const rec = p.ebuf + el * N * RC::SIZE;
+    R* const rec = (R*)p.ebuf + el * N * RC::SIZE;
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < SLOTS; k++) {
         const int o = tid + k * NT;
         if (o >= NOUT) continue;
         bad |= !isfinite(acc[k]);
-        if (o < N * N) rec[(o / N) * RC::SIZE + RC::KP + o % N] += acc[k];
-        else rec[(o - N * N) * RC::SIZE + RC::FP] += acc[k];
+        R* const x = o < N * N ? &rec[(o / N) * RC::SIZE + RC::KP + o % N] : &rec[(o - N * N) * RC::SIZE + RC::FP];
+        *x = (R)((double)*x + acc[k]);
     }
     if (tid == 0) {
         bad |= !isfinite(lc);
@@ -2225,8 +2297,11 @@ constexpr int GW = 64;      // lanes per gather workgroup (one wavefront)
 constexpr int MAXDEG = 128; // neighbour slots per row held in LDS
 constexpr int GPF = 8;      // records loaded ahead
 
-// W consecutive record entries per lane load; one entry per lane keeps the LDS adds of a
-// load on consecutive entries (conflict-free)
+// W consecutive record entries per lane load: one FP64 entry (8 B), or two FP32 entries
+// (8 B) of the mixed mode's records, so that a load moves the same bytes in both modes (the
+// gather is bound by the loads in flight per wavefront).  Record lengths are even, so FP32
+// pairs never straddle two records.
+typedef float flt2 __attribute__((ext_vector_type(2)));
 template <typename T>
 struct RecLoad;
 template <>
@@ -2238,10 +2313,10 @@ struct RecLoad<double> {
 };
 template <>
 struct RecLoad<float> {
-    static constexpr int W = 1;
-    typedef float V;
-    static __device__ __forceinline__ V zero() { return 0.f; }
-    static __device__ __forceinline__ double get(V x, int) { return (double)x; }
+    static constexpr int W = 2;
+    typedef flt2 V;
+    static __device__ __forceinline__ V zero() { return V{0.f, 0.f}; }
+    static __device__ __forceinline__ double get(V x, int w) { return (double)(w == 0 ? x.x : x.y); }
 };
 
 // T: element record type (double, or float in mixed mode); the rows accumulate in FP64.
@@ -2251,8 +2326,9 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
     typedef typename RecLoad<T>::V V;
     constexpr int N = RC::N, W = RecLoad<T>::W, PER = (RC::SIZE + GW * W - 1) / (GW * W);
     static_assert(RC::SIZE % W == 0, "record length must be a whole number of loads");
-    // LDS row accumulators: K_UU [DIM*DIM][MAXDEG], K_pp [MAXDEG], K_TT [MAXDEG], f_U, f_p, f_T
-    constexpr int OKP = DIM * DIM * MAXDEG, OKT = OKP + MAXDEG, OF = OKT + MAXDEG, LEN = OF + DIM + 2;
+    // LDS row accumulators: K_UU [DIM*DIM][MAXDEG], K_pp [MAXDEG], K_TT [MAXDEG], f_U, f_p, f_T,
+    // the node's lumped weight
+    constexpr int OKP = DIM * DIM * MAXDEG, OKT = OKP + MAXDEG, OF = OKT + MAXDEG, OWL = OF + DIM + 2, LEN = OWL + 1;
     __shared__ double acc[LEN];
     const int lane = threadIdx.x;
     const int64_t r = blk;
@@ -2278,6 +2354,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
             else if (t == RC::FP) { bs = OF + DIM; ac = dph; }
             else if (t < RC::FT) { bs = OKT; l = t - RC::KT; ac = p.do_th; }
             else if (t == RC::FT) { bs = OF + DIM + 1; ac = dT; }
+            else if (t == RC::WL) { bs = OWL; ac = dph && p.lag; }
             base[u][w] = bs; lb[u][w] = l; act[u][w] = ac;
             ld[u] = ld[u] || ac;
         }
@@ -2302,7 +2379,8 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
                 // the record's column-slot row: one byte per lane < N, read across lanes below
                 lcv[q] = lane < N ? (int)p.loc[(e * N + a) * N + lane] : 0;
 #pragma unroll
-                for (int u = 0; u < PER; u++) x[q][u] = ld[u] ? rec[lane + u * GW] : RecLoad<T>::zero();
+                for (int u = 0; u < PER; u++)  // loads stay inside the record
+                    x[q][u] = ld[u] && lane + u * GW < RC::SIZE / W ? rec[lane + u * GW] : RecLoad<T>::zero();
             }
 #pragma unroll
             for (int q = 0; q < GPF; q++) {
@@ -2329,9 +2407,33 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
         if (lane < DIM) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = acc[OF + lane];
     }
     if (dph) {
-        double* K = p.sph.K + (int64_t)p.fpos_ph * p.sph.M + (int64_t)p.sph.nf * r0 + (int64_t)p.fpos_ph * deg;
-        for (int i = lane; i < deg; i += GW) K[i] = acc[OKP + i];
-        if (lane == 0) p.sph.f[(int64_t)p.fpos_ph * p.Ng + r] = acc[OF + DIM];
+        // multiplier irreversibility: diagonal entries of the (phi, L, L2) blocks of this
+        // node, from its summed lumped weight and nodal values
+        LagTerms lt{};
+        int self = -1;
+        if (p.lag) {
+            const int64_t e0 = p.n2e_elem[k0];
+            const int a0 = p.n2e_loc[k0];
+            self = p.loc[(e0 * N + a0) * N + a0];  // the node's slot in its own row
+            auto nodal = [&](int i, const double* v) { return p.ph_fpos[i] >= 0 ? v[(int64_t)p.ph_fpos[i] * p.Ng + r] : 0.0; };
+            lt = lagrange_terms(p.lag, p.mc.Gc_in, p.mc.l, acc[OWL], nodal(0, p.sph.x), nodal(0, p.sph.xold),
+                                nodal(1, p.sph.x), nodal(2, p.sph.x));
+        }
+        // rows of phi [and of the multipliers], each column block in field order
+        const double dg[3][3] = {{lt.kpp, lt.kpl, lt.kpl2}, {lt.klp, lt.kll, 0.0}, {lt.kl2p, 0.0, lt.kl2l2}};
+        const double fr[3] = {acc[OF + DIM] + lt.fph, lt.fl, lt.fl2};
+        for (int ri = 0; ri < 3; ri++) {
+            if (p.ph_fpos[ri] < 0) continue;
+            double* const K = p.sph.K + p.ph_rb[ri] + (int64_t)p.ph_cnt[ri] * r0;
+            for (int cj = 0; cj < 3; cj++) {
+                const int cb = p.ph_col[ri][cj];
+                if (cb < 0) continue;
+                double* const Kb = K + (int64_t)cb * deg;
+                for (int i = lane; i < deg; i += GW)
+                    Kb[i] = (ri == 0 && cj == 0 ? acc[OKP + i] : 0.0) + (i == self ? dg[ri][cj] : 0.0);
+            }
+            if (lane == 0) p.sph.f[(int64_t)p.ph_fpos[ri] * p.Ng + r] = fr[ri];
+        }
     }
     if (dT) {
         if (p.do_th) {
@@ -3087,10 +3189,10 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
 hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
     if (p.nlist <= 0) return hipSuccess;
     dim3 grid((unsigned)p.nlist), block(NT);
-    // Element records stay FP64 in both modes: the gather reads FP32 records no faster.
+    // mixed mode: FP32 tiles and FP32 element records (the tiles' accumulators are FP32)
     if (p.mixed) {
-        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, float, double, NT>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((assemble_kernel<3, float, double, NT>), grid, block, 0, s, p);
+        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, float, float, NT>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((assemble_kernel<3, float, float, NT>), grid, block, 0, s, p);
     } else {
         if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, double, double, NT>), grid, block, 0, s, p);
         else hipLaunchKernelGGL((assemble_kernel<3, double, double, NT>), grid, block, 0, s, p);
@@ -3100,16 +3202,27 @@ hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
 
 hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s) {
     if (p.NE == 0) return hipSuccess;
-    if (dim == 2) hipLaunchKernelGGL(ho_phase_kernel<2>, dim3((unsigned)p.NE), dim3(256), 0, s, p);
-    else hipLaunchKernelGGL(ho_phase_kernel<3>, dim3((unsigned)p.NE), dim3(256), 0, s, p);
+    const dim3 grid((unsigned)p.NE), block(256);
+    if (p.mixed) {
+        if (dim == 2) hipLaunchKernelGGL((ho_phase_kernel<2, float>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((ho_phase_kernel<3, float>), grid, block, 0, s, p);
+    } else {
+        if (dim == 2) hipLaunchKernelGGL((ho_phase_kernel<2, double>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((ho_phase_kernel<3, double>), grid, block, 0, s, p);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_gather(const KParams& p, int dim, hipStream_t s) {
     if (p.Ng <= 0) return hipSuccess;
     const dim3 grid((unsigned)p.Ng), block(GW);
-    if (dim == 2) hipLaunchKernelGGL((gather_kernel<2, double>), grid, block, 0, s, p);
-    else hipLaunchKernelGGL((gather_kernel<3, double>), grid, block, 0, s, p);
+    if (p.mixed) {
+        if (dim == 2) hipLaunchKernelGGL((gather_kernel<2, float>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((gather_kernel<3, float>), grid, block, 0, s, p);
+    } else {
+        if (dim == 2) hipLaunchKernelGGL((gather_kernel<2, double>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((gather_kernel<3, double>), grid, block, 0, s, p);
+    }
     return hipGetLastError();
 }
 
@@ -3234,8 +3347,8 @@ hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipS
     const bool ps = persistent(grid, a.tot[1]);
 #define IFE_AB(D, T, L)                                                                               \
     do {                                                                                              \
-        if (ps) hipLaunchKernelGGL((assemble_batch<D, T, double, L, true>), g, dim3(L), 0, s, a.P, bp, a.nd); \
-        else hipLaunchKernelGGL((assemble_batch<D, T, double, L, false>), g, dim3(L), 0, s, a.P, bp, a.nd);  \
+        if (ps) hipLaunchKernelGGL((assemble_batch<D, T, T, L, true>), g, dim3(L), 0, s, a.P, bp, a.nd);      \
+        else hipLaunchKernelGGL((assemble_batch<D, T, T, L, false>), g, dim3(L), 0, s, a.P, bp, a.nd);       \
     } while (0)
     if (a.dim == 3 || nta != 192) {
         if (a.mixed) { if (a.dim == 2) IFE_AB(2, float, NT); else IFE_AB(3, float, NT); }
@@ -3252,13 +3365,14 @@ hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s) 
     if (a.tot[2] <= 0) return hipSuccess;
     const dim3 g = batch_grid(grid, a.tot[2]), b(GW);
     const bool ps = persistent(grid, a.tot[2]);
-    if (a.dim == 2) {
-        if (ps) hipLaunchKernelGGL((gather_batch<2, double, true>), g, b, 0, s, a.P, a.bptr[2], a.nd);
-        else hipLaunchKernelGGL((gather_batch<2, double, false>), g, b, 0, s, a.P, a.bptr[2], a.nd);
-    } else {
-        if (ps) hipLaunchKernelGGL((gather_batch<3, double, true>), g, b, 0, s, a.P, a.bptr[2], a.nd);
-        else hipLaunchKernelGGL((gather_batch<3, double, false>), g, b, 0, s, a.P, a.bptr[2], a.nd);
-    }
+#define IFE_GB(D, R)                                                                            \
+    do {                                                                                        \
+        if (ps) hipLaunchKernelGGL((gather_batch<D, R, true>), g, b, 0, s, a.P, a.bptr[2], a.nd);  \
+        else hipLaunchKernelGGL((gather_batch<D, R, false>), g, b, 0, s, a.P, a.bptr[2], a.nd);    \
+    } while (0)
+    if (a.mixed) { if (a.dim == 2) IFE_GB(2, float); else IFE_GB(3, float); }
+    else { if (a.dim == 2) IFE_GB(2, double); else IFE_GB(3, double); }
+#undef IFE_GB
     return hipGetLastError();
 }
 

@@ -60,7 +60,10 @@ typedef enum { IFE_DAM_NONE = 0, IFE_DAM_STEP2, IFE_DAM_STEP5, IFE_DAM_STEP8, IF
 typedef enum { IFE_DIST_LINEAR = 0, IFE_DIST_CZM, IFE_DIST_LINEAR2, IFE_DIST_QUADRATIC,
                IFE_DIST_HO_LINEAR, IFE_DIST_HO_LINEAR2, IFE_DIST_HO_CZM, IFE_DIST_HO_QUADRATIC,
                IFE_DIST_LOCAL, IFE_DIST_CUSTOM } ife_distribution;
-/* PhaseFieldUtility::IrreversibilityMethods, same order (PhaseFieldUtil.h:49-58) */
+/* PhaseFieldUtility::IrreversibilityMethods, same order (PhaseFieldUtil.h:49-58).  The
+ * multiplier methods add DOF types on the phase nodes, in the phase step: LAGRANGE and
+ * AUGLAGR "lMult_pf", AUGLAGR2 and AUGLAGR3 "lMult_pf" and "lMult_pf2" (PhaseField.h:57-59);
+ * they must be listed in the physics' DofNames. */
 typedef enum { IFE_IRR_NONE = 0, IFE_IRR_HIST, IFE_IRR_PENALTY, IFE_IRR_LAGRANGE,
                IFE_IRR_AUGLAGR, IFE_IRR_AUGLAGR2, IFE_IRR_AUGLAGR3 } ife_irreversibility;
 

@@ -1844,6 +1844,7 @@ struct Model {
     int Step_u = -1, Step_phase = -1, Step_T = 99;
     std::vector<int> fields_u;
     int field_phase = -1, field_T = -1;
+    int field_L = -1, field_L2 = -1;  // Lagrange-multiplier fields of the phase step
     bool HasT = false;
     double g[3] = {0, 0, 0};
     // history [comp][elem][ip]
@@ -2155,7 +2156,123 @@ static void EnforcePenalty(const Model& md, const Material& M, const Vec& Phase,
     }
 }
 
-// ---- PhaseFieldDamage::Assemble_Phase (PhaseField.cpp:485-731), Hist/None/Penalty irreversibility
+static bool lagrange(int method) {
+    return method == IFE_IRR_LAGRANGE || method == IFE_IRR_AUGLAGR || method == IFE_IRR_AUGLAGR2 ||
+           method == IFE_IRR_AUGLAGR3;
+}
+static bool two_multipliers(int method) { return method == IFE_IRR_AUGLAGR2 || method == IFE_IRR_AUGLAGR3; }
+
+// Multiplier terms of PhaseFieldUtility::EnforceIrreversible (PhaseFieldUtil.cpp:350-370) for
+// LagrangeMult (LagrMultIrr :383-398), AugmentedLagrangeMult (AugLagrIrr :400-438) and
+// AugmentedLagrangeMult2 (AugLagrIrr2 :441-478); AugmentedLagrangeMult3 adds nothing there.
+// Limits as in PenaltyIrr; the PhaseFieldUtility's Gc and l are the input values, LagFact = 1
+// (PhaseFieldUtil.h:74).  The n x n blocks are diagonal.
+struct LagBlocks {
+    Vec F_L, F_L2;
+    Mat K_PhL, K_LPh, K_LL, K_PhL2, K_L2Ph, K_L2L2;
+    void reset(int n) {
+        for (Vec* v : {&F_L, &F_L2}) v->assign(n, 0.0);
+        for (Mat* m : {&K_PhL, &K_LPh, &K_LL, &K_PhL2, &K_L2Ph, &K_L2L2}) *m = Mat(n, n);
+    }
+};
+
+static void EnforceLagrange(const Model& md, const Material& M, const Vec& Phase, const Vec& PhaseOld, const Vec& L,
+                            const Vec& L2, const Vec& WLumped, Vec& F_Ph, Mat& K_PhPh, LagBlocks& b) {
+    const int method = md.opt.Irreversible;
+    const double Gc = M.in.Gc, l = M.in.l, LagFact = 1.0;
+    for (size_t n = 0; n < Phase.size(); n++) {
+        const double lim0 = std::min(1.0 - 1.0e-3, std::max(0.0, PhaseOld[n])), lim1 = 1.0 + 1e-1;
+        const double W = WLumped[n];
+        if (method == IFE_IRR_LAGRANGE) {
+            F_Ph[n] += W * L[n];
+            b.K_PhL(n, n) += W;
+            if (Phase[n] >= 1.0) {
+                b.F_L[n] += W * (Phase[n] - lim1);
+                b.K_LPh(n, n) += W;
+            } else if (Phase[n] < PhaseOld[n]) {
+                b.F_L[n] += W * (Phase[n] - lim0);
+                b.K_LPh(n, n) += W;
+            } else {
+                b.F_L[n] += W * L[n];
+                b.K_LL(n, n) += W;
+            }
+        } else if (method == IFE_IRR_AUGLAGR) {
+            const double sc = Gc / l / LagFact, d = LagFact;
+            b.F_L[n] += -W * sc * L[n] * (1.0 + 1.0e-10);
+            b.K_LL(n, n) += -W * sc * (1.0 + 1.0e-10);
+            const double pre = L[n] + LagFact * (Phase[n] - lim0);
+            F_Ph[n] += W * sc * L[n] * d;
+            b.K_PhL(n, n) += W * sc * d;
+            if (pre <= 0) {
+                F_Ph[n] += W * sc * (pre - L[n]) * d;
+                K_PhPh(n, n) += W * sc * d * d;
+                b.F_L[n] += W * sc * pre;
+                b.K_LL(n, n) += W * sc;
+                b.K_LPh(n, n) += W * sc * d;
+            }
+        } else if (method == IFE_IRR_AUGLAGR2) {
+            const double d = LagFact;
+            b.F_L[n] += -W * Gc / l / LagFact * L[n] * (1.0 - 1.0e-8);
+            b.K_LL(n, n) += -W * Gc / l / LagFact * (1.0 - 1.0e-8);
+            b.F_L2[n] += -W * Gc / l / LagFact * L2[n] * (1.0 - 1.0e-8);
+            b.K_L2L2(n, n) += -W * Gc / l / LagFact * (1.0 - 1.0e-8);
+            F_Ph[n] += W * Gc / l / LagFact * L[n] * d;
+            b.K_PhL(n, n) += W * Gc / l / LagFact * d;
+            F_Ph[n] += W * Gc / l / LagFact * L2[n] * d;
+            b.K_PhL2(n, n) += W * Gc / l / LagFact * d;
+            const double pre = L[n] + LagFact * (Phase[n] - lim0);
+            if (pre <= 0) {
+                F_Ph[n] += W * Gc / l / LagFact * (pre - L[n]) * d;
+                K_PhPh(n, n) += W * Gc / l / LagFact * d * d;
+                b.F_L[n] += W * Gc / l / LagFact * pre;
+                b.K_LL(n, n) += W * Gc / l / LagFact;
+                b.K_LPh(n, n) += W * Gc / l / LagFact * d;
+            }
+            const double pre2 = L2[n] + LagFact * (Phase[n] - lim1);
+            if (pre2 >= 0) {
+                F_Ph[n] += W * Gc / l / LagFact * (pre2 - L2[n]) * d;
+                K_PhPh(n, n) += W * Gc / l / LagFact * d * d;
+                b.F_L2[n] += W * Gc / l / LagFact * pre2;
+                b.K_L2L2(n, n) += W * Gc / l / LagFact;
+                b.K_L2Ph(n, n) += W * Gc / l / LagFact * d;
+            }
+        }
+    }
+}
+
+// the phase step's multiplier values of an element and the blocks' scatter
+// (PhaseField.cpp:567-575,717-728; CossPhaseField.cpp:624-632,755-766)
+static void lagrange_values(const Ctx& c, const Model& md, const std::vector<int64_t>& El, std::vector<int64_t>& dL,
+                            std::vector<int64_t>& dL2, Vec& L, Vec& L2) {
+    const int n = (int)El.size();
+    L.assign(n, 0.0);
+    L2.assign(n, 0.0);
+    if (!lagrange(md.opt.Irreversible)) return;
+    std::vector<int> f1 = {md.field_L};
+    getDofForNodes(c, El, f1, dL);
+    for (int a = 0; a < n; a++) L[a] = c.state[IFE_STATE][md.Step_phase][dL[a]];
+    if (two_multipliers(md.opt.Irreversible)) {
+        std::vector<int> f2 = {md.field_L2};
+        getDofForNodes(c, El, f2, dL2);
+        for (int a = 0; a < n; a++) L2[a] = c.state[IFE_STATE][md.Step_phase][dL2[a]];
+    }
+}
+static void scatter_lagrange(const Model& md, Sink& sink, const std::vector<int64_t>& dofsPhase,
+                             const std::vector<int64_t>& dL, const std::vector<int64_t>& dL2, const LagBlocks& b) {
+    if (!lagrange(md.opt.Irreversible)) return;
+    sink.vec(dL, b.F_L);
+    sink.mat(dL, dofsPhase, b.K_LPh);
+    sink.mat(dofsPhase, dL, b.K_PhL);
+    sink.mat(dL, dL, b.K_LL);
+    if (two_multipliers(md.opt.Irreversible)) {
+        sink.vec(dL2, b.F_L2);
+        sink.mat(dL2, dofsPhase, b.K_L2Ph);
+        sink.mat(dofsPhase, dL2, b.K_PhL2);
+        sink.mat(dL2, dL2, b.K_L2L2);
+    }
+}
+
+// ---- PhaseFieldDamage::Assemble_Phase (PhaseField.cpp:485-731)
 static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats& st) {
     const int n = c.n, nip = c.nip, dim = c.dim, nu = dim * n;
     Material M = md.M;
@@ -2170,6 +2287,9 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
     const int su = md.Step_u, sph = md.Step_phase;
     double dt = c.dt;
     const size_t NIPT = (size_t)c.NE * nip;
+    std::vector<int64_t> dL(n), dL2(n);
+    Vec L, L2;
+    LagBlocks lb;
     for (int64_t el = e0; el < e1; el++) {
         getShapeGrads(c, el, Su);
         if (needs_grad2(md.pf.DistributionFunction)) getShapeGrads2(c, el, Sph);  // PhaseField.cpp:528-532
@@ -2183,6 +2303,8 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
             dPhase[a] = c.state[IFE_DSTATE][sph][dofsPhase[a]];
             PhaseOld[a] = c.state[IFE_STATE_OLD][sph][dofsPhase[a]];
         }
+        lagrange_values(c, md, El, dL, dL2, L, L2);
+        lb.reset(n);
         if (md.HasT) {
             getShapeGrads(c, el, ST);
             if (!md.opt.phase_T_from_dof0) getDofForNodes(c, El, fT, dofsT);
@@ -2291,8 +2413,10 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
             K_PhPh(a, a) += WLumped[a] * (M.pf_visc / dt + md.opt.ViscousStab * M.Gc / M.pf_l);
         }
         EnforcePenalty(md, M, Phase, PhaseOld, WLumped, F_Ph, K_PhPh);
+        EnforceLagrange(md, M, Phase, PhaseOld, L, L2, WLumped, F_Ph, K_PhPh, lb);
         sink.vec(dofsPhase, F_Ph);
         sink.mat(dofsPhase, dofsPhase, K_PhPh);
+        scatter_lagrange(md, sink, dofsPhase, dL, dL2, lb);
     }
 }
 
@@ -2485,6 +2609,9 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
     const int su = md.Step_u, sph = md.Step_phase;
     const double dt = c.dt, t = c.t;
     const size_t NIPT = (size_t)c.NE * nip;
+    std::vector<int64_t> dL(n), dL2(n);
+    Vec L, L2;
+    LagBlocks lb;
     for (int64_t el = e0; el < e1; el++) {
         getShapeGrads(c, el, Su);
         if (needs_grad2(md.pf.DistributionFunction)) getShapeGrads2(c, el, Sph);  // PhaseField.cpp:528-532
@@ -2497,6 +2624,8 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
             Phase[a] = c.state[IFE_STATE][sph][dofsPhase[a]];
             PhaseOld[a] = c.state[IFE_STATE_OLD][sph][dofsPhase[a]];
         }
+        lagrange_values(c, md, El, dL, dL2, L, L2);
+        lb.reset(n);
         if (md.HasT) {
             getShapeGrads(c, el, ST);
             for (int a = 0; a < n; a++) T[a] = c.state[IFE_STATE][md.Step_T][dofsT[a]];
@@ -2580,8 +2709,10 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
             K_PhPh(a, a) += WLumped[a] * (M.pf_visc / dt + md.opt.ViscousStab * M.Gc / M.pf_l);
         }
         EnforcePenalty(md, M, Phase, PhaseOld, WLumped, F_Ph, K_PhPh);
+        EnforceLagrange(md, M, Phase, PhaseOld, L, L2, WLumped, F_Ph, K_PhPh, lb);
         sink.vec(dofsPhase, F_Ph);
         sink.mat(dofsPhase, dofsPhase, K_PhPh);
+        scatter_lagrange(md, sink, dofsPhase, dL, dL2, lb);
     }
 }
 
@@ -2956,9 +3087,18 @@ void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_mod
                     if (c->dim == 3) { md.g[0] = 0; md.g[1] = 0; md.g[2] = -9.81; }
                     else { md.g[0] = 0; md.g[1] = -9.81; md.g[2] = 0; }
                 }
-                if (!(md.opt.Irreversible == IFE_IRR_HIST || md.opt.Irreversible == IFE_IRR_NONE ||
-                      md.opt.Irreversible == IFE_IRR_PENALTY))
-                    throw std::invalid_argument("irreversibility method not supported by the oracle");
+                if (md.opt.Irreversible < IFE_IRR_NONE || md.opt.Irreversible > IFE_IRR_AUGLAGR3)
+                    throw std::invalid_argument("unknown irreversibility method");
+                // multiplier DOFs on the phase nodes, in the phase step (PhaseField.cpp:96-101,270-280)
+                if (lagrange(md.opt.Irreversible)) {
+                    md.field_L = need("lMult_pf");
+                    if (c->fstep[md.field_L] != md.Step_phase) throw std::invalid_argument("lMult_pf outside the phase step");
+                    if (two_multipliers(md.opt.Irreversible)) {
+                        md.field_L2 = need("lMult_pf2");
+                        if (c->fstep[md.field_L2] != md.Step_phase)
+                            throw std::invalid_argument("lMult_pf2 outside the phase step");
+                    }
+                }
                 size_t NIPT = (size_t)c->NE * c->nip;
                 md.Energy.assign(NIPT, 0); md.EnergyOld.assign(NIPT, 0);
                 const size_t nps = md.type == MODEL_COSS ? 18 : 6;
@@ -3003,7 +3143,19 @@ void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_mod
                         if (is_pf(md.type)) {
                             std::vector<int64_t> d(c->n * md.fields_u.size());
                             if (s == md.Step_u) { getDofForNodes(*c, El, md.fields_u, d); fn(d); }
-                            if (s == md.Step_phase) { std::vector<int> f1 = {md.field_phase}; getDofForNodes(*c, El, f1, dph); fn(dph); }
+                            if (s == md.Step_phase) {
+                                // K_PhPh, and the multiplier blocks (phi, L) x (phi, L) [and (phi, L2) x (phi, L2)]
+                                std::vector<int> f1 = {md.field_phase};
+                                getDofForNodes(*c, El, f1, dph);
+                                fn(dph);
+                                for (int fl : {md.field_L, md.field_L2}) {
+                                    if (fl < 0) continue;
+                                    std::vector<int> f2 = {md.field_phase, fl};
+                                    std::vector<int64_t> d2(2 * c->n);
+                                    getDofForNodes(*c, El, f2, d2);
+                                    fn(d2);
+                                }
+                            }
                         } else if (s == md.Step_T) {
                             std::vector<int> f1 = {md.field_T};
                             getDofForNodes(*c, El, f1, dT);

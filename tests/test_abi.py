@@ -114,6 +114,24 @@ def test_numbering_and_pattern_match_oracle(name, nel):
             assert np.all(np.diff(row) > 0)
 
 
+@pytest.mark.parametrize("method", ["LagrangeMult", "AugmentedLagrangeMult", "AugmentedLagrangeMult2",
+                                    "AugmentedLagrangeMult3"])
+@pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (4, 3)), ("hex3d_coss", (2, 1, 2))])
+def test_multiplier_numbering_and_pattern_match_oracle(name, nel, method):
+    """The multiplier DOFs of the Lagrange methods share the phase step; with two
+    multipliers the (lMult_pf, lMult_pf2) field pairs are absent from the pattern."""
+    from ice_fem_2025a_amd.config import add_multipliers
+    prob = add_multipliers(make_problem(name, nel=nel), method)
+    e = Physics(prob, device=False)
+    o = Oracle(prob)
+    for f in range(len(prob.dof_names)):
+        assert np.array_equal(e.dof_numbers(f), o.dof_numbers(f))
+    for s in range(o.nsteps):
+        r1, c1 = e.pattern(s)
+        r2, c2 = o.pattern(s)
+        assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+
+
 def test_numbering_is_field_blocked():
     prob = make_problem("hex3d_glen_at2", nel=(2, 2, 2))
     e = Physics(prob, device=False)

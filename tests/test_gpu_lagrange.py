@@ -1,0 +1,62 @@
+"""GPU parity of the Lagrange-multiplier irreversibility methods (LagrangeMult,
+AugmentedLagrangeMult, AugmentedLagrangeMult2, AugmentedLagrangeMult3; PhaseFieldUtil.cpp:350-478,
+multiplier DOFs and blocks PhaseField.cpp:96-101,567-575,717-728 and CossPhaseField.cpp) against
+the oracle: the phase step holds phi and the multiplier fields; pattern identical, K and f of
+every step within 1e-12 (row/step scaled).  The seeded states visit every branch of the
+multiplier terms (config.add_multipliers).  Mixed mode on one case against the 1e-6 bound."""
+import sys
+
+import numpy as np
+import pytest
+
+from ice_fem_2025a_amd.config import add_multipliers, make_problem
+from ice_fem_2025a_amd.engine import Physics
+from oracle.oracle import Oracle
+from parity_util import RTOL_FP64, RTOL_MIXED, csr_err, vec_err
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["LagrangeMult", "AugmentedLagrangeMult", "AugmentedLagrangeMult2", "AugmentedLagrangeMult3"]
+COSS_TIGHT = (("rm_abs_tol", 1e-40), ("rm_rel_tol", 1e-30), ("rm_max_iterations", 60))
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _compare(prob, tol, precision="fp64", tight=()):
+    o, e = Oracle(prob), Physics(prob, precision=precision)
+    for k, v in tight:
+        o.set_param(0, k, v)
+        e.set_param(0, k, v)
+    for s in range(o.nsteps):
+        e.assemble(s)
+        Ke, fe = e.Kf(s)
+        Ko, fo = o.assemble(s)
+        rp, col = o.pattern(s)
+        r2, c2 = e.pattern(s)
+        assert np.array_equal(rp, r2) and np.array_equal(col, c2), f"pattern of step {s}"
+        ek, ef = csr_err(Ke, Ko, rp), vec_err(fe, fo)
+        _say(f"  step {s}: K err {ek:.2e}, f err {ef:.2e}")
+        assert ek <= tol and ef <= tol, f"step {s}"
+    assert e.status()["nonfinite"] == 0
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (6, 5)), ("hex3d_glen_at2", (3, 2, 4))])
+def test_multipliers_against_oracle(require_gpu, name, nel, method):
+    _say(f"[multipliers] {name} nel={nel} {method}")
+    _compare(add_multipliers(make_problem(name, nel=nel), method), RTOL_FP64)
+
+
+@pytest.mark.parametrize("method", ["AugmentedLagrangeMult", "AugmentedLagrangeMult2"])
+def test_multipliers_cosserat_against_oracle(require_gpu, method):
+    _say(f"[multipliers] hex3d_coss (2,2,2) {method}")
+    _compare(add_multipliers(make_problem("hex3d_coss", nel=(2, 2, 2)), method), RTOL_FP64, tight=COSS_TIGHT)
+
+
+def test_multipliers_mixed_mode(require_gpu):
+    _say("[multipliers] hex3d_glen_at2 (3,3,3) AugmentedLagrangeMult2, mixed")
+    _compare(add_multipliers(make_problem("hex3d_glen_at2", nel=(3, 3, 3)), "AugmentedLagrangeMult2"), RTOL_MIXED,
+             precision="mixed")
