@@ -130,6 +130,14 @@ WORKLOADS["hex3d_coss"] = {
     "dt": 1.0,
 }
 
+# 2D Cosserat workload: the Cosserat physics in plane strain with the DOFs of the shipped 2D
+# inputs (ux, uy, oz; TestCases/TriAxialCompression/TriAxial2D.json), 2.5 mm elements as the
+# 3D block.
+WORKLOADS["quad2d_coss"] = copy.deepcopy(WORKLOADS["hex3d_coss"])
+WORKLOADS["quad2d_coss"]["index"] = 6
+WORKLOADS["quad2d_coss"]["mesh"] = {"dim": 2, "ipcount1D": 3, "nel": [64, 32], "lengths": [0.16, 0.08]}
+WORKLOADS["quad2d_coss"]["Dofs"] = {"DofNames": ["ux", "uy", "oz", "phase", "T"], "DofStep": [0, 0, 0, 1, 2]}
+
 # One domain of the batched 2D workload: the 2D LE/AT2 physics plus temperature and the
 # thermal model (dofs u(2)+phi+T).
 WORKLOADS["quad2d_thermal"] = copy.deepcopy(WORKLOADS["quad2d_le_at2"])
@@ -328,17 +336,19 @@ def make_problem(name, nel=None, seed=None, h=None):
             prob.node_state[(abi.IFE_STATE, c)] = uold + du
             prob.node_state[(abi.IFE_STATE_OLD, c)] = uold
             prob.node_state[(abi.IFE_DSTATE, c)] = du / dt
-        # the Cosserat workload's PhiViscLim is 0.2: phi ~ U[0, 0.4) there, so that about half
+        # the Cosserat workloads' PhiViscLim is 0.2: phi ~ U[0, 0.4) there, so that about half
         # of the IPs run the visco-plastic map (U[0, 0.9) elsewhere)
-        phi_hi = 0.4 if "ox" in prob.dof_names else 0.9
+        is_coss = prob.models[0].type_name == "Fracture/CossPhaseField"
+        coss = [c for c in ("ox", "oy", "oz") if is_coss and c in prob.dof_names]
+        phi_hi = 0.4 if coss else 0.9
         prob.node_state[(abi.IFE_STATE, "phase")] = rng.uniform(0.0, phi_hi, NN)
         prob.node_state[(abi.IFE_STATE_OLD, "phase")] = prob.node_state[(abi.IFE_STATE, "phase")].copy()
         prob.node_state[(abi.IFE_STATE, "T")] = rng.uniform(250.0, 273.0, NN)
         prob.node_state[(abi.IFE_DSTATE, "T")] = rng.uniform(-1e-2, 1e-2, NN)
         prob.history[(0, "EnergyOld")] = rng.uniform(0.0, 5.0e3, (1, E * nip))
         prob.history[(0, "Energy")] = prob.history[(0, "EnergyOld")].copy()
-        if "ox" in prob.dof_names:  # micro-rotations: log-uniform magnitudes in [1e-6, 1e-3] rad
-            for c in ("ox", "oy", "oz"):
+        if coss:  # micro-rotations: log-uniform magnitudes in [1e-6, 1e-3] rad
+            for c in coss:
                 mag = np.exp(rng.uniform(np.log(1e-6), np.log(1e-3), NN))
                 prob.node_state[(abi.IFE_STATE, c)] = mag * rng.choice([-1.0, 1.0], NN)
     if name == "hex3d_adjoint":

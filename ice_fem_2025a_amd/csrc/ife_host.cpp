@@ -709,7 +709,7 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
         // Cosserat: the material kernel updates every IP; the u step's 162-dof element
         // blocks go straight into the zeroed CSR values colour by colour; the phase and T
         // steps use the shared element and gather kernels
-        timed(ph->mat_recs, 0, [&] { return launch_coss_material(p, s); });
+        timed(ph->mat_recs, 0, [&] { return launch_coss_material(p, ph->dim, s); });
         if (p.do_u) {
             const Step& su = ph->steps[pfm->Step_u];
             HIPCHK(launch_fill(p.su.K, (int64_t)su.col.size(), 0.0, s));
@@ -718,7 +718,7 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
                 KParams q = p;
                 q.elist = ph->d_colors.p + ph->color_ptr[c];
                 q.nlist = ph->color_ptr[c + 1] - ph->color_ptr[c];
-                timed(ph->recs, q.nlist, [&] { return launch_coss_assemble(q, s); });
+                timed(ph->recs, q.nlist, [&] { return launch_coss_assemble(q, ph->dim, s); });
             }
         }
         KParams q = p;
@@ -1065,17 +1065,18 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             // CossPhaseField::Setup (CossPhaseField.cpp:168-280)
             if (ph->pf_model()) THROW(IFE_ERR_UNSUPPORTED, "one phase-field model per physics");
             if (!pf || !opts) THROW(IFE_ERR_INVALID, "phase-field options required");
-            if (ph->dim != 3) THROW(IFE_ERR_UNSUPPORTED, "Fracture/CossPhaseField is implemented for 3D meshes");
             m->type = MT_PHASEFIELD;
             m->coss = true;
             m->pf = *pf;
             m->opt = *opts;
             m->opt.LumpedPFCapacity = 0;  // CossPhaseField has no lumped capacity / driving forms
             m->opt.LumpedDriving = 0;
-            const char* un[6] = {"ux", "uy", "uz", "ox", "oy", "oz"};
-            m->nfu = 6;
-            for (int d = 0; d < 6; d++) m->field_u[d] = need(un[d]);
-            for (int d = 1; d < 6; d++)
+            // u-step fields: ux uy uz ox oy oz (3D, :172-183), ux uy oz (2D, CossPhaseField.h:15)
+            const char* un3[6] = {"ux", "uy", "uz", "ox", "oy", "oz"};
+            const char* un2[3] = {"ux", "uy", "oz"};
+            m->nfu = ph->dim == 3 ? 6 : 3;
+            for (int d = 0; d < m->nfu; d++) m->field_u[d] = need(ph->dim == 3 ? un3[d] : un2[d]);
+            for (int d = 1; d < m->nfu; d++)
                 if (ph->fstep[m->field_u[d]] != ph->fstep[m->field_u[0]])
                     THROW(IFE_ERR_INVALID, "Fracture/CossPhaseField requires the u and w fields in one solver step");
             m->Step_u = ph->fstep[m->field_u[0]];

@@ -161,8 +161,8 @@ hipError_t launch_gather(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s);
 // Cosserat: material update of every IP (u, phase, T coefficients), then the u-step element
 // kernel over one colour of elements (adds into the zeroed CSR values and f)
-hipError_t launch_coss_material(const KParams& p, hipStream_t s);
-hipError_t launch_coss_assemble(const KParams& p, hipStream_t s);
+hipError_t launch_coss_material(const KParams& p, int dim, hipStream_t s);
+hipError_t launch_coss_assemble(const KParams& p, int dim, hipStream_t s);
 int record_size(int dim);
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);

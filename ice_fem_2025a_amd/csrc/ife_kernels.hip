@@ -2752,22 +2752,27 @@ __device__ double coss_spec_energy(const CossD& D, const double* ee) {
     return fmax(0.0, 0.5 * E);
 }
 
+template <int DIM>
 struct CossSmem {
-    static constexpr int N = 27, NIP = 27, EPB = NT / 27;
+    // u-step fields: ux uy uz ox oy oz (3D), ux uy oz (2D)
+    static constexpr int N = DIM == 2 ? 9 : 27, NIP = N, NFU = DIM == 2 ? 3 : 6, EPB = NT / NIP;
     double X[EPB][N][4];
-    double U[EPB][6][N], Uo[EPB][6][N], Ph[EPB][N], PhO[EPB][N], T[EPB][N];
-    double C1[EPB][3][9];
+    double U[EPB][NFU][N], Uo[EPB][NFU][N], Ph[EPB][N], PhO[EPB][N], T[EPB][N];
+    double C1[EPB][DIM][9];
     unsigned cnt[4];
 };
 
 // Material-point kernel of the Cosserat model: one lane per IP.  Shape functions and
-// geometry as the solid kernel forms them; the u step (CossPhaseField.cpp:388-461: 18-strain,
+// geometry as the solid kernel forms them (2D: B2_2's transposed Jacobian, unit weights);
+// the u step (CossPhaseField.cpp:388-461: 18-strain,
 // CosseratMaterial::UpdatePlasticStrains, energy split) writes w (Dam D_coss DPlast), w sigma
 // and the lumped weights to cossb; the phase step (:645-733) and the T step (:790-802) write
 // the same per-IP coefficients as the solid model for the shared element kernel.
+template <int DIM>
 __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
-    constexpr int N = 27, NIP = 27, EPB = CossSmem::EPB;
-    __shared__ CossSmem sm;
+    using SM = CossSmem<DIM>;
+    constexpr int N = SM::N, NIP = SM::NIP, EPB = SM::EPB, NFU = SM::NFU;
+    __shared__ SM sm;
     const int tid = threadIdx.x;
     const MatConst& mc = p.mc;
     const int64_t e0 = (int64_t)blockIdx.x * EPB;
@@ -2779,7 +2784,7 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
         const int v = p.enode[el * N + a], r = p.erank[el * N + a];
         const double* nd = p.nodes + (int64_t)v * 4;
         for (int c = 0; c < 4; c++) sm.X[le][a][c] = nd[c];
-        for (int d = 0; d < 6; d++) {
+        for (int d = 0; d < NFU; d++) {
             const int64_t dd = (int64_t)p.fpos_u[d] * p.Ng + r;
             sm.U[le][d][a] = p.su.x[dd];
             sm.Uo[le][d][a] = p.su.xold[dd];
@@ -2789,10 +2794,10 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
         sm.PhO[le][a] = p.sph.xold[dp];
         sm.T[le][a] = p.has_T ? p.sT.x[(int64_t)p.fpos_T * p.Ng + r] : 0.0;
     }
-    for (int i = tid; i < EPB * 27; i += NT) {
-        const int le = i / 27, k = i % 27;
+    for (int i = tid; i < EPB * DIM * 9; i += NT) {
+        const int le = i / (DIM * 9), k = i % (DIM * 9);
         const int64_t el = e0 + le;
-        if (el < p.NE) (&sm.C1[le][0][0])[k] = p.ext[el * 27 + k];
+        if (el < p.NE) (&sm.C1[le][0][0])[k] = p.ext[el * DIM * 9 + k];
     }
     __syncthreads();
     const int le = tid / NIP, ip = tid % NIP;
@@ -2803,8 +2808,8 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
         double* const ipo = p.ipb + el * IPB_NF * NIP + ip;
         double* const cbo = p.cossb + el * COSS_NF * NIP + ip;
         const int q[3] = {ip % 3, (ip / 3) % 3, ip / 9};
-        double b1[3][3], g1[3][3];
-        for (int d = 0; d < 3; d++)
+        double b1[DIM][3], g1[DIM][3];
+        for (int d = 0; d < DIM; d++)
             for (int a = 0; a < 3; a++) {
                 double s = 0, sd = 0;
                 for (int k = 0; k < 3; k++) {
@@ -2815,53 +2820,76 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
                 g1[d][a] = sd;
             }
         // W-weighted sums against the nodal fields (see material_body)
-        enum { F_ONE = 0, F_X = 1, F_PH = 4, F_PHO, F_U, F_UO = F_U + 6, F_T = F_UO + 6, NFLD };
-        double SC[NFLD], SG[3][NFLD];
-        for (int k = 0; k < NFLD; k++) { SC[k] = 0; SG[0][k] = 0; SG[1][k] = 0; SG[2][k] = 0; }
+        enum { F_ONE = 0, F_X = 1, F_PH = F_X + DIM, F_PHO, F_U, F_UO = F_U + NFU, F_T = F_UO + NFU, NFLD };
+        double SC[NFLD], SG[DIM][NFLD];
+        for (int k = 0; k < NFLD; k++) {
+            SC[k] = 0;
+            for (int c = 0; c < DIM; c++) SG[c][k] = 0;
+        }
 #pragma unroll 1
         for (int a = 0; a < N; a++) {
             const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
-            const double cb = b1[0][ax] * b1[1][ay] * b1[2][az];
-            const double gb[3] = {g1[0][ax] * b1[1][ay] * b1[2][az], b1[0][ax] * g1[1][ay] * b1[2][az],
-                                  b1[0][ax] * b1[1][ay] * g1[2][az]};
-            const double Wa = sm.X[le][a][3];
+            double cb, gb[DIM];
+            if (DIM == 3) {
+                cb = b1[0][ax] * b1[1][ay] * b1[DIM - 1][az];
+                gb[0] = g1[0][ax] * b1[1][ay] * b1[DIM - 1][az];
+                gb[1] = b1[0][ax] * g1[1][ay] * b1[DIM - 1][az];
+                gb[DIM - 1] = b1[0][ax] * b1[1][ay] * g1[DIM - 1][az];
+            } else {
+                cb = b1[0][ax] * b1[1][ay];
+                gb[0] = g1[0][ax] * b1[1][ay];
+                gb[DIM - 1] = b1[0][ax] * g1[DIM - 1][ay];
+            }
+            const double Wa = DIM == 3 ? sm.X[le][a][3] : 1.0;
             double fv[NFLD];
             fv[F_ONE] = 1.0;
-            for (int d = 0; d < 3; d++) fv[F_X + d] = sm.X[le][a][d];
+            for (int d = 0; d < DIM; d++) fv[F_X + d] = sm.X[le][a][d];
             fv[F_PH] = sm.Ph[le][a];
             fv[F_PHO] = sm.PhO[le][a];
-            for (int d = 0; d < 6; d++) { fv[F_U + d] = sm.U[le][d][a]; fv[F_UO + d] = sm.Uo[le][d][a]; }
+            for (int d = 0; d < NFU; d++) { fv[F_U + d] = sm.U[le][d][a]; fv[F_UO + d] = sm.Uo[le][d][a]; }
             fv[F_T] = sm.T[le][a];
             for (int k = 0; k < NFLD; k++) {
                 SC[k] += Wa * cb * fv[k];
-                for (int c = 0; c < 3; c++) SG[c][k] += Wa * gb[c] * fv[k];
+                for (int c = 0; c < DIM; c++) SG[c][k] += Wa * gb[c] * fv[k];
             }
         }
-        const double wF = SC[F_ONE], rw = 1.0 / wF;
-        const double dW[3] = {SG[0][F_ONE], SG[1][F_ONE], SG[2][F_ONE]};
-        auto val = [&](int k) { return SC[k] * rw; };
-        auto rgrad = [&](int k, int c) { return SG[c][k] * rw - dW[c] * SC[k] * rw * rw; };
-        double J[3][3];
-        for (int c = 0; c < 3; c++)
-            for (int d = 0; d < 3; d++) J[c][d] = rgrad(F_X + d, c);
-        double Ji[3][3];
-        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
-                     c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        const double det = J[0][0] * c00 + J[0][1] * c10 + J[0][2] * c20, id = 1.0 / det;
-        Ji[0][0] = c00 * id; Ji[1][0] = c10 * id; Ji[2][0] = c20 * id;
-        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-        const double dt3 = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[1][0] * (J[0][1] * J[2][2] - J[0][2] * J[2][1]) +
-                           J[2][0] * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
-        const double w = p.wq[q[0]] * p.wq[q[1]] * p.wq[q[2]] * fabs(dt3);
-        for (int c = 0; c < 3; c++) {
+        // 3D: rational basis (NURBS3_Cube2.cpp:96-112); 2D: N = CB, GXi = GB (B2_2.cpp:72-88)
+        const double rw = DIM == 3 ? 1.0 / SC[F_ONE] : 1.0;
+        double dW[DIM];
+        for (int c = 0; c < DIM; c++) dW[c] = DIM == 3 ? SG[c][F_ONE] : 0.0;
+        auto val = [&](int k) { return DIM == 3 ? SC[k] * rw : SC[k]; };
+        auto rgrad = [&](int k, int c) { return DIM == 3 ? SG[c][k] * rw - dW[c] * SC[k] * rw * rw : SG[c][k]; };
+        double J[DIM][DIM];
+        for (int c = 0; c < DIM; c++)
+            for (int d = 0; d < DIM; d++) J[c][d] = rgrad(F_X + d, c);
+        double Ji[DIM][DIM], w;
+        if constexpr (DIM == 3) {
+            const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c10 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                         c20 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+            const double det = J[0][0] * c00 + J[0][1] * c10 + J[0][2] * c20, id = 1.0 / det;
+            Ji[0][0] = c00 * id; Ji[1][0] = c10 * id; Ji[2][0] = c20 * id;
+            Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+            Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+            Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+            Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+            Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+            Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+            const double dt3 = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                               J[1][0] * (J[0][1] * J[2][2] - J[0][2] * J[2][1]) + J[2][0] * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+            w = p.wq[q[0]] * p.wq[q[1]] * p.wq[q[2]] * fabs(dt3);
+        } else {  // J = (GB X)^T, then inv(J) applied to GB (B2_2.cpp:79-85)
+            const double Jt[2][2] = {{J[0][0], J[1][0]}, {J[0][1], J[1][1]}};
+            const double det = Jt[0][0] * Jt[1][1] - Jt[0][1] * Jt[1][0], id = 1.0 / det;
+            Ji[0][0] = Jt[1][1] * id;
+            Ji[1][0] = -Jt[1][0] * id;
+            Ji[0][1] = -Jt[0][1] * id;
+            Ji[1][1] = Jt[0][0] * id;
+            w = p.wq[q[0]] * p.wq[q[1]] * fabs(det);
+        }
+        for (int c = 0; c < DIM; c++) {
             double qq = 0;
-            for (int d = 0; d < 3; d++) {
-                ipo[(IPB_JI + c * 3 + d) * NIP] = Ji[c][d] * rw;
+            for (int d = 0; d < DIM; d++) {
+                ipo[(IPB_JI + c * DIM + d) * NIP] = Ji[c][d] * rw;
                 qq += Ji[c][d] * dW[d];
             }
             ipo[(IPB_DW + c) * NIP] = qq * rw * rw;
@@ -2869,9 +2897,13 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
         ipo[IPB_WF * NIP] = rw;
         ipo[IPB_W * NIP] = w;
         auto grad = [&](int k, double g[3]) {
-            double r3[3];
-            for (int d = 0; d < 3; d++) r3[d] = rgrad(k, d);
-            for (int c = 0; c < 3; c++) g[c] = Ji[c][0] * r3[0] + Ji[c][1] * r3[1] + Ji[c][2] * r3[2];
+            double r3[DIM];
+            for (int d = 0; d < DIM; d++) r3[d] = rgrad(k, d);
+            for (int c = 0; c < 3; c++) {
+                g[c] = 0.0;
+                if (c < DIM)
+                    for (int d = 0; d < DIM; d++) g[c] += Ji[c][d] * r3[d];
+            }
         };
         const double phi = val(F_PH), phio = val(F_PHO);
         double gphi[3], gT[3] = {0, 0, 0};
@@ -2880,19 +2912,28 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
         if (p.has_T) grad(F_T, gT);
         const double T0 = p.has_T ? p.sT.x[0] : 0.0;  // Assemble_Phase reads T dof 0 for every node (:575)
         const double Tph = p.has_T ? T0 * val(F_ONE) : mc.T_default;
-        // 18-strain (ConstructBMatCoss): grad u, N w, l grad w
-        double gu[6][3], wv[3];
-        for (int a = 0; a < 6; a++) grad(F_U + a, gu[a]);
-        for (int k = 0; k < 3; k++) wv[k] = val(F_U + 3 + k);
+        // 18-strain (ConstructBMatCoss, :1168-1223): grad u, N w, l grad w
+        double gu[NFU][3];
+        for (int a = 0; a < NFU; a++) grad(F_U + a, gu[a]);
         const double lc = mc.l_coss;
         double eps[18];
-        eps[0] = gu[0][0]; eps[1] = gu[1][1]; eps[2] = gu[2][2];
-        eps[3] = gu[1][0] - wv[2]; eps[4] = gu[0][1] + wv[2];
-        eps[5] = gu[2][1] - wv[0]; eps[6] = gu[1][2] + wv[0];
-        eps[7] = gu[2][0] - wv[1]; eps[8] = gu[0][2] + wv[1];
-        eps[9] = lc * gu[3][0]; eps[10] = lc * gu[4][1]; eps[11] = lc * gu[5][2];
-        eps[12] = lc * gu[3][0]; eps[13] = lc * gu[4][1]; eps[14] = lc * gu[4][1];
-        eps[15] = lc * gu[5][2]; eps[16] = lc * gu[3][0]; eps[17] = lc * gu[5][2];
+        if constexpr (DIM == 3) {
+            double wv[3];
+            for (int k = 0; k < 3; k++) wv[k] = val(F_U + 3 + k);
+            eps[0] = gu[0][0]; eps[1] = gu[1][1]; eps[2] = gu[2][2];
+            eps[3] = gu[1][0] - wv[2]; eps[4] = gu[0][1] + wv[2];
+            eps[5] = gu[2][1] - wv[0]; eps[6] = gu[1][2] + wv[0];
+            eps[7] = gu[2][0] - wv[1]; eps[8] = gu[0][2] + wv[1];
+            eps[9] = lc * gu[3][0]; eps[10] = lc * gu[4][1]; eps[11] = lc * gu[5][2];
+            eps[12] = lc * gu[3][0]; eps[13] = lc * gu[4][1]; eps[14] = lc * gu[4][1];
+            eps[15] = lc * gu[5][2]; eps[16] = lc * gu[3][0]; eps[17] = lc * gu[5][2];
+        } else {  // [ux uy oz]: xx, yy, xy, yx and l grad w_z in rows 14 and 16 (:1174-1185)
+            const double wz = val(F_U + 2);
+            for (int r = 0; r < 18; r++) eps[r] = 0.0;
+            eps[0] = gu[0][0]; eps[1] = gu[1][1];
+            eps[3] = gu[1][0] - wz; eps[4] = gu[0][1] + wz;
+            eps[14] = lc * gu[2][0]; eps[16] = lc * gu[2][1];
+        }
         double epl[18];
         if (p.do_u || p.do_ph)
             for (int r = 0; r < 18; r++) epl[r] = p.PS[r * S + gi];
@@ -2994,13 +3035,14 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
                 FN += w * dDamV * PD;
                 A += w * ddDamV * PD;
             }
-            if (mc.Inertia && mc.ConsistentDegradation) {  // six velocity components
+            if (mc.Inertia && mc.ConsistentDegradation) {  // every u-step velocity component
                 double v2 = 0;
-                for (int d = 0; d < 6; d++) {
+                for (int d = 0; d < NFU; d++) {
                     double vd = 0;
                     for (int a = 0; a < N; a++) {
                         const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
-                        const double Na = sm.X[le][a][3] * b1[0][ax] * b1[1][ay] * b1[2][az] * rw;
+                        const double Na = DIM == 3 ? sm.X[le][a][3] * b1[0][ax] * b1[1][ay] * b1[DIM - 1][az] * rw
+                                                   : b1[0][ax] * b1[1][ay];
                         vd += Na * p.su.dx[(int64_t)p.fpos_u[d] * p.Ng + p.erank[el * N + a]];
                     }
                     v2 += vd * vd;
@@ -3010,14 +3052,14 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
             }
             if (mc.ConsistentDegradation) {
                 double gu_ = 0;
-                for (int d = 0; d < 3; d++) gu_ += mc.g[d] * val(F_U + d);
+                for (int d = 0; d < DIM; d++) gu_ += mc.g[d] * val(F_U + d);
                 FN += -w * mc.Density * gu_ * dDamG;
                 A += -w * mc.Density * gu_ * ddDamG;
             }
             ipo[(IPB_PHA) * NIP] = A;
             ipo[(IPB_PHB) * NIP] = w * m.Gc * cGG;
             ipo[(IPB_PHFN) * NIP] = FN;
-            for (int c = 0; c < 3; c++) ipo[(IPB_PHFG + c) * NIP] = w * m.Gc * cGG * gphi[c];
+            for (int c = 0; c < DIM; c++) ipo[(IPB_PHFG + c) * NIP] = w * m.Gc * cGG * gphi[c];
             ipo[(IPB_LDRIVE) * NIP] = 0.0;
             ipo[(IPB_GW) * NIP] = gamma * w;
             ipo[(IPB_GC) * NIP] = m.Gc;
@@ -3028,7 +3070,7 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
             if (p.do_Tpf) fn = -w * ((VD - p.VDOld[gi]) / p.dt);
             ipo[(IPB_THFN) * NIP] = fn;
             ipo[(IPB_THB) * NIP] = p.do_th ? w * mc.k : 0.0;
-            for (int c = 0; c < 3; c++) ipo[(IPB_THFG + c) * NIP] = p.do_th ? w * mc.k * gT[c] : 0.0;
+            for (int c = 0; c < DIM; c++) ipo[(IPB_THFG + c) * NIP] = p.do_th ? w * mc.k * gT[c] : 0.0;
         }
     }
     if (my_calls) {
@@ -3059,18 +3101,33 @@ __constant__ const signed char coss_bas[6][5] = {
     {1, 2, 3, 0, 0}, {2, 1, 3, 0, 0}, {3, 2, 1, 0, 0}, {0, 0, 1, 1, 1}, {0, 0, 2, 2, 2}, {0, 0, 3, 3, 3}};
 __constant__ const signed char coss_cof[6][5] = {
     {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {1, 0, 2, 2, 2}, {1, 0, 2, 2, 2}, {1, 0, 2, 2, 2}};
+// 2D fields ux uy oz (:1174-1185)
+__constant__ const signed char coss2_nrow[3] = {2, 2, 4};
+__constant__ const signed char coss2_row[3][4] = {{0, 4, 0, 0}, {1, 3, 0, 0}, {3, 4, 14, 16}};
+__constant__ const signed char coss2_bas[3][4] = {{1, 2, 0, 0}, {2, 1, 0, 0}, {0, 0, 1, 2}};
+__constant__ const signed char coss2_cof[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {1, 0, 2, 2}};
+
+template <int DIM>
+struct CossRows {
+    static __device__ __forceinline__ int nrow(int f) { return DIM == 3 ? coss_nrow[f] : coss2_nrow[f]; }
+    static __device__ __forceinline__ int row(int f, int x) { return DIM == 3 ? coss_row[f][x] : coss2_row[f][x]; }
+    static __device__ __forceinline__ int bas(int f, int x) { return DIM == 3 ? coss_bas[f][x] : coss2_bas[f][x]; }
+    static __device__ __forceinline__ int cof(int f, int x) { return DIM == 3 ? coss_cof[f][x] : coss2_cof[f][x]; }
+};
 
 // Cosserat u-step element kernel: one workgroup per element of one colour (no two elements
 // of a colour share a node, so the element adds into its rows of the zeroed CSR values and
 // f without atomics).  K_ab(alpha a, beta b) = sum_ip sum_kl Phi_k(a) M^ab_kl Phi_l(b) with
-// Phi = (N, G_x, G_y, G_z) and M^ab = P_a^T (w Dam D_coss DPlast) P_b, P_a the field's
-// B-matrix rows; one field pair at a time, its 4x4 blocks for all IPs staged in LDS.
+// Phi = (N, G_x, G_y[, G_z]) and M^ab = P_a^T (w Dam D_coss DPlast) P_b, P_a the field's
+// B-matrix rows; one field pair at a time, its (1+DIM)^2 blocks for all IPs staged in LDS.
+template <int DIM>
 __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
-    constexpr int N = 27, NIP = 27, NP = 28;
-    __shared__ double Phi[NIP][4][NP];
-    __shared__ double Mb[NIP][16];
+    using CR = CossRows<DIM>;
+    constexpr int N = DIM == 2 ? 9 : 27, NIP = N, NP = DIM == 2 ? 9 : 28, NB = 1 + DIM, NFU = DIM == 2 ? 3 : 6;
+    __shared__ double Phi[NIP][NB][NP];
+    __shared__ double Mb[NIP][NB * NB];
     __shared__ double Xs[N][4];
-    __shared__ double b1s[3][3][3], db1s[3][3][3];
+    __shared__ double b1s[DIM][3][3], db1s[DIM][3][3];
     __shared__ int32_t rk[N];
     __shared__ int64_t rp0[N], rdeg[N];
     const int tid = threadIdx.x;
@@ -3086,9 +3143,9 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
         rp0[tid] = p.nbr_ptr[r];
         rdeg[tid] = p.nbr_ptr[r + 1] - p.nbr_ptr[r];
     }
-    if (tid < 27) {
+    if (tid < DIM * 9) {
         const int d = tid / 9, q = (tid / 3) % 3, a = tid % 3;
-        const double* C = p.ext + (el * 3 + d) * 9;
+        const double* C = p.ext + (el * DIM + d) * 9;
         double s = 0, sd = 0;
         for (int k = 0; k < 3; k++) { s += C[a * 3 + k] * p.B1[q][k]; sd += C[a * 3 + k] * p.dB1[q][k]; }
         b1s[d][q][a] = s;
@@ -3097,16 +3154,16 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
     __syncthreads();
     for (int i = tid; i < NIP * NP; i += NT) {
         const int ip = i / NP, a = i % NP;
-        if (a >= N) { for (int k = 0; k < 4; k++) Phi[ip][k][a] = 0.0; continue; }
+        if (a >= N) { for (int k = 0; k < NB; k++) Phi[ip][k][a] = 0.0; continue; }
         const int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9, ax = a % 3, ay = (a / 3) % 3, az = a / 9;
-        const double bx = b1s[0][qx][ax], by = b1s[1][qy][ay], bz = b1s[2][qz][az];
-        const double gx = db1s[0][qx][ax], gy = db1s[1][qy][ay], gz = db1s[2][qz][az];
+        const double bx = b1s[0][qx][ax], by = b1s[1][qy][ay], bz = DIM == 3 ? b1s[DIM - 1][qz][az] : 1.0;
+        const double gx = db1s[0][qx][ax], gy = db1s[1][qy][ay], gz = DIM == 3 ? db1s[DIM - 1][qz][az] : 0.0;
         const double cbv = bx * by * bz, gb[3] = {gx * by * bz, bx * gy * bz, bx * by * gz};
-        const double Wa = Xs[a][3];
+        const double Wa = DIM == 3 ? Xs[a][3] : 1.0;  // 2D: N = CB, G = JI GB (IPB_WF = 1, IPB_DW = 0)
         Phi[ip][0][a] = Wa * cbv * ipc[IPB_WF * NIP + ip];
-        for (int c = 0; c < 3; c++) {
+        for (int c = 0; c < DIM; c++) {
             double s = 0;
-            for (int d = 0; d < 3; d++) s += ipc[(IPB_JI + c * 3 + d) * NIP + ip] * gb[d];
+            for (int d = 0; d < DIM; d++) s += ipc[(IPB_JI + c * DIM + d) * NIP + ip] * gb[d];
             Phi[ip][1 + c][a] = Wa * (s - cbv * ipc[(IPB_DW + c) * NIP + ip]);
         }
     }
@@ -3115,20 +3172,19 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
     auto cof = [&](int c) { return c == 0 ? 1.0 : (c == 1 ? -1.0 : lc); };
     double bad = 0.0;
 #pragma unroll 1
-    for (int blk = 0; blk < 36; blk++) {
-        const int al = blk / 6, be = blk % 6;
-        for (int i = tid; i < NIP * 16; i += NT) {
-            const int ip = i / 16, k = (i / 4) % 4, l = i % 4;
+    for (int blk = 0; blk < NFU * NFU; blk++) {
+        const int al = blk / NFU, be = blk % NFU;
+        for (int i = tid; i < NIP * NB * NB; i += NT) {
+            const int ip = i / (NB * NB), k = (i / NB) % NB, l = i % NB;
             double m = 0;
-            for (int x = 0; x < coss_nrow[al]; x++) {
-                if (coss_bas[al][x] != k) continue;
-                for (int y = 0; y < coss_nrow[be]; y++) {
-                    if (coss_bas[be][y] != l) continue;
-                    m += cof(coss_cof[al][x]) * cof(coss_cof[be][y]) *
-                         cb[(COSS_C + coss_row[al][x] * 18 + coss_row[be][y]) * NIP + ip];
+            for (int x = 0; x < CR::nrow(al); x++) {
+                if (CR::bas(al, x) != k) continue;
+                for (int y = 0; y < CR::nrow(be); y++) {
+                    if (CR::bas(be, y) != l) continue;
+                    m += cof(CR::cof(al, x)) * cof(CR::cof(be, y)) * cb[(COSS_C + CR::row(al, x) * 18 + CR::row(be, y)) * NIP + ip];
                 }
             }
-            Mb[ip][i % 16] = m;
+            Mb[ip][i % (NB * NB)] = m;
         }
         __syncthreads();
         for (int e = tid; e < N * N; e += NT) {
@@ -3136,11 +3192,14 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
             double acc = 0;
 #pragma unroll 3
             for (int ip = 0; ip < NIP; ip++) {
-                const double pb0 = Phi[ip][0][b], pb1 = Phi[ip][1][b], pb2 = Phi[ip][2][b], pb3 = Phi[ip][3][b];
+                double pb[NB];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const double t = Mb[ip][k * 4] * pb0 + Mb[ip][k * 4 + 1] * pb1 + Mb[ip][k * 4 + 2] * pb2 +
-                                     Mb[ip][k * 4 + 3] * pb3;
+                for (int l = 0; l < NB; l++) pb[l] = Phi[ip][l][b];
+#pragma unroll
+                for (int k = 0; k < NB; k++) {
+                    double t = 0;
+#pragma unroll
+                    for (int l = 0; l < NB; l++) t += Mb[ip][k * NB + l] * pb[l];
                     acc += Phi[ip][k][a] * t;
                 }
             }
@@ -3148,8 +3207,11 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
                 double WI = 0;
                 for (int ip = 0; ip < NIP; ip++) WI += cb[COSS_WI * NIP + ip] * Phi[ip][0][a];
                 const double l2 = lc * lc;
-                if (mc.Inertia) acc += WI * (al == 5 ? 2 * l2 * mc.Density / (1 + mc.Poisson) : mc.Density) * p.ddu_dt;
-                if (mc.Damping > 0.0) acc += WI * (al >= 3 ? 2 * l2 * mc.Damping : mc.Damping) * p.du_dt;
+                // inertia: the micro-rotation oz takes rhoCoss (3D: ox, oy take Density, :496-507)
+                const bool oz = al == NFU - 1;
+                const bool rot = DIM == 3 ? al >= 3 : al == 2;
+                if (mc.Inertia) acc += WI * (oz ? 2 * l2 * mc.Density / (1 + mc.Poisson) : mc.Density) * p.ddu_dt;
+                if (mc.Damping > 0.0) acc += WI * (rot ? 2 * l2 * mc.Damping : mc.Damping) * p.du_dt;
             }
             bad += acc - acc;
             double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * rp0[a] +
@@ -3159,12 +3221,12 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
         __syncthreads();
     }
     // F_U (:454-455) and the lumped nodal terms (:474-527)
-    for (int t = tid; t < 6 * N; t += NT) {
+    for (int t = tid; t < NFU * N; t += NT) {
         const int al = t / N, a = t % N;
         double acc = 0;
         for (int ip = 0; ip < NIP; ip++)
-            for (int x = 0; x < coss_nrow[al]; x++)
-                acc += cof(coss_cof[al][x]) * cb[(COSS_SIG + coss_row[al][x]) * NIP + ip] * Phi[ip][coss_bas[al][x]][a];
+            for (int x = 0; x < CR::nrow(al); x++)
+                acc += cof(CR::cof(al, x)) * cb[(COSS_SIG + CR::row(al, x)) * NIP + ip] * Phi[ip][CR::bas(al, x)][a];
         double WI = 0, WG = 0;
         for (int ip = 0; ip < NIP; ip++) {
             WI += cb[COSS_WI * NIP + ip] * Phi[ip][0][a];
@@ -3172,12 +3234,13 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
         }
         const int64_t dof = (int64_t)p.fpos_u[al] * p.Ng + rk[a];
         const double l2 = lc * lc;
+        const bool rot = DIM == 3 ? al >= 3 : al == 2;
         if (mc.Inertia) {
             const double rhoC = 2 * l2 * mc.Density / (1 + mc.Poisson);
-            acc += WI * (al < 3 ? mc.Density : rhoC) * p.su.ddx[dof];
+            acc += WI * (rot ? rhoC : mc.Density) * p.su.ddx[dof];
         }
-        if (mc.Damping > 0.0) acc += WI * (al < 3 ? mc.Damping : 2 * l2 * mc.Damping) * p.su.dx[dof];
-        if (al < 3) acc += -WG * mc.Density * mc.g[al];
+        if (mc.Damping > 0.0) acc += WI * (rot ? 2 * l2 * mc.Damping : mc.Damping) * p.su.dx[dof];
+        if (al < DIM) acc += -WG * mc.Density * mc.g[al];
         bad += acc - acc;
         p.su.f[dof] += acc;
     }
@@ -3228,16 +3291,19 @@ hipError_t launch_gather(const KParams& p, int dim, hipStream_t s) {
 
 int record_size(int dim) { return dim == 2 ? Rec<2>::SIZE : Rec<3>::SIZE; }
 
-hipError_t launch_coss_material(const KParams& p, hipStream_t s) {
-    const int64_t blocks = (p.NE + CossSmem::EPB - 1) / CossSmem::EPB;
+hipError_t launch_coss_material(const KParams& p, int dim, hipStream_t s) {
+    const int epb = dim == 2 ? CossSmem<2>::EPB : CossSmem<3>::EPB;
+    const int64_t blocks = (p.NE + epb - 1) / epb;
     if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(coss_material_kernel, dim3((unsigned)blocks), dim3(NT), 0, s, p);
+    if (dim == 2) hipLaunchKernelGGL(coss_material_kernel<2>, dim3((unsigned)blocks), dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL(coss_material_kernel<3>, dim3((unsigned)blocks), dim3(NT), 0, s, p);
     return hipGetLastError();
 }
 
-hipError_t launch_coss_assemble(const KParams& p, hipStream_t s) {
+hipError_t launch_coss_assemble(const KParams& p, int dim, hipStream_t s) {
     if (p.nlist <= 0) return hipSuccess;
-    hipLaunchKernelGGL(coss_assemble_kernel, dim3((unsigned)p.nlist), dim3(NT), 0, s, p);
+    if (dim == 2) hipLaunchKernelGGL(coss_assemble_kernel<2>, dim3((unsigned)p.nlist), dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL(coss_assemble_kernel<3>, dim3((unsigned)p.nlist), dim3(NT), 0, s, p);
     return hipGetLastError();
 }
 

@@ -2421,12 +2421,24 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
 }
 
 // ---- PhaseFieldDamage::Assemble_T (PhaseField.cpp:733-768) ------------------------
-// ---- CossPhaseField (Models/Fracture/PhaseField/CossPhaseField.cpp), 3D ----------------
+// ---- CossPhaseField (Models/Fracture/PhaseField/CossPhaseField.cpp) ----------------------
 // ConstructBMatCoss (:1168-1223): rows xx yy zz xy yx yz zy xz zx (strains), then nine
-// curvature rows scaled by l_coss; columns field-blocked [ux uy uz ox oy oz]
+// curvature rows scaled by l_coss; columns field-blocked [ux uy uz ox oy oz] (3D) or
+// [ux uy oz] (2D: rows xx, yy, xy, yx and the curvature rows 14 and 16 of oz, :1174-1185)
 static void ConstructBMatCoss(Mat& B, const Shapes& S, int ip, double l) {
     const int n = S.n;
     B.zero();
+    if (S.dim == 2) {
+        for (int a = 0; a < n; a++) {
+            const double N = S.Nv(ip, a), g0 = S.Gv(ip, 0, a), g1 = S.Gv(ip, 1, a);
+            B(0, a) = g0; B(1, n + a) = g1;
+            B(3, n + a) = g0; B(3, 2 * n + a) = -N;
+            B(4, a) = g1; B(4, 2 * n + a) = N;
+            B(14, 2 * n + a) = l * g0;
+            B(16, 2 * n + a) = l * g1;
+        }
+        return;
+    }
     for (int a = 0; a < n; a++) {
         const double N = S.Nv(ip, a), g0 = S.Gv(ip, 0, a), g1 = S.Gv(ip, 1, a), g2 = S.Gv(ip, 2, a);
         B(0, a) = g0; B(1, n + a) = g1; B(2, 2 * n + a) = g2;
@@ -2445,10 +2457,10 @@ static void ConstructBMatCoss(Mat& B, const Shapes& S, int ip, double l) {
 
 // CossPhaseField::Assemble_U (:337-534)
 static void Assemble_U_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats& st) {
-    const int n = c.n, nip = c.nip, nu = 6 * n;
+    const int n = c.n, nip = c.nip, nu = (int)md.fields_u.size() * n;
     Material M = md.M;
     Shapes Su, Sph, ST;
-    Su.resize(nip, n, 3); Sph.resize(nip, n, 3); ST.resize(nip, n, 3);
+    Su.resize(nip, n, c.dim); Sph.resize(nip, n, c.dim); ST.resize(nip, n, c.dim);
     Mat B(18, nu);
     std::vector<int64_t> El(n), dofsU(nu), dofsPhase(n), dofsT(n);
     std::vector<int> fph = {md.field_phase}, fT = {md.field_T};
@@ -2555,6 +2567,29 @@ static void Assemble_U_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e
         }
         const double l2 = M.l_coss * M.l_coss;
         for (int a = 0; a < n; a++) {
+            if (c.dim == 2) {  // [ux uy oz] (:470-527 with dim 2)
+                const int ny = a + n, oz = a + 2 * n;
+                if (md.opt.Inertia) {
+                    const double rhoCoss = 2 * l2 * M.Density / (1 + M.Poisson);
+                    F_U[a] += WLumpedI[a] * M.Density * ddU[a];
+                    K_UU(a, a) += WLumpedI[a] * M.Density * ddXdt2;
+                    F_U[ny] += WLumpedI[a] * M.Density * ddU[ny];
+                    K_UU(ny, ny) += WLumpedI[a] * M.Density * ddXdt2;
+                    F_U[oz] += WLumpedI[a] * rhoCoss * ddU[oz];
+                    K_UU(oz, oz) += WLumpedI[a] * rhoCoss * ddXdt2;
+                }
+                if (M.Damping > 0.0) {
+                    F_U[a] += WLumpedI[a] * M.Damping * dU[a];
+                    K_UU(a, a) += WLumpedI[a] * M.Damping * dXdt;
+                    F_U[ny] += WLumpedI[a] * M.Damping * dU[ny];
+                    K_UU(ny, ny) += WLumpedI[a] * M.Damping * dXdt;
+                    F_U[oz] += WLumpedI[a] * 2 * l2 * M.Damping * dU[oz];
+                    K_UU(oz, oz) += WLumpedI[a] * 2 * l2 * M.Damping * dXdt;
+                }
+                F_U[a] += -WLumpedG[a] * M.Density * md.g[0];
+                F_U[ny] += -WLumpedG[a] * M.Density * md.g[1];
+                continue;
+            }
             const int ny = a + n, nz = a + 2 * n, ox = a + 3 * n, oy = a + 4 * n, oz = a + 5 * n;
             if (md.opt.Inertia) {
                 const double rhoCoss = 2 * l2 * M.Density / (1 + M.Poisson);
@@ -2596,10 +2631,10 @@ static void Assemble_U_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e
 
 // CossPhaseField::Assemble_Phase (:536-769), Hist/None irreversibility, non-HO distributions
 static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1, Stats& st) {
-    const int n = c.n, nip = c.nip, nu = 6 * n;
+    const int n = c.n, nip = c.nip, nu = (int)md.fields_u.size() * n;
     Material M = md.M;
     Shapes Su, Sph, ST;
-    Su.resize(nip, n, 3); Sph.resize(nip, n, 3); ST.resize(nip, n, 3);
+    Su.resize(nip, n, c.dim); Sph.resize(nip, n, c.dim); ST.resize(nip, n, c.dim);
     Mat B(18, nu);
     std::vector<int64_t> El(n), dofsU(nu), dofsPhase(n), dofsT(n, 0);  // dofsT never filled (:575)
     std::vector<int> fph = {md.field_phase};
@@ -2662,7 +2697,7 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
             } else {
                 md.Energy[gi] = Energy;
             }
-            double gamma = GetDistributionFunction(md.pf.DistributionFunction, M.pf_l, 3, Dist, dDist, Sph, ip, Phase.data());
+            double gamma = GetDistributionFunction(md.pf.DistributionFunction, M.pf_l, c.dim, Dist, dDist, Sph, ip, Phase.data());
             double w = Su.w[ip];
             st.lcrack += gamma * w;
             for (int a = 0; a < n; a++) F_Ph[a] += w * M.Gc * Dist[a];
@@ -2680,8 +2715,8 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
                     for (int a = 0; a < n; a++) K_PhPh(a, b) += w * ddDamV * Sph.Nv(ip, a) * Sph.Nv(ip, b) * PlasticDiss;
             }
             if (md.opt.Inertia && md.pf.ConsistentDegradation) {
-                double v2 = 0;  // (N_uu dU)^T (N_uu dU), six components
-                for (int d = 0; d < 6; d++) {
+                double v2 = 0;  // (N_uu dU)^T (N_uu dU): every u-step field (six in 3D, ux uy oz in 2D)
+                for (int d = 0; d < (int)md.fields_u.size(); d++) {
                     double vd = 0;
                     for (int a = 0; a < n; a++) vd += Su.Nv(ip, a) * dU[d * n + a];
                     v2 += vd * vd;
@@ -2692,8 +2727,8 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
                         K_PhPh(a, b) += -w * 0.5 * M.Density * v2 * Sph.Nv(ip, a) * Sph.Nv(ip, b) * ddDamI;
             }
             if (md.pf.ConsistentDegradation) {
-                double gu = 0;
-                for (int d = 0; d < 3; d++) {
+                double gu = 0;  // g^T N_uu U: the displacement rows
+                for (int d = 0; d < c.dim; d++) {
                     double ud = 0;
                     for (int a = 0; a < n; a++) ud += Su.Nv(ip, a) * U[d * n + a];
                     gu += md.g[d] * ud;
@@ -3069,12 +3104,15 @@ void* ifo_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, int n_mod
                 return f;
             };
             if (is_pf(md.type)) {
-                if (md.type == MODEL_COSS && c->dim != 3) throw std::invalid_argument("Cosserat model: 3D only");
                 md.fields_u.push_back(need("ux"));
                 md.fields_u.push_back(need("uy"));
                 if (c->dim == 3) md.fields_u.push_back(need("uz"));
-                if (md.type == MODEL_COSS)
-                    for (const char* nm : {"ox", "oy", "oz"}) md.fields_u.push_back(need(nm));
+                if (md.type == MODEL_COSS) {  // CossPhaseField.h:15 (2D), CossPhaseField.cpp:172-183 (3D)
+                    if (c->dim == 3)
+                        for (const char* nm : {"ox", "oy", "oz"}) md.fields_u.push_back(need(nm));
+                    else
+                        md.fields_u.push_back(need("oz"));
+                }
                 for (int f : md.fields_u)
                     if (c->fstep[f] != c->fstep[md.fields_u[0]]) throw std::invalid_argument("u fields in different steps");
                 md.Step_u = c->fstep[md.fields_u[0]];

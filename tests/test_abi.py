@@ -96,7 +96,8 @@ def test_dense_extraction_is_factored_and_non_kronecker_rejected():
 
 @pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (5, 3)), ("quad2d_le_at2", (1, 1)),
                                       ("hex3d_glen_at2", (3, 2, 4)), ("hex3d_glen_at2", (1, 1, 1)),
-                                      ("hex3d_coss", (3, 2, 4)), ("quad2d_thermal", (4, 3))])
+                                      ("hex3d_coss", (3, 2, 4)), ("quad2d_thermal", (4, 3)),
+                                      ("quad2d_coss", (4, 3))])
 def test_numbering_and_pattern_match_oracle(name, nel):
     prob = make_problem(name, nel=nel)
     e = Physics(prob, device=False)
@@ -116,7 +117,8 @@ def test_numbering_and_pattern_match_oracle(name, nel):
 
 @pytest.mark.parametrize("method", ["LagrangeMult", "AugmentedLagrangeMult", "AugmentedLagrangeMult2",
                                     "AugmentedLagrangeMult3"])
-@pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (4, 3)), ("hex3d_coss", (2, 1, 2))])
+@pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (4, 3)), ("hex3d_coss", (2, 1, 2)),
+                                      ("quad2d_coss", (3, 2))])
 def test_multiplier_numbering_and_pattern_match_oracle(name, nel, method):
     """The multiplier DOFs of the Lagrange methods share the phase step; with two
     multipliers the (lMult_pf, lMult_pf2) field pairs are absent from the pattern."""

@@ -4,7 +4,8 @@ phase and T steps, the 18-component plastic strain, the four-component material 
 the dissipation and the energy history; identical return-map call and iteration counts.
 Cases: the hex3d_coss workload (damaged visco-elasto-plastic, SpecStress energy, gravity,
 viscous heating) at small sizes, its linear-elastic variant with no split, and the
-visco-elasto-plastic variant with inertia.
+visco-elasto-plastic variant with inertia; the same for the 2D workload quad2d_coss
+(fields ux, uy, oz).
 
 Tolerances.  1e-12 (row/step scaled) wherever the result is a well-conditioned function
 of the inputs: the linear-elastic variant, and the visco-elasto-plastic map run to
@@ -49,17 +50,22 @@ def _variant(prob, variant):
     elif variant == "inertia":
         m.material.Rheology = abi.RHEOLOGY["ViscoElasticPlastic"]
         m.opts.Inertia = 1
-        prob.node_state[(abi.IFE_DDSTATE, "ox")] = np.linspace(-1.0, 1.0, prob.mesh.n_nodes)
+        acc = "ox" if "ox" in prob.dof_names else "ux"
+        prob.node_state[(abi.IFE_DDSTATE, acc)] = np.linspace(-1.0, 1.0, prob.mesh.n_nodes)
+        prob.node_state[(abi.IFE_DDSTATE, "oz")] = np.linspace(-0.3, 0.7, prob.mesh.n_nodes)
         prob.node_state[(abi.IFE_DSTATE, "oz")] = np.linspace(0.5, -0.5, prob.mesh.n_nodes)
     return prob
 
 
-@pytest.mark.parametrize("variant,nel,tight", [("dvep", (2, 2, 2), True), ("dvep", (3, 2, 4), True),
-                                               ("le", (3, 3, 2), False), ("inertia", (2, 3, 2), True),
-                                               ("dvep", (3, 2, 4), False), ("dvep", (8, 6, 5), False)])
-def test_cosserat_against_oracle(require_gpu, variant, nel, tight):
-    _say(f"[coss] {variant} nel={nel} tight={tight}")
-    prob = _variant(make_problem("hex3d_coss", nel=nel), variant)
+@pytest.mark.parametrize("name,variant,nel,tight", [
+    ("hex3d_coss", "dvep", (2, 2, 2), True), ("hex3d_coss", "dvep", (3, 2, 4), True),
+    ("hex3d_coss", "le", (3, 3, 2), False), ("hex3d_coss", "inertia", (2, 3, 2), True),
+    ("hex3d_coss", "dvep", (3, 2, 4), False), ("hex3d_coss", "dvep", (8, 6, 5), False),
+    ("quad2d_coss", "dvep", (5, 4), True), ("quad2d_coss", "le", (6, 3), False),
+    ("quad2d_coss", "inertia", (4, 4), True), ("quad2d_coss", "dvep", (16, 8), False)])
+def test_cosserat_against_oracle(require_gpu, name, variant, nel, tight):
+    _say(f"[coss] {name} {variant} nel={nel} tight={tight}")
+    prob = _variant(make_problem(name, nel=nel), variant)
     o = Oracle(prob)
     e = Physics(prob)
     if tight:

@@ -106,9 +106,10 @@ def test_ho_density_against_oracle(require_gpu, name, nel, method, distort):
 
 
 @pytest.mark.parametrize("method", ["HO_Quadratic", "HO_Linear"])
-def test_ho_density_cosserat_against_oracle(require_gpu, method):
-    _say(f"[ho] hex3d_coss (2,2,2) {method}")
-    _compare(_problem("hex3d_coss", (2, 2, 2), method, distort=False), tight=COSS_TIGHT)
+@pytest.mark.parametrize("name,nel", [("hex3d_coss", (2, 2, 2)), ("quad2d_coss", (5, 4))])
+def test_ho_density_cosserat_against_oracle(require_gpu, name, nel, method):
+    _say(f"[ho] {name} {nel} {method}")
+    _compare(_problem(name, nel, method, distort=False), tight=COSS_TIGHT)
 
 
 @pytest.mark.parametrize("method", ["HO_Quadratic", "HO_CZM", "HO_Linear", "HO_Linear2", "Linear2", "Local"])

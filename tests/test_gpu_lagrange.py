@@ -51,9 +51,10 @@ def test_multipliers_against_oracle(require_gpu, name, nel, method):
 
 
 @pytest.mark.parametrize("method", ["AugmentedLagrangeMult", "AugmentedLagrangeMult2"])
-def test_multipliers_cosserat_against_oracle(require_gpu, method):
-    _say(f"[multipliers] hex3d_coss (2,2,2) {method}")
-    _compare(add_multipliers(make_problem("hex3d_coss", nel=(2, 2, 2)), method), RTOL_FP64, tight=COSS_TIGHT)
+@pytest.mark.parametrize("name,nel", [("hex3d_coss", (2, 2, 2)), ("quad2d_coss", (5, 4))])
+def test_multipliers_cosserat_against_oracle(require_gpu, name, nel, method):
+    _say(f"[multipliers] {name} {nel} {method}")
+    _compare(add_multipliers(make_problem(name, nel=nel), method), RTOL_FP64, tight=COSS_TIGHT)
 
 
 def test_multipliers_mixed_mode(require_gpu):

@@ -251,3 +251,136 @@ def test_cosserat_assembly_runs_and_is_finite():
     st = o.stats()
     assert st["return_map_calls"] > 0 and st["return_map_maxit_hits"] == 0
     assert o.history(0, "PlasticStrain").shape[0] == 18
+
+
+def _elastic_2d(nel=(3, 2)):
+    prob = make_problem("quad2d_coss", nel=nel)
+    m = prob.models[0]
+    m.material.Rheology = abi.RHEOLOGY["LinearElastic"]
+    m.material.Damping = 0.0
+    m.material.Split = abi.SPLIT["None"]
+    m.opts.Gravity = 0
+    return prob
+
+
+def _dense(o, s):
+    K, f = o.assemble(s)
+    rp, col = o.pattern(s)
+    n = o.n_rows(s)
+    D = np.zeros((n, n))
+    for r in range(n):
+        D[r, col[rp[r]:rp[r + 1]]] = K[rp[r]:rp[r + 1]]
+    return D, f
+
+
+def test_2d_linear_elastic_u_step_with_rigid_null_space():
+    """2D Cosserat [ux uy oz]: translations and the rigid rotation u = theta (-y, x) with
+    oz = theta are in the null space; the opposite micro-rotation is not."""
+    prob = _elastic_2d()
+    o = Oracle(prob)
+    Kd, f = _dense(o, 0)
+    U = o.state(0, abi.IFE_STATE).copy()
+    assert np.allclose(Kd, Kd.T, rtol=0, atol=1e-9 * np.abs(Kd).max())
+    assert np.allclose(Kd @ U, f, rtol=0, atol=1e-10 * np.abs(f).max())
+    X = prob.mesh.coords
+    n = o.n_rows(0)
+    dofs = {nm: o.dof_numbers(prob.dof_names.index(nm)) for nm in ("ux", "uy", "oz")}
+    scale = np.abs(Kd).max()
+    for nm in ("ux", "uy"):
+        v = np.zeros(n)
+        v[dofs[nm]] = 1.0
+        assert np.abs(Kd @ v).max() <= 1e-10 * scale
+    th = 0.4
+    v = np.zeros(n)
+    v[dofs["ux"]] = -th * X[:, 1]
+    v[dofs["uy"]] = th * X[:, 0]
+    v[dofs["oz"]] = th
+    assert np.abs(Kd @ v).max() <= 1e-9 * scale * np.abs(v).max()
+    v[dofs["oz"]] = -th
+    assert np.abs(Kd @ v).max() > 1e-3 * scale * np.abs(v).max()
+
+
+@pytest.mark.parametrize("rheology", ["LinearElastic", "DamViscoElasticPlastic"])
+def test_2d_equals_a_z_invariant_3d_slab(rheology):
+    """Plane strain: a 3D slab of one element in z with u_z = w_x = w_y = 0 and z-invariant
+    fields has the 2D strains, so summing its K and f over the three z control points of
+    each (x, y) column gives Lz times the 2D K and f, for the u, phase and T steps.  The
+    curvature rows differ by construction (2D: l grad w_z in rows 14 and 16; 3D: w_z enters
+    only through d/dz), so l_coss is taken negligible here; the curvature rows are pinned by
+    the energy test below."""
+    nel2, Lz = (3, 2), 2.5e-3
+    p2 = make_problem("quad2d_coss", nel=nel2)
+    p3 = make_problem("hex3d_coss", nel=(3, 2, 1))
+    m3 = p3.inputs["mesh"]
+    p3.mesh.coords[:, :2] = np.tile(p2.mesh.coords, (3, 1))  # same x, y at every z level
+    p3.mesh.coords[:, 2] = np.repeat(p3.mesh.coords[::len(p2.mesh.coords), 2], len(p2.mesh.coords))
+    assert np.isclose(m3["lengths"][2], Lz)
+    for p in (p2, p3):
+        p.models[0].material.Rheology = abi.RHEOLOGY[rheology]
+        p.models[0].material.l_coss = 1e-12
+        p.models[0].opts.Gravity = 0
+    NN2 = p2.mesh.n_nodes
+    for (kind, nm), v in list(p2.node_state.items()):
+        p3.node_state[(kind, nm)] = np.tile(v, 3)
+    for nm in ("uz", "ox", "oy"):
+        for kind in (abi.IFE_STATE, abi.IFE_STATE_OLD, abi.IFE_DSTATE, abi.IFE_DDSTATE):
+            p3.node_state[(kind, nm)] = np.zeros(3 * NN2)
+    for key in list(p3.history):
+        p3.history[key] = np.zeros_like(p3.history[key])
+    for key in list(p2.history):
+        p2.history[key] = np.zeros_like(p2.history[key])
+    o2, o3 = Oracle(p2), Oracle(p3)
+    for o in (o2, o3):  # converged maps: the reference's stop rule leaves rounding-level noise
+        for k, v in (("rm_abs_tol", 1e-40), ("rm_rel_tol", 1e-30), ("rm_max_iterations", 60)):
+            o.set_param(0, k, v)
+    names2 = {0: ["ux", "uy", "oz"], 1: ["phase"], 2: ["T"]}
+    for s in range(o2.nsteps):
+        K2, f2 = _dense(o2, s)
+        K3, f3 = _dense(o3, s)
+        d2 = np.concatenate([o2.dof_numbers(p2.dof_names.index(nm)) for nm in names2[s]])
+        d3 = np.stack([np.concatenate([o3.dof_numbers(p3.dof_names.index(nm))[k * NN2:(k + 1) * NN2]
+                                       for nm in names2[s]]) for k in range(3)])
+        fz = sum(f3[d3[k]] for k in range(3))
+        Kz = sum(K3[np.ix_(d3[k], d3[kk])] for k in range(3) for kk in range(3))
+        assert np.allclose(fz, Lz * f2[d2], rtol=1e-9, atol=1e-9 * np.abs(Lz * f2).max())
+        assert np.allclose(Kz, Lz * K2[np.ix_(d2, d2)], rtol=1e-9, atol=1e-9 * np.abs(Lz * K2).max())
+
+
+@pytest.mark.parametrize("a,b", [(1.0, 0.0), (0.0, 1.0), (0.7, -0.4)])
+def test_2d_micro_rotation_energy_against_quadrature(a, b):
+    """A linear micro-rotation w_z = a x + b y (reproduced exactly by the quadratic B-splines
+    through their Greville abscissae) with u = 0 and phi = 0: U^T K U equals twice the
+    integral of the Cosserat energy density of eps(x) = (xy: -w_z, yx: +w_z, row 14: l a,
+    row 16: l b), integrated by 3x3 Gauss per element with the material's own energy."""
+    prob = _elastic_2d((3, 2))
+    mat, pf = _mat(prob)
+    prob.node_state[(abi.IFE_STATE, "phase")][:] = 0.0
+    X = prob.mesh.coords
+    for nm in ("ux", "uy"):
+        prob.node_state[(abi.IFE_STATE, nm)] = np.zeros(len(X))
+    prob.node_state[(abi.IFE_STATE, "oz")] = a * X[:, 0] + b * X[:, 1]
+    o = Oracle(prob)
+    Kd, f = _dense(o, 0)
+    U = o.state(0, abi.IFE_STATE).copy()
+    UKU = U @ Kd @ U
+    Lx, Ly = prob.inputs["mesh"]["lengths"]
+    nx, ny = prob.inputs["mesh"]["nel"]
+    xg = np.array([-np.sqrt(0.6), 0.0, np.sqrt(0.6)])
+    wg = np.array([5.0, 8.0, 5.0]) / 9.0
+    lc = mat.l_coss
+    Dd, Du, Sd, Su = (np.zeros(324), np.zeros(324), np.zeros(18), np.zeros(18))
+    total = 0.0
+    hx, hy = Lx / nx, Ly / ny
+    for i in range(nx):
+        for j in range(ny):
+            for gx, wx in zip(xg, wg):
+                for gy, wy in zip(xg, wg):
+                    x = (i + 0.5 + 0.5 * gx) * hx
+                    y = (j + 0.5 + 0.5 * gy) * hy
+                    w = a * x + b * y
+                    eps = np.zeros(18)
+                    eps[3], eps[4], eps[14], eps[16] = -w, w, lc * a, lc * b
+                    psi = L.ifo_coss_energy_split(C.byref(mat), C.byref(pf), mat.TRef, _p(eps), _p(Dd), _p(Du), _p(Sd),
+                                                  _p(Su))
+                    total += psi * wx * wy * 0.25 * hx * hy
+    assert abs(UKU - 2 * total) <= 1e-10 * abs(2 * total)
