@@ -1089,8 +1089,10 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             if (!(r == IFE_RHEO_LE || r == IFE_RHEO_VEP || r == IFE_RHEO_DVEP))
                 THROW(IFE_ERR_UNSUPPORTED, "rheology not defined for a Cosserat material (LE, VEP, dVEP are)");
             m->HistSize = r == IFE_RHEO_LE ? 0 : 4;
-            if (!(mat->Split == IFE_SPLIT_NONE || (mat->Split == IFE_SPLIT_SPECSTRESS && !pf->ConsistentEnergy)))
-                THROW(IFE_ERR_UNSUPPORTED, "Cosserat energy split: NoSplit, or SpecStress without a consistent energy");
+            // CosseratMaterial::EnergySplit (CosseratMaterial.cpp:109-152) throws for the others
+            if (!(mat->Split == IFE_SPLIT_NONE || mat->Split == IFE_SPLIT_SPECSTRESS ||
+                  mat->Split == IFE_SPLIT_DRUCKERPRAGER || mat->Split == IFE_SPLIT_STARCONVEX))
+                THROW(IFE_ERR_UNSUPPORTED, "Cosserat energy split: NoSplit, SpecStress, DruckerPrager or StarConvex");
             int dfn = pf->DistributionFunction;
             if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_LOCAL)  // Custom needs a user callback
                 THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");

@@ -2752,6 +2752,156 @@ __device__ double coss_spec_energy(const CossD& D, const double* ee) {
     return fmax(0.0, 0.5 * E);
 }
 
+// Cosserat energy splits with their stresses and tangents (CosseratMaterial::EnergySplit,
+// CosseratMaterial.cpp:109-369) for StarConvex, DruckerPrager and the consistent SpecStress
+// split, dense 18x18 in private memory (the kernel variant that carries them is launched only
+// for these splits).  Moduli as SetTemperature scales them: Shear, Lame, Bulk = Lame + 2/3
+// Shear, G_coss, all times dam.  Returns the energy.
+__device__ double coss_split_full(const MatConst& mc, double dam, const CossD& D, const double e[18], double DD[18][18],
+                                  double DU[18][18], double SD[18], double SU[18]) {
+    const double Young = mc.Young0 * dam, Sh = Young / (2.0 * (1 + mc.Poisson));
+    const double La = Young * mc.Poisson / ((1.0 + mc.Poisson) * (1.0 - 2.0 * mc.Poisson)), K = La + 2.0 / 3.0 * Sh;
+    const double Gc = mc.G_coss0 * dam;
+    for (int i = 0; i < 18; i++) {
+        SD[i] = 0.0;
+        SU[i] = 0.0;
+        for (int j = 0; j < 18; j++) { DD[i][j] = 0.0; DU[i][j] = 0.0; }
+    }
+    auto ic = [](int i) { return i < 3 ? 1.0 : 0.0; };
+    double E = 0.0;
+    if (mc.Split == IFE_SPLIT_STARCONVEX) {  // :155-204
+        // volumetric p over rows 0-2; dev = e(0:9) - p/3 on rows 0-2; the pair matrix
+        // [[1,-1],[-1,1]] on (3,4), (5,6), (7,8); curvature rows 2 G_coss (no energy)
+        const double p = e[0] + e[1] + e[2];
+        const bool pos = p >= 0;
+        const double pp = pos ? p : 0.0, pm = pos ? 0.0 : p;
+        double dev[9];
+        for (int i = 0; i < 9; i++) dev[i] = e[i] - (i < 3 ? p / 3.0 : 0.0);
+        double dd = 0, ds = 0, sk[9];
+        for (int i = 0; i < 9; i++) {
+            sk[i] = i < 3 ? 0.0 : (((i - 3) & 1) == 0 ? dev[i] - dev[i + 1] : dev[i] - dev[i - 1]);
+            dd += dev[i] * dev[i];
+            ds += dev[i] * sk[i];
+        }
+        E = 0.5 * K * pp * pp + Sh * dd + 0.5 * (Sh - Gc) * ds - 0.5 * K * mc.StarConvexGamma * pm * pm;
+        double mean = 0;  // the deviatoric projector applied to 2 Sh dev on rows 0-2
+        for (int i = 0; i < 3; i++) mean += 2 * Sh * dev[i];
+        mean /= 3.0;
+        for (int i = 0; i < 9; i++) {
+            SD[i] = (i < 3 ? K * pp + 2 * Sh * dev[i] - mean : 2 * Sh * dev[i] + (Sh - Gc) * sk[i]);
+            if (i < 3) SU[i] = K * pm;
+        }
+        for (int i = 9; i < 18; i++) SD[i] = 2 * Gc * e[i];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                DD[i][j] = K * (pos ? 1.0 : 0.0) + 2 * Sh * ((i == j ? 1.0 : 0.0) - 1.0 / 3.0);
+                DU[i][j] = K * (pos ? 0.0 : 1.0);
+            }
+        for (int i = 3; i < 9; i++) {
+            const int pj = ((i - 3) & 1) == 0 ? i + 1 : i - 1;
+            DD[i][i] = 2 * Sh + (Sh - Gc);
+            DD[i][pj] = -(Sh - Gc);
+        }
+        for (int i = 9; i < 18; i++) DD[i][i] = 2 * Gc;
+        E = fmax(0.0, E);
+    } else if (mc.Split == IFE_SPLIT_DRUCKERPRAGER) {  // :206-261
+        const double B = mc.B_dp, I1 = e[0] + e[1] + e[2];
+        double dJ2[18], J2 = 0;
+        for (int i = 0; i < 18; i++) {
+            double t = 0;
+            for (int j = 0; j < 18; j++) t += e[j] * coss_J2ij(j, i);
+            dJ2[i] = t;
+            J2 += 0.5 * e[i] * t;
+        }
+        const double sq = sqrt(J2 + 1.0e-16);
+        auto dsq = [&](int i) { return 0.5 / sq * dJ2[i]; };
+        auto ddsq = [&](int i, int j) { return 0.5 / sq * coss_J2ij(i, j) - 0.25 / (sq * sq * sq) * dJ2[i] * dJ2[j]; };
+        if (-6.0 * B * sqrt(J2) < I1) {
+            E = 0.5 * K * I1 * I1 + 2.0 * Sh * J2;
+            for (int i = 0; i < 18; i++) {
+                SD[i] = K * I1 * ic(i) + 2.0 * Sh * dJ2[i];
+                for (int j = 0; j < 18; j++) DD[i][j] = K * ic(i) * ic(j) + 2.0 * Sh * coss_J2ij(i, j);
+            }
+        } else if (2 * Sh * sqrt(J2) < 3 * B * K * I1) {
+            for (int i = 0; i < 18; i++) {
+                SU[i] = K * I1 * ic(i) + 2.0 * Sh * dJ2[i];
+                for (int j = 0; j < 18; j++) DU[i][j] = K * ic(i) * ic(j) + 2.0 * Sh * coss_J2ij(i, j);
+            }
+        } else {
+            const double den = 18.0 * B * B * K + 2.0 * Sh;
+            const double q = -3.0 * B * K * I1 + 2 * Sh * sq, q2 = I1 + 6 * B * sq;
+            E = q * q / den;
+            for (int i = 0; i < 18; i++) {
+                const double dqi = -3.0 * B * K * ic(i) + 2.0 * Sh * dsq(i), dq2i = ic(i) + 6 * B * dsq(i);
+                SD[i] = 2.0 * q * dqi / den;
+                SU[i] = K * Sh / den * 2.0 * q2 * dq2i;
+                for (int j = 0; j < 18; j++) {
+                    const double dqj = -3.0 * B * K * ic(j) + 2.0 * Sh * dsq(j), dq2j = ic(j) + 6 * B * dsq(j);
+                    DD[i][j] = (2.0 * q * 2.0 * Sh * ddsq(i, j) + 2.0 * dqi * dqj) / den;
+                    DU[i][j] = K * Sh / den * (2.0 * q2 * 6 * B * ddsq(i, j) + 2.0 * dq2i * dq2j);
+                }
+            }
+        }
+    } else {  // consistent SpecStress (:263-369)
+        double S[18];
+        coss_Dmul(D, e, 1.0, S);
+        double sn[9];
+        for (int i = 0; i < 9; i++) sn[i] = fabs(S[i]) < 1.0e-12 ? (i + 1) * 1.0e-12 : S[i];
+        const double SM[3][3] = {{sn[0], sn[3], sn[7]}, {sn[4], sn[1], sn[5]}, {sn[8], sn[6], sn[2]}};
+        double A[3][3], lam[3], V[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) A[i][j] = 0.5 * (SM[i][j] + SM[j][i]);
+        eig3(A, lam, V);
+        double Pp[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double t = 0;
+                for (int k = 0; k < 3; k++) t += V[i][k] * fmax(lam[k], 0.0) * V[j][k];
+                Pp[i][j] = t;
+            }
+        const double sv[9] = {Pp[0][0], Pp[1][1], Pp[2][2], Pp[0][1], Pp[1][0], Pp[1][2], Pp[2][1], Pp[0][2], Pp[2][0]};
+        for (int i = 0; i < 9; i++) E += sv[i] * e[i];
+        E *= 0.5;
+        // projector on the flat tensor index 3a + b (as the reference assembles it), applied
+        // to D_coss in the strain vector's row order; DU = D_coss - P D_coss
+        double P[9][9];
+        for (int r = 0; r < 9; r++)
+            for (int c = 0; c < 9; c++) {
+                const int a = r / 3, b = r % 3, cc = c / 3, d = c % 3;
+                double t = 0;
+                for (int k = 0; k < 3; k++)
+                    if (lam[k] > 0.0) t += V[a][k] * V[b][k] * V[cc][k] * V[d][k];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        if (i == j || (lam[i] <= 0.0 && lam[j] <= 0.0)) continue;
+                        const double den = lam[i] - lam[j];
+                        if (fabs(den) < 1e-12) continue;
+                        const double cf = (fmax(lam[i], 0.0) - fmax(lam[j], 0.0)) / den;
+                        t += 0.5 * cf * (V[a][i] * V[b][j] * V[cc][j] * V[d][i] + V[a][j] * V[b][i] * V[cc][i] * V[d][j]);
+                    }
+                P[r][c] = t;
+            }
+        for (int i = 0; i < 18; i++)
+            for (int j = 0; j < 18; j++) {
+                double t = 0;
+                if (i < 9) {
+                    for (int k = 0; k < 9; k++) t += P[i][k] * coss_Dij(D, k, j);
+                } else {
+                    t = coss_Dij(D, i, j);
+                }
+                DD[i][j] = t;
+                DU[i][j] = coss_Dij(D, i, j) - t;
+            }
+        for (int i = 0; i < 18; i++) {
+            double a = 0, b = 0;
+            for (int j = 0; j < 18; j++) { a += DD[i][j] * e[j]; b += DU[i][j] * e[j]; }
+            SD[i] = a;
+            SU[i] = b;
+        }
+    }
+    return E;
+}
+
 template <int DIM>
 struct CossSmem {
     // u-step fields: ux uy uz ox oy oz (3D), ux uy oz (2D)
@@ -2768,7 +2918,8 @@ struct CossSmem {
 // CosseratMaterial::UpdatePlasticStrains, energy split) writes w (Dam D_coss DPlast), w sigma
 // and the lumped weights to cossb; the phase step (:645-733) and the T step (:790-802) write
 // the same per-IP coefficients as the solid model for the shared element kernel.
-template <int DIM>
+// GEN: the StarConvex, DruckerPrager or consistent SpecStress split (coss_split_full)
+template <int DIM, bool GEN>
 __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
     using SM = CossSmem<DIM>;
     constexpr int N = SM::N, NIP = SM::NIP, EPB = SM::EPB, NFU = SM::NFU;
@@ -2976,19 +3127,40 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
             p.DVD[gi] = p.DVDOld[gi] + ddiss * p.dt;
             double ee[18], sig[18];
             for (int r = 0; r < 18; r++) ee[r] = eps[r] - epl[r];
-            coss_Dmul(D, ee, 1.0, sig);  // energy not consistent: S_Dam = D_coss ee, S_Undam = 0
-            for (int r = 0; r < 18; r++) {
-                const double v = w * (Dam * sig[r]);
-                my_bad |= !isfinite(v);
-                cbo[(COSS_SIG + r) * NIP] = v;
-            }
-            // w (Dam D_coss) DPlast
+            bool cons = false;
+            if constexpr (GEN) cons = mc.ConsistentEnergy;
+            if (cons) {
+                // w (Dam S_Dam + S_Undam) and w (Dam D_Dam + D_UnDam) DPlast (:457-458)
+                double DDm[18][18], DUm[18][18], SDv[18], SUv[18];
+                coss_split_full(mc, m.dam, D, ee, DDm, DUm, SDv, SUv);
+                for (int r = 0; r < 18; r++) {
+                    const double v = w * (Dam * SDv[r] + SUv[r]);
+                    my_bad |= !isfinite(v);
+                    cbo[(COSS_SIG + r) * NIP] = v;
+                }
 #pragma unroll 1
-            for (int j = 0; j < 18; j++) {
-                double col[18], o[18];
-                for (int k = 0; k < 18; k++) col[k] = elastic ? (k == j ? 1.0 : 0.0) : DM[k][j];
-                coss_Dmul(D, col, Dam, o);
-                for (int i = 0; i < 18; i++) cbo[(COSS_C + i * 18 + j) * NIP] = w * o[i];
+                for (int i = 0; i < 18; i++)
+                    for (int j = 0; j < 18; j++) {
+                        double t = 0;
+                        for (int k = 0; k < 18; k++)
+                            t += (Dam * DDm[i][k] + DUm[i][k]) * (elastic ? (k == j ? 1.0 : 0.0) : DM[k][j]);
+                        cbo[(COSS_C + i * 18 + j) * NIP] = w * t;
+                    }
+            } else {
+                coss_Dmul(D, ee, 1.0, sig);  // energy not consistent: S_Dam = D_coss ee, S_Undam = 0
+                for (int r = 0; r < 18; r++) {
+                    const double v = w * (Dam * sig[r]);
+                    my_bad |= !isfinite(v);
+                    cbo[(COSS_SIG + r) * NIP] = v;
+                }
+                // w (Dam D_coss) DPlast
+#pragma unroll 1
+                for (int j = 0; j < 18; j++) {
+                    double col[18], o[18];
+                    for (int k = 0; k < 18; k++) col[k] = elastic ? (k == j ? 1.0 : 0.0) : DM[k][j];
+                    coss_Dmul(D, col, Dam, o);
+                    for (int i = 0; i < 18; i++) cbo[(COSS_C + i * 18 + j) * NIP] = w * o[i];
+                }
             }
             cbo[COSS_WI * NIP] = w * DamI;
             cbo[COSS_WG * NIP] = w * DamG;  // (1 - porosity) = 1
@@ -3006,7 +3178,10 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
             double ee[18];
             for (int r = 0; r < 18; r++) ee[r] = eps[r] - epl[r];
             double Energy;
-            if (mc.Split == IFE_SPLIT_NONE) {
+            if constexpr (GEN) {
+                double DDm[18][18], DUm[18][18], SDv[18], SUv[18];
+                Energy = coss_split_full(mc, m.dam, D, ee, DDm, DUm, SDv, SUv);
+            } else if (mc.Split == IFE_SPLIT_NONE) {
                 double s18[18];
                 coss_Dmul(D, ee, 1.0, s18);
                 Energy = 0;
@@ -3295,8 +3470,17 @@ hipError_t launch_coss_material(const KParams& p, int dim, hipStream_t s) {
     const int epb = dim == 2 ? CossSmem<2>::EPB : CossSmem<3>::EPB;
     const int64_t blocks = (p.NE + epb - 1) / epb;
     if (blocks <= 0) return hipSuccess;
-    if (dim == 2) hipLaunchKernelGGL(coss_material_kernel<2>, dim3((unsigned)blocks), dim3(NT), 0, s, p);
-    else hipLaunchKernelGGL(coss_material_kernel<3>, dim3((unsigned)blocks), dim3(NT), 0, s, p);
+    const int sp = p.mc.Split;
+    const bool gen = sp == IFE_SPLIT_STARCONVEX || sp == IFE_SPLIT_DRUCKERPRAGER ||
+                     (sp == IFE_SPLIT_SPECSTRESS && p.mc.ConsistentEnergy);
+    const dim3 g((unsigned)blocks), b(NT);
+    if (gen) {
+        if (dim == 2) hipLaunchKernelGGL((coss_material_kernel<2, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((coss_material_kernel<3, true>), g, b, 0, s, p);
+    } else {
+        if (dim == 2) hipLaunchKernelGGL((coss_material_kernel<2, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((coss_material_kernel<3, false>), g, b, 0, s, p);
+    }
     return hipGetLastError();
 }
 

@@ -1446,9 +1446,184 @@ struct Material {
         return std::max(0.0, E);
     }
 
-    // CosseratMaterial::EnergySplit (:109-152): NoSplit and SpecStress (consistent energy
-    // off); the stress and tangent are D_coss eps and D_coss whenever the energy is not
-    // consistent
+    // CosseratMaterial::StarConvexSplit (:155-204).  ic: the volumetric rows 0-2; the
+    // deviatoric macro strain dev = e(0:9) - p/3 icl with dDev = I - icl icl^T / 3; SkewMat
+    // acts on the pairs (3,4), (5,6), (7,8) as [[1,-1],[-1,1]]; the curvature rows carry
+    // 2 G_coss in stress and tangent but no energy; the star-convex correction enters the
+    // energy only
+    double CossStarConvex(C18x18 DD, C18x18 DU, C18 SD, C18 SU, const double* e) const {
+        const double p = e[0] + e[1] + e[2];
+        double dev[9];
+        for (int i = 0; i < 9; i++) dev[i] = e[i] - (i < 3 ? p / 3.0 : 0.0);
+        auto dDev = [](int i, int j) { return (i == j ? 1.0 : 0.0) - (i < 3 && j < 3 ? 1.0 / 3.0 : 0.0); };
+        auto skew = [](int i, int j) {
+            if (i < 3 || j < 3) return 0.0;
+            if (i == j) return 1.0;
+            return (i - 3) / 2 == (j - 3) / 2 ? -1.0 : 0.0;
+        };
+        const bool pos = p >= 0;
+        const double pp = pos ? p : 0.0, dpp = pos ? 1.0 : 0.0, pm = pos ? 0.0 : p, dpm = pos ? 0.0 : 1.0;
+        double E = 0.5 * Bulk * pp * pp;
+        for (int i = 0; i < 3; i++) {
+            SD[i] += Bulk * pp * dpp;
+            SU[i] += Bulk * pm * dpm;
+            for (int j = 0; j < 3; j++) {
+                DD[i][j] += Bulk * dpp * dpp;
+                DU[i][j] += Bulk * dpm * dpm;
+            }
+        }
+        double dd = 0, ds = 0;
+        for (int i = 0; i < 9; i++) dd += dev[i] * dev[i];
+        double sdev[9];  // Skew dev
+        for (int i = 0; i < 9; i++) {
+            double t = 0;
+            for (int j = 0; j < 9; j++) t += skew(i, j) * dev[j];
+            sdev[i] = t;
+            ds += dev[i] * t;
+        }
+        E += Shear * dd + 0.5 * (Shear - G_coss) * ds;
+        // Macro^T dDev (2 Shear dev + (Shear - G_coss) Skew dev), and the tangent
+        // Macro^T dDev (2 Shear I + (Shear - G_coss) Skew) dDev Macro
+        for (int i = 0; i < 9; i++) {
+            double t = 0;
+            for (int k = 0; k < 9; k++) t += dDev(i, k) * (2 * Shear * dev[k] + (Shear - G_coss) * sdev[k]);
+            SD[i] += t;
+        }
+        for (int i = 0; i < 9; i++)
+            for (int j = 0; j < 9; j++) {
+                double t = 0;
+                for (int k = 0; k < 9; k++)
+                    for (int l = 0; l < 9; l++)
+                        t += dDev(i, k) * (2 * Shear * (k == l ? 1.0 : 0.0) + (Shear - G_coss) * skew(k, l)) * dDev(l, j);
+                DD[i][j] += t;
+            }
+        for (int i = 9; i < 18; i++) {
+            SD[i] += 2 * G_coss * e[i];
+            DD[i][i] += 2 * G_coss;
+        }
+        E += -0.5 * Bulk * StarConvexGamma * pm * pm;
+        return std::max(0.0, E);
+    }
+
+    // CosseratMaterial::DruckerPragerSplit (:206-261), with I1 over rows 0-2 and
+    // J2 = 1/2 e^T J2Mat_Coss e
+    double CossDruckerPrager(C18x18 DD, C18x18 DU, C18 SD, C18 SU, const double* e) const {
+        const double I1 = e[0] + e[1] + e[2];
+        double dJ2[18], J2 = 0;
+        for (int i = 0; i < 18; i++) {
+            double t = 0;
+            for (int j = 0; j < 18; j++) t += e[j] * J2c[j][i];
+            dJ2[i] = t;
+            J2 += 0.5 * e[i] * t;
+        }
+        auto ic = [](int i) { return i < 3 ? 1.0 : 0.0; };
+        const double sq = std::sqrt(J2 + 1.0e-16);
+        double dsq[18];
+        for (int i = 0; i < 18; i++) dsq[i] = 0.5 / sq * dJ2[i];
+        auto ddsq = [&](int i, int j) { return 0.5 / sq * J2c[i][j] - 0.25 / sq / sq / sq * dJ2[i] * dJ2[j]; };
+        double E;
+        if (-6.0 * B_dp * std::sqrt(J2) < I1) {
+            E = 0.5 * Bulk * I1 * I1 + 2.0 * Shear * J2;
+            for (int i = 0; i < 18; i++) {
+                SD[i] = Bulk * I1 * ic(i) + 2.0 * Shear * dJ2[i];
+                for (int j = 0; j < 18; j++) DD[i][j] = Bulk * ic(i) * ic(j) + 2.0 * Shear * J2c[i][j];
+            }
+        } else if (2 * Shear * std::sqrt(J2) < 3 * B_dp * Bulk * I1) {
+            E = 0.0;
+            for (int i = 0; i < 18; i++) {
+                SU[i] = Bulk * I1 * ic(i) + 2.0 * Shear * dJ2[i];
+                for (int j = 0; j < 18; j++) DU[i][j] = Bulk * ic(i) * ic(j) + 2.0 * Shear * J2c[i][j];
+            }
+        } else {
+            double pre = 1.0 / (18.0 * B_dp * B_dp * Bulk + 2.0 * Shear);
+            double q = -3.0 * B_dp * Bulk * I1 + 2 * Shear * sq, dq[18];
+            for (int i = 0; i < 18; i++) dq[i] = -3.0 * B_dp * Bulk * ic(i) + 2.0 * Shear * dsq[i];
+            E = pre * q * q;
+            for (int i = 0; i < 18; i++) {
+                SD[i] = pre * 2.0 * q * dq[i];
+                for (int j = 0; j < 18; j++) DD[i][j] = pre * (2.0 * q * 2.0 * Shear * ddsq(i, j) + 2.0 * dq[i] * dq[j]);
+            }
+            pre = Bulk * Shear / (18.0 * B_dp * B_dp * Bulk + 2.0 * Shear);
+            q = I1 + 6 * B_dp * sq;
+            for (int i = 0; i < 18; i++) dq[i] = ic(i) + 6 * B_dp * dsq[i];
+            for (int i = 0; i < 18; i++) {
+                SU[i] = pre * 2.0 * q * dq[i];
+                for (int j = 0; j < 18; j++) DU[i][j] = pre * (2.0 * q * 6 * B_dp * ddsq(i, j) + 2.0 * dq[i] * dq[j]);
+            }
+        }
+        return E;
+    }
+
+    // CosseratMaterial::SpecStressSplitConsistent (:263-369): eigen-split of the symmetrised
+    // macro stress (entries below 1e-12 replaced by (i+1) 1e-12); the projector P_plus is
+    // assembled with the flat tensor index 3a + b of its 9x9 block and applied to D_coss in
+    // the strain vector's own row order, as written; the rotation rows 9-17 pass through
+    double CossSpecStressConsistent(C18x18 DD, C18x18 DU, C18 SD, C18 SU, const double* e) const {
+        C18 St;
+        Dc_mul(e, 1.0, St);
+        double sn[9];
+        for (int i = 0; i < 9; i++) {
+            sn[i] = St[i];
+            if (std::fabs(sn[i]) < 1.0e-12) sn[i] = (i + 1) * 1.0e-12;
+        }
+        const double SM[3][3] = {{sn[0], sn[3], sn[7]}, {sn[4], sn[1], sn[5]}, {sn[8], sn[6], sn[2]}};
+        double A[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) A[i][j] = 0.5 * (SM[i][j] + SM[j][i]);
+        double lam[3], V[3][3];
+        eig3(A, lam, V);
+        double Pp[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double t = 0;
+                for (int k = 0; k < 3; k++) t += V[i][k] * std::max(lam[k], 0.0) * V[j][k];
+                Pp[i][j] = t;
+            }
+        const double sv[9] = {Pp[0][0], Pp[1][1], Pp[2][2], Pp[0][1], Pp[1][0], Pp[1][2], Pp[2][1], Pp[0][2], Pp[2][0]};
+        double E = 0;
+        for (int i = 0; i < 9; i++) E += sv[i] * e[i];
+        E *= 0.5;
+        double P[18][18] = {};
+        for (int k = 0; k < 3; k++) {
+            if (!(lam[k] > 0.0)) continue;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    for (int c = 0; c < 3; c++)
+                        for (int d = 0; d < 3; d++) P[3 * a + b][3 * c + d] += V[a][k] * V[b][k] * V[c][k] * V[d][k];
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                if (i == j) continue;
+                if (lam[i] <= 0.0 && lam[j] <= 0.0) continue;
+                const double den = lam[i] - lam[j];
+                if (std::fabs(den) < 1e-12) continue;
+                const double cf = (std::max(lam[i], 0.0) - std::max(lam[j], 0.0)) / den;
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                        for (int c = 0; c < 3; c++)
+                            for (int d = 0; d < 3; d++)
+                                P[3 * a + b][3 * c + d] += 0.5 * cf * (V[a][i] * V[b][j] * V[c][j] * V[d][i] +
+                                                                       V[a][j] * V[b][i] * V[c][i] * V[d][j]);
+            }
+        for (int i = 9; i < 18; i++) P[i][i] = 1.0;
+        for (int i = 0; i < 18; i++)
+            for (int j = 0; j < 18; j++) {
+                double t = 0;
+                for (int k = 0; k < 18; k++) t += P[i][k] * Dc[k][j];
+                DD[i][j] = t;
+                DU[i][j] = Dc[i][j] - t;  // (I - P) D_coss
+            }
+        for (int i = 0; i < 18; i++) {
+            double a = 0, b = 0;
+            for (int j = 0; j < 18; j++) { a += DD[i][j] * e[j]; b += DU[i][j] * e[j]; }
+            SD[i] = a;
+            SU[i] = b;
+        }
+        return E;
+    }
+
+    // CosseratMaterial::EnergySplit (:109-152); the stress and tangent are D_coss eps and
+    // D_coss whenever the energy is not consistent
     double CossEnergySplit(C18x18 D_Dam, C18x18 D_UnDam, C18 S_Dam, C18 S_Undam, const double* Strain) const {
         double Energy = 0;
         for (int i = 0; i < 18; i++) {
@@ -1462,10 +1637,14 @@ struct Material {
             for (int i = 0; i < 18; i++) Energy += S_Dam[i] * Strain[i];
             Energy *= 0.5;
         } else if (StressSplit == IFE_SPLIT_SPECSTRESS) {
-            if (ConsistentEnergy) throw std::invalid_argument("Cosserat SpecStress split with a consistent energy not supported");
-            Energy = CossSpecStressEnergy(Strain);
+            if (ConsistentEnergy) Energy = CossSpecStressConsistent(D_Dam, D_UnDam, S_Dam, S_Undam, Strain);
+            else Energy = CossSpecStressEnergy(Strain);
+        } else if (StressSplit == IFE_SPLIT_DRUCKERPRAGER) {
+            Energy = CossDruckerPrager(D_Dam, D_UnDam, S_Dam, S_Undam, Strain);
+        } else if (StressSplit == IFE_SPLIT_STARCONVEX) {
+            Energy = CossStarConvex(D_Dam, D_UnDam, S_Dam, S_Undam, Strain);
         } else {
-            throw std::invalid_argument("Cosserat energy split not supported");
+            throw std::invalid_argument("Stress split function type not defined for a Cosserat Material");
         }
         if (ConsistentEnergy == false) {
             Dc_mul(Strain, 1.0, S_Dam);

@@ -221,7 +221,7 @@ def test_batch_workload_sizes_are_seeded():
 
 
 def test_cosserat_model_checks():
-    """Fracture/CossPhaseField: 3D only, LE / VEP / dVEP, NoSplit or non-consistent SpecStress."""
+    """Fracture/CossPhaseField: LE / VEP / dVEP; NoSplit, SpecStress, DruckerPrager or StarConvex."""
     lib = abi.load_library()
     from ice_fem_2025a_amd.engine import _mesh_desc, _dofs_desc
     prob = make_problem("hex3d_coss", nel=(1, 1, 1))

@@ -102,6 +102,40 @@ def test_cosserat_against_oracle(require_gpu, name, variant, nel, tight):
     assert o.history(0, "PlasticStrain").shape[0] == 18
 
 
+@pytest.mark.parametrize("name,nel", [("hex3d_coss", (2, 3, 2)), ("quad2d_coss", (5, 4))])
+@pytest.mark.parametrize("split,consistent,B", [("StarConvex", 1, 0.0), ("StarConvex", 0, 0.0),
+                                                ("DruckerPrager", 1, 0.3), ("DruckerPrager", 1, -0.3),
+                                                ("DruckerPrager", 0, 0.3), ("SpecStress", 1, 0.0)])
+def test_cosserat_splits_against_oracle(require_gpu, name, nel, split, consistent, B):
+    """The StarConvex, DruckerPrager and consistent SpecStress splits of the Cosserat
+    material (CosseratMaterial.cpp:155-369), with the converged visco-elasto-plastic map:
+    K and f of every step and the energy history within 1e-12."""
+    _say(f"[coss split] {name} {split} consistent={consistent} B={B}")
+    prob = make_problem(name, nel=nel)
+    m = prob.models[0]
+    m.material.Split = abi.SPLIT[split]
+    m.material.B_DP = B
+    m.material.StarConvexGamma = 0.5
+    m.pf.ConsistentEnergy = consistent
+    o, e = Oracle(prob), Physics(prob)
+    for k, v in TIGHT:
+        o.set_param(0, k, v)
+        e.set_param(0, k, v)
+    for s in range(o.nsteps):
+        e.assemble(s)
+        Ke, fe = e.Kf(s)
+        Ko, fo = o.assemble(s)
+        rp, _ = o.pattern(s)
+        ek, ef = csr_err(Ke, Ko, rp), vec_err(fe, fo)
+        _say(f"  step {s}: K err {ek:.2e}, f err {ef:.2e}")
+        assert ek <= RTOL_FP64 and ef <= RTOL_FP64, f"step {s}"
+    ho, he = o.history(0, "Energy"), e.history(0, "Energy")
+    err = np.max(np.abs(he[:ho.shape[0]] - ho)) / max(np.max(np.abs(ho)), 1e-300)
+    _say(f"  energy history: {err:.2e}")
+    assert err <= 1e-12
+    assert e.status()["nonfinite"] == 0
+
+
 def test_cosserat_assemble_all_matches_stepwise(require_gpu):
     prob = make_problem("hex3d_coss", nel=(3, 3, 3))
     a, b = Physics(prob), Physics(prob)

@@ -384,3 +384,132 @@ def test_2d_micro_rotation_energy_against_quadrature(a, b):
                                                   _p(Su))
                     total += psi * wx * wy * 0.25 * hx * hy
     assert abs(UKU - 2 * total) <= 1e-10 * abs(2 * total)
+
+
+def _split(mat, pf, eps):
+    Dd, Du, Sd, Su = (np.zeros(324), np.zeros(324), np.zeros(18), np.zeros(18))
+    E = L.ifo_coss_energy_split(C.byref(mat), C.byref(pf), mat.TRef, _p(eps), _p(Dd), _p(Du), _p(Sd), _p(Su))
+    assert np.isfinite(E)
+    return E, Dd.reshape(18, 18), Du.reshape(18, 18), Sd, Su
+
+
+def _split_material(split, consistent=True):
+    prob = make_problem("hex3d_coss", nel=(1, 1, 1))
+    mat, pf = _mat(prob)
+    mat.Split = abi.SPLIT[split]
+    pf.ConsistentEnergy = 1 if consistent else 0
+    return mat, pf
+
+
+def _strain(rng, vol):
+    e = rng.normal(size=18) * 1e-4
+    e[:3] += vol - e[:3].mean()  # prescribed volumetric part
+    return e
+
+
+def _fd_check(mat, pf, e, rows_energy, h=1e-10):
+    """S_Dam = dE/de on rows_energy, D_Dam = dS_Dam/de and D_UnDam = dS_Undam/de (central
+    differences; the splits are smooth away from their regime boundaries)."""
+    E0, Dd, Du, Sd, Su = _split(mat, pf, e)
+    fdS = np.zeros(18)
+    fdDd, fdDu = np.zeros((18, 18)), np.zeros((18, 18))
+    for j in range(18):
+        ep, em = e.copy(), e.copy()
+        ep[j] += h
+        em[j] -= h
+        Ep, _, _, Sdp, Sup = _split(mat, pf, ep)
+        Em, _, _, Sdm, Sum = _split(mat, pf, em)
+        fdS[j] = (Ep - Em) / (2 * h)
+        fdDd[:, j] = (Sdp - Sdm) / (2 * h)
+        fdDu[:, j] = (Sup - Sum) / (2 * h)
+    sc = max(np.abs(Sd).max(), np.abs(Su).max())
+    assert np.allclose(fdS[rows_energy], Sd[rows_energy], rtol=1e-6, atol=1e-6 * sc)
+    dsc = max(np.abs(Dd).max(), np.abs(Du).max())
+    assert np.allclose(fdDd, Dd, rtol=1e-5, atol=1e-6 * dsc)
+    assert np.allclose(fdDu, Du, rtol=1e-5, atol=1e-6 * dsc)
+    return E0, Dd, Du, Sd, Su
+
+
+@pytest.mark.parametrize("vol", [3e-4, -3e-4])
+def test_cosserat_star_convex_split_derivatives(vol):
+    """StarConvexSplit (CosseratMaterial.cpp:155-204): consistent tangents everywhere; the
+    energy derivative equals the damaged stress on the macro rows for p > 0 (for p < 0 the
+    star-convex correction enters the energy only, and the curvature rows carry stress but
+    no energy, as written)."""
+    mat, pf = _split_material("StarConvex")
+    e = _strain(np.random.default_rng(3), vol)
+    E, Dd, Du, Sd, Su = _fd_check(mat, pf, e, list(range(9)) if vol > 0 else [])
+    assert E >= 0.0
+    K = mat.Young / (3 * (1 - 2 * mat.Poisson))
+    if vol > 0:
+        assert np.allclose(Su, 0.0) and np.allclose(Du, 0.0)
+    else:  # the compressive volumetric part is undamaged: K p on rows 0-2
+        p = e[:3].sum()
+        assert np.allclose(Su[:3], K * p, rtol=1e-12) and np.allclose(Su[3:], 0.0)
+
+
+@pytest.mark.parametrize("B,vol,regime", [(0.3, 2e-4, "intact"), (0.3, -3e-4, "mixed"),
+                                          (-0.3, -3e-3, "undamaged")])
+def test_cosserat_drucker_prager_split_derivatives(B, vol, regime):
+    """DruckerPragerSplit (CosseratMaterial.cpp:206-261) with J2 = 1/2 e^T J2Mat_Coss e, in
+    its three regimes (the fully undamaged one needs B < 0, as the conditions are written):
+    S_Dam = dE/de and both tangents by central differences; the intact regime is
+    1/2 K I1^2 + 2 G J2 with nothing undamaged, the undamaged one has no energy."""
+    mat, pf = _split_material("DruckerPrager")
+    mat.B_DP = B
+    e = _strain(np.random.default_rng(4), vol)
+    E, Dd, Du, Sd, Su = _fd_check(mat, pf, e, list(range(18)))
+    K = mat.Young / (3 * (1 - 2 * mat.Poisson))
+    G = mat.Young / (2 * (1 + mat.Poisson))
+    I1 = e[:3].sum()
+    _, _, _, J2c_e, _ = _split(*_split_material("None", consistent=False), np.zeros(18))
+    if regime == "intact":
+        assert np.allclose(Su, 0.0) and np.allclose(Du, 0.0)
+        J2 = (E - 0.5 * K * I1 * I1) / (2 * G)
+        assert J2 > 0
+    elif regime == "undamaged":
+        assert E == 0.0 and np.allclose(Sd, 0.0) and np.allclose(Dd, 0.0)
+    else:
+        assert E > 0 and np.abs(Su).max() > 0
+
+
+def test_cosserat_consistent_spec_stress_split():
+    """SpecStressSplitConsistent (CosseratMaterial.cpp:263-369): the energy is half the
+    positive part of the symmetrised macro stress (numpy eigh) against the macro strain;
+    D_Dam + D_UnDam = D_coss; with all principal stresses positive the projector is, by the
+    completeness of the eigenvectors, the transposition (a, b) -> (b, a) of the flat tensor
+    index 3a + b (so D_Dam = P D_coss in closed form), with all negative it vanishes on the
+    macro rows; S = D e for both parts."""
+    mat, pf = _split_material("SpecStress")
+    _, Dc, _, _, _ = _split(*_split_material("None", consistent=False), np.zeros(18))
+    rng = np.random.default_rng(5)
+    seen = set()
+    for vol in (4e-4, -4e-4, 0.0):
+        e = _strain(rng, vol)
+        E, Dd, Du, Sd, Su = _split(mat, pf, e)
+        sig = Dc @ e
+        S = np.array([[sig[0], sig[3], sig[7]], [sig[4], sig[1], sig[5]], [sig[8], sig[6], sig[2]]])
+        lam, V = np.linalg.eigh(0.5 * (S + S.T))
+        Pp = V @ np.diag(np.maximum(lam, 0)) @ V.T
+        sv = np.array([Pp[0, 0], Pp[1, 1], Pp[2, 2], Pp[0, 1], Pp[1, 0], Pp[1, 2], Pp[2, 1], Pp[0, 2], Pp[2, 0]])
+        assert np.isclose(E, 0.5 * sv @ e[:9], rtol=1e-10, atol=1e-12 * np.abs(sig).max() * np.abs(e).max())
+        assert np.allclose(Dd + Du, Dc, rtol=0, atol=1e-9 * np.abs(Dc).max())
+        assert np.allclose(Sd, Dd @ e, atol=1e-9 * np.abs(sig).max()) and np.allclose(Su, Du @ e, atol=1e-9 * np.abs(sig).max())
+        P = np.eye(18)
+        if np.all(lam > 0):
+            # sum_i u_i u_i u_i u_i + the pair terms with coefficient 1 sum to
+            # delta_ad delta_bc by completeness: the transposition of the flat index
+            tr = np.zeros((9, 9))
+            for a in range(3):
+                for b in range(3):
+                    tr[3 * a + b, 3 * b + a] = 1.0
+            P[:9, :9] = tr
+            assert np.allclose(Dd, P @ Dc, rtol=0, atol=1e-9 * np.abs(Dc).max())
+            seen.add("+")
+        elif np.all(lam < 0):
+            P[:9, :9] = 0.0
+            assert np.allclose(Dd, P @ Dc, rtol=0, atol=1e-9 * np.abs(Dc).max())
+            seen.add("-")
+        else:
+            seen.add("mixed")
+    assert seen == {"+", "-", "mixed"}
