@@ -53,7 +53,8 @@ class Material(C.Structure):
 class PhaseFieldOpts(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("DamFunction", "DensityDamage", "GravityDamage", "ViscoDriving")] + \
                [(n, C.c_double) for n in ("ViscoDrivingFactor", "k0", "p")] + \
-               [(n, C.c_int32) for n in ("DistributionFunction", "ConsistentEnergy", "ConsistentDegradation")]
+               [(n, C.c_int32) for n in ("DistributionFunction", "ConsistentEnergy", "ConsistentDegradation")] + \
+               [("Coefs", C.c_double * 5)]
 
 
 class ModelOpts(C.Structure):

@@ -216,6 +216,8 @@ def phasefield_struct(inputs, cosserat=False):
     s.DistributionFunction = abi.DISTRIBUTION[pf.get("DistributionFunction", "Quadratic")]
     s.ConsistentEnergy = int(pf.get("ConsistentEnergy", True))
     s.ConsistentDegradation = int(pf.get("ConsistentDegradation", False))
+    for i, c in enumerate(pf.get("Coefs", [0.0] * 5)):  # Custom density: w, a_1, a_2, b, c
+        s.Coefs[i] = c
     return s
 
 

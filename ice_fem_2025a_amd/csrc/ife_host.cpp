@@ -282,6 +282,7 @@ MatConst make_matconst(const ife_physics* ph, const Model* pfm, const Model* thm
         mc.DamFunction = pf.DamFunction; mc.DensityDamage = pf.DensityDamage; mc.GravityDamage = pf.GravityDamage;
         mc.ViscoDriving = pf.ViscoDriving; mc.ViscoDrivingFactor = pf.ViscoDrivingFactor; mc.k0 = pf.k0; mc.p = pf.p;
         mc.Distribution = pf.DistributionFunction; mc.ConsistentDegradation = pf.ConsistentDegradation;
+        for (int i = 0; i < 5; i++) mc.coefs[i] = pf.Coefs[i];
         const ife_model_opts& o = pfm->opt;
         mc.ViscousStab = o.ViscousStab; mc.Irreversible = o.Irreversible; mc.Inertia = o.Inertia;
         mc.LumpedCap = o.LumpedPFCapacity; mc.LumpedDrive = o.LumpedDriving; mc.ViscousHeating = o.ViscousHeating;
@@ -535,9 +536,9 @@ void setup_device(ife_physics* ph) {
     ph->ready = true;
 }
 
-bool is_ho_distribution(int method) {  // PhaseFieldUtil.cpp:18-22
+bool is_ho_distribution(int method) {  // NeedsGrad2 (PhaseFieldUtil.cpp:18-22)
     return method == IFE_DIST_HO_LINEAR || method == IFE_DIST_HO_LINEAR2 || method == IFE_DIST_HO_CZM ||
-           method == IFE_DIST_HO_QUADRATIC;
+           method == IFE_DIST_HO_QUADRATIC || method == IFE_DIST_CUSTOM;
 }
 
 void require_ready(const ife_physics* ph) {
@@ -1055,8 +1056,9 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
             if (mat->Split == IFE_SPLIT_SPECSTRESS && pf->ConsistentEnergy)
                 THROW(IFE_ERR_UNSUPPORTED, "SpecStress split with ConsistentEnergy is not defined by the reference");
             int dfn = pf->DistributionFunction;
-            if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_LOCAL)  // Custom needs a user callback
-                THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
+            if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_CUSTOM) THROW(IFE_ERR_INVALID, "unknown distribution function");
+            if (dfn == IFE_DIST_CUSTOM && pf->Coefs[0] == 0.0)
+                THROW(IFE_ERR_INVALID, "Custom distribution: Coefs[0] (w) must be nonzero");
             add_multipliers(m.get());
             int dams[] = {pf->DamFunction, pf->DensityDamage, pf->GravityDamage, pf->ViscoDriving};
             for (int d : dams)
@@ -1094,8 +1096,9 @@ int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* na
                   mat->Split == IFE_SPLIT_DRUCKERPRAGER || mat->Split == IFE_SPLIT_STARCONVEX))
                 THROW(IFE_ERR_UNSUPPORTED, "Cosserat energy split: NoSplit, SpecStress, DruckerPrager or StarConvex");
             int dfn = pf->DistributionFunction;
-            if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_LOCAL)  // Custom needs a user callback
-                THROW(IFE_ERR_UNSUPPORTED, "distribution function not implemented by the engine");
+            if (dfn < IFE_DIST_LINEAR || dfn > IFE_DIST_CUSTOM) THROW(IFE_ERR_INVALID, "unknown distribution function");
+            if (dfn == IFE_DIST_CUSTOM && pf->Coefs[0] == 0.0)
+                THROW(IFE_ERR_INVALID, "Custom distribution: Coefs[0] (w) must be nonzero");
             add_multipliers(m.get());
             if (mat->l_coss <= 0.0) THROW(IFE_ERR_INVALID, "l_coss must be positive");
         } else if (tn == "Thermal/ThermalModel") {

@@ -37,6 +37,7 @@ struct MatConst {
     int DamFunction, DensityDamage, GravityDamage, ViscoDriving;
     double ViscoDrivingFactor, k0, p;
     int Distribution;
+    double coefs[5];  // Custom density: w, a_1, a_2, b, c
     int ConsistentDegradation;
     // model options
     double ViscousStab;

@@ -846,8 +846,13 @@ __device__ double energy_split(const MatConst& mc, const MatIP& m, const double 
 // crack-density coefficients (PhaseFieldUtil.cpp:185-280; of the higher-order forms the N-parts):
 // Dist = dN*N + cGG*G^T grad(phi),  dDist = cNN*N N^T + cGG*G^T G,  gamma
 __device__ void dist_coeffs(int method, double l, double phi, double gg, double& dN, double& cNN, double& cGG,
-                            double& gamma) {
+                            double& gamma, const double* cf) {
     switch (method) {
+        case IFE_DIST_CUSTOM: {  // 1/(w l) (a_1 phi + a_2 phi^2 + b l^2 |grad phi|^2 + c l^4 ...) (:269-273)
+            const double c = 1.0 / (cf[0] * l);
+            gamma = c * (cf[1] * phi + cf[2] * phi * phi);
+            dN = c * (cf[1] + 2.0 * cf[2] * phi); cNN = c * 2.0 * cf[2]; cGG = 0.0;
+        } break;
         case IFE_DIST_QUADRATIC:
             gamma = 1.0 / (2.0 * l) * (phi * phi + l * l * gg);
             dN = 1.0 / (2.0 * l) * 2.0 * phi; cNN = 1.0 / (2.0 * l) * 2.0; cGG = 1.0 / (2.0 * l) * 2.0 * l * l;
@@ -892,15 +897,16 @@ __device__ void dist_coeffs(int method, double l, double phi, double gg, double&
 
 __device__ __forceinline__ bool is_ho(int method) {
     return method == IFE_DIST_HO_QUADRATIC || method == IFE_DIST_HO_CZM || method == IFE_DIST_HO_LINEAR ||
-           method == IFE_DIST_HO_LINEAR2;
+           method == IFE_DIST_HO_LINEAR2 || method == IFE_DIST_CUSTOM;
 }
 
 // Derivative parts of the higher-order densities (PhaseFieldUtil.cpp:236-278): dDist gains
 // kG G^T G + kG2 G2^T DC G2 and Dist the same applied to phi, with DC = diag(1,1[,1],2[,2,2]);
 // gamma gains kG/2 |grad phi|^2 + kG2/2 ddphi^T DC ddphi.
-__device__ __forceinline__ void ho_coeffs(int method, double l, double& kG, double& kG2) {
+__device__ __forceinline__ void ho_coeffs(int method, double l, double& kG, double& kG2, const double* cf) {
     const double l2 = l * l, l4 = l2 * l2;
     switch (method) {
+        case IFE_DIST_CUSTOM: kG = 1.0 / (cf[0] * l) * 2.0 * cf[3] * l2; kG2 = 1.0 / (cf[0] * l) * 2.0 * cf[4] * l4; break;
         case IFE_DIST_HO_QUADRATIC: kG = 1.0 / (2.0 * l) * l2; kG2 = 1.0 / (2.0 * l) * (l4 / 8.0); break;
         case IFE_DIST_HO_CZM: kG = 1.0 / (3.020 * l) * l2; kG2 = 1.0 / (3.020 * l) * (l4 / 8.0); break;
         case IFE_DIST_HO_LINEAR: kG = 0.0; kG2 = 1.0 / (19.2 * l) * (2.0 * l4); break;
@@ -909,8 +915,14 @@ __device__ __forceinline__ void ho_coeffs(int method, double l, double& kG, doub
 }
 
 // GetCapacityOnly (PhaseFieldUtil.cpp:282-348): IP part (N-coefficients) and lumped part
-__device__ void cap_coeffs(int method, double l, double phi, double& dN, double& cNN, double& LD, double& dLD) {
+__device__ void cap_coeffs(int method, double l, double phi, double& dN, double& cNN, double& LD, double& dLD,
+                           const double* cf) {
     switch (method) {
+        case IFE_DIST_CUSTOM: {
+            const double c = 1.0 / (cf[0] * l);
+            dN = c * (cf[1] + 2.0 * cf[2] * phi); cNN = c * 2.0 * cf[2];
+            LD = c * (cf[1] + 2.0 * cf[2] * phi); dLD = c * 2.0 * cf[2];
+        } break;
         case IFE_DIST_QUADRATIC: dN = 1.0 / (2.0 * l) * 2.0 * phi; cNN = 1.0 / (2.0 * l) * 2.0; LD = 1.0 / l * phi; dLD = 1.0 / l; break;
         case IFE_DIST_CZM:
             dN = 1.0 / (3.141592 * l) * 2.0 * (1.0 - phi); cNN = 1.0 / (3.141592 * l) * (-2.0);
@@ -1348,11 +1360,11 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
             double gg = 0;
             for (int c = 0; c < DIM; c++) gg += gphi[c] * gphi[c];
             double dN, cNN, cGG, gamma;
-            dist_coeffs(mc.Distribution, mc.l, phi, gg, dN, cNN, cGG, gamma);
+            dist_coeffs(mc.Distribution, mc.l, phi, gg, dN, cNN, cGG, gamma, mc.coefs);
             double capN = 0, capNN = 0;
             if (mc.LumpedCap) {
                 double u1, u2;
-                cap_coeffs(mc.Distribution, mc.l, phi, capN, capNN, u1, u2);
+                cap_coeffs(mc.Distribution, mc.l, phi, capN, capNN, u1, u2, mc.coefs);
             }
             if constexpr (ADJ) {
                 // phase residual: Gc multiplies w (dN - capN) N and w cGG grad(phi) . G
@@ -1782,7 +1794,7 @@ __device__ __forceinline__ double nodal_terms(Smem<DIM, T>& sm, const KParams& p
         }
         if (mc.LumpedCap) {
             double u1, u2, LD, dLD;
-            cap_coeffs(mc.Distribution, mc.l, sm.Ph[a], u1, u2, LD, dLD);
+            cap_coeffs(mc.Distribution, mc.l, sm.Ph[a], u1, u2, LD, dLD, mc.coefs);
             kd += s * GcL * dLD;
             f += s * GcL * LD;
         }
@@ -2128,7 +2140,7 @@ __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
         sm.t1[kind][d][q][a] = s;
     }
     double kG, kG2;
-    ho_coeffs(p.mc.Distribution, p.mc.l, kG, kG2);
+    ho_coeffs(p.mc.Distribution, p.mc.l, kG, kG2, p.mc.coefs);
     const double* const ipc = p.ipb + el * IPB_NF * NIP;
     double acc[SLOTS];
     for (int k = 0; k < SLOTS; k++) acc[k] = 0.0;
@@ -3199,7 +3211,7 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
             double gg = 0;
             for (int c = 0; c < 3; c++) gg += gphi[c] * gphi[c];
             double dN, cNN, cGG, gamma;
-            dist_coeffs(mc.Distribution, mc.l, phi, gg, dN, cNN, cGG, gamma);
+            dist_coeffs(mc.Distribution, mc.l, phi, gg, dN, cNN, cGG, gamma, mc.coefs);
             double A = w * m.Gc * cNN, FN = w * m.Gc * dN;
             FN += w * dDam * Energy;
             A += w * ddDam * Energy;

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define IFE_API_VERSION 2
+#define IFE_API_VERSION 3
 
 typedef enum {
     IFE_OK = 0,
@@ -144,6 +144,9 @@ typedef struct {
     double ViscoDrivingFactor, k0, p;
     int32_t DistributionFunction;   /* ife_distribution */
     int32_t ConsistentEnergy, ConsistentDegradation;
+    /* properties.PhaseField.Coefs of the Custom density (w, a_1, a_2, b, c;
+     * PhaseFieldUtil.cpp:26-29, PhaseFieldUtil.h:96); unused by the other densities */
+    double Coefs[5];
 } ife_phasefield_opts;
 
 /* Models.<name>.* (PhaseField.cpp:167-283). */

@@ -1854,7 +1854,7 @@ struct Material {
 // PhaseFieldUtility::GetDistributionFunction (PhaseFieldUtil.cpp:185-221), non-HO forms.
 // Dist (n), dDist (n x n column-major)
 static double GetDistributionFunction(int method, double l, int dim, Vec& Dist, Mat& dDist, const Shapes& S, int ip,
-                                      const double* Phase) {
+                                      const double* Phase, const double* Coefs) {
     const int n = S.n;
     double phi = 0;
     for (int a = 0; a < n; a++) phi += S.Nv(ip, a) * Phase[a];
@@ -1902,6 +1902,7 @@ static double GetDistributionFunction(int method, double l, int dim, Vec& Dist, 
             for (int b = 0; b < n; b++)
                 for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / (l) * (2 * S.Nv(ip, a) * S.Nv(ip, b));
             break;
+        case IFE_DIST_CUSTOM:
         case IFE_DIST_HO_QUADRATIC:
         case IFE_DIST_HO_CZM:
         case IFE_DIST_HO_LINEAR:
@@ -1921,7 +1922,18 @@ static double GetDistributionFunction(int method, double l, int dim, Vec& Dist, 
             auto G2tD = [&](int a) { double t = 0; for (int r = 0; r < ng; r++) t += S.G2v(ip, r, a) * DC[r] * ddG[r]; return t; };
             auto G2G2 = [&](int a, int b) { double t = 0; for (int r = 0; r < ng; r++) t += S.G2v(ip, r, a) * DC[r] * S.G2v(ip, r, b); return t; };
             const double l4 = l * l * l * l;
-            if (method == IFE_DIST_HO_QUADRATIC || method == IFE_DIST_HO_CZM) {
+            if (method == IFE_DIST_CUSTOM) {  // Coefs = w, a_1, a_2, b, c (:269-273)
+                if (!Coefs) throw std::invalid_argument("Custom distribution needs its coefficients");
+                const double c = 1.0 / (Coefs[0] * l);
+                gamma = c * (Coefs[1] * phi + Coefs[2] * phi * phi + Coefs[3] * l * l * dGdG + Coefs[4] * l4 * ddd);
+                for (int a = 0; a < n; a++)
+                    Dist[a] = c * (Coefs[1] * S.Nv(ip, a) + 2.0 * Coefs[2] * S.Nv(ip, a) * phi + 2.0 * Coefs[3] * l * l * GtdG(a) +
+                                   2.0 * Coefs[4] * l4 * G2tD(a));
+                for (int b = 0; b < n; b++)
+                    for (int a = 0; a < n; a++)
+                        dDist(a, b) = c * (2.0 * Coefs[2] * S.Nv(ip, a) * S.Nv(ip, b) + 2.0 * Coefs[3] * l * l * GtG(a, b) +
+                                           2.0 * Coefs[4] * l4 * G2G2(a, b));
+            } else if (method == IFE_DIST_HO_QUADRATIC || method == IFE_DIST_HO_CZM) {
                 const bool q = method == IFE_DIST_HO_QUADRATIC;
                 const double c = q ? 1.0 / (2.0 * l) : 1.0 / (3.020 * l);
                 gamma = c * ((q ? phi * phi : 2.0 * phi - phi * phi) + 0.5 * l * l * dGdG + l4 / 16.0 * ddd);
@@ -1951,7 +1963,7 @@ static double GetDistributionFunction(int method, double l, int dim, Vec& Dist, 
 
 // PhaseFieldUtility::GetCapacityOnly (PhaseFieldUtil.cpp:282-348)
 static void GetCapacityOnly(int method, double l, Vec& Dist, Mat& dDist, double& DL, double& dDL, const Shapes& S, int ip,
-                            double phi) {
+                            double phi, const double* Coefs) {
     const int n = S.n;
     switch (method) {
         case IFE_DIST_QUADRATIC:
@@ -1982,6 +1994,13 @@ static void GetCapacityOnly(int method, double l, Vec& Dist, Mat& dDist, double&
                 for (int a = 0; a < n; a++) dDist(a, b) = 1.0 / l * (2.0 * S.Nv(ip, a) * S.Nv(ip, b));
             DL = 2.0 / l * phi; dDL = 2.0 / l;
             break;
+        case IFE_DIST_CUSTOM: {  // PhaseFieldUtil.cpp:338-343
+            const double c = 1.0 / (Coefs[0] * l);
+            for (int a = 0; a < n; a++) Dist[a] = c * (Coefs[1] * S.Nv(ip, a) + 2.0 * Coefs[2] * S.Nv(ip, a) * phi);
+            for (int b = 0; b < n; b++)
+                for (int a = 0; a < n; a++) dDist(a, b) = c * (2.0 * Coefs[2] * S.Nv(ip, a) * S.Nv(ip, b));
+            DL = c * (Coefs[1] + 2.0 * Coefs[2] * phi); dDL = c * 2.0 * Coefs[2];
+        } break;
         case IFE_DIST_HO_QUADRATIC:  // PhaseFieldUtil.cpp:319-345
             for (int a = 0; a < n; a++) Dist[a] = 1.0 / (2.0 * l) * (2.0 * S.Nv(ip, a) * phi);
             for (int b = 0; b < n; b++)
@@ -2095,9 +2114,9 @@ static void getShapeGrads2(const Ctx& c, int64_t e, Shapes& S) {
     else shapeGrads2_NURBS3_Cube2(c.et, Xe.data(), Ce, We.data(), S);
 }
 
-static bool needs_grad2(int method) {  // PhaseFieldUtil.cpp:18-22 (Custom is not supported)
+static bool needs_grad2(int method) {  // PhaseFieldUtil.cpp:18-22
     return method == IFE_DIST_HO_LINEAR || method == IFE_DIST_HO_LINEAR2 || method == IFE_DIST_HO_CZM ||
-           method == IFE_DIST_HO_QUADRATIC;
+           method == IFE_DIST_HO_QUADRATIC || method == IFE_DIST_CUSTOM;
 }
 
 // DofSpace::getDofForNodes (DofSpace.cpp:987-1008): out[i + j*n] = dof(node_i, field_j)
@@ -2522,10 +2541,10 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
             } else {
                 md.Energy[gi] = Energy;
             }
-            double gamma = GetDistributionFunction(md.pf.DistributionFunction, M.pf_l, dim, Dist, dDist, Sph, ip, Phase.data());
+            double gamma = GetDistributionFunction(md.pf.DistributionFunction, M.pf_l, dim, Dist, dDist, Sph, ip, Phase.data(), md.pf.Coefs);
             if (md.opt.LumpedPFCapacity) {
                 double u1, u2;
-                GetCapacityOnly(md.pf.DistributionFunction, M.pf_l, DistCap, dDistCap, u1, u2, Sph, ip, phi);
+                GetCapacityOnly(md.pf.DistributionFunction, M.pf_l, DistCap, dDistCap, u1, u2, Sph, ip, phi, md.pf.Coefs);
             } else {
                 DistCap.assign(n, 0.0);
                 dDistCap.zero();
@@ -2584,7 +2603,7 @@ static void Assemble_Phase(Ctx& c, Model& md, Sink& sink, int64_t e0, int64_t e1
             }
             if (md.opt.LumpedPFCapacity) {
                 double LD, dLD;
-                GetCapacityOnly(md.pf.DistributionFunction, M.pf_l, DistCap, dDistCap, LD, dLD, Sph, 0, Phase[a]);
+                GetCapacityOnly(md.pf.DistributionFunction, M.pf_l, DistCap, dDistCap, LD, dLD, Sph, 0, Phase[a], md.pf.Coefs);
                 F_Ph[a] += WLumped[a] * M.Gc * LD;
                 K_PhPh(a, a) += WLumped[a] * M.Gc * dLD;
             }
@@ -2876,7 +2895,7 @@ static void Assemble_Phase_Coss(Ctx& c, Model& md, Sink& sink, int64_t e0, int64
             } else {
                 md.Energy[gi] = Energy;
             }
-            double gamma = GetDistributionFunction(md.pf.DistributionFunction, M.pf_l, c.dim, Dist, dDist, Sph, ip, Phase.data());
+            double gamma = GetDistributionFunction(md.pf.DistributionFunction, M.pf_l, c.dim, Dist, dDist, Sph, ip, Phase.data(), md.pf.Coefs);
             double w = Su.w[ip];
             st.lcrack += gamma * w;
             for (int a = 0; a < n; a++) F_Ph[a] += w * M.Gc * Dist[a];
@@ -3170,7 +3189,7 @@ int ifo_shape_grads2(void* h, int64_t el, double* N, double* G, double* G2, doub
 // GetDistributionFunction of `method` at element el, IP ip for nodal phase values Phase
 // (n): Dist (n), dDist (n x n column-major); returns gamma (NaN on an unknown method)
 double ifo_distribution(void* h, int64_t el, int ip, int method, double l, const double* Phase, double* Dist,
-                        double* dDist) {
+                        double* dDist, const double* Coefs) {
     Ctx* c = (Ctx*)h;
     Shapes S;
     S.resize(c->nip, c->n, c->dim);
@@ -3180,7 +3199,7 @@ double ifo_distribution(void* h, int64_t el, int ip, int method, double l, const
     Mat dD(c->n, c->n);
     double g;
     try {
-        g = GetDistributionFunction(method, l, c->dim, D, dD, S, ip, Phase);
+        g = GetDistributionFunction(method, l, c->dim, D, dD, S, ip, Phase, Coefs);
     } catch (std::exception& e) {
         g_err = e.what();
         return std::numeric_limits<double>::quiet_NaN();

@@ -48,7 +48,7 @@ def load():
         "ifo_return_map_vep": (None, [C.POINTER(abi.Material), C.c_double, C.c_double, C.c_double, dp, dp, dp, dp,
                                       dp, dp, dp, C.POINTER(C.c_int)]),
         "ifo_shape_grads2": (C.c_int, [P, C.c_int64, dp, dp, dp, dp]),
-        "ifo_distribution": (C.c_double, [P, C.c_int64, C.c_int, C.c_int, C.c_double, dp, dp, dp]),
+        "ifo_distribution": (C.c_double, [P, C.c_int64, C.c_int, C.c_int, C.c_double, dp, dp, dp, dp]),
         "ifo_coss_yield": (C.c_double, [C.POINTER(abi.Material), C.c_double, C.c_int, dp, dp, dp]),
         "ifo_coss_energy_split": (C.c_double, [C.POINTER(abi.Material), C.POINTER(abi.PhaseFieldOpts), C.c_double, dp,
                                                dp, dp, dp, dp]),

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     for name in decl:
         assert hasattr(lib, name), name
     assert set(decl) == set(abi.EXPORTED)
-    assert lib.ife_api_version() == 2
+    assert lib.ife_api_version() == 3
 
 
 def test_unknown_model_is_rejected():
@@ -247,8 +247,9 @@ def test_cosserat_model_checks():
 @pytest.mark.parametrize("name,model", [("hex3d_glen_at2", b"Fracture/PhaseField"), ("quad2d_le_at2", b"Fracture/PhaseField"),
                                         ("hex3d_coss", b"Fracture/CossPhaseField")])
 def test_every_reference_distribution_is_accepted(name, model):
-    """The nine built-in crack densities (with the lumped capacity form where the model has
-    one) are accepted; Custom, which needs a user callback, is rejected as unsupported."""
+    """The ten crack densities (Custom with its coefficients) are accepted with and without
+    the lumped capacity form; Custom with w = 0 (a division by zero in the reference) and an
+    unknown value are rejected as invalid."""
     lib = abi.load_library()
     from ice_fem_2025a_amd.engine import _mesh_desc, _dofs_desc
     prob = make_problem(name, nel=(1,) * (3 if name.startswith("hex") else 2))
@@ -256,13 +257,16 @@ def test_every_reference_distribution_is_accepted(name, model):
     md = _mesh_desc(prob.mesh, keep)
     dd = _dofs_desc(prob.dof_names, prob.dof_steps, keep)
     m = prob.models[0]
-    for method, code in [(v, abi.IFE_OK) for k, v in abi.DISTRIBUTION.items() if k != "Custom"] + \
-                        [(abi.DISTRIBUTION["Custom"], 2)]:
+    cases = [(v, (2.0, 0.3, 1.0, 0.8, 0.05), abi.IFE_OK) for v in abi.DISTRIBUTION.values()]
+    cases += [(abi.DISTRIBUTION["Custom"], (0.0, 0.3, 1.0, 0.8, 0.05), 1), (len(abi.DISTRIBUTION), (2.0,) * 5, 1)]
+    for method, coefs, code in cases:
         for lumped in (0, 1):
             h = C.c_void_p()
             assert lib.ife_physics_create(C.byref(md), C.byref(dd), C.byref(h)) == abi.IFE_OK
             pf = abi.PhaseFieldOpts.from_buffer_copy(m.pf)
             pf.DistributionFunction = method
+            for i, c in enumerate(coefs):
+                pf.Coefs[i] = c
             opts = abi.ModelOpts.from_buffer_copy(m.opts)
             opts.LumpedPFCapacity = lumped
             assert lib.ife_physics_add_model(h, model, b"x", 0, C.byref(m.material), C.byref(pf), C.byref(opts),

@@ -1,5 +1,5 @@
-"""GPU parity of the higher-order crack densities (GetDistributionFunction with second shape
-derivatives, PhaseFieldUtil.cpp:223-280, Mesh::getShapeGrads2 in the phase step,
+"""GPU parity of the higher-order crack densities and the Custom density (GetDistributionFunction
+with second shape derivatives, PhaseFieldUtil.cpp:223-273, Mesh::getShapeGrads2 in the phase step,
 PhaseField.cpp:528-532) and of the lumped capacity forms (GetCapacityOnly,
 PhaseFieldUtil.cpp:282-348) against the oracle: K and f of every step within 1e-12
 (row/step scaled), the crack length within 1e-12 relative.
@@ -25,7 +25,8 @@ from parity_util import RTOL_FP64, csr_err, vec_err
 
 pytestmark = pytest.mark.gpu
 
-HO = ["HO_Quadratic", "HO_CZM", "HO_Linear", "HO_Linear2"]
+HO = ["HO_Quadratic", "HO_CZM", "HO_Linear", "HO_Linear2", "Custom"]
+CUSTOM = (2.0, 0.3, 1.0, 0.8, 0.05)  # Custom density coefficients w, a_1, a_2, b, c
 COSS_TIGHT = (("rm_abs_tol", 1e-40), ("rm_rel_tol", 1e-30), ("rm_max_iterations", 60))
 
 
@@ -52,6 +53,8 @@ def _problem(name, nel, method, distort, lumped=False):
     m = prob.mesh
     prob.models[0].material.l = 1.5 * min(l / n for l, n in zip(m.lengths, m.nel))  # l^4 |G2|^2 ~ |N|^2
     prob.models[0].pf.DistributionFunction = abi.DISTRIBUTION[method]
+    for i, c in enumerate(CUSTOM):
+        prob.models[0].pf.Coefs[i] = c
     if lumped:
         prob.models[0].opts.LumpedPFCapacity = 1
     if distort:
@@ -88,7 +91,8 @@ def _derivative_share(name, nel, method, distort):
     s = _phase_step(make_problem(name, nel=nel))
     a = Oracle(_problem(name, nel, method, distort))
     Ka, _ = a.assemble(s)
-    base = {"HO_Quadratic": "Quadratic", "HO_CZM": "CZM", "HO_Linear": "Linear2", "HO_Linear2": "Linear2"}[method]
+    base = {"HO_Quadratic": "Quadratic", "HO_CZM": "CZM", "HO_Linear": "Linear2", "HO_Linear2": "Linear2",
+            "Custom": "Quadratic"}[method]
     b = Oracle(_problem(name, nel, base, distort))
     Kb, _ = b.assemble(s)
     return np.abs(Ka - Kb).max() / np.abs(Ka).max()
@@ -105,14 +109,14 @@ def test_ho_density_against_oracle(require_gpu, name, nel, method, distort):
     _compare(_problem(name, nel, method, distort))
 
 
-@pytest.mark.parametrize("method", ["HO_Quadratic", "HO_Linear"])
+@pytest.mark.parametrize("method", ["HO_Quadratic", "HO_Linear", "Custom"])
 @pytest.mark.parametrize("name,nel", [("hex3d_coss", (2, 2, 2)), ("quad2d_coss", (5, 4))])
 def test_ho_density_cosserat_against_oracle(require_gpu, name, nel, method):
     _say(f"[ho] {name} {nel} {method}")
     _compare(_problem(name, nel, method, distort=False), tight=COSS_TIGHT)
 
 
-@pytest.mark.parametrize("method", ["HO_Quadratic", "HO_CZM", "HO_Linear", "HO_Linear2", "Linear2", "Local"])
+@pytest.mark.parametrize("method", ["HO_Quadratic", "HO_CZM", "HO_Linear", "HO_Linear2", "Linear2", "Local", "Custom"])
 @pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (4, 3)), ("hex3d_glen_at2", (2, 3, 2))])
 def test_lumped_capacity_against_oracle(require_gpu, name, nel, method):
     _say(f"[lumped capacity] {name} nel={nel} {method}")

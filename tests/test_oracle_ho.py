@@ -23,6 +23,7 @@ from ice_fem_2025a_amd.mesh import open_uniform_knots
 from oracle.oracle import Oracle
 
 dp = C.POINTER(C.c_double)
+CUSTOM = np.array([2.0, 0.3, 1.0, 0.8, 0.05])  # Custom density coefficients w, a_1, a_2, b, c
 
 
 def _p(a):
@@ -85,7 +86,7 @@ def test_rational_second_derivatives_with_weights():
         assert np.abs(G2 @ X).max() > 1e-3 * np.abs(G2).max() * np.abs(X).max() / 10
 
 
-@pytest.mark.parametrize("method", ["HO_Quadratic", "HO_CZM", "HO_Linear", "HO_Linear2", "Quadratic"])
+@pytest.mark.parametrize("method", ["HO_Quadratic", "HO_CZM", "HO_Linear", "HO_Linear2", "Quadratic", "Custom"])
 @pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (3, 2)), ("hex3d_glen_at2", (2, 2, 2))])
 def test_distribution_derivatives_by_central_differences(method, name, nel):
     prob = make_problem(name, nel=nel)
@@ -98,7 +99,7 @@ def test_distribution_derivatives_by_central_differences(method, name, nel):
         for ip in (0, prob.nip // 2):
             ph = rng.uniform(0, 0.9, n)
             D, dD = np.zeros(n), np.zeros(n * n)
-            g0 = o.L.ifo_distribution(o.h, el, ip, m, l, _p(ph), _p(D), _p(dD))
+            g0 = o.L.ifo_distribution(o.h, el, ip, m, l, _p(ph), _p(D), _p(dD), _p(CUSTOM))
             assert np.isfinite(g0)
             h = 1e-4
             fdD = np.zeros(n)
@@ -108,8 +109,8 @@ def test_distribution_derivatives_by_central_differences(method, name, nel):
                 pp[a] += h
                 pm[a] -= h
                 Dp, Dm, t = np.zeros(n), np.zeros(n), np.zeros(n * n)
-                gp = o.L.ifo_distribution(o.h, el, ip, m, l, _p(pp), _p(Dp), _p(t))
-                gm = o.L.ifo_distribution(o.h, el, ip, m, l, _p(pm), _p(Dm), _p(t))
+                gp = o.L.ifo_distribution(o.h, el, ip, m, l, _p(pp), _p(Dp), _p(t), _p(CUSTOM))
+                gm = o.L.ifo_distribution(o.h, el, ip, m, l, _p(pm), _p(Dm), _p(t), _p(CUSTOM))
                 fdD[a] = (gp - gm) / (2 * h)
                 fddD[:, a] = (Dp - Dm) / (2 * h)
             assert np.allclose(D, fdD, rtol=1e-7, atol=1e-9 * np.abs(D).max())
@@ -122,3 +123,31 @@ def test_ho_phase_assembly_runs():
     o = Oracle(prob)
     K, f = o.assemble(1)
     assert np.all(np.isfinite(K)) and np.all(np.isfinite(f))
+
+
+@pytest.mark.parametrize("coefs,method", [((2.0, 0.0, 1.0, 0.5, 1.0 / 16.0), "HO_Quadratic"),
+                                          ((2.0, 0.0, 1.0, 1.0, 0.0), "Quadratic"),
+                                          ((19.2, 9.0, 0.0, 0.0, 1.0), "HO_Linear"),
+                                          ((4.0, 2.0, 0.0, 0.5, 1.0 / 16.0), "HO_Linear2")])
+@pytest.mark.parametrize("name,nel", [("quad2d_le_at2", (3, 2)), ("hex3d_glen_at2", (2, 2, 2))])
+def test_custom_density_reduces_to_the_named_ones(name, nel, coefs, method):
+    """The Custom density 1/(w l) (a_1 phi + a_2 phi^2 + b l^2 |grad phi|^2 + c l^4 ddphi^T DC
+    ddphi) with the coefficients of a named density reproduces it (PhaseFieldUtil.cpp:185-273);
+    the generated meshes are affine, where getShapeGrads and getShapeGrads2 agree."""
+    prob = make_problem(name, nel=nel)
+    o = Oracle(prob)
+    n = prob.mesh.nodes_per_elem
+    cf = np.array(coefs)
+    rng = np.random.default_rng(8)
+    for el in (0, prob.mesh.n_elems - 1):
+        for ip in (0, prob.nip - 1):
+            ph = rng.uniform(0, 0.9, n)
+            out = []
+            for m, c in ((abi.DISTRIBUTION["Custom"], cf), (abi.DISTRIBUTION[method], CUSTOM)):
+                D, dD = np.zeros(n), np.zeros(n * n)
+                g = o.L.ifo_distribution(o.h, el, ip, m, 2e-3, _p(ph), _p(D), _p(dD), _p(c))
+                out.append((g, D, dD))
+            (g1, D1, dD1), (g2, D2, dD2) = out
+            assert np.isclose(g1, g2, rtol=1e-12)
+            assert np.allclose(D1, D2, rtol=1e-12, atol=1e-12 * np.abs(D2).max())
+            assert np.allclose(dD1, dD2, rtol=1e-12, atol=1e-12 * np.abs(dD2).max())
