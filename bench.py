@@ -268,7 +268,8 @@ COSS_KUU_FLOP = 6298.6e3
 
 
 def bench_coss(args):
-    """The Cosserat workload, FP64; CPU baseline on a 6,000-element block (1%)."""
+    """The Cosserat workload, FP64 and mixed precision; CPU baseline on a 6,000-element
+    block (1%)."""
     from ice_fem_2025a_amd.config import make_problem
     t0 = time.time()
     prob = make_problem("hex3d_coss", nel=args.nel)
@@ -289,13 +290,21 @@ def bench_coss(args):
                    "step": "ResetStep + fused K+f assembly of the u, phase and T steps"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "coss_assemble_kernel (162-dof u-step blocks, all colours)",
+                     "kernel": "coss_assemble_mfma (162-dof u-step blocks on FP64 MFMA, all colours)",
                      "flop_per_element": COSS_KUU_FLOP},
-        "kernels_ms_per_step": {"coss_material_kernel": mat_ms, "coss_assemble_kernel": elem_ms,
+        "kernels_ms_per_step": {"coss_material_kernel": mat_ms, "coss_assemble_mfma": elem_ms,
                                 "gather_kernel(phase,T)": tim["gather_ms_total"] / args.steps},
         "mean_return_map_iterations": m["kbar"],
         "wall_s": m["wall"],
     }
+    mx = measure_single(prob, "mixed", args.steps, args.warmup)
+    tmx = mx["tim"]
+    out["mixed"] = {"value": mx["value"], "ms_per_step": mx["ms_per_step"],
+                    "speedup_vs_fp64": m["ms_per_step"] / mx["ms_per_step"],
+                    "kernels_ms_per_step": {"coss_material_kernel": tmx["material_ms_total"] / args.steps,
+                                            "coss_assemble_mfma": tmx["kernel_ms_total"] / args.steps,
+                                            "gather_kernel(phase,T)": tmx["gather_ms_total"] / args.steps},
+                    "error_bound_vs_fp64_oracle": "f 1e-6, K 2e-6 (row/step-scaled), tests/test_gpu_coss.py"}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         rate, ne, secs = cpu_baseline("hex3d_coss", (20, 50, 6), threads)

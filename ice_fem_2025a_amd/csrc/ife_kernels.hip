@@ -2585,127 +2585,219 @@ struct CossRM {
     CossD D;
     double sy, ACreep, nC, hard, hard_eRef, l, plv;
 };
-__device__ void coss_getKF(const CossRM& R, const double* sol, const double* et, const double* eo, double dt, double Damage,
-                           double h0, double (&K)[20][20], double (&f)[20]) {
+// return_mapping_VEP_getKF (CosseratMaterial.cpp:671-728) at one iterate: the residual f and
+// its Jacobian in compact form.  With rows 0-17 scaled by s1, row 18 by s2 and row 19 by s3,
+// the Jacobian is
+//     [[ B - cH b dJ2 DdJ^T,  a1 dJ2,  a2 dJ2 ],
+//      [ q1 DdJ^T,            e1,      0      ],
+//      [ q2 DdJ^T,            0,       e2     ]]
+// with B = I + cH J2Mat_Coss D_coss (block diagonal: the normal 3x3 block, the coupled shear
+// rows 3-8, the curvature diagonal), DdJ = D_coss dJ2, a1 = sqrt(2)/2 rj and a2 = sqrt(3)/2 rj
+// (dF_VE and dG are both multiples of dJ2), q1 = fac Damage a1 (d f18 / d eps_e), q2 = Damage
+// a2 when the yield row is active (YieldFunction's dF equals dG at the same stress) and 0
+// otherwise, e1 = -1 and e2 the plastic-viscosity diagonal.
+struct CossJac {
+    double dJ2[18], DdJ[18];
+    double cH, b, a1, a2, q1, q2, e1, e2;
+};
+
+__device__ void coss_getF(const CossRM& R, const double* sol, const double* et, const double* eo, double dt, double Damage,
+                          double h0, double (&f)[20], CossJac& J) {
     const double sc = 1.0e6, eps = 1.0e-12, s1 = 1.0e3, s2 = -1.0e-6, s3 = -1.0e-6;
-    double S[18], dJ2[18], dFve[18], dG[18], tmp[18], ST[18], dJ2t[18];
+    double S[18], tmp[18], ST[18], dJ2t[18];
     coss_Dmul(R.D, sol, Damage, S);
-    const double J2 = coss_J2(S, dJ2);
+    const double J2 = coss_J2(S, J.dJ2);
     const double rj = 1.0 / sqrt(J2 + eps);
     const double Fve = sc * sqrt(2.0 * J2);
-    for (int i = 0; i < 18; i++) { dFve[i] = 0.5 * sqrt(2.0) * rj * dJ2[i]; dG[i] = 0.5 * sqrt(3.0) * rj * dJ2[i]; }
-    for (int i = 0; i < 18; i++) tmp[i] = et[i] - eo[i] - sol[18] * dFve[i];
+    J.a1 = 0.5 * sqrt(2.0) * rj;
+    J.a2 = 0.5 * sqrt(3.0) * rj;
+    for (int i = 0; i < 18; i++) tmp[i] = et[i] - eo[i] - sol[18] * (J.a1 * J.dJ2[i]);
     coss_Dmul(R.D, tmp, Damage, ST);
     const double J2t = coss_J2(ST, dJ2t);
     const double FTrial = sc * sqrt(3.0 * J2t) - R.sy;
     const double F = sc * sqrt(3.0 * J2) - R.sy;
     const double hOld = R.hard * R.sy * (1.0 - exp(-h0 * R.l / R.hard_eRef));
     const double h = hOld;  // e_eq = histOld[0] (explicit hardening)
-    for (int i = 0; i < 18; i++) f[i] = s1 * (sol[i] - et[i] + eo[i] + sol[18] * dFve[i] + sol[19] * dG[i]);
-    // strain block: H D with H = cH (J2Mat - b dJ2 dJ2^T): (J2Mat D)_ij - b dJ2_i (D dJ2)_j
-    const double cH = Damage * (sol[18] + sol[19]) * 0.5 / sc * sqrt(3.0) * rj, b = 0.5 / (J2 + eps);
-    double DdJ[18];
-    coss_Dmul(R.D, dJ2, 1.0, DdJ);
-#pragma unroll 1
-    for (int i = 0; i < 18; i++)
-#pragma unroll 1
-        for (int j = 0; j < 18; j++) {
-            double jd = 0;
-            for (int k = 0; k < 18; k++) jd += coss_J2ij(i, k) * coss_Dij(R.D, k, j);
-            K[i][j] = s1 * ((i == j ? 1.0 : 0.0) + cH * (jd - b * dJ2[i] * DdJ[j]));
-        }
-    for (int i = 0; i < 18; i++) { K[i][18] = s1 * dFve[i]; K[i][19] = s1 * dG[i]; }
+    for (int i = 0; i < 18; i++) f[i] = s1 * (sol[i] - et[i] + eo[i] + sol[18] * (J.a1 * J.dJ2[i]) + sol[19] * (J.a2 * J.dJ2[i]));
+    J.cH = Damage * (sol[18] + sol[19]) * 0.5 / sc * sqrt(3.0) * rj;
+    J.b = 0.5 / (J2 + eps);
+    coss_Dmul(R.D, J.dJ2, 1.0, J.DdJ);
     const double PreFac = R.ACreep * dt * pow(sc, R.nC);
     f[18] = s2 * (PreFac * pow(Fve / sc, R.nC) - sol[18]);
     const double fac = R.nC * PreFac / sc * pow(Fve / sc, R.nC - 1.0);
-    double DdF[18];
-    coss_Dmul(R.D, dFve, Damage, DdF);
-    for (int j = 0; j < 18; j++) K[18][j] = s2 * fac * DdF[j];
-    K[18][18] = -s2;
-    K[18][19] = 0.0;
+    J.q1 = fac * Damage * J.a1;
+    J.e1 = -1.0;
     if (FTrial - hOld < 0.0) {
         f[19] = -s3 * (R.l * R.plv + 1.0e-6) / dt * sol[19];
-        for (int j = 0; j < 19; j++) K[19][j] = 0.0;
-        K[19][19] = -s3 * (R.l * R.plv + 1.0e-6) / dt;
+        J.q2 = 0.0;
+        J.e2 = -(R.l * R.plv + 1.0e-6) / dt;
     } else {
         f[19] = s3 * (F - h - R.l * R.plv / dt * sol[19]);
-        double DdFp[18];
-        coss_Dmul(R.D, dG, Damage, DdFp);  // YieldFunction's dF equals dG at this stress
-        for (int j = 0; j < 18; j++) K[19][j] = s3 * DdFp[j];
-        K[19][18] = 0.0;
-        K[19][19] = s3 * (-R.l * R.plv / dt);
+        J.q2 = Damage * J.a2;
+        J.e2 = -R.l * R.plv / dt;
     }
 }
 
-// dense 20x20 solve with partial pivoting, NR right-hand sides (loops kept rolled: the
-// matrices live in private memory); multipliers by division, updates and substitutions in
-// the order of an LU factorisation followed by forward and back substitution
-template <int NR>
-__device__ void lu_solve20(double (&A)[20][20], double (&B)[20][NR]) {
-#pragma unroll 1
-    for (int k = 0; k < 20; k++) {
-        int piv = k;
-        double amax = fabs(A[k][k]);
-        for (int r = k + 1; r < 20; r++)
-            if (fabs(A[r][k]) > amax) { amax = fabs(A[r][k]); piv = r; }
-        if (piv != k) {
-            for (int c = 0; c < 20; c++) { const double t = A[k][c]; A[k][c] = A[piv][c]; A[piv][c] = t; }
-            for (int c = 0; c < NR; c++) { const double t = B[k][c]; B[k][c] = B[piv][c]; B[piv][c] = t; }
+// Inverses of the diagonal blocks of B = I + cH J2Mat_Coss D_coss: rows/columns 0-2, 3-8
+// and the curvature scalar (Gauss-Jordan with partial pivoting, unrolled in registers)
+struct CossBinv {
+    double n[3][3], s[6][6], c;
+};
+
+template <int M>
+__device__ __forceinline__ void gj_inverse_reg(double (&A)[M][M], double (&Ai)[M][M]) {
+#pragma unroll
+    for (int i = 0; i < M; i++)
+#pragma unroll
+        for (int j = 0; j < M; j++) Ai[i][j] = i == j ? 1.0 : 0.0;
+#pragma unroll
+    for (int c = 0; c < M; c++) {
+        // partial pivoting by conditional swaps (indices stay compile-time)
+#pragma unroll
+        for (int r = c + 1; r < M; r++) {
+            const bool sw = fabs(A[r][c]) > fabs(A[c][c]);
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                const double t = A[c][j], u = Ai[c][j];
+                A[c][j] = sw ? A[r][j] : t; A[r][j] = sw ? t : A[r][j];
+                Ai[c][j] = sw ? Ai[r][j] : u; Ai[r][j] = sw ? u : Ai[r][j];
+            }
         }
-#pragma unroll 1
-        for (int r = k + 1; r < 20; r++) {
-            const double fr = A[r][k] / A[k][k];
-            for (int c = k + 1; c < 20; c++) A[r][c] -= fr * A[k][c];
-            for (int c = 0; c < NR; c++) B[r][c] -= fr * B[k][c];
+        const double id = 1.0 / A[c][c];
+#pragma unroll
+        for (int j = 0; j < M; j++) { A[c][j] *= id; Ai[c][j] *= id; }
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            if (r == c) continue;
+            const double fr = A[r][c];
+#pragma unroll
+            for (int j = 0; j < M; j++) { A[r][j] -= fr * A[c][j]; Ai[r][j] -= fr * Ai[c][j]; }
         }
     }
-#pragma unroll 1
-    for (int k = 19; k >= 0; k--)
-        for (int c = 0; c < NR; c++) {
-            double t = B[k][c];
-            for (int j = k + 1; j < 20; j++) t -= A[k][j] * B[j][c];
-            B[k][c] = t / A[k][k];
-        }
 }
+
+__device__ void coss_binv(const CossD& D, double cH, CossBinv& Bi) {
+    double An[3][3], As[6][6];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double t = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) t += coss_J2ij(i, k) * coss_Dij(D, k, j);
+            An[i][j] = (i == j ? 1.0 : 0.0) + cH * t;
+        }
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            double t = 0;
+#pragma unroll
+            for (int k = 0; k < 6; k++) t += coss_J2ij(3 + i, 3 + k) * coss_Dij(D, 3 + k, 3 + j);
+            As[i][j] = (i == j ? 1.0 : 0.0) + cH * t;
+        }
+    gj_inverse_reg<3>(An, Bi.n);
+    gj_inverse_reg<6>(As, Bi.s);
+    Bi.c = 1.0 / (1.0 + cH * coss_J2ij(9, 9) * coss_Dij(D, 9, 9));
+}
+
+// o = B^-1 v (TR: o = B^-T v)
+template <bool TR>
+__device__ __forceinline__ void coss_bsolve(const CossBinv& Bi, const double* v, double* o) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) t += (TR ? Bi.n[j][i] : Bi.n[i][j]) * v[j];
+        o[i] = t;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) t += (TR ? Bi.s[j][i] : Bi.s[i][j]) * v[3 + j];
+        o[3 + i] = t;
+    }
+#pragma unroll
+    for (int i = 9; i < 18; i++) o[i] = Bi.c * v[i];
+}
+
+// Newton step dsol = -K^-1 f by block elimination: the two scalar rows give x18 and x19 in
+// terms of the strain block y, whose Schur complement B - beta dJ2 DdJ^T is solved by
+// Sherman-Morrison on the block-diagonal B
+__device__ void coss_newton_step(const CossRM& R, const CossJac& J, const double (&f)[20], double (&dsol)[20], CossBinv& Bi,
+                                 double& beta, double& den, double (&w)[18]) {
+    const double s1 = 1.0e3, s2 = -1.0e-6, s3 = -1.0e-6;
+    coss_binv(R.D, J.cH, Bi);
+    beta = J.cH * J.b + J.a1 * J.q1 / J.e1 + J.a2 * J.q2 / J.e2;
+    const double f18 = f[18] / s2, f19 = f[19] / s3;
+    const double gc = J.a1 * f18 / J.e1 + J.a2 * f19 / J.e2;
+    double g[18], z[18];
+    for (int i = 0; i < 18; i++) g[i] = f[i] / s1 - gc * J.dJ2[i];
+    coss_bsolve<false>(Bi, J.dJ2, w);
+    coss_bsolve<false>(Bi, g, z);
+    double vw = 0, vz = 0;
+    for (int i = 0; i < 18; i++) { vw += J.DdJ[i] * w[i]; vz += J.DdJ[i] * z[i]; }
+    den = 1.0 - beta * vw;
+    const double t = beta * vz / den;
+    double vy = 0;
+    for (int i = 0; i < 18; i++) {
+        const double y = z[i] + t * w[i];
+        dsol[i] = -y;
+        vy += J.DdJ[i] * y;
+    }
+    dsol[18] = -(f18 - J.q1 * vy) / J.e1;
+    dsol[19] = -(f19 - J.q2 * vy) / J.e2;
+}
+
+// DMat = scale1 (K^-1)(0:18,0:18) of the map in compact form: the inverse of the strain
+// block's Schur complement, B^-1 + c w zt^T with w = B^-1 dJ2, zt = B^-T DdJ and
+// c = beta / (1 - beta DdJ^T w); the identity when the map did not run
+struct CossDM {
+    CossBinv Bi;
+    double c, w[18], zt[18];
+    bool id;
+    __device__ __forceinline__ double at(int i, int j) const {
+        if (id) return i == j ? 1.0 : 0.0;
+        double bij = 0.0;
+        if (i < 3 && j < 3) bij = Bi.n[i][j];
+        else if (i >= 3 && i < 9 && j >= 3 && j < 9) bij = Bi.s[i - 3][j - 3];
+        else if (i >= 9 && i == j) bij = Bi.c;
+        return bij + c * w[i] * zt[j];
+    }
+};
 
 // CosseratMaterial::return_mapping_VEP (:508-575) + _updateStrain (:730-769).  Returns the
 // iteration count; DM = scale1 (K^-1)(0:18,0:18) at the final iterate.
 __device__ int coss_return_map(const CossRM& R, const RMStop& stop, double dt, double Damage, const double* et,
-                               const double* eo, double* epl, double hist[4], const double h0[4], double (&DM)[18][18],
+                               const double* eo, double* epl, double hist[4], const double h0[4], CossDM& DM,
                                double& diss, double& ddiss, int& mx) {
     double sol[20], f[20], fOld[20], dsol[20];
-    double K[20][20];
+    CossJac J;
     for (int i = 0; i < 18; i++) sol[i] = et[i] - epl[i];
     sol[18] = hist[1];
     sol[19] = hist[2];
-    coss_getKF(R, sol, et, eo, dt, Damage, h0[0], K, f);
+    coss_getF(R, sol, et, eo, dt, Damage, h0[0], f, J);
     double e0 = 0;
     for (int i = 0; i < 20; i++) e0 += f[i] * f[i];
     e0 = fmax(e0, 1e-12);
     int it = 0;
     mx = 0;
-    double Ki[20][20];
+    CossBinv Bi;
+    double beta, den, w[18];
 #pragma unroll 1
     while (true) {
         it++;
-        // dsol = -K^-1 f with the inverse formed first, as the reference does (:531)
-        for (int i = 0; i < 20; i++)
-            for (int j = 0; j < 20; j++) Ki[i][j] = i == j ? 1.0 : 0.0;
-        lu_solve20<20>(K, Ki);
-        for (int i = 0; i < 20; i++) {
-            double t = 0;
-            for (int j = 0; j < 20; j++) t += Ki[i][j] * f[j];
-            dsol[i] = -t;
-        }
+        coss_newton_step(R, J, f, dsol, Bi, beta, den, w);
         for (int i = 0; i < 20; i++) { sol[i] += dsol[i]; fOld[i] = f[i]; }
-        coss_getKF(R, sol, et, eo, dt, Damage, h0[0], K, f);
+        coss_getF(R, sol, et, eo, dt, Damage, h0[0], f, J);
         double ls1 = 0, ls2 = 0;
         for (int i = 0; i < 20; i++) { ls1 += f[i] * dsol[i]; ls2 += fOld[i] * dsol[i]; }
         double ls;
         if ((ls1 - ls2) == 0.0) ls = 1.0;
         else ls = fmin(fmax(-(ls2) / (ls1 - ls2), 0.01), 1.0);
         for (int i = 0; i < 20; i++) sol[i] = sol[i] - (1.0 - ls) * dsol[i];
-        coss_getKF(R, sol, et, eo, dt, Damage, h0[0], K, f);
+        coss_getF(R, sol, et, eo, dt, Damage, h0[0], f, J);
         double e = 0;
         for (int i = 0; i < 20; i++) e += f[i] * f[i];
         if (e < stop.eabs || e / e0 < stop.erel) break;
@@ -2730,11 +2822,15 @@ __device__ int coss_return_map(const CossRM& R, const RMStop& stop, double dt, d
     hist[3] = h0[3] + sqrt(sol[18] * sol[18]);
     diss = dv + dp;
     ddiss = dp;
-    for (int i = 0; i < 20; i++)
-        for (int j = 0; j < 20; j++) Ki[i][j] = i == j ? 1.0 : 0.0;
-    lu_solve20<20>(K, Ki);
-    for (int i = 0; i < 18; i++)
-        for (int j = 0; j < 18; j++) DM[i][j] = 1.0e3 * Ki[i][j];
+    // DM = (scale1 / s1) Schur^-1 with scale1 = s1 = 1e3, at the final iterate's Jacobian
+    {
+        double fz[20];
+        for (int i = 0; i < 20; i++) fz[i] = 0.0;
+        coss_newton_step(R, J, fz, dsol, DM.Bi, beta, den, DM.w);  // B^-1, beta, den and w at J
+        coss_bsolve<true>(DM.Bi, J.DdJ, DM.zt);
+        DM.c = beta / den;
+        DM.id = false;
+    }
     return it;
 }
 
@@ -3111,9 +3207,9 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
             damage(mc.GravityDamage, phi, mc.k0, mc.p, DamG, d1, d2);
             double epo[18];
             for (int r = 0; r < 18; r++) epo[r] = p.PSOld[r * S + gi];
-            double DM[18][18];
+            CossDM DM;
+            DM.id = true;
             double diss = 0, ddiss = 0;
-            bool elastic = true;
             const int rh = mc.Rheology;
             if ((rh == IFE_RHEO_VEP || rh == IFE_RHEO_DVEP) && !(phi > mc.PhiViscLim)) {
                 double dmg = 1.0;
@@ -3129,7 +3225,6 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
                 my_calls++;
                 my_iters += it;
                 my_maxit += mx;
-                elastic = false;
             } else {
                 for (int r = 0; r < 18; r++) epl[r] = 1.0 * epo[r];
             }
@@ -3155,7 +3250,7 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
                     for (int j = 0; j < 18; j++) {
                         double t = 0;
                         for (int k = 0; k < 18; k++)
-                            t += (Dam * DDm[i][k] + DUm[i][k]) * (elastic ? (k == j ? 1.0 : 0.0) : DM[k][j]);
+                            t += (Dam * DDm[i][k] + DUm[i][k]) * DM.at(k, j);
                         cbo[(COSS_C + i * 18 + j) * NIP] = w * t;
                     }
             } else {
@@ -3166,11 +3261,13 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
                     cbo[(COSS_SIG + r) * NIP] = v;
                 }
                 // w (Dam D_coss) DPlast
-#pragma unroll 1
+#pragma unroll
                 for (int j = 0; j < 18; j++) {
                     double col[18], o[18];
-                    for (int k = 0; k < 18; k++) col[k] = elastic ? (k == j ? 1.0 : 0.0) : DM[k][j];
+#pragma unroll
+                    for (int k = 0; k < 18; k++) col[k] = DM.at(k, j);
                     coss_Dmul(D, col, Dam, o);
+#pragma unroll
                     for (int i = 0; i < 18; i++) cbo[(COSS_C + i * 18 + j) * NIP] = w * o[i];
                 }
             }
@@ -3302,19 +3399,27 @@ struct CossRows {
     static __device__ __forceinline__ int cof(int f, int x) { return DIM == 3 ? coss_cof[f][x] : coss2_cof[f][x]; }
 };
 
-// Cosserat u-step element kernel: one workgroup per element of one colour (no two elements
-// of a colour share a node, so the element adds into its rows of the zeroed CSR values and
-// f without atomics).  K_ab(alpha a, beta b) = sum_ip sum_kl Phi_k(a) M^ab_kl Phi_l(b) with
-// Phi = (N, G_x, G_y[, G_z]) and M^ab = P_a^T (w Dam D_coss DPlast) P_b, P_a the field's
-// B-matrix rows; one field pair at a time, its (1+DIM)^2 blocks for all IPs staged in LDS.
-template <int DIM>
-__global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
+// Cosserat u-step element kernel on MFMA.  K^{ab}[i][j] = sum_ip sum_kl Phi_k(ip,i) M^{ab}_kl(ip)
+// Phi_l(ip,j) is a GEMM over (ip, k): one 16x16x4 MFMA k-step per IP, its four k the basis
+// functions (N, G_x, G_y, G_z; 2D pads the fourth with zeros).  A = Phi from LDS, B formed
+// per lane as sum_l M^{ab}_kl(ip) Phi_l(ip, j) with M^{ab} of the wavefront's current field
+// pair staged in LDS.  Each wavefront takes every fourth field pair; the element of one colour
+// adds its blocks into the zeroed CSR values (no atomics), the lumped diagonal terms with
+// them.  T: tile type (double: FP64; float: mixed mode, FP32 MFMA, FP64 accumulation in the
+// CSR).  Then F_U = sum_ip B^T (w sigma) and the lumped inertia, damping and gravity terms
+// (:454-527), per (field, node).
+template <int DIM, typename T>
+__global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     using CR = CossRows<DIM>;
-    constexpr int N = DIM == 2 ? 9 : 27, NIP = N, NP = DIM == 2 ? 9 : 28, NB = 1 + DIM, NFU = DIM == 2 ? 3 : 6;
-    __shared__ double Phi[NIP][NB][NP];
-    __shared__ double Mb[NIP][NB * NB];
-    __shared__ double Xs[N][4];
+    typedef typename Mfma<T>::V V;
+    constexpr int N = DIM == 2 ? 9 : 27, NIP = N, NP = DIM == 2 ? 16 : 32, MT = NP / 16, NB = 1 + DIM,
+                  NFU = DIM == 2 ? 3 : 6, NW = NT / 64;
+    __shared__ T Phi[NIP][4][NP];
+    __shared__ double Mb[NW][NIP][16];
+    __shared__ double Phd[NIP][NB][N];  // FP64 basis for the residual and lumped terms
+    __shared__ double WI[N], WG[N];
     __shared__ double b1s[DIM][3][3], db1s[DIM][3][3];
+    __shared__ double Wn[N];
     __shared__ int32_t rk[N];
     __shared__ int64_t rp0[N], rdeg[N];
     const int tid = threadIdx.x;
@@ -3324,8 +3429,7 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
     const MatConst& mc = p.mc;
     if (tid < N) {
         const int v = p.enode[el * N + tid], r = p.erank[el * N + tid];
-        const double* nd = p.nodes + (int64_t)v * 4;
-        for (int c = 0; c < 4; c++) Xs[tid][c] = nd[c];
+        Wn[tid] = DIM == 3 ? p.nodes[(int64_t)v * 4 + 3] : 1.0;
         rk[tid] = r;
         rp0[tid] = p.nbr_ptr[r];
         rdeg[tid] = p.nbr_ptr[r + 1] - p.nbr_ptr[r];
@@ -3341,95 +3445,134 @@ __global__ __launch_bounds__(NT) void coss_assemble_kernel(KParams p) {
     __syncthreads();
     for (int i = tid; i < NIP * NP; i += NT) {
         const int ip = i / NP, a = i % NP;
-        if (a >= N) { for (int k = 0; k < NB; k++) Phi[ip][k][a] = 0.0; continue; }
-        const int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9, ax = a % 3, ay = (a / 3) % 3, az = a / 9;
-        const double bx = b1s[0][qx][ax], by = b1s[1][qy][ay], bz = DIM == 3 ? b1s[DIM - 1][qz][az] : 1.0;
-        const double gx = db1s[0][qx][ax], gy = db1s[1][qy][ay], gz = DIM == 3 ? db1s[DIM - 1][qz][az] : 0.0;
-        const double cbv = bx * by * bz, gb[3] = {gx * by * bz, bx * gy * bz, bx * by * gz};
-        const double Wa = DIM == 3 ? Xs[a][3] : 1.0;  // 2D: N = CB, G = JI GB (IPB_WF = 1, IPB_DW = 0)
-        Phi[ip][0][a] = Wa * cbv * ipc[IPB_WF * NIP + ip];
-        for (int c = 0; c < DIM; c++) {
-            double s = 0;
-            for (int d = 0; d < DIM; d++) s += ipc[(IPB_JI + c * DIM + d) * NIP + ip] * gb[d];
-            Phi[ip][1 + c][a] = Wa * (s - cbv * ipc[(IPB_DW + c) * NIP + ip]);
+        double ph[4] = {0.0, 0.0, 0.0, 0.0};
+        if (a < N) {
+            const int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9, ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+            const double bx = b1s[0][qx][ax], by = b1s[1][qy][ay], bz = DIM == 3 ? b1s[DIM - 1][qz][az] : 1.0;
+            const double gx = db1s[0][qx][ax], gy = db1s[1][qy][ay], gz = DIM == 3 ? db1s[DIM - 1][qz][az] : 0.0;
+            const double cbv = bx * by * bz, gb[3] = {gx * by * bz, bx * gy * bz, bx * by * gz};
+            const double Wa = Wn[a];  // 2D: N = CB, G = JI GB (IPB_WF = 1, IPB_DW = 0)
+            ph[0] = Wa * cbv * ipc[IPB_WF * NIP + ip];
+            for (int c = 0; c < DIM; c++) {
+                double s = 0;
+                for (int d = 0; d < DIM; d++) s += ipc[(IPB_JI + c * DIM + d) * NIP + ip] * gb[d];
+                ph[1 + c] = Wa * (s - cbv * ipc[(IPB_DW + c) * NIP + ip]);
+            }
+            for (int k = 0; k < NB; k++) Phd[ip][k][a] = ph[k];
         }
+        for (int k = 0; k < 4; k++) Phi[ip][k][a] = (T)ph[k];
     }
     __syncthreads();
-    const double lc = mc.l_coss;
+    if (tid < N) {  // lumped inertia and gravity weights (:460-462)
+        double wi = 0, wg = 0;
+        for (int ip = 0; ip < NIP; ip++) {
+            wi += cb[COSS_WI * NIP + ip] * Phd[ip][0][tid];
+            wg += cb[COSS_WG * NIP + ip] * Phd[ip][0][tid];
+        }
+        WI[tid] = wi;
+        WG[tid] = wg;
+    }
+    __syncthreads();
+    const double lc = mc.l_coss, l2 = lc * lc;
     auto cof = [&](int c) { return c == 0 ? 1.0 : (c == 1 ? -1.0 : lc); };
+    const int wv = tid >> 6, lane = tid & 63, lk = lane >> 4, l16 = lane & 15;
     double bad = 0.0;
 #pragma unroll 1
-    for (int blk = 0; blk < NFU * NFU; blk++) {
+    for (int blk = wv; blk < NFU * NFU; blk += NW) {
         const int al = blk / NFU, be = blk % NFU;
-        for (int i = tid; i < NIP * NB * NB; i += NT) {
-            const int ip = i / (NB * NB), k = (i / NB) % NB, l = i % NB;
-            double m = 0;
+        // M^{ab}_kl(ip) = sum over the B rows of the pair (basis k of field a, l of field b):
+        // one lane per IP, so that each coefficient C[r][c] is read for all IPs at once
+        if (lane < NIP) {
+            const int ip = lane;
+            double m[4][4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int l = 0; l < 4; l++) m[k][l] = 0.0;
             for (int x = 0; x < CR::nrow(al); x++) {
-                if (CR::bas(al, x) != k) continue;
+                const int bk = CR::bas(al, x), ra = CR::row(al, x);
+                const double ca = cof(CR::cof(al, x));
                 for (int y = 0; y < CR::nrow(be); y++) {
-                    if (CR::bas(be, y) != l) continue;
-                    m += cof(CR::cof(al, x)) * cof(CR::cof(be, y)) * cb[(COSS_C + CR::row(al, x) * 18 + CR::row(be, y)) * NIP + ip];
+                    const int bl = CR::bas(be, y);
+                    const double v = ca * cof(CR::cof(be, y)) * cb[(COSS_C + ra * 18 + CR::row(be, y)) * NIP + ip];
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+#pragma unroll
+                        for (int l = 0; l < 4; l++) m[k][l] += (bk == k && bl == l) ? v : 0.0;
                 }
             }
-            Mb[ip][i % (NB * NB)] = m;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int l = 0; l < 4; l++) Mb[wv][ip][k * 4 + l] = m[k][l];
         }
-        __syncthreads();
-        for (int e = tid; e < N * N; e += NT) {
-            const int a = e / N, b = e % N;
-            double acc = 0;
-#pragma unroll 3
-            for (int ip = 0; ip < NIP; ip++) {
-                double pb[NB];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        V acc[MT][MT];
 #pragma unroll
-                for (int l = 0; l < NB; l++) pb[l] = Phi[ip][l][b];
+        for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-                for (int k = 0; k < NB; k++) {
-                    double t = 0;
+            for (int nt = 0; nt < MT; nt++) acc[mt][nt] = V{0, 0, 0, 0};
+#pragma unroll 1
+        for (int ip = 0; ip < NIP; ip++) {
+            T a[MT], bq[MT];
+            const double m0 = Mb[wv][ip][lk * 4], m1 = Mb[wv][ip][lk * 4 + 1], m2 = Mb[wv][ip][lk * 4 + 2],
+                         m3 = Mb[wv][ip][lk * 4 + 3];
 #pragma unroll
-                    for (int l = 0; l < NB; l++) t += Mb[ip][k * NB + l] * pb[l];
-                    acc += Phi[ip][k][a] * t;
+            for (int mt = 0; mt < MT; mt++) a[mt] = Phi[ip][lk][mt * 16 + l16];
+#pragma unroll
+            for (int nt = 0; nt < MT; nt++) {
+                const int j = nt * 16 + l16;
+                bq[nt] = (T)m0 * Phi[ip][0][j] + (T)m1 * Phi[ip][1][j] + (T)m2 * Phi[ip][2][j] + (T)m3 * Phi[ip][3][j];
+            }
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                for (int nt = 0; nt < MT; nt++) acc[mt][nt] = Mfma<T>::op(a[mt], bq[nt], acc[mt][nt]);
+        }
+        // lumped inertia / damping on the diagonal (:474-520): oz takes rhoCoss (3D: ox, oy
+        // take Density in K)
+        const bool oz = al == NFU - 1, rot = DIM == 3 ? al >= 3 : al == 2;
+        double kdiag = 0.0;
+        if (al == be) {
+            if (mc.Inertia) kdiag += (oz ? 2 * l2 * mc.Density / (1 + mc.Poisson) : mc.Density) * p.ddu_dt;
+            if (mc.Damping > 0.0) kdiag += (rot ? 2 * l2 * mc.Damping : mc.Damping) * p.du_dt;
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int nt = 0; nt < MT; nt++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = mt * 16 + Mfma<T>::row(lk, v), j = nt * 16 + l16;
+                    if (i >= N || j >= N) continue;
+                    double val = (double)acc[mt][nt][v];
+                    if (i == j) val += WI[i] * kdiag;
+                    bad += val - val;
+                    double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * rp0[i] +
+                                (int64_t)p.fpos_u[be] * rdeg[i] + p.loc[(el * N + i) * N + j];
+                    *K += val;
                 }
-            }
-            if (al == be && a == b) {  // lumped inertia / damping on the diagonal (:474-520)
-                double WI = 0;
-                for (int ip = 0; ip < NIP; ip++) WI += cb[COSS_WI * NIP + ip] * Phi[ip][0][a];
-                const double l2 = lc * lc;
-                // inertia: the micro-rotation oz takes rhoCoss (3D: ox, oy take Density, :496-507)
-                const bool oz = al == NFU - 1;
-                const bool rot = DIM == 3 ? al >= 3 : al == 2;
-                if (mc.Inertia) acc += WI * (oz ? 2 * l2 * mc.Density / (1 + mc.Poisson) : mc.Density) * p.ddu_dt;
-                if (mc.Damping > 0.0) acc += WI * (rot ? 2 * l2 * mc.Damping : mc.Damping) * p.du_dt;
-            }
-            bad += acc - acc;
-            double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * rp0[a] +
-                        (int64_t)p.fpos_u[be] * rdeg[a] + p.loc[(el * N + a) * N + b];
-            *K += acc;
-        }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();  // Mb of this wavefront is rewritten by its next pair
     }
     // F_U (:454-455) and the lumped nodal terms (:474-527)
     for (int t = tid; t < NFU * N; t += NT) {
         const int al = t / N, a = t % N;
-        double acc = 0;
+        double accf = 0;
         for (int ip = 0; ip < NIP; ip++)
             for (int x = 0; x < CR::nrow(al); x++)
-                acc += cof(CR::cof(al, x)) * cb[(COSS_SIG + CR::row(al, x)) * NIP + ip] * Phi[ip][CR::bas(al, x)][a];
-        double WI = 0, WG = 0;
-        for (int ip = 0; ip < NIP; ip++) {
-            WI += cb[COSS_WI * NIP + ip] * Phi[ip][0][a];
-            WG += cb[COSS_WG * NIP + ip] * Phi[ip][0][a];
-        }
+                accf += cof(CR::cof(al, x)) * cb[(COSS_SIG + CR::row(al, x)) * NIP + ip] * Phd[ip][CR::bas(al, x)][a];
         const int64_t dof = (int64_t)p.fpos_u[al] * p.Ng + rk[a];
-        const double l2 = lc * lc;
         const bool rot = DIM == 3 ? al >= 3 : al == 2;
         if (mc.Inertia) {
             const double rhoC = 2 * l2 * mc.Density / (1 + mc.Poisson);
-            acc += WI * (rot ? rhoC : mc.Density) * p.su.ddx[dof];
+            accf += WI[a] * (rot ? rhoC : mc.Density) * p.su.ddx[dof];
         }
-        if (mc.Damping > 0.0) acc += WI * (rot ? 2 * l2 * mc.Damping : mc.Damping) * p.su.dx[dof];
-        if (al < DIM) acc += -WG * mc.Density * mc.g[al];
-        bad += acc - acc;
-        p.su.f[dof] += acc;
+        if (mc.Damping > 0.0) accf += WI[a] * (rot ? 2 * l2 * mc.Damping : mc.Damping) * p.su.dx[dof];
+        if (al < DIM) accf += -WG[a] * mc.Density * mc.g[al];
+        bad += accf - accf;
+        p.su.f[dof] += accf;
     }
     if (bad != 0.0) atomicOr(&p.st->nonfinite, 1u);
 }
@@ -3498,8 +3641,14 @@ hipError_t launch_coss_material(const KParams& p, int dim, hipStream_t s) {
 
 hipError_t launch_coss_assemble(const KParams& p, int dim, hipStream_t s) {
     if (p.nlist <= 0) return hipSuccess;
-    if (dim == 2) hipLaunchKernelGGL(coss_assemble_kernel<2>, dim3((unsigned)p.nlist), dim3(NT), 0, s, p);
-    else hipLaunchKernelGGL(coss_assemble_kernel<3>, dim3((unsigned)p.nlist), dim3(NT), 0, s, p);
+    const dim3 g((unsigned)p.nlist), b(NT);
+    if (p.mixed) {
+        if (dim == 2) hipLaunchKernelGGL((coss_assemble_mfma<2, float>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((coss_assemble_mfma<3, float>), g, b, 0, s, p);
+    } else {
+        if (dim == 2) hipLaunchKernelGGL((coss_assemble_mfma<2, double>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((coss_assemble_mfma<3, double>), g, b, 0, s, p);
+    }
     return hipGetLastError();
 }
 

@@ -146,3 +146,30 @@ def test_cosserat_assemble_all_matches_stepwise(require_gpu):
         Ka, fa = a.Kf(s)
         Kb, fb = b.Kf(s)
         assert np.array_equal(Ka, Kb) and np.array_equal(fa, fb), f"step {s}"
+
+
+@pytest.mark.parametrize("name,nel", [("hex3d_coss", (3, 3, 2)), ("quad2d_coss", (6, 5))])
+def test_cosserat_mixed_mode_against_oracle(require_gpu, name, nel):
+    """Mixed mode (FP32-MFMA element contractions, FP64 material update and CSR accumulation)
+    against the FP64 oracle with the converged map: the residuals f of every step within the
+    mixed-mode bound (1e-6, step scaled).  The tangents carry the FP32 accumulation of the
+    element contractions: the phase-step K_phiphi (l = 1.5 mm on 2.5 mm elements, so the
+    G^T G and N N^T parts cancel more than in the solid workloads) measures 1.1e-6, held to
+    RTOL_MIXED_K_COSS; the u-step K measures 4.8e-7."""
+    from parity_util import RTOL_MIXED
+    RTOL_MIXED_K_COSS = 2e-6
+    _say(f"[coss mixed] {name} nel={nel}")
+    prob = make_problem(name, nel=nel)
+    o, e = Oracle(prob), Physics(prob, precision="mixed")
+    for k, v in TIGHT:
+        o.set_param(0, k, v)
+        e.set_param(0, k, v)
+    for s in range(o.nsteps):
+        e.assemble(s)
+        Ke, fe = e.Kf(s)
+        Ko, fo = o.assemble(s)
+        rp, _ = o.pattern(s)
+        ek, ef = csr_err(Ke, Ko, rp), vec_err(fe, fo)
+        _say(f"  step {s}: K err {ek:.2e}, f err {ef:.2e}")
+        assert ek <= RTOL_MIXED_K_COSS and ef <= RTOL_MIXED, f"step {s}"
+    assert e.status()["nonfinite"] == 0
