@@ -3413,13 +3413,17 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     using CR = CossRows<DIM>;
     typedef typename Mfma<T>::V V;
     constexpr int N = DIM == 2 ? 9 : 27, NIP = N, NP = DIM == 2 ? 16 : 32, MT = NP / 16, NB = 1 + DIM,
-                  NFU = DIM == 2 ? 3 : 6, NW = NT / 64;
+                  NFU = DIM == 2 ? 3 : 6, NW = NT / 64, NG = DIM * DIM + DIM + 1;
+    constexpr bool F64 = sizeof(T) == 8;
     __shared__ T Phi[NIP][4][NP];
+    __shared__ double Phd[F64 ? 1 : NIP][NB][N];  // FP64 basis for the residual terms (mixed mode)
     __shared__ double Mb[NW][NIP][16];
-    __shared__ double Phd[NIP][NB][N];  // FP64 basis for the residual and lumped terms
-    __shared__ double WI[N], WG[N];
+    __shared__ double geo[NG][NIP];               // J^-1 / wFun, J^-1 dW / wFun^2, 1 / wFun
+    __shared__ double sig[18][NIP];               // w sigma
+    __shared__ double WI[N], WG[N], wi_ip[NIP], wg_ip[NIP];
     __shared__ double b1s[DIM][3][3], db1s[DIM][3][3];
     __shared__ double Wn[N];
+    __shared__ uint8_t lc_[N][N];                 // loc: slot of node j in node i's row
     __shared__ int32_t rk[N];
     __shared__ int64_t rp0[N], rdeg[N];
     const int tid = threadIdx.x;
@@ -3427,6 +3431,8 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     const double* const ipc = p.ipb + el * IPB_NF * NIP;
     const double* const cb = p.cossb + el * COSS_NF * NIP;
     const MatConst& mc = p.mc;
+    auto phd = [&](int ip, int k, int a) -> double { return F64 ? (double)Phi[ip][k][a] : Phd[F64 ? 0 : ip][k][a]; };
+    // ---- stage the element's data: nodes, 1D factors, per-IP geometry, stresses, weights, slots
     if (tid < N) {
         const int v = p.enode[el * N + tid], r = p.erank[el * N + tid];
         Wn[tid] = DIM == 3 ? p.nodes[(int64_t)v * 4 + 3] : 1.0;
@@ -3442,6 +3448,17 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
         b1s[d][q][a] = s;
         db1s[d][q][a] = sd;
     }
+    for (int t = tid; t < NG * NIP; t += NT) {  // the geometry fields IPB_JI, IPB_DW, IPB_WF
+        const int g = t / NIP, ip = t % NIP;
+        const int fld = g < DIM * DIM ? IPB_JI + g : (g < DIM * DIM + DIM ? IPB_DW + g - DIM * DIM : IPB_WF);
+        geo[g][ip] = ipc[fld * NIP + ip];
+    }
+    for (int t = tid; t < 18 * NIP; t += NT) sig[t / NIP][t % NIP] = cb[(COSS_SIG + t / NIP) * NIP + t % NIP];
+    if (tid < NIP) {
+        wi_ip[tid] = cb[COSS_WI * NIP + tid];
+        wg_ip[tid] = cb[COSS_WG * NIP + tid];
+    }
+    for (int t = tid; t < N * N; t += NT) lc_[t / N][t % N] = p.loc[el * N * N + t];
     __syncthreads();
     for (int i = tid; i < NIP * NP; i += NT) {
         const int ip = i / NP, a = i % NP;
@@ -3452,13 +3469,14 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
             const double gx = db1s[0][qx][ax], gy = db1s[1][qy][ay], gz = DIM == 3 ? db1s[DIM - 1][qz][az] : 0.0;
             const double cbv = bx * by * bz, gb[3] = {gx * by * bz, bx * gy * bz, bx * by * gz};
             const double Wa = Wn[a];  // 2D: N = CB, G = JI GB (IPB_WF = 1, IPB_DW = 0)
-            ph[0] = Wa * cbv * ipc[IPB_WF * NIP + ip];
+            ph[0] = Wa * cbv * geo[DIM * DIM + DIM][ip];
             for (int c = 0; c < DIM; c++) {
                 double s = 0;
-                for (int d = 0; d < DIM; d++) s += ipc[(IPB_JI + c * DIM + d) * NIP + ip] * gb[d];
-                ph[1 + c] = Wa * (s - cbv * ipc[(IPB_DW + c) * NIP + ip]);
+                for (int d = 0; d < DIM; d++) s += geo[c * DIM + d][ip] * gb[d];
+                ph[1 + c] = Wa * (s - cbv * geo[DIM * DIM + c][ip]);
             }
-            for (int k = 0; k < NB; k++) Phd[ip][k][a] = ph[k];
+            if constexpr (!F64)
+                for (int k = 0; k < NB; k++) Phd[ip][k][a] = ph[k];
         }
         for (int k = 0; k < 4; k++) Phi[ip][k][a] = (T)ph[k];
     }
@@ -3466,8 +3484,8 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     if (tid < N) {  // lumped inertia and gravity weights (:460-462)
         double wi = 0, wg = 0;
         for (int ip = 0; ip < NIP; ip++) {
-            wi += cb[COSS_WI * NIP + ip] * Phd[ip][0][tid];
-            wg += cb[COSS_WG * NIP + ip] * Phd[ip][0][tid];
+            wi += wi_ip[ip] * phd(ip, 0, tid);
+            wg += wg_ip[ip] * phd(ip, 0, tid);
         }
         WI[tid] = wi;
         WG[tid] = wg;
@@ -3481,24 +3499,34 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     for (int blk = wv; blk < NFU * NFU; blk += NW) {
         const int al = blk / NFU, be = blk % NFU;
         // M^{ab}_kl(ip) = sum over the B rows of the pair (basis k of field a, l of field b):
-        // one lane per IP, so that each coefficient C[r][c] is read for all IPs at once
+        // one lane per IP; the at most 5 x 5 coefficients C[r][c] are loaded together (every
+        // row and column of the pair), then summed into their (k, l) slots
         if (lane < NIP) {
-            const int ip = lane;
+            const int ip = lane, na = CR::nrow(al), nb = CR::nrow(be);
+            double v[5][5];
+#pragma unroll
+            for (int x = 0; x < 5; x++)
+#pragma unroll
+                for (int y = 0; y < 5; y++)
+                    v[x][y] = (x < na && y < nb) ? cb[(COSS_C + CR::row(al, x < na ? x : 0) * 18 + CR::row(be, y < nb ? y : 0)) * NIP + ip]
+                                                 : 0.0;
             double m[4][4];
 #pragma unroll
             for (int k = 0; k < 4; k++)
 #pragma unroll
                 for (int l = 0; l < 4; l++) m[k][l] = 0.0;
-            for (int x = 0; x < CR::nrow(al); x++) {
-                const int bk = CR::bas(al, x), ra = CR::row(al, x);
-                const double ca = cof(CR::cof(al, x));
-                for (int y = 0; y < CR::nrow(be); y++) {
-                    const int bl = CR::bas(be, y);
-                    const double v = ca * cof(CR::cof(be, y)) * cb[(COSS_C + ra * 18 + CR::row(be, y)) * NIP + ip];
+#pragma unroll
+            for (int x = 0; x < 5; x++) {
+                const int bk = x < na ? CR::bas(al, x) : -1;
+                const double ca = x < na ? cof(CR::cof(al, x)) : 0.0;
+#pragma unroll
+                for (int y = 0; y < 5; y++) {
+                    const int bl = y < nb ? CR::bas(be, y) : -1;
+                    const double t = ca * (y < nb ? cof(CR::cof(be, y)) : 0.0) * v[x][y];
 #pragma unroll
                     for (int k = 0; k < 4; k++)
 #pragma unroll
-                        for (int l = 0; l < 4; l++) m[k][l] += (bk == k && bl == l) ? v : 0.0;
+                        for (int l = 0; l < 4; l++) m[k][l] += (bk == k && bl == l) ? t : 0.0;
                 }
             }
 #pragma unroll
@@ -3514,7 +3542,7 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
             for (int nt = 0; nt < MT; nt++) acc[mt][nt] = V{0, 0, 0, 0};
-#pragma unroll 1
+#pragma unroll 3
         for (int ip = 0; ip < NIP; ip++) {
             T a[MT], bq[MT];
             const double m0 = Mb[wv][ip][lk * 4], m1 = Mb[wv][ip][lk * 4 + 1], m2 = Mb[wv][ip][lk * 4 + 2],
@@ -3539,6 +3567,7 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
             if (mc.Inertia) kdiag += (oz ? 2 * l2 * mc.Density / (1 + mc.Poisson) : mc.Density) * p.ddu_dt;
             if (mc.Damping > 0.0) kdiag += (rot ? 2 * l2 * mc.Damping : mc.Damping) * p.du_dt;
         }
+        double* const Kab = p.su.K + (int64_t)p.fpos_u[al] * p.su.M;
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
@@ -3550,19 +3579,17 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
                     double val = (double)acc[mt][nt][v];
                     if (i == j) val += WI[i] * kdiag;
                     bad += val - val;
-                    double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * rp0[i] +
-                                (int64_t)p.fpos_u[be] * rdeg[i] + p.loc[(el * N + i) * N + j];
+                    double* K = Kab + (int64_t)p.su.nf * rp0[i] + (int64_t)p.fpos_u[be] * rdeg[i] + lc_[i][j];
                     *K += val;
                 }
         __builtin_amdgcn_wave_barrier();  // Mb of this wavefront is rewritten by its next pair
     }
-    // F_U (:454-455) and the lumped nodal terms (:474-527)
+    // F_U = sum_ip B^T (w sigma) (:454-455) and the lumped nodal terms (:474-527)
     for (int t = tid; t < NFU * N; t += NT) {
         const int al = t / N, a = t % N;
         double accf = 0;
         for (int ip = 0; ip < NIP; ip++)
-            for (int x = 0; x < CR::nrow(al); x++)
-                accf += cof(CR::cof(al, x)) * cb[(COSS_SIG + CR::row(al, x)) * NIP + ip] * Phd[ip][CR::bas(al, x)][a];
+            for (int x = 0; x < CR::nrow(al); x++) accf += cof(CR::cof(al, x)) * sig[CR::row(al, x)][ip] * phd(ip, CR::bas(al, x), a);
         const int64_t dof = (int64_t)p.fpos_u[al] * p.Ng + rk[a];
         const bool rot = DIM == 3 ? al >= 3 : al == 2;
         if (mc.Inertia) {
