@@ -800,7 +800,7 @@ struct ife_batch {
     DevBuf<int64_t> bp[3];      // block prefix sums: material, element, gather kernels
     int64_t tot[3] = {0, 0, 0};
     int64_t grid[3] = {0, 0, 0};  // workgroups per launch (0: one per block)
-    int nta = 256;
+    int nta = 256;  // lanes per element of the element kernel (64 in 2D: one wavefront)
     hipStream_t cap = nullptr;    // capture stream (default priority)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -909,13 +909,14 @@ void batch_tune(ife_batch* b, hipStream_t s, ife_batch_tuning* out) {
             double ms = time_it([&] { return launch_batch_material(a, g, s); });
             if (ms < best[0]) { best[0] = ms; b->grid[0] = g; }
         }
-        std::vector<int> ntas = {256};
-        if (b->dim == 2) ntas.push_back(192);
-        for (int nt : ntas)
-            for (int64_t g : candidates(b->tot[1])) {
-                double ms = time_it([&] { return launch_batch_assemble(a, g, nt, s); });
-                if (ms < best[1]) { best[1] = ms; b->grid[1] = g; b->nta = nt; }
-            }
+        // element kernel: 3D one 256-lane workgroup per element, 2D one wavefront per element
+        // (four per workgroup); the candidates are grid shapes in workgroups
+        const int nt = b->dim == 2 ? 64 : 256;
+        const int64_t wgs = b->dim == 2 ? (b->tot[1] + 3) / 4 : b->tot[1];
+        for (int64_t g : candidates(wgs)) {
+            double ms = time_it([&] { return launch_batch_assemble(a, g, nt, s); });
+            if (ms < best[1]) { best[1] = ms; b->grid[1] = g; b->nta = nt; }
+        }
         for (int64_t g : candidates(b->tot[2])) {
             double ms = time_it([&] { return launch_batch_gather(a, g, s); });
             if (ms < best[2]) { best[2] = ms; b->grid[2] = g; }
@@ -1387,7 +1388,7 @@ int ife_batch_create(ife_physics* const* domains, int32_t n_domains, ife_batch**
             ife_physics* ph = domains[d];
             if (!ph) THROW(IFE_ERR_INVALID, "batch: null domain");
             require_ready(ph);
-            if (d == 0) b->dim = ph->dim;
+            if (d == 0) { b->dim = ph->dim; b->nta = ph->dim == 2 ? 64 : 256; }
             if (ph->dim != b->dim) THROW(IFE_ERR_UNSUPPORTED, "batch: domains need the same dimension");
             b->doms.push_back(ph);
             pre[0].push_back(pre[0].back() + blocks_material(ph->dim, ph->NE));
@@ -1445,12 +1446,13 @@ int ife_batch_set_shape(ife_batch* b, const ife_batch_tuning* shape) {
         if (!b || !shape) THROW(IFE_ERR_INVALID, "null argument");
         if (shape->grid_material < 0 || shape->grid_assemble < 0 || shape->grid_gather < 0)
             THROW(IFE_ERR_INVALID, "batch: negative grid");
-        if (!(shape->lanes_assemble == 256 || (shape->lanes_assemble == 192 && b->dim == 2)))
-            THROW(IFE_ERR_INVALID, "batch: element kernel lanes must be 256 (or 192 in 2D)");
+        const int lanes = b->dim == 2 ? 64 : 256;
+        if (shape->lanes_assemble != 0 && shape->lanes_assemble != lanes)
+            THROW(IFE_ERR_INVALID, "batch: element kernel lanes per element must be 256 (3D) or 64 (2D, one wavefront), or 0");
         b->grid[0] = shape->grid_material;
         b->grid[1] = shape->grid_assemble;
         b->grid[2] = shape->grid_gather;
-        b->nta = shape->lanes_assemble;
+        b->nta = lanes;
         b->drop_graph();
     });
 }

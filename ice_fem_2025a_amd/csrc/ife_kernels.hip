@@ -1147,9 +1147,9 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                     ipo[(IPB_JI + c * DIM + d) * NIP] = Ji[c][d] * rw;
                     q += Ji[c][d] * dW[d];
                 }
-                ipo[(IPB_DW + c) * NIP] = q * rw * rw;
+                if (DIM == 3) ipo[(IPB_DW + c) * NIP] = q * rw * rw;  // 2D: polynomial basis
             }
-            ipo[IPB_WF * NIP] = rw;
+            if (DIM == 3) ipo[IPB_WF * NIP] = rw;
             ipo[IPB_W * NIP] = w;
         }
         // field values and gradients at the IP
@@ -1312,17 +1312,20 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                     }
                 }
             }
+            // 2D: the element kernel reads the Voigt rows xx, yy, xy (0, 1, 3) only
+            auto used = [](int r) { return DIM == 3 || r == 0 || r == 1 || r == 3; };
             if constexpr (!ADJ)
             for (int r = 0; r < 6; r++) {
                 double v = w * sig[r];
                 my_bad |= !isfinite(v);
-                ipo[(IPB_SIG + r) * NIP] = v;
+                if (used(r)) ipo[(IPB_SIG + r) * NIP] = v;
             }
             // Mt = w * S * DPlast = w * (S - (S a) b^T inv)  (VEP: the full DPlast)
             if (ADJ) {
             } else if ((VEP || EXACT) && !elastic) {
                 for (int i = 0; i < 6; i++)
                     for (int j = 0; j < 6; j++) {
+                        if (!used(i) || !used(j)) continue;
                         double t = 0;
                         for (int k = 0; k < 6; k++) t += Sm[i][k] * DM[k][j];
                         ipo[(IPB_M + i * 6 + j) * NIP] = w * t;
@@ -1334,7 +1337,7 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                     for (int k = 0; k < 6; k++) sa += Sm[i][k] * ra[k];
                 for (int j = 0; j < 6; j++) {
                     double v = elastic ? Sm[i][j] : (Sm[i][j] - sa * rb[j] * rinv);
-                    ipo[(IPB_M + i * 6 + j) * NIP] = w * v;
+                    if (used(i) && used(j)) ipo[(IPB_M + i * 6 + j) * NIP] = w * v;
                 }
             }
         } else if (p.has_pf && p.do_Tpf) {
@@ -1544,6 +1547,7 @@ struct Smem {
     static constexpr int N = ET<DIM>::N, NIP = ET<DIM>::NIP;
     static constexpr int NP = (N + 15) / 16 * 16;    // nodes padded to whole 16-wide tiles
     static constexpr int IPP = (NIP + 3) / 4 * 4;    // IPs padded to whole 4-deep k steps
+    static constexpr int FW = IPB_W, FLD = IPB_LDRIVE, FGC = IPB_GC;  // ipc rows read by nodal_terms
     double X[N][4];            // x, y, z, W
     T Nsh[IPP][NP];            // N, zero beyond (NIP, N)
     T G[IPP][DIM][NP];         // dN/dx, zero beyond (NIP, N)
@@ -1758,13 +1762,13 @@ __device__ __forceinline__ void kpt_store(const Smem<DIM, T>& sm, R* rec, const 
 // Nodal (lumped) terms of node a: lumped weight WL = sum_ip w N_a, inertia, damping and
 // gravity (PhaseField.cpp:436-477), lumped driving / capacity, phase viscosity and
 // stabilisation (PhaseField.cpp:693-711), heat capacity (ThermalModel.cpp:128-129).
-template <int DIM, typename T>
-__device__ __forceinline__ double nodal_terms(Smem<DIM, T>& sm, const KParams& p, const int a, const bool du, const bool dph,
+template <int DIM, typename SM>
+__device__ __forceinline__ double nodal_terms(SM& sm, const KParams& p, const int a, const bool du, const bool dph,
                                               const bool dth) {
-    constexpr int NIP = Smem<DIM, T>::NIP;
+    constexpr int NIP = SM::NIP;
     const MatConst& mc = p.mc;
     double s = 0;
-    for (int ip = 0; ip < NIP; ip++) s += sm.ipc[IPB_W][ip] * (double)sm.Nsh[ip][a];
+    for (int ip = 0; ip < NIP; ip++) s += sm.ipc[SM::FW][ip] * (double)sm.Nsh[ip][a];
     if (du) {
         double DamI, DamG, d1, d2;
         damage(mc.DensityDamage, sm.Ph[a], mc.k0, mc.p, DamI, d1, d2);
@@ -1782,11 +1786,11 @@ __device__ __forceinline__ double nodal_terms(Smem<DIM, T>& sm, const KParams& p
         }
     }
     if (dph) {
-        const double GcL = sm.ipc[IPB_GC][NIP - 1];
+        const double GcL = sm.ipc[SM::FGC][NIP - 1];
         double kd = 0, f = 0;
         if (mc.LumpedDrive) {
             double ld = 0;
-            for (int ip = 0; ip < NIP; ip++) ld += sm.ipc[IPB_LDRIVE][ip] * (double)sm.Nsh[ip][a];
+            for (int ip = 0; ip < NIP; ip++) ld += sm.ipc[SM::FLD][ip] * (double)sm.Nsh[ip][a];
             double D0, dD0, ddD0;
             damage(mc.DamFunction, sm.Ph[a], mc.k0, mc.p, D0, dD0, ddD0);
             kd += ld * ddD0;
@@ -1926,7 +1930,7 @@ __device__ __forceinline__ void assemble_body(const KParams& p, const int64_t bl
         if (du) kuu_loop<DIM, MT, T>(sm, wv, lk, l16, acc);
     } else if (wv == DIM) {
         if (lane < N) {
-            const double wl = nodal_terms<DIM, T>(sm, p, lane, du, dph, p.do_th);
+            const double wl = nodal_terms<DIM, SM>(sm, p, lane, du, dph, p.do_th);
             if (dph && p.lag) rec[lane * RC::SIZE + RC::WL] = (R)wl;
         }
         if (dph && lane == 0) {
@@ -1972,6 +1976,264 @@ __global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_batch(co
 }
 
 // ---------------------------------------------------------------------------------------
+// 2D element kernel, one wavefront per element (W2_NW elements per workgroup).  The 9-node,
+// 9-IP quadrilateral is too small for 16x16 tiles: the contractions run on the VALU in the
+// tile type T (FP64, or FP32 in mixed mode).  Every row of K_UU, K_pp and K_TT has the form
+//   K[r][j] = sum_ip A_r(ip) N(ip, j) + B_r0(ip) G_0(ip, j) + B_r1(ip) G_1(ip, j)
+// with, for the row (a, i) x b of K_UU,  B_rd = sum_c G_c(ip, i) M[vrow(a,c)][vrow(b,d)], A = 0;
+// for row i of K_pp,  A = PHA N(ip, i), B_rd = PHB G_d(ip, i);  for K_TT, A = 0, B_rd = THB G_d(ip, i).
+// Lanes 0-35 hold the 36 K_UU rows, 36-44 the K_pp rows, 45-53 the K_TT rows, and all run one
+// instruction stream: each lane reads its own coefficient rows (a zero row where a term is
+// absent).  F rows likewise: F_r += N(ip, i) c_n + G_0(ip, i) c_0 + G_1(ip, i) c_1.
+// Only the 29 per-IP coefficient rows the 2D element uses are read (compact layout W2C_*).
+// Records are staged in LDS and stored coalesced.
+enum {
+    W2C_M = 0,       // M[r][c], r, c in Voigt rows (xx, yy, xy) = (0, 1, 3): 3 x 3
+    W2C_SIG = 9,     // SIG rows 0, 1, 3
+    W2C_PHA = 12, W2C_PHB = 13, W2C_PHFN = 14, W2C_PHFG = 15,  // PHFG x, y
+    W2C_THB = 17, W2C_THFN = 18, W2C_THFG = 19,                // THFG x, y
+    W2C_LD = 21, W2C_GW = 22, W2C_GC = 23, W2C_JI = 24,        // JI 2 x 2
+    W2C_W = 28, W2C_NF = 29, W2C_Z = 29                        // row 29: zeros
+};
+// source row (ife::IPB_*) of compact row f, by arithmetic (no table load on the path)
+__device__ __forceinline__ int w2c_src(int f) {
+    const int q = f / 3, r = f % 3;
+    const int vq = q == 2 ? 3 : q, vr = r == 2 ? 3 : r;
+    return f < W2C_SIG   ? IPB_M + vq * 6 + vr
+           : f < W2C_PHA ? IPB_SIG + (f == W2C_SIG + 2 ? 3 : f - W2C_SIG)
+           : f < W2C_THB ? IPB_PHA + (f - W2C_PHA)
+           : f < W2C_LD  ? IPB_THB + (f - W2C_THB)
+           : f == W2C_LD ? IPB_LDRIVE
+           : f < W2C_JI  ? IPB_GW + (f - W2C_GW)
+           : f < W2C_W   ? IPB_JI + (f - W2C_JI)
+                         : IPB_W;
+}
+
+__device__ __forceinline__ int w2_voigt(int a, int c) { return vrow<2>(a, c) == 3 ? 2 : vrow<2>(a, c); }
+
+template <typename T, typename R>
+struct alignas(16) Wave2D {
+    static constexpr int N = 9, NIP = 9, DIM = 2;
+    static constexpr int NR = sizeof(T) == 4 ? 12 : 10;  // row pitch of N and G: whole 16-byte reads
+    static constexpr int FW = W2C_W, FLD = W2C_LD, FGC = W2C_GC;  // ipc rows read by nodal_terms
+    T Nsh[NIP][NR];
+    T G[NIP][2][NR];
+    double dU[2][N], ddU[2][N];
+    double Ph[N], PhO[N], dT[N];
+    double ipc[W2C_NF + 1][NIP];
+    T czs[sizeof(T) == 4 ? (W2C_NF + 1) * NIP : 1];  // FP32 copy of ipc (mixed mode)
+    double kdu[N], kdp[N], kdt[N];
+    double fnu[2][N], fnp[N], fnt[N];
+    double b1[2][3][3], db1[2][3][3];
+    R recs[N * Rec<2>::SIZE];
+    __device__ __forceinline__ T cz(int f, int ip) const {
+        if constexpr (sizeof(T) == 8) return ipc[f][ip];
+        else return czs[f * NIP + ip];
+    }
+};
+
+__device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ void assemble2d_wave_body(const KParams& p, const int64_t blk, Wave2D<T, R>& sm, const int lane) {
+    using RC = Rec<2>;
+    using SM = Wave2D<T, R>;
+    constexpr int N = 9, NIP = 9;
+    const int64_t el = p.elist ? (int64_t)p.elist[blk] : blk;
+    R* const grec = (R*)p.ebuf + el * N * RC::SIZE;
+    R* const rec = sm.recs;  // staged in LDS, stored to grec at the end
+    const double* const src = p.ipb + el * IPB_NF * NIP;
+    // all loads issued before the first LDS store, in dependence order: node ranks and
+    // extraction rows, the coefficient rows, then the nodal state (indexed by the ranks)
+    constexpr int KL = (W2C_NF * NIP + 63) / 64;
+    int r = 0;
+    if (lane < N) r = p.erank[el * N + lane];
+    const bool ex = lane >= 32 && lane < 32 + 18;
+    const int xd = (lane - 32) / 9, xq = ((lane - 32) / 3) % 3, xa = (lane - 32) % 3;
+    double cx[3] = {0.0, 0.0, 0.0};
+    if (ex)
+        for (int k = 0; k < 3; k++) cx[k] = p.ext[(el * 2 + xd) * 9 + xa * 3 + k];
+    double cv[KL];
+#pragma unroll
+    for (int k = 0; k < KL; k++) {
+        const int t = lane + 64 * k, f = t / NIP;
+        cv[k] = t < W2C_NF * NIP ? src[w2c_src(f) * NIP + (t - f * NIP)] : 0.0;
+    }
+    double nv[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // dU x y, ddU x y, phi, phi_old, dT
+    if (lane < N) {
+        if (p.has_pf) {
+            for (int a = 0; a < 2; a++) {
+                const int64_t d = (int64_t)p.fpos_u[a] * p.Ng + r;
+                nv[a] = p.su.dx[d];
+                nv[2 + a] = p.su.ddx[d];
+            }
+            const int64_t dp = (int64_t)p.fpos_ph * p.Ng + r;
+            nv[4] = p.sph.x[dp];
+            nv[5] = p.sph.xold[dp];
+        }
+        if (p.has_T) nv[6] = p.sT.dx[(int64_t)p.fpos_T * p.Ng + r];
+    }
+#pragma unroll
+    for (int k = 0; k < KL; k++) {
+        const int t = lane + 64 * k;
+        if (t < W2C_NF * NIP) {
+            (&sm.ipc[0][0])[t] = cv[k];
+            if constexpr (sizeof(T) == 4) sm.czs[t] = (T)cv[k];
+        }
+    }
+    if (lane < NIP) {
+        sm.ipc[W2C_Z][lane] = 0.0;
+        if constexpr (sizeof(T) == 4) sm.czs[W2C_Z * NIP + lane] = (T)0;
+    }
+    if (lane < N) {
+        sm.dU[0][lane] = nv[0];
+        sm.dU[1][lane] = nv[1];
+        sm.ddU[0][lane] = nv[2];
+        sm.ddU[1][lane] = nv[3];
+        sm.Ph[lane] = nv[4];
+        sm.PhO[lane] = nv[5];
+        sm.dT[lane] = nv[6];
+    } else if (ex) {
+        double s = 0, sd = 0;
+        for (int k = 0; k < 3; k++) { s += cx[k] * p.B1[xq][k]; sd += cx[k] * p.dB1[xq][k]; }
+        sm.b1[xd][xq][xa] = s;
+        sm.db1[xd][xq][xa] = sd;
+    }
+    wave_sync();
+    // shape functions: N = CB, G = JI GB (B2_2, with J^-1 from the material kernel)
+    for (int t = lane; t < NIP * N; t += 64) {
+        const int ip = t / N, a = t % N;
+        const int qx = ip % 3, qy = ip / 3, ax = a % 3, ay = a / 3;
+        const double bx = sm.b1[0][qx][ax], by = sm.b1[1][qy][ay], gx = sm.db1[0][qx][ax], gy = sm.db1[1][qy][ay];
+        const double gb[2] = {gx * by, bx * gy};
+        sm.Nsh[ip][a] = (T)(bx * by);
+        for (int c = 0; c < 2; c++)
+            sm.G[ip][c][a] = (T)(sm.ipc[W2C_JI + c * 2][ip] * gb[0] + sm.ipc[W2C_JI + c * 2 + 1][ip] * gb[1]);
+    }
+    wave_sync();
+    const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
+    if (lane < N) {
+        const double wl = nodal_terms<2, SM>(sm, p, lane, du, dph, p.do_th);
+        if (dph && p.lag) rec[lane * RC::SIZE + RC::WL] = (R)wl;
+    }
+    if (dph && lane == 0) {
+        double s = 0;
+        for (int ip = 0; ip < NIP; ip++) s += sm.ipc[W2C_GW][ip];
+        p.lcrack_elem[el] = s;
+    }
+    wave_sync();
+    // the lane's row: coefficient rows ma (A), mcd (B_d = sum_c G_c m_cd), fn, f0, f1 (F)
+    int i, ma = W2C_Z, m00, m01, m10, m11, fn = W2C_Z, f0, f1, roff, foff = -1, dj = -1;
+    double kd = 0.0, fadd = 0.0;
+    bool act;
+    if (lane < 36) {
+        const int al = lane / 18, be = lane % 2;
+        i = (lane / 2) % 9;
+        m00 = W2C_M + w2_voigt(al, 0) * 3 + w2_voigt(be, 0);
+        m01 = W2C_M + w2_voigt(al, 0) * 3 + w2_voigt(be, 1);
+        m10 = W2C_M + w2_voigt(al, 1) * 3 + w2_voigt(be, 0);
+        m11 = W2C_M + w2_voigt(al, 1) * 3 + w2_voigt(be, 1);
+        f0 = be == 0 ? W2C_SIG + w2_voigt(al, 0) : W2C_Z;
+        f1 = be == 0 ? W2C_SIG + w2_voigt(al, 1) : W2C_Z;
+        roff = i * RC::SIZE + RC::KU + (al * 2 + be) * N;
+        if (al == be) { dj = i; kd = sm.kdu[i]; }
+        if (be == 0) { foff = i * RC::SIZE + RC::FU + al; fadd = sm.fnu[al][i]; }
+        act = du;
+    } else {
+        const bool ph = lane < 36 + N;
+        i = ph ? lane - 36 : lane - 36 - N;
+        if (i >= N) i = 0;  // lanes 54-63: no row
+        if (ph) ma = W2C_PHA;
+        m00 = m11 = ph ? W2C_PHB : W2C_THB;
+        m01 = m10 = W2C_Z;
+        fn = ph ? W2C_PHFN : W2C_THFN;
+        f0 = ph ? W2C_PHFG : W2C_THFG;
+        f1 = f0 + 1;
+        roff = i * RC::SIZE + (ph ? RC::KP : RC::KT);
+        foff = i * RC::SIZE + (ph ? RC::FP : RC::FT);
+        dj = i;
+        kd = ph ? sm.kdp[i] : sm.kdt[i];
+        fadd = ph ? sm.fnp[i] : sm.fnt[i];
+        act = lane < 36 + 2 * N && (ph ? dph : dT);
+    }
+    double chk = 0.0;
+    if (act) {
+        T acc[N], fr = 0;
+#pragma unroll
+        for (int j = 0; j < N; j++) acc[j] = 0;
+#pragma unroll
+        for (int ip = 0; ip < NIP; ip++) {
+            const T ni = sm.Nsh[ip][i], g0 = sm.G[ip][0][i], g1 = sm.G[ip][1][i];
+            const T A = ni * sm.cz(ma, ip);
+            const T B0 = tfma(g0, sm.cz(m00, ip), g1 * sm.cz(m10, ip));
+            const T B1 = tfma(g0, sm.cz(m01, ip), g1 * sm.cz(m11, ip));
+            fr = tfma(ni, sm.cz(fn, ip), tfma(g0, sm.cz(f0, ip), tfma(g1, sm.cz(f1, ip), fr)));
+#pragma unroll
+            for (int j = 0; j < N; j++) acc[j] = tfma(A, sm.Nsh[ip][j], tfma(B0, sm.G[ip][0][j], tfma(B1, sm.G[ip][1][j], acc[j])));
+        }
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double v = (double)acc[j] + (j == dj ? kd : 0.0);
+            chk = fma(v, 0.0, chk);
+            rec[roff + j] = (R)v;
+        }
+        if (foff >= 0) {
+            const double v = (double)fr + fadd;
+            chk = fma(v, 0.0, chk);
+            rec[foff] = (R)v;
+        }
+    }
+    if (chk != 0.0) atomicOr(&p.st->nonfinite, 1u);
+    wave_sync();
+    // coalesced store of the parts of the records this pass wrote: K_UU, F_U | K_pp, F_p |
+    // K_TT (heat-equation pass only), F_T | WL (multiplier irreversibility)
+    for (int t = lane; t < N * RC::SIZE; t += 64) {
+        const int o = t % RC::SIZE;
+        const bool w = o < RC::KP ? du
+                       : o < RC::KT ? dph
+                       : o < RC::FT ? (dT && p.do_th)
+                       : o == RC::FT ? dT
+                       : o == RC::WL ? (dph && p.lag)
+                                     : false;
+        if (w) grec[t] = rec[t];
+    }
+    __builtin_amdgcn_wave_barrier();  // the wavefront's LDS is reused by its next element
+}
+
+constexpr int W2_NW = 4;  // elements (wavefronts) per workgroup of the 2D wave kernel
+
+template <typename T, typename R>
+__global__ __launch_bounds__(64 * W2_NW) void assemble2d_wave(KParams p) {
+    __shared__ Wave2D<T, R> sm[W2_NW];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t blk = (int64_t)blockIdx.x * W2_NW + wv;
+    if (blk < p.nlist) assemble2d_wave_body<T, R>(p, blk, sm[wv], lane);
+}
+
+// batched form: element b of the concatenated domains goes to wavefront b % W2_NW of
+// workgroup b / W2_NW (flat), or the workgroups stride over them (persistent)
+template <typename T, typename R, bool PERSIST>
+__global__ __launch_bounds__(64 * W2_NW) void assemble2d_wave_batch(const KParams* P_, const int64_t* bptr_, int nd) {
+    KParamsC* P = (KParamsC*)P_;
+    I64C* bptr = (I64C*)bptr_;
+    __shared__ Wave2D<T, R> sm[W2_NW];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t total = bptr[nd];
+    const int64_t stride = PERSIST ? (int64_t)gridDim.x * W2_NW : total;
+    for (int64_t b = (int64_t)blockIdx.x * W2_NW + wv; b < total; b += stride) {
+        const int d = batch_domain(bptr, nd, b);
+        assemble2d_wave_body<T, R>((const KParams&)P[d], b - bptr[d], sm[wv], lane);
+        if (!PERSIST) break;
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Multiplier irreversibility of one node (PhaseFieldUtility::EnforceIrreversible,
 // PhaseFieldUtil.cpp:350-370): LagrangeMult (LagrMultIrr :383-398), AugmentedLagrangeMult
@@ -3609,13 +3871,16 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
 hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
     if (p.nlist <= 0) return hipSuccess;
     dim3 grid((unsigned)p.nlist), block(NT);
-    // mixed mode: FP32 tiles and FP32 element records (the tiles' accumulators are FP32)
-    if (p.mixed) {
-        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, float, float, NT>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((assemble_kernel<3, float, float, NT>), grid, block, 0, s, p);
+    // 2D: one wavefront per element on the VALU; 3D: 16x16 tiles.  Mixed mode: FP32 tiles /
+    // contractions and FP32 element records (the accumulators are FP32)
+    if (dim == 2) {
+        const dim3 g2((unsigned)((p.nlist + W2_NW - 1) / W2_NW)), b2(64 * W2_NW);
+        if (p.mixed) hipLaunchKernelGGL((assemble2d_wave<float, float>), g2, b2, 0, s, p);
+        else hipLaunchKernelGGL((assemble2d_wave<double, double>), g2, b2, 0, s, p);
+    } else if (p.mixed) {
+        hipLaunchKernelGGL((assemble_kernel<3, float, float, NT>), grid, block, 0, s, p);
     } else {
-        if (dim == 2) hipLaunchKernelGGL((assemble_kernel<2, double, double, NT>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((assemble_kernel<3, double, double, NT>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((assemble_kernel<3, double, double, NT>), grid, block, 0, s, p);
     }
     return hipGetLastError();
 }
@@ -3780,21 +4045,30 @@ hipError_t launch_batch_material(const BatchArgs& a, int64_t grid, hipStream_t s
 
 hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipStream_t s) {
     if (a.tot[1] <= 0) return hipSuccess;
-    const dim3 g = batch_grid(grid, a.tot[1]);
     const int64_t* bp = a.bptr[1];
+    if (a.dim == 2) {  // one wavefront per element, W2_NW per workgroup
+        const int64_t wgs = (a.tot[1] + W2_NW - 1) / W2_NW;
+        const bool ps2 = persistent(grid, wgs);
+        const dim3 g2 = batch_grid(grid, wgs), b2(64 * W2_NW);
+        if (a.mixed) {
+            if (ps2) hipLaunchKernelGGL((assemble2d_wave_batch<float, float, true>), g2, b2, 0, s, a.P, bp, a.nd);
+            else hipLaunchKernelGGL((assemble2d_wave_batch<float, float, false>), g2, b2, 0, s, a.P, bp, a.nd);
+        } else {
+            if (ps2) hipLaunchKernelGGL((assemble2d_wave_batch<double, double, true>), g2, b2, 0, s, a.P, bp, a.nd);
+            else hipLaunchKernelGGL((assemble2d_wave_batch<double, double, false>), g2, b2, 0, s, a.P, bp, a.nd);
+        }
+        return hipGetLastError();
+    }
+    const dim3 g = batch_grid(grid, a.tot[1]);
     const bool ps = persistent(grid, a.tot[1]);
 #define IFE_AB(D, T, L)                                                                               \
     do {                                                                                              \
         if (ps) hipLaunchKernelGGL((assemble_batch<D, T, T, L, true>), g, dim3(L), 0, s, a.P, bp, a.nd);      \
         else hipLaunchKernelGGL((assemble_batch<D, T, T, L, false>), g, dim3(L), 0, s, a.P, bp, a.nd);       \
     } while (0)
-    if (a.dim == 3 || nta != 192) {
-        if (a.mixed) { if (a.dim == 2) IFE_AB(2, float, NT); else IFE_AB(3, float, NT); }
-        else { if (a.dim == 2) IFE_AB(2, double, NT); else IFE_AB(3, double, NT); }
-    } else {  // 2D with three waves: no idle wave
-        if (a.mixed) IFE_AB(2, float, 192);
-        else IFE_AB(2, double, 192);
-    }
+    (void)nta;  // 3D: 256 lanes (four wavefront roles)
+    if (a.mixed) IFE_AB(3, float, NT);
+    else IFE_AB(3, double, NT);
 #undef IFE_AB
     return hipGetLastError();
 }

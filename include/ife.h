@@ -197,8 +197,9 @@ typedef struct ife_batch ife_batch;
 /* Launch shape of a batch and what the autotuner measured.  grid_* = workgroups per
  * launch of the material, element and gather kernels (0: one workgroup per block of
  * work; otherwise persistent workgroups that stride over the blocks of all domains);
- * lanes_assemble = lanes per element workgroup (256, or 192 in 2D: one wavefront per
- * role, no idle wavefront). */
+ * lanes_assemble = lanes per element (3D: 256, a workgroup of four wavefronts per element;
+ * 2D: 64, one wavefront per element, four elements per workgroup); ife_batch_set_shape
+ * accepts that value or 0. */
 typedef struct {
     int64_t grid_material, grid_assemble, grid_gather;
     int32_t lanes_assemble;
