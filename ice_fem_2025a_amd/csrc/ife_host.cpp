@@ -391,7 +391,8 @@ void plan(ife_physics* ph) {
     lists.shrink_to_fit();
     int64_t maxdeg = 0;
     for (int64_t v = 0; v < Ng; v++) maxdeg = std::max(maxdeg, ph->nbr_ptr[v + 1] - ph->nbr_ptr[v]);
-    if (maxdeg > 128) THROW(IFE_ERR_UNSUPPORTED, "node degree above 128 (gather rows are staged in LDS)");
+    if (maxdeg > (ph->dim == 2 ? 64 : 128))
+        THROW(IFE_ERR_UNSUPPORTED, "node degree above 128 in 3D or 64 in 2D (gather rows are staged in LDS)");
     // ---- per-step CSR: row (fp, v) = [g*Ng + nbr(v) for the fields g paired with fp] ----
     // Every field pair of a step is present except the two multipliers of AugmentedLagrangeMult2
     // and 3, whose blocks the phase step never inserts (PhaseField.cpp:717-728)

@@ -2568,8 +2568,13 @@ __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
 // record the lanes' slots are distinct.
 // ---------------------------------------------------------------------------------------
 constexpr int GW = 64;      // lanes per gather workgroup (one wavefront)
-constexpr int MAXDEG = 128; // neighbour slots per row held in LDS
-constexpr int GPF = 8;      // records loaded ahead
+// neighbour slots per row held in LDS: 128 in 3D (27-node hexes: 125 in a structured mesh),
+// 64 in 2D (25); the host rejects meshes above these
+template <int DIM>
+constexpr int gather_maxdeg() { return DIM == 2 ? 64 : 128; }
+// records loaded ahead: 8 in 3D (up to 27 elements per node), 9 in 2D (up to 9: one batch)
+template <int DIM>
+constexpr int gather_gpf() { return DIM == 2 ? 9 : 8; }
 
 // W consecutive record entries per lane load: one FP64 entry (8 B), or two FP32 entries
 // (8 B) of the mixed mode's records, so that a load moves the same bytes in both modes (the
@@ -2599,6 +2604,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
     using RC = Rec<DIM>;
     typedef typename RecLoad<T>::V V;
     constexpr int N = RC::N, W = RecLoad<T>::W, PER = (RC::SIZE + GW * W - 1) / (GW * W);
+    constexpr int MAXDEG = gather_maxdeg<DIM>(), GPF = gather_gpf<DIM>();
     static_assert(RC::SIZE % W == 0, "record length must be a whole number of loads");
     // LDS row accumulators: K_UU [DIM*DIM][MAXDEG], K_pp [MAXDEG], K_TT [MAXDEG], f_U, f_p, f_T,
     // the node's lumped weight
