@@ -19,7 +19,8 @@ Also reported:
 
 Other workloads (--workload):
   quad2d_batch   64 independent 2D domains (u(2)+phi+T, 2k..60k elements each, ~1M in
-                 total) in FP32-MFMA mode, assembled by one autotuned batched launch per
+                 total) in mixed precision (FP32 element contractions, FP64 material
+                 update and row sums), assembled by one autotuned batched launch per
                  kernel captured in a hipGraph; a step is one graph replay.
   hex3d_adjoint  32^3 hexes with per-element Gc and A: the adjoint pass
                  (d(lambda^T r)/d(Gc_e, A_e, n)) after one assembly; a step is one adjoint.
@@ -165,10 +166,11 @@ def bench_batch(args):
     alg_bytes = 8 * nnz + 8 * dofs + 16 * nodes + 144 * E + 40 * nodes + 16 * 9 * E
     gbs = alg_bytes / (ms_per_step / 1e3) / 1e9
     out = {
-        "metric": "elements assembled/sec (K+f), batched 2D Q2 Bezier quad domains, FP32-MFMA",
+        "metric": "elements assembled/sec (K+f), batched 2D Q2 Bezier quad domains, "
+                  + ("FP64" if args.precision == "fp64" else "mixed precision"),
         "value": value, "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-mfma+f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (seeded Philox sizes and states on generated B-spline patches)",
         "config": {"workload": "quad2d_batch", "domains": len(doms), "elements": E, "dofs": dofs, "nnz": nnz,
                    "elements_per_domain": [p.mesh.n_elems for p in probs][:64],
