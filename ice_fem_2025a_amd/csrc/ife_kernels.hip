@@ -1,15 +1,17 @@
 // ife_kernels.hip -- element residual/tangent assembly kernels for gfx950 (MI355X).
 //
-// Two kernels per assembly:
+// Three kernels per assembly:
 //   material_kernel  one lane per integration point over all elements: shape functions
 //                    at the IP, strains, the Glen return map, energy split, history max and
 //                    the crack-density / heating terms; writes per-IP coefficients.  The
 //                    data-dependent Newton loop runs here, where many waves hide its latency.
-//   assemble_kernel  one workgroup (256 lanes, 4 wavefronts) per element: geometry and
+//   assemble_kernel  3D: one workgroup (256 lanes, 4 wavefronts) per element: geometry and
 //                    extraction gathered into LDS, shape functions built there once and shared
 //                    by every staggered step, the element tangents as 16x16 MFMA tiles
 //                    contracted over (IP, gradient component), and the element's node rows
 //                    written contiguously to an element-record buffer.
+//   assemble2d_wave  2D: one wavefront per element (9 nodes, 9 IPs: too small for tiles), the
+//                    same records from one FMA stream per K/f row.
 //   gather_kernel    one wavefront per node: adds the records of the node's elements in
 //                    ascending element order and writes the node's CSR rows once (no atomics,
 //                    no colouring, deterministic).
@@ -1822,7 +1824,8 @@ __device__ __forceinline__ double nodal_terms(SM& sm, const KParams& p, const in
 
 // T: the tile type of the element contractions (double: FP64 mode; float: mixed mode, FP32
 // MFMA); R: the element record type.
-// NTA: lanes per workgroup, 64 (DIM + 1) or 256 (one idle wave in 2D).
+// NTA: lanes per workgroup (256: three K_UU wavefronts and the phase/thermal one).  Launched
+// for DIM = 3; 2D elements go to assemble2d_wave.
 template <int DIM, typename T, typename R, int NTA>
 __device__ __forceinline__ void assemble_body(const KParams& p, const int64_t blk) {
     using E = ET<DIM>;

@@ -382,8 +382,8 @@ int ife_batch_capture(ife_batch* b);
 int ife_batch_launch(ife_batch* b, void* stream);
 
 /* Autotuner: times every candidate launch shape of each kernel on the current inputs
- * (grid = one workgroup per block, or 256, 512, ... persistent workgroups; element
- * kernel with 256 or, in 2D, 192 lanes) and keeps the fastest.  Runs assemblies on
+ * (grid = one workgroup per block, or 256, 512, ... persistent workgroups) and keeps
+ * the fastest.  Runs assemblies on
  * `stream` and synchronises it; drops a captured graph. */
 int ife_batch_tune(ife_batch* b, void* stream, ife_batch_tuning* out);
 
