@@ -34,6 +34,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -65,6 +67,17 @@ def cpu_baseline(workload, sample_nel, threads):
     return E / t, E, t
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the baseline record."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def traffic_from_profile(precision):
     """HBM bytes per element-kernel launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (profiles/), or None if there is none."""
@@ -78,7 +91,41 @@ def traffic_from_profile(precision):
     return None
 
 
-def measure_single(prob, precision, steps, warmup):
+def measure_constraints(ph, prob, stream, reps=10):
+    """Dirichlet constraints on every field of the bottom-face nodes of each step; time
+    `reps` constraint eliminations of all steps (K_con, f_con from the last assembly); the
+    step-0 residual norms for dx = 1.  Returns ms per elimination of all steps and its bytes."""
+    from ice_fem_2025a_amd import abi, hip
+    m = prob.mesh
+    zmin = m.coords[:, m.dim - 1].min()
+    face = np.nonzero(np.abs(m.coords[:, m.dim - 1] - zmin) < 1e-12)[0]
+    nbytes, ncon = 0, 0
+    for s in range(ph.nsteps):
+        fields = [i for i, st in enumerate(prob.dof_steps) if st == s]
+        dofs = np.concatenate([ph.dof_numbers(fi)[face] for fi in fields])
+        ph.set_constraints(s, dofs, ph.get_state(s, abi.IFE_STATE)[dofs])
+        rp, col, _, fc = ph.constrained(s)
+        nnz, n = ph.nnz(s), ph.n_rows(s)
+        # spmv over K (values, columns, x, y), K_con gather (offset, value, write), f_con
+        nbytes += 12 * nnz + 16 * n + 20 * col.size + 32 * fc.size
+        ncon += dofs.size
+    for s in range(ph.nsteps):
+        ph.constrain(s, stream)
+    stream.synchronize()
+    ev0, ev1 = hip.Event(), hip.Event()
+    ev0.record(stream)
+    for _ in range(reps):
+        for s in range(ph.nsteps):
+            ph.constrain(s, stream)
+    ev1.record(stream)
+    stream.synchronize()
+    ms = ev0.elapsed_ms(ev1) / reps
+    norms = ph.residual_norms(0, np.ones(ph.constrained(0)[3].size), stream)
+    return {"ms_per_assembly": ms, "constrained_dofs": int(ncon), "algorithmic_bytes": int(nbytes),
+            "achieved_GBps": nbytes / (ms / 1e3) / 1e9, "step0_norms_dx_ones": list(norms)}
+
+
+def measure_single(prob, precision, steps, warmup, constraints=False):
     """Time `steps` ResetStep + assemble_all steps of one domain on one stream."""
     from ice_fem_2025a_amd import hip
     from ice_fem_2025a_amd.engine import Physics
@@ -113,10 +160,11 @@ def measure_single(prob, precision, steps, warmup):
     ph.set_profiling(False)
     dofs = sum(ph.n_rows(s) for s in range(ph.nsteps))
     nnz = sum(ph.nnz(s) for s in range(ph.nsteps))
+    cons = measure_constraints(ph, prob, stream) if constraints else None
     ph.close()
     ms_per_step = ms_total / steps
     return {"ms_per_step": ms_per_step, "value": E / (ms_per_step / 1e3), "kbar": kbar, "tim": tim, "wall": wall,
-            "dofs": dofs, "nnz": nnz}
+            "dofs": dofs, "nnz": nnz, "constraints": cons}
 
 
 def bench_batch(args):
@@ -320,8 +368,8 @@ def bench_coss(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None)
     ap.add_argument("--workload", default="hex3d_glen_at2")
     ap.add_argument("--nel", type=int, nargs="*", default=None, help="override element counts")
@@ -349,7 +397,7 @@ def main():
     t0 = time.time()
     prob = make_problem(args.workload, nel=args.nel)
     E = prob.mesh.n_elems
-    m = measure_single(prob, args.precision, args.steps, args.warmup)
+    m = measure_single(prob, args.precision, args.steps, args.warmup, constraints=True)
     print(f"# setup+run {time.time() - t0:.1f}s: {E} elements, {m['dofs']} dofs, {m['nnz']} nnz",
           file=sys.stderr, flush=True)
     kbar, tim, wall, ms_per_step, value = m["kbar"], m["tim"], m["wall"], m["ms_per_step"], m["value"]
@@ -388,6 +436,8 @@ def main():
         "whole_step": {"flop_per_element": fpe, "achieved_tflops": step_tflops,
                        "mean_return_map_iterations": kbar},
         "wall_s": wall,
+        "constraint_elimination": dict(m["constraints"], note="Physics::Constraint of all three steps on the "
+                                       "device (bottom-face Dirichlet dofs), after the timed assemblies; not in value"),
     }
     if args.precision == "fp64" and not args.precision_given:
         mx = measure_single(prob, "mixed", args.steps, args.warmup)
@@ -401,9 +451,12 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         rate, ne, secs = cpu_baseline(args.workload, (30, 30, 30) if args.workload.startswith("hex3d") else None,
                                       threads)
+        rate1, ne1, secs1 = cpu_baseline(args.workload, (10, 10, 10) if args.workload.startswith("hex3d") else None, 1)
         out["cpu_baseline"] = {"value": rate, "unit": "elements/s", "cores": threads, "kind": "port",
                                "sample": f"{ne} elements (30x30x30 block of the same workload), all three steps, "
-                                         f"{secs:.1f}s wall on {threads} threads"}
+                                         f"{secs:.1f}s wall on {threads} threads",
+                               "one_thread": {"value": rate1, "sample": f"{ne1} elements (10x10x10 block), {secs1:.1f}s"},
+                               "cpu_model": cpu_model()}
         out["gpu_over_cpu"] = value / rate
     print(json.dumps(out), flush=True)
 

@@ -114,6 +114,12 @@ _PROTOS = [
     ("ife_model_reset_step", C.c_int, [_P, C.c_int32, _P]),
     ("ife_physics_adjoint", C.c_int, [_P, C.POINTER(_P), _P, _P, _P, _P]),
     ("ife_physics_spmv", C.c_int, [_P, C.c_int32, _P, _P, _P, _P]),
+    ("ife_physics_set_constraints", C.c_int, [_P, C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    ("ife_physics_constrain", C.c_int, [_P, C.c_int32, _P]),
+    ("ife_physics_constrained", C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                          C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.POINTER(C.c_int32)),
+                                          C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    ("ife_physics_residual_norms", C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(C.c_double)]),
     ("ife_model_set_element_params", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("ife_model_set_param", C.c_int, [_P, C.c_int32, C.c_char_p, C.c_double]),
     ("ife_batch_create", C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),

@@ -19,6 +19,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <map>
 #include <vector>
 
 #include "ife_internal.h"
@@ -138,6 +139,15 @@ struct Step {
     DevBuf<int64_t> d_row_ptr;
     DevBuf<int32_t> d_col;
     DevBuf<double> K, f, x[4];
+    // Dirichlet constraints (Constrainer): constrained dofs ascending with their values, free
+    // dofs ascending, the constrained pattern with each entry's offset in its source row
+    bool has_cons = false;
+    int64_t nfree = 0;
+    std::vector<int64_t> c_row_ptr;
+    std::vector<int32_t> c_col;
+    DevBuf<int64_t> d_cdofs, d_free, d_crp;
+    DevBuf<int32_t> d_ccol, d_rel;
+    DevBuf<double> d_cvals, d_g, d_y, d_kcon, d_fcon, d_part, d_norms;
 };
 
 }  // namespace
@@ -1226,6 +1236,142 @@ int ife_physics_spmv(ife_physics* ph, int32_t step, const double* K_vals, const 
         if (!st.has_matrix) THROW(IFE_ERR_STATE, "step has no matrix");
         const double* K = K_vals ? K_vals : st.K.p;
         HIPCHK(launch_spmv(st.d_row_ptr.p, st.d_col.p, K, x, y, st.nrows, (hipStream_t)stream));
+    });
+}
+
+int ife_physics_set_constraints(ife_physics* ph, int32_t step, int64_t n, const int64_t* dofs, const double* values) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (n < 0 || (n > 0 && (!dofs || !values))) THROW(IFE_ERR_INVALID, "bad constraint arrays");
+        Step& st = ph->steps[step];
+        if (!st.has_matrix) THROW(IFE_ERR_STATE, "step has no matrix");
+        // Constrainer::AddConstraint (Constrainer.cpp:45-54): one value per dof, map order
+        std::map<int64_t, double> con;
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t d = dofs[i];
+            if (d < 0 || d >= st.nrows) THROW(IFE_ERR_INVALID, "constrained dof out of range: " + std::to_string(d));
+            auto it = con.find(d);
+            if (it != con.end()) {
+                if (it->second != values[i])
+                    THROW(IFE_ERR_INVALID, "Degree of freedom " + std::to_string(d) + " is doubly constrained with different values");
+            } else {
+                con.emplace(d, values[i]);
+            }
+        }
+        const int64_t N = st.nrows, nc = (int64_t)con.size(), nfree = N - nc;
+        std::vector<int64_t> cd, fr;
+        std::vector<double> cv;
+        std::vector<int64_t> fmap(N, -1);
+        cd.reserve(nc); cv.reserve(nc); fr.reserve(nfree);
+        {
+            auto it = con.begin();
+            for (int64_t d = 0; d < N; d++) {
+                if (it != con.end() && it->first == d) { cd.push_back(d); cv.push_back(it->second); ++it; }
+                else { fmap[d] = (int64_t)fr.size(); fr.push_back(d); }
+            }
+        }
+        // K_con = U^T K U: the free rows, their free columns renumbered (monotone, so sorted)
+        std::vector<int64_t> crp(nfree + 1, 0);
+        for (int64_t k = 0; k < nfree; k++) {
+            const int64_t r = fr[k];
+            int64_t c = 0;
+            for (int64_t q = st.row_ptr[r]; q < st.row_ptr[r + 1]; q++) c += fmap[st.col[q]] >= 0;
+            crp[k + 1] = crp[k] + c;
+        }
+        std::vector<int32_t> ccol(crp[nfree]), rel(crp[nfree]);
+        parallel_for(nfree, [&](int64_t a, int64_t b) {
+            for (int64_t k = a; k < b; k++) {
+                const int64_t r = fr[k];
+                int64_t o = crp[k];
+                for (int64_t q = st.row_ptr[r]; q < st.row_ptr[r + 1]; q++) {
+                    const int64_t m = fmap[st.col[q]];
+                    if (m < 0) continue;
+                    ccol[o] = (int32_t)m;
+                    rel[o] = (int32_t)(q - st.row_ptr[r]);
+                    o++;
+                }
+            }
+        });
+        st.has_cons = false;
+        st.nfree = nfree;
+        st.c_row_ptr = std::move(crp);
+        st.c_col = std::move(ccol);
+        st.d_cdofs.upload(cd.data(), cd.size());
+        st.d_cvals.upload(cv.data(), cv.size());
+        st.d_free.upload(fr.data(), fr.size());
+        st.d_crp.upload(st.c_row_ptr.data(), st.c_row_ptr.size());
+        st.d_ccol.upload(st.c_col.data(), st.c_col.size());
+        st.d_rel.upload(rel.data(), rel.size());
+        std::vector<double> zero(N, 0.0);  // g is zero at the free dofs for good
+        st.d_g.upload(zero.data(), zero.size());
+        st.d_y.alloc(N);
+        st.d_kcon.alloc(st.c_col.size());
+        st.d_fcon.alloc(nfree);
+        st.d_part.alloc((size_t)3 * norms_partials());
+        st.d_norms.alloc(3);
+        st.has_cons = true;
+    });
+}
+
+int ife_physics_constrain(ife_physics* ph, int32_t step, void* stream) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        Step& st = ph->steps[step];
+        if (!st.has_cons) THROW(IFE_ERR_STATE, "no constraints set for this step (ife_physics_set_constraints)");
+        ConsArgs a{};
+        a.n = st.nrows;
+        a.nc = (int64_t)st.d_cdofs.n;
+        a.nfree = st.nfree;
+        a.krp = st.d_row_ptr.p;
+        a.kcol = st.d_col.p;
+        a.Kv = st.K.p;
+        a.f = st.f.p;
+        a.x = st.x[IFE_STATE].p;
+        a.cdofs = st.d_cdofs.p;
+        a.cvals = st.d_cvals.p;
+        a.free = st.d_free.p;
+        a.crp = st.d_crp.p;
+        a.rel = st.d_rel.p;
+        a.g = st.d_g.p;
+        a.y = st.d_y.p;
+        a.Kc = st.d_kcon.p;
+        a.fcon = st.d_fcon.p;
+        HIPCHK(launch_constrain(a, (hipStream_t)stream));
+    });
+}
+
+int ife_physics_constrained(const ife_physics* ph, int32_t step, int64_t* n_free, int64_t* nnz,
+                            const int64_t** row_ptr_host, const int32_t** col_host, const int64_t** row_ptr_dev,
+                            const int32_t** col_dev, double** K_con, double** f_con) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        const Step& st = ph->steps[step];
+        if (!st.has_cons) THROW(IFE_ERR_STATE, "no constraints set for this step (ife_physics_set_constraints)");
+        if (n_free) *n_free = st.nfree;
+        if (nnz) *nnz = (int64_t)st.c_col.size();
+        if (row_ptr_host) *row_ptr_host = st.c_row_ptr.data();
+        if (col_host) *col_host = st.c_col.data();
+        if (row_ptr_dev) *row_ptr_dev = st.d_crp.p;
+        if (col_dev) *col_dev = st.d_ccol.p;
+        if (K_con) *K_con = st.d_kcon.p;
+        if (f_con) *f_con = st.d_fcon.p;
+    });
+}
+
+int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, void* stream, double out[3]) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (!dx || !out) THROW(IFE_ERR_INVALID, "null argument");
+        Step& st = ph->steps[step];
+        if (!st.has_cons) THROW(IFE_ERR_STATE, "no constraints set for this step (ife_physics_set_constraints)");
+        hipStream_t s = (hipStream_t)stream;
+        HIPCHK(launch_norms(dx, st.d_fcon.p, st.nfree, st.d_part.p, st.d_norms.p, s));
+        HIPCHK(hipMemcpyAsync(out, st.d_norms.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
     });
 }
 

@@ -2784,6 +2784,77 @@ __global__ __launch_bounds__(256) void spmv_kernel(const int64_t* rp, const int3
     if (lane == 0) y[r] = t;
 }
 
+// ---------------------------------------------------------------------------------------
+// Dirichlet constraint elimination (Physics::Constraint, physics.cpp:329-352, with the
+// selection matrices of Constrainer::Assemble, Constrainer.cpp:98-110): U selects the free
+// dofs, C the constrained ones, both in ascending dof order, so
+//   K_con = U^T K U    = K restricted to free rows and free columns (columns stay sorted),
+//   f_con = -U^T (f - K C c),   c_d = value_d - x_d  (the constraint's remaining increment).
+// ---------------------------------------------------------------------------------------
+// g = -C c: written at the constrained dofs; the free entries of g stay zero
+__global__ void cons_g_kernel(const int64_t* cd, const double* cv, const double* x, double* g, int64_t nc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nc) g[cd[i]] = -(cv[i] - x[cd[i]]);
+}
+
+// f_con[k] = -(f + K g)[free_k]   (y = K g from spmv_kernel)
+__global__ void cons_f_kernel(const int64_t* fr, const double* f, const double* y, double* fc, int64_t nfree) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nfree) fc[k] = -(f[fr[k]] + y[fr[k]]);
+}
+
+// K_con values: one wavefront per free row; entry e of the row is the row's source entry
+// rel[e] of K (relative to the source row's start)
+__global__ __launch_bounds__(256) void cons_k_kernel(const int64_t* crp, const int32_t* rel, const int64_t* krp,
+                                                      const int64_t* fr, const double* Kv, double* Kc, int64_t nfree) {
+    const int64_t k = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (k >= nfree) return;
+    const int64_t b = crp[k], e = crp[k + 1], s0 = krp[fr[k]];
+    for (int64_t i = b + lane; i < e; i += 64) Kc[i] = Kv[s0 + rel[i]];
+}
+
+// Residual norms of NonLinSolver::CheckConvergence (NonLinSolver.cpp:208-218) before the
+// division by the dof count: sum |dx|, sum |f_con|, sum |f_con dx|.  Fixed grid and tree
+// order, so the sums are reproducible: NB block partials, then one block adds them in order.
+constexpr int NORM_NB = 1024;
+__global__ __launch_bounds__(256) void norms_partial_kernel(const double* dx, const double* fc, double* part, int64_t n) {
+    __shared__ double red[3][256];
+    double a = 0, b = 0, c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)NORM_NB * 256) {
+        const double u = dx[i], v = fc[i];
+        a += fabs(u);
+        b += fabs(v);
+        c += fabs(u * v);
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[threadIdx.x * NORM_NB + blockIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void norms_final_kernel(const double* part, double* out, double n) {
+    __shared__ double red[3][256];
+    for (int q = 0; q < 3; q++) {
+        double t = 0;
+        for (int i = threadIdx.x; i < NORM_NB; i += 256) t += part[q * NORM_NB + i];
+        red[q][threadIdx.x] = t;
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[threadIdx.x] = n > 0 ? red[threadIdx.x][0] / n : 0.0;  // Err / nDofs
+}
+
 __global__ void copy_kernel(double* d, const double* s, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -3996,6 +4067,29 @@ hipError_t launch_spmv(const int64_t* row_ptr, const int32_t* col, const double*
     hipLaunchKernelGGL(spmv_kernel, dim3((unsigned)blocks), dim3(256), 0, s, row_ptr, col, val, x, y, n_rows);
     return hipGetLastError();
 }
+
+hipError_t launch_constrain(const ConsArgs& a, hipStream_t s) {
+    if (a.nc > 0)
+        hipLaunchKernelGGL(cons_g_kernel, dim3((unsigned)((a.nc + 255) / 256)), dim3(256), 0, s, a.cdofs, a.cvals, a.x, a.g,
+                           a.nc);
+    hipError_t e = launch_spmv(a.krp, a.kcol, a.Kv, a.g, a.y, a.n, s);
+    if (e != hipSuccess) return e;
+    if (a.nfree > 0) {
+        hipLaunchKernelGGL(cons_f_kernel, dim3((unsigned)((a.nfree + 255) / 256)), dim3(256), 0, s, a.free, a.f, a.y, a.fcon,
+                           a.nfree);
+        hipLaunchKernelGGL(cons_k_kernel, dim3((unsigned)((a.nfree + 3) / 4)), dim3(256), 0, s, a.crp, a.rel, a.krp, a.free,
+                           a.Kv, a.Kc, a.nfree);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_norms(const double* dx, const double* fc, int64_t n, double* part, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(norms_partial_kernel, dim3(NORM_NB), dim3(256), 0, s, dx, fc, part, n);
+    hipLaunchKernelGGL(norms_final_kernel, dim3(1), dim3(256), 0, s, part, out, (double)n);
+    return hipGetLastError();
+}
+
+int norms_partials() { return NORM_NB; }
 
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s) {
     if (n <= 0) return hipSuccess;

@@ -216,6 +216,45 @@ class Physics:
         abi.check(self.L.ife_physics_spmv(self.h, step, None, dx.ptr, dy.ptr, None))
         return dy.get()
 
+    # ---- constraints and residual norms (Physics::Constraint, CheckConvergence) ----------
+    def set_constraints(self, step, dofs, values):
+        """Dirichlet constraints of a step: global dofs and their state values."""
+        d = np.ascontiguousarray(dofs, dtype=np.int64)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if d.shape != v.shape:
+            raise ValueError("dofs and values differ in length")
+        abi.check(self.L.ife_physics_set_constraints(
+            self.h, step, d.size, d.ctypes.data_as(C.POINTER(C.c_int64)), v.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def constrain(self, step, stream=None):
+        """K_con = U^T K U, f_con = -U^T (f - K C c) on the device, from the last assembly."""
+        abi.check(self.L.ife_physics_constrain(self.h, step, stream.handle if stream else None))
+
+    def constrained(self, step):
+        """Host copies of the constrained system: row_ptr, col, K_con values, f_con."""
+        nf, nnz = C.c_int64(), C.c_int64()
+        rp, col = C.POINTER(C.c_int64)(), C.POINTER(C.c_int32)()
+        kp, fp = C.c_void_p(), C.c_void_p()
+        abi.check(self.L.ife_physics_constrained(self.h, step, C.byref(nf), C.byref(nnz), C.byref(rp), C.byref(col),
+                                                 None, None, C.byref(kp), C.byref(fp)))
+        r = np.ctypeslib.as_array(rp, shape=(nf.value + 1,)).copy()
+        c = np.ctypeslib.as_array(col, shape=(nnz.value,)).copy() if nnz.value else np.zeros(0, np.int32)
+        hip.synchronize()
+        K = np.empty(nnz.value)
+        f = np.empty(nf.value)
+        if K.size:
+            hip.from_device(K, kp.value)
+        if f.size:
+            hip.from_device(f, fp.value)
+        return r, c, K, f
+
+    def residual_norms(self, step, dx, stream=None):
+        """(|dx|_1, |f_con|_1, |f_con .* dx|_1) / n_free for a host increment dx."""
+        d = hip.DeviceArray.from_host(np.ascontiguousarray(dx, dtype=np.float64))
+        out = (C.c_double * 3)()
+        abi.check(self.L.ife_physics_residual_norms(self.h, step, d.ptr, stream.handle if stream else None, out))
+        return tuple(out)
+
     def get_state(self, step, kind):
         out = np.empty(self.n_rows(step))
         hip.synchronize()

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define IFE_API_VERSION 3
+#define IFE_API_VERSION 4
 
 typedef enum {
     IFE_OK = 0,
@@ -318,6 +318,37 @@ int ife_physics_adjoint(ife_physics* ph, const double* const* lambda, double* g_
  * residual. */
 int ife_physics_spmv(ife_physics* ph, int32_t step, const double* K_vals, const double* x,
                      double* y, void* stream);
+
+/* ---- Dirichlet constraints and residual norms (the consumer of K and f) ----
+ * Constrainer::AddConstraint + Constrainer::Assemble (Physics/Constrainer.cpp:45-121):
+ * `n` (dof, value) pairs for `step` (global dof numbers of ife_physics_dof_numbers; values
+ * are the constrained state values).  A dof given twice with different values is an error,
+ * as in the reference; the same value twice is accepted.  Replaces the step's previous
+ * constraints (n = 0 clears them).  Builds the constrained pattern and allocates its device
+ * buffers: a set-up call, not part of an assembly. */
+int ife_physics_set_constraints(ife_physics* ph, int32_t step, int64_t n, const int64_t* dofs,
+                                const double* values);
+
+/* Physics::Constraint (Physics/physics.cpp:329-352) on the engine-owned K and f of the last
+ * assembly of `step`, on `stream`: K_con = U^T K U and f_con = -U^T (f - K C c), U and C the
+ * selections of the free and constrained dofs (ascending dof order), c_d = value_d - x_d
+ * with x the step's state (IFE_STATE).  K_con keeps K's sorted columns. */
+int ife_physics_constrain(ife_physics* ph, int32_t step, void* stream);
+
+/* The constrained system of a step: n_free rows, its CSR (row_ptr int64, col int32; host
+ * and device copies) and the device K_con values and f_con.  Any pointer may be NULL. */
+int ife_physics_constrained(const ife_physics* ph, int32_t step, int64_t* n_free, int64_t* nnz,
+                            const int64_t** row_ptr_host, const int32_t** col_host,
+                            const int64_t** row_ptr_dev, const int32_t** col_dev, double** K_con,
+                            double** f_con);
+
+/* The residual norms of NonLinSolver::CheckConvergence (Solvers/NonLinSolver.cpp:208-218):
+ * out = (|dx|_1, |f_con|_1, |f_con .* dx|_1) / n_free, with dx the device increment of the
+ * constrained system (n_free doubles) and f_con from the last ife_physics_constrain.
+ * FP64, reproducible (fixed reduction order).  Synchronises `stream`.  The normalisation by
+ * the first iteration's norms and the convergence test stay with the caller's solver. */
+int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, void* stream,
+                               double out[3]);
 
 /* Enable/disable per-launch event timing (off by default). */
 int ife_physics_set_profiling(ife_physics* ph, int32_t enable);

@@ -250,3 +250,46 @@ class Oracle:
         if t < 0:
             raise RuntimeError("oracle: " + self.L.ifo_last_error().decode())
         return t
+
+
+# ---- Dirichlet constraints and residual norms (numpy/scipy restatement) -----------------
+def constraint_matrices(n, dofs, values, x):
+    """Constrainer::Assemble (Physics/Constrainer.cpp:68-121) on one rank: the map of
+    (dof, value) pairs in ascending dof order; U (n x n_free) and C (n x n_con) with a 1 at
+    (dof, running free/constrained index); c_d = value_d - x_d.  Duplicate dofs with
+    different values raise, as Constrainer::AddConstraint (:45-54) does."""
+    import scipy.sparse as sp
+    con = {}
+    for d, v in zip(np.asarray(dofs, dtype=np.int64).tolist(), np.asarray(values, dtype=np.float64).tolist()):
+        if d in con and con[d] != v:
+            raise ValueError(f"Degree of freedom {d} is doubly constrained with different values")
+        con[d] = v
+    cd = np.array(sorted(con), dtype=np.int64)
+    fr = np.setdiff1d(np.arange(n, dtype=np.int64), cd)
+    U = sp.csr_matrix((np.ones(fr.size), (fr, np.arange(fr.size))), shape=(n, fr.size))
+    Cm = sp.csr_matrix((np.ones(cd.size), (cd, np.arange(cd.size))), shape=(n, cd.size))
+    c = np.array([con[d] for d in cd.tolist()], dtype=np.float64) - np.asarray(x, dtype=np.float64)[cd]
+    return U, Cm, c
+
+
+def constrain(row_ptr, col, K, f, x, dofs, values):
+    """Physics::Constraint (Physics/physics.cpp:329-352): K_con = U^T K U (PtAP), and
+    F_c = C c, F_c' = f - K F_c, f_con = -U^T F_c'.  Returns (K_con as scipy CSR, f_con)."""
+    import scipy.sparse as sp
+    n = len(row_ptr) - 1
+    Km = sp.csr_matrix((np.asarray(K, dtype=np.float64), np.asarray(col), np.asarray(row_ptr)), shape=(n, n))
+    U, Cm, c = constraint_matrices(n, dofs, values, x)
+    Kc = (U.T @ Km @ U).tocsr()
+    Fc = -(Cm @ c)
+    Fc2 = Km @ Fc + np.asarray(f, dtype=np.float64)
+    fcon = -(U.T @ Fc2)
+    return Kc, fcon
+
+
+def residual_norms(dx, fcon):
+    """NonLinSolver::CheckConvergence's norms (Solvers/NonLinSolver.cpp:208-218): NORM_1 of
+    dx, f_con and f_con .* dx, each divided by the dof count."""
+    dx = np.asarray(dx, dtype=np.float64)
+    fcon = np.asarray(fcon, dtype=np.float64)
+    n = dx.size
+    return (np.abs(dx).sum() / n, np.abs(fcon).sum() / n, np.abs(fcon * dx).sum() / n)
