@@ -16,6 +16,7 @@
 #ifndef IFE_MODEL_HPP
 #define IFE_MODEL_HPP
 
+#include <array>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -131,6 +132,25 @@ class Physics {
     }
     void AssembleAll(void* stream = nullptr) { check(ife_physics_assemble_all(h_, nullptr, nullptr, stream)); }
     void Kf(int32_t step, double** K, double** f) { check(ife_physics_Kf(h_, step, K, f)); }
+
+    // Constrainer::AddConstraint/Assemble and Physics::Constraint (physics.cpp:329-352):
+    // Dirichlet (dof, value) pairs, then K_con = U^T K U, f_con = -U^T (f - K C c) on the device
+    void SetConstraints(int32_t step, const std::vector<int64_t>& dofs, const std::vector<double>& values) {
+        if (dofs.size() != values.size()) throw Error(IFE_ERR_INVALID, "dofs and values differ in length");
+        check(ife_physics_set_constraints(h_, step, (int64_t)dofs.size(), dofs.data(), values.data()));
+    }
+    void Constraint(int32_t step, void* stream = nullptr) { check(ife_physics_constrain(h_, step, stream)); }
+    // Device CSR of the constrained system and its f_con
+    void Constrained(int32_t step, int64_t* n_free, int64_t* nnz, const int64_t** row_ptr_dev, const int32_t** col_dev,
+                     double** K_con, double** f_con) const {
+        check(ife_physics_constrained(h_, step, n_free, nnz, nullptr, nullptr, row_ptr_dev, col_dev, K_con, f_con));
+    }
+    // NonLinSolver::CheckConvergence's three norms over the free dofs (divided by their count)
+    std::array<double, 3> ResidualNorms(int32_t step, const double* dx_dev, void* stream = nullptr) {
+        std::array<double, 3> e{};
+        check(ife_physics_residual_norms(h_, step, dx_dev, stream, e.data()));
+        return e;
+    }
 
     // Throws like NonLinSolver::CheckConvergence on non-finite results (NonLinSolver.cpp:250-269)
     ife_assembly_status Status(void* stream = nullptr, bool throw_on_nonfinite = true) {

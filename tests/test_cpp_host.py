@@ -79,6 +79,7 @@ def test_cpp_host_driver_matches_oracle(require_gpu, tmp_path, name, nel):
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stderr
     assert "unknown model rejected" in r.stdout
+    assert "doubly constrained dof rejected" in r.stdout and "constraints ok" in r.stdout
     res = read_output(pout)
     for s, (K, f) in enumerate(res):
         Ko, fo = o.assemble(s)
