@@ -9,7 +9,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "_build", "libife.so")
 
 IFE_OK = 0
-IFE_STATE, IFE_STATE_OLD, IFE_DSTATE, IFE_DDSTATE = 0, 1, 2, 3
+IFE_STATE, IFE_STATE_OLD, IFE_DSTATE, IFE_DDSTATE, IFE_DSTATE_OLD, IFE_DDSTATE_OLD = 0, 1, 2, 3, 4, 5
+IFE_TS_NEWMARK, IFE_TS_ALPHA, IFE_TS_EULER, IFE_TS_BDF2 = 0, 1, 2, 3
 IFE_PREC_FP64, IFE_PREC_MIXED = 0, 1
 
 # enums, same order as the reference's (see include/ife.h)
@@ -120,6 +121,10 @@ _PROTOS = [
                                           C.POINTER(C.POINTER(C.c_int64)), C.POINTER(C.POINTER(C.c_int32)),
                                           C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     ("ife_physics_residual_norms", C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(C.c_double)]),
+    ("ife_physics_set_time_scheme", C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
+    ("ife_physics_update_state", C.c_int, [_P, C.c_int32, _P, _P]),
+    ("ife_physics_commit_state", C.c_int, [_P, _P]),
+    ("ife_physics_reset_state", C.c_int, [_P, _P]),
     ("ife_model_set_element_params", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("ife_model_set_param", C.c_int, [_P, C.c_int32, C.c_char_p, C.c_double]),
     ("ife_batch_create", C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),

@@ -138,7 +138,7 @@ struct Step {
     std::vector<int32_t> col;
     DevBuf<int64_t> d_row_ptr;
     DevBuf<int32_t> d_col;
-    DevBuf<double> K, f, x[4];
+    DevBuf<double> K, f, x[6];  // ife_state_kind order
     // Dirichlet constraints (Constrainer): constrained dofs ascending with their values, free
     // dofs ascending, the constrained pattern with each entry's offset in its source row
     bool has_cons = false;
@@ -192,6 +192,8 @@ struct ife_physics {
     DevBuf<Status> d_status;
     // time / precision
     double t = 0, dt = 1, du_dt = 1, ddu_dt = 1;
+    int ts = IFE_TS_EULER;  // time scheme of UpdateVelAcc and its constants
+    double ts_c0 = 1.0, ts_c1 = 0.0;
     int precision = IFE_PREC_FP64;
     bool last_had_phase = false;
     // launch timing
@@ -521,7 +523,7 @@ void setup_device(ife_physics* ph) {
         st.f.alloc(std::max<int64_t>(st.nrows, 1));
         HIPCHK(hipMemset(st.K.p, 0, st.K.n * sizeof(double)));
         HIPCHK(hipMemset(st.f.p, 0, st.f.n * sizeof(double)));
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < 6; k++) {
             st.x[k].alloc(std::max<int64_t>(st.nrows, 1));
             HIPCHK(hipMemset(st.x[k].p, 0, st.x[k].n * sizeof(double)));
         }
@@ -1189,7 +1191,7 @@ int ife_physics_coloring(const ife_physics* ph, int32_t* n_colors, const int64_t
 int ife_physics_state(ife_physics* ph, int32_t step, int32_t kind, double** dev_ptr) {
     return guard([&] {
         require_ready(ph);
-        if (step < 0 || step >= ph->nsteps || kind < 0 || kind > 3 || !dev_ptr) THROW(IFE_ERR_INVALID, "bad argument");
+        if (step < 0 || step >= ph->nsteps || kind < 0 || kind > IFE_DDSTATE_OLD || !dev_ptr) THROW(IFE_ERR_INVALID, "bad argument");
         *dev_ptr = ph->steps[step].x[kind].p;
     });
 }
@@ -1372,6 +1374,70 @@ int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, 
         HIPCHK(launch_norms(dx, st.d_fcon.p, st.nfree, st.d_part.p, st.d_norms.p, s));
         HIPCHK(hipMemcpyAsync(out, st.d_norms.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int ife_physics_set_time_scheme(ife_physics* ph, int32_t scheme, double c0, double c1) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        if (scheme < IFE_TS_NEWMARK || scheme > IFE_TS_BDF2) THROW(IFE_ERR_INVALID, "Time Discretisation type not defined");
+        if (scheme == IFE_TS_NEWMARK && !(c0 != 0.0)) THROW(IFE_ERR_INVALID, "Newmark beta must be non-zero");
+        if (scheme == IFE_TS_ALPHA && !(c0 != 0.0)) THROW(IFE_ERR_INVALID, "alpha must be non-zero");
+        ph->ts = scheme;
+        ph->ts_c0 = scheme == IFE_TS_EULER ? 1.0 : c0;
+        ph->ts_c1 = c1;
+    });
+}
+
+static void vel_acc_all(ife_physics* ph, hipStream_t s) {
+    for (int q = 0; q < ph->nsteps; q++) {
+        Step& st = ph->steps[q];
+        HIPCHK(launch_vel_acc(ph->ts, ph->ts_c0, ph->ts_c1, ph->dt, st.x[IFE_STATE].p, st.x[IFE_STATE_OLD].p,
+                              st.x[IFE_DSTATE_OLD].p, st.x[IFE_DDSTATE_OLD].p, st.x[IFE_DSTATE].p, st.x[IFE_DDSTATE].p,
+                              st.nrows, s));
+    }
+}
+
+int ife_physics_update_state(ife_physics* ph, int32_t step, const double* dx, void* stream) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (!dx) THROW(IFE_ERR_INVALID, "null increment");
+        Step& st = ph->steps[step];
+        hipStream_t s = (hipStream_t)stream;
+        if (st.has_cons)
+            HIPCHK(launch_update_state(st.x[IFE_STATE].p, dx, st.d_free.p, st.nfree, st.d_cdofs.p, st.d_g.p,
+                                       (int64_t)st.d_cdofs.n, s));
+        else
+            HIPCHK(launch_update_state(st.x[IFE_STATE].p, dx, nullptr, st.nrows, nullptr, nullptr, 0, s));
+        vel_acc_all(ph, s);
+    });
+}
+
+int ife_physics_commit_state(ife_physics* ph, void* stream) {
+    return guard([&] {
+        require_ready(ph);
+        hipStream_t s = (hipStream_t)stream;
+        vel_acc_all(ph, s);
+        for (int q = 0; q < ph->nsteps; q++) {
+            Step& st = ph->steps[q];
+            HIPCHK(launch_copy(st.x[IFE_STATE_OLD].p, st.x[IFE_STATE].p, st.nrows, s));
+            HIPCHK(launch_copy(st.x[IFE_DSTATE_OLD].p, st.x[IFE_DSTATE].p, st.nrows, s));
+            HIPCHK(launch_copy(st.x[IFE_DDSTATE_OLD].p, st.x[IFE_DDSTATE].p, st.nrows, s));
+        }
+    });
+}
+
+int ife_physics_reset_state(ife_physics* ph, void* stream) {
+    return guard([&] {
+        require_ready(ph);
+        hipStream_t s = (hipStream_t)stream;
+        for (int q = 0; q < ph->nsteps; q++) {
+            Step& st = ph->steps[q];
+            HIPCHK(launch_copy(st.x[IFE_STATE].p, st.x[IFE_STATE_OLD].p, st.nrows, s));
+            HIPCHK(launch_copy(st.x[IFE_DSTATE].p, st.x[IFE_DSTATE_OLD].p, st.nrows, s));
+            HIPCHK(launch_copy(st.x[IFE_DDSTATE].p, st.x[IFE_DDSTATE_OLD].p, st.nrows, s));
+        }
     });
 }
 

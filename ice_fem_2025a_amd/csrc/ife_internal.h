@@ -176,6 +176,12 @@ hipError_t launch_constrain(const ConsArgs& a, hipStream_t s);
 // sum |dx| / n, sum |fc| / n, sum |fc dx| / n into out[3] (device); part: norms_partials() x 3
 hipError_t launch_norms(const double* dx, const double* fc, int64_t n, double* part, double* out, hipStream_t s);
 int norms_partials();
+// x[free[k]] += dx[k] (free NULL: x[k] += dx[k] over n), then x[cd[i]] -= g[cd[i]] (g = -C c)
+hipError_t launch_update_state(double* x, const double* dx, const int64_t* free, int64_t n, const int64_t* cd,
+                               const double* g, int64_t nc, hipStream_t s);
+// TimeScheme::UpdateVelAcc on n entries: v, a from x, x_old, v_old, a_old
+hipError_t launch_vel_acc(int scheme, double c0, double c1, double dt, const double* x, const double* xo, const double* vo,
+                          const double* ao, double* v, double* a, int64_t n, hipStream_t s);
 hipError_t launch_gather(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s);
 // Cosserat: material update of every IP (u, phase, T coefficients), then the u-step element

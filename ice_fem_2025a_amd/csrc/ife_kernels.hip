@@ -4091,6 +4091,58 @@ hipError_t launch_norms(const double* dx, const double* fc, int64_t n, double* p
 
 int norms_partials() { return NORM_NB; }
 
+// Physics::UpdateState (physics.cpp:408-411): x += U dx, then + C c at the constrained dofs
+__global__ void update_free_kernel(double* x, const double* dx, const int64_t* fr, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[fr ? fr[k] : k] += dx[k];
+}
+__global__ void update_cons_kernel(double* x, const int64_t* cd, const double* g, int64_t nc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nc) x[cd[i]] -= g[cd[i]];
+}
+
+hipError_t launch_update_state(double* x, const double* dx, const int64_t* free, int64_t n, const int64_t* cd,
+                               const double* g, int64_t nc, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(update_free_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, dx, free, n);
+    if (nc > 0) hipLaunchKernelGGL(update_cons_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, x, cd, g, nc);
+    return hipGetLastError();
+}
+
+// TimeScheme::UpdateVelAcc (TimeSchemes.cpp:36-62), each VecAXPBYPCZ / VecAYPX of the
+// reference as one expression per entry, in its order
+__global__ void vel_acc_kernel(int scheme, double c0, double c1, double dt, const double* x, const double* xo,
+                               const double* vo, const double* ao, double* v, double* a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double X = x[i], XO = xo[i], VO = vo[i], AO = ao[i];
+    double A, V;
+    if (scheme == IFE_TS_NEWMARK) {
+        const double be = c0, ga = c1;
+        A = 1.0 / (dt * dt * be) * X + (-1.0 / (dt * dt * be)) * XO;
+        A = -1.0 / (be * dt) * VO + (-(1.0 / (2.0 * be) - 1.0)) * AO + A;
+        V = ga / (dt * be) * X + (-ga / (dt * be)) * XO;
+        V = (-(ga / be - 1.0)) * VO + (-(dt * ga / (2.0 * be) - dt)) * AO + V;
+    } else if (scheme == IFE_TS_BDF2) {
+        A = 1.0 / (dt * dt) * X + (-1.0 / (dt * dt)) * XO;
+        V = 1.0 / dt * X + (-1.0 / dt) * XO;
+    } else {  // Alpha (Euler: alpha = 1)
+        const double al = scheme == IFE_TS_EULER ? 1.0 : c0;
+        A = 0.0;
+        V = VO;
+        V = 1.0 / (dt * al) * X + (-1.0 / (dt * al)) * XO + (1.0 - 1.0 / al) * V;
+    }
+    v[i] = V;
+    a[i] = A;
+}
+
+hipError_t launch_vel_acc(int scheme, double c0, double c1, double dt, const double* x, const double* xo, const double* vo,
+                          const double* ao, double* v, double* a, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(vel_acc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, scheme, c0, c1, dt, x, xo, vo, ao,
+                       v, a, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     int64_t blocks = (n + 255) / 256;

@@ -255,6 +255,23 @@ class Physics:
         abi.check(self.L.ife_physics_residual_norms(self.h, step, d.ptr, stream.handle if stream else None, out))
         return tuple(out)
 
+    # ---- state updates (Physics::UpdateState, UpdateVelAcc, Commit, ResetStep) ----------
+    def set_time_scheme(self, scheme, c0=0.0, c1=0.0):
+        abi.check(self.L.ife_physics_set_time_scheme(self.h, scheme, c0, c1))
+
+    def update_state(self, step, dx, stream=None):
+        """x += U dx + C c for `step` (dx over the free dofs when constraints are set), then
+        velocities and accelerations of every step."""
+        d = hip.DeviceArray.from_host(np.ascontiguousarray(dx, dtype=np.float64))
+        abi.check(self.L.ife_physics_update_state(self.h, step, d.ptr, stream.handle if stream else None))
+        hip.synchronize()
+
+    def commit_state(self, stream=None):
+        abi.check(self.L.ife_physics_commit_state(self.h, stream.handle if stream else None))
+
+    def reset_state(self, stream=None):
+        abi.check(self.L.ife_physics_reset_state(self.h, stream.handle if stream else None))
+
     def get_state(self, step, kind):
         out = np.empty(self.n_rows(step))
         hip.synchronize()

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define IFE_API_VERSION 4
+#define IFE_API_VERSION 5
 
 typedef enum {
     IFE_OK = 0,
@@ -69,7 +69,15 @@ typedef enum { IFE_IRR_NONE = 0, IFE_IRR_HIST, IFE_IRR_PENALTY, IFE_IRR_LAGRANGE
 
 /* Which state vector of a step (Physics::StateVectors/StateVectorsOld/dStateVectors/
  * ddStateVectors, physics.h:34). */
-typedef enum { IFE_STATE = 0, IFE_STATE_OLD = 1, IFE_DSTATE = 2, IFE_DDSTATE = 3 } ife_state_kind;
+/* StateVectors, StateVectorsOld, dStateVectors, ddStateVectors, dStateVectorsOld,
+ * ddStateVectorsOld (Physics/physics.h:34-35) */
+typedef enum {
+    IFE_STATE = 0, IFE_STATE_OLD = 1, IFE_DSTATE = 2, IFE_DDSTATE = 3, IFE_DSTATE_OLD = 4, IFE_DDSTATE_OLD = 5
+} ife_state_kind;
+
+/* TimeScheme::DiscType (Physics/TimeSchemes.cpp:5-21): Newmark (c0 = beta, c1 = gamma),
+ * Alpha (c0 = alpha), Euler (= Alpha with alpha 1), BDF2. */
+typedef enum { IFE_TS_NEWMARK = 0, IFE_TS_ALPHA = 1, IFE_TS_EULER = 2, IFE_TS_BDF2 = 3 } ife_time_scheme;
 
 /* Arithmetic of the element contractions. */
 typedef enum {
@@ -349,6 +357,26 @@ int ife_physics_constrained(const ife_physics* ph, int32_t step, int64_t* n_free
  * the first iteration's norms and the convergence test stay with the caller's solver. */
 int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, void* stream,
                                double out[3]);
+
+/* ---- State updates on the device (a Newton iteration without host round trips) ----
+ * TimeScheme(inputs) (Physics/TimeSchemes.cpp:5-21): the scheme UpdateVelAcc uses with the
+ * dt of ife_physics_set_time.  Default: Euler. */
+int ife_physics_set_time_scheme(ife_physics* ph, int32_t scheme, double c0, double c1);
+
+/* Physics::UpdateState (Physics/physics.cpp:406-419): x += U dx + C c for `step` (dx: device,
+ * n_free doubles, with c from the last ife_physics_constrain; without constraints dx has
+ * n_rows doubles and x += dx), then Physics::UpdateVelAcc (physics.cpp:484-497,
+ * TimeSchemes.cpp:36-62) for every step. */
+int ife_physics_update_state(ife_physics* ph, int32_t step, const double* dx, void* stream);
+
+/* The state part of Physics::Commit(TIMEDEP) (physics.cpp:501-527): UpdateVelAcc, then the
+ * old vectors (x, v, a) take the current ones, every step.  Model history commits stay with
+ * ife_model_commit. */
+int ife_physics_commit_state(ife_physics* ph, void* stream);
+
+/* The state part of Physics::ResetStep (physics.cpp:393-404): current (x, v, a) from the old
+ * vectors, every step.  Model history resets stay with ife_model_reset_step. */
+int ife_physics_reset_state(ife_physics* ph, void* stream);
 
 /* Enable/disable per-launch event timing (off by default). */
 int ife_physics_set_profiling(ife_physics* ph, int32_t enable);

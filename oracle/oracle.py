@@ -293,3 +293,35 @@ def residual_norms(dx, fcon):
     fcon = np.asarray(fcon, dtype=np.float64)
     n = dx.size
     return (np.abs(dx).sum() / n, np.abs(fcon).sum() / n, np.abs(fcon * dx).sum() / n)
+
+
+# ---- state updates (numpy restatement) ---------------------------------------------------
+def update_vel_acc(scheme, c0, c1, dt, x, xo, vo, ao):
+    """TimeScheme::UpdateVelAcc (Physics/TimeSchemes.cpp:36-62): velocity and acceleration
+    from the current and old state.  scheme: 0 Newmark (c0 beta, c1 gamma), 1 Alpha (c0
+    alpha), 2 Euler (alpha 1), 3 BDF2.  Returns (v, a)."""
+    x, xo, vo, ao = (np.asarray(t, dtype=np.float64) for t in (x, xo, vo, ao))
+    if scheme == 0:
+        be, ga = c0, c1
+        a = (x - xo) / (be * dt * dt) - vo / (be * dt) - (1.0 / (2.0 * be) - 1.0) * ao
+        v = ga / (be * dt) * (x - xo) - (ga / be - 1.0) * vo - (dt * ga / (2.0 * be) - dt) * ao
+    elif scheme == 3:
+        a = (x - xo) / (dt * dt)
+        v = (x - xo) / dt
+    else:
+        al = 1.0 if scheme == 2 else c0
+        a = np.zeros_like(x)
+        v = (x - xo) / (al * dt) + (1.0 - 1.0 / al) * vo
+    return v, a
+
+
+def update_state(x, dx, dofs=None, values=None, x_at_constrain=None):
+    """Physics::UpdateState (Physics/physics.cpp:408-411): x + U dx + C c, with c = value -
+    x as Constrainer::Assemble formed it (the state at the last constraint elimination).
+    Without constraints U is the identity."""
+    x = np.array(x, dtype=np.float64)
+    dx = np.asarray(dx, dtype=np.float64)
+    if dofs is None or len(dofs) == 0:
+        return x + dx
+    U, Cm, c = constraint_matrices(x.size, dofs, values, x if x_at_constrain is None else x_at_constrain)
+    return x + U @ dx + Cm @ c

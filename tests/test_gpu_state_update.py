@@ -1,0 +1,108 @@
+"""GPU state updates against the oracle's restatement: Physics::UpdateState (x += U dx + C c,
+physics.cpp:408-411) followed by UpdateVelAcc for every step (TimeSchemes.cpp:36-62) in each
+time scheme, the state part of Physics::Commit(TIMEDEP) (physics.cpp:501-527) and of
+Physics::ResetStep (physics.cpp:393-404).  Seeded random states of every kind; within
+1e-14 of the oracle (the device evaluates the reference's VecAXPBYPCZ sequence, the oracle
+the closed form)."""
+import sys
+
+import numpy as np
+import pytest
+
+from ice_fem_2025a_amd import abi
+from ice_fem_2025a_amd.config import make_problem
+from ice_fem_2025a_amd.engine import Physics
+from oracle.oracle import update_state, update_vel_acc
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [abi.IFE_STATE, abi.IFE_STATE_OLD, abi.IFE_DSTATE, abi.IFE_DDSTATE, abi.IFE_DSTATE_OLD, abi.IFE_DDSTATE_OLD]
+SCHEMES = [(abi.IFE_TS_NEWMARK, 0.25, 0.5), (abi.IFE_TS_NEWMARK, 0.3025, 0.6), (abi.IFE_TS_ALPHA, 0.7, 0.0),
+           (abi.IFE_TS_EULER, 0.0, 0.0), (abi.IFE_TS_BDF2, 0.0, 0.0)]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _close(a, b, tol=1e-14):
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() <= tol * scale
+
+
+def _states(e, seed):
+    rng = np.random.default_rng(seed)
+    st = {}
+    for s in range(e.nsteps):
+        for k in KINDS:
+            v = rng.normal(size=e.n_rows(s)) * (1e-3 if k in (abi.IFE_STATE, abi.IFE_STATE_OLD) else 1.0)
+            e.set_state(s, k, v)
+            st[(s, k)] = v
+    return st
+
+
+@pytest.mark.parametrize("scheme,c0,c1", SCHEMES)
+@pytest.mark.parametrize("name,nel", [("quad2d_thermal", (5, 4)), ("hex3d_glen_at2", (2, 2, 2))])
+def test_update_state_and_velocities(require_gpu, name, nel, scheme, c0, c1):
+    _say(f"[state update] {name} scheme {scheme} ({c0}, {c1})")
+    prob = make_problem(name, nel=nel)
+    e = Physics(prob)
+    e.set_time_scheme(scheme, c0, c1)
+    dt = 0.37
+    e.set_time(1.0, dt, prob.du_dt, prob.ddu_dt)
+    st = _states(e, seed=11 + scheme)
+    e.assemble(0)
+    rng = np.random.default_rng(5)
+    n0 = e.n_rows(0)
+    dofs = rng.choice(n0, n0 // 7, replace=False)
+    vals = st[(0, abi.IFE_STATE)][dofs] + rng.normal(scale=1e-4, size=dofs.size)
+    e.set_constraints(0, dofs, vals)
+    e.constrain(0)
+    dx = rng.normal(size=n0 - dofs.size) * 1e-4
+    e.update_state(0, dx)
+    x_ref = update_state(st[(0, abi.IFE_STATE)], dx, dofs, vals)
+    x_got = e.get_state(0, abi.IFE_STATE)
+    assert _close(x_got, x_ref)
+    np.testing.assert_allclose(x_got[dofs], vals, rtol=0, atol=1e-18 + 1e-15 * np.abs(vals).max())
+    for s in range(e.nsteps):
+        x = x_got if s == 0 else st[(s, abi.IFE_STATE)]
+        v_ref, a_ref = update_vel_acc(scheme, c0, c1, dt, x, st[(s, abi.IFE_STATE_OLD)], st[(s, abi.IFE_DSTATE_OLD)],
+                                      st[(s, abi.IFE_DDSTATE_OLD)])
+        v, a = e.get_state(s, abi.IFE_DSTATE), e.get_state(s, abi.IFE_DDSTATE)
+        assert _close(v, v_ref, 1e-13), f"v step {s}"
+        if a_ref.any():
+            assert _close(a, a_ref, 1e-13), f"a step {s}"
+        else:
+            assert not a.any()
+    # a step without constraints: x += dx over all its dofs
+    if e.nsteps > 1:
+        x1 = e.get_state(1, abi.IFE_STATE)
+        d1 = rng.normal(size=e.n_rows(1))
+        e.update_state(1, d1)
+        np.testing.assert_array_equal(e.get_state(1, abi.IFE_STATE), x1 + d1)
+
+
+def test_commit_and_reset_state(require_gpu):
+    prob = make_problem("quad2d_thermal", nel=(4, 3))
+    e = Physics(prob)
+    e.set_time_scheme(abi.IFE_TS_NEWMARK, 0.25, 0.5)
+    st = _states(e, seed=21)
+    e.commit_state()
+    for s in range(e.nsteps):
+        x, xo = e.get_state(s, abi.IFE_STATE), e.get_state(s, abi.IFE_STATE_OLD)
+        np.testing.assert_array_equal(x, st[(s, abi.IFE_STATE)])
+        np.testing.assert_array_equal(xo, x)
+        for cur, old in ((abi.IFE_DSTATE, abi.IFE_DSTATE_OLD), (abi.IFE_DDSTATE, abi.IFE_DDSTATE_OLD)):
+            np.testing.assert_array_equal(e.get_state(s, old), e.get_state(s, cur))
+    rng = np.random.default_rng(9)
+    for s in range(e.nsteps):
+        for k in (abi.IFE_STATE, abi.IFE_DSTATE, abi.IFE_DDSTATE):
+            e.set_state(s, k, rng.normal(size=e.n_rows(s)))
+    e.reset_state()
+    for s in range(e.nsteps):
+        for cur, old in ((abi.IFE_STATE, abi.IFE_STATE_OLD), (abi.IFE_DSTATE, abi.IFE_DSTATE_OLD),
+                         (abi.IFE_DDSTATE, abi.IFE_DDSTATE_OLD)):
+            np.testing.assert_array_equal(e.get_state(s, cur), e.get_state(s, old))
+    with pytest.raises(abi.IfeError, match="not defined"):
+        e.set_time_scheme(7, 0.0, 0.0)
