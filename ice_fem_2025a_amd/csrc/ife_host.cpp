@@ -1389,6 +1389,18 @@ int ife_physics_set_time_scheme(ife_physics* ph, int32_t scheme, double c0, doub
     });
 }
 
+int ife_physics_set_dt(ife_physics* ph, double t, double dt) {
+    return guard([&] {
+        if (!ph) THROW(IFE_ERR_INVALID, "null physics");
+        if (!(dt > 0)) THROW(IFE_ERR_INVALID, "dt must be positive");
+        double du, ddu;
+        if (ph->ts == IFE_TS_NEWMARK) { du = ph->ts_c1 / (dt * ph->ts_c0); ddu = 1.0 / (dt * dt * ph->ts_c0); }
+        else if (ph->ts == IFE_TS_BDF2) { du = 1.0 / dt; ddu = 1.0 / (dt * dt); }
+        else { du = 1.0 / (dt * ph->ts_c0); ddu = 0.0; }
+        ph->t = t; ph->dt = dt; ph->du_dt = du; ph->ddu_dt = ddu;
+    });
+}
+
 static void vel_acc_all(ife_physics* ph, hipStream_t s) {
     for (int q = 0; q < ph->nsteps; q++) {
         Step& st = ph->steps[q];

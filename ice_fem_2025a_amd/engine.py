@@ -259,6 +259,10 @@ class Physics:
     def set_time_scheme(self, scheme, c0=0.0, c1=0.0):
         abi.check(self.L.ife_physics_set_time_scheme(self.h, scheme, c0, c1))
 
+    def set_dt(self, t, dt):
+        """TimeScheme::Set_dt: t, dt and the scheme's du_dt, ddu_dt."""
+        abi.check(self.L.ife_physics_set_dt(self.h, t, dt))
+
     def update_state(self, step, dx, stream=None):
         """x += U dx + C c for `step` (dx over the free dofs when constraints are set), then
         velocities and accelerations of every step."""

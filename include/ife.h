@@ -363,6 +363,11 @@ int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, 
  * dt of ife_physics_set_time.  Default: Euler. */
 int ife_physics_set_time_scheme(ife_physics* ph, int32_t scheme, double c0, double c1);
 
+/* TimeScheme::Set_dt (Physics/TimeSchemes.cpp:65-76): time t and increment dt, with the
+ * assembly coefficients du_dt, ddu_dt of the scheme (Newmark gamma/(beta dt), 1/(beta dt^2);
+ * Alpha/Euler 1/(alpha dt), 0; BDF2 1/dt, 1/dt^2) -- ife_physics_set_time with them. */
+int ife_physics_set_dt(ife_physics* ph, double t, double dt);
+
 /* Physics::UpdateState (Physics/physics.cpp:406-419): x += U dx + C c for `step` (dx: device,
  * n_free doubles, with c from the last ife_physics_constrain; without constraints dx has
  * n_rows doubles and x += dx), then Physics::UpdateVelAcc (physics.cpp:484-497,

@@ -145,6 +145,16 @@ class Physics {
                      double** K_con, double** f_con) const {
         check(ife_physics_constrained(h_, step, n_free, nnz, nullptr, nullptr, row_ptr_dev, col_dev, K_con, f_con));
     }
+    // TimeScheme and the state vectors: Set_dt, UpdateState + UpdateVelAcc, Commit/ResetStep
+    void SetTimeScheme(ife_time_scheme scheme, double c0 = 0.0, double c1 = 0.0) {
+        check(ife_physics_set_time_scheme(h_, scheme, c0, c1));
+    }
+    void SetDt(double t, double dt) { check(ife_physics_set_dt(h_, t, dt)); }
+    void UpdateState(int32_t step, const double* dx_dev, void* stream = nullptr) {
+        check(ife_physics_update_state(h_, step, dx_dev, stream));
+    }
+    void CommitState(void* stream = nullptr) { check(ife_physics_commit_state(h_, stream)); }
+    void ResetState(void* stream = nullptr) { check(ife_physics_reset_state(h_, stream)); }
     // NonLinSolver::CheckConvergence's three norms over the free dofs (divided by their count)
     std::array<double, 3> ResidualNorms(int32_t step, const double* dx_dev, void* stream = nullptr) {
         std::array<double, 3> e{};
