@@ -5,7 +5,7 @@ The key paths are the reference's own (inputData::GetRequired/GetOptional, e.g.
 properties.<Mat>.SolidProperties.Young, SolidMaterial.cpp:6-7; Models.<name>.Material,
 PhaseField.cpp:191), so an IceFEM input file's material/model blocks translate 1:1.
 
-Workloads (the BASELINE configurations, named by what they are):
+Workloads (the benchmark configurations, named by what they are):
   "quad2d_le_at2"   64x32 B2_2 quads on 0.1x0.05 m, u(2)+phi, linear elastic + AT2
   "hex3d_glen_at2"  40x40x80 NURBS3_Cube2 hexes, u(3)+phi+T, Glen viscous + AT2 + thermal
   "hex3d_vep_at2"   16x16x16 hexes, as hex3d_glen_at2 with the visco-elasto-plastic rheology
