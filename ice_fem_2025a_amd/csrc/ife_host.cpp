@@ -1363,6 +1363,17 @@ int ife_physics_constrained(const ife_physics* ph, int32_t step, int64_t* n_free
     });
 }
 
+int ife_physics_residual_norms_async(ife_physics* ph, int32_t step, const double* dx, double* out_dev, void* stream) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (!dx || !out_dev) THROW(IFE_ERR_INVALID, "null argument");
+        Step& st = ph->steps[step];
+        if (!st.has_cons) THROW(IFE_ERR_STATE, "no constraints set for this step (ife_physics_set_constraints)");
+        HIPCHK(launch_norms(dx, st.d_fcon.p, st.nfree, st.d_part.p, out_dev, (hipStream_t)stream));
+    });
+}
+
 int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, void* stream, double out[3]) {
     return guard([&] {
         require_ready(ph);

@@ -270,6 +270,15 @@ class Physics:
         abi.check(self.L.ife_physics_update_state(self.h, step, d.ptr, stream.handle if stream else None))
         hip.synchronize()
 
+    def update_state_ptr(self, step, dx_ptr, stream=None):
+        """update_state with a device increment (no host copy, no synchronisation)."""
+        abi.check(self.L.ife_physics_update_state(self.h, step, dx_ptr, stream.handle if stream else None))
+
+    def residual_norms_async(self, step, dx_ptr, out_ptr, stream=None):
+        """The residual norms into device memory (3 doubles), stream-ordered."""
+        abi.check(self.L.ife_physics_residual_norms_async(self.h, step, dx_ptr, out_ptr,
+                                                          stream.handle if stream else None))
+
     def commit_state(self, stream=None):
         abi.check(self.L.ife_physics_commit_state(self.h, stream.handle if stream else None))
 

@@ -125,6 +125,56 @@ class Stream:
         _chk(lib().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
 
 
+class Graph:
+    """Stream capture into a hipGraph: `with g.capture(stream): ...` records the work issued
+    on `stream`; launch() replays it."""
+
+    def __init__(self):
+        h = lib()
+        h.hipStreamBeginCapture.argtypes = [C.c_void_p, C.c_int]
+        h.hipStreamBeginCapture.restype = C.c_int
+        h.hipStreamEndCapture.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        h.hipStreamEndCapture.restype = C.c_int
+        h.hipGraphInstantiate.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t]
+        h.hipGraphInstantiate.restype = C.c_int
+        h.hipGraphLaunch.argtypes = [C.c_void_p, C.c_void_p]
+        h.hipGraphLaunch.restype = C.c_int
+        h.hipGraphExecDestroy.argtypes = [C.c_void_p]
+        h.hipGraphDestroy.argtypes = [C.c_void_p]
+        self.graph, self.exec = C.c_void_p(), C.c_void_p()
+
+    class _Capture:
+        def __init__(self, g, stream):
+            self.g, self.stream = g, stream
+
+        def __enter__(self):
+            # relaxed mode: a host-side free by the garbage collector does not void the capture
+            _chk(lib().hipStreamBeginCapture(self.stream.handle, 2), "hipStreamBeginCapture")
+            return self.g
+
+        def __exit__(self, et, ev, tb):
+            rc = lib().hipStreamEndCapture(self.stream.handle, C.byref(self.g.graph))
+            if et is None:
+                _chk(rc, "hipStreamEndCapture")
+                _chk(lib().hipGraphInstantiate(C.byref(self.g.exec), self.g.graph, None, None, 0), "hipGraphInstantiate")
+            return False
+
+    def capture(self, stream):
+        return Graph._Capture(self, stream)
+
+    def launch(self, stream):
+        _chk(lib().hipGraphLaunch(self.exec, stream.handle), "hipGraphLaunch")
+
+    def __del__(self):
+        try:
+            if self.exec:
+                lib().hipGraphExecDestroy(self.exec)
+            if self.graph:
+                lib().hipGraphDestroy(self.graph)
+        except Exception:
+            pass
+
+
 class Event:
     def __init__(self):
         e = C.c_void_p()

@@ -358,6 +358,12 @@ int ife_physics_constrained(const ife_physics* ph, int32_t step, int64_t* n_free
 int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, void* stream,
                                double out[3]);
 
+/* The same norms written to device memory (out_dev: 3 doubles), stream-ordered and without
+ * synchronisation, so that a Newton iteration (update, assemble, constrain, norms) can be
+ * captured in a hipGraph. */
+int ife_physics_residual_norms_async(ife_physics* ph, int32_t step, const double* dx, double* out_dev,
+                                     void* stream);
+
 /* ---- State updates on the device (a Newton iteration without host round trips) ----
  * TimeScheme(inputs) (Physics/TimeSchemes.cpp:5-21): the scheme UpdateVelAcc uses with the
  * dt of ife_physics_set_time.  Default: Euler. */
