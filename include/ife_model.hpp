@@ -156,6 +156,10 @@ class Physics {
     void CommitState(void* stream = nullptr) { check(ife_physics_commit_state(h_, stream)); }
     void ResetState(void* stream = nullptr) { check(ife_physics_reset_state(h_, stream)); }
     // NonLinSolver::CheckConvergence's three norms over the free dofs (divided by their count)
+    // ... into device memory, stream-ordered (capturable in a hipGraph)
+    void ResidualNormsAsync(int32_t step, const double* dx_dev, double* out_dev, void* stream = nullptr) {
+        check(ife_physics_residual_norms_async(h_, step, dx_dev, out_dev, stream));
+    }
     std::array<double, 3> ResidualNorms(int32_t step, const double* dx_dev, void* stream = nullptr) {
         std::array<double, 3> e{};
         check(ife_physics_residual_norms(h_, step, dx_dev, stream, e.data()));
