@@ -106,8 +106,9 @@ def measure_constraints(ph, prob, stream, reps=10):
         ph.set_constraints(s, dofs, ph.get_state(s, abi.IFE_STATE)[dofs])
         rp, col, _, fc = ph.constrained(s)
         nnz, n = ph.nnz(s), ph.n_rows(s)
-        # spmv over K (values, columns, x, y), K_con gather (offset, value, write), f_con
-        nbytes += 12 * nnz + 16 * n + 20 * col.size + 32 * fc.size
+        # K_con gather (offset, value, write); f_con per free row (row index, f, list bounds,
+        # write); entries in constrained columns (offset, value, column, g: at most nnz - nnz_con)
+        nbytes += 20 * col.size + 40 * fc.size + 24 * (nnz - col.size)
         ncon += dofs.size
     for s in range(ph.nsteps):
         ph.constrain(s, stream)
@@ -236,11 +237,11 @@ def bench_batch(args):
         from oracle.oracle import Oracle
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         ne = secs = 0.0
-        for p in probs:  # whole domains until ~20k elements
+        for p in probs:  # whole domains until ~400k elements (about 10 s of CPU work)
             o = Oracle(p)
             secs += o.time_assembly(list(range(o.nsteps)), 0, p.mesh.n_elems, threads)
             ne += p.mesh.n_elems
-            if ne >= 20000:
+            if ne >= 400000:
                 break
         out["cpu_baseline"] = {"value": ne / secs, "unit": "elements/s", "cores": threads, "kind": "port",
                                "sample": f"{int(ne)} elements (the first domains of the batch), all three steps, "

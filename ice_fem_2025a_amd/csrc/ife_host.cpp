@@ -145,9 +145,9 @@ struct Step {
     int64_t nfree = 0;
     std::vector<int64_t> c_row_ptr;
     std::vector<int32_t> c_col;
-    DevBuf<int64_t> d_cdofs, d_free, d_crp;
-    DevBuf<int32_t> d_ccol, d_rel;
-    DevBuf<double> d_cvals, d_g, d_y, d_kcon, d_fcon, d_part, d_norms;
+    DevBuf<int64_t> d_cdofs, d_free, d_crp, d_cfp;
+    DevBuf<int32_t> d_ccol, d_rel, d_cfr;
+    DevBuf<double> d_cvals, d_g, d_kcon, d_fcon, d_part, d_norms;
 };
 
 }  // namespace
@@ -1274,21 +1274,28 @@ int ife_physics_set_constraints(ife_physics* ph, int32_t step, int64_t n, const 
             }
         }
         // K_con = U^T K U: the free rows, their free columns renumbered (monotone, so sorted)
-        std::vector<int64_t> crp(nfree + 1, 0);
+        // and per free row the entries in constrained columns (the K C c part of f_con)
+        std::vector<int64_t> crp(nfree + 1, 0), cfp(nfree + 1, 0);
         for (int64_t k = 0; k < nfree; k++) {
             const int64_t r = fr[k];
             int64_t c = 0;
             for (int64_t q = st.row_ptr[r]; q < st.row_ptr[r + 1]; q++) c += fmap[st.col[q]] >= 0;
             crp[k + 1] = crp[k] + c;
+            cfp[k + 1] = cfp[k] + (st.row_ptr[r + 1] - st.row_ptr[r] - c);
         }
+        std::vector<int32_t> cfr(cfp[nfree]);
         std::vector<int32_t> ccol(crp[nfree]), rel(crp[nfree]);
         parallel_for(nfree, [&](int64_t a, int64_t b) {
             for (int64_t k = a; k < b; k++) {
                 const int64_t r = fr[k];
                 int64_t o = crp[k];
+                int64_t oc = cfp[k];
                 for (int64_t q = st.row_ptr[r]; q < st.row_ptr[r + 1]; q++) {
                     const int64_t m = fmap[st.col[q]];
-                    if (m < 0) continue;
+                    if (m < 0) {
+                        cfr[oc++] = (int32_t)(q - st.row_ptr[r]);
+                        continue;
+                    }
                     ccol[o] = (int32_t)m;
                     rel[o] = (int32_t)(q - st.row_ptr[r]);
                     o++;
@@ -1305,9 +1312,10 @@ int ife_physics_set_constraints(ife_physics* ph, int32_t step, int64_t n, const 
         st.d_crp.upload(st.c_row_ptr.data(), st.c_row_ptr.size());
         st.d_ccol.upload(st.c_col.data(), st.c_col.size());
         st.d_rel.upload(rel.data(), rel.size());
+        st.d_cfp.upload(cfp.data(), cfp.size());
+        st.d_cfr.upload(cfr.data(), cfr.size());
         std::vector<double> zero(N, 0.0);  // g is zero at the free dofs for good
         st.d_g.upload(zero.data(), zero.size());
-        st.d_y.alloc(N);
         st.d_kcon.alloc(st.c_col.size());
         st.d_fcon.alloc(nfree);
         st.d_part.alloc((size_t)3 * norms_partials());
@@ -1336,8 +1344,9 @@ int ife_physics_constrain(ife_physics* ph, int32_t step, void* stream) {
         a.free = st.d_free.p;
         a.crp = st.d_crp.p;
         a.rel = st.d_rel.p;
+        a.cfp = st.d_cfp.p;
+        a.cfr = st.d_cfr.p;
         a.g = st.d_g.p;
-        a.y = st.d_y.p;
         a.Kc = st.d_kcon.p;
         a.fcon = st.d_fcon.p;
         HIPCHK(launch_constrain(a, (hipStream_t)stream));

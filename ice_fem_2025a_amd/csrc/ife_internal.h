@@ -159,18 +159,19 @@ hipError_t launch_adjoint(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_spmv(const int64_t* row_ptr, const int32_t* col, const double* val, const double* x, double* y,
                        int64_t n_rows, hipStream_t s);
 // Dirichlet constraint elimination of one step (Physics::Constraint): K (n rows, krp/kcol/Kv),
-// f, state x; nc constrained dofs (ascending) with values; nfree free dofs (ascending) and
-// the K_con pattern crp plus each entry's offset rel in its source row; work vectors g
-// (n, zero at free dofs) and y (n); outputs Kc (K_con values) and fcon.
+// f, state x; nc constrained dofs (ascending) with values; nfree free dofs (ascending); the
+// K_con pattern crp plus each entry's offset rel in its source row; per free row the offsets
+// of its entries in constrained columns (cfp/cfr); work vector g (n, zero at free dofs);
+// outputs Kc (K_con values) and fcon.
 struct ConsArgs {
     int64_t n, nc, nfree;
     const int64_t* krp;
     const int32_t* kcol;
     const double *Kv, *f, *x;
-    const int64_t *cdofs, *free, *crp;
+    const int64_t *cdofs, *free, *crp, *cfp;
     const double* cvals;
-    const int32_t* rel;
-    double *g, *y, *Kc, *fcon;
+    const int32_t *rel, *cfr;
+    double *g, *Kc, *fcon;
 };
 hipError_t launch_constrain(const ConsArgs& a, hipStream_t s);
 // sum |dx| / n, sum |fc| / n, sum |fc dx| / n into out[3] (device); part: norms_partials() x 3

@@ -2797,10 +2797,20 @@ __global__ void cons_g_kernel(const int64_t* cd, const double* cv, const double*
     if (i < nc) g[cd[i]] = -(cv[i] - x[cd[i]]);
 }
 
-// f_con[k] = -(f + K g)[free_k]   (y = K g from spmv_kernel)
-__global__ void cons_f_kernel(const int64_t* fr, const double* f, const double* y, double* fc, int64_t nfree) {
+// f_con[k] = -(f + K g)[free_k]: only a row's entries in constrained columns meet a non-zero
+// of g; their offsets in the source row are listed per free row (cfp/cfr), most lists empty
+__global__ void cons_f_kernel(const int64_t* fr, const double* f, const int64_t* krp, const int32_t* kcol,
+                              const double* Kv, const double* g, const int64_t* cfp, const int32_t* cfr, double* fc,
+                              int64_t nfree) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nfree) fc[k] = -(f[fr[k]] + y[fr[k]]);
+    if (k >= nfree) return;
+    const int64_t r = fr[k], s0 = krp[r];
+    double y = 0.0;
+    for (int64_t e = cfp[k]; e < cfp[k + 1]; e++) {
+        const int64_t q = s0 + cfr[e];
+        y += Kv[q] * g[kcol[q]];
+    }
+    fc[k] = -(f[r] + y);
 }
 
 // K_con values: one wavefront per free row; entry e of the row is the row's source entry
@@ -4072,11 +4082,9 @@ hipError_t launch_constrain(const ConsArgs& a, hipStream_t s) {
     if (a.nc > 0)
         hipLaunchKernelGGL(cons_g_kernel, dim3((unsigned)((a.nc + 255) / 256)), dim3(256), 0, s, a.cdofs, a.cvals, a.x, a.g,
                            a.nc);
-    hipError_t e = launch_spmv(a.krp, a.kcol, a.Kv, a.g, a.y, a.n, s);
-    if (e != hipSuccess) return e;
     if (a.nfree > 0) {
-        hipLaunchKernelGGL(cons_f_kernel, dim3((unsigned)((a.nfree + 255) / 256)), dim3(256), 0, s, a.free, a.f, a.y, a.fcon,
-                           a.nfree);
+        hipLaunchKernelGGL(cons_f_kernel, dim3((unsigned)((a.nfree + 255) / 256)), dim3(256), 0, s, a.free, a.f, a.krp,
+                           a.kcol, a.Kv, a.g, a.cfp, a.cfr, a.fcon, a.nfree);
         hipLaunchKernelGGL(cons_k_kernel, dim3((unsigned)((a.nfree + 3) / 4)), dim3(256), 0, s, a.crp, a.rel, a.krp, a.free,
                            a.Kv, a.Kc, a.nfree);
     }
