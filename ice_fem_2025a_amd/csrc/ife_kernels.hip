@@ -2044,31 +2044,44 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The element's independent loads (node ranks, extraction rows, coefficient rows), issued
+// together; a persistent wavefront issues them one element ahead.
+constexpr int W2_KL = (W2C_NF * 9 + 63) / 64;
+struct W2Load {
+    int r;
+    double cx[3];
+    double cv[W2_KL];
+};
+
+__device__ __forceinline__ void w2_load(const KParams& p, const int64_t el, const int lane, W2Load& L) {
+    constexpr int N = 9, NIP = 9;
+    const double* const src = p.ipb + el * IPB_NF * NIP;
+    L.r = lane < N ? p.erank[el * N + lane] : 0;
+    const bool ex = lane >= 32 && lane < 32 + 18;
+    const int xd = (lane - 32) / 9, xa = (lane - 32) % 3;
+    for (int k = 0; k < 3; k++) L.cx[k] = ex ? p.ext[(el * 2 + xd) * 9 + xa * 3 + k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < W2_KL; k++) {
+        const int t = lane + 64 * k, f = t / NIP;
+        L.cv[k] = t < W2C_NF * NIP ? src[w2c_src(f) * NIP + (t - f * NIP)] : 0.0;
+    }
+}
+
 template <typename T, typename R>
-__device__ __forceinline__ void assemble2d_wave_body(const KParams& p, const int64_t blk, Wave2D<T, R>& sm, const int lane) {
+__device__ __forceinline__ void assemble2d_wave_elem(const KParams& p, const int64_t el, const W2Load& L, Wave2D<T, R>& sm,
+                                                     const int lane) {
     using RC = Rec<2>;
     using SM = Wave2D<T, R>;
     constexpr int N = 9, NIP = 9;
-    const int64_t el = p.elist ? (int64_t)p.elist[blk] : blk;
+    constexpr int KL = W2_KL;
     R* const grec = (R*)p.ebuf + el * N * RC::SIZE;
     R* const rec = sm.recs;  // staged in LDS, stored to grec at the end
-    const double* const src = p.ipb + el * IPB_NF * NIP;
-    // all loads issued before the first LDS store, in dependence order: node ranks and
-    // extraction rows, the coefficient rows, then the nodal state (indexed by the ranks)
-    constexpr int KL = (W2C_NF * NIP + 63) / 64;
-    int r = 0;
-    if (lane < N) r = p.erank[el * N + lane];
+    // the nodal state (indexed by the ranks), then the LDS stores
+    const int r = L.r;
     const bool ex = lane >= 32 && lane < 32 + 18;
     const int xd = (lane - 32) / 9, xq = ((lane - 32) / 3) % 3, xa = (lane - 32) % 3;
-    double cx[3] = {0.0, 0.0, 0.0};
-    if (ex)
-        for (int k = 0; k < 3; k++) cx[k] = p.ext[(el * 2 + xd) * 9 + xa * 3 + k];
-    double cv[KL];
-#pragma unroll
-    for (int k = 0; k < KL; k++) {
-        const int t = lane + 64 * k, f = t / NIP;
-        cv[k] = t < W2C_NF * NIP ? src[w2c_src(f) * NIP + (t - f * NIP)] : 0.0;
-    }
+    const double* const cx = L.cx;
+    const double* const cv = L.cv;
     double nv[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // dU x y, ddU x y, phi, phi_old, dT
     if (lane < N) {
         if (p.has_pf) {
@@ -2210,6 +2223,14 @@ __device__ __forceinline__ void assemble2d_wave_body(const KParams& p, const int
     __builtin_amdgcn_wave_barrier();  // the wavefront's LDS is reused by its next element
 }
 
+template <typename T, typename R>
+__device__ __forceinline__ void assemble2d_wave_body(const KParams& p, const int64_t blk, Wave2D<T, R>& sm, const int lane) {
+    const int64_t el = p.elist ? (int64_t)p.elist[blk] : blk;
+    W2Load L;
+    w2_load(p, el, lane, L);
+    assemble2d_wave_elem<T, R>(p, el, L, sm, lane);
+}
+
 constexpr int W2_NW = 4;  // elements (wavefronts) per workgroup of the 2D wave kernel
 
 template <typename T, typename R>
@@ -2229,11 +2250,37 @@ __global__ __launch_bounds__(64 * W2_NW) void assemble2d_wave_batch(const KParam
     __shared__ Wave2D<T, R> sm[W2_NW];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t total = bptr[nd];
-    const int64_t stride = PERSIST ? (int64_t)gridDim.x * W2_NW : total;
-    for (int64_t b = (int64_t)blockIdx.x * W2_NW + wv; b < total; b += stride) {
+    int64_t b = (int64_t)blockIdx.x * W2_NW + wv;
+    if (b >= total) return;
+    if constexpr (!PERSIST) {
         const int d = batch_domain(bptr, nd, b);
         assemble2d_wave_body<T, R>((const KParams&)P[d], b - bptr[d], sm[wv], lane);
-        if (!PERSIST) break;
+    } else {
+        // one element ahead: the next element's independent loads are in flight while this
+        // one is contracted
+        const int64_t stride = (int64_t)gridDim.x * W2_NW;
+        auto elem_of = [&](int64_t bb, int& d) {
+            d = batch_domain(bptr, nd, bb);
+            const KParams& q = (const KParams&)P[d];
+            return q.elist ? (int64_t)q.elist[bb - bptr[d]] : bb - bptr[d];
+        };
+        int d;
+        int64_t el = elem_of(b, d);
+        W2Load cur, nxt;
+        w2_load((const KParams&)P[d], el, lane, cur);
+        for (; b < total; b += stride) {
+            const int64_t bn = b + stride;
+            int dn = d;
+            int64_t eln = 0;
+            if (bn < total) {
+                eln = elem_of(bn, dn);
+                w2_load((const KParams&)P[dn], eln, lane, nxt);
+            }
+            assemble2d_wave_elem<T, R>((const KParams&)P[d], el, cur, sm[wv], lane);
+            cur = nxt;
+            d = dn;
+            el = eln;
+        }
     }
 }
 
