@@ -61,7 +61,8 @@ struct StepIO {
 };
 
 // Per-IP coefficients written by the material kernel and contracted by the element
-// kernel, layout [field][e*nip + ip]:
+// kernel, layout [e][row][ip] (IPB_NF rows per element; 2D stores only the rows its element
+// kernel reads, compact, at the front of the element's block -- ipb_slot in ife_kernels.hip):
 //   M    w*(Dam*D_Dam + D_UnDam)*DPlast (6x6, row-major)    SIG  w*(Dam*S_Dam + S_UnDam)
 //   PHA  coefficient of N N^T in K_phiphi                     PHB  coefficient of G^T G
 //   PHFN coefficient of N in F_phi                            PHFG coefficient of G^T (3)

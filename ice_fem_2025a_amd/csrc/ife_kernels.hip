@@ -31,6 +31,44 @@
 
 namespace ife {
 
+// Per-IP coefficient rows.  3D stores every ife::IPB_* row.  2D stores only the 29 rows its
+// element kernel reads, contiguously in the order W2C_* below, so a 2D element's block is
+// one contiguous 2.1 kB write (material kernel) and read (element kernel).
+enum {
+    W2C_M = 0,       // M[r][c], r, c in Voigt rows (xx, yy, xy) = (0, 1, 3): 3 x 3
+    W2C_SIG = 9,     // SIG rows 0, 1, 3
+    W2C_PHA = 12, W2C_PHB = 13, W2C_PHFN = 14, W2C_PHFG = 15,  // PHFG x, y
+    W2C_THB = 17, W2C_THFN = 18, W2C_THFG = 19,                // THFG x, y
+    W2C_LD = 21, W2C_GW = 22, W2C_GC = 23, W2C_JI = 24,        // JI 2 x 2
+    W2C_W = 28, W2C_NF = 29, W2C_Z = 29                        // row 29: zeros
+};
+// storage row of IPB row f in DIM dimensions (-1: not stored)
+template <int DIM>
+__device__ __forceinline__ int ipb_slot(int f) {
+    if (DIM == 3) return f;
+    auto vk = [](int r) { return r == 0 ? 0 : r == 1 ? 1 : r == 3 ? 2 : -1; };
+    if (f < IPB_SIG) {
+        const int r = vk((f - IPB_M) / 6), c = vk((f - IPB_M) % 6);
+        return r < 0 || c < 0 ? -1 : W2C_M + r * 3 + c;
+    }
+    if (f < IPB_PHA) { const int r = vk(f - IPB_SIG); return r < 0 ? -1 : W2C_SIG + r; }
+    if (f <= IPB_PHFG + 1) return W2C_PHA + (f - IPB_PHA);
+    if (f == IPB_LDRIVE) return W2C_LD;
+    if (f >= IPB_THB && f <= IPB_THFG + 1) return W2C_THB + (f - IPB_THB);
+    if (f == IPB_GW) return W2C_GW;
+    if (f == IPB_GC) return W2C_GC;
+    if (f >= IPB_JI && f < IPB_JI + 4) return W2C_JI + (f - IPB_JI);
+    if (f == IPB_W) return W2C_W;
+    return -1;
+}
+
+template <int DIM>
+__device__ __forceinline__ void ipb_put(double* ipo, int f, double v) {
+    constexpr int NIP = DIM == 2 ? 9 : 27;
+    const int s = ipb_slot<DIM>(f);
+    if (s >= 0) ipo[s * NIP] = v;
+}
+
 namespace {
 
 constexpr int NT = 256;  // lanes per workgroup
@@ -1146,13 +1184,13 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
             for (int c = 0; c < DIM; c++) {
                 double q = 0;
                 for (int d = 0; d < DIM; d++) {
-                    ipo[(IPB_JI + c * DIM + d) * NIP] = Ji[c][d] * rw;
+                    ipb_put<DIM>(ipo, (IPB_JI + c * DIM + d), Ji[c][d] * rw);
                     q += Ji[c][d] * dW[d];
                 }
-                if (DIM == 3) ipo[(IPB_DW + c) * NIP] = q * rw * rw;  // 2D: polynomial basis
+                if (DIM == 3) ipb_put<DIM>(ipo, (IPB_DW + c), q * rw * rw);  // 2D: polynomial basis
             }
-            if (DIM == 3) ipo[IPB_WF * NIP] = rw;
-            ipo[IPB_W * NIP] = w;
+            if (DIM == 3) ipb_put<DIM>(ipo, IPB_WF, rw);
+            ipb_put<DIM>(ipo, IPB_W, w);
         }
         // field values and gradients at the IP
         double gu[DIM][DIM], guo[DIM][DIM], gphi[DIM], gT[DIM];
@@ -1320,7 +1358,7 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
             for (int r = 0; r < 6; r++) {
                 double v = w * sig[r];
                 my_bad |= !isfinite(v);
-                if (used(r)) ipo[(IPB_SIG + r) * NIP] = v;
+                if (used(r)) ipb_put<DIM>(ipo, (IPB_SIG + r), v);
             }
             // Mt = w * S * DPlast = w * (S - (S a) b^T inv)  (VEP: the full DPlast)
             if (ADJ) {
@@ -1330,7 +1368,7 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                         if (!used(i) || !used(j)) continue;
                         double t = 0;
                         for (int k = 0; k < 6; k++) t += Sm[i][k] * DM[k][j];
-                        ipo[(IPB_M + i * 6 + j) * NIP] = w * t;
+                        ipb_put<DIM>(ipo, (IPB_M + i * 6 + j), w * t);
                     }
             } else
             for (int i = 0; i < 6; i++) {
@@ -1339,7 +1377,7 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                     for (int k = 0; k < 6; k++) sa += Sm[i][k] * ra[k];
                 for (int j = 0; j < 6; j++) {
                     double v = elastic ? Sm[i][j] : (Sm[i][j] - sa * rb[j] * rinv);
-                    if (used(i) && used(j)) ipo[(IPB_M + i * 6 + j) * NIP] = w * v;
+                    if (used(i) && used(j)) ipb_put<DIM>(ipo, (IPB_M + i * 6 + j), w * v);
                 }
             }
         } else if (p.has_pf && p.do_Tpf) {
@@ -1436,22 +1474,22 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                 A += -w * mc.Density * gu_ * ddDamG;
             }
             if constexpr (!ADJ) {
-                ipo[(IPB_PHA) * NIP] = A;
-                ipo[(IPB_PHB) * NIP] = w * m.Gc * cGG;
-                ipo[(IPB_PHFN) * NIP] = FN;
-                for (int c = 0; c < DIM; c++) ipo[(IPB_PHFG + c) * NIP] = w * m.Gc * cGG * gphi[c];
-                ipo[(IPB_LDRIVE) * NIP] = LDR;
-                ipo[(IPB_GW) * NIP] = gamma * w;
-                ipo[(IPB_GC) * NIP] = m.Gc;
+                ipb_put<DIM>(ipo, (IPB_PHA), A);
+                ipb_put<DIM>(ipo, (IPB_PHB), w * m.Gc * cGG);
+                ipb_put<DIM>(ipo, (IPB_PHFN), FN);
+                for (int c = 0; c < DIM; c++) ipb_put<DIM>(ipo, (IPB_PHFG + c), w * m.Gc * cGG * gphi[c]);
+                ipb_put<DIM>(ipo, (IPB_LDRIVE), LDR);
+                ipb_put<DIM>(ipo, (IPB_GW), gamma * w);
+                ipb_put<DIM>(ipo, (IPB_GC), m.Gc);
             }
             my_bad |= !isfinite(A) || !isfinite(FN);
         }
         if (!ADJ && (p.do_Tpf || p.do_th)) {
             double fn = 0;
             if (p.has_pf && p.do_Tpf) fn = -w * ((VD - p.VDOld[gi]) / p.dt);
-            ipo[(IPB_THFN) * NIP] = fn;
-            ipo[(IPB_THB) * NIP] = p.do_th ? w * mc.k : 0.0;
-            for (int c = 0; c < DIM; c++) ipo[(IPB_THFG + c) * NIP] = p.do_th ? w * mc.k * gT[c] : 0.0;
+            ipb_put<DIM>(ipo, (IPB_THFN), fn);
+            ipb_put<DIM>(ipo, (IPB_THB), p.do_th ? w * mc.k : 0.0);
+            for (int c = 0; c < DIM; c++) ipb_put<DIM>(ipo, (IPB_THFG + c), p.do_th ? w * mc.k * gT[c] : 0.0);
         }
         if constexpr (ADJ) {
             for (int k = 0; k < 3; k++) {
@@ -1990,28 +2028,6 @@ __global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_batch(co
 // absent).  F rows likewise: F_r += N(ip, i) c_n + G_0(ip, i) c_0 + G_1(ip, i) c_1.
 // Only the 29 per-IP coefficient rows the 2D element uses are read (compact layout W2C_*).
 // Records are staged in LDS and stored coalesced.
-enum {
-    W2C_M = 0,       // M[r][c], r, c in Voigt rows (xx, yy, xy) = (0, 1, 3): 3 x 3
-    W2C_SIG = 9,     // SIG rows 0, 1, 3
-    W2C_PHA = 12, W2C_PHB = 13, W2C_PHFN = 14, W2C_PHFG = 15,  // PHFG x, y
-    W2C_THB = 17, W2C_THFN = 18, W2C_THFG = 19,                // THFG x, y
-    W2C_LD = 21, W2C_GW = 22, W2C_GC = 23, W2C_JI = 24,        // JI 2 x 2
-    W2C_W = 28, W2C_NF = 29, W2C_Z = 29                        // row 29: zeros
-};
-// source row (ife::IPB_*) of compact row f, by arithmetic (no table load on the path)
-__device__ __forceinline__ int w2c_src(int f) {
-    const int q = f / 3, r = f % 3;
-    const int vq = q == 2 ? 3 : q, vr = r == 2 ? 3 : r;
-    return f < W2C_SIG   ? IPB_M + vq * 6 + vr
-           : f < W2C_PHA ? IPB_SIG + (f == W2C_SIG + 2 ? 3 : f - W2C_SIG)
-           : f < W2C_THB ? IPB_PHA + (f - W2C_PHA)
-           : f < W2C_LD  ? IPB_THB + (f - W2C_THB)
-           : f == W2C_LD ? IPB_LDRIVE
-           : f < W2C_JI  ? IPB_GW + (f - W2C_GW)
-           : f < W2C_W   ? IPB_JI + (f - W2C_JI)
-                         : IPB_W;
-}
-
 __device__ __forceinline__ int w2_voigt(int a, int c) { return vrow<2>(a, c) == 3 ? 2 : vrow<2>(a, c); }
 
 template <typename T, typename R>
@@ -2063,7 +2079,7 @@ __device__ __forceinline__ void w2_load(const KParams& p, const int64_t el, cons
 #pragma unroll
     for (int k = 0; k < W2_KL; k++) {
         const int t = lane + 64 * k, f = t / NIP;
-        L.cv[k] = t < W2C_NF * NIP ? src[w2c_src(f) * NIP + (t - f * NIP)] : 0.0;
+        L.cv[k] = t < W2C_NF * NIP ? src[t] : 0.0;  // the 2D block is stored compact (ipb_slot)
     }
 }
 
@@ -2560,7 +2576,7 @@ __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
             double s = 0;
             for (int a = 0; a < N; a++) s += sm.Phi[i][k][a] * sm.ph[a];
             const double kc = k < DIM ? kG : kG2 * (k - DIM < DIM ? 1.0 : 2.0);
-            const double w = ipc[IPB_W * NIP + ip], wgc = w * ipc[IPB_GC * NIP + ip];
+            const double w = ipc[ipb_slot<DIM>(IPB_W) * NIP + ip], wgc = w * ipc[ipb_slot<DIM>(IPB_GC) * NIP + ip];
             sm.cK[i][k] = wgc * kc;
             sm.cF[i][k] = wgc * kc * s;
             sm.gw[i][k] = 0.5 * kc * s * s * w;
@@ -3547,13 +3563,13 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
         for (int c = 0; c < DIM; c++) {
             double qq = 0;
             for (int d = 0; d < DIM; d++) {
-                ipo[(IPB_JI + c * DIM + d) * NIP] = Ji[c][d] * rw;
+                ipb_put<DIM>(ipo, (IPB_JI + c * DIM + d), Ji[c][d] * rw);
                 qq += Ji[c][d] * dW[d];
             }
-            ipo[(IPB_DW + c) * NIP] = qq * rw * rw;
+            ipb_put<DIM>(ipo, (IPB_DW + c), qq * rw * rw);
         }
-        ipo[IPB_WF * NIP] = rw;
-        ipo[IPB_W * NIP] = w;
+        ipb_put<DIM>(ipo, IPB_WF, rw);
+        ipb_put<DIM>(ipo, IPB_W, w);
         auto grad = [&](int k, double g[3]) {
             double r3[DIM];
             for (int d = 0; d < DIM; d++) r3[d] = rgrad(k, d);
@@ -3739,21 +3755,21 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
                 FN += -w * mc.Density * gu_ * dDamG;
                 A += -w * mc.Density * gu_ * ddDamG;
             }
-            ipo[(IPB_PHA) * NIP] = A;
-            ipo[(IPB_PHB) * NIP] = w * m.Gc * cGG;
-            ipo[(IPB_PHFN) * NIP] = FN;
-            for (int c = 0; c < DIM; c++) ipo[(IPB_PHFG + c) * NIP] = w * m.Gc * cGG * gphi[c];
-            ipo[(IPB_LDRIVE) * NIP] = 0.0;
-            ipo[(IPB_GW) * NIP] = gamma * w;
-            ipo[(IPB_GC) * NIP] = m.Gc;
+            ipb_put<DIM>(ipo, (IPB_PHA), A);
+            ipb_put<DIM>(ipo, (IPB_PHB), w * m.Gc * cGG);
+            ipb_put<DIM>(ipo, (IPB_PHFN), FN);
+            for (int c = 0; c < DIM; c++) ipb_put<DIM>(ipo, (IPB_PHFG + c), w * m.Gc * cGG * gphi[c]);
+            ipb_put<DIM>(ipo, (IPB_LDRIVE), 0.0);
+            ipb_put<DIM>(ipo, (IPB_GW), gamma * w);
+            ipb_put<DIM>(ipo, (IPB_GC), m.Gc);
             my_bad |= !isfinite(A) || !isfinite(FN);
         }
         if (p.do_Tpf || p.do_th) {
             double fn = 0;
             if (p.do_Tpf) fn = -w * ((VD - p.VDOld[gi]) / p.dt);
-            ipo[(IPB_THFN) * NIP] = fn;
-            ipo[(IPB_THB) * NIP] = p.do_th ? w * mc.k : 0.0;
-            for (int c = 0; c < DIM; c++) ipo[(IPB_THFG + c) * NIP] = p.do_th ? w * mc.k * gT[c] : 0.0;
+            ipb_put<DIM>(ipo, (IPB_THFN), fn);
+            ipb_put<DIM>(ipo, (IPB_THB), p.do_th ? w * mc.k : 0.0);
+            for (int c = 0; c < DIM; c++) ipb_put<DIM>(ipo, (IPB_THFG + c), p.do_th ? w * mc.k * gT[c] : 0.0);
         }
     }
     if (my_calls) {
@@ -3850,7 +3866,8 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     for (int t = tid; t < NG * NIP; t += NT) {  // the geometry fields IPB_JI, IPB_DW, IPB_WF
         const int g = t / NIP, ip = t % NIP;
         const int fld = g < DIM * DIM ? IPB_JI + g : (g < DIM * DIM + DIM ? IPB_DW + g - DIM * DIM : IPB_WF);
-        geo[g][ip] = ipc[fld * NIP + ip];
+        const int sl = ipb_slot<DIM>(fld);  // 2D stores no DW, WF: polynomial basis, DW = 0, WF = 1
+        geo[g][ip] = sl >= 0 ? ipc[sl * NIP + ip] : (fld == IPB_WF ? 1.0 : 0.0);
     }
     for (int t = tid; t < 18 * NIP; t += NT) sig[t / NIP][t % NIP] = cb[(COSS_SIG + t / NIP) * NIP + t % NIP];
     if (tid < NIP) {
