@@ -195,7 +195,9 @@ struct ife_physics {
     int ts = IFE_TS_EULER;  // time scheme of UpdateVelAcc and its constants
     double ts_c0 = 1.0, ts_c1 = 0.0;
     int precision = IFE_PREC_FP64;
-    bool last_had_phase = false;
+    // a phase step has been assembled: LCrack then holds its crack length, as the model's
+    // LCrack member keeps the last Assemble_Phase's sum through other steps (PhaseField.cpp:540,650)
+    bool phase_assembled = false;
     // launch timing
     bool profiling = false;
     struct Rec { hipEvent_t a, b; int64_t elems; };
@@ -698,7 +700,7 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
     p.do_ph = pfm && in(pfm->Step_phase);
     p.do_Tpf = pfm && pfm->HasT && pfm->opt.ViscousHeating && in(pfm->Step_T);
     p.do_th = thm && in(thm->Step_T);
-    ph->last_had_phase = p.do_ph;
+    if (p.do_ph) ph->phase_assembled = true;
     HIPCHK(launch_zero_status(ph->d_status.p, s));
     if (!(p.do_u || p.do_ph || p.do_Tpf || p.do_th)) return;
     // material update over all IPs, element records for all elements in one launch, then
@@ -869,7 +871,7 @@ void batch_enqueue(ife_batch* b, hipStream_t s) {
     HIPCHK(launch_batch_material(a, b->grid[0], s));
     HIPCHK(launch_batch_assemble(a, b->grid[1], b->nta, s));
     HIPCHK(launch_batch_gather(a, b->grid[2], s));
-    for (auto* ph : b->doms) ph->last_had_phase = ph->pf_model() != nullptr;
+    for (auto* ph : b->doms) ph->phase_assembled |= ph->pf_model() != nullptr;
 }
 
 void batch_capture(ife_batch* b) {
@@ -1498,7 +1500,7 @@ int ife_physics_status(ife_physics* ph, void* stream, ife_assembly_status* out) 
         out->return_map_maxit_hits = (int64_t)st.rm_maxit;
         out->nonfinite_seen = (int32_t)st.nonfinite;
         double s_ = 0;
-        if (ph->last_had_phase)
+        if (ph->phase_assembled)
             for (int64_t e = 0; e < ph->NE; e++) s_ += lc[e];
         out->LCrack = s_;
     });
@@ -1675,7 +1677,7 @@ int ife_batch_launch(ife_batch* b, void* stream) {
         if (!b) THROW(IFE_ERR_INVALID, "null batch");
         if (!b->exec) batch_capture(b);
         HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
-        for (auto* ph : b->doms) ph->last_had_phase = ph->pf_model() != nullptr;
+        for (auto* ph : b->doms) ph->phase_assembled |= ph->pf_model() != nullptr;
     });
 }
 

@@ -366,6 +366,68 @@ __device__ __forceinline__ double Pij(int i, int j, double shear) {
     return i == j ? shear : 0.0;
 }
 
+// In-place LU factorisation with partial pivoting (largest magnitude, first on ties; whole
+// rows swapped, multipliers divided by the pivot), the factorisation the reference's
+// fixed-size K.inverse() runs (SolidMaterial.cpp:326, :355).  perm[i] is the original row now
+// at row i.  Register-resident: pivots are applied with selects, not dynamic indexing.
+template <int M>
+__device__ void lu_pp(double (&A)[M][M], int (&perm)[M]) {
+#pragma unroll
+    for (int i = 0; i < M; i++) perm[i] = i;
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        int piv = k;
+        double amax = fabs(A[k][k]);
+#pragma unroll
+        for (int r = k + 1; r < M; r++) {
+            const double v = fabs(A[r][k]);
+            if (v > amax) { amax = v; piv = r; }
+        }
+#pragma unroll
+        for (int r = k + 1; r < M; r++) {
+            const bool sw = r == piv;
+#pragma unroll
+            for (int c = 0; c < M; c++) {
+                const double t = A[k][c];
+                A[k][c] = sw ? A[r][c] : t;
+                A[r][c] = sw ? t : A[r][c];
+            }
+            const int t = perm[k];
+            perm[k] = sw ? perm[r] : t;
+            perm[r] = sw ? t : perm[r];
+        }
+        const double p = A[k][k];
+        if (p != 0.0) {
+#pragma unroll
+            for (int r = k + 1; r < M; r++) {
+                A[r][k] /= p;
+                const double l = A[r][k];
+#pragma unroll
+                for (int c = k + 1; c < M; c++) A[r][c] -= l * A[k][c];
+            }
+        }
+    }
+}
+
+// Column `col` of A^-1 from the factors of lu_pp: forward substitution with the unit lower
+// factor on the permuted unit vector, then back substitution (the column solve of the
+// reference's inverse()).
+template <int M>
+__device__ void lu_inv_col(const double (&LU)[M][M], const int (&perm)[M], int col, double (&y)[M]) {
+#pragma unroll
+    for (int i = 0; i < M; i++) y[i] = perm[i] == col ? 1.0 : 0.0;
+#pragma unroll
+    for (int i = 0; i < M; i++)
+#pragma unroll
+        for (int j = 0; j < i; j++) y[i] -= LU[i][j] * y[j];
+#pragma unroll
+    for (int i = M - 1; i >= 0; i--) {
+#pragma unroll
+        for (int j = i + 1; j < M; j++) y[i] -= LU[i][j] * y[j];
+        y[i] /= LU[i][i];
+    }
+}
+
 // A x = B for an M x M system and NR right-hand sides, Gaussian elimination with partial
 // pivoting (largest magnitude, first on ties); A and B are overwritten, B by the solution.
 template <int M, int NR>
@@ -541,10 +603,17 @@ __device__ void return_map_vep(const RMStop& stop, const VEPConst& v, const MatI
     maxit = 0;
     while (true) {
         it += 1;
-        double x[8][1];
-        for (int i = 0; i < 8; i++) x[i][0] = f[i];
-        gauss_pp<8, 1>(K, x);  // K is rebuilt below
-        for (int i = 0; i < 8; i++) { dsol[i] = -x[i][0]; sol[i] += dsol[i]; fOld[i] = f[i]; }
+        // dsol = -K^-1 f with the explicit inverse, as the reference (SolidMaterial.cpp:326):
+        // dsol_i = sum_j (-Kinv_ij) f_j in ascending j, one inverse column at a time
+        int perm[8];
+        lu_pp<8>(K, perm);  // K is rebuilt below
+        for (int i = 0; i < 8; i++) dsol[i] = 0.0;
+        for (int j = 0; j < 8; j++) {
+            double col[8];
+            lu_inv_col<8>(K, perm, j, col);
+            for (int i = 0; i < 8; i++) dsol[i] += -col[i] * f[j];
+        }
+        for (int i = 0; i < 8; i++) { sol[i] += dsol[i]; fOld[i] = f[i]; }
         getKF_vep(v, m, nC, dt, h0, sol, et, eo, K, f);
         double ls1 = 0, ls2 = 0;
         for (int i = 0; i < 8; i++) { ls1 += f[i] * dsol[i]; ls2 += fOld[i] * dsol[i]; }
@@ -576,12 +645,14 @@ __device__ void return_map_vep(const RMStop& stop, const VEPConst& v, const MatI
     hist[2] = sol[7];
     diss = DV + DP;
     ddiss = DP;
-    double B[8][6];
-    for (int i = 0; i < 8; i++)
-        for (int j = 0; j < 6; j++) B[i][j] = i == j ? 1.0 : 0.0;
-    gauss_pp<8, 6>(K, B);
-    for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++) DMat[i][j] = B[i][j];
+    // DMat = (K^-1)(0:5, 0:5) (SolidMaterial.cpp:355-356)
+    int perm[8];
+    lu_pp<8>(K, perm);
+    for (int j = 0; j < 6; j++) {
+        double col[8];
+        lu_inv_col<8>(K, perm, j, col);
+        for (int i = 0; i < 6; i++) DMat[i][j] = col[i];
+    }
 }
 
 

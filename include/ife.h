@@ -181,7 +181,8 @@ typedef struct {
     int64_t return_map_iterations;  /* sum over IPs: mean = iterations / calls */
     int64_t return_map_maxit_hits;  /* reference logs these (SolidMaterial.cpp:246-250) */
     int32_t nonfinite_seen;         /* NaN/Inf in any element residual/tangent */
-    double  LCrack;                 /* sum gamma*w of the last phase assembly (PhaseField.cpp:650) */
+    double  LCrack;                 /* sum gamma*w of the last phase assembly (PhaseField.cpp:540,650);
+                                       kept through assemblies of other steps */
 } ife_assembly_status;
 
 /* Kernel timing of assemble calls (HIP events around every kernel launch),
