@@ -122,6 +122,7 @@ _PROTOS = [
                                           C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     ("ife_physics_residual_norms", C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(C.c_double)]),
     ("ife_physics_residual_norms_async", C.c_int, [_P, C.c_int32, _P, _P, _P]),
+    ("ife_residual_norms_noalloc", C.c_int, [_P, _P, C.c_int64, _P, _P]),
     ("ife_physics_set_time_scheme", C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
     ("ife_physics_set_dt", C.c_int, [_P, C.c_double, C.c_double]),
     ("ife_physics_update_state", C.c_int, [_P, C.c_int32, _P, _P]),

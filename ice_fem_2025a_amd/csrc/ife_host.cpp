@@ -1399,6 +1399,15 @@ int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, 
     });
 }
 
+int ife_residual_norms_noalloc(const double* dx, const double* f_con, int64_t n, double* out_dev, void* stream) {
+    return guard([&] {
+        if (!dx || !f_con || !out_dev) THROW(IFE_ERR_INVALID, "null argument");
+        if (n <= 0 || n > IFE_NOALLOC_NORMS_MAX_N)
+            THROW(IFE_ERR_INVALID, "ife_residual_norms_noalloc: n must be in [1, 2^53] (got " + std::to_string(n) + ")");
+        HIPCHK(launch_norms_noalloc(dx, f_con, n, out_dev, (hipStream_t)stream));
+    });
+}
+
 int ife_physics_set_time_scheme(ife_physics* ph, int32_t scheme, double c0, double c1) {
     return guard([&] {
         if (!ph) THROW(IFE_ERR_INVALID, "null physics");

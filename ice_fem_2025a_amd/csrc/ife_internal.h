@@ -178,6 +178,8 @@ hipError_t launch_constrain(const ConsArgs& a, hipStream_t s);
 // sum |dx| / n, sum |fc| / n, sum |fc dx| / n into out[3] (device); part: norms_partials() x 3
 hipError_t launch_norms(const double* dx, const double* fc, int64_t n, double* part, double* out, hipStream_t s);
 int norms_partials();
+// launch_norms with its partials in a file-scope device array (no allocation)
+hipError_t launch_norms_noalloc(const double* dx, const double* fc, int64_t n, double* out, hipStream_t s);
 // x[free[k]] += dx[k] (free NULL: x[k] += dx[k] over n), then x[cd[i]] -= g[cd[i]] (g = -C c)
 hipError_t launch_update_state(double* x, const double* dx, const int64_t* free, int64_t n, const int64_t* cd,
                                const double* g, int64_t nc, hipStream_t s);

@@ -27,6 +27,8 @@
 //   degradation      Models/Fracture/PhaseField/PhaseFieldUtil.cpp:77-280
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "ife_internal.h"
 
 namespace ife {
@@ -4384,6 +4386,45 @@ hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s) 
     else { if (a.dim == 2) IFE_GB(2, double); else IFE_GB(3, double); }
 #undef IFE_GB
     return hipGetLastError();
+}
+
+// ---- allocation-free residual norms (ife_residual_norms_noalloc) -------------------------
+// The block partials of norms_partial_kernel live in this file-scope device array instead of
+// a buffer from ife_physics_set_constraints: 3 x NORM_NB doubles (24 KiB), which every GPU
+// the library runs on holds until the process ends.  Calls on one GPU share it, so calls that
+// overlap on one GPU race.  norms_partial_kernel writes every partial before
+// norms_final_kernel reads it, so nothing needs clearing between calls.
+static __device__ __align__(256) double scratch[3 * NORM_NB];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int MAX_DEVICES = 64;
+double* g_scratch_dev[MAX_DEVICES] = {};  // device address of `scratch`, per device
+}  // namespace
+
+// Device address of `scratch` on the current device, looked up once per device (in host code
+// the array's name is a host-side stand-in, not a device address).
+hipError_t noalloc_scratch(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_norms_noalloc(const double* dx, const double* fc, int64_t n, double* out, hipStream_t s) {
+    double* part = nullptr;
+    const hipError_t e = noalloc_scratch(&part);
+    if (e != hipSuccess) return e;
+    return launch_norms(dx, fc, n, part, out, s);  // the same kernels, partials in `scratch`
 }
 
 }  // namespace ife

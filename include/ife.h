@@ -365,6 +365,19 @@ int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, 
 int ife_physics_residual_norms_async(ife_physics* ph, int32_t step, const double* dx, double* out_dev,
                                      void* stream);
 
+/* The norms of ife_physics_residual_norms_async from caller-supplied vectors, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight solver loops, harnesses that forbid hipMalloc)
+ * and have no ife_physics set-up to lend a buffer.  out_dev (3 doubles) = (|dx|_1, |f_con|_1,
+ * |f_con .* dx|_1) / n over n doubles of device vectors dx and f_con, with the same kernels,
+ * reduction order and bits as ife_physics_residual_norms_async.  Its block partials live in
+ * one static 24 KiB device array per GPU (held until the process ends), so calls that overlap
+ * on one GPU race.  n must lie in [1, IFE_NOALLOC_NORMS_MAX_N] (2^53, the counts the FP64
+ * normalisation divides by exactly); anything else is IFE_ERR_INVALID.  Stream-ordered, no
+ * synchronisation. */
+#define IFE_NOALLOC_NORMS_MAX_N (INT64_C(1) << 53)
+int ife_residual_norms_noalloc(const double* dx, const double* f_con, int64_t n, double* out_dev, void* stream);
+
 /* ---- State updates on the device (a Newton iteration without host round trips) ----
  * TimeScheme(inputs) (Physics/TimeSchemes.cpp:5-21): the scheme UpdateVelAcc uses with the
  * dt of ife_physics_set_time.  Default: Euler. */
