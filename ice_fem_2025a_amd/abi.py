@@ -139,6 +139,17 @@ _PROTOS = [
     ("ife_batch_set_shape", C.c_int, [_P, C.POINTER(BatchTuning)]),
     ("ife_batch_get_shape", C.c_int, [_P, C.POINTER(BatchTuning)]),
     ("ife_batch_destroy", None, [_P]),
+    ("ife_mesh_load_hdf5", C.c_int, [C.c_char_p, C.c_int32, C.POINTER(_P)]),
+    ("ife_mesh_file_free", None, [_P]),
+    ("ife_mesh_file_info", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]),
+    ("ife_mesh_file_element_group", C.c_int, [_P, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p),
+                                              C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    ("ife_mesh_file_node_group", C.c_int, [_P, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(C.c_int64),
+                                           C.POINTER(C.POINTER(C.c_int64))]),
+    ("ife_mesh_file_desc", C.c_int, [_P, C.c_int32, C.POINTER(MeshDesc)]),
+    ("ife_mesh_save_hdf5", C.c_int, [C.c_char_p, C.POINTER(MeshDesc), C.c_char_p, C.c_int32, C.POINTER(C.c_char_p),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("ife_last_error", C.c_char_p, []),
     ("ife_api_version", C.c_int, []),
 ]

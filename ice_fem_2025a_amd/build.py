@@ -14,7 +14,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_build")
 OUT = os.path.join(OUT_DIR, "libife.so")
-SOURCES = ["ife_host.cpp", "ife_kernels.hip"]
+SOURCES = ["ife_host.cpp", "ife_mesh_hdf5.cpp", "ife_kernels.hip"]
 HEADERS = [os.path.join(CSRC, "ife_internal.h"), os.path.join(ROOT, "include", "ife.h")]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -26,7 +26,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I" + os.path.join(ROOT, "include"),
 ]
-LINK = ["--offload-arch=gfx950", "-shared", "-fPIC"]
+LINK = ["--offload-arch=gfx950", "-shared", "-fPIC", "-ldl"]
 
 
 def _compile(src, force, verbose):
@@ -51,7 +51,7 @@ def build_engine(verbose=False, force=True):
     with ThreadPoolExecutor(len(srcs)) as ex:
         objs = list(ex.map(lambda s: _compile(s, force, verbose), srcs))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + LINK + ["-o", tmp] + objs
+    cmd = [HIPCC] + ["-o", tmp] + objs + LINK
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -152,6 +152,10 @@ struct Step {
 
 }  // namespace
 
+namespace ife {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace ife
+
 struct ife_physics {
     // mesh
     int dim = 0, ip1d = 3, n = 0, nip = 0;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/ife.h"
 
 namespace ife {
@@ -178,6 +180,8 @@ hipError_t launch_constrain(const ConsArgs& a, hipStream_t s);
 // sum |dx| / n, sum |fc| / n, sum |fc dx| / n into out[3] (device); part: norms_partials() x 3
 hipError_t launch_norms(const double* dx, const double* fc, int64_t n, double* part, double* out, hipStream_t s);
 int norms_partials();
+// the message ife_last_error() returns (for entry points defined outside ife_host.cpp)
+void set_last_error(const std::string& msg);
 // launch_norms with its partials in a file-scope device array (no allocation)
 hipError_t launch_norms_noalloc(const double* dx, const double* fc, int64_t n, double* out, hipStream_t s);
 // x[free[k]] += dx[k] (free NULL: x[k] += dx[k] over n), then x[cd[i]] -= g[cd[i]] (g = -C c)

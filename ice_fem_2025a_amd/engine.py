@@ -20,15 +20,21 @@ def _mesh_desc(mesh, keep):
     coords = np.ascontiguousarray(mesh.coords, dtype=np.float64)
     weights = np.ascontiguousarray(mesh.weights, dtype=np.float64)
     elems = np.ascontiguousarray(mesh.elems, dtype=np.int64)
-    e1 = np.ascontiguousarray(mesh.extraction_1d, dtype=np.float64)
-    keep += [coords, weights, elems, e1]
+    keep += [coords, weights, elems]
     d = abi.MeshDesc()
     d.dim, d.ipcount1D = mesh.dim, mesh.ipcount1D
     d.n_nodes, d.n_elems, d.nodes_per_elem = mesh.n_nodes, mesh.n_elems, mesh.nodes_per_elem
     d.coords = coords.ctypes.data_as(C.POINTER(C.c_double))
     d.weights = weights.ctypes.data_as(C.POINTER(C.c_double))
     d.elems = elems.ctypes.data_as(C.POINTER(C.c_int64))
-    d.extraction_1d = e1.ctypes.data_as(C.POINTER(C.c_double))
+    if getattr(mesh, "extraction_1d", None) is not None:  # generated patch: 1D factors
+        e1 = np.ascontiguousarray(mesh.extraction_1d, dtype=np.float64)
+        keep.append(e1)
+        d.extraction_1d = e1.ctypes.data_as(C.POINTER(C.c_double))
+    else:  # a mesh file: dense operators, factored by the engine
+        ex = np.ascontiguousarray(mesh.extraction, dtype=np.float64)
+        keep.append(ex)
+        d.extraction = ex.ctypes.data_as(C.POINTER(C.c_double))
     return d
 
 

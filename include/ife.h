@@ -477,6 +477,37 @@ int ife_batch_get_shape(const ife_batch* b, ife_batch_tuning* out);
 
 void ife_batch_destroy(ife_batch* b);
 
+/* ---- IceFEM HDF5 meshes (Mesh::LoadFromFile, mesh/mesh.cpp:53-303; the format written by
+ * TestCases/TriAxialCompression/Meshgen/SaveToHDF.m:1-83) -------------------------------------
+ * The single-rank read of /nodes, /node_DataTypes + /node_Data (the "W" weights),
+ * /nodegroupnames + /nodegroups/<name>, /elementgroupnames, /elementgrouptypes,
+ * /elementgroups/<name> and /elementgroups/<name>_Data (mesh.cpp:91-126,193-208,237-262;
+ * elementgroup.cpp:17-67).  The HDF5 C library (1.10 or later) is loaded at the first call
+ * (IFE_HDF5_LIB, else libhdf5.so on the loader path, else /opt/conda/lib/libhdf5.so); without it
+ * these return IFE_ERR_UNSUPPORTED.  Host-side; no GPU is touched. */
+typedef struct ife_mesh_file ife_mesh_file;
+
+/* Reads the whole file.  ipcount1D is the input's mesh.ipcount1D (Mesh::Mesh, mesh.cpp:12-35). */
+int ife_mesh_load_hdf5(const char* path, int32_t ipcount1D, ife_mesh_file** out);
+void ife_mesh_file_free(ife_mesh_file* m);
+int ife_mesh_file_info(const ife_mesh_file* m, int32_t* dim, int64_t* n_nodes, int32_t* n_element_groups,
+                       int32_t* n_node_groups);
+/* Mesh::ElementGroups[group]: name, element type ("B2_2", "NURBS3_Cube2", ...), sizes. */
+int ife_mesh_file_element_group(const ife_mesh_file* m, int32_t group, const char** name, const char** type,
+                                int64_t* n_elems, int32_t* nodes_per_elem);
+/* Mesh::NodeGroups[group] (nodegroup.cpp): name and 0-based node ids; pointers stay valid until free. */
+int ife_mesh_file_node_group(const ife_mesh_file* m, int32_t group, const char** name, int64_t* n_nodes,
+                             const int64_t** nodes);
+/* The ife_mesh_desc of one element group (dense extraction operators), for ife_physics_create;
+ * its pointers point into m and stay valid until ife_mesh_file_free. */
+int ife_mesh_file_desc(const ife_mesh_file* m, int32_t group, ife_mesh_desc* out);
+/* SaveToHDF.m for one element group of B2_2 / NURBS3_Cube2 elements and optional node groups
+ * (group g: node_group_nodes[node_group_ptr[g] .. node_group_ptr[g+1]), 0-based), with a one-rank
+ * /Partition; strings are variable-length UTF-8 as MATLAB's 'string' type writes them. */
+int ife_mesh_save_hdf5(const char* path, const ife_mesh_desc* mesh, const char* group_name, int32_t n_node_groups,
+                       const char* const* node_group_names, const int64_t* node_group_ptr,
+                       const int64_t* node_group_nodes);
+
 const char* ife_last_error(void);
 int ife_api_version(void);
 
