@@ -65,6 +65,19 @@ class ModelOpts(C.Structure):
                [("porosity", C.c_double), ("phase_T_from_dof0", C.c_int32), ("T_default", C.c_double)]
 
 
+class SolveOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("restart", C.c_int32), ("max_iterations", C.c_int32),
+                ("preconditioner", C.c_int32)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("restarts", C.c_int32), ("converged", C.c_int32),
+                ("residual_norm", C.c_double), ("rhs_norm", C.c_double)]
+
+
+IFE_PC_NONE, IFE_PC_JACOBI, IFE_PC_ILU0 = 0, 1, 2
+
+
 class AssemblyStatus(C.Structure):
     _fields_ = [("elements_assembled", C.c_int64), ("return_map_calls", C.c_int64),
                 ("return_map_iterations", C.c_int64), ("return_map_maxit_hits", C.c_int64),
@@ -123,6 +136,7 @@ _PROTOS = [
     ("ife_physics_residual_norms", C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(C.c_double)]),
     ("ife_physics_residual_norms_async", C.c_int, [_P, C.c_int32, _P, _P, _P]),
     ("ife_residual_norms_noalloc", C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    ("ife_physics_solve", C.c_int, [_P, C.c_int32, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveInfo)]),
     ("ife_physics_set_time_scheme", C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
     ("ife_physics_set_dt", C.c_int, [_P, C.c_double, C.c_double]),
     ("ife_physics_update_state", C.c_int, [_P, C.c_int32, _P, _P]),

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_build")
 OUT = os.path.join(OUT_DIR, "libife.so")
-SOURCES = ["ife_host.cpp", "ife_mesh_hdf5.cpp", "ife_kernels.hip"]
+SOURCES = ["ife_host.cpp", "ife_mesh_hdf5.cpp", "ife_krylov.hip", "ife_kernels.hip"]
 HEADERS = [os.path.join(CSRC, "ife_internal.h"), os.path.join(ROOT, "include", "ife.h")]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -26,7 +26,8 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I" + os.path.join(ROOT, "include"),
 ]
-LINK = ["--offload-arch=gfx950", "-shared", "-fPIC", "-ldl"]
+LINK = ["--offload-arch=gfx950", "-shared", "-fPIC", "-ldl", "-L/opt/rocm/lib", "-lrocsparse",
+        "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def _compile(src, force, verbose):

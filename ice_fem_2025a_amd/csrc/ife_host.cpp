@@ -148,6 +148,7 @@ struct Step {
     DevBuf<int64_t> d_cdofs, d_free, d_crp, d_cfp;
     DevBuf<int32_t> d_ccol, d_rel, d_cfr;
     DevBuf<double> d_cvals, d_g, d_kcon, d_fcon, d_part, d_norms;
+    KrylovState* kry = nullptr;  // ife_physics_solve's buffers (freed with the physics)
 };
 
 }  // namespace
@@ -1013,7 +1014,11 @@ int ife_physics_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs, ife
     });
 }
 
-void ife_physics_destroy(ife_physics* ph) { delete ph; }
+void ife_physics_destroy(ife_physics* ph) {
+    if (!ph) return;
+    for (auto& st : ph->steps) krylov_destroy(st.kry);
+    delete ph;
+}
 
 int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* name, int32_t elem_group,
                           const ife_material* mat, const ife_phasefield_opts* pf, const ife_model_opts* opts,
@@ -1400,6 +1405,36 @@ int ife_physics_residual_norms(ife_physics* ph, int32_t step, const double* dx, 
         HIPCHK(launch_norms(dx, st.d_fcon.p, st.nfree, st.d_part.p, st.d_norms.p, s));
         HIPCHK(hipMemcpyAsync(out, st.d_norms.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int ife_physics_solve(ife_physics* ph, int32_t step, const ife_solve_opts* opts, double* dx_dev, void* stream,
+                      ife_solve_info* info) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (!dx_dev) THROW(IFE_ERR_INVALID, "null dx");
+        Step& st = ph->steps[step];
+        KrylovOpts o{1.0e-9, 1.0e-15, 100, 10000, IFE_PC_ILU0};  // GMRESLinSolver.cpp:23-24
+        if (opts) {
+            o = KrylovOpts{opts->rtol, opts->atol, opts->restart, opts->max_iterations, opts->preconditioner};
+            if (!(o.rtol >= 0) || !(o.atol >= 0)) THROW(IFE_ERR_INVALID, "tolerances must be non-negative");
+            if (o.restart < 1 || o.restart > 1000) THROW(IFE_ERR_INVALID, "restart must be in [1, 1000]");
+            if (o.max_it < 1) THROW(IFE_ERR_INVALID, "max_iterations must be positive");
+            if (o.pc < IFE_PC_NONE || o.pc > IFE_PC_ILU0) THROW(IFE_ERR_INVALID, "unknown preconditioner");
+        }
+        KrylovSystem sys;
+        if (st.has_cons) {
+            sys = KrylovSystem{st.nfree, (int64_t)st.c_col.size(), st.d_crp.p, st.d_ccol.p, st.d_kcon.p, st.d_fcon.p, 1.0};
+        } else {
+            sys = KrylovSystem{st.nrows, (int64_t)st.col.size(), st.d_row_ptr.p, st.d_col.p, st.K.p, st.f.p, -1.0};
+        }
+        if (!st.kry) st.kry = krylov_create();
+        KrylovResult r;
+        std::string err;
+        const int rc = krylov_solve(st.kry, sys, o, dx_dev, (hipStream_t)stream, &r, &err);
+        if (rc != IFE_OK) THROW(rc, "ife_physics_solve: " + err);
+        if (info) *info = ife_solve_info{r.iterations, r.restarts, r.converged, r.res_norm, r.rhs_norm};
     });
 }
 

@@ -208,4 +208,31 @@ hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipS
 hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s);
 int blocks_material(int dim, int64_t n_elems);
 
+// ---- linear solve (ife_krylov.hip) ----------------------------------------------------
+// The system K x = rhs_sign * rhs: n rows, CSR with 64-bit row offsets and sorted 32-bit
+// columns on the device.
+struct KrylovSystem {
+    int64_t n, nnz;
+    const int64_t* row_ptr;
+    const int32_t* col;
+    const double* val;
+    const double* rhs;
+    double rhs_sign;
+};
+struct KrylovOpts {
+    double rtol, atol;
+    int restart, max_it, pc;  // pc: ife_preconditioner
+};
+struct KrylovResult {
+    int iterations, restarts, converged;
+    double res_norm, rhs_norm;
+};
+struct KrylovState;  // buffers and the preconditioner analysis of one step's system
+KrylovState* krylov_create();
+void krylov_destroy(KrylovState* k);
+// GMRES(restart), right preconditioned, zero initial guess; returns an ife_status_code and
+// fills err on failure.  Synchronises `s` once per iteration.
+int krylov_solve(KrylovState* k, const KrylovSystem& sys, const KrylovOpts& o, double* x, hipStream_t s, KrylovResult* res,
+                 std::string* err);
+
 }  // namespace ife

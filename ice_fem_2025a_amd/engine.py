@@ -261,6 +261,31 @@ class Physics:
         abi.check(self.L.ife_physics_residual_norms(self.h, step, d.ptr, stream.handle if stream else None, out))
         return tuple(out)
 
+    # ---- the linear solve (GMRESLinSolver::solve) ---------------------------------------
+    def solve_device(self, step, dx_ptr, rtol=None, atol=None, restart=None, max_iterations=None,
+                     preconditioner=None, stream=None):
+        """GMRES on the step's constrained system (K dx = -f without constraints) into the device
+        vector dx_ptr; the reference's settings (rtol 1e-9, atol 1e-15, restart 100) where an
+        argument is None.  Returns the solve info as a dict."""
+        o = None
+        if any(v is not None for v in (rtol, atol, restart, max_iterations, preconditioner)):
+            o = abi.SolveOpts(1e-9 if rtol is None else rtol, 1e-15 if atol is None else atol,
+                              100 if restart is None else restart, 10000 if max_iterations is None else max_iterations,
+                              abi.IFE_PC_ILU0 if preconditioner is None else preconditioner)
+        info = abi.SolveInfo()
+        abi.check(self.L.ife_physics_solve(self.h, step, C.byref(o) if o is not None else None, dx_ptr,
+                                           stream.handle if stream else None, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in abi.SolveInfo._fields_}
+
+    def solve(self, step, **kw):
+        """solve_device into a fresh device vector; returns (dx on the host, info)."""
+        nf = C.c_int64()
+        rc = self.L.ife_physics_constrained(self.h, step, C.byref(nf), None, None, None, None, None, None, None)
+        n = nf.value if rc == abi.IFE_OK else self.n_rows(step)  # no constraints: all rows
+        d = hip.DeviceArray(n)
+        info = self.solve_device(step, d.ptr, **kw)
+        return d.get(), info
+
     # ---- state updates (Physics::UpdateState, UpdateVelAcc, Commit, ResetStep) ----------
     def set_time_scheme(self, scheme, c0=0.0, c1=0.0):
         abi.check(self.L.ife_physics_set_time_scheme(self.h, scheme, c0, c1))

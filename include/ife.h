@@ -378,6 +378,39 @@ int ife_physics_residual_norms_async(ife_physics* ph, int32_t step, const double
 #define IFE_NOALLOC_NORMS_MAX_N (INT64_C(1) << 53)
 int ife_residual_norms_noalloc(const double* dx, const double* f_con, int64_t n, double* out_dev, void* stream);
 
+/* ---- The linear solve of a Newton iteration (GMRESLinSolver::solve,
+ * Solvers/LinSolvers/GMRESLinSolver.cpp:19-96) -------------------------------------------- */
+typedef enum { IFE_PC_NONE = 0, IFE_PC_JACOBI = 1, IFE_PC_ILU0 = 2 } ife_preconditioner;
+
+/* KSPSetTolerances / KSPGMRESSetRestart (GMRESLinSolver.cpp:23-24): rtol 1e-9, atol 1e-15,
+ * restart 100, at most max_iterations (PETSc's default 10000).  The reference preconditions
+ * with diagonal scaling (:34-35) and ASM(1) with LU sub-solves (:37-70), on one rank a direct
+ * solve; here the default is ILU(0) of the diagonally scaled matrix. */
+typedef struct {
+    double rtol, atol;
+    int32_t restart, max_iterations, preconditioner;
+} ife_solve_opts;
+
+typedef struct {
+    int32_t iterations;      /* GMRES iterations (Krylov vectors) in all */
+    int32_t restarts;        /* GMRES cycles */
+    int32_t converged;       /* 1 if |r| <= max(rtol |b|, atol) */
+    double residual_norm;    /* |b - K dx|_2, the true residual after the last cycle */
+    double rhs_norm;         /* |b|_2 */
+} ife_solve_info;
+
+/* NonLinSolver's KSPSolve(K_con, f_con, dx) of `step` on the device (NonLinSolver.cpp:159-166,
+ * GMRESLinSolver.cpp:92): restarted GMRES, right preconditioned (so the residual it reduces
+ * is the true residual of K_con dx = f_con), zero initial guess.  The system is that of the
+ * last ife_physics_constrain of the step; a step without constraints solves K dx = -f (the
+ * constrained system with no constrained dofs, physics.cpp:329-352).  opts NULL: the
+ * reference's tolerances and ILU(0).  dx_dev: n_free (or n_rows) doubles.  The first call of
+ * a step allocates its Krylov basis and preconditioner (a set-up, like KSPSetUp); later
+ * calls reuse them.  Synchronises `stream` once per iteration.  A non-converged solve is
+ * not an error: info->converged is 0, as KSPSetErrorIfNotConverged(PETSC_FALSE) (:22). */
+int ife_physics_solve(ife_physics* ph, int32_t step, const ife_solve_opts* opts, double* dx_dev, void* stream,
+                      ife_solve_info* info);
+
 /* ---- State updates on the device (a Newton iteration without host round trips) ----
  * TimeScheme(inputs) (Physics/TimeSchemes.cpp:5-21): the scheme UpdateVelAcc uses with the
  * dt of ife_physics_set_time.  Default: Euler. */
