@@ -78,6 +78,11 @@ class SolveInfo(C.Structure):
 IFE_PC_NONE, IFE_PC_JACOBI, IFE_PC_ILU0 = 0, 1, 2
 
 
+class LineSearchInfo(C.Structure):
+    _fields_ = [("trials", C.c_int32), ("reassembled", C.c_int32), ("eta", C.c_double), ("e0", C.c_double),
+                ("e_min", C.c_double), ("L", C.c_double * 10), ("e", C.c_double * 10)]
+
+
 class AssemblyStatus(C.Structure):
     _fields_ = [("elements_assembled", C.c_int64), ("return_map_calls", C.c_int64),
                 ("return_map_iterations", C.c_int64), ("return_map_maxit_hits", C.c_int64),
@@ -136,6 +141,9 @@ _PROTOS = [
     ("ife_physics_residual_norms", C.c_int, [_P, C.c_int32, _P, _P, C.POINTER(C.c_double)]),
     ("ife_physics_residual_norms_async", C.c_int, [_P, C.c_int32, _P, _P, _P]),
     ("ife_residual_norms_noalloc", C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    ("ife_physics_constrain_f", C.c_int, [_P, C.c_int32, _P]),
+    ("ife_physics_line_search", C.c_int, [_P, C.c_int32, _P, C.c_double, C.c_double, _P, _P]),
+    ("ife_physics_line_search_info", C.c_int, [_P, C.c_int32, _P, C.POINTER(LineSearchInfo)]),
     ("ife_physics_solve", C.c_int, [_P, C.c_int32, C.POINTER(SolveOpts), _P, _P, C.POINTER(SolveInfo)]),
     ("ife_physics_set_time_scheme", C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
     ("ife_physics_set_dt", C.c_int, [_P, C.c_double, C.c_double]),

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_build")
 OUT = os.path.join(OUT_DIR, "libife.so")
-SOURCES = ["ife_host.cpp", "ife_mesh_hdf5.cpp", "ife_krylov.hip", "ife_kernels.hip"]
+SOURCES = ["ife_host.cpp", "ife_mesh_hdf5.cpp", "ife_krylov.hip", "ife_linesearch.hip", "ife_kernels.hip"]
 HEADERS = [os.path.join(CSRC, "ife_internal.h"), os.path.join(ROOT, "include", "ife.h")]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

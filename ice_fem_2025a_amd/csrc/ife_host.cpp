@@ -149,6 +149,7 @@ struct Step {
     DevBuf<int32_t> d_ccol, d_rel, d_cfr;
     DevBuf<double> d_cvals, d_g, d_kcon, d_fcon, d_part, d_norms;
     KrylovState* kry = nullptr;  // ife_physics_solve's buffers (freed with the physics)
+    DevBuf<LSDev> d_ls;          // the device line search's record
 };
 
 }  // namespace
@@ -676,9 +677,11 @@ KParams all_steps_params(ife_physics* ph) {
     return p;
 }
 
-void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* fv, hipStream_t s) {
+void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* fv, hipStream_t s,
+         const int* gate = nullptr) {
     require_ready(ph);
     KParams p = base_params(ph);
+    p.gate = gate;
     Model* pfm = ph->pf_model();
     Model* thm = ph->th_model();
     std::vector<int> which;
@@ -706,7 +709,7 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
     p.do_Tpf = pfm && pfm->HasT && pfm->opt.ViscousHeating && in(pfm->Step_T);
     p.do_th = thm && in(thm->Step_T);
     if (p.do_ph) ph->phase_assembled = true;
-    HIPCHK(launch_zero_status(ph->d_status.p, s));
+    HIPCHK(launch_zero_status(ph->d_status.p, s, gate));
     if (!(p.do_u || p.do_ph || p.do_Tpf || p.do_th)) return;
     // material update over all IPs, element records for all elements in one launch, then
     // the per-node gather into the CSR rows
@@ -733,8 +736,8 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
         timed(ph->mat_recs, 0, [&] { return launch_coss_material(p, ph->dim, s); });
         if (p.do_u) {
             const Step& su = ph->steps[pfm->Step_u];
-            HIPCHK(launch_fill(p.su.K, (int64_t)su.col.size(), 0.0, s));
-            HIPCHK(launch_fill(p.su.f, su.nrows, 0.0, s));
+            HIPCHK(launch_fill(p.su.K, (int64_t)su.col.size(), 0.0, s, gate));
+            HIPCHK(launch_fill(p.su.f, su.nrows, 0.0, s, gate));
             for (size_t c = 0; c + 1 < ph->color_ptr.size(); c++) {
                 KParams q = p;
                 q.elist = ph->d_colors.p + ph->color_ptr[c];
@@ -1331,17 +1334,21 @@ int ife_physics_set_constraints(ife_physics* ph, int32_t step, int64_t n, const 
         st.d_fcon.alloc(nfree);
         st.d_part.alloc((size_t)3 * norms_partials());
         st.d_norms.alloc(3);
+        st.d_ls.alloc(1);
         st.has_cons = true;
     });
 }
 
-int ife_physics_constrain(ife_physics* ph, int32_t step, void* stream) {
-    return guard([&] {
+namespace {
+// Physics::Constraint (f_only 0) or Physics::ConstraintFOnly (f_only 1) of a step
+void constrain_step(ife_physics* ph, int32_t step, int f_only, const int* gate, hipStream_t s) {
         require_ready(ph);
         if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
         Step& st = ph->steps[step];
         if (!st.has_cons) THROW(IFE_ERR_STATE, "no constraints set for this step (ife_physics_set_constraints)");
         ConsArgs a{};
+        a.f_only = f_only;
+        a.gate = gate;
         a.n = st.nrows;
         a.nc = (int64_t)st.d_cdofs.n;
         a.nfree = st.nfree;
@@ -1360,8 +1367,16 @@ int ife_physics_constrain(ife_physics* ph, int32_t step, void* stream) {
         a.g = st.d_g.p;
         a.Kc = st.d_kcon.p;
         a.fcon = st.d_fcon.p;
-        HIPCHK(launch_constrain(a, (hipStream_t)stream));
-    });
+        HIPCHK(launch_constrain(a, s));
+}
+}  // namespace
+
+int ife_physics_constrain(ife_physics* ph, int32_t step, void* stream) {
+    return guard([&] { constrain_step(ph, step, 0, nullptr, (hipStream_t)stream); });
+}
+
+int ife_physics_constrain_f(ife_physics* ph, int32_t step, void* stream) {
+    return guard([&] { constrain_step(ph, step, 1, nullptr, (hipStream_t)stream); });
 }
 
 int ife_physics_constrained(const ife_physics* ph, int32_t step, int64_t* n_free, int64_t* nnz,
@@ -1471,12 +1486,12 @@ int ife_physics_set_dt(ife_physics* ph, double t, double dt) {
     });
 }
 
-static void vel_acc_all(ife_physics* ph, hipStream_t s) {
+static void vel_acc_all(ife_physics* ph, hipStream_t s, const int* gate = nullptr) {
     for (int q = 0; q < ph->nsteps; q++) {
         Step& st = ph->steps[q];
         HIPCHK(launch_vel_acc(ph->ts, ph->ts_c0, ph->ts_c1, ph->dt, st.x[IFE_STATE].p, st.x[IFE_STATE_OLD].p,
                               st.x[IFE_DSTATE_OLD].p, st.x[IFE_DDSTATE_OLD].p, st.x[IFE_DSTATE].p, st.x[IFE_DDSTATE].p,
-                              st.nrows, s));
+                              st.nrows, s, gate));
     }
 }
 
@@ -1493,6 +1508,65 @@ int ife_physics_update_state(ife_physics* ph, int32_t step, const double* dx, vo
         else
             HIPCHK(launch_update_state(st.x[IFE_STATE].p, dx, nullptr, st.nrows, nullptr, nullptr, 0, s));
         vel_acc_all(ph, s);
+    });
+}
+
+int ife_physics_line_search(ife_physics* ph, int32_t step, double* dx, double lim_lo, double lim_hi, double* eta_dev,
+                            void* stream) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (!dx) THROW(IFE_ERR_INVALID, "null increment");
+        if (!(lim_lo > 0.0) || !(lim_hi >= lim_lo)) THROW(IFE_ERR_INVALID, "line-search limits must satisfy 0 < lo <= hi");
+        Step& st = ph->steps[step];
+        if (!st.has_cons) THROW(IFE_ERR_STATE, "the line search works on the constrained system (ife_physics_set_constraints)");
+        hipStream_t s = (hipStream_t)stream;
+        LSDev* ls = st.d_ls.p;
+        double* x = st.x[IFE_STATE].p;
+        const int64_t nc = (int64_t)st.d_cdofs.n;
+        HIPCHK(launch_ls_init(ls, st.d_fcon.p, dx, st.nfree, s));
+        for (int it = 0; it < LS_MAX; it++) {
+            const int* gate = &ls->run[it];
+            const double* L = &ls->L[it];
+            // UpdateState(L dx), AssembleFOnly, ConstraintFOnly, e = f_con . (L dx) (:320-327)
+            HIPCHK(launch_update_state(x, dx, st.d_free.p, st.nfree, st.d_cdofs.p, st.d_g.p, nc, s, L, 1.0, gate));
+            vel_acc_all(ph, s, gate);
+            run(ph, false, step, nullptr, nullptr, s, gate);
+            constrain_step(ph, step, 1, gate, s);
+            HIPCHK(launch_ls_trial_end(ls, it, st.d_fcon.p, dx, st.nfree, lim_lo, lim_hi, s));
+            // UpdateState(-L dx) (:354-356)
+            HIPCHK(launch_update_state(x, dx, st.d_free.p, st.nfree, st.d_cdofs.p, st.d_g.p, nc, s, L, -1.0, gate));
+            vel_acc_all(ph, s, gate);
+        }
+        // dx *= Lcorr, UpdateState(dx) (:379-380)
+        HIPCHK(launch_ls_finish(ls, dx, st.nfree, eta_dev, s));
+        HIPCHK(launch_update_state(x, dx, st.d_free.p, st.nfree, st.d_cdofs.p, st.d_g.p, nc, s));
+        vel_acc_all(ph, s);
+        // SolveStep_Standard: Assemble + Constraint when eta < 1 - 1e-6 (NonLinSolver.cpp:170-175)
+        run(ph, false, step, nullptr, nullptr, s, &ls->reassemble);
+        constrain_step(ph, step, 0, &ls->reassemble, s);
+    });
+}
+
+int ife_physics_line_search_info(ife_physics* ph, int32_t step, void* stream, ife_line_search_info* out) {
+    return guard([&] {
+        require_ready(ph);
+        if (step < 0 || step >= ph->nsteps) THROW(IFE_ERR_INVALID, "step out of range");
+        if (!out) THROW(IFE_ERR_INVALID, "null argument");
+        Step& st = ph->steps[step];
+        if (!st.has_cons) THROW(IFE_ERR_STATE, "no constraints set for this step (ife_physics_set_constraints)");
+        LSDev h;
+        HIPCHK(hipMemcpyAsync(&h, st.d_ls.p, sizeof(LSDev), hipMemcpyDeviceToHost, (hipStream_t)stream));
+        HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+        out->trials = h.trials;
+        out->eta = h.eta;
+        out->e0 = h.e0;
+        out->e_min = h.eMin;
+        out->reassembled = h.reassemble;
+        for (int i = 0; i < 10; i++) {
+            out->L[i] = h.L[i];
+            out->e[i] = h.e[i];
+        }
     });
 }
 

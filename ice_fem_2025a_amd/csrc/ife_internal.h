@@ -141,6 +141,9 @@ struct KParams {
     // 18x18, w sigma (18), w DamI, w DamG (lumped inertia and gravity weights)
     int coss;
     double* cossb;
+    // device flag: when set and zero, every kernel of the assembly returns at once (the
+    // skipped trials of the device line search); null: always run
+    const int* gate;
 };
 enum { COSS_C = 0, COSS_SIG = 324, COSS_WI = 342, COSS_WG = 343, COSS_NF = 344 };
 
@@ -175,6 +178,8 @@ struct ConsArgs {
     const double* cvals;
     const int32_t *rel, *cfr;
     double *g, *Kc, *fcon;
+    int f_only;        // Physics::ConstraintFOnly: f_con from the stored g, K_con untouched
+    const int* gate;   // as KParams::gate
 };
 hipError_t launch_constrain(const ConsArgs& a, hipStream_t s);
 // sum |dx| / n, sum |fc| / n, sum |fc dx| / n into out[3] (device); part: norms_partials() x 3
@@ -185,11 +190,13 @@ void set_last_error(const std::string& msg);
 // launch_norms with its partials in a file-scope device array (no allocation)
 hipError_t launch_norms_noalloc(const double* dx, const double* fc, int64_t n, double* out, hipStream_t s);
 // x[free[k]] += dx[k] (free NULL: x[k] += dx[k] over n), then x[cd[i]] -= g[cd[i]] (g = -C c)
+// scale (device): the increment is (sgn * *scale) dx; gate as KParams::gate
 hipError_t launch_update_state(double* x, const double* dx, const int64_t* free, int64_t n, const int64_t* cd,
-                               const double* g, int64_t nc, hipStream_t s);
+                               const double* g, int64_t nc, hipStream_t s, const double* scale = nullptr, double sgn = 1.0,
+                               const int* gate = nullptr);
 // TimeScheme::UpdateVelAcc on n entries: v, a from x, x_old, v_old, a_old
 hipError_t launch_vel_acc(int scheme, double c0, double c1, double dt, const double* x, const double* xo, const double* vo,
-                          const double* ao, double* v, double* a, int64_t n, hipStream_t s);
+                          const double* ao, double* v, double* a, int64_t n, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_gather(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s);
 // Cosserat: material update of every IP (u, phase, T coefficients), then the u-step element
@@ -197,9 +204,9 @@ hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_coss_material(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_coss_assemble(const KParams& p, int dim, hipStream_t s);
 int record_size(int dim);
-hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s);
+hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
-hipError_t launch_zero_status(Status* st, hipStream_t s);
+hipError_t launch_zero_status(Status* st, hipStream_t s, const int* gate = nullptr);
 // batched forms; grid = number of workgroups launched (0 or >= the total: one per block,
 // else persistent workgroups striding over the blocks); nta = lanes of the element kernel
 hipError_t launch_batch_zero(const BatchArgs& a, hipStream_t s);
@@ -207,6 +214,25 @@ hipError_t launch_batch_material(const BatchArgs& a, int64_t grid, hipStream_t s
 hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipStream_t s);
 hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s);
 int blocks_material(int dim, int64_t n_elems);
+
+// ---- line search (ife_linesearch.hip): NonLinSolver::HOLineSearch on the device ----------
+constexpr int LS_MAX = 10;   // maxLS (NonLinSolver.cpp:313)
+constexpr int LS_NB = 256;   // blocks of its dot products
+struct LSDev {
+    double e0, Lcorr, eMin, eta;
+    double L[LS_MAX + 2], e[LS_MAX + 2];
+    int run[LS_MAX + 2];     // trial it runs iff run[it] (the gate of its kernels)
+    int found, reassemble, trials, pad;
+    double part[LS_NB];
+};
+// e0 = f_con . dx; L[0] = 1, run[0] = 1, the other trials off
+hipError_t launch_ls_init(LSDev* ls, const double* fcon, const double* dx, int64_t n, hipStream_t s);
+// after trial it (gated by run[it]): e[it] = f_con . (L[it] dx), the acceptance test and the
+// next step length, run[it + 1]
+hipError_t launch_ls_trial_end(LSDev* ls, int it, const double* fcon, const double* dx, int64_t n, double lim_lo,
+                               double lim_hi, hipStream_t s);
+// dx *= Lcorr; eta = Lcorr (also to eta_out when set); reassemble = Lcorr < 1 - 1e-6
+hipError_t launch_ls_finish(LSDev* ls, double* dx, int64_t n, double* eta_out, hipStream_t s);
 
 // ---- linear solve (ife_krylov.hip) ----------------------------------------------------
 // The system K x = rhs_sign * rhs: n rows, CSR with 64-bit row offsets and sorted 32-bit

@@ -1625,6 +1625,7 @@ __device__ __forceinline__ int batch_domain(I64C* bptr, int nd, int64_t b) {
 
 template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL>
 __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
+    if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     material_body<DIM, VEP, EXACT, ADJ, FULL>(p, blockIdx.x);
 }
 
@@ -2067,6 +2068,7 @@ __device__ __forceinline__ void assemble_body(const KParams& p, const int64_t bl
 
 template <int DIM, typename T, typename R, int NTA>
 __global__ __launch_bounds__(NTA, sizeof(T) == 8 ? 3 : 4) void assemble_kernel(KParams p) {
+    if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     assemble_body<DIM, T, R, NTA>(p, blockIdx.x);
 }
 
@@ -2324,6 +2326,7 @@ constexpr int W2_NW = 4;  // elements (wavefronts) per workgroup of the 2D wave 
 
 template <typename T, typename R>
 __global__ __launch_bounds__(64 * W2_NW) void assemble2d_wave(KParams p) {
+    if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     __shared__ Wave2D<T, R> sm[W2_NW];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t blk = (int64_t)blockIdx.x * W2_NW + wv;
@@ -2519,6 +2522,7 @@ __device__ void gj_inverse(double A[M][M], double Ai[M][M]) {
 
 template <int DIM, typename R>
 __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
+    if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     using H = HoSmem<DIM>;
     using RC = Rec<DIM>;
     constexpr int N = H::N, NIP = H::NIP, NG2 = H::NG2, NK = H::NK, NR = H::NR, CH = H::CH, NT = 256;
@@ -2865,6 +2869,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
 
 template <int DIM, typename T>
 __global__ __launch_bounds__(GW) void gather_kernel(KParams p) {
+    if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     gather_body<DIM, T>(p, blockIdx.x);
 }
 
@@ -2886,13 +2891,15 @@ __global__ __launch_bounds__(GW) void gather_batch(const KParams* P_, const int6
     }
 }
 
-__global__ void fill_kernel(double* x, int64_t n, double v) {
+__global__ void fill_kernel(double* x, int64_t n, double v, const int* gate) {
+    if (gate && !*gate) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) x[i] = v;
 }
 
-__global__ void zero_status_kernel(Status* s) {
+__global__ void zero_status_kernel(Status* s, const int* gate) {
+    if (gate && !*gate) return;
     if (threadIdx.x == 0) {
         s->elements = 0; s->rm_calls = 0; s->rm_iters = 0; s->rm_maxit = 0; s->nonfinite = 0;
     }
@@ -2928,7 +2935,9 @@ __global__ __launch_bounds__(256) void spmv_kernel(const int64_t* rp, const int3
 //   f_con = -U^T (f - K C c),   c_d = value_d - x_d  (the constraint's remaining increment).
 // ---------------------------------------------------------------------------------------
 // g = -C c: written at the constrained dofs; the free entries of g stay zero
-__global__ void cons_g_kernel(const int64_t* cd, const double* cv, const double* x, double* g, int64_t nc) {
+__global__ void cons_g_kernel(const int64_t* cd, const double* cv, const double* x, double* g, int64_t nc,
+                              const int* gate) {
+    if (gate && !*gate) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nc) g[cd[i]] = -(cv[i] - x[cd[i]]);
 }
@@ -2937,7 +2946,8 @@ __global__ void cons_g_kernel(const int64_t* cd, const double* cv, const double*
 // of g; their offsets in the source row are listed per free row (cfp/cfr), most lists empty
 __global__ void cons_f_kernel(const int64_t* fr, const double* f, const int64_t* krp, const int32_t* kcol,
                               const double* Kv, const double* g, const int64_t* cfp, const int32_t* cfr, double* fc,
-                              int64_t nfree) {
+                              int64_t nfree, const int* gate) {
+    if (gate && !*gate) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nfree) return;
     const int64_t r = fr[k], s0 = krp[r];
@@ -2952,7 +2962,9 @@ __global__ void cons_f_kernel(const int64_t* fr, const double* f, const int64_t*
 // K_con values: one wavefront per free row; entry e of the row is the row's source entry
 // rel[e] of K (relative to the source row's start)
 __global__ __launch_bounds__(256) void cons_k_kernel(const int64_t* crp, const int32_t* rel, const int64_t* krp,
-                                                      const int64_t* fr, const double* Kv, double* Kc, int64_t nfree) {
+                                                      const int64_t* fr, const double* Kv, double* Kc, int64_t nfree,
+                                                      const int* gate) {
+    if (gate && !*gate) return;
     const int64_t k = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
     if (k >= nfree) return;
@@ -3517,6 +3529,7 @@ struct CossSmem {
 // GEN: the StarConvex, DruckerPrager or consistent SpecStress split (coss_split_full)
 template <int DIM, bool GEN>
 __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
+    if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     using SM = CossSmem<DIM>;
     constexpr int N = SM::N, NIP = SM::NIP, EPB = SM::EPB, NFU = SM::NFU;
     __shared__ SM sm;
@@ -3898,6 +3911,7 @@ struct CossRows {
 // (:454-527), per (field, node).
 template <int DIM, typename T>
 __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
+    if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     using CR = CossRows<DIM>;
     typedef typename Mfma<T>::V V;
     constexpr int N = DIM == 2 ? 9 : 27, NIP = N, NP = DIM == 2 ? 16 : 32, MT = NP / 16, NB = 1 + DIM,
@@ -4216,14 +4230,17 @@ hipError_t launch_spmv(const int64_t* row_ptr, const int32_t* col, const double*
 }
 
 hipError_t launch_constrain(const ConsArgs& a, hipStream_t s) {
-    if (a.nc > 0)
+    // f_only (Physics::ConstraintFOnly, physics.cpp:314-325): f_con from the g of the last full
+    // elimination, K_con untouched
+    if (a.nc > 0 && !a.f_only)
         hipLaunchKernelGGL(cons_g_kernel, dim3((unsigned)((a.nc + 255) / 256)), dim3(256), 0, s, a.cdofs, a.cvals, a.x, a.g,
-                           a.nc);
+                           a.nc, a.gate);
     if (a.nfree > 0) {
         hipLaunchKernelGGL(cons_f_kernel, dim3((unsigned)((a.nfree + 255) / 256)), dim3(256), 0, s, a.free, a.f, a.krp,
-                           a.kcol, a.Kv, a.g, a.cfp, a.cfr, a.fcon, a.nfree);
-        hipLaunchKernelGGL(cons_k_kernel, dim3((unsigned)((a.nfree + 3) / 4)), dim3(256), 0, s, a.crp, a.rel, a.krp, a.free,
-                           a.Kv, a.Kc, a.nfree);
+                           a.kcol, a.Kv, a.g, a.cfp, a.cfr, a.fcon, a.nfree, a.gate);
+        if (!a.f_only)
+            hipLaunchKernelGGL(cons_k_kernel, dim3((unsigned)((a.nfree + 3) / 4)), dim3(256), 0, s, a.crp, a.rel, a.krp,
+                               a.free, a.Kv, a.Kc, a.nfree, a.gate);
     }
     return hipGetLastError();
 }
@@ -4236,27 +4253,39 @@ hipError_t launch_norms(const double* dx, const double* fc, int64_t n, double* p
 
 int norms_partials() { return NORM_NB; }
 
-// Physics::UpdateState (physics.cpp:408-411): x += U dx, then + C c at the constrained dofs
-__global__ void update_free_kernel(double* x, const double* dx, const int64_t* fr, int64_t n) {
+// Physics::UpdateState (physics.cpp:408-411): x += U dx, then + C c at the constrained dofs.
+// With `scale` (device), the increment is (sgn * scale) dx, the VecScale'd copy of the
+// reference's line search (NonLinSolver.cpp:323-324, :356-357).
+__global__ void update_free_kernel(double* x, const double* dx, const int64_t* fr, int64_t n, const double* scale,
+                                   double sgn, const int* gate) {
+    if (gate && !*gate) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) x[fr ? fr[k] : k] += dx[k];
+    if (k >= n) return;
+    if (scale) x[fr ? fr[k] : k] += (sgn * *scale) * dx[k];
+    else x[fr ? fr[k] : k] += dx[k];
 }
-__global__ void update_cons_kernel(double* x, const int64_t* cd, const double* g, int64_t nc) {
+__global__ void update_cons_kernel(double* x, const int64_t* cd, const double* g, int64_t nc, const int* gate) {
+    if (gate && !*gate) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nc) x[cd[i]] -= g[cd[i]];
 }
 
 hipError_t launch_update_state(double* x, const double* dx, const int64_t* free, int64_t n, const int64_t* cd,
-                               const double* g, int64_t nc, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(update_free_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, dx, free, n);
-    if (nc > 0) hipLaunchKernelGGL(update_cons_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, x, cd, g, nc);
+                               const double* g, int64_t nc, hipStream_t s, const double* scale, double sgn,
+                               const int* gate) {
+    if (n > 0)
+        hipLaunchKernelGGL(update_free_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, dx, free, n, scale, sgn,
+                           gate);
+    if (nc > 0)
+        hipLaunchKernelGGL(update_cons_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, x, cd, g, nc, gate);
     return hipGetLastError();
 }
 
 // TimeScheme::UpdateVelAcc (TimeSchemes.cpp:36-62), each VecAXPBYPCZ / VecAYPX of the
 // reference as one expression per entry, in its order
 __global__ void vel_acc_kernel(int scheme, double c0, double c1, double dt, const double* x, const double* xo,
-                               const double* vo, const double* ao, double* v, double* a, int64_t n) {
+                               const double* vo, const double* ao, double* v, double* a, int64_t n, const int* gate) {
+    if (gate && !*gate) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double X = x[i], XO = xo[i], VO = vo[i], AO = ao[i];
@@ -4281,18 +4310,18 @@ __global__ void vel_acc_kernel(int scheme, double c0, double c1, double dt, cons
 }
 
 hipError_t launch_vel_acc(int scheme, double c0, double c1, double dt, const double* x, const double* xo, const double* vo,
-                          const double* ao, double* v, double* a, int64_t n, hipStream_t s) {
+                          const double* ao, double* v, double* a, int64_t n, hipStream_t s, const int* gate) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(vel_acc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, scheme, c0, c1, dt, x, xo, vo, ao,
-                       v, a, n);
+                       v, a, n, gate);
     return hipGetLastError();
 }
 
-hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s) {
+hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s, const int* gate) {
     if (n <= 0) return hipSuccess;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, v);
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, v, gate);
     return hipGetLastError();
 }
 
@@ -4304,8 +4333,8 @@ hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_zero_status(Status* st, hipStream_t s) {
-    hipLaunchKernelGGL(zero_status_kernel, dim3(1), dim3(64), 0, s, st);
+hipError_t launch_zero_status(Status* st, hipStream_t s, const int* gate) {
+    hipLaunchKernelGGL(zero_status_kernel, dim3(1), dim3(64), 0, s, st, gate);
     return hipGetLastError();
 }
 

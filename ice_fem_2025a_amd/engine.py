@@ -261,6 +261,22 @@ class Physics:
         abi.check(self.L.ife_physics_residual_norms(self.h, step, d.ptr, stream.handle if stream else None, out))
         return tuple(out)
 
+    def constrain_f(self, step, stream=None):
+        """Physics::ConstraintFOnly: f_con of the last assembly with the stored c; K_con kept."""
+        abi.check(self.L.ife_physics_constrain_f(self.h, step, stream.handle if stream else None))
+
+    def line_search(self, step, dx_ptr, lim_lo, lim_hi, eta_ptr=None, stream=None):
+        """NonLinSolver::HOLineSearch on the device (dx scaled in place, state updated,
+        re-assembled when eta < 1 - 1e-6); stream-ordered, capturable."""
+        abi.check(self.L.ife_physics_line_search(self.h, step, dx_ptr, lim_lo, lim_hi, eta_ptr,
+                                                 stream.handle if stream else None))
+
+    def line_search_info(self, step, stream=None):
+        i = abi.LineSearchInfo()
+        abi.check(self.L.ife_physics_line_search_info(self.h, step, stream.handle if stream else None, C.byref(i)))
+        return {"trials": i.trials, "reassembled": i.reassembled, "eta": i.eta, "e0": i.e0, "e_min": i.e_min,
+                "L": list(i.L)[:max(i.trials, 1)], "e": list(i.e)[:i.trials]}
+
     # ---- the linear solve (GMRESLinSolver::solve) ---------------------------------------
     def solve_device(self, step, dx_ptr, rtol=None, atol=None, restart=None, max_iterations=None,
                      preconditioner=None, stream=None):

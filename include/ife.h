@@ -427,6 +427,35 @@ int ife_physics_set_dt(ife_physics* ph, double t, double dt);
  * TimeSchemes.cpp:36-62) for every step. */
 int ife_physics_update_state(ife_physics* ph, int32_t step, const double* dx, void* stream);
 
+/* Physics::ConstraintFOnly (physics.cpp:314-325): f_con = -U^T (f - K C c) of the last
+ * assembly with the c of the last ife_physics_constrain; K_con is left as it was. */
+int ife_physics_constrain_f(ife_physics* ph, int32_t step, void* stream);
+
+/* NonLinSolver::HOLineSearch (Solvers/NonLinSolver.cpp:312-384) on the constrained system of
+ * `step`, and the re-assembly SolveStep_Standard runs after it when eta < 1 - 1e-6
+ * (:169-175), entirely on the device and stream-ordered: no host synchronisation, so a
+ * whole Newton iteration can be captured in a hipGraph.  dx (device, n_free doubles) is the
+ * solver's increment, f_con that of the last ife_physics_constrain; lim_lo, lim_hi are
+ * nonlinSolver.linesearchLims (:50-52).  Up to 10 trials (x += L dx, Assemble,
+ * ConstraintFOnly, e = f_con . (L dx), x -= L dx) with the reference's acceptance test and
+ * step rule; trials after the accepted one are launched but return at once (their kernels
+ * read a device flag).  On return dx holds eta dx, the state is updated by it, and K, f,
+ * K_con, f_con are those of the new state (re-assembled unless eta >= 1 - 1e-6, where, as
+ * in the reference, f_con is from the last trial and K_con from before the search).  eta_dev
+ * (device double, may be NULL) receives eta.  Needs ife_physics_set_constraints for the step. */
+int ife_physics_line_search(ife_physics* ph, int32_t step, double* dx, double lim_lo, double lim_hi, double* eta_dev,
+                            void* stream);
+
+/* The record of the last line search of a step (synchronises `stream`). */
+typedef struct {
+    int32_t trials;       /* trial assemblies run */
+    int32_t reassembled;  /* 1 if eta < 1 - 1e-6 triggered the re-assembly */
+    double eta;           /* the accepted step length L_Search_corr */
+    double e0, e_min;     /* f_con . dx before the search; the accepted trial's value */
+    double L[10], e[10];  /* step length and f_con . (L dx) of each trial */
+} ife_line_search_info;
+int ife_physics_line_search_info(ife_physics* ph, int32_t step, void* stream, ife_line_search_info* out);
+
 /* The state part of Physics::Commit(TIMEDEP) (physics.cpp:501-527): UpdateVelAcc, then the
  * old vectors (x, v, a) take the current ones, every step.  Model history commits stay with
  * ife_model_commit. */
