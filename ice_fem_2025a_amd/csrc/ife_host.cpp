@@ -189,6 +189,11 @@ struct ife_physics {
     std::vector<int> fpos;          // field -> position within its step (-1 unused)
     // device
     DevBuf<double> d_nodes, d_ext, d_lcrack, d_ipb, d_ebuf;
+    // Cosserat u-step element records for crec_cap elements at a time (FP64-sized; mixed
+    // mode's FP32 records fit twice as many); crec_chunk: elements per chunk set by the
+    // "coss_record_chunk" parameter (0: as many as fit, negative: the coloured additions)
+    DevBuf<double> d_crec;
+    int64_t crec_cap = 0, crec_chunk = 0;
     DevBuf<int64_t> d_n2e_ptr;
     DevBuf<int32_t> d_n2e_elem;
     DevBuf<uint8_t> d_n2e_loc;
@@ -517,7 +522,8 @@ void setup_device(ife_physics* ph) {
     ph->d_colors.upload(ph->color_elems.data(), ph->color_elems.size());
     ph->d_status.alloc(1);
     HIPCHK(hipMemset(ph->d_status.p, 0, sizeof(Status)));
-    ph->d_ipb.alloc((size_t)IPB_NF * ph->NE * ph->nip);
+    // 2D stores the 29 rows its element kernel reads (W2C_NF in ife_kernels.hip), 3D all 70
+    ph->d_ipb.alloc((size_t)(ph->dim == 2 ? 29 : IPB_NF) * ph->NE * ph->nip);
     ph->d_ebuf.alloc((size_t)ph->NE * ph->n * record_size(ph->dim));
     ph->d_n2e_ptr.upload(ph->n2e_ptr.data(), ph->n2e_ptr.size());
     ph->d_n2e_elem.upload(ph->n2e_elem.data(), ph->n2e_elem.size());
@@ -552,6 +558,22 @@ void setup_device(ife_physics* ph) {
         m->A_e.alloc(ph->NE);
         DevBuf<double>* all[] = {&m->E, &m->EOld, &m->VD, &m->VDOld, &m->DVD, &m->DVDOld, &m->PS, &m->PSOld, &m->MH, &m->MHOld};
         for (auto* b : all) HIPCHK(hipMemset(b->p, 0, b->n * sizeof(double)));
+    }
+    // Cosserat u-step records: up to 64 GiB, leaving 16 GiB of the device free; without room
+    // for 256 elements the u step falls back to the coloured additions into the CSR
+    bool any_coss = false;
+    for (auto* m : ph->models) any_coss = any_coss || (m->type == MT_PHASEFIELD && m->coss);
+    if (any_coss) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const size_t per = (size_t)ph->n * coss_record_size(ph->dim) * sizeof(double);
+        const size_t reserve = (size_t)16 << 30, cap = (size_t)64 << 30;
+        const size_t budget = fr > reserve ? std::min(fr - reserve, cap) : 0;
+        const int64_t ne = std::min<int64_t>(ph->NE, (int64_t)(budget / per));
+        if (ne >= std::min<int64_t>(ph->NE, 256)) {
+            ph->d_crec.alloc((size_t)ne * ph->n * coss_record_size(ph->dim));
+            ph->crec_cap = ne;
+        }
     }
     HIPCHK(hipDeviceSynchronize());
     ph->ready = true;
@@ -734,7 +756,22 @@ void run(ife_physics* ph, bool all, int step, double* const* Kv, double* const* 
         // blocks go straight into the zeroed CSR values colour by colour; the phase and T
         // steps use the shared element and gather kernels
         timed(ph->mat_recs, 0, [&] { return launch_coss_material(p, ph->dim, s); });
-        if (p.do_u) {
+        if (p.do_u && ph->crec_cap > 0 && ph->crec_chunk >= 0) {
+            // element records of a chunk of elements, then the rows of the chunk's nodes,
+            // chunk after chunk in element order (FP32 records: twice the elements per chunk)
+            int64_t chunk = ph->crec_cap * (p.mixed ? 2 : 1);
+            if (ph->crec_chunk > 0) chunk = std::min(chunk, ph->crec_chunk);
+            for (int64_t e0 = 0; e0 < ph->NE; e0 += chunk) {
+                const int64_t e1 = std::min(ph->NE, e0 + chunk);
+                KParams q = p;
+                q.elist = nullptr;
+                q.nlist = e1 - e0;
+                q.crec = ph->d_crec.p;
+                q.crec_e0 = e0;
+                timed(ph->recs, q.nlist, [&] { return launch_coss_assemble(q, ph->dim, s); });
+                timed(ph->gat_recs, 0, [&] { return launch_coss_gather(q, ph->dim, e0, e1, s); });
+            }
+        } else if (p.do_u) {
             const Step& su = ph->steps[pfm->Step_u];
             HIPCHK(launch_fill(p.su.K, (int64_t)su.col.size(), 0.0, s, gate));
             HIPCHK(launch_fill(p.su.f, su.nrows, 0.0, s, gate));
@@ -1735,6 +1772,12 @@ int ife_model_set_param(ife_physics* ph, int32_t model, const char* name, double
         else if (nm == "rm_rel_tol") m->rm_erel = value;
         else if (nm == "rm_max_iterations") m->rm_maxit = (int)value;
         else if (nm == "exact_tangent") m->exact_tangent = value != 0.0;
+        else if (nm == "coss_record_chunk") {
+            // elements per u-step record chunk: 0 as many as fit, negative: no records (the
+            // element blocks added into the CSR colour by colour)
+            if (!m->coss) THROW(IFE_ERR_INVALID, "coss_record_chunk belongs to a Fracture/CossPhaseField model");
+            ph->crec_chunk = (int64_t)value;
+        }
         else THROW(IFE_ERR_NOT_FOUND, "unknown parameter " + nm);
     });
 }

@@ -125,7 +125,7 @@ struct KParams {
     int64_t NIPT;
     Status* st;
     double* lcrack_elem;     // [e]
-    double* ipb;             // per-IP coefficients [e][IPB_NF][nip]
+    double* ipb;             // per-IP coefficients [e][rows][nip]: IPB_NF rows in 3D, 29 in 2D
     const double* Gc_e;      // per-element Gc of the phase step (null: SetTemperature's)
     const double* A_e;       // per-element creep prefactor A0 (null: the material's)
     // adjoint: cotangents of the u, phase and T step residuals (global DOF order of each step)
@@ -137,15 +137,21 @@ struct KParams {
     const int32_t* n2e_elem;
     const uint8_t* n2e_loc;  // local index of the node in that element
     int mixed;               // tangent contractions in FP32
-    // Cosserat model: per-IP coefficients [e][COSS_NF][nip]: w (Dam D_coss DPlast) row-major
-    // 18x18, w sigma (18), w DamI, w DamG (lumped inertia and gravity weights)
+    // Cosserat model: per-IP coefficients [e][COSS_NF][nip]: the u-step slots M^{ab}_kl
+    // (B-row-weighted sums of C = w Dam D_coss DPlast, 225 in 3D, 49 in 2D), w sigma (18),
+    // w DamI, w DamG (lumped inertia and gravity weights)
     int coss;
     double* cossb;
     // device flag: when set and zero, every kernel of the assembly returns at once (the
     // skipped trials of the device line search); null: always run
     const int* gate;
+    // Cosserat u step through element records: crec holds the records of the elements
+    // [crec_e0, crec_e0 + chunk) (record of element e, node a at (e - crec_e0) * n + a); null:
+    // the element blocks are added into the CSR values colour by colour
+    void* crec;
+    int64_t crec_e0;
 };
-enum { COSS_C = 0, COSS_SIG = 324, COSS_WI = 342, COSS_WG = 343, COSS_NF = 344 };
+enum { COSS_M = 0, COSS_SIG = 225, COSS_WI = 243, COSS_WG = 244, COSS_NF = 245 };
 
 // One batched launch over nd domains: P[d] is domain d's KParams (device memory), bptr[k]
 // (device, nd + 1 entries) the prefix sums of the domains' block counts for kernel k
@@ -203,6 +209,13 @@ hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s);
 // kernel over one colour of elements (adds into the zeroed CSR values and f)
 hipError_t launch_coss_material(const KParams& p, int dim, hipStream_t s);
 hipError_t launch_coss_assemble(const KParams& p, int dim, hipStream_t s);
+// the u-step CSR rows and residuals of every node with an element in [e0, e1) from the
+// records of those elements (p.crec), added in element order onto the rows' partial sums of
+// the elements before e0 (zero when there are none)
+hipError_t launch_coss_gather(const KParams& p, int dim, int64_t e0, int64_t e1, hipStream_t s);
+// entries of one Cosserat u-step record (6 or 3 fields squared times the element's nodes,
+// plus the residuals)
+int coss_record_size(int dim);
 int record_size(int dim);
 hipError_t launch_fill(double* x, int64_t n, double v, hipStream_t s, const int* gate = nullptr);
 hipError_t launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <utility>
 
 #include "ife_internal.h"
 
@@ -44,6 +45,10 @@ enum {
     W2C_LD = 21, W2C_GW = 22, W2C_GC = 23, W2C_JI = 24,        // JI 2 x 2
     W2C_W = 28, W2C_NF = 29, W2C_Z = 29                        // row 29: zeros
 };
+// rows of one element's per-IP coefficient block: every IPB_* row in 3D, the W2C_* rows in 2D
+template <int DIM>
+constexpr int ipb_rows() { return DIM == 3 ? IPB_NF : W2C_NF; }
+
 // storage row of IPB row f in DIM dimensions (-1: not stored)
 template <int DIM>
 __device__ __forceinline__ int ipb_slot(int f) {
@@ -91,6 +96,15 @@ struct Rec {
     // entry keeps the length even
     static constexpr int KU = 0, FU = DIM * DIM * N, KP = FU + DIM, FP = KP + N, KT = FP + 1, FT = KT + N, WL = FT + 1,
                          SIZE = FT + 3;
+};
+
+// Cosserat u-step element record of node a: the node's NFU rows of the element block, column
+// field beta then node j ([alpha][beta][j]), then the NFU residuals.  Even length, so FP32
+// records load in pairs.
+template <int DIM>
+struct CRec {
+    static constexpr int N = DIM == 2 ? 9 : 27, NFU = DIM == 2 ? 3 : 6, F = NFU * NFU * N, SIZE = F + NFU;
+    static_assert(SIZE % 2 == 0, "record length must be even");
 };
 
 // Voigt row (xx,yy,zz,xy,yz,xz) holding d u_alpha / d x_c (ConstructBMat, utility/math.cpp:3-26)
@@ -1142,7 +1156,7 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
     unsigned my_calls = 0, my_iters = 0, my_maxit = 0, my_bad = 0;
     if (le < EPB && el < p.NE) {
         const int64_t gi = el * NIP + ip;
-        double* const ipo = p.ipb + el * IPB_NF * NIP + ip;  // [e][field][ip]
+        double* const ipo = p.ipb + el * ipb_rows<DIM>() * NIP + ip;  // [e][field][ip]
         const int64_t S = p.NIPT;
         int q[3] = {ip % 3, (ip / 3) % 3, ip / 9};
         // 1D B-spline values/derivatives at this IP: b = C_d * Bernstein(xi_q)
@@ -2146,7 +2160,7 @@ struct W2Load {
 
 __device__ __forceinline__ void w2_load(const KParams& p, const int64_t el, const int lane, W2Load& L) {
     constexpr int N = 9, NIP = 9;
-    const double* const src = p.ipb + el * IPB_NF * NIP;
+    const double* const src = p.ipb + el * ipb_rows<2>() * NIP;
     L.r = lane < N ? p.erank[el * N + lane] : 0;
     const bool ex = lane >= 32 && lane < 32 + 18;
     const int xd = (lane - 32) / 9, xa = (lane - 32) % 3;
@@ -2546,7 +2560,7 @@ __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
     }
     double kG, kG2;
     ho_coeffs(p.mc.Distribution, p.mc.l, kG, kG2, p.mc.coefs);
-    const double* const ipc = p.ipb + el * IPB_NF * NIP;
+    const double* const ipc = p.ipb + el * ipb_rows<DIM>() * NIP;
     double acc[SLOTS];
     for (int k = 0; k < SLOTS; k++) acc[k] = 0.0;
     double lc = 0.0;
@@ -3510,6 +3524,225 @@ __device__ double coss_split_full(const MatConst& mc, double dam, const CossD& D
     return E;
 }
 
+// B-matrix rows of each u-step field (ConstructBMatCoss, CossPhaseField.cpp:1168-1223):
+// (row of the 18-strain, basis k: 0 N, 1..3 G_x..G_z, coefficient: 0 +1, 1 -1, 2 l_coss);
+// 3D fields ux uy uz ox oy oz, 2D fields ux uy oz (:1174-1185)
+struct CossBRows {
+    signed char nrow[6], row[6][5], bas[6][5], cof[6][5];
+};
+template <int DIM>
+constexpr CossBRows coss_brows() {
+    if constexpr (DIM == 3)
+        return {{3, 3, 3, 5, 5, 5},
+                {{0, 4, 8, 0, 0}, {1, 3, 6, 0, 0}, {2, 5, 7, 0, 0}, {5, 6, 9, 12, 16}, {7, 8, 10, 13, 14}, {3, 4, 11, 15, 17}},
+                {{1, 2, 3, 0, 0}, {2, 1, 3, 0, 0}, {3, 2, 1, 0, 0}, {0, 0, 1, 1, 1}, {0, 0, 2, 2, 2}, {0, 0, 3, 3, 3}},
+                {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {1, 0, 2, 2, 2}, {1, 0, 2, 2, 2}, {1, 0, 2, 2, 2}}};
+    else
+        return {{2, 2, 4, 0, 0, 0},
+                {{0, 4, 0, 0, 0}, {1, 3, 0, 0, 0}, {3, 4, 14, 16, 0}, {}, {}, {}},
+                {{1, 2, 0, 0, 0}, {2, 1, 0, 0, 0}, {0, 0, 1, 2, 0}, {}, {}, {}},
+                {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {1, 0, 2, 2, 0}, {}, {}, {}}};
+}
+
+// The u-step coefficient slots.  For field pair (a, b) and bases (k, l), M^{ab}_kl(ip) =
+// sum over the B rows x of a with basis k and y of b with basis l of cof(a, x) cof(b, y)
+// C[row(a, x)][row(b, y)] (C = w Dam D_coss DPlast).  Only the (k, l) with at least one term
+// get a slot: 225 per IP in 3D (of 36 x 16), 49 in 2D.
+struct CossSlots {
+    short moff[37];           // first slot of pair a * nfu + b; moff[nfu * nfu] = slot count
+    signed char smap[36][16];  // slot of (k * 4 + l) within the pair, -1: no term
+};
+template <int DIM>
+constexpr CossSlots coss_slots() {
+    constexpr CossBRows R = coss_brows<DIM>();
+    constexpr int NF = DIM == 3 ? 6 : 3;
+    CossSlots t{};
+    int q = 0;
+    for (int pr = 0; pr < 36; pr++) {
+        t.moff[pr] = (short)q;
+        for (int kl = 0; kl < 16; kl++) {
+            bool used = false;
+            if (pr < NF * NF)
+                for (int x = 0; x < R.nrow[pr / NF]; x++)
+                    for (int y = 0; y < R.nrow[pr % NF]; y++)
+                        used |= R.bas[pr / NF][x] == kl / 4 && R.bas[pr % NF][y] == kl % 4;
+            t.smap[pr][kl] = used ? (signed char)(q++ - t.moff[pr]) : (signed char)-1;
+        }
+    }
+    t.moff[36] = (short)q;
+    return t;
+}
+static_assert(coss_slots<3>().moff[36] == 225 && coss_slots<2>().moff[36] == 49, "u-step slot counts");
+static_assert(COSS_SIG >= 225, "Cosserat per-IP slot block");
+
+// Writes the u-step slots of one IP (cbo: its slot 0, stride NIP): col(j, o) yields column j
+// of C.  The columns are taken grouped by (field b, basis l), so the at most 15 slots of a
+// group accumulate in registers; each column is formed once per B row of b that reads it.
+template <int DIM, int NIP, typename Col>
+__device__ __noinline__ void coss_put_slots(double lc, Col&& col, double* cbo) {
+    constexpr CossBRows R = coss_brows<DIM>();
+    constexpr CossSlots ST = coss_slots<DIM>();
+    constexpr int NF = DIM == 3 ? 6 : 3;
+    auto cf = [&](int c) { return c == 0 ? 1.0 : (c == 1 ? -1.0 : lc); };
+#pragma unroll
+    for (int b = 0; b < NF; b++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            bool any = false;
+#pragma unroll
+            for (int y = 0; y < 5; y++) any |= y < R.nrow[b] && R.bas[b][y] == l;
+            if (!any) continue;
+            double acc[NF][4];
+#pragma unroll
+            for (int a = 0; a < NF; a++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) acc[a][k] = 0.0;
+#pragma unroll
+            for (int y = 0; y < 5; y++) {
+                if (y >= R.nrow[b] || R.bas[b][y] != l) continue;
+                double o[18];
+                col(R.row[b][y], o);
+                const double cy = cf(R.cof[b][y]);
+#pragma unroll
+                for (int a = 0; a < NF; a++)
+#pragma unroll
+                    for (int x = 0; x < 5; x++)
+                        if (x < R.nrow[a]) acc[a][R.bas[a][x]] += cf(R.cof[a][x]) * cy * o[R.row[a][x]];
+            }
+#pragma unroll
+            for (int a = 0; a < NF; a++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int sl = ST.smap[a * NF + b][k * 4 + l];
+                    if (sl >= 0) cbo[(COSS_M + ST.moff[a * NF + b] + sl) * NIP] = acc[a][k];
+                }
+        }
+}
+
+// The u-step slots of one IP when C = s D_coss DPlast (s = w Dam) with DPlast = B + c w zt^T
+// as CossDM holds it (B: 3 x 3 normal block, 6 x 6 shear block, c_B on the diagonal of rows
+// 9..17; identity without plasticity).  G = D_coss B has the same blocks (46 values), so a slot
+// is the sum of its terms cof cof G[r][c] inside the blocks plus the rank-one part c P_ak Q_bl,
+// P_ak = sum_x cof u[row(a, x)] (u = D_coss w), Q_bl = sum_y cof zt[row(b, y)].  CossSlotOps
+// lists, slot after slot, the G terms (x-major, y-minor) and then the slot's end; it is
+// expanded at compile time, so every index into G, P and Q is a constant.
+struct CossSlotOps {
+    short n;
+    unsigned char end[400], gi[400], ca[400], cb[400], p[400], q[400];  // end: 1 at a slot's end
+    unsigned char slot[400];                                             // slot of an end
+};
+template <int DIM>
+constexpr CossSlotOps coss_slot_ops() {
+    constexpr CossBRows R = coss_brows<DIM>();
+    constexpr int NF = DIM == 3 ? 6 : 3;
+    CossSlotOps o{};
+    int n = 0, nslot = 0;
+    for (int a = 0; a < NF; a++)
+        for (int b = 0; b < NF; b++)
+            for (int kl = 0; kl < 16; kl++) {
+                bool any = false;
+                for (int x = 0; x < R.nrow[a]; x++)
+                    for (int y = 0; y < R.nrow[b]; y++) {
+                        if (R.bas[a][x] != kl / 4 || R.bas[b][y] != kl % 4) continue;
+                        any = true;
+                        const int r = R.row[a][x], c = R.row[b][y];
+                        int gi = -1;
+                        if (r < 3 && c < 3) gi = r * 3 + c;
+                        else if (r >= 3 && r < 9 && c >= 3 && c < 9) gi = 9 + (r - 3) * 6 + (c - 3);
+                        else if (r >= 9 && r == c) gi = 45;
+                        if (gi < 0) continue;  // G[r][c] = 0
+                        o.end[n] = 0;
+                        o.gi[n] = (unsigned char)gi;
+                        o.ca[n] = (unsigned char)R.cof[a][x];
+                        o.cb[n] = (unsigned char)R.cof[b][y];
+                        n++;
+                    }
+                if (!any) continue;
+                o.end[n] = 1;
+                o.slot[n] = (unsigned char)nslot++;
+                o.p[n] = (unsigned char)(a * 4 + kl / 4);
+                o.q[n] = (unsigned char)(b * 4 + kl % 4);
+                n++;
+            }
+    o.n = (short)n;
+    return o;
+}
+static_assert(coss_slot_ops<3>().n == 162 + 225 && coss_slot_ops<2>().n <= 400, "u-step slot terms");
+template <int DIM>
+struct CossSlotOpsT {
+    static constexpr CossSlotOps v = coss_slot_ops<DIM>();
+};
+// one op of the list, its indices compile-time constants (so G, P, Q stay in registers)
+template <int DIM, int NIP, int Q, int NF>
+__device__ __forceinline__ void coss_slot_op(const double (&G)[46], const double (&P)[NF * 4], const double (&Qv)[NF * 4],
+                                             double lc, double c, bool r1, double s, double& t, double* cbo) {
+    constexpr const CossSlotOps& O = CossSlotOpsT<DIM>::v;
+    auto cf = [&](int k) { return k == 0 ? 1.0 : (k == 1 ? -1.0 : lc); };
+    if constexpr (!O.end[Q]) {
+        t += cf(O.ca[Q]) * cf(O.cb[Q]) * G[O.gi[Q]];
+    } else {
+        if (r1) t += c * P[O.p[Q]] * Qv[O.q[Q]];
+        cbo[(COSS_M + O.slot[Q]) * NIP] = s * t;
+        t = 0.0;
+    }
+}
+template <int DIM, int NIP, int NF, int... Q>
+__device__ __forceinline__ void coss_slot_ops_run(std::integer_sequence<int, Q...>, const double (&G)[46],
+                                                  const double (&P)[NF * 4], const double (&Qv)[NF * 4], double lc,
+                                                  double c, bool r1, double s, double* cbo) {
+    double t = 0.0;
+    (coss_slot_op<DIM, NIP, Q, NF>(G, P, Qv, lc, c, r1, s, t, cbo), ...);
+}
+
+template <int DIM, int NIP>
+__device__ __forceinline__ void coss_put_slots_dm(const CossD& D, const CossDM& DM, double s, double lc, double* cbo) {
+    constexpr CossBRows R = coss_brows<DIM>();
+    constexpr int NF = DIM == 3 ? 6 : 3;
+    auto cf = [&](int c) { return c == 0 ? 1.0 : (c == 1 ? -1.0 : lc); };
+    double G[46];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if (DM.id) {
+                G[i * 3 + j] = i == j ? D.dl : D.dlo;
+            } else {
+                double t = 0.0;
+#pragma unroll
+                for (int m = 0; m < 3; m++) t += (i == m ? D.dl : D.dlo) * DM.Bi.n[m][j];
+                G[i * 3 + j] = t;
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++)
+            G[9 + i * 6 + j] = DM.id ? (i == j ? D.sp : (j == (i ^ 1) ? D.sm : 0.0))
+                                     : D.sp * DM.Bi.s[i][j] + D.sm * DM.Bi.s[i ^ 1][j];
+    G[45] = DM.id ? D.s2 : D.s2 * DM.Bi.c;
+    const bool r1 = !DM.id;
+    double P[NF * 4], Q[NF * 4];
+    {
+        double u[18];
+        coss_Dmul(D, DM.w, 1.0, u);
+#pragma unroll
+        for (int k = 0; k < NF * 4; k++) {
+            P[k] = 0.0;
+            Q[k] = 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < NF; a++)
+#pragma unroll
+            for (int x = 0; x < 5; x++)
+                if (x < R.nrow[a]) {
+                    P[a * 4 + R.bas[a][x]] += cf(R.cof[a][x]) * u[R.row[a][x]];
+                    Q[a * 4 + R.bas[a][x]] += cf(R.cof[a][x]) * DM.zt[R.row[a][x]];
+                }
+    }
+    coss_slot_ops_run<DIM, NIP, NF>(std::make_integer_sequence<int, CossSlotOpsT<DIM>::v.n>{}, G, P, Q, lc, DM.c, r1, s,
+                                    cbo);
+}
+
 template <int DIM>
 struct CossSmem {
     // u-step fields: ux uy uz ox oy oz (3D), ux uy oz (2D)
@@ -3565,7 +3798,7 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
     unsigned my_calls = 0, my_iters = 0, my_maxit = 0, my_bad = 0;
     if (le < EPB && el < p.NE) {
         const int64_t gi = el * NIP + ip, S = p.NIPT;
-        double* const ipo = p.ipb + el * IPB_NF * NIP + ip;
+        double* const ipo = p.ipb + el * ipb_rows<DIM>() * NIP + ip;
         double* const cbo = p.cossb + el * COSS_NF * NIP + ip;
         const int q[3] = {ip % 3, (ip / 3) % 3, ip / 9};
         double b1[DIM][3], g1[DIM][3];
@@ -3746,14 +3979,14 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
                     my_bad |= !isfinite(v);
                     cbo[(COSS_SIG + r) * NIP] = v;
                 }
+                coss_put_slots<DIM, NIP>(lc, [&](int j, double o[18]) {
 #pragma unroll 1
-                for (int i = 0; i < 18; i++)
-                    for (int j = 0; j < 18; j++) {
+                    for (int i = 0; i < 18; i++) {
                         double t = 0;
-                        for (int k = 0; k < 18; k++)
-                            t += (Dam * DDm[i][k] + DUm[i][k]) * DM.at(k, j);
-                        cbo[(COSS_C + i * 18 + j) * NIP] = w * t;
+                        for (int k = 0; k < 18; k++) t += (Dam * DDm[i][k] + DUm[i][k]) * DM.at(k, j);
+                        o[i] = w * t;
                     }
+                }, cbo);
             } else {
                 coss_Dmul(D, ee, 1.0, sig);  // energy not consistent: S_Dam = D_coss ee, S_Undam = 0
                 for (int r = 0; r < 18; r++) {
@@ -3762,15 +3995,7 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
                     cbo[(COSS_SIG + r) * NIP] = v;
                 }
                 // w (Dam D_coss) DPlast
-#pragma unroll
-                for (int j = 0; j < 18; j++) {
-                    double col[18], o[18];
-#pragma unroll
-                    for (int k = 0; k < 18; k++) col[k] = DM.at(k, j);
-                    coss_Dmul(D, col, Dam, o);
-#pragma unroll
-                    for (int i = 0; i < 18; i++) cbo[(COSS_C + i * 18 + j) * NIP] = w * o[i];
-                }
+                coss_put_slots_dm<DIM, NIP>(D, DM, w * Dam, lc, cbo);
             }
             cbo[COSS_WI * NIP] = w * DamI;
             cbo[COSS_WG * NIP] = w * DamG;  // (1 - porosity) = 1
@@ -3877,27 +4102,19 @@ __global__ __launch_bounds__(NT, 1) void coss_material_kernel(KParams p) {
     }
 }
 
-// B-matrix rows of each u-step field (ConstructBMatCoss, CossPhaseField.cpp:1184-1221):
-// (row, basis k: 0 N, 1..3 G_x..G_z, coefficient: 0 +1, 1 -1, 2 l_coss)
-__constant__ const signed char coss_nrow[6] = {3, 3, 3, 5, 5, 5};
-__constant__ const signed char coss_row[6][5] = {
-    {0, 4, 8, 0, 0}, {1, 3, 6, 0, 0}, {2, 5, 7, 0, 0}, {5, 6, 9, 12, 16}, {7, 8, 10, 13, 14}, {3, 4, 11, 15, 17}};
-__constant__ const signed char coss_bas[6][5] = {
-    {1, 2, 3, 0, 0}, {2, 1, 3, 0, 0}, {3, 2, 1, 0, 0}, {0, 0, 1, 1, 1}, {0, 0, 2, 2, 2}, {0, 0, 3, 3, 3}};
-__constant__ const signed char coss_cof[6][5] = {
-    {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {1, 0, 2, 2, 2}, {1, 0, 2, 2, 2}, {1, 0, 2, 2, 2}};
-// 2D fields ux uy oz (:1174-1185)
-__constant__ const signed char coss2_nrow[3] = {2, 2, 4};
-__constant__ const signed char coss2_row[3][4] = {{0, 4, 0, 0}, {1, 3, 0, 0}, {3, 4, 14, 16}};
-__constant__ const signed char coss2_bas[3][4] = {{1, 2, 0, 0}, {2, 1, 0, 0}, {0, 0, 1, 2}};
-__constant__ const signed char coss2_cof[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {1, 0, 2, 2}};
+// B-matrix row tables, device copies for runtime field indices
+__constant__ const CossBRows coss_brows3 = coss_brows<3>();
+__constant__ const CossBRows coss_brows2 = coss_brows<2>();
+__constant__ const CossSlots coss_slots3 = coss_slots<3>();
+__constant__ const CossSlots coss_slots2 = coss_slots<2>();
 
 template <int DIM>
 struct CossRows {
-    static __device__ __forceinline__ int nrow(int f) { return DIM == 3 ? coss_nrow[f] : coss2_nrow[f]; }
-    static __device__ __forceinline__ int row(int f, int x) { return DIM == 3 ? coss_row[f][x] : coss2_row[f][x]; }
-    static __device__ __forceinline__ int bas(int f, int x) { return DIM == 3 ? coss_bas[f][x] : coss2_bas[f][x]; }
-    static __device__ __forceinline__ int cof(int f, int x) { return DIM == 3 ? coss_cof[f][x] : coss2_cof[f][x]; }
+    static __device__ __forceinline__ const CossBRows& t() { return DIM == 3 ? coss_brows3 : coss_brows2; }
+    static __device__ __forceinline__ int nrow(int f) { return t().nrow[f]; }
+    static __device__ __forceinline__ int row(int f, int x) { return t().row[f][x]; }
+    static __device__ __forceinline__ int bas(int f, int x) { return t().bas[f][x]; }
+    static __device__ __forceinline__ int cof(int f, int x) { return t().cof[f][x]; }
 };
 
 // Cosserat u-step element kernel on MFMA.  K^{ab}[i][j] = sum_ip sum_kl Phi_k(ip,i) M^{ab}_kl(ip)
@@ -3928,10 +4145,13 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     __shared__ uint8_t lc_[N][N];                 // loc: slot of node j in node i's row
     __shared__ int32_t rk[N];
     __shared__ int64_t rp0[N], rdeg[N];
+    __shared__ signed char sl_map[NFU * NFU][16];
+    __shared__ short sl_off[NFU * NFU];
     const int tid = threadIdx.x;
-    const int64_t el = p.elist[blockIdx.x];
-    const double* const ipc = p.ipb + el * IPB_NF * NIP;
+    const int64_t el = p.elist ? (int64_t)p.elist[blockIdx.x] : p.crec_e0 + blockIdx.x;
+    const double* const ipc = p.ipb + el * ipb_rows<DIM>() * NIP;
     const double* const cb = p.cossb + el * COSS_NF * NIP;
+    T* const crec = p.crec ? (T*)p.crec + (el - p.crec_e0) * N * CRec<DIM>::SIZE : nullptr;  // this element's records
     const MatConst& mc = p.mc;
     auto phd = [&](int ip, int k, int a) -> double { return F64 ? (double)Phi[ip][k][a] : Phd[F64 ? 0 : ip][k][a]; };
     // ---- stage the element's data: nodes, 1D factors, per-IP geometry, stresses, weights, slots
@@ -3962,6 +4182,11 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
         wg_ip[tid] = cb[COSS_WG * NIP + tid];
     }
     for (int t = tid; t < N * N; t += NT) lc_[t / N][t % N] = p.loc[el * N * N + t];
+    {
+        const CossSlots& ST = DIM == 3 ? coss_slots3 : coss_slots2;
+        for (int t = tid; t < NFU * NFU * 16; t += NT) sl_map[t / 16][t % 16] = ST.smap[t / 16][t % 16];
+        if (tid < NFU * NFU) sl_off[tid] = ST.moff[tid];
+    }
     __syncthreads();
     for (int i = tid; i < NIP * NP; i += NT) {
         const int ip = i / NP, a = i % NP;
@@ -3996,50 +4221,34 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     __syncthreads();
     const double lc = mc.l_coss, l2 = lc * lc;
     auto cof = [&](int c) { return c == 0 ? 1.0 : (c == 1 ? -1.0 : lc); };
-    const int wv = tid >> 6, lane = tid & 63, lk = lane >> 4, l16 = lane & 15;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, lk = lane >> 4, l16 = lane & 15;
     double bad = 0.0;
+    // the slots of pair pr for lanes t = lane + 64 u: IP t % NIP, (k, l) = t / NIP
+    constexpr int MU = (NIP * 16 + 63) / 64;
+    double mpre[MU];
+    auto fetch = [&](int pr) {
+        const double* const cm = cb + (COSS_M + sl_off[pr]) * NIP;
+#pragma unroll
+        for (int u = 0; u < MU; u++) {
+            const int t = lane + 64 * u, kl = t / NIP;
+            const int sl = t < NIP * 16 ? sl_map[pr][kl] : -1;
+            mpre[u] = sl >= 0 ? cm[sl * NIP + t % NIP] : 0.0;
+        }
+    };
+    fetch(wv);
 #pragma unroll 1
     for (int blk = wv; blk < NFU * NFU; blk += NW) {
         const int al = blk / NFU, be = blk % NFU;
-        // M^{ab}_kl(ip) = sum over the B rows of the pair (basis k of field a, l of field b):
-        // one lane per IP; the at most 5 x 5 coefficients C[r][c] are loaded together (every
-        // row and column of the pair), then summed into their (k, l) slots
-        if (lane < NIP) {
-            const int ip = lane, na = CR::nrow(al), nb = CR::nrow(be);
-            double v[5][5];
+        // this pair's 4 x 4 blocks M^{ab}_kl(ip), fetched during the previous pair's products
 #pragma unroll
-            for (int x = 0; x < 5; x++)
-#pragma unroll
-                for (int y = 0; y < 5; y++)
-                    v[x][y] = (x < na && y < nb) ? cb[(COSS_C + CR::row(al, x < na ? x : 0) * 18 + CR::row(be, y < nb ? y : 0)) * NIP + ip]
-                                                 : 0.0;
-            double m[4][4];
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-#pragma unroll
-                for (int l = 0; l < 4; l++) m[k][l] = 0.0;
-#pragma unroll
-            for (int x = 0; x < 5; x++) {
-                const int bk = x < na ? CR::bas(al, x) : -1;
-                const double ca = x < na ? cof(CR::cof(al, x)) : 0.0;
-#pragma unroll
-                for (int y = 0; y < 5; y++) {
-                    const int bl = y < nb ? CR::bas(be, y) : -1;
-                    const double t = ca * (y < nb ? cof(CR::cof(be, y)) : 0.0) * v[x][y];
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-#pragma unroll
-                        for (int l = 0; l < 4; l++) m[k][l] += (bk == k && bl == l) ? t : 0.0;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-#pragma unroll
-                for (int l = 0; l < 4; l++) Mb[wv][ip][k * 4 + l] = m[k][l];
+        for (int u = 0; u < MU; u++) {
+            const int t = lane + 64 * u;
+            if (t < NIP * 16) Mb[wv][t % NIP][t / NIP] = mpre[u];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (blk + NW < NFU * NFU) fetch(blk + NW);
         V acc[MT][MT];
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
@@ -4082,8 +4291,12 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
                     double val = (double)acc[mt][nt][v];
                     if (i == j) val += WI[i] * kdiag;
                     bad += val - val;
-                    double* K = Kab + (int64_t)p.su.nf * rp0[i] + (int64_t)p.fpos_u[be] * rdeg[i] + lc_[i][j];
-                    *K += val;
+                    if (crec) {  // node i's record: [al][be][j], 16 consecutive j per lane group
+                        crec[i * CRec<DIM>::SIZE + (al * NFU + be) * N + j] = (T)val;
+                    } else {
+                        double* K = Kab + (int64_t)p.su.nf * rp0[i] + (int64_t)p.fpos_u[be] * rdeg[i] + lc_[i][j];
+                        *K += val;
+                    }
                 }
         __builtin_amdgcn_wave_barrier();  // Mb of this wavefront is rewritten by its next pair
     }
@@ -4102,9 +4315,100 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
         if (mc.Damping > 0.0) accf += WI[a] * (rot ? 2 * l2 * mc.Damping : mc.Damping) * p.su.dx[dof];
         if (al < DIM) accf += -WG[a] * mc.Density * mc.g[al];
         bad += accf - accf;
-        p.su.f[dof] += accf;
+        if (crec) crec[a * CRec<DIM>::SIZE + CRec<DIM>::F + al] = (T)accf;
+        else p.su.f[dof] += accf;
     }
     if (bad != 0.0) atomicOr(&p.st->nonfinite, 1u);
+}
+
+// Cosserat u-step gather: one wavefront per node (group rank), as gather_body for the solid
+// model, over the node's elements in [e0, e1) only: their records are added in increasing
+// element index into LDS copies of the node's NFU rows (NFU x NFU column blocks of deg
+// slots) and residuals, which start from the rows' current values when the node has
+// elements before e0 (the partial sums of the earlier chunks, so every entry is the sum of
+// its element contributions in element order) and from zero otherwise; each row is written
+// once, contiguously.
+template <int DIM, typename T>
+__global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, int64_t e1) {
+    if (p.gate && !*p.gate) return;
+    using CR = CRec<DIM>;
+    typedef typename RecLoad<T>::V V;
+    constexpr int N = CR::N, NFU = CR::NFU, NPAIR = NFU * NFU, W = RecLoad<T>::W;
+    constexpr int PER = (CR::SIZE + GW * W - 1) / (GW * W), MAXDEG = gather_maxdeg<DIM>(), GPF = 4;
+    constexpr int OF = NPAIR * MAXDEG, LEN = OF + NFU;
+    __shared__ double acc[LEN];
+    const int lane = threadIdx.x;
+    const int64_t r = blockIdx.x;
+    const int64_t k0 = p.n2e_ptr[r], k1 = p.n2e_ptr[r + 1];
+    if (k1 == k0) return;
+    const int64_t efirst = p.n2e_elem[k0], elast = p.n2e_elem[k1 - 1];  // ascending
+    if (elast < e0 || efirst >= e1) return;                            // no element in this chunk
+    const int64_t r0 = p.nbr_ptr[r];
+    const int deg = (int)(p.nbr_ptr[r + 1] - r0);
+    const bool cont = efirst < e0;
+    auto Krow = [&](int al, int be) {
+        return p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * r0 + (int64_t)p.fpos_u[be] * deg;
+    };
+    for (int q = 0; q < NPAIR; q++) {
+        const double* K = Krow(q / NFU, q % NFU);
+        for (int i = lane; i < MAXDEG; i += GW) acc[q * MAXDEG + i] = (cont && i < deg) ? K[i] : 0.0;
+    }
+    if (lane < NFU) acc[OF + lane] = cont ? p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] : 0.0;
+    // per lane: record entries t = (lane + GW u) W + w -> LDS base slot and column index (-1:
+    // a residual)
+    int base[PER][W], lb[PER][W];
+    bool act[PER][W];
+#pragma unroll
+    for (int u = 0; u < PER; u++)
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const int t = (lane + u * GW) * W + w;
+            act[u][w] = t < CR::SIZE;
+            base[u][w] = t < CR::F ? (t / N) * MAXDEG : OF + (t - CR::F);
+            lb[u][w] = t < CR::F ? t % N : -1;
+        }
+    __syncthreads();
+    for (int64_t c0 = k0; c0 < k1; c0 += GW) {
+        const int nc = (int)(k1 - c0 < GW ? k1 - c0 : GW);
+        const int64_t my_e = lane < nc ? p.n2e_elem[c0 + lane] : 0;
+        const int my_a = lane < nc ? (int)p.n2e_loc[c0 + lane] : 0;
+        for (int kb = 0; kb < nc; kb += GPF) {
+            V x[GPF][PER];
+            int lcv[GPF];
+            bool use[GPF];
+#pragma unroll
+            for (int q = 0; q < GPF; q++) {
+                const int k = kb + q < nc ? kb + q : nc - 1;
+                const int64_t e = __builtin_amdgcn_readlane((int)my_e, k);
+                const int a = __builtin_amdgcn_readlane(my_a, k);
+                use[q] = kb + q < nc && e >= e0 && e < e1;
+                const int64_t ec = use[q] ? e : e0;  // a record inside the chunk's buffer
+                const V* rec = (const V*)((const T*)p.crec + ((ec - p.crec_e0) * N + a) * CR::SIZE);
+                lcv[q] = lane < N ? (int)p.loc[(e * N + a) * N + lane] : 0;
+#pragma unroll
+                for (int u = 0; u < PER; u++)
+                    x[q][u] = use[q] && lane + u * GW < CR::SIZE / W ? rec[lane + u * GW] : RecLoad<T>::zero();
+            }
+#pragma unroll
+            for (int q = 0; q < GPF; q++) {
+                if (use[q]) {
+#pragma unroll
+                    for (int u = 0; u < PER; u++)
+#pragma unroll
+                        for (int w = 0; w < W; w++) {
+                            const int sl = __shfl(lcv[q], lb[u][w] >= 0 ? lb[u][w] : 0);
+                            if (act[u][w]) atomicAdd(&acc[base[u][w] + (lb[u][w] >= 0 ? sl : 0)], RecLoad<T>::get(x[q][u], w));
+                        }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = 0; q < NPAIR; q++) {
+        double* K = Krow(q / NFU, q % NFU);
+        for (int i = lane; i < deg; i += GW) K[i] = acc[q * MAXDEG + i];
+    }
+    if (lane < NFU) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = acc[OF + lane];
 }
 
 }  // namespace
@@ -4168,6 +4472,21 @@ hipError_t launch_coss_material(const KParams& p, int dim, hipStream_t s) {
     } else {
         if (dim == 2) hipLaunchKernelGGL((coss_material_kernel<2, false>), g, b, 0, s, p);
         else hipLaunchKernelGGL((coss_material_kernel<3, false>), g, b, 0, s, p);
+    }
+    return hipGetLastError();
+}
+
+int coss_record_size(int dim) { return dim == 2 ? CRec<2>::SIZE : CRec<3>::SIZE; }
+
+hipError_t launch_coss_gather(const KParams& p, int dim, int64_t e0, int64_t e1, hipStream_t s) {
+    if (p.Ng <= 0 || e1 <= e0) return hipSuccess;
+    const dim3 g((unsigned)p.Ng), b(GW);
+    if (p.mixed) {
+        if (dim == 2) hipLaunchKernelGGL((coss_gather_kernel<2, float>), g, b, 0, s, p, e0, e1);
+        else hipLaunchKernelGGL((coss_gather_kernel<3, float>), g, b, 0, s, p, e0, e1);
+    } else {
+        if (dim == 2) hipLaunchKernelGGL((coss_gather_kernel<2, double>), g, b, 0, s, p, e0, e1);
+        else hipLaunchKernelGGL((coss_gather_kernel<3, double>), g, b, 0, s, p, e0, e1);
     }
     return hipGetLastError();
 }

@@ -500,7 +500,10 @@ int ife_model_set_element_params(ife_physics* ph, int32_t model, const double* G
  * the visco-elasto-plastic map) / 250, SolidMaterial.cpp:200-269), "exact_tangent" (1: the
  * consistent tangent of the Glen map, including the second derivative of the yield
  * function that the reference drops, SolidMaterial.cpp:289).  A negative tolerance or
- * non-positive iteration count restores the reference's value. */
+ * non-positive iteration count restores the reference's value. Fracture/CossPhaseField:
+ * "coss_record_chunk" the elements of one chunk of u-step element records (0, the default: as
+ * many as fit in 64 GiB; negative: no records, the element blocks are added into the CSR
+ * colour by colour). */
 int ife_model_set_param(ife_physics* ph, int32_t model, const char* name, double value);
 
 /* ---- batches of independent domains (no counterpart in the reference, which runs one
