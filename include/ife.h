@@ -228,9 +228,10 @@ int ife_physics_create(const ife_mesh_desc* mesh, const ife_dofs_desc* dofs,
                        ife_physics** out);
 
 /* CreateModel(name) + BaseModel::init (ModelRegister.cpp:57-76, BaseModel.h:62,
- * PhaseField.cpp:33-101, ThermalModel.cpp:31-36).  type_name is the registered name:
- * "Fracture/PhaseField" or "Thermal/ThermalModel".  u/phase/T fields are looked up by
- * the reference's DOF names (ux,uy[,uz], phase, T).  Models run in the order added,
+ * PhaseField.cpp:33-101, CossPhaseField.cpp:34-110,171-315, ThermalModel.cpp:31-36).  type_name is
+ * the registered name: "Fracture/PhaseField", "Fracture/CossPhaseField" or
+ * "Thermal/ThermalModel".  u/phase/T fields are looked up by the reference's DOF names
+ * (ux,uy[,uz], Cosserat ox,oy,oz (2D: oz), phase, T).  Models run in the order added,
  * as Physics::Assemble loops Models[i] (physics.cpp:269-284).  mat/pf/opts may be NULL
  * for the thermal model except mat (ThermalProperties, Density). */
 int ife_physics_add_model(ife_physics* ph, const char* type_name, const char* name,
