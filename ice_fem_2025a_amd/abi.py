@@ -101,7 +101,8 @@ class BatchTuning(C.Structure):
     _fields_ = [("grid_material", C.c_int64), ("grid_assemble", C.c_int64), ("grid_gather", C.c_int64),
                 ("lanes_assemble", C.c_int32), ("candidates_timed", C.c_int32), ("n_domains", C.c_int32),
                 ("blocks_material", C.c_int64), ("elements", C.c_int64), ("nodes", C.c_int64),
-                ("ms_material", C.c_double), ("ms_assemble", C.c_double), ("ms_gather", C.c_double)]
+                ("ms_material", C.c_double), ("ms_assemble", C.c_double), ("ms_gather", C.c_double),
+                ("elems_per_wave", C.c_int32)]
 
 
 _PROTOS = [

@@ -194,6 +194,9 @@ struct ife_physics {
     // "coss_record_chunk" parameter (0: as many as fit, negative: the coloured additions)
     DevBuf<double> d_crec;
     int64_t crec_cap = 0, crec_chunk = 0;
+    // 2D element kernel in mixed mode ("elements_per_wave"): 1 one wavefront per element on the
+    // VALU, 4 / 8 / 16 that many elements per wavefront on MFMA
+    int epw2d = 1;
     DevBuf<int64_t> d_n2e_ptr;
     DevBuf<int32_t> d_n2e_elem;
     DevBuf<uint8_t> d_n2e_loc;
@@ -681,6 +684,7 @@ KParams base_params(ife_physics* ph) {
     p.n2e_elem = ph->d_n2e_elem.p;
     p.n2e_loc = ph->d_n2e_loc.p;
     p.mixed = ph->precision == IFE_PREC_MIXED;
+    p.epw2d = ph->epw2d;
     return p;
 }
 
@@ -861,6 +865,11 @@ struct ife_batch {
     int64_t tot[3] = {0, 0, 0};
     int64_t grid[3] = {0, 0, 0};  // workgroups per launch (0: one per block)
     int nta = 256;  // lanes per element of the element kernel (64 in 2D: one wavefront)
+    // 2D element kernel: elements per wavefront (1: assemble2d_wave; 4, 8, 16: assemble2d_mfma,
+    // mixed mode), and the block tables of the MFMA forms (ceil(elements / epw) per domain)
+    int epw = 1;
+    DevBuf<int64_t> bpw[3];
+    int64_t totw[3] = {0, 0, 0};
     hipStream_t cap = nullptr;    // capture stream (default priority)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -880,7 +889,14 @@ struct ife_batch {
         a.nd = (int)doms.size();
         a.dim = dim; a.mixed = mixed; a.vep = vep; a.exact = exact; a.full = full;
         for (int k = 0; k < 3; k++) { a.bptr[k] = bp[k].p; a.tot[k] = tot[k]; }
+        set_epw(a, epw);
         return a;
+    }
+    void set_epw(BatchArgs& a, int e) const {
+        a.epw = e;
+        const int k = e == 4 ? 0 : e == 8 ? 1 : e == 16 ? 2 : -1;
+        a.bptr_w = k >= 0 ? bpw[k].p : nullptr;
+        a.tot_w = k >= 0 ? totw[k] : 0;
     }
 };
 
@@ -970,12 +986,24 @@ void batch_tune(ife_batch* b, hipStream_t s, ife_batch_tuning* out) {
             if (ms < best[0]) { best[0] = ms; b->grid[0] = g; }
         }
         // element kernel: 3D one 256-lane workgroup per element, 2D one wavefront per element
-        // (four per workgroup); the candidates are grid shapes in workgroups
+        // (four per workgroup); the candidates are grid shapes in workgroups.  2D mixed mode
+        // also times the MFMA kernel with 4, 8 and 16 elements per wavefront (one launch
+        // wavefront per group of elements).
         const int nt = b->dim == 2 ? 64 : 256;
         const int64_t wgs = b->dim == 2 ? (b->tot[1] + 3) / 4 : b->tot[1];
+        BatchArgs aw = a;
+        b->set_epw(aw, 1);
         for (int64_t g : candidates(wgs)) {
-            double ms = time_it([&] { return launch_batch_assemble(a, g, nt, s); });
-            if (ms < best[1]) { best[1] = ms; b->grid[1] = g; b->nta = nt; }
+            double ms = time_it([&] { return launch_batch_assemble(aw, g, nt, s); });
+            if (ms < best[1]) { best[1] = ms; b->grid[1] = g; b->nta = nt; b->epw = 1; }
+        }
+        if (b->dim == 2 && b->mixed) {
+            for (int e : {4, 8, 16}) {
+                BatchArgs am = a;
+                b->set_epw(am, e);
+                double ms = time_it([&] { return launch_batch_assemble(am, 0, nt, s); });
+                if (ms < best[1]) { best[1] = ms; b->grid[1] = 0; b->nta = nt; b->epw = e; }
+            }
         }
         for (int64_t g : candidates(b->tot[2])) {
             double ms = time_it([&] { return launch_batch_gather(a, g, s); });
@@ -994,6 +1022,7 @@ void batch_tune(ife_batch* b, hipStream_t s, ife_batch_tuning* out) {
         out->grid_assemble = b->grid[1];
         out->grid_gather = b->grid[2];
         out->lanes_assemble = b->nta;
+        out->elems_per_wave = b->epw;
         out->candidates_timed = timed;
         out->ms_material = best[0];
         out->ms_assemble = best[1];
@@ -1765,8 +1794,16 @@ int ife_model_set_param(ife_physics* ph, int32_t model, const char* name, double
         if (!ph || !name) THROW(IFE_ERR_INVALID, "null argument");
         if (model < 0 || model >= (int)ph->models.size()) THROW(IFE_ERR_INVALID, "bad model index");
         Model* m = ph->models[model];
-        if (m->type != MT_PHASEFIELD) THROW(IFE_ERR_INVALID, "parameters belong to a Fracture/PhaseField model");
         std::string nm(name);
+        if (nm == "elements_per_wave") {
+            if (!(value == 1.0 || value == 4.0 || value == 8.0 || value == 16.0))
+                THROW(IFE_ERR_INVALID, "elements_per_wave must be 1, 4, 8 or 16");
+            const int v = (int)value;
+            if (v > 1 && ph->dim != 2) THROW(IFE_ERR_INVALID, "elements_per_wave above 1 is a 2D element kernel");
+            ph->epw2d = v;
+            return;
+        }
+        if (m->type != MT_PHASEFIELD) THROW(IFE_ERR_INVALID, "parameters belong to a Fracture/PhaseField model");
         if (nm == "n") m->n_override = value;
         else if (nm == "rm_abs_tol") m->rm_eabs = value;
         else if (nm == "rm_rel_tol") m->rm_erel = value;
@@ -1810,6 +1847,15 @@ int ife_batch_create(ife_physics* const* domains, int32_t n_domains, ife_batch**
         for (int k = 0; k < 3; k++) {
             b->bp[k].upload(pre[k].data(), pre[k].size());
             b->tot[k] = pre[k].back();
+        }
+        if (b->dim == 2) {  // MFMA element kernel: ceil(elements / epw) wavefronts per domain
+            const int epws[3] = {4, 8, 16};
+            for (int k = 0; k < 3; k++) {
+                std::vector<int64_t> pw{0};
+                for (auto* ph : b->doms) pw.push_back(pw.back() + (ph->NE + epws[k] - 1) / epws[k]);
+                b->bpw[k].upload(pw.data(), pw.size());
+                b->totw[k] = pw.back();
+            }
         }
         batch_update(b.get());
         *out = b.release();
@@ -1861,10 +1907,18 @@ int ife_batch_set_shape(ife_batch* b, const ife_batch_tuning* shape) {
         const int lanes = b->dim == 2 ? 64 : 256;
         if (shape->lanes_assemble != 0 && shape->lanes_assemble != lanes)
             THROW(IFE_ERR_INVALID, "batch: element kernel lanes per element must be 256 (3D) or 64 (2D, one wavefront), or 0");
+        const int e = shape->elems_per_wave;
+        if (!(e == 0 || e == 1 || e == 4 || e == 8 || e == 16))
+            THROW(IFE_ERR_INVALID, "batch: elements per wavefront must be 0, 1, 4, 8 or 16");
+        if (e > 1 && !(b->dim == 2 && b->mixed))
+            THROW(IFE_ERR_INVALID, "batch: 4, 8 or 16 elements per wavefront (MFMA) is the 2D mixed-precision kernel");
+        if (e > 1 && shape->grid_assemble != 0)
+            THROW(IFE_ERR_INVALID, "batch: the MFMA element kernel launches one wavefront per element group (grid 0)");
         b->grid[0] = shape->grid_material;
         b->grid[1] = shape->grid_assemble;
         b->grid[2] = shape->grid_gather;
         b->nta = lanes;
+        b->epw = e > 1 ? e : 1;
         b->drop_graph();
     });
 }
@@ -1877,6 +1931,7 @@ int ife_batch_get_shape(const ife_batch* b, ife_batch_tuning* out) {
         out->grid_assemble = b->grid[1];
         out->grid_gather = b->grid[2];
         out->lanes_assemble = b->nta;
+        out->elems_per_wave = b->epw;
         out->n_domains = (int32_t)b->doms.size();
         out->blocks_material = b->tot[0];
         out->elements = b->tot[1];

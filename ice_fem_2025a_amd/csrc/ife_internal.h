@@ -150,6 +150,9 @@ struct KParams {
     // the element blocks are added into the CSR values colour by colour
     void* crec;
     int64_t crec_e0;
+    // 2D element kernel in mixed mode: 1 one wavefront per element on the VALU
+    // (assemble2d_wave); 4, 8 or 16 that many elements per wavefront on MFMA (assemble2d_mfma)
+    int epw2d;
 };
 enum { COSS_M = 0, COSS_SIG = 225, COSS_WI = 243, COSS_WG = 244, COSS_NF = 245 };
 
@@ -162,6 +165,11 @@ struct BatchArgs {
     int nd, dim, mixed, vep, exact, full;
     const int64_t* bptr[3];
     int64_t tot[3];
+    // 2D element kernel: elements per wavefront (1: assemble2d_wave; 4, 8, 16: assemble2d_mfma,
+    // mixed mode) and, for the MFMA forms, the prefix sums of ceil(elements / epw) per domain
+    int epw;
+    const int64_t* bptr_w;
+    int64_t tot_w;
 };
 
 // Launchers (ife_kernels.hip)

@@ -1887,64 +1887,100 @@ __device__ __forceinline__ void kpt_store(const Smem<DIM, T>& sm, R* rec, const 
         }
 }
 
-// Nodal (lumped) terms of node a: lumped weight WL = sum_ip w N_a, inertia, damping and
-// gravity (PhaseField.cpp:436-477), lumped driving / capacity, phase viscosity and
-// stabilisation (PhaseField.cpp:693-711), heat capacity (ThermalModel.cpp:128-129).
-template <int DIM, typename SM>
-__device__ __forceinline__ double nodal_terms(SM& sm, const KParams& p, const int a, const bool du, const bool dph,
-                                              const bool dth) {
-    constexpr int NIP = SM::NIP;
+// Nodal (lumped) terms of one node from its lumped weight s = sum_ip w N_a, lumped driving
+// weight ld = sum_ip LDRIVE N_a, the element's Gc at its last IP and the nodal values:
+// inertia, damping and gravity (PhaseField.cpp:436-477), lumped driving / capacity, phase
+// viscosity and stabilisation (PhaseField.cpp:693-711), heat capacity (ThermalModel.cpp:128-129).
+struct NodalIn {
+    double s, ld, GcL, ph, pho, dT;
+    double dU[3], ddU[3];
+};
+struct NodalOut {
+    double kdu, kdp, kdt, fnp, fnt;
+    double fnu[3];
+};
+template <int DIM>
+__device__ __forceinline__ void nodal_core(const KParams& p, const NodalIn& in, const bool du, const bool dph,
+                                           const bool dth, NodalOut& o) {
     const MatConst& mc = p.mc;
-    double s = 0;
-    for (int ip = 0; ip < NIP; ip++) s += sm.ipc[SM::FW][ip] * (double)sm.Nsh[ip][a];
+    const double s = in.s;
     if (du) {
         double DamI, DamG, d1, d2;
-        damage(mc.DensityDamage, sm.Ph[a], mc.k0, mc.p, DamI, d1, d2);
-        damage(mc.GravityDamage, sm.Ph[a], mc.k0, mc.p, DamG, d1, d2);
+        damage(mc.DensityDamage, in.ph, mc.k0, mc.p, DamI, d1, d2);
+        damage(mc.GravityDamage, in.ph, mc.k0, mc.p, DamG, d1, d2);
         double kd = 0;
         if (mc.Inertia) kd += s * DamI * mc.Density * p.ddu_dt;
         if (mc.Damping > 0.0) kd += s * mc.Damping * p.du_dt;
-        sm.kdu[a] = kd;
+        o.kdu = kd;
         for (int al = 0; al < DIM; al++) {
             double f = 0;
-            if (mc.Inertia) f += s * DamI * mc.Density * sm.ddU[al][a];
-            if (mc.Damping > 0.0) f += s * mc.Damping * sm.dU[al][a];
+            if (mc.Inertia) f += s * DamI * mc.Density * in.ddU[al];
+            if (mc.Damping > 0.0) f += s * mc.Damping * in.dU[al];
             f += -s * DamG * mc.Density * mc.g[al];
-            sm.fnu[al][a] = f;
+            o.fnu[al] = f;
         }
     }
     if (dph) {
-        const double GcL = sm.ipc[SM::FGC][NIP - 1];
+        const double GcL = in.GcL;
         double kd = 0, f = 0;
         if (mc.LumpedDrive) {
-            double ld = 0;
-            for (int ip = 0; ip < NIP; ip++) ld += sm.ipc[SM::FLD][ip] * (double)sm.Nsh[ip][a];
             double D0, dD0, ddD0;
-            damage(mc.DamFunction, sm.Ph[a], mc.k0, mc.p, D0, dD0, ddD0);
-            kd += ld * ddD0;
-            f += ld * dD0;
+            damage(mc.DamFunction, in.ph, mc.k0, mc.p, D0, dD0, ddD0);
+            kd += in.ld * ddD0;
+            f += in.ld * dD0;
         }
         if (mc.LumpedCap) {
             double u1, u2, LD, dLD;
-            cap_coeffs(mc.Distribution, mc.l, sm.Ph[a], u1, u2, LD, dLD, mc.coefs);
+            cap_coeffs(mc.Distribution, mc.l, in.ph, u1, u2, LD, dLD, mc.coefs);
             kd += s * GcL * dLD;
             f += s * GcL * LD;
         }
         kd += s * (mc.pf_visc / p.dt + mc.ViscousStab * GcL / mc.l);
-        f += s * mc.pf_visc / p.dt * (sm.Ph[a] - sm.PhO[a]);
+        f += s * mc.pf_visc / p.dt * (in.ph - in.pho);
         if (mc.Irreversible == IFE_IRR_PENALTY) {
             // PhaseFieldUtility::PenaltyIrr (PhaseFieldUtil.cpp:350-381): limits max(0, phi_old)
             // (at most 1 - 1e-3) and 1.1, the input Gc, PenaltyFactor 1e6 (PhaseFieldUtil.h:75)
-            const double lo = fmin(1.0 - 1.0e-3, fmax(0.0, (double)sm.PhO[a])), hi = 1.0 + 1e-1, pen = 1.0e6;
-            const double ph = sm.Ph[a];
+            const double lo = fmin(1.0 - 1.0e-3, fmax(0.0, in.pho)), hi = 1.0 + 1e-1, pen = 1.0e6;
+            const double ph = in.ph;
             if (ph < lo) { f += s * mc.Gc_in * pen * (ph - lo); kd += s * mc.Gc_in * pen; }
             if (ph > hi) { f += s * mc.Gc_in * pen * (ph - hi); kd += s * mc.Gc_in * pen; }
         }
-        sm.kdp[a] = kd;
-        sm.fnp[a] = f;
+        o.kdp = kd;
+        o.fnp = f;
     }
-    sm.kdt[a] = dth ? s * mc.Density_th * mc.cp * p.du_dt : 0.0;
-    sm.fnt[a] = dth ? s * mc.Density_th * mc.cp * sm.dT[a] : 0.0;
+    o.kdt = dth ? s * mc.Density_th * mc.cp * p.du_dt : 0.0;
+    o.fnt = dth ? s * mc.Density_th * mc.cp * in.dT : 0.0;
+}
+
+// The nodal terms of node a of the element staged in sm (nodal_core); returns its lumped weight.
+template <int DIM, typename SM>
+__device__ __forceinline__ double nodal_terms(SM& sm, const KParams& p, const int a, const bool du, const bool dph,
+                                              const bool dth) {
+    constexpr int NIP = SM::NIP;
+    NodalIn in;
+    double s = 0;
+    for (int ip = 0; ip < NIP; ip++) s += sm.ipc[SM::FW][ip] * (double)sm.Nsh[ip][a];
+    in.s = s;
+    in.ld = 0.0;
+    if (dph && p.mc.LumpedDrive)
+        for (int ip = 0; ip < NIP; ip++) in.ld += sm.ipc[SM::FLD][ip] * (double)sm.Nsh[ip][a];
+    in.GcL = sm.ipc[SM::FGC][NIP - 1];
+    in.ph = sm.Ph[a];
+    in.pho = sm.PhO[a];
+    in.dT = sm.dT[a];
+    for (int al = 0; al < DIM; al++) { in.dU[al] = sm.dU[al][a]; in.ddU[al] = sm.ddU[al][a]; }
+    NodalOut o;
+    nodal_core<DIM>(p, in, du, dph, dth, o);
+    if (du) {
+        sm.kdu[a] = o.kdu;
+        for (int al = 0; al < DIM; al++) sm.fnu[al][a] = o.fnu[al];
+    }
+    if (dph) {
+        sm.kdp[a] = o.kdp;
+        sm.fnp[a] = o.fnp;
+    }
+    sm.kdt[a] = o.kdt;
+    sm.fnt[a] = o.fnt;
     return s;
 }
 
@@ -2388,6 +2424,349 @@ __global__ __launch_bounds__(64 * W2_NW) void assemble2d_wave_batch(const KParam
             el = eln;
         }
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// 2D element kernel on MFMA (mixed mode): EPW elements per wavefront, contracted with
+// v_mfma_f32_4x4x1_16b_f32.  That instruction runs 16 independent 4x4 outer-product updates:
+// block b (lanes 4b..4b+3) takes A[i] from lane 4b+i and B[j] from lane 4b+j and accumulates
+// D[i][j] in lane 4b+j, register i.  Block b works on element e = b % EPW as its slot
+// s = b / EPW, so an element owns 16 / EPW blocks of every instruction.
+// The 9 node records of an element (Rec<2>, 60 entries each) are one 9 x 60 product,
+//   C[i][col] = sum_{c,ip} G_c(ip,i) B1[(c,ip)][col] + sum_ip N(ip,i) B2[ip][col],
+// with the columns in record order:
+//   K_UU (alpha,beta,j)  B1 = sum_d M[v(alpha,c)][v(beta,d)] G_d(ip,j)
+//   F_U alpha            B1 = SIG[v(alpha,c)]
+//   K_pp j               B1 = PHB G_c(ip,j),   B2 = PHA N(ip,j)
+//   F_p                  B1 = PHFG_c,          B2 = PHFN
+//   K_TT j               B1 = THB G_c(ip,j)
+//   F_T                  B1 = THFG_c,          B2 = THFN
+// (the sums assemble2d_wave forms row by row).  Rows are 3 blocks of 4 (the 3 padding rows
+// are computed but never stored), columns 15 blocks of 4; slot s of an element takes column
+// blocks s, s + 16/EPW, ...  Loads, the shape functions and the nodal terms run one (element,
+// node) or (element, IP, node) item per lane, so every phase keeps all 64 lanes busy.  The
+// element records and the gather are the ones the other kernels use.
+// ---------------------------------------------------------------------------------------
+constexpr int M2_CR = W2C_THFG + 3;  // FP32 coefficient rows: W2C_M .. W2C_THFG + 1, then a zero row
+constexpr int M2_Z = M2_CR - 1;
+
+template <int EPW>
+struct alignas(16) M2D {
+    static constexpr int N = 9, NIP = 9;
+    float Nsh[EPW][NIP][N];
+    float G[EPW][NIP][2][N];
+    float cz[EPW][M2_CR][NIP];
+    double w64[EPW][4][NIP];  // rows W, LD, GC, GW in FP64 (nodal terms, crack length)
+    union {
+        struct {
+            double ji[EPW][4][NIP];                         // J^-1 rows (shape functions)
+            double b1[EPW][2][3][3], db1[EPW][2][3][3];     // 1D factors
+        } s;
+        double nod[EPW][8][N];  // nodal additions: kdu, fnu x, fnu y, kdp, fnp, kdt, fnt, lumped weight
+    } u;
+    int64_t el[EPW];  // element of each slot, -1 past the end of the launch
+};
+
+// B1 of record column col for the term c: cz[r0] * (one ? 1 : G[o0]) + cz[r1] * G[o1], with
+// o = d * 9 + j an offset into G[e][ip]; columns past the record give zero rows
+struct M2Col {
+    int r0, r1, o0, o1;
+    bool one;
+};
+__device__ __forceinline__ M2Col m2_col(const int col, const int c) {
+    M2Col b{M2_Z, M2_Z, 0, 0, false};
+    if (col < 36) {
+        const int al = col / 18, be = (col / 9) % 2, j = col % 9;
+        b.r0 = W2C_M + w2_voigt(al, c) * 3 + w2_voigt(be, 0);
+        b.r1 = W2C_M + w2_voigt(al, c) * 3 + w2_voigt(be, 1);
+        b.o0 = j;
+        b.o1 = 9 + j;
+    } else if (col < 38) {
+        b.r0 = W2C_SIG + w2_voigt(col - 36, c);
+        b.one = true;
+    } else if (col < 47) {
+        b.r0 = W2C_PHB;
+        b.o0 = c * 9 + col - 38;
+    } else if (col == 47) {
+        b.r0 = W2C_PHFG + c;
+        b.one = true;
+    } else if (col < 57) {
+        b.r0 = W2C_THB;
+        b.o0 = c * 9 + col - 48;
+    } else if (col == 57) {
+        b.r0 = W2C_THFG + c;
+        b.one = true;
+    }
+    return b;
+}
+// B2 of record column col: cz[r] * (one ? 1 : N[o])
+__device__ __forceinline__ M2Col m2_coln(const int col) {
+    M2Col b{M2_Z, M2_Z, 0, 0, false};
+    if (col >= 38 && col < 47) { b.r0 = W2C_PHA; b.o0 = col - 38; }
+    else if (col == 47) { b.r0 = W2C_PHFN; b.one = true; }
+    else if (col == 57) { b.r0 = W2C_THFN; b.one = true; }
+    return b;
+}
+
+typedef float m2f4 __attribute__((ext_vector_type(4)));
+
+template <int EPW>
+__device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int64_t wb, M2D<EPW>& sm, const int lane) {
+    using RC = Rec<2>;
+    constexpr int N = 9, NIP = 9, NB = W2C_NF * NIP;        // one element's compact coefficient block
+    constexpr int SL = 16 / EPW, NG = (15 + SL - 1) / SL;    // slots per element, column-block groups
+    constexpr int PG = EPW == 16 ? 5 : 4;                     // groups per pass (accumulators held)
+    constexpr int NPR = (EPW * N + 63) / 64;                  // (element, node) items per lane
+    static_assert(RC::SIZE == 60 && RC::FU == 36 && RC::KP == 38 && RC::FP == 47 && RC::KT == 48 && RC::FT == 57 &&
+                      RC::WL == 58, "record layout of the 2D columns");
+    const int64_t e0 = wb * EPW;
+    if (lane < EPW) {
+        const int64_t k = e0 + lane;
+        sm.el[lane] = k < p.nlist ? (p.elist ? (int64_t)p.elist[k] : k) : -1;
+    }
+    wave_sync();
+    const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
+
+    // ---- loads ---------------------------------------------------------------------------
+    // (element, node) items: rank, then the nodal state it indexes
+    int rk[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        const int t = lane + 64 * k, e = t / N, a = t % N;
+        const int64_t el = t < EPW * N ? sm.el[e] : -1;
+        rk[k] = el >= 0 ? p.erank[el * N + a] : -1;
+    }
+    // the coefficient blocks, 8 loads in flight per lane, converted into the LDS rows
+    constexpr int KT = (EPW * NB + 63) / 64;
+#pragma unroll 1
+    for (int k0 = 0; k0 < KT; k0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int t = lane + 64 * (k0 + u), e = t / NB;
+            const int64_t el = t < EPW * NB ? sm.el[e] : -1;
+            v[u] = el >= 0 ? p.ipb[el * NB + (t - e * NB)] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int t = lane + 64 * (k0 + u);
+            if (t >= EPW * NB) continue;
+            const int e = t / NB, rem = t - e * NB, row = rem / NIP, ip = rem - row * NIP;
+            if (row < M2_Z) sm.cz[e][row][ip] = (float)v[u];
+            else if (row == W2C_W) sm.w64[e][0][ip] = v[u];
+            else if (row == W2C_LD) sm.w64[e][1][ip] = v[u];
+            else if (row == W2C_GC) sm.w64[e][2][ip] = v[u];
+            else if (row == W2C_GW) sm.w64[e][3][ip] = v[u];
+            else if (row >= W2C_JI && row < W2C_JI + 4) sm.u.s.ji[e][row - W2C_JI][ip] = v[u];
+        }
+    }
+    for (int t = lane; t < EPW * NIP; t += 64) sm.cz[t / NIP][M2_Z][t % NIP] = 0.0f;
+    double nv[NPR][7];  // dU x y, ddU x y, phi, phi_old, dT
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        for (int x = 0; x < 7; x++) nv[k][x] = 0.0;
+        const int r = rk[k];
+        if (r < 0) continue;
+        if (p.has_pf) {
+            for (int a = 0; a < 2; a++) {
+                const int64_t d = (int64_t)p.fpos_u[a] * p.Ng + r;
+                nv[k][a] = p.su.dx[d];
+                nv[k][2 + a] = p.su.ddx[d];
+            }
+            const int64_t dp = (int64_t)p.fpos_ph * p.Ng + r;
+            nv[k][4] = p.sph.x[dp];
+            nv[k][5] = p.sph.xold[dp];
+        }
+        if (p.has_T) nv[k][6] = p.sT.dx[(int64_t)p.fpos_T * p.Ng + r];
+    }
+    // 1D factors per (element, direction, function): b1 = C B1^T, db1 = C dB1^T
+    for (int t = lane; t < EPW * 6; t += 64) {
+        const int e = t / 6, d = (t / 3) % 2, a = t % 3;
+        const int64_t el = sm.el[e];
+        double cx[3] = {0.0, 0.0, 0.0};
+        if (el >= 0)
+            for (int k = 0; k < 3; k++) cx[k] = p.ext[(el * 2 + d) * 9 + a * 3 + k];
+        for (int q = 0; q < 3; q++) {
+            double s = 0, sd = 0;
+            for (int k = 0; k < 3; k++) { s += cx[k] * p.B1[q][k]; sd += cx[k] * p.dB1[q][k]; }
+            sm.u.s.b1[e][d][q][a] = s;
+            sm.u.s.db1[e][d][q][a] = sd;
+        }
+    }
+    wave_sync();
+    // ---- shape functions: N = CB, G = JI GB (B2_2, J^-1 from the material kernel) ----------
+    for (int t = lane; t < EPW * NIP * N; t += 64) {
+        const int e = t / (NIP * N), rem = t % (NIP * N), ip = rem / N, a = rem % N;
+        const int qx = ip % 3, qy = ip / 3, ax = a % 3, ay = a / 3;
+        const double bx = sm.u.s.b1[e][0][qx][ax], by = sm.u.s.b1[e][1][qy][ay];
+        const double gx = sm.u.s.db1[e][0][qx][ax], gy = sm.u.s.db1[e][1][qy][ay];
+        const double gb[2] = {gx * by, bx * gy};
+        sm.Nsh[e][ip][a] = (float)(bx * by);
+        for (int c = 0; c < 2; c++)
+            sm.G[e][ip][c][a] = (float)(sm.u.s.ji[e][c * 2][ip] * gb[0] + sm.u.s.ji[e][c * 2 + 1][ip] * gb[1]);
+    }
+    wave_sync();
+    // ---- nodal terms per (element, node) (nodal_core), crack length per element ---------------
+    double lc = 0.0;
+    if (dph && lane < EPW) {
+        for (int ip = 0; ip < NIP; ip++) lc += sm.w64[lane][3][ip];
+    }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+        const int t = lane + 64 * k, e = t / N, a = t % N;
+        if (rk[k] < 0) continue;
+        NodalIn in;
+        double s = 0;
+        for (int ip = 0; ip < NIP; ip++) s += sm.w64[e][0][ip] * (double)sm.Nsh[e][ip][a];
+        in.s = s;
+        in.ld = 0.0;
+        if (dph && p.mc.LumpedDrive)
+            for (int ip = 0; ip < NIP; ip++) in.ld += sm.w64[e][1][ip] * (double)sm.Nsh[e][ip][a];
+        in.GcL = sm.w64[e][2][NIP - 1];
+        in.dU[0] = nv[k][0]; in.dU[1] = nv[k][1]; in.dU[2] = 0.0;
+        in.ddU[0] = nv[k][2]; in.ddU[1] = nv[k][3]; in.ddU[2] = 0.0;
+        in.ph = nv[k][4];
+        in.pho = nv[k][5];
+        in.dT = nv[k][6];
+        NodalOut o{};
+        nodal_core<2>(p, in, du, dph, p.do_th, o);
+        // the union's shape-function part is dead: every lane is past the shape pass
+        double* const nd = &sm.u.nod[e][0][a];
+        nd[0] = o.kdu;
+        nd[N] = o.fnu[0];
+        nd[2 * N] = o.fnu[1];
+        nd[3 * N] = o.kdp;
+        nd[4 * N] = o.fnp;
+        nd[5 * N] = o.kdt;
+        nd[6 * N] = o.fnt;
+        nd[7 * N] = s;
+    }
+    if (dph && lane < EPW && sm.el[lane] >= 0) p.lcrack_elem[sm.el[lane]] = lc;
+    wave_sync();
+
+    // ---- contractions on MFMA, records out ----------------------------------------------------
+    const int blk = lane >> 2, q = lane & 3, e = blk % EPW, s = blk / EPW;
+    const float* const Ge = &sm.G[e][0][0][0];
+    const float* const Ne = &sm.Nsh[e][0][0];
+    const float* const Ce = &sm.cz[e][0][0];
+    const int64_t el = sm.el[e];
+    const int cb_lo = du ? 0 : 9, cb_hi = (dph || dT) ? 15 : 10;  // column blocks holding active parts
+    double chk = 0.0;
+#pragma unroll
+    for (int g0 = 0; g0 < NG; g0 += PG) {
+        if (g0 * SL >= cb_hi || (g0 + PG) * SL <= cb_lo) continue;
+        m2f4 acc[PG][3];
+#pragma unroll
+        for (int gi = 0; gi < PG; gi++)
+#pragma unroll
+            for (int rb = 0; rb < 3; rb++) acc[gi][rb] = m2f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            M2Col cd[PG];
+#pragma unroll
+            for (int gi = 0; gi < PG; gi++) cd[gi] = m2_col(4 * ((g0 + gi) * SL + s) + q, c);
+#pragma unroll 1
+            for (int ip = 0; ip < NIP; ip++) {
+                const float* const Gi = Ge + ip * 18;
+                const float* const Ci = Ce + ip;
+                float A[3];
+#pragma unroll
+                for (int rb = 0; rb < 3; rb++) A[rb] = Gi[c * 9 + min(4 * rb + q, 8)];
+#pragma unroll
+                for (int gi = 0; gi < PG; gi++) {
+                    if (g0 + gi >= NG) continue;
+                    const float x0 = cd[gi].one ? 1.0f : Gi[cd[gi].o0];
+                    const float B = __builtin_fmaf(Ci[cd[gi].r0 * NIP], x0, Ci[cd[gi].r1 * NIP] * Gi[cd[gi].o1]);
+#pragma unroll
+                    for (int rb = 0; rb < 3; rb++)
+                        acc[gi][rb] = __builtin_amdgcn_mfma_f32_4x4x1f32(A[rb], B, acc[gi][rb], 0, 0, 0);
+                }
+            }
+        }
+        if (dph || dT) {  // N-part: column blocks 9..14
+            M2Col cn[PG];
+#pragma unroll
+            for (int gi = 0; gi < PG; gi++) cn[gi] = m2_coln(4 * ((g0 + gi) * SL + s) + q);
+#pragma unroll 1
+            for (int ip = 0; ip < NIP; ip++) {
+                const float* const Ni = Ne + ip * N;
+                const float* const Ci = Ce + ip;
+                float A[3];
+#pragma unroll
+                for (int rb = 0; rb < 3; rb++) A[rb] = Ni[min(4 * rb + q, 8)];
+#pragma unroll
+                for (int gi = 0; gi < PG; gi++) {
+                    if (g0 + gi >= NG || (g0 + gi) * SL + SL - 1 < 9) continue;  // no slot reaches block 9
+                    const float B = Ci[cn[gi].r0 * NIP] * (cn[gi].one ? 1.0f : Ni[cn[gi].o0]);
+#pragma unroll
+                    for (int rb = 0; rb < 3; rb++)
+                        acc[gi][rb] = __builtin_amdgcn_mfma_f32_4x4x1f32(A[rb], B, acc[gi][rb], 0, 0, 0);
+                }
+            }
+        }
+        // write-back: nodal additions on their entries, only the parts this pass assembles
+        if (el >= 0) {
+            float* const rec = (float*)p.ebuf + el * N * RC::SIZE;
+#pragma unroll
+            for (int gi = 0; gi < PG; gi++) {
+                const int col = 4 * ((g0 + gi) * SL + s) + q;
+                if (g0 + gi >= NG || col >= RC::SIZE) continue;
+                const bool w = col < RC::KP ? du
+                               : col < RC::KT ? dph
+                               : col < RC::FT ? (dT && p.do_th)
+                               : col == RC::FT ? dT
+                               : col == RC::WL ? (dph && p.lag)
+                                               : false;
+                if (!w) continue;
+#pragma unroll
+                for (int rb = 0; rb < 3; rb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int i = 4 * rb + r;
+                        if (i >= N) continue;
+                        double v = (double)acc[gi][rb][r];
+                        if (col < 36) {
+                            const int al = col / 18, be = (col / 9) % 2, j = col % 9;
+                            if (al == be && j == i) v += sm.u.nod[e][0][i];
+                        } else if (col < RC::KP) {
+                            v += sm.u.nod[e][1 + col - RC::FU][i];
+                        } else if (col < RC::FP) {
+                            if (col - RC::KP == i) v += sm.u.nod[e][3][i];
+                        } else if (col == RC::FP) {
+                            v += sm.u.nod[e][4][i];
+                        } else if (col < RC::FT) {
+                            if (col - RC::KT == i) v += sm.u.nod[e][5][i];
+                        } else if (col == RC::FT) {
+                            v += sm.u.nod[e][6][i];
+                        } else {
+                            v = sm.u.nod[e][7][i];  // the element's lumped weight of node i
+                        }
+                        chk = fma(v, 0.0, chk);
+                        rec[i * RC::SIZE + col] = (float)v;
+                    }
+            }
+        }
+    }
+    if (chk != 0.0) atomicOr(&p.st->nonfinite, 1u);
+}
+
+template <int EPW>
+__global__ __launch_bounds__(64) void assemble2d_mfma(KParams p) {
+    if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
+    __shared__ M2D<EPW> sm;
+    assemble2d_mfma_body<EPW>(p, blockIdx.x, sm, threadIdx.x);
+}
+
+// batched form: one wavefront per EPW consecutive elements of one domain (the block table
+// counts ceil(elements / EPW) per domain)
+template <int EPW>
+__global__ __launch_bounds__(64) void assemble2d_mfma_batch(const KParams* P_, const int64_t* bptr_, int nd) {
+    KParamsC* P = (KParamsC*)P_;
+    I64C* bptr = (I64C*)bptr_;
+    __shared__ M2D<EPW> sm;
+    const int64_t b = blockIdx.x;
+    const int d = batch_domain(bptr, nd, b);
+    assemble2d_mfma_body<EPW>((const KParams&)P[d], b - bptr[d], sm, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -4419,6 +4798,13 @@ hipError_t launch_assemble(const KParams& p, int dim, hipStream_t s) {
     // 2D: one wavefront per element on the VALU; 3D: 16x16 tiles.  Mixed mode: FP32 tiles /
     // contractions and FP32 element records (the accumulators are FP32)
     if (dim == 2) {
+        if (p.mixed && (p.epw2d == 4 || p.epw2d == 8 || p.epw2d == 16)) {
+            const dim3 gm((unsigned)((p.nlist + p.epw2d - 1) / p.epw2d)), bm(64);
+            if (p.epw2d == 4) hipLaunchKernelGGL((assemble2d_mfma<4>), gm, bm, 0, s, p);
+            else if (p.epw2d == 8) hipLaunchKernelGGL((assemble2d_mfma<8>), gm, bm, 0, s, p);
+            else hipLaunchKernelGGL((assemble2d_mfma<16>), gm, bm, 0, s, p);
+            return hipGetLastError();
+        }
         const dim3 g2((unsigned)((p.nlist + W2_NW - 1) / W2_NW)), b2(64 * W2_NW);
         if (p.mixed) hipLaunchKernelGGL((assemble2d_wave<float, float>), g2, b2, 0, s, p);
         else hipLaunchKernelGGL((assemble2d_wave<double, double>), g2, b2, 0, s, p);
@@ -4694,6 +5080,15 @@ hipError_t launch_batch_material(const BatchArgs& a, int64_t grid, hipStream_t s
 hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipStream_t s) {
     if (a.tot[1] <= 0) return hipSuccess;
     const int64_t* bp = a.bptr[1];
+    if (a.dim == 2 && a.mixed && a.epw > 1) {  // a.epw elements per wavefront on MFMA, flat
+        if (a.tot_w <= 0) return hipSuccess;
+        const dim3 gm((unsigned)a.tot_w), bm(64);
+        if (a.epw == 4) hipLaunchKernelGGL((assemble2d_mfma_batch<4>), gm, bm, 0, s, a.P, a.bptr_w, a.nd);
+        else if (a.epw == 8) hipLaunchKernelGGL((assemble2d_mfma_batch<8>), gm, bm, 0, s, a.P, a.bptr_w, a.nd);
+        else if (a.epw == 16) hipLaunchKernelGGL((assemble2d_mfma_batch<16>), gm, bm, 0, s, a.P, a.bptr_w, a.nd);
+        else return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     if (a.dim == 2) {  // one wavefront per element, W2_NW per workgroup
         const int64_t wgs = (a.tot[1] + W2_NW - 1) / W2_NW;
         const bool ps2 = persistent(grid, wgs);

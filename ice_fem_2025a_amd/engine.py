@@ -433,8 +433,9 @@ class Batch:
         abi.check(self.L.ife_batch_get_shape(self.h, C.byref(t)))
         return self._todict(t)
 
-    def set_shape(self, grid_material=0, grid_assemble=0, grid_gather=0, lanes_assemble=0):
+    def set_shape(self, grid_material=0, grid_assemble=0, grid_gather=0, lanes_assemble=0, elems_per_wave=0):
         t = abi.BatchTuning()
         t.grid_material, t.grid_assemble, t.grid_gather = grid_material, grid_assemble, grid_gather
         t.lanes_assemble = lanes_assemble
+        t.elems_per_wave = elems_per_wave
         abi.check(self.L.ife_batch_set_shape(self.h, C.byref(t)))

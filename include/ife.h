@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define IFE_API_VERSION 5
+#define IFE_API_VERSION 6
 
 typedef enum {
     IFE_OK = 0,
@@ -218,6 +218,12 @@ typedef struct {
     int64_t elements;
     int64_t nodes;
     double ms_material, ms_assemble, ms_gather;  /* tuner: median time of the chosen shapes */
+    /* 2D element kernel: 1 = one wavefront per element, contractions on the VALU; 4, 8, 16 =
+     * that many elements per wavefront, contractions on MFMA (v_mfma_f32_4x4x1_16b_f32, mixed
+     * precision only, grid_assemble 0).  ife_batch_set_shape takes 0 as 1.  The MFMA forms
+     * agree with each other bit for bit and with the one-wavefront kernel within the mixed
+     * bound (they sum the same products in another order). */
+    int32_t elems_per_wave;
 } ife_batch_tuning;
 
 /* ---- life cycle ---------------------------------------------------------------- */
@@ -504,7 +510,9 @@ int ife_model_set_element_params(ife_physics* ph, int32_t model, const double* G
  * non-positive iteration count restores the reference's value. Fracture/CossPhaseField:
  * "coss_record_chunk" the elements of one chunk of u-step element records (0, the default: as
  * many as fit in 64 GiB; negative: no records, the element blocks are added into the CSR
- * colour by colour). */
+ * colour by colour).  Any model of a 2D physics: "elements_per_wave" the 2D element kernel of
+ * mixed precision (1, the default: one wavefront per element on the VALU; 4, 8 or 16: that
+ * many elements per wavefront on MFMA; FP64 mode always uses the first). */
 int ife_model_set_param(ife_physics* ph, int32_t model, const char* name, double value);
 
 /* ---- batches of independent domains (no counterpart in the reference, which runs one
@@ -532,7 +540,8 @@ int ife_batch_capture(ife_batch* b);
 int ife_batch_launch(ife_batch* b, void* stream);
 
 /* Autotuner: times every candidate launch shape of each kernel on the current inputs
- * (grid = one workgroup per block, or 256, 512, ... persistent workgroups) and keeps
+ * (grid = one workgroup per block, or 256, 512, ... persistent workgroups; in 2D mixed
+ * precision also the MFMA element kernel with 4, 8 and 16 elements per wavefront) and keeps
  * the fastest.  Runs assemblies on
  * `stream` and synchronises it; drops a captured graph. */
 int ife_batch_tune(ife_batch* b, void* stream, ife_batch_tuning* out);

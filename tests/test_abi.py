@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     for name in decl:
         assert hasattr(lib, name), name
     assert set(decl) == set(abi.EXPORTED)
-    assert lib.ife_api_version() == 5
+    assert lib.ife_api_version() == 6
 
 
 def test_unknown_model_is_rejected():

@@ -123,6 +123,15 @@ def test_batch_every_shape_gives_the_same_bits(require_gpu, kind):
     t = b.tune()
     _say(f"  tuned: {t}")
     assert t["candidates_timed"] >= 3 and t["lanes_assemble"] == (64 if kind == "2d" else 256)
+    if t["elems_per_wave"] > 1:
+        # the tuner chose the MFMA element kernel, whose bits are its own (the same products
+        # summed in another order; tests/test_gpu_mfma2d.py): compare with its single-domain form
+        ref = []
+        for e in doms:
+            e.set_param(0, "elements_per_wave", t["elems_per_wave"])
+            e.reset_step(0)
+            e.assemble_all()
+            ref.append(_snapshot(e))
     for e in doms:
         e.reset_step(0)
     b.launch()
