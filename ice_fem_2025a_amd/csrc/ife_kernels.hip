@@ -195,12 +195,24 @@ __device__ void damage(int f, double phi, double k0, double p, double& D, double
     }
 }
 
+// x / c for a constant c, given rc = RN(1/c): q0 = x rc, the exact residual r = x - q0 c by
+// one FMA, then q0 + r rc by another -- Markstein's division by a constant, which returns the
+// correctly rounded quotient x / c (checked against the division for c = 1e6 on 4e8 random
+// operands over 400 binades) in 3 FP64 operations instead of a general division's ~11
+__device__ __forceinline__ double div_c(double x, double c, double rc) {
+    const double q0 = x * rc;
+    const double r = fma(-q0, c, x);
+    return fma(r, rc, q0);
+}
+constexpr double SC_VE = 1.0e6, RSC_VE = 1.0e-6;  // the yield functions' stress scale and RN(1 / scale)
+
 // SolidMaterial::YieldFunctionVE (SolidMaterial.cpp:477-492), dF only
 __device__ double yield_ve(const double S[6], double dF[6]) {
-    const double sc = 1.0e6, eps = 1.0e-12;
+    const double eps = 1.0e-12;
     const double d = 1.0 - 1.0 / 3.0, o = 0.0 - 1.0 / 3.0;
+    const double sc = SC_VE;
     double s[6];
-    for (int i = 0; i < 6; i++) s[i] = S[i] / sc;
+    for (int i = 0; i < 6; i++) s[i] = div_c(S[i], SC_VE, RSC_VE);
     double dJ2[6];
     dJ2[0] = d * s[0] + o * s[1] + o * s[2];
     dJ2[1] = o * s[0] + d * s[1] + o * s[2];
@@ -245,11 +257,10 @@ __device__ void getKF(const MatIP& m, double nC, double PreFac, double Damage, c
     Dmul(m, sol, S);
     double F = yield_ve(S, a);
     for (int i = 0; i < 6; i++) f[i] = sol[i] - eps_t[i] + eps_old[i] + sol[6] * a[i];
-    const double sc = 1.0e6;
     // (F/sc)^n and n (F/sc)^(n-1) from one power: x^n = x^(n-1) x
-    const double x = F / sc, pm1 = pow_nm1(x, nC);
+    const double x = div_c(F, SC_VE, RSC_VE), pm1 = pow_nm1(x, nC);
     f[6] = PreFac * (pm1 * x) - sol[6];
-    double fac = nC * PreFac / sc * pm1;
+    double fac = div_c(nC * PreFac, SC_VE, RSC_VE) * pm1;
     double Da[6], da[6];
     for (int i = 0; i < 6; i++) da[i] = Damage * a[i];
     Dmul(m, da, Da);  // D symmetric: (Damage dF^T D)_j = (D (Damage dF))_j
@@ -1676,18 +1687,48 @@ struct Smem {
     static constexpr int NP = (N + 15) / 16 * 16;    // nodes padded to whole 16-wide tiles
     static constexpr int IPP = (NIP + 3) / 4 * 4;    // IPs padded to whole 4-deep k steps
     static constexpr int FW = IPB_W, FLD = IPB_LDRIVE, FGC = IPB_GC;  // ipc rows read by nodal_terms
+    // FP32 rows of N and G are padded by 16 entries, so the four IPs a k-step reads at once
+    // (lanes 16 lk .. 16 lk + 15, IP 4 s + lk) fall on different LDS banks
+    static constexpr int PADS = sizeof(T) == 4 ? 16 : 0;
+    struct GRow {
+        T v[DIM][NP];
+        T pad[PADS > 0 ? PADS : 1];
+    };
     double X[N][4];            // x, y, z, W
-    T Nsh[IPP][NP];            // N, zero beyond (NIP, N)
-    T G[IPP][DIM][NP];         // dN/dx, zero beyond (NIP, N)
+    T Nsh[IPP][NP + PADS];     // N, zero beyond (NIP, N)
+    GRow G[IPP];               // dN/dx (G[ip].v[c][a]), zero beyond (NIP, N)
     struct {
         double b1[DIM][3][3];      // 1D B-spline values  [dir][gauss pt][fn]
         double db1[DIM][3][3];     // 1D derivatives
     } shp;
     double dU[DIM][N], ddU[DIM][N];
     double Ph[N], PhO[N], dT[N];
-    double ipc[IPB_NF][IPP];   // per-IP coefficients from the material kernel (ife::IPB_*), zero beyond NIP
+    // per-IP coefficients from the material kernel (ife::IPB_*), zero beyond NIP.  FP64 mode:
+    // every row in FP64.  Mixed mode: the contraction rows (below IPB_GW) once in FP32, the rows
+    // from IPB_LDRIVE on (lumped weights, geometry) in FP64.
+    static constexpr bool MIX = sizeof(T) == 4;
+    static constexpr int D0 = MIX ? IPB_LDRIVE : 0;  // first row held in FP64
+    static constexpr int NF32 = MIX ? IPB_GW : 1;    // rows held in FP32
+    double ipc[IPB_NF - D0][IPP];
+    T ipf[NF32][IPP];
     double kdu[N], kdp[N], kdt[N];       // nodal diagonal additions to K_UU, K_pp, K_TT
     double fnu[DIM][N], fnp[N], fnt[N];  // nodal (lumped) additions to F_U, F_p, F_T
+    // row f at IP ip: cd in FP64 (f >= D0), cf in the tile type (f < IPB_GW)
+    __device__ __forceinline__ double cd(int f, int ip) const { return ipc[f - D0][ip]; }
+    __device__ __forceinline__ T cf(int f, int ip) const {
+        if constexpr (MIX) return ipf[f][ip];
+        else return ipc[f][ip];
+    }
+    __device__ __forceinline__ const T* cfrow(int f) const {
+        if constexpr (MIX) return &ipf[f][0];
+        else return &ipc[f][0];
+    }
+    __device__ __forceinline__ void put(int f, int ip, double v) {
+        if (f >= D0) ipc[f - D0][ip] = v;
+        if constexpr (MIX) {
+            if (f < IPB_GW) ipf[f][ip] = (T)v;
+        }
+    }
 };
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -1732,14 +1773,14 @@ __device__ __forceinline__ void kuu_loop(const Smem<DIM, T>& sm, const int al, c
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
         const int r = vrow<DIM>(al, c);
-        const double* const Mrow = &sm.ipc[IPB_M + r * 6][0];
-        const double* const Srow = &sm.ipc[IPB_SIG + r][0];
+        const T* const Mrow = sm.cfrow(IPB_M + r * 6);
+        const T* const Srow = sm.cfrow(IPB_SIG + r);
 #pragma unroll 1
         for (int s = 0; s < KSTEP; s++) {
             const int ip = s * 4 + lk;
             T a[MT];
 #pragma unroll
-            for (int mt = 0; mt < MT; mt++) a[mt] = sm.G[ip][c][mt * 16 + l16];
+            for (int mt = 0; mt < MT; mt++) a[mt] = sm.G[ip].v[c][mt * 16 + l16];
             T m[6];
 #pragma unroll
             for (int q = 0; q < 6; q++) m[q] = (T)Mrow[q * IPP + ip];
@@ -1749,7 +1790,7 @@ __device__ __forceinline__ void kuu_loop(const Smem<DIM, T>& sm, const int al, c
                 const int j = jt * 16 + l16;
                 T g[DIM];
 #pragma unroll
-                for (int d = 0; d < DIM; d++) g[d] = sm.G[ip][d][j];
+                for (int d = 0; d < DIM; d++) g[d] = sm.G[ip].v[d][j];
 #pragma unroll
                 for (int be = 0; be < DIM; be++) {
                     T bb = 0;
@@ -1764,34 +1805,57 @@ __device__ __forceinline__ void kuu_loop(const Smem<DIM, T>& sm, const int al, c
     }
 }
 
+// Write-back of the K_UU block row of alpha and F_U(alpha).  The nodal terms touch only the
+// diagonal of the beta = alpha block and the residual column, so they are added to those
+// accumulator entries first (in FP64, one rounding to R, as for every stored value); then
+// every entry goes out straight from the accumulators.
 template <int DIM, int MT, typename T, typename R>
 __device__ __forceinline__ void kuu_store(const Smem<DIM, T>& sm, R* rec, const int al, const int lk, const int l16,
-                                          const typename Mfma<T>::V (&acc)[DIM][MT][MT], double& chk) {
+                                          typename Mfma<T>::V (&acc)[DIM][MT][MT], double& chk) {
     using RC = Rec<DIM>;
-    constexpr int N = Smem<DIM, T>::N, FCOL = N;
+    constexpr int N = Smem<DIM, T>::N, FCOL = N, FT_ = FCOL / 16, FL = FCOL % 16;
 #pragma unroll
-    for (int be = 0; be < DIM; be++)
+    for (int be = 0; be < DIM; be++) {
+        if (be != al) continue;  // uniform: the wave's own alpha
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-            for (int jt = 0; jt < MT; jt++) {
-                const int j = jt * 16 + l16;
+            for (int v = 0; v < 4; v++) {
+                const int i = mt * 16 + Mfma<T>::row(lk, v);
+                if (i < N && l16 == (i & 15)) acc[be][mt][mt][v] = (T)((double)acc[be][mt][mt][v] + sm.kdu[i]);
+            }
+    }
+    if (l16 == FL) {
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = mt * 16 + Mfma<T>::row(lk, v);
+                if (i < N) acc[0][mt][FT_][v] = (T)((double)acc[0][mt][FT_][v] + sm.fnu[al][i]);
+            }
+    }
+    T ck = 0;
+#pragma unroll
+    for (int jt = 0; jt < MT; jt++) {
+        const int j = jt * 16 + l16;
+#pragma unroll
+        for (int be = 0; be < DIM; be++) {
+            const bool jok = j < N || (be == 0 && j == FCOL);
+            const int coff = j < N ? RC::KU + (al * DIM + be) * N + j : RC::FU + al;
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int i = mt * 16 + Mfma<T>::row(lk, v);
-                    if (i >= N) continue;
-                    if (j < N) {
-                        double val = acc[be][mt][jt][v];
-                        if (al == be && i == j) val += sm.kdu[i];
-                        chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::KU + (al * DIM + be) * N + j] = (R)val;
-                    } else if (be == 0 && j == FCOL) {
-                        double val = (double)acc[be][mt][jt][v] + sm.fnu[al][i];
-                        chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::FU + al] = (R)val;
+                    const T val = acc[be][mt][jt][v];
+                    if (jok && i < N) {
+                        ck = ck + val * (T)0;
+                        rec[i * RC::SIZE + coff] = (R)val;
                     }
                 }
-            }
+        }
+    }
+    chk += (double)ck;
 }
 
 // K_pp = sum_ip phA N N^T + phB G^T G (acc[0]) and K_TT = sum_ip thB G^T G (acc[1]) in one
@@ -1813,14 +1877,14 @@ __device__ __forceinline__ void kpt_loop(const Smem<DIM, T>& sm, const int lk, c
 #pragma unroll
     for (int q = 0; q <= DIM; q++) {
         const int qg = q > 0 ? q - 1 : 0;
-        const double* const cpr = &sm.ipc[q == 0 ? IPB_PHA : IPB_PHB][0];
-        const double* const fpr = &sm.ipc[q == 0 ? IPB_PHFN : IPB_PHFG + qg][0];
-        const double* const ctr = &sm.ipc[IPB_THB][0];
-        const double* const ftr = &sm.ipc[q == 0 ? IPB_THFN : IPB_THFG + qg][0];
+        const T* const cpr = sm.cfrow(q == 0 ? IPB_PHA : IPB_PHB);
+        const T* const fpr = sm.cfrow(q == 0 ? IPB_PHFN : IPB_PHFG + qg);
+        const T* const ctr = sm.cfrow(IPB_THB);
+        const T* const ftr = sm.cfrow(q == 0 ? IPB_THFN : IPB_THFG + qg);
 #pragma unroll 1
         for (int s = 0; s < KSTEP; s++) {
             const int ip = s * 4 + lk;
-            const T* const X = q == 0 ? &sm.Nsh[ip][0] : &sm.G[ip][qg][0];
+            const T* const X = q == 0 ? &sm.Nsh[ip][0] : &sm.G[ip].v[qg][0];
             T a[MT];
 #pragma unroll
             for (int mt = 0; mt < MT; mt++) a[mt] = X[mt * 16 + l16];
@@ -1845,46 +1909,51 @@ __device__ __forceinline__ void kpt_loop(const Smem<DIM, T>& sm, const int lk, c
     }
 }
 
+// Write-back of K_pp, F_p (acc[0]) and K_TT, F_T (acc[1]), nodal terms added as in kuu_store.
 template <int DIM, int MT, typename T, typename R>
 __device__ __forceinline__ void kpt_store(const Smem<DIM, T>& sm, R* rec, const int lk, const int l16, const bool dph,
-                                          const bool dT, const bool dth, const typename Mfma<T>::V (&acc)[DIM][MT][MT],
+                                          const bool dT, const bool dth, typename Mfma<T>::V (&acc)[DIM][MT][MT],
                                           double& chk) {
     using RC = Rec<DIM>;
-    constexpr int N = Smem<DIM, T>::N, FCOL = N;
+    constexpr int N = Smem<DIM, T>::N, FCOL = N, FT_ = FCOL / 16, FL = FCOL % 16;
 #pragma unroll
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-        for (int nt = 0; nt < MT; nt++) {
-            const int j = nt * 16 + l16;
+        for (int v = 0; v < 4; v++) {
+            const int i = mt * 16 + Mfma<T>::row(lk, v);
+            if (i < N && l16 == (i & 15)) {
+                acc[0][mt][mt][v] = (T)((double)acc[0][mt][mt][v] + sm.kdp[i]);
+                acc[1][mt][mt][v] = (T)((double)acc[1][mt][mt][v] + sm.kdt[i]);
+            }
+            if (i < N && l16 == FL) {
+                acc[0][mt][FT_][v] = (T)((double)acc[0][mt][FT_][v] + sm.fnp[i]);
+                acc[1][mt][FT_][v] = (T)((double)acc[1][mt][FT_][v] + sm.fnt[i]);
+            }
+        }
+    T ck = 0;
+#pragma unroll
+    for (int nt = 0; nt < MT; nt++) {
+        const int j = nt * 16 + l16;
+        const bool kj = j < N, fj = j == FCOL;
+        const bool wp = dph && (kj || fj), wt = (dth && kj) || (dT && fj);
+        const int op = kj ? RC::KP + j : RC::FP, ot = kj ? RC::KT + j : RC::FT;
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int i = mt * 16 + Mfma<T>::row(lk, v);
-                if (i >= N) continue;
-                if (j < N) {
-                    if (dph) {
-                        double val = (double)acc[0][mt][nt][v] + (i == j ? sm.kdp[i] : 0.0);
-                        chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::KP + j] = (R)val;
-                    }
-                    if (dth) {
-                        double val = (double)acc[1][mt][nt][v] + (i == j ? sm.kdt[i] : 0.0);
-                        chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::KT + j] = (R)val;
-                    }
-                } else if (j == FCOL) {
-                    if (dph) {
-                        double val = (double)acc[0][mt][nt][v] + sm.fnp[i];
-                        chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::FP] = (R)val;
-                    }
-                    if (dT) {
-                        double val = (double)acc[1][mt][nt][v] + sm.fnt[i];
-                        chk = fma(val, 0.0, chk);
-                        rec[i * RC::SIZE + RC::FT] = (R)val;
-                    }
+                const T vp = acc[0][mt][nt][v], vt = acc[1][mt][nt][v];
+                if (wp && i < N) {
+                    ck = ck + vp * (T)0;
+                    rec[i * RC::SIZE + op] = (R)vp;
+                }
+                if (wt && i < N) {
+                    ck = ck + vt * (T)0;
+                    rec[i * RC::SIZE + ot] = (R)vt;
                 }
             }
-        }
+    }
+    chk += (double)ck;
 }
 
 // Nodal (lumped) terms of one node from its lumped weight s = sum_ip w N_a, lumped driving
@@ -1959,12 +2028,12 @@ __device__ __forceinline__ double nodal_terms(SM& sm, const KParams& p, const in
     constexpr int NIP = SM::NIP;
     NodalIn in;
     double s = 0;
-    for (int ip = 0; ip < NIP; ip++) s += sm.ipc[SM::FW][ip] * (double)sm.Nsh[ip][a];
+    for (int ip = 0; ip < NIP; ip++) s += sm.cd(SM::FW, ip) * (double)sm.Nsh[ip][a];
     in.s = s;
     in.ld = 0.0;
     if (dph && p.mc.LumpedDrive)
-        for (int ip = 0; ip < NIP; ip++) in.ld += sm.ipc[SM::FLD][ip] * (double)sm.Nsh[ip][a];
-    in.GcL = sm.ipc[SM::FGC][NIP - 1];
+        for (int ip = 0; ip < NIP; ip++) in.ld += sm.cd(SM::FLD, ip) * (double)sm.Nsh[ip][a];
+    in.GcL = sm.cd(SM::FGC, NIP - 1);
     in.ph = sm.Ph[a];
     in.pho = sm.PhO[a];
     in.dT = sm.dT[a];
@@ -2040,47 +2109,60 @@ __device__ __forceinline__ void assemble_body(const KParams& p, const int64_t bl
 #pragma unroll
     for (int k = 0; k < IPC_PER; k++) {
         const int t = tid + k * NTA;
-        if (t < IPC_NV) sm.ipc[t / NIP][t % NIP] = ipv[k];
+        if (t < IPC_NV) sm.put(t / NIP, t % NIP, ipv[k]);
     }
-    for (int t = tid; t < IPB_NF * (IPP - NIP); t += NTA) sm.ipc[t / (IPP - NIP)][NIP + t % (IPP - NIP)] = 0.0;
+    for (int t = tid; t < IPB_NF * (IPP - NIP); t += NTA) sm.put(t / (IPP - NIP), NIP + t % (IPP - NIP), 0.0);
     __syncthreads();
 
     // ---- shape functions ------------------------------------------------------------
-    // One elementwise pass over (ip, node): CB, GB from the tensor-product 1D factors, the
+    // One pass over (ip, node): CB, GB from the tensor-product 1D factors, the
     // rational basis N = W CB / wFun, GXi = W (GB / wFun - CB dW / wFun^2) (3D,
     // NURBS3_Cube2.cpp:96-112) and G = J^-1 GXi (B2_2.cpp:79-85 in 2D: N = CB, GXi = GB,
     // J^-1 in the transposed form), with J^-1 / wFun, 1 / wFun and J^-1 dW / wFun^2 taken
     // from the material kernel, which forms them at the same IP.  Padding entries are zeroed.
-    for (int i = tid; i < IPP * NP; i += NTA) {
-        const int ip = i / NP, a = i % NP;
-        if (ip >= NIP || a >= N) {
-            sm.Nsh[ip][a] = 0;
-            for (int c = 0; c < DIM; c++) sm.G[ip][c][a] = 0;
-            continue;
-        }
+    static_assert(DIM == 3, "the 3D element kernel");
+    // lane (ip, g): IP ip and the nodes a = g + 9 m (m = 0, 1, 2), which share the x and y
+    // factors, so the lane reads its IP's coefficients and those factors once
+    if (tid < NIP * 9) {
+        const int ip = tid / 9, g = tid % 9;
         const int qx = ip % 3, qy = (ip / 3) % 3, qz = ip / 9;
-        const int ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+        const int ax = g % 3, ay = g / 3;
         const double bx = sm.shp.b1[0][qx][ax], by = sm.shp.b1[1][qy][ay];
         const double gx = sm.shp.db1[0][qx][ax], gy = sm.shp.db1[1][qy][ay];
-        double cb, gb[DIM];
-        if (DIM == 2) {
-            cb = bx * by;
-            gb[0] = gx * by;
-            gb[1] = bx * gy;
-        } else {
-            const double bz = sm.shp.b1[DIM - 1][qz][az], gz = sm.shp.db1[DIM - 1][qz][az];
-            cb = bx * by * bz;
-            gb[0] = gx * by * bz;
-            gb[1] = bx * gy * bz;
-            gb[DIM - 1] = bx * by * gz;
+        const double bxy = bx * by, gxy = gx * by, bgy = bx * gy;
+        const double wf = sm.cd(IPB_WF, ip);
+        double ji[3][3], dw[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            dw[c] = sm.cd(IPB_DW + c, ip);
+#pragma unroll
+            for (int d = 0; d < 3; d++) ji[c][d] = sm.cd(IPB_JI + c * 3 + d, ip);
         }
-        const double Wa = DIM == 3 ? sm.X[a][3] : 1.0;
-        sm.Nsh[ip][a] = (T)(DIM == 3 ? Wa * cb * sm.ipc[IPB_WF][ip] : cb);
-        for (int c = 0; c < DIM; c++) {
-            double s = 0;
-            for (int d = 0; d < DIM; d++) s += sm.ipc[IPB_JI + c * DIM + d][ip] * gb[d];
-            sm.G[ip][c][a] = (T)(DIM == 3 ? Wa * (s - cb * sm.ipc[IPB_DW + c][ip]) : s);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const int a = g + 9 * m;
+            const double bz = sm.shp.b1[2][qz][m], gz = sm.shp.db1[2][qz][m];
+            const double cb = bxy * bz;
+            const double gb[3] = {gxy * bz, bgy * bz, bxy * gz};
+            const double Wa = sm.X[a][3];
+            sm.Nsh[ip][a] = (T)(Wa * cb * wf);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double s = 0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) s += ji[c][d] * gb[d];
+                sm.G[ip].v[c][a] = (T)(Wa * (s - cb * dw[c]));
+            }
         }
+    }
+    // padding: the IP rows beyond NIP and the node columns beyond N are zero
+    for (int t = tid; t < (IPP - NIP) * NP + NIP * (NP - N); t += NTA) {
+        int ip, a;
+        if (t < (IPP - NIP) * NP) { ip = NIP + t / NP; a = t % NP; }
+        else { const int u = t - (IPP - NIP) * NP; ip = u / (NP - N); a = N + u % (NP - N); }
+        sm.Nsh[ip][a] = 0;
+#pragma unroll
+        for (int c = 0; c < DIM; c++) sm.G[ip].v[c][a] = 0;
     }
     __syncthreads();
 
@@ -2100,7 +2182,7 @@ __device__ __forceinline__ void assemble_body(const KParams& p, const int64_t bl
         }
         if (dph && lane == 0) {
             double s = 0;
-            for (int ip = 0; ip < NIP; ip++) s += sm.ipc[IPB_GW][ip];
+            for (int ip = 0; ip < NIP; ip++) s += sm.cd(IPB_GW, ip);
             p.lcrack_elem[el] = s;
         }
         if (dph || dT) kpt_loop<DIM, MT, T>(sm, lk, l16, dph, dT, acc);
@@ -2174,6 +2256,7 @@ struct alignas(16) Wave2D {
         if constexpr (sizeof(T) == 8) return ipc[f][ip];
         else return czs[f * NIP + ip];
     }
+    __device__ __forceinline__ double cd(int f, int ip) const { return ipc[f][ip]; }
 };
 
 __device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
