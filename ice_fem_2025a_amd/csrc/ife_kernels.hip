@@ -1900,6 +1900,9 @@ __device__ __forceinline__ void kpt_loop(const Smem<DIM, T>& sm, const int lk, c
                 }
 #pragma unroll
                 for (int mt = 0; mt < MT; mt++) {
+                    // K_pp and K_TT are symmetric (scalar coefficients per IP): the tiles below the
+                    // diagonal are written from their mirror images (kpt_store)
+                    if (mt > nt) continue;
                     if (dph) acc[0][mt][nt] = Mfma<T>::op(a[mt], bp, acc[0][mt][nt]);
                     // q = 0 feeds K_TT nothing: only the tile holding the residual column
                     if (dT && (q > 0 || nt == MT - 1)) acc[1][mt][nt] = Mfma<T>::op(a[mt], bt, acc[1][mt][nt]);
@@ -1938,7 +1941,7 @@ __device__ __forceinline__ void kpt_store(const Smem<DIM, T>& sm, R* rec, const 
         const bool wp = dph && (kj || fj), wt = (dth && kj) || (dT && fj);
         const int op = kj ? RC::KP + j : RC::FP, ot = kj ? RC::KT + j : RC::FT;
 #pragma unroll
-        for (int mt = 0; mt < MT; mt++)
+        for (int mt = 0; mt <= nt; mt++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int i = mt * 16 + Mfma<T>::row(lk, v);
@@ -1950,6 +1953,12 @@ __device__ __forceinline__ void kpt_store(const Smem<DIM, T>& sm, R* rec, const 
                 if (wt && i < N) {
                     ck = ck + vt * (T)0;
                     rec[i * RC::SIZE + ot] = (R)vt;
+                }
+                // the mirror entry (j, i) of a tile above the diagonal (never a diagonal entry
+                // nor the residual column)
+                if (mt < nt && kj && i < N) {
+                    if (dph) rec[j * RC::SIZE + RC::KP + i] = (R)vp;
+                    if (dth) rec[j * RC::SIZE + RC::KT + i] = (R)vt;
                 }
             }
     }
