@@ -859,7 +859,7 @@ DevBuf<double>* history_buf(Model* m, const std::string& nm, int64_t& ncomp) {
 // ==================================================================================
 struct ife_batch {
     std::vector<ife_physics*> doms;
-    int dim = 0, mixed = 0, vep = 0, exact = 0, full = 0;
+    int dim = 0, mixed = 0, vep = 0, exact = 0, full = 0, lin = 0;
     DevBuf<KParams> dP;
     DevBuf<int64_t> bp[3];      // block prefix sums: material, element, gather kernels
     int64_t tot[3] = {0, 0, 0};
@@ -887,7 +887,7 @@ struct ife_batch {
         BatchArgs a{};
         a.P = dP.p;
         a.nd = (int)doms.size();
-        a.dim = dim; a.mixed = mixed; a.vep = vep; a.exact = exact; a.full = full;
+        a.dim = dim; a.mixed = mixed; a.vep = vep; a.exact = exact; a.full = full; a.lin = lin;
         for (int k = 0; k < 3; k++) { a.bptr[k] = bp[k].p; a.tot[k] = tot[k]; }
         set_epw(a, epw);
         return a;
@@ -906,9 +906,10 @@ namespace {
 // device parameter array; the block tables depend only on the meshes.
 void batch_update(ife_batch* b) {
     std::vector<KParams> P;
-    int vep = -1, exact = -1, full = 0, mixed = -1;
+    int vep = -1, exact = -1, full = 0, mixed = -1, lin = 1;
     for (auto* ph : b->doms) {
         KParams p = all_steps_params(ph);
+        lin &= p.mc.Rheology == IFE_RHEO_LE;
         const int v = p.mc.Rheology == IFE_RHEO_VEP;
         const int x = !v && p.mc.exact_tangent;
         const int sp = p.mc.Split;
@@ -921,7 +922,7 @@ void batch_update(ife_batch* b) {
             THROW(IFE_ERR_UNSUPPORTED, "batch: the higher-order crack densities run through ife_physics_assemble");
         P.push_back(p);
     }
-    b->vep = vep; b->exact = exact; b->full = full; b->mixed = mixed;
+    b->vep = vep; b->exact = exact; b->full = full; b->mixed = mixed; b->lin = lin;
     b->dP.upload(P.data(), P.size());
     b->drop_graph();  // kernel arguments point at dP, which upload may have moved
 }

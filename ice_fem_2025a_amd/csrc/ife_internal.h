@@ -163,6 +163,7 @@ enum { COSS_M = 0, COSS_SIG = 225, COSS_WI = 243, COSS_WG = 244, COSS_NF = 245 }
 struct BatchArgs {
     const KParams* P;
     int nd, dim, mixed, vep, exact, full;
+    int lin;  // every domain linear elastic: the material kernel without a return map
     const int64_t* bptr[3];
     int64_t tot[3];
     // 2D element kernel: elements per wavefront (1: assemble2d_wave; 4, 8, 16: assemble2d_mfma,

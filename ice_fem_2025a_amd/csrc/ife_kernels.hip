@@ -1120,8 +1120,10 @@ struct MatSmem {
 // (u residual via the stress, phase residual via the history energy, T residual via the
 // viscous heating).
 // FULL: every energy split; false: None / VolStrains / VolStress only.
+// LIN: the linear-elastic rheology only (no return map is compiled in, so the kernel needs far
+// fewer registers: the 2D form without a split runs four workgroups per CU instead of two).
 // blk: the workgroup's block of EPB elements within the domain p.
-template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL>
+template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL, bool LIN = false>
 __device__ __forceinline__ void material_body(const KParams& p, const int64_t blk) {
     using E = ET<DIM>;
     constexpr int N = E::N, NIP = E::NIP, EPB = NT / NIP;
@@ -1358,7 +1360,7 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                     if constexpr (!ADJ)
                         for (int k = 0; k < 3; k++) p.MH[k * S + gi] = hist[k];
                 }
-            } else if (rh == IFE_RHEO_VE || rh == IFE_RHEO_DVE) {
+            } else if (!LIN && (rh == IFE_RHEO_VE || rh == IFE_RHEO_DVE)) {
                 if (!(phi > mc.PhiViscLim)) {
                     double dmg = 1.0, u1, u2;
                     if (rh == IFE_RHEO_DVE) damage(mc.DamFunction, phi, mc.k0, mc.p, dmg, u1, u2);
@@ -1402,7 +1404,7 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                             for (int j = 0; j < 6; j++) DM[i][j] = B[i][j];
                     }
                 }
-            } else if (rh == IFE_RHEO_VE_EXPL || rh == IFE_RHEO_DVE_EXPL) {
+            } else if (!LIN && (rh == IFE_RHEO_VE_EXPL || rh == IFE_RHEO_DVE_EXPL)) {
                 if (!(phio > mc.PhiViscLim)) {
                     double dmg = 1.0, u1, u2;
                     if (rh == IFE_RHEO_DVE_EXPL) damage(mc.DamFunction, phio, mc.k0, mc.p, dmg, u1, u2);
@@ -1648,10 +1650,10 @@ __device__ __forceinline__ int batch_domain(I64C* bptr, int nd, int64_t b) {
     return lo;
 }
 
-template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL>
-__global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
+template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL, bool LIN = false>
+__global__ __launch_bounds__(NT, LIN && DIM == 2 && !FULL ? 4 : 2) void material_kernel(KParams p) {
     if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
-    material_body<DIM, VEP, EXACT, ADJ, FULL>(p, blockIdx.x);
+    material_body<DIM, VEP, EXACT, ADJ, FULL, LIN>(p, blockIdx.x);
 }
 
 // Batched form: the blocks of all domains in one launch.  PERSIST = false: one workgroup per
@@ -1659,20 +1661,20 @@ __global__ __launch_bounds__(NT, 2) void material_kernel(KParams p) {
 // over the blocks, with a barrier at the top of each item that keeps the next item's LDS
 // staging behind the previous item's last LDS reads.  (The loop costs registers, so both
 // forms exist and the launch-shape tuner picks.)
-template <int DIM, bool VEP, bool EXACT, bool FULL, bool PERSIST>
-__global__ __launch_bounds__(NT, 2) void material_batch(const KParams* P_, const int64_t* bptr_, int nd) {
+template <int DIM, bool VEP, bool EXACT, bool FULL, bool PERSIST, bool LIN = false>
+__global__ __launch_bounds__(NT, LIN && DIM == 2 && !FULL ? 4 : 2) void material_batch(const KParams* P_, const int64_t* bptr_, int nd) {
     KParamsC* P = (KParamsC*)P_;
     I64C* bptr = (I64C*)bptr_;
     if constexpr (!PERSIST) {
         const int64_t b = blockIdx.x;
         const int d = batch_domain(bptr, nd, b);
-        material_body<DIM, VEP, EXACT, false, FULL>((const KParams&)P[d], b - bptr[d]);
+        material_body<DIM, VEP, EXACT, false, FULL, LIN>((const KParams&)P[d], b - bptr[d]);
     } else {
         const int64_t total = bptr[nd];
         for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
             const int d = batch_domain(bptr, nd, b);
             __syncthreads();
-            material_body<DIM, VEP, EXACT, false, FULL>((const KParams&)P[d], b - bptr[d]);
+            material_body<DIM, VEP, EXACT, false, FULL, LIN>((const KParams&)P[d], b - bptr[d]);
         }
     }
 }
@@ -4992,8 +4994,17 @@ hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
     const bool exact = !vep && p.mc.exact_tangent;
     const int sp = p.mc.Split;
     const bool full = !(sp == IFE_SPLIT_NONE || sp == IFE_SPLIT_VOLSTRAINS || sp == IFE_SPLIT_VOLSTRESS);
+    const bool lin = p.mc.Rheology == IFE_RHEO_LE;
     const dim3 g((unsigned)blocks), b(NT);
-    if (vep) {
+    if (lin) {
+        if (full) {
+            if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false, true, true>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((material_kernel<3, false, false, false, true, true>), g, b, 0, s, p);
+        } else {
+            if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false, false, true>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((material_kernel<3, false, false, false, false, true>), g, b, 0, s, p);
+        }
+    } else if (vep) {
         if (dim == 2) hipLaunchKernelGGL((material_kernel<2, true, false, false, true>), g, b, 0, s, p);
         else hipLaunchKernelGGL((material_kernel<3, true, false, false, true>), g, b, 0, s, p);
     } else if (exact) {
@@ -5163,11 +5174,22 @@ hipError_t launch_batch_material(const BatchArgs& a, int64_t grid, hipStream_t s
         if (ps) hipLaunchKernelGGL((material_batch<D, V, X, F, true>), g, b, 0, s, a.P, bp, a.nd);  \
         else hipLaunchKernelGGL((material_batch<D, V, X, F, false>), g, b, 0, s, a.P, bp, a.nd);    \
     } while (0)
+#define IFE_MBL(D, F)                                                                                     \
+    do {                                                                                                  \
+        if (ps) hipLaunchKernelGGL((material_batch<D, false, false, F, true, true>), g, b, 0, s, a.P, bp, a.nd);  \
+        else hipLaunchKernelGGL((material_batch<D, false, false, F, false, true>), g, b, 0, s, a.P, bp, a.nd);    \
+    } while (0)
+    if (a.lin) {
+        if (a.full) { if (a.dim == 2) IFE_MBL(2, true); else IFE_MBL(3, true); }
+        else { if (a.dim == 2) IFE_MBL(2, false); else IFE_MBL(3, false); }
+        return hipGetLastError();
+    }
     if (a.vep) { if (a.dim == 2) IFE_MB(2, true, false, true); else IFE_MB(3, true, false, true); }
     else if (a.exact) { if (a.dim == 2) IFE_MB(2, false, true, true); else IFE_MB(3, false, true, true); }
     else if (a.full) { if (a.dim == 2) IFE_MB(2, false, false, true); else IFE_MB(3, false, false, true); }
     else { if (a.dim == 2) IFE_MB(2, false, false, false); else IFE_MB(3, false, false, false); }
 #undef IFE_MB
+#undef IFE_MBL
     return hipGetLastError();
 }
 
