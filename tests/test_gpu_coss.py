@@ -13,11 +13,16 @@ convergence (stop rule tightened, TIGHT).  With the reference's own stop rule
 (e < 1e-6 or e/e0 < 1e-5 on rows scaled by 1e3 and -1e-6, CosseratMaterial.cpp:553-557) the
 map stops after one Newton step, where the line-search factor is formed from the
 near-zero residual of the new iterate, so the result carries rounding noise of that
-residual: the oracle itself moves by 4e-9 (K) and 9e-7 (plastic strain, material history,
-dissipation) between builds with and without FMA contraction, and the engine differs from
-it by 5e-8 (K) and 2e-5 (history, at the worst IP).  Those cases are held to
-RTOL_REF_RULE on K and f and HIST_TOL_REF_RULE on the history; once the map is converged
-the same cases agree to 1e-12 (the TIGHT cases), which is what pins the implementation."""
+residual.  The test measures that noise: one rounding of the input (the u step's state moved
+by one ulp at random, four draws) moves the oracle's own u-step K by 1.5e-8 to 3.3e-8, its
+phase-step K by ~7e-11 and its history by 1.9e-6 to 2.8e-5 on these cases.  The engine's
+arithmetic differs from the oracle's in the rounding of every intermediate result (its 20x20
+solve is restructured), not in one input, so it is held to 100 times that measured one-rounding
+movement, step by step (and to RTOL_REF_RULE / HIST_TOL_REF_RULE in any case); the test prints
+the ratios (measured: u-step K 0.8 to 2.5, phase-step K 2.7 to 60, T step 0.7 to 6, history
+0.8 to 6).
+Once the map is converged the same cases agree to 1e-12 (the TIGHT cases), which is what pins
+the implementation."""
 import sys
 
 import numpy as np
@@ -35,11 +40,36 @@ HIST = ["PlasticStrain", "MatHist", "ViscDiss", "DamViscDiss", "Energy"]
 TIGHT = [("rm_abs_tol", 1e-40), ("rm_rel_tol", 1e-30), ("rm_max_iterations", 60)]
 RTOL_REF_RULE = 2e-7
 HIST_TOL_REF_RULE = 1e-4
+NOISE_MULT = 100.0  # engine deviation allowed, in units of the oracle's one-rounding movement
 
 
 def _say(msg):
     print(msg, flush=True)
     sys.stdout.flush()
+
+
+def _ulp_sensitivity(prob, trials=4):
+    """How far the oracle's K and f of every step (row / step scaled) and its histories (scaled
+    by their largest entry) move when the u step's state moves by one ulp at random: the
+    rounding floor of the map under the reference's stop rule on this state.  Returns the
+    per-step K/f movement and the history movement."""
+    o = Oracle(prob)
+    ref = [o.assemble(s) for s in range(o.nsteps)]
+    rps = [o.pattern(s)[0] for s in range(o.nsteps)]
+    h0 = {nm: o.history(0, nm) for nm in HIST}
+    wk, wh = [0.0] * o.nsteps, 0.0
+    for t in range(trials):
+        o2 = Oracle(prob)
+        x = o2.state(0, abi.IFE_STATE)
+        x *= 1.0 + np.random.default_rng(t).choice([-1.0, 1.0], x.size) * 2.0 ** -52
+        for s in range(o2.nsteps):
+            K1, f1 = o2.assemble(s)
+            wk[s] = max(wk[s], csr_err(K1, ref[s][0], rps[s]), vec_err(f1, ref[s][1]))
+        for nm in HIST:
+            a, b = o2.history(0, nm), h0[nm]
+            if a is not None:
+                wh = max(wh, np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    return wk, wh
 
 
 def _variant(prob, variant):
@@ -72,14 +102,25 @@ def test_cosserat_against_oracle(require_gpu, name, variant, nel, tight):
         for k, v in TIGHT:
             o.set_param(0, k, v)
             e.set_param(0, k, v)
-    tol = RTOL_FP64 if (tight or variant == "le") else RTOL_REF_RULE
+    ref_rule = not (tight or variant == "le")
+    tols = [RTOL_REF_RULE if ref_rule else RTOL_FP64] * o.nsteps
+    htol = HIST_TOL_REF_RULE if ref_rule else 1e-12
+    sk, sh = [0.0] * o.nsteps, 0.0
+    if ref_rule:
+        sk, sh = _ulp_sensitivity(prob)
+        _say(f"  one-ulp sensitivity of the oracle: K/f per step {', '.join(f'{v:.2e}' for v in sk)}, history {sh:.2e}")
+        assert sk[0] > 0 and sh > 0  # the reference's stop rule leaves rounding noise
+        tols = [min(t, max(RTOL_FP64, NOISE_MULT * v)) for t, v in zip(tols, sk)]
+        htol = min(htol, NOISE_MULT * sh)
     for s in range(o.nsteps):
+        tol = tols[s]
         e.assemble(s)
         Ke, fe = e.Kf(s)
         Ko, fo = o.assemble(s)
         rp, _ = o.pattern(s)
         ek, ef = csr_err(Ke, Ko, rp), vec_err(fe, fo)
-        _say(f"  step {s}: K err {ek:.2e}, f err {ef:.2e}")
+        ratio = f" ({max(ek, ef) / (sk[s] if sk[s] > 0 else 1e-300):.1f} x the one-rounding movement)" if ref_rule else ""
+        _say(f"  step {s}: K err {ek:.2e}, f err {ef:.2e}{ratio}")
         assert ek <= tol and ef <= tol, f"step {s}"
         if s == 0:
             so, se = o.stats(), e.status()
@@ -97,8 +138,8 @@ def test_cosserat_against_oracle(require_gpu, name, variant, nel, tight):
         he = e.history(0, nm)[:ho.shape[0]]
         sc = max(np.max(np.abs(ho)), 1e-300)
         err = np.max(np.abs(he - ho)) / sc
-        _say(f"  history {nm}: {err:.2e}")
-        assert err <= (1e-12 if tol == RTOL_FP64 else HIST_TOL_REF_RULE), nm
+        _say(f"  history {nm}: {err:.2e}" + (f" ({err / sh:.1f} x)" if ref_rule else ""))
+        assert err <= htol, nm
     assert o.history(0, "PlasticStrain").shape[0] == 18
 
 
