@@ -863,6 +863,7 @@ struct ife_batch {
     DevBuf<KParams> dP;
     DevBuf<int64_t> bp[3];      // block prefix sums: material, element, gather kernels
     int64_t tot[3] = {0, 0, 0};
+    int64_t nodes = 0;          // element-group nodes over all domains
     int64_t grid[3] = {0, 0, 0};  // workgroups per launch (0: one per block)
     int nta = 256;  // lanes per element of the element kernel (64 in 2D: one wavefront)
     // 2D element kernel: elements per wavefront (1: assemble2d_wave; 4, 8, 16: assemble2d_mfma,
@@ -1843,7 +1844,9 @@ int ife_batch_create(ife_physics* const* domains, int32_t n_domains, ife_batch**
             b->doms.push_back(ph);
             pre[0].push_back(pre[0].back() + blocks_material(ph->dim, ph->NE));
             pre[1].push_back(pre[1].back() + ph->NE);
-            pre[2].push_back(pre[2].back() + ph->Ng);
+            const int npw = gather_nodes_per_wave(ph->dim);
+            pre[2].push_back(pre[2].back() + (ph->Ng + npw - 1) / npw);
+            b->nodes += ph->Ng;
         }
         for (int k = 0; k < 3; k++) {
             b->bp[k].upload(pre[k].data(), pre[k].size());
@@ -1936,7 +1939,7 @@ int ife_batch_get_shape(const ife_batch* b, ife_batch_tuning* out) {
         out->n_domains = (int32_t)b->doms.size();
         out->blocks_material = b->tot[0];
         out->elements = b->tot[1];
-        out->nodes = b->tot[2];
+        out->nodes = b->nodes;
     });
 }
 

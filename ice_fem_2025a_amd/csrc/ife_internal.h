@@ -236,6 +236,7 @@ hipError_t launch_batch_material(const BatchArgs& a, int64_t grid, hipStream_t s
 hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipStream_t s);
 hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s);
 int blocks_material(int dim, int64_t n_elems);
+int gather_nodes_per_wave(int dim);  // element-group nodes per gather wavefront
 
 // ---- line search (ife_linesearch.hip): NonLinSolver::HOLineSearch on the device ----------
 constexpr int LS_MAX = 10;   // maxLS (NonLinSolver.cpp:313)

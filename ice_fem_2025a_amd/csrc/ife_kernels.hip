@@ -3202,11 +3202,13 @@ constexpr int GW = 64;      // lanes per gather workgroup (one wavefront)
 // 64 in 2D (25); the host rejects meshes above these
 template <int DIM>
 constexpr int gather_maxdeg() { return DIM == 2 ? 64 : 128; }
-// records loaded ahead: 9, so a node's records come in whole batches (2D: up to 9 elements
-// per node, one batch; 3D: up to 27, three).  Measured on the 40x40x80 hex block against 3, 4,
-// 6, 8 and 27 ahead: 9 was fastest in both precisions (27 costs occupancy).
+// records loaded ahead (all their loads in flight before the first LDS add).  Measured on the
+// 40x40x80 hex block (3D: 3, 4, 6, 9, 12, 16 ahead) and the 64-domain quad batch (2D: 3, 4, 6,
+// 9, 12, 16): 6 in 3D in both precisions, 9 in 2D (a 2D node has at most 4 elements; the
+// loads past a node's count re-read its last record from cache and keep the half-wavefronts'
+// loads aligned)
 template <int DIM, typename T>
-constexpr int gather_gpf() { return 9; }
+constexpr int gather_gpf() { return DIM == 2 ? 9 : 6; }
 
 // W consecutive record entries per lane load: one FP64 entry (8 B), or two FP32 entries
 // (8 B) of the mixed mode's records, so that a load moves the same bytes in both modes (the
@@ -3230,25 +3232,48 @@ struct RecLoad<float> {
     static __device__ __forceinline__ double get(V x, int w) { return (double)(w == 0 ? x.x : x.y); }
 };
 
+// Nodes per gather wavefront: 3D one (a record of 304 entries keeps 64 lanes loading); 2D two,
+// one per half-wavefront (a record of 60 entries is 30 FP32-pair or 60 FP64 loads), so each
+// wavefront runs two nodes' dependent loads (element list, then records) side by side.
+template <int DIM>
+constexpr int gather_npw() { return DIM == 2 ? 2 : 1; }
+
 // T: element record type (double, or float in mixed mode); the rows accumulate in FP64.
+// blk: the wavefront's block of gather_npw nodes (group ranks blk * NPW + h, h = half).
 template <int DIM, typename T>
 __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk) {
     using RC = Rec<DIM>;
     typedef typename RecLoad<T>::V V;
-    constexpr int N = RC::N, W = RecLoad<T>::W, PER = (RC::SIZE + GW * W - 1) / (GW * W);
+    constexpr int NPW = gather_npw<DIM>(), HW = GW / NPW;  // lanes per node
+    constexpr int N = RC::N, W = RecLoad<T>::W, PER = (RC::SIZE + HW * W - 1) / (HW * W);
     constexpr int MAXDEG = gather_maxdeg<DIM>(), GPF = gather_gpf<DIM, T>();
     static_assert(RC::SIZE % W == 0, "record length must be a whole number of loads");
-    // LDS row accumulators: K_UU [DIM*DIM][MAXDEG], K_pp [MAXDEG], K_TT [MAXDEG], f_U, f_p, f_T,
-    // the node's lumped weight
+    static_assert(N <= HW, "a node's column-slot row must fit its lanes");
+    // LDS row accumulators per node: K_UU [DIM*DIM][MAXDEG], K_pp [MAXDEG], K_TT [MAXDEG], f_U,
+    // f_p, f_T, the node's lumped weight
     constexpr int OKP = DIM * DIM * MAXDEG, OKT = OKP + MAXDEG, OF = OKT + MAXDEG, OWL = OF + DIM + 2, LEN = OWL + 1;
-    __shared__ double acc[LEN];
+    __shared__ double accs[NPW][LEN];
     const int lane = threadIdx.x;
-    const int64_t r = blk;
-    const int64_t r0 = p.nbr_ptr[r];
-    const int deg = (int)(p.nbr_ptr[r + 1] - r0);
+    // node (half) of this lane and lane within it; with one node per wavefront every node
+    // quantity below stays wavefront-uniform (scalar registers and address arithmetic)
+    const int h = NPW == 1 ? 0 : lane / HW, hl = NPW == 1 ? lane : lane % HW, hb = h * HW;
+    double* const acc = accs[h];
+    const int64_t r = blk * NPW + h;
+    const bool live = NPW == 1 || r < p.Ng;  // one node per wavefront: the grid is the node count
+    const int64_t rs = live ? r : 0;  // a valid rank for the loads of a node past the end
+    const int64_t r0 = p.nbr_ptr[rs];
+    const int deg = live ? (int)(p.nbr_ptr[rs + 1] - r0) : 0;
     const bool du = p.has_pf && p.do_u, dph = p.has_pf && p.do_ph, dT = p.do_Tpf || p.do_th;
-    for (int i = lane; i < LEN; i += GW) acc[i] = 0.0;
-    // per lane: record entries t = (lane + GW u) W + w -> LDS base slot, index into the
+    // the element list's bounds and its first HW entries are loaded before the LDS rows are
+    // cleared, so their round trips overlap the clearing and the barrier
+    const int64_t k0 = live ? p.n2e_ptr[rs] : 0, k1 = live ? p.n2e_ptr[rs + 1] : 0;
+    const int64_t cnt = k1 - k0;  // this node's elements (at least one for a live node)
+    // list loads are branch-free: lanes past the count re-read entry 0 of the chunk (a value
+    // nothing reads), so no load waits inside a predicated block
+    int my_e = p.n2e_elem[k0 + (hl < cnt ? hl : 0)];
+    int my_a = p.n2e_loc[k0 + (hl < cnt ? hl : 0)];
+    for (int i = hl; i < LEN; i += HW) acc[i] = 0.0;
+    // per lane: record entries t = (hl + HW u) W + w -> LDS base slot, index into the
     // record's column-slot row (-1: fixed slot), active flag
     int base[PER][W], lb[PER][W];
     bool act[PER][W], ld[PER];
@@ -3257,7 +3282,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
         ld[u] = false;
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            const int t = (lane + u * GW) * W + w;
+            const int t = (hl + u * HW) * W + w;
             int bs = 0, l = -1;
             bool ac = false;
             if (t < RC::FU) { bs = (t / N) * MAXDEG; l = t % N; ac = du; }
@@ -3272,27 +3297,44 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
         }
     }
     __syncthreads();
-    const int64_t k0 = p.n2e_ptr[r], k1 = p.n2e_ptr[r + 1];
-    // the node's (element, local index) list, GW entries at a time, held one per lane and
-    // read back with readlane: the record loads of a batch need no other memory round trip
-    for (int64_t c0 = k0; c0 < k1; c0 += GW) {
-        const int nc = (int)(k1 - c0 < GW ? k1 - c0 : GW);
-        const int my_e = lane < nc ? p.n2e_elem[c0 + lane] : 0;
-        const int my_a = lane < nc ? (int)p.n2e_loc[c0 + lane] : 0;
-        for (int kb = 0; kb < nc; kb += GPF) {
+    int64_t cmax = cnt;  // the wavefront's largest count (uniform)
+    if constexpr (NPW == 2) cmax = max((int64_t)__shfl(cnt, 0), (int64_t)__shfl(cnt, HW));
+    // the node's (element, local index) list, HW entries at a time, held one per lane of its
+    // half and read back by lane index: the record loads of a batch need no other round trip
+    for (int64_t c0 = 0; c0 < cmax; c0 += HW) {
+        const int nc = (int)(cnt - c0 < 0 ? 0 : (cnt - c0 < HW ? cnt - c0 : HW));
+        int ncmax = nc;
+        if constexpr (NPW == 2) ncmax = max(__shfl(nc, 0), __shfl(nc, HW));
+        if (c0 > 0) {
+            const int64_t ki = k0 + (c0 < cnt ? c0 + (hl < nc ? hl : 0) : 0);
+            my_e = p.n2e_elem[ki];
+            my_a = p.n2e_loc[ki];
+        }
+        for (int kb = 0; kb < ncmax; kb += GPF) {
             V x[GPF][PER];
             int lcv[GPF];
 #pragma unroll
             for (int q = 0; q < GPF; q++) {
-                const int k = kb + q < nc ? kb + q : nc - 1;
-                const int64_t e = __builtin_amdgcn_readlane(my_e, k);
-                const int a = __builtin_amdgcn_readlane(my_a, k);
+                // past the end of a node's list the last record is loaded again (a cache hit
+                // on the lines just requested; nothing adds it): branch-free loads
+                const int k = kb + q < nc ? kb + q : (nc > 0 ? nc - 1 : 0);
+                int64_t e;
+                int a;
+                if constexpr (NPW == 1) {
+                    e = __builtin_amdgcn_readlane(my_e, k);
+                    a = __builtin_amdgcn_readlane(my_a, k);
+                } else {
+                    e = __shfl(my_e, hb + k);
+                    a = __shfl(my_a, hb + k);
+                }
                 const V* rec = (const V*)((const T*)p.ebuf + (e * N + a) * RC::SIZE);
-                // the record's column-slot row: one byte per lane < N, read across lanes below
-                lcv[q] = lane < N ? (int)p.loc[(e * N + a) * N + lane] : 0;
+                // the record's column-slot row: one byte per lane < N, read across lanes below;
+                // lanes from N on re-read byte N - 1 so that the load is unpredicated (a load in
+                // a predicated block is waited for at once, which drains the records in flight)
+                lcv[q] = p.loc[(e * N + a) * N + (hl < N ? hl : N - 1)];
 #pragma unroll
                 for (int u = 0; u < PER; u++)  // loads stay inside the record
-                    x[q][u] = ld[u] && lane + u * GW < RC::SIZE / W ? rec[lane + u * GW] : RecLoad<T>::zero();
+                    x[q][u] = ld[u] && hl + u * HW < RC::SIZE / W ? rec[hl + u * HW] : RecLoad<T>::zero();
             }
 #pragma unroll
             for (int q = 0; q < GPF; q++) {
@@ -3301,7 +3343,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
                     for (int u = 0; u < PER; u++)
 #pragma unroll
                         for (int w = 0; w < W; w++) {
-                            const int sl = __shfl(lcv[q], lb[u][w] >= 0 ? lb[u][w] : 0);
+                            const int sl = __shfl(lcv[q], hb + (lb[u][w] >= 0 ? lb[u][w] : 0));
                             if (act[u][w])  // ds_add_f64, no return
                                 atomicAdd(&acc[base[u][w] + (lb[u][w] >= 0 ? sl : 0)], RecLoad<T>::get(x[q][u], w));
                         }
@@ -3310,13 +3352,14 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
         }
     }
     __syncthreads();
+    if (!live) return;
     if (du) {
         for (int al = 0; al < DIM; al++) {
             double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * r0;
             for (int be = 0; be < DIM; be++)
-                for (int i = lane; i < deg; i += GW) K[(int64_t)p.fpos_u[be] * deg + i] = acc[(al * DIM + be) * MAXDEG + i];
+                for (int i = hl; i < deg; i += HW) K[(int64_t)p.fpos_u[be] * deg + i] = acc[(al * DIM + be) * MAXDEG + i];
         }
-        if (lane < DIM) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = acc[OF + lane];
+        if (hl < DIM) p.su.f[(int64_t)p.fpos_u[hl] * p.Ng + r] = acc[OF + hl];
     }
     if (dph) {
         // multiplier irreversibility: diagonal entries of the (phi, L, L2) blocks of this
@@ -3341,18 +3384,18 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
                 const int cb = p.ph_col[ri][cj];
                 if (cb < 0) continue;
                 double* const Kb = K + (int64_t)cb * deg;
-                for (int i = lane; i < deg; i += GW)
+                for (int i = hl; i < deg; i += HW)
                     Kb[i] = (ri == 0 && cj == 0 ? acc[OKP + i] : 0.0) + (i == self ? dg[ri][cj] : 0.0);
             }
-            if (lane == 0) p.sph.f[(int64_t)p.ph_fpos[ri] * p.Ng + r] = fr[ri];
+            if (hl == 0) p.sph.f[(int64_t)p.ph_fpos[ri] * p.Ng + r] = fr[ri];
         }
     }
     if (dT) {
         if (p.do_th) {
             double* K = p.sT.K + (int64_t)p.fpos_Tth * p.sT.M + (int64_t)p.sT.nf * r0 + (int64_t)p.fpos_Tth * deg;
-            for (int i = lane; i < deg; i += GW) K[i] = acc[OKT + i];
+            for (int i = hl; i < deg; i += HW) K[i] = acc[OKT + i];
         }
-        if (lane == 0) p.sT.f[(int64_t)p.fpos_Tth * p.Ng + r] = acc[OF + DIM + 1];
+        if (hl == 0) p.sT.f[(int64_t)p.fpos_Tth * p.Ng + r] = acc[OF + DIM + 1];
     }
 }
 
@@ -4845,8 +4888,10 @@ __global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, 
     __syncthreads();
     for (int64_t c0 = k0; c0 < k1; c0 += GW) {
         const int nc = (int)(k1 - c0 < GW ? k1 - c0 : GW);
-        const int64_t my_e = lane < nc ? p.n2e_elem[c0 + lane] : 0;
-        const int my_a = lane < nc ? (int)p.n2e_loc[c0 + lane] : 0;
+        // branch-free list loads (lanes past the count re-read the chunk's first entry, which
+        // nothing reads): a load in a predicated block is waited for at once
+        const int64_t my_e = p.n2e_elem[c0 + (lane < nc ? lane : 0)];
+        const int my_a = p.n2e_loc[c0 + (lane < nc ? lane : 0)];
         for (int kb = 0; kb < nc; kb += GPF) {
             V x[GPF][PER];
             int lcv[GPF];
@@ -4859,7 +4904,9 @@ __global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, 
                 use[q] = kb + q < nc && e >= e0 && e < e1;
                 const int64_t ec = use[q] ? e : e0;  // a record inside the chunk's buffer
                 const V* rec = (const V*)((const T*)p.crec + ((ec - p.crec_e0) * N + a) * CR::SIZE);
-                lcv[q] = lane < N ? (int)p.loc[(e * N + a) * N + lane] : 0;
+                // column-slot row, unpredicated (lanes from N on re-read byte N - 1), so that the
+                // records' loads stay in flight together
+                lcv[q] = p.loc[(e * N + a) * N + (lane < N ? lane : N - 1)];
 #pragma unroll
                 for (int u = 0; u < PER; u++)
                     x[q][u] = use[q] && lane + u * GW < CR::SIZE / W ? rec[lane + u * GW] : RecLoad<T>::zero();
@@ -4927,7 +4974,8 @@ hipError_t launch_ho_phase(const KParams& p, int dim, hipStream_t s) {
 
 hipError_t launch_gather(const KParams& p, int dim, hipStream_t s) {
     if (p.Ng <= 0) return hipSuccess;
-    const dim3 grid((unsigned)p.Ng), block(GW);
+    const int npw = gather_nodes_per_wave(dim);
+    const dim3 grid((unsigned)((p.Ng + npw - 1) / npw)), block(GW);
     if (p.mixed) {
         if (dim == 2) hipLaunchKernelGGL((gather_kernel<2, float>), grid, block, 0, s, p);
         else hipLaunchKernelGGL((gather_kernel<3, float>), grid, block, 0, s, p);
@@ -5147,6 +5195,8 @@ hipError_t launch_zero_status(Status* st, hipStream_t s, const int* gate) {
     hipLaunchKernelGGL(zero_status_kernel, dim3(1), dim3(64), 0, s, st, gate);
     return hipGetLastError();
 }
+
+int gather_nodes_per_wave(int dim) { return dim == 2 ? gather_npw<2>() : gather_npw<3>(); }
 
 int blocks_material(int dim, int64_t n_elems) {
     const int epb = NT / (dim == 2 ? 9 : 27);
