@@ -1122,11 +1122,15 @@ struct MatSmem {
 // FULL: every energy split; false: None / VolStrains / VolStress only.
 // LIN: the linear-elastic rheology only (no return map is compiled in, so the kernel needs far
 // fewer registers: the 2D form without a split runs four workgroups per CU instead of two).
+// OLD: the explicit Glen rheologies may run (VE_EXPL, DVE_EXPL), which read the previous
+// step's strain and phase.  The implicit ones (VE, DVE) and LIN read neither, so OLD = false
+// drops the old displacement and phase fields from the staging and the per-node sums.
 // blk: the workgroup's block of EPB elements within the domain p.
-template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL, bool LIN = false>
+template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL, bool LIN = false, bool OLD = true>
 __device__ __forceinline__ void material_body(const KParams& p, const int64_t blk) {
     using E = ET<DIM>;
     constexpr int N = E::N, NIP = E::NIP, EPB = NT / NIP;
+    constexpr bool NOLD = VEP || ADJ || (!LIN && OLD);  // old-step fields needed
     __shared__ MatSmem<DIM, ADJ> sm;
     const int tid = threadIdx.x;
     const MatConst& mc = p.mc;
@@ -1145,11 +1149,11 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
             for (int d = 0; d < DIM; d++) {
                 int64_t dd = (int64_t)p.fpos_u[d] * p.Ng + r;
                 sm.U[le][d][a] = p.su.x[dd];
-                sm.Uo[le][d][a] = p.su.xold[dd];
+                if constexpr (NOLD) sm.Uo[le][d][a] = p.su.xold[dd];
             }
             int64_t dp = (int64_t)p.fpos_ph * p.Ng + r;
             sm.Ph[le][a] = p.sph.x[dp];
-            sm.PhO[le][a] = p.sph.xold[dp];
+            if constexpr (NOLD) sm.PhO[le][a] = p.sph.xold[dp];
         }
         if (p.has_T) sm.T[le][a] = p.sT.x[(int64_t)p.fpos_T * p.Ng + r];
         if constexpr (ADJ) {
@@ -1222,15 +1226,17 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
             fv[F_ONE] = 1.0;
             for (int d = 0; d < DIM; d++) fv[F_X + d] = sm.X[le][a][d];
             fv[F_PH] = sm.Ph[le][a];
-            fv[F_PHO] = sm.PhO[le][a];
+            fv[F_PHO] = NOLD ? sm.PhO[le][a] : 0.0;
             for (int d = 0; d < DIM; d++) {
                 fv[F_U + d] = sm.U[le][d][a];
-                fv[F_UO + d] = sm.Uo[le][d][a];
+                fv[F_UO + d] = NOLD ? sm.Uo[le][d][a] : 0.0;
             }
             fv[F_T] = sm.T[le][a];
             if constexpr (ADJ)
                 for (int k = 0; k < DIM + 2; k++) fv[F_L + k] = sm.L[le][k][a];
+#pragma unroll
             for (int k = 0; k < NFLD; k++) {
+                if (!NOLD && (k == F_PHO || (k >= F_UO && k < F_UO + DIM))) continue;  // sums stay 0
                 SC[k] += wc * fv[k];
                 for (int c = 0; c < DIM; c++) SG[c][k] += wg[c] * fv[k];
             }
@@ -1310,11 +1316,11 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
             for (int c = 0; c < DIM; c++) { gu[al][c] = 0; guo[al][c] = 0; }
         if (p.has_pf) {
             phi = val(F_PH);
-            phio = val(F_PHO);
+            if constexpr (NOLD) phio = val(F_PHO);
             grad(F_PH, gphi);
             for (int al = 0; al < DIM; al++) {
                 grad(F_U + al, gu[al]);
-                grad(F_UO + al, guo[al]);
+                if constexpr (NOLD) grad(F_UO + al, guo[al]);
             }
         }
         if (p.has_T) {
@@ -1404,7 +1410,7 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
                             for (int j = 0; j < 6; j++) DM[i][j] = B[i][j];
                     }
                 }
-            } else if (!LIN && (rh == IFE_RHEO_VE_EXPL || rh == IFE_RHEO_DVE_EXPL)) {
+            } else if (!LIN && NOLD && (rh == IFE_RHEO_VE_EXPL || rh == IFE_RHEO_DVE_EXPL)) {
                 if (!(phio > mc.PhiViscLim)) {
                     double dmg = 1.0, u1, u2;
                     if (rh == IFE_RHEO_DVE_EXPL) damage(mc.DamFunction, phio, mc.k0, mc.p, dmg, u1, u2);
@@ -1650,10 +1656,10 @@ __device__ __forceinline__ int batch_domain(I64C* bptr, int nd, int64_t b) {
     return lo;
 }
 
-template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL, bool LIN = false>
+template <int DIM, bool VEP, bool EXACT, bool ADJ, bool FULL, bool LIN = false, bool OLD = true>
 __global__ __launch_bounds__(NT, LIN && DIM == 2 && !FULL ? 4 : 2) void material_kernel(KParams p) {
     if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
-    material_body<DIM, VEP, EXACT, ADJ, FULL, LIN>(p, blockIdx.x);
+    material_body<DIM, VEP, EXACT, ADJ, FULL, LIN, OLD>(p, blockIdx.x);
 }
 
 // Batched form: the blocks of all domains in one launch.  PERSIST = false: one workgroup per
@@ -5043,6 +5049,7 @@ hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
     const int sp = p.mc.Split;
     const bool full = !(sp == IFE_SPLIT_NONE || sp == IFE_SPLIT_VOLSTRAINS || sp == IFE_SPLIT_VOLSTRESS);
     const bool lin = p.mc.Rheology == IFE_RHEO_LE;
+    const bool expl = p.mc.Rheology == IFE_RHEO_VE_EXPL || p.mc.Rheology == IFE_RHEO_DVE_EXPL;
     const dim3 g((unsigned)blocks), b(NT);
     if (lin) {
         if (full) {
@@ -5055,9 +5062,17 @@ hipError_t launch_material(const KParams& p, int dim, hipStream_t s) {
     } else if (vep) {
         if (dim == 2) hipLaunchKernelGGL((material_kernel<2, true, false, false, true>), g, b, 0, s, p);
         else hipLaunchKernelGGL((material_kernel<3, true, false, false, true>), g, b, 0, s, p);
-    } else if (exact) {
-        if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, true, false, true>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((material_kernel<3, false, true, false, true>), g, b, 0, s, p);
+    } else if (!expl) {  // implicit Glen rheologies: no old-step fields
+        if (exact) {
+            if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, true, false, true, false, false>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((material_kernel<3, false, true, false, true, false, false>), g, b, 0, s, p);
+        } else if (full) {
+            if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false, true, false, false>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((material_kernel<3, false, false, false, true, false, false>), g, b, 0, s, p);
+        } else {
+            if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false, false, false, false>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((material_kernel<3, false, false, false, false, false, false>), g, b, 0, s, p);
+        }
     } else if (full) {
         if (dim == 2) hipLaunchKernelGGL((material_kernel<2, false, false, false, true>), g, b, 0, s, p);
         else hipLaunchKernelGGL((material_kernel<3, false, false, false, true>), g, b, 0, s, p);
