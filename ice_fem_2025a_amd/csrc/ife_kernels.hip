@@ -4845,21 +4845,32 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     if (bad != 0.0) atomicOr(&p.st->nonfinite, 1u);
 }
 
+// Pairs (alpha, beta) of a Cosserat node's row blocks accumulated per pass of the u-step
+// gather: 3D 12 of 36 (three passes, 12.3 kB of LDS rows instead of 36.9 kB, so that more
+// wavefronts share a CU), 2D all 9 (one pass, 4.6 kB).
+template <int DIM>
+constexpr int coss_gather_pairs() { return DIM == 2 ? 9 : 12; }
+
 // Cosserat u-step gather: one wavefront per node (group rank), as gather_body for the solid
 // model, over the node's elements in [e0, e1) only: their records are added in increasing
 // element index into LDS copies of the node's NFU rows (NFU x NFU column blocks of deg
 // slots) and residuals, which start from the rows' current values when the node has
 // elements before e0 (the partial sums of the earlier chunks, so every entry is the sum of
 // its element contributions in element order) and from zero otherwise; each row is written
-// once, contiguously.
+// once, contiguously.  The column blocks go in passes of coss_gather_pairs blocks, each pass
+// reading its contiguous slice of every record (the residuals with the last pass), so each
+// record entry is still read once.
 template <int DIM, typename T>
 __global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, int64_t e1) {
     if (p.gate && !*p.gate) return;
     using CR = CRec<DIM>;
     typedef typename RecLoad<T>::V V;
     constexpr int N = CR::N, NFU = CR::NFU, NPAIR = NFU * NFU, W = RecLoad<T>::W;
-    constexpr int PER = (CR::SIZE + GW * W - 1) / (GW * W), MAXDEG = gather_maxdeg<DIM>(), GPF = 4;
-    constexpr int OF = NPAIR * MAXDEG, LEN = OF + NFU;
+    constexpr int PG = coss_gather_pairs<DIM>(), NPASS = NPAIR / PG;
+    static_assert(NPAIR % PG == 0 && (NPASS == 1 || (PG * N) % W == 0), "passes must split the record at whole loads");
+    constexpr int PLEN = PG * N + NFU;  // entries of the last pass (the others: PG * N)
+    constexpr int PER = (PLEN + GW * W - 1) / (GW * W), MAXDEG = gather_maxdeg<DIM>(), GPF = 4;
+    constexpr int OF = PG * MAXDEG, LEN = OF + NFU;
     __shared__ double acc[LEN];
     const int lane = threadIdx.x;
     const int64_t r = blockIdx.x;
@@ -4870,73 +4881,78 @@ __global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, 
     const int64_t r0 = p.nbr_ptr[r];
     const int deg = (int)(p.nbr_ptr[r + 1] - r0);
     const bool cont = efirst < e0;
-    auto Krow = [&](int al, int be) {
-        return p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * r0 + (int64_t)p.fpos_u[be] * deg;
+    auto Krow = [&](int q) {  // the row of column block q = alpha NFU + beta
+        return p.su.K + (int64_t)p.fpos_u[q / NFU] * p.su.M + (int64_t)p.su.nf * r0 + (int64_t)p.fpos_u[q % NFU] * deg;
     };
-    for (int q = 0; q < NPAIR; q++) {
-        const double* K = Krow(q / NFU, q % NFU);
-        for (int i = lane; i < MAXDEG; i += GW) acc[q * MAXDEG + i] = (cont && i < deg) ? K[i] : 0.0;
-    }
-    if (lane < NFU) acc[OF + lane] = cont ? p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] : 0.0;
-    // per lane: record entries t = (lane + GW u) W + w -> LDS base slot and column index (-1:
-    // a residual)
-    int base[PER][W], lb[PER][W];
-    bool act[PER][W];
-#pragma unroll
-    for (int u = 0; u < PER; u++)
-#pragma unroll
-        for (int w = 0; w < W; w++) {
-            const int t = (lane + u * GW) * W + w;
-            act[u][w] = t < CR::SIZE;
-            base[u][w] = t < CR::F ? (t / N) * MAXDEG : OF + (t - CR::F);
-            lb[u][w] = t < CR::F ? t % N : -1;
+    for (int g = 0; g < NPASS; g++) {
+        const bool last = g == NPASS - 1;
+        const int t0 = g * PG * N, nent = PG * N + (last ? NFU : 0);  // the pass's record slice
+        for (int q = 0; q < PG; q++) {
+            const double* K = Krow(g * PG + q);
+            for (int i = lane; i < MAXDEG; i += GW) acc[q * MAXDEG + i] = (cont && i < deg) ? K[i] : 0.0;
         }
-    __syncthreads();
-    for (int64_t c0 = k0; c0 < k1; c0 += GW) {
-        const int nc = (int)(k1 - c0 < GW ? k1 - c0 : GW);
-        // branch-free list loads (lanes past the count re-read the chunk's first entry, which
-        // nothing reads): a load in a predicated block is waited for at once
-        const int64_t my_e = p.n2e_elem[c0 + (lane < nc ? lane : 0)];
-        const int my_a = p.n2e_loc[c0 + (lane < nc ? lane : 0)];
-        for (int kb = 0; kb < nc; kb += GPF) {
-            V x[GPF][PER];
-            int lcv[GPF];
-            bool use[GPF];
+        if (last && lane < NFU) acc[OF + lane] = cont ? p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] : 0.0;
+        // per lane: slice entries tt = (lane + GW u) W + w -> LDS base slot and column index
+        // (-1: a residual)
+        int base[PER][W], lb[PER][W];
+        bool act[PER][W];
 #pragma unroll
-            for (int q = 0; q < GPF; q++) {
-                const int k = kb + q < nc ? kb + q : nc - 1;
-                const int64_t e = __builtin_amdgcn_readlane((int)my_e, k);
-                const int a = __builtin_amdgcn_readlane(my_a, k);
-                use[q] = kb + q < nc && e >= e0 && e < e1;
-                const int64_t ec = use[q] ? e : e0;  // a record inside the chunk's buffer
-                const V* rec = (const V*)((const T*)p.crec + ((ec - p.crec_e0) * N + a) * CR::SIZE);
-                // column-slot row, unpredicated (lanes from N on re-read byte N - 1), so that the
-                // records' loads stay in flight together
-                lcv[q] = p.loc[(e * N + a) * N + (lane < N ? lane : N - 1)];
+        for (int u = 0; u < PER; u++)
 #pragma unroll
-                for (int u = 0; u < PER; u++)
-                    x[q][u] = use[q] && lane + u * GW < CR::SIZE / W ? rec[lane + u * GW] : RecLoad<T>::zero();
+            for (int w = 0; w < W; w++) {
+                const int tt = (lane + u * GW) * W + w;
+                act[u][w] = tt < nent;
+                base[u][w] = tt < PG * N ? (tt / N) * MAXDEG : OF + (tt - PG * N);
+                lb[u][w] = tt < PG * N ? tt % N : -1;
             }
+        __syncthreads();
+        for (int64_t c0 = k0; c0 < k1; c0 += GW) {
+            const int nc = (int)(k1 - c0 < GW ? k1 - c0 : GW);
+            // branch-free list loads (lanes past the count re-read the chunk's first entry,
+            // which nothing reads): a load in a predicated block is waited for at once
+            const int64_t my_e = p.n2e_elem[c0 + (lane < nc ? lane : 0)];
+            const int my_a = p.n2e_loc[c0 + (lane < nc ? lane : 0)];
+            for (int kb = 0; kb < nc; kb += GPF) {
+                V x[GPF][PER];
+                int lcv[GPF];
+                bool use[GPF];
 #pragma unroll
-            for (int q = 0; q < GPF; q++) {
-                if (use[q]) {
+                for (int q = 0; q < GPF; q++) {
+                    const int k = kb + q < nc ? kb + q : nc - 1;
+                    const int64_t e = __builtin_amdgcn_readlane((int)my_e, k);
+                    const int a = __builtin_amdgcn_readlane(my_a, k);
+                    use[q] = kb + q < nc && e >= e0 && e < e1;
+                    const int64_t ec = use[q] ? e : e0;  // a record inside the chunk's buffer
+                    const V* rec = (const V*)((const T*)p.crec + ((ec - p.crec_e0) * N + a) * CR::SIZE + t0);
+                    // column-slot row, unpredicated (lanes from N on re-read byte N - 1), so that
+                    // the records' loads stay in flight together
+                    lcv[q] = p.loc[(e * N + a) * N + (lane < N ? lane : N - 1)];
 #pragma unroll
                     for (int u = 0; u < PER; u++)
+                        x[q][u] = use[q] && (lane + u * GW) * W < nent ? rec[lane + u * GW] : RecLoad<T>::zero();
+                }
 #pragma unroll
-                        for (int w = 0; w < W; w++) {
-                            const int sl = __shfl(lcv[q], lb[u][w] >= 0 ? lb[u][w] : 0);
-                            if (act[u][w]) atomicAdd(&acc[base[u][w] + (lb[u][w] >= 0 ? sl : 0)], RecLoad<T>::get(x[q][u], w));
-                        }
+                for (int q = 0; q < GPF; q++) {
+                    if (use[q]) {
+#pragma unroll
+                        for (int u = 0; u < PER; u++)
+#pragma unroll
+                            for (int w = 0; w < W; w++) {
+                                const int sl = __shfl(lcv[q], lb[u][w] >= 0 ? lb[u][w] : 0);
+                                if (act[u][w]) atomicAdd(&acc[base[u][w] + (lb[u][w] >= 0 ? sl : 0)], RecLoad<T>::get(x[q][u], w));
+                            }
+                    }
                 }
             }
         }
+        __syncthreads();
+        for (int q = 0; q < PG; q++) {
+            double* K = Krow(g * PG + q);
+            for (int i = lane; i < deg; i += GW) K[i] = acc[q * MAXDEG + i];
+        }
+        if (last && lane < NFU) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = acc[OF + lane];
+        __syncthreads();  // the next pass clears the rows
     }
-    __syncthreads();
-    for (int q = 0; q < NPAIR; q++) {
-        double* K = Krow(q / NFU, q % NFU);
-        for (int i = lane; i < deg; i += GW) K[i] = acc[q * MAXDEG + i];
-    }
-    if (lane < NFU) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = acc[OF + lane];
 }
 
 }  // namespace
