@@ -3209,10 +3209,9 @@ constexpr int GW = 64;      // lanes per gather workgroup (one wavefront)
 template <int DIM>
 constexpr int gather_maxdeg() { return DIM == 2 ? 64 : 128; }
 // records loaded ahead (all their loads in flight before the first LDS add).  Measured on the
-// 40x40x80 hex block (3D: 3, 4, 6, 9, 12, 16 ahead) and the 64-domain quad batch (2D: 3, 4, 6,
-// 9, 12, 16): 6 in 3D in both precisions, 9 in 2D (a 2D node has at most 4 elements; the
-// loads past a node's count re-read its last record from cache and keep the half-wavefronts'
-// loads aligned)
+// 40x40x80 hex block (3D: 3, 4, 6, 9, 12, 16 ahead; a node has up to 27 elements) and the
+// 64-domain quad batch (2D: 3, 4, 6, 9, 12, 16; up to 9 elements, so one batch): 6 in 3D in
+// both precisions, 9 in 2D
 template <int DIM, typename T>
 constexpr int gather_gpf() { return DIM == 2 ? 9 : 6; }
 
