@@ -1214,31 +1214,55 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
             SC[k] = 0;
             for (int c = 0; c < DIM; c++) SG[c][k] = 0;
         }
+        // Nodes in x-fastest order as an (az, ay) loop around three unrolled ax nodes: the
+        // x factors are indexed by constants and the y, z factors picked once per row by a
+        // wavefront-uniform select (a run-time index into b1 / g1 reads through indirect
+        // register moves; unrolling all 27 nodes spills).  Products and sums keep the order
+        // of cbgb.
+        auto pick = [](const double v[3], int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); };
 #pragma unroll 1
-        for (int a = 0; a < N; a++) {
-            double cb, gb[DIM];
-            cbgb(a, cb, gb);
-            const double Wa = DIM == 3 ? sm.X[le][a][3] : 1.0;
-            const double wc = Wa * cb;
-            double wg[DIM];
-            for (int c = 0; c < DIM; c++) wg[c] = Wa * gb[c];
-            double fv[NFLD];
-            fv[F_ONE] = 1.0;
-            for (int d = 0; d < DIM; d++) fv[F_X + d] = sm.X[le][a][d];
-            fv[F_PH] = sm.Ph[le][a];
-            fv[F_PHO] = NOLD ? sm.PhO[le][a] : 0.0;
-            for (int d = 0; d < DIM; d++) {
-                fv[F_U + d] = sm.U[le][d][a];
-                fv[F_UO + d] = NOLD ? sm.Uo[le][d][a] : 0.0;
-            }
-            fv[F_T] = sm.T[le][a];
-            if constexpr (ADJ)
-                for (int k = 0; k < DIM + 2; k++) fv[F_L + k] = sm.L[le][k][a];
+        for (int az = 0; az < (DIM == 3 ? 3 : 1); az++) {
+            const double bz = DIM == 3 ? pick(b1[DIM - 1], az) : 1.0, gz = DIM == 3 ? pick(g1[DIM - 1], az) : 0.0;
+#pragma unroll 1
+            for (int ay = 0; ay < 3; ay++) {
+                const double by = pick(b1[1], ay), gy = pick(g1[1], ay);
 #pragma unroll
-            for (int k = 0; k < NFLD; k++) {
-                if (!NOLD && (k == F_PHO || (k >= F_UO && k < F_UO + DIM))) continue;  // sums stay 0
-                SC[k] += wc * fv[k];
-                for (int c = 0; c < DIM; c++) SG[c][k] += wg[c] * fv[k];
+                for (int ax = 0; ax < 3; ax++) {
+                    const int a = az * 9 + ay * 3 + ax;
+                    double cb, gb[DIM];
+                    if (DIM == 2) {
+                        cb = b1[0][ax] * by;
+                        gb[0] = g1[0][ax] * by;
+                        gb[1] = b1[0][ax] * gy;
+                    } else {
+                        cb = b1[0][ax] * by * bz;
+                        gb[0] = g1[0][ax] * by * bz;
+                        gb[1] = b1[0][ax] * gy * bz;
+                        gb[DIM - 1] = b1[0][ax] * by * gz;
+                    }
+                    const double Wa = DIM == 3 ? sm.X[le][a][3] : 1.0;
+                    const double wc = Wa * cb;
+                    double wg[DIM];
+                    for (int c = 0; c < DIM; c++) wg[c] = Wa * gb[c];
+                    double fv[NFLD];
+                    fv[F_ONE] = 1.0;
+                    for (int d = 0; d < DIM; d++) fv[F_X + d] = sm.X[le][a][d];
+                    fv[F_PH] = sm.Ph[le][a];
+                    fv[F_PHO] = NOLD ? sm.PhO[le][a] : 0.0;
+                    for (int d = 0; d < DIM; d++) {
+                        fv[F_U + d] = sm.U[le][d][a];
+                        fv[F_UO + d] = NOLD ? sm.Uo[le][d][a] : 0.0;
+                    }
+                    fv[F_T] = sm.T[le][a];
+                    if constexpr (ADJ)
+                        for (int k = 0; k < DIM + 2; k++) fv[F_L + k] = sm.L[le][k][a];
+#pragma unroll
+                    for (int k = 0; k < NFLD; k++) {
+                        if (!NOLD && (k == F_PHO || (k >= F_UO && k < F_UO + DIM))) continue;  // sums stay 0
+                        SC[k] += wc * fv[k];
+                        for (int c = 0; c < DIM; c++) SG[c][k] += wg[c] * fv[k];
+                    }
+                }
             }
         }
         const double wF = DIM == 3 ? SC[F_ONE] : 1.0;
