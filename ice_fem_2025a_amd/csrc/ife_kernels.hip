@@ -2660,19 +2660,21 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
         const int64_t el = t < EPW * N ? sm.el[e] : -1;
         rk[k] = el >= 0 ? p.erank[el * N + a] : -1;
     }
-    // the coefficient blocks, 8 loads in flight per lane, converted into the LDS rows
-    constexpr int KT = (EPW * NB + 63) / 64;
-#pragma unroll 1
-    for (int k0 = 0; k0 < KT; k0 += 8) {
-        double v[8];
+    // the coefficient blocks, converted into the LDS rows: KT loads per lane, issued in as few
+    // batches of at most 24 as possible (one at 4 elements per wavefront), with the nodal
+    // state's loads (which need only the ranks) in flight behind the first batch
+    constexpr int KT = (EPW * NB + 63) / 64, NBAT = (KT + 23) / 24, KB = (KT + NBAT - 1) / NBAT;
+    auto cload = [&](const int k0, double (&v)[KB]) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < KB; u++) {
             const int t = lane + 64 * (k0 + u), e = t / NB;
             const int64_t el = t < EPW * NB ? sm.el[e] : -1;
             v[u] = el >= 0 ? p.ipb[el * NB + (t - e * NB)] : 0.0;
         }
+    };
+    auto cstage = [&](const int k0, const double (&v)[KB]) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < KB; u++) {
             const int t = lane + 64 * (k0 + u);
             if (t >= EPW * NB) continue;
             const int e = t / NB, rem = t - e * NB, row = rem / NIP, ip = rem - row * NIP;
@@ -2683,8 +2685,9 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
             else if (row == W2C_GW) sm.w64[e][3][ip] = v[u];
             else if (row >= W2C_JI && row < W2C_JI + 4) sm.u.s.ji[e][row - W2C_JI][ip] = v[u];
         }
-    }
-    for (int t = lane; t < EPW * NIP; t += 64) sm.cz[t / NIP][M2_Z][t % NIP] = 0.0f;
+    };
+    double cv[KB];
+    cload(0, cv);
     double nv[NPR][7];  // dU x y, ddU x y, phi, phi_old, dT
 #pragma unroll
     for (int k = 0; k < NPR; k++) {
@@ -2703,6 +2706,13 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
         }
         if (p.has_T) nv[k][6] = p.sT.dx[(int64_t)p.fpos_T * p.Ng + r];
     }
+    cstage(0, cv);
+#pragma unroll 1
+    for (int k0 = KB; k0 < KT; k0 += KB) {
+        cload(k0, cv);
+        cstage(k0, cv);
+    }
+    for (int t = lane; t < EPW * NIP; t += 64) sm.cz[t / NIP][M2_Z][t % NIP] = 0.0f;
     // 1D factors per (element, direction, function): b1 = C B1^T, db1 = C dB1^T
     for (int t = lane; t < EPW * 6; t += 64) {
         const int e = t / 6, d = (t / 3) % 2, a = t % 3;
@@ -2842,6 +2852,21 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
                                : col == RC::WL ? (dph && p.lag)
                                                : false;
                 if (!w) continue;
+                // this column's nodal addition, decided once: kind 1 adds nod[nr][jd] to row jd
+                // only (diagonal entries), 2 adds nod[nr][i] to every row i (residuals), 3 stores
+                // nod[7][i] (the lumped weights); the rows then take selects, not branches
+                int kind = 0, nr = 0, jd = 0;
+                if (col < 36) {
+                    const int al = col / 18, be = (col / 9) % 2;
+                    if (al == be) { kind = 1; jd = col % 9; }
+                } else if (col < RC::KP) { kind = 2; nr = 1 + col - RC::FU; }
+                else if (col < RC::FP) { kind = 1; nr = 3; jd = col - RC::KP; }
+                else if (col == RC::FP) { kind = 2; nr = 4; }
+                else if (col < RC::FT) { kind = 1; nr = 5; jd = col - RC::KT; }
+                else if (col == RC::FT) { kind = 2; nr = 6; }
+                else { kind = 3; nr = 7; }
+                const double* const nrow = &sm.u.nod[e][nr][0];
+                const double dv = kind == 1 ? nrow[jd] : 0.0;
 #pragma unroll
                 for (int rb = 0; rb < 3; rb++)
 #pragma unroll
@@ -2849,22 +2874,8 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
                         const int i = 4 * rb + r;
                         if (i >= N) continue;
                         double v = (double)acc[gi][rb][r];
-                        if (col < 36) {
-                            const int al = col / 18, be = (col / 9) % 2, j = col % 9;
-                            if (al == be && j == i) v += sm.u.nod[e][0][i];
-                        } else if (col < RC::KP) {
-                            v += sm.u.nod[e][1 + col - RC::FU][i];
-                        } else if (col < RC::FP) {
-                            if (col - RC::KP == i) v += sm.u.nod[e][3][i];
-                        } else if (col == RC::FP) {
-                            v += sm.u.nod[e][4][i];
-                        } else if (col < RC::FT) {
-                            if (col - RC::KT == i) v += sm.u.nod[e][5][i];
-                        } else if (col == RC::FT) {
-                            v += sm.u.nod[e][6][i];
-                        } else {
-                            v = sm.u.nod[e][7][i];  // the element's lumped weight of node i
-                        }
+                        const double rv = kind >= 2 ? nrow[i] : 0.0;
+                        v = kind == 3 ? rv : (kind == 2 ? v + rv : (kind == 1 && i == jd ? v + dv : v));
                         chk = fma(v, 0.0, chk);
                         rec[i * RC::SIZE + col] = (float)v;
                     }
