@@ -2799,7 +2799,7 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
             M2Col cd[PG];
 #pragma unroll
             for (int gi = 0; gi < PG; gi++) cd[gi] = m2_col(4 * ((g0 + gi) * SL + s) + q, c);
-#pragma unroll 1
+#pragma unroll
             for (int ip = 0; ip < NIP; ip++) {
                 const float* const Gi = Ge + ip * 18;
                 const float* const Ci = Ce + ip;
@@ -2821,7 +2821,7 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
             M2Col cn[PG];
 #pragma unroll
             for (int gi = 0; gi < PG; gi++) cn[gi] = m2_coln(4 * ((g0 + gi) * SL + s) + q);
-#pragma unroll 1
+#pragma unroll
             for (int ip = 0; ip < NIP; ip++) {
                 const float* const Ni = Ne + ip * N;
                 const float* const Ci = Ce + ip;
