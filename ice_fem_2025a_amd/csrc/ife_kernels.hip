@@ -1137,6 +1137,15 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
     const int64_t e0 = blk * EPB;
     if (tid < 4) sm.cnt[tid] = 0;
     // ---- stage the block's elements ----------------------------------------------
+    // extraction factors: loaded first into registers (independent of the node ids, so their
+    // round trip overlaps the node loads), stored into LDS after the node staging
+    constexpr int NXT = EPB * DIM * 9, XPER = (NXT + NT - 1) / NT;
+    double xv[XPER];
+#pragma unroll
+    for (int u = 0; u < XPER; u++) {
+        const int i = tid + u * NT, le = i / (DIM * 9), k = i % (DIM * 9);
+        xv[u] = i < NXT && e0 + le < p.NE ? p.ext[(e0 + le) * DIM * 9 + k] : 0.0;
+    }
     for (int i = tid; i < EPB * N; i += NT) {
         int le = i / N, a = i % N;
         int64_t el = e0 + le;
@@ -1162,10 +1171,10 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
             sm.L[le][DIM + 1][a] = p.lam_T ? p.lam_T[(int64_t)p.fpos_T * p.Ng + r] : 0.0;
         }
     }
-    for (int i = tid; i < EPB * DIM * 9; i += NT) {
-        int le = i / (DIM * 9), k = i % (DIM * 9);
-        int64_t el = e0 + le;
-        if (el < p.NE) (&sm.C1[le][0][0])[k] = p.ext[el * DIM * 9 + k];
+#pragma unroll
+    for (int u = 0; u < XPER; u++) {
+        const int i = tid + u * NT, le = i / (DIM * 9), k = i % (DIM * 9);
+        if (i < NXT && e0 + le < p.NE) (&sm.C1[le][0][0])[k] = xv[u];
     }
     __syncthreads();
     const int le = tid / NIP, ip = tid % NIP;
