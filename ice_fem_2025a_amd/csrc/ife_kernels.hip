@@ -4693,8 +4693,10 @@ struct CossRows {
 // them.  T: tile type (double: FP64; float: mixed mode, FP32 MFMA, FP64 accumulation in the
 // CSR).  Then F_U = sum_ip B^T (w sigma) and the lumped inertia, damping and gravity terms
 // (:454-527), per (field, node).
+// 3D: three workgroups per CU (FP64: 152 VGPRs, no AGPRs, no spills; the default allocation
+// took 188 registers, two workgroups).
 template <int DIM, typename T>
-__global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
+__global__ __launch_bounds__(NT, DIM == 3 ? 3 : 1) void coss_assemble_mfma(KParams p) {
     if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     using CR = CossRows<DIM>;
     typedef typename Mfma<T>::V V;
@@ -4703,8 +4705,22 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
     constexpr bool F64 = sizeof(T) == 8;
     __shared__ T Phi[NIP][4][NP];
     __shared__ double Phd[F64 ? 1 : NIP][NB][N];  // FP64 basis for the residual terms (mixed mode)
-    __shared__ double Mb[NW][NIP][16];
-    __shared__ double geo[NG][NIP];               // J^-1 / wFun, J^-1 dW / wFun^2, 1 / wFun
+    // geo (J^-1 / wFun, J^-1 dW / wFun^2, 1 / wFun) is read only while the basis is built,
+    // Mb (each wavefront's current pair of 4 x 4 slot blocks, in the tile type the products
+    // take) only in the pair loop.  Mixed mode lets them share their LDS, which fits three
+    // workgroups per CU (51.5 kB; 61.2 kB before); FP64 fits three without it, and measured
+    // 1% slower with it.
+    struct GeoMbSep {
+        T Mb[NW][NIP][16];
+        double geo[NG][NIP];
+    };
+    union GeoMbU {
+        double geo[NG][NIP];
+        T Mb[NW][NIP][16];
+    };
+    __shared__ typename std::conditional<F64, GeoMbSep, GeoMbU>::type gm;
+    auto& geo = gm.geo;
+    auto& Mb = gm.Mb;
     __shared__ double sig[18][NIP];               // w sigma
     __shared__ double WI[N], WG[N], wi_ip[NIP], wg_ip[NIP];
     __shared__ double b1s[DIM][3][3], db1s[DIM][3][3];
@@ -4810,7 +4826,7 @@ __global__ __launch_bounds__(NT) void coss_assemble_mfma(KParams p) {
 #pragma unroll
         for (int u = 0; u < MU; u++) {
             const int t = lane + 64 * u;
-            if (t < NIP * 16) Mb[wv][t % NIP][t / NIP] = mpre[u];
+            if (t < NIP * 16) Mb[wv][t % NIP][t / NIP] = (T)mpre[u];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
