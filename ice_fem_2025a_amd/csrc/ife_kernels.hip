@@ -2894,8 +2894,10 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
     if (chk != 0.0) atomicOr(&p.st->nonfinite, 1u);
 }
 
+// Four wavefronts per SIMD as the allocation target: 4 elements per wavefront then take 112
+// VGPRs and no AGPRs (164 registers and three wavefronts without it); LDS allows 15 per CU.
 template <int EPW>
-__global__ __launch_bounds__(64) void assemble2d_mfma(KParams p) {
+__global__ __launch_bounds__(64, 4) void assemble2d_mfma(KParams p) {
     if (p.gate && !*p.gate) return;  // a skipped trial of the device line search
     __shared__ M2D<EPW> sm;
     assemble2d_mfma_body<EPW>(p, blockIdx.x, sm, threadIdx.x);
@@ -2904,7 +2906,7 @@ __global__ __launch_bounds__(64) void assemble2d_mfma(KParams p) {
 // batched form: one wavefront per EPW consecutive elements of one domain (the block table
 // counts ceil(elements / EPW) per domain)
 template <int EPW>
-__global__ __launch_bounds__(64) void assemble2d_mfma_batch(const KParams* P_, const int64_t* bptr_, int nd) {
+__global__ __launch_bounds__(64, 4) void assemble2d_mfma_batch(const KParams* P_, const int64_t* bptr_, int nd) {
     KParamsC* P = (KParamsC*)P_;
     I64C* bptr = (I64C*)bptr_;
     __shared__ M2D<EPW> sm;
