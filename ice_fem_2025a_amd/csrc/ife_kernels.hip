@@ -12,9 +12,11 @@
 //                    written contiguously to an element-record buffer.
 //   assemble2d_wave  2D: one wavefront per element (9 nodes, 9 IPs: too small for tiles), the
 //                    same records from one FMA stream per K/f row.
-//   gather_kernel    one wavefront per node: adds the records of the node's elements in
-//                    ascending element order and writes the node's CSR rows once (no atomics,
-//                    no colouring, deterministic).
+//   assemble2d_mfma  2D, mixed precision: 4, 8 or 16 elements per wavefront on
+//                    v_mfma_f32_4x4x1_16b_f32, the same records.
+//   gather_kernel    one wavefront per node (3D; 2D: one half-wavefront per node): adds the
+//                    records of the node's elements in ascending element order and writes
+//                    the node's CSR rows once (no atomics, no colouring, deterministic).
 //
 // Reference semantics (paths relative to the IceFEM source tree):
 //   shape functions  mesh/ElementTypes/Bezier/B2_2.cpp:72-88 (2D, transposed-Jacobian form)
@@ -3236,7 +3238,8 @@ __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
     if (bad) atomicOr(&p.st->nonfinite, 1u);
 }
 
-// Gather kernel: one wavefront per node (group rank r).  Adds the records of the node's
+// Gather kernel: one wavefront per node (group rank r) in 3D, one half-wavefront per node
+// in 2D (gather_npw).  Adds the records of the node's
 // adjacent elements, in increasing element index (the order in which the reference's
 // element loop adds them through MatSetValues/VecSetValues), into LDS copies of the
 // node's CSR rows, then writes each row once, contiguously.  Every CSR entry of the
@@ -3246,6 +3249,7 @@ __global__ __launch_bounds__(256) void ho_phase_kernel(KParams p) {
 // The workgroup is a single wavefront, whose LDS instructions execute in issue order, so
 // the adds of record k+1 into a slot follow those of record k: no barrier is needed
 // between records, and the loads of GPF records are issued before any of them is added.
+// (In 2D the two half-wavefronts' nodes have LDS rows of their own.)
 // The adds are LDS atomics without return (no read-add-write latency chain); within one
 // record the lanes' slots are distinct.
 // ---------------------------------------------------------------------------------------
