@@ -2797,6 +2797,7 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
     const int64_t el = sm.el[e];
     const int cb_lo = du ? 0 : 9, cb_hi = (dph || dT) ? 15 : 10;  // column blocks holding active parts
     double chk = 0.0;
+    float chkf = 0.f;  // both stay 0 unless a stored value is NaN or infinite
 #pragma unroll
     for (int g0 = 0; g0 < NG; g0 += PG) {
         if (g0 * SL >= cb_hi || (g0 + PG) * SL <= cb_lo) continue;
@@ -2878,6 +2879,32 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
                 else { kind = 3; nr = 7; }
                 const double* const nrow = &sm.u.nod[e][nr][0];
                 const double dv = kind == 1 ? nrow[jd] : 0.0;
+                // groups without residual / weight columns (known at compile time): every row
+                // stores its accumulator as is ((float)(double)x == x), the diagonal row is
+                // stored again with its FP64 nodal addition
+                const int c_lo = 4 * (g0 + gi) * SL, c_hi = c_lo + 4 * SL - 1;
+                const bool fast = !(c_lo <= RC::KP - 1 && c_hi >= RC::FU) && !(c_lo <= RC::FP && c_hi >= RC::FP) &&
+                                  !(c_lo <= RC::WL && c_hi >= RC::FT);
+                if (fast) {
+                    float aj = 0.f;
+#pragma unroll
+                    for (int rb = 0; rb < 3; rb++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int i = 4 * rb + r;
+                            if (i >= N) continue;
+                            const float a = acc[gi][rb][r];
+                            chkf = fmaf(a, 0.f, chkf);
+                            aj = i == jd ? a : aj;
+                            rec[i * RC::SIZE + col] = a;
+                        }
+                    if (kind == 1) {
+                        const double v = (double)aj + dv;
+                        chk = fma(v, 0.0, chk);
+                        rec[jd * RC::SIZE + col] = (float)v;
+                    }
+                    continue;
+                }
 #pragma unroll
                 for (int rb = 0; rb < 3; rb++)
 #pragma unroll
@@ -2893,7 +2920,7 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
             }
         }
     }
-    if (chk != 0.0) atomicOr(&p.st->nonfinite, 1u);
+    if (chk != 0.0 || chkf != 0.f) atomicOr(&p.st->nonfinite, 1u);
 }
 
 // Four wavefronts per SIMD as the allocation target: 4 elements per wavefront then take 112
