@@ -2689,12 +2689,15 @@ __device__ __forceinline__ void assemble2d_mfma_body(const KParams& p, const int
             const int t = lane + 64 * (k0 + u);
             if (t >= EPW * NB) continue;
             const int e = t / NB, rem = t - e * NB, row = rem / NIP, ip = rem - row * NIP;
-            if (row < M2_Z) sm.cz[e][row][ip] = (float)v[u];
-            else if (row == W2C_W) sm.w64[e][0][ip] = v[u];
-            else if (row == W2C_LD) sm.w64[e][1][ip] = v[u];
-            else if (row == W2C_GC) sm.w64[e][2][ip] = v[u];
-            else if (row == W2C_GW) sm.w64[e][3][ip] = v[u];
-            else if (row >= W2C_JI && row < W2C_JI + 4) sm.u.s.ji[e][row - W2C_JI][ip] = v[u];
+            // two stores: the FP32 rows, or one FP64 row picked by address (rows differ across
+            // lanes, so a branch per destination would run every branch)
+            if (row < M2_Z) {
+                sm.cz[e][row][ip] = (float)v[u];
+            } else {
+                const int wr = row == W2C_W ? 0 : (row == W2C_LD ? 1 : (row == W2C_GC ? 2 : 3));
+                double* const dst = row >= W2C_JI && row < W2C_JI + 4 ? &sm.u.s.ji[e][row - W2C_JI][0] : &sm.w64[e][wr][0];
+                dst[ip] = v[u];
+            }
         }
     };
     double cv[KB];
