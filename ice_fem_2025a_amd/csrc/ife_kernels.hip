@@ -1155,18 +1155,32 @@ __device__ __forceinline__ void material_body(const KParams& p, const int64_t bl
         int v = p.enode[el * N + a];
         int r = p.erank[el * N + a];
         const double* nd = p.nodes + (int64_t)v * 4;
-        for (int c = 0; c < 4; c++) sm.X[le][a][c] = nd[c];
+        // every load of the node first, then the LDS stores: one round trip (a store right
+        // after its load waits for it before the next loads issue)
+        double xn[4], un[DIM], uon[DIM], phn = 0.0, phon = 0.0, tn = 0.0;
+        for (int c = 0; c < 4; c++) xn[c] = nd[c];
+        for (int d = 0; d < DIM; d++) { un[d] = 0.0; uon[d] = 0.0; }
         if (p.has_pf) {
             for (int d = 0; d < DIM; d++) {
                 int64_t dd = (int64_t)p.fpos_u[d] * p.Ng + r;
-                sm.U[le][d][a] = p.su.x[dd];
-                if constexpr (NOLD) sm.Uo[le][d][a] = p.su.xold[dd];
+                un[d] = p.su.x[dd];
+                if constexpr (NOLD) uon[d] = p.su.xold[dd];
             }
             int64_t dp = (int64_t)p.fpos_ph * p.Ng + r;
-            sm.Ph[le][a] = p.sph.x[dp];
-            if constexpr (NOLD) sm.PhO[le][a] = p.sph.xold[dp];
+            phn = p.sph.x[dp];
+            if constexpr (NOLD) phon = p.sph.xold[dp];
         }
-        if (p.has_T) sm.T[le][a] = p.sT.x[(int64_t)p.fpos_T * p.Ng + r];
+        if (p.has_T) tn = p.sT.x[(int64_t)p.fpos_T * p.Ng + r];
+        for (int c = 0; c < 4; c++) sm.X[le][a][c] = xn[c];
+        if (p.has_pf) {
+            for (int d = 0; d < DIM; d++) {
+                sm.U[le][d][a] = un[d];
+                if constexpr (NOLD) sm.Uo[le][d][a] = uon[d];
+            }
+            sm.Ph[le][a] = phn;
+            if constexpr (NOLD) sm.PhO[le][a] = phon;
+        }
+        if (p.has_T) sm.T[le][a] = tn;
         if constexpr (ADJ) {
             for (int d = 0; d < DIM; d++) sm.L[le][d][a] = p.lam_u ? p.lam_u[(int64_t)p.fpos_u[d] * p.Ng + r] : 0.0;
             sm.L[le][DIM][a] = p.lam_ph ? p.lam_ph[(int64_t)p.fpos_ph * p.Ng + r] : 0.0;
