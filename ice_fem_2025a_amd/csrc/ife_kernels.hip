@@ -5053,7 +5053,7 @@ __global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, 
                     lcv[q] = p.loc[(e * N + a) * N + (lane < N ? lane : N - 1)];
 #pragma unroll
                     for (int u = 0; u < PER; u++)
-                        x[q][u] = use[q] && (lane + u * GW) * W < nent ? rec[lane + u * GW] : RecLoad<T>::zero();
+                        x[q][u] = use[q] && (lane + u * GW) * W < nent ? rd<DIM == 3>(&rec[lane + u * GW]) : RecLoad<T>::zero();
                 }
 #pragma unroll
                 for (int q = 0; q < GPF; q++) {
@@ -5072,7 +5072,7 @@ __global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, 
         __syncthreads();
         for (int q = 0; q < PG; q++) {
             double* K = Krow(g * PG + q);
-            for (int i = lane; i < deg; i += GW) K[i] = acc[q * MAXDEG + i];
+            for (int i = lane; i < deg; i += GW) wr<DIM == 3>(acc[q * MAXDEG + i], &K[i]);
         }
         if (last && lane < NFU) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = acc[OF + lane];
         __syncthreads();  // the next pass clears the rows
