@@ -526,7 +526,7 @@ void setup_device(ife_physics* ph) {
     ph->d_status.alloc(1);
     HIPCHK(hipMemset(ph->d_status.p, 0, sizeof(Status)));
     // 2D stores the 29 rows its element kernel reads (W2C_NF in ife_kernels.hip), 3D all 70
-    ph->d_ipb.alloc((size_t)(ph->dim == 2 ? 29 : IPB_NF) * ph->NE * ph->nip);
+    ph->d_ipb.alloc((size_t)coefficient_rows(ph->dim) * ph->NE * ph->nip);
     ph->d_ebuf.alloc((size_t)ph->NE * ph->n * record_size(ph->dim));
     ph->d_n2e_ptr.upload(ph->n2e_ptr.data(), ph->n2e_ptr.size());
     ph->d_n2e_elem.upload(ph->n2e_elem.data(), ph->n2e_elem.size());

@@ -237,6 +237,7 @@ hipError_t launch_batch_assemble(const BatchArgs& a, int64_t grid, int nta, hipS
 hipError_t launch_batch_gather(const BatchArgs& a, int64_t grid, hipStream_t s);
 int blocks_material(int dim, int64_t n_elems);
 int gather_nodes_per_wave(int dim);  // element-group nodes per gather wavefront
+int coefficient_rows(int dim);      // per-IP coefficient rows per element (2D: fewer)
 
 // ---- line search (ife_linesearch.hip): NonLinSolver::HOLineSearch on the device ----------
 constexpr int LS_MAX = 10;   // maxLS (NonLinSolver.cpp:313)

@@ -5352,6 +5352,7 @@ hipError_t launch_zero_status(Status* st, hipStream_t s, const int* gate) {
 }
 
 int gather_nodes_per_wave(int dim) { return dim == 2 ? gather_npw<2>() : gather_npw<3>(); }
+int coefficient_rows(int dim) { return dim == 2 ? ipb_rows<2>() : ipb_rows<3>(); }
 
 int blocks_material(int dim, int64_t n_elems) {
     const int epb = NT / (dim == 2 ? 9 : 27);
