@@ -3309,26 +3309,27 @@ constexpr int gather_maxdeg() { return DIM == 2 ? 64 : 128; }
 template <int DIM, typename T>
 constexpr int gather_gpf() { return DIM == 2 ? 9 : 6; }
 
+// Loads and stores of data touched once per launch (element records in the gather, CSR
+// values out of it), optionally with the streaming (non-temporal) cache policy: in 3D it
+// frees the L2 for the node-to-element lists and column-slot rows that the gather re-reads
+// (measured: 3D gather 2.05 -> 1.94 ms FP64, Cosserat u gather 31.5-33.0 -> 29.3-29.9 ms;
+// the 2D batched gather is ~2% faster without)
+template <bool NT_, typename X>
+__device__ __forceinline__ X load_once(const X* a) {
+    if constexpr (NT_) return __builtin_nontemporal_load(a);
+    else return *a;
+}
+template <bool NT_, typename X>
+__device__ __forceinline__ void store_once(X v, X* a) {
+    if constexpr (NT_) __builtin_nontemporal_store(v, a);
+    else *a = v;
+}
+
 // W consecutive record entries per lane load: one FP64 entry (8 B), or two FP32 entries
 // (8 B) of the mixed mode's records, so that a load moves the same bytes in both modes (the
 // gather is bound by the loads in flight per wavefront).  Record lengths are even, so FP32
 // pairs never straddle two records.
 typedef float flt2 __attribute__((ext_vector_type(2)));
-// Loads and stores of data touched once per launch (element records in the gather, CSR
-// values out of it), optionally with the streaming (non-temporal) cache policy: in 3D it
-// frees the L2 for the node-to-element lists and column-slot rows that the gather re-reads
-// (measured: 3D gather 2.04 -> 1.93 ms FP64; the 2D batched gather is ~2% faster without)
-template <bool NT_, typename X>
-__device__ __forceinline__ X rd(const X* a) {
-    if constexpr (NT_) return __builtin_nontemporal_load(a);
-    else return *a;
-}
-template <bool NT_, typename X>
-__device__ __forceinline__ void wr(X v, X* a) {
-    if constexpr (NT_) __builtin_nontemporal_store(v, a);
-    else *a = v;
-}
-
 template <typename T>
 struct RecLoad;
 template <>
@@ -3448,7 +3449,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
                 lcv[q] = p.loc[(e * N + a) * N + (hl < N ? hl : N - 1)];
 #pragma unroll
                 for (int u = 0; u < PER; u++)  // loads stay inside the record
-                    x[q][u] = ld[u] && hl + u * HW < RC::SIZE / W ? rd<DIM == 3>(&rec[hl + u * HW]) : RecLoad<T>::zero();
+                    x[q][u] = ld[u] && hl + u * HW < RC::SIZE / W ? load_once<DIM == 3>(&rec[hl + u * HW]) : RecLoad<T>::zero();
             }
 #pragma unroll
             for (int q = 0; q < GPF; q++) {
@@ -3471,7 +3472,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
         for (int al = 0; al < DIM; al++) {
             double* K = p.su.K + (int64_t)p.fpos_u[al] * p.su.M + (int64_t)p.su.nf * r0;
             for (int be = 0; be < DIM; be++)
-                for (int i = hl; i < deg; i += HW) wr<DIM == 3>(acc[(al * DIM + be) * MAXDEG + i], &K[(int64_t)p.fpos_u[be] * deg + i]);
+                for (int i = hl; i < deg; i += HW) store_once<DIM == 3>(acc[(al * DIM + be) * MAXDEG + i], &K[(int64_t)p.fpos_u[be] * deg + i]);
         }
         if (hl < DIM) p.su.f[(int64_t)p.fpos_u[hl] * p.Ng + r] = acc[OF + hl];
     }
@@ -3499,7 +3500,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
                 if (cb < 0) continue;
                 double* const Kb = K + (int64_t)cb * deg;
                 for (int i = hl; i < deg; i += HW)
-                    wr<DIM == 3>((ri == 0 && cj == 0 ? acc[OKP + i] : 0.0) + (i == self ? dg[ri][cj] : 0.0), &Kb[i]);
+                    store_once<DIM == 3>((ri == 0 && cj == 0 ? acc[OKP + i] : 0.0) + (i == self ? dg[ri][cj] : 0.0), &Kb[i]);
             }
             if (hl == 0) p.sph.f[(int64_t)p.ph_fpos[ri] * p.Ng + r] = fr[ri];
         }
@@ -3507,7 +3508,7 @@ __device__ __forceinline__ void gather_body(const KParams& p, const int64_t blk)
     if (dT) {
         if (p.do_th) {
             double* K = p.sT.K + (int64_t)p.fpos_Tth * p.sT.M + (int64_t)p.sT.nf * r0 + (int64_t)p.fpos_Tth * deg;
-            for (int i = hl; i < deg; i += HW) wr<DIM == 3>(acc[OKT + i], &K[i]);
+            for (int i = hl; i < deg; i += HW) store_once<DIM == 3>(acc[OKT + i], &K[i]);
         }
         if (hl == 0) p.sT.f[(int64_t)p.fpos_Tth * p.Ng + r] = acc[OF + DIM + 1];
     }
@@ -5053,7 +5054,7 @@ __global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, 
                     lcv[q] = p.loc[(e * N + a) * N + (lane < N ? lane : N - 1)];
 #pragma unroll
                     for (int u = 0; u < PER; u++)
-                        x[q][u] = use[q] && (lane + u * GW) * W < nent ? rd<DIM == 3>(&rec[lane + u * GW]) : RecLoad<T>::zero();
+                        x[q][u] = use[q] && (lane + u * GW) * W < nent ? load_once<DIM == 3>(&rec[lane + u * GW]) : RecLoad<T>::zero();
                 }
 #pragma unroll
                 for (int q = 0; q < GPF; q++) {
@@ -5072,7 +5073,7 @@ __global__ __launch_bounds__(GW) void coss_gather_kernel(KParams p, int64_t e0, 
         __syncthreads();
         for (int q = 0; q < PG; q++) {
             double* K = Krow(g * PG + q);
-            for (int i = lane; i < deg; i += GW) wr<DIM == 3>(acc[q * MAXDEG + i], &K[i]);
+            for (int i = lane; i < deg; i += GW) store_once<DIM == 3>(acc[q * MAXDEG + i], &K[i]);
         }
         if (last && lane < NFU) p.su.f[(int64_t)p.fpos_u[lane] * p.Ng + r] = acc[OF + lane];
         __syncthreads();  // the next pass clears the rows
