@@ -3304,10 +3304,12 @@ template <int DIM>
 constexpr int gather_maxdeg() { return DIM == 2 ? 64 : 128; }
 // records loaded ahead (all their loads in flight before the first LDS add).  Measured on the
 // 40x40x80 hex block (3D: 3, 4, 6, 9, 12, 16 ahead; a node has up to 27 elements) and the
-// 64-domain quad batch (2D: 3, 4, 6, 9, 12, 16; up to 9 elements, so one batch): 6 in 3D in
-// both precisions, 9 in 2D
+// 64-domain quad batch (2D: 3, 4, 6, 9, 12, 16; up to 9 elements, so one batch): 9 in 2D; in
+// 3D, with the records read under the streaming cache policy, 4 for FP64 records and 9 for
+// the FP32 pairs of the mixed mode (4, 6, 9 ahead: FP64 1.92 / 1.93 / 1.94 ms, FP32 1.09 /
+// 1.09 / 1.07 ms)
 template <int DIM, typename T>
-constexpr int gather_gpf() { return DIM == 2 ? 9 : 6; }
+constexpr int gather_gpf() { return DIM == 2 ? 9 : (sizeof(T) == 8 ? 4 : 9); }
 
 // Loads and stores of data touched once per launch (element records in the gather, CSR
 // values out of it), optionally with the streaming (non-temporal) cache policy: in 3D it
