@@ -3312,10 +3312,10 @@ template <int DIM, typename T>
 constexpr int gather_gpf() { return DIM == 2 ? 9 : (sizeof(T) == 8 ? 4 : 9); }
 
 // Loads and stores of data touched once per launch (element records in the gather, CSR
-// values out of it), optionally with the streaming (non-temporal) cache policy: in 3D it
-// frees the L2 for the node-to-element lists and column-slot rows that the gather re-reads
-// (measured: 3D gather 2.05 -> 1.94 ms FP64, Cosserat u gather 31.5-33.0 -> 29.3-29.9 ms;
-// the 2D batched gather is ~2% faster without)
+// values out of it), optionally with the streaming (non-temporal) cache policy.  Measured on
+// the 3D FP64 gather: 2.05 ms without it, 1.92 ms with it on the record loads and the CSR
+// stores, 2.12 ms on the loads alone, 1.99 ms on the stores alone; the Cosserat u gather
+// 31.5-33.0 -> 29.3-29.9 ms; the batched 2D gather is ~2% faster without it.
 template <bool NT_, typename X>
 __device__ __forceinline__ X load_once(const X* a) {
     if constexpr (NT_) return __builtin_nontemporal_load(a);
